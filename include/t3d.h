@@ -1,0 +1,342 @@
+/*
+ * t3d.h — C-ABI of the MI355X geometry pipeline for Tetra3D's Camera.Render.
+ *
+ * This boundary replaces the CPU path that Camera.Render runs per frame
+ * (camera.go:1269-2114): per-model uniforms and frustum test, Model.ProcessVertices
+ * (model.go:600-995: transform, skinning, cull, depth key), the 512-bin triangle
+ * bucket sort (sortingTriangles.go:38-125), per-vertex lighting
+ * (camera.go:1610-1644, light.go) and the ebiten.Vertex assembly
+ * (camera.go:1646-1820).  It stops where the reference calls
+ * DrawTriangles / DrawTrianglesShader (camera.go:1920-2036): the output is, per
+ * render pair, the colour/depth vertex arrays and index list that flush() hands
+ * to ebiten.
+ *
+ * Conventions
+ *   - Matrices are Tetra3D Matrix4: row-major float32 [4][4], row vectors,
+ *     v' = v·M, translation in row 3 (matrix.go:9-10, 619-637).
+ *   - All arithmetic the library does is IEEE float32 in the reference's
+ *     operation order, with no fused multiply-add.
+ *   - Every entry point returns a t3d_status; on failure t3d_last_error(ctx)
+ *     holds a message.  Nothing aborts across the boundary.
+ *   - A context is externally synchronised: one calling thread at a time
+ *     (the reference keeps all scratch in package globals, tetra3d.go:21-35,
+ *     so it is single-threaded too).  The serving entry points are the
+ *     exception and are documented as thread-safe below.
+ *   - Handles are opaque non-zero uint64 values.
+ */
+#ifndef T3D_H
+#define T3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define T3D_ABI_VERSION 1
+
+typedef int32_t t3d_status;
+enum {
+  T3D_OK = 0,
+  T3D_ERR_INVALID_ARGUMENT = -1, /* malformed descriptor (reference: panics, mesh.go:681, 2279) */
+  T3D_ERR_OUT_OF_MEMORY = -2,    /* device allocation failed */
+  T3D_ERR_BUDGET = -3,           /* request exceeds the context's HBM budget */
+  T3D_ERR_DEVICE = -4,           /* HIP runtime error */
+  T3D_ERR_UNSUPPORTED = -5,      /* feature outside the GPU contract (Go callbacks, billboards) */
+  T3D_ERR_NOT_FOUND = -6,        /* unknown or freed handle */
+  T3D_ERR_BUSY = -7,             /* resource still referenced by in-flight work */
+  T3D_ERR_NOT_READY = -8,        /* job not finished yet (poll) */
+  T3D_ERR_CANCELLED = -9         /* job was cancelled */
+};
+
+/* TriangleSortMode (material.go:9-13) */
+enum { T3D_SORT_BACK_TO_FRONT = 0, T3D_SORT_FRONT_TO_BACK = 1, T3D_SORT_NONE = 2 };
+/* TransparencyMode (material.go:15-27) */
+enum {
+  T3D_TRANSPARENCY_AUTO = 0,
+  T3D_TRANSPARENCY_OPAQUE = 1,
+  T3D_TRANSPARENCY_ALPHA_CLIP = 2,
+  T3D_TRANSPARENCY_TRANSPARENT = 3
+};
+/* LightingMode (material.go:35-38) */
+enum { T3D_LIGHTING_DEFAULT = 0, T3D_LIGHTING_FIXED_NORMALS = 1, T3D_LIGHTING_DOUBLE_SIDED = 2 };
+/* LightVolumeShadingMode (material.go:50-55) */
+enum {
+  T3D_LV_PER_VERTEX_WITHOUT_NORMAL = 0,
+  T3D_LV_PER_VERTEX_WITH_NORMAL = 1,
+  T3D_LV_PER_OBJECT = 2,
+  T3D_LV_OFF = 3
+};
+/* FogBlendMode (world.go:8-11) */
+enum { T3D_FOG_ADD = 0, T3D_FOG_SUB = 1, T3D_FOG_OVERWRITE = 2, T3D_FOG_TRANSPARENT = 3 };
+/* Light node types handled on the path (node.go:25-43) */
+enum { T3D_LIGHT_AMBIENT = 0, T3D_LIGHT_POINT = 1, T3D_LIGHT_DIRECTIONAL = 2, T3D_LIGHT_VOLUME = 3 };
+/* Execution forms of t3d_render */
+enum {
+  T3D_EXEC_PLAIN = 0,   /* one kernel chain per render pair, like one ProcessVertices+flush each */
+  T3D_EXEC_BATCHED = 1  /* all render pairs of the frame in one launch chain, replayed from a HIP graph */
+};
+
+/* Matrix4 (matrix.go:9-10): m[row][col], row vectors, translation in row 3. */
+typedef struct { float m[4][4]; } t3d_mat4;
+
+/* ebiten.Vertex (ebiten v2.9.7): DstX, DstY, SrcX, SrcY, ColorR..A, Custom0..3.
+ * Written by camera.go:1679-1810; fields the reference leaves untouched are 0 here. */
+typedef struct {
+  float dst_x, dst_y, src_x, src_y;
+  float color_r, color_g, color_b, color_a;
+  float custom0, custom1, custom2, custom3;
+} t3d_vertex;
+
+/* MeshPart triangle range (mesh.go:2183-2190): triangles
+ * [triangle_start, triangle_start + triangle_count) of the mesh. */
+typedef struct {
+  uint32_t triangle_start;
+  uint32_t triangle_count;
+} t3d_part_range;
+
+/* Mesh vertex SoA + triangles (mesh.go:260-302, Triangle mesh.go:1695-1711).
+ * Indices are absolute mesh vertex indices (Triangle.VertexIndexA/B/C after
+ * AddTriangles adds VertexIndexStart, mesh.go:2314).  Triangle.Center is
+ * derived from `positions` at upload exactly as RecalculateCenter does
+ * (mesh.go:1980-1982). */
+typedef struct {
+  uint32_t vertex_count;
+  const float* positions;            /* [V][3] VertexPositions */
+  const float* normals;              /* [V][3] VertexNormals */
+  const float* colors;               /* [V][4] VertexColors[VertexActiveColorChannel], NULL if channel < 0 */
+  const float* uvs;                  /* [V][2] VertexUVs, NULL = all zero */
+  const uint16_t* bone_indices;      /* [V][4] index into the model's bone palette, NULL for unskinned meshes */
+  const float* bone_weights;         /* [V][4] VertexWeights (0 = unused influence slot) */
+  uint32_t shape_key_count;          /* MeshShapeKey count (mesh.go:796-822) */
+  const float* shape_key_positions;  /* [K][V][3] */
+  const float* shape_key_normals;    /* [K][V][3] */
+  uint32_t triangle_count;
+  const uint32_t* indices;           /* [T][3] */
+  uint32_t part_count;
+  const t3d_part_range* parts;       /* [part_count] */
+  float dimensions_min[3];           /* Mesh.Dimensions (mesh.go:13-15) */
+  float dimensions_max[3];
+} t3d_mesh_desc;
+
+/* Material state read by the path (material.go:66-131). */
+typedef struct {
+  int32_t present;                   /* MeshPart.Material != nil */
+  int32_t visible;                   /* Material.Visible (mesh.go:2402-2407) */
+  float color[4];                    /* Material.Color */
+  int32_t backface_culling;          /* Material.BackfaceCulling */
+  int32_t triangle_sort_mode;        /* T3D_SORT_* */
+  int32_t shadeless;
+  int32_t fogless;
+  int32_t lighting_mode;             /* T3D_LIGHTING_* */
+  int32_t light_volume_shading_mode; /* T3D_LV_* */
+  int32_t transparency_mode;         /* T3D_TRANSPARENCY_* */
+  int32_t blend_source_over;         /* Material.Blend == ebiten.BlendSourceOver */
+  int32_t render_order;              /* Material.RenderOrder */
+  int32_t use_texture;               /* Material.Texture != nil && Material.UseTexture */
+  float texture_size[2];             /* texture Bounds().Dx(), Dy() */
+  int32_t billboard;                 /* BillboardEnabled: unsupported on the GPU path */
+  int32_t custom_depth_function;     /* CustomDepthFunction != nil: unsupported */
+  int32_t depth_mode;                /* BillboardedDepthMode */
+} t3d_material_desc;
+
+/* A light as seen by Camera.Render (light.go:12-30 and the concrete types). */
+typedef struct {
+  int32_t type;                      /* T3D_LIGHT_* */
+  int32_t visible;                   /* Node IsVisible() */
+  float color[3];                    /* LightBase.color RGB */
+  float energy;                      /* LightBase.energy */
+  float range;                       /* PointLight.Range() (light.go:269-277) */
+  t3d_mat4 transform;                /* Node.Transform(): world position is row 3 */
+  t3d_mat4 world_rotation;           /* Node.WorldRotation(): directional forward is row 2 */
+  /* LightVolume (light.go:376-680) */
+  uint64_t volume;                   /* handle from t3d_volume_upload (cell colours) */
+  float volume_min[3];               /* LightVolume.dimensions */
+  float volume_max[3];
+  float cell_size[3];
+  int32_t cell_count[3];
+  float offset[3];                   /* LightVolumeOffset */
+  float light_bias[3];               /* LightVolumeLightBias */
+} t3d_light_desc;
+
+/* Model (model.go:20-61) with the node state the path reads. */
+typedef struct {
+  uint64_t mesh;                     /* handle from t3d_mesh_upload */
+  t3d_mat4 transform;                /* Node.Transform() (node.go:572-608) */
+  t3d_mat4 world_rotation;           /* Node.WorldRotation() (node.go:916-938) */
+  float color[4];                    /* Model.Color */
+  int32_t visible;
+  int32_t shadeless;
+  int32_t frustum_culling;           /* Model.FrustumCulling */
+  int32_t auto_batch_static;         /* AutoBatchMode == AutoBatchStatic: skipped (camera.go:2069) */
+  int32_t skinned;
+  uint32_t bone_count;
+  const t3d_mat4* bone_influences;   /* [bone_count] bone.boneInfluence = inverseBind·world (node.go:592-594) */
+  const float* shape_key_weights;    /* [mesh shape_key_count] MeshShapeKey weights */
+  const t3d_material_desc* materials;/* [mesh part_count], one per MeshPart; NULL = no materials */
+  int32_t light_group_active;        /* LightGroup != nil && LightGroup.Active */
+  uint32_t light_group_count;
+  const t3d_light_desc* light_group; /* LightGroup.Lights */
+  int32_t has_vertex_transform_function; /* Go callback: unsupported */
+  int32_t has_vertex_clip_function;      /* Go callback: unsupported */
+} t3d_model_desc;
+
+/* Camera state (camera.go:256-321, 324-547). */
+typedef struct {
+  int32_t width, height;             /* result texture size */
+  int32_t perspective;
+  float fov;                         /* vertical FOV in degrees */
+  float near, far;
+  float ortho_scale;
+  t3d_mat4 transform;                /* Node.Transform(): world position is row 3 */
+  t3d_mat4 world_rotation;           /* Node.WorldRotation() at the last Clear() (camera.go:1085-1088) */
+  int32_t render_depth;              /* Camera.RenderDepth */
+  int32_t render_normals;            /* Camera.RenderNormals: unsupported */
+  int32_t perspective_corrected_texture_mapping;
+  float depth_margin;                /* Camera.DepthMargin */
+  float vertex_snapping;             /* Camera.VertexSnapping */
+  int32_t max_light_count;           /* Camera.MaxLightCount */
+  int32_t has_max_light_count_focus;
+  float max_light_count_focus[3];    /* MaxLightCountFocusPoint world position */
+} t3d_camera_desc;
+
+/* World (world.go:25-42). present = 0 means scene.World == nil. */
+typedef struct {
+  int32_t present;
+  int32_t lighting_on;
+  int32_t fog_on;
+  int32_t fog_mode;                  /* T3D_FOG_* */
+  float fog_color[4];
+  float fog_range[2];
+  float ambient_color[3];            /* World.AmbientLight */
+  float ambient_energy;
+  int32_t ambient_visible;
+} t3d_world_desc;
+
+/* One Camera.Render(scene, lights, models) call. */
+typedef struct {
+  const t3d_camera_desc* camera;
+  const t3d_world_desc* world;
+  uint32_t light_count;
+  const t3d_light_desc* lights;      /* the `lights` NodeIterator, in order */
+  uint32_t model_count;
+  const t3d_model_desc* models;      /* the `models` NodeIterator, in order */
+  int32_t exec_mode;                 /* T3D_EXEC_* */
+} t3d_render_desc;
+
+/* One render pair = one ProcessVertices + flush (camera.go:2057-2109). */
+typedef struct {
+  uint32_t model_index;              /* index into t3d_render_desc.models */
+  uint32_t part_index;               /* MeshPart index within the mesh */
+  uint32_t triangle_count;           /* MeshPart.TriangleCount() */
+  uint32_t survivor_count;           /* triangles drawn: the flush's vertexListIndex / 3 */
+  uint64_t vertex_offset;            /* first vertex of the pair in the frame's vertex arrays */
+} t3d_pair_result;
+
+typedef struct t3d_ctx t3d_ctx;
+typedef struct t3d_frame t3d_frame;
+
+/* ---- context --------------------------------------------------------- */
+
+/* Create a context on HIP device `device`.  hbm_budget_bytes bounds the
+ * device memory the context may hold (meshes + frames); 0 = no bound. */
+t3d_status t3d_ctx_create(int32_t device, uint64_t hbm_budget_bytes, t3d_ctx** out);
+t3d_status t3d_ctx_destroy(t3d_ctx* ctx);
+const char* t3d_last_error(const t3d_ctx* ctx);
+int32_t t3d_abi_version(void);
+/* Device bytes currently held by the context. */
+uint64_t t3d_ctx_memory_used(const t3d_ctx* ctx);
+
+/* ---- resources (resident in HBM) ------------------------------------- */
+
+/* Upload a Mesh; replaces the reference's per-frame reads of Mesh.Vertex*
+ * slices (mesh.go:274-290). */
+t3d_status t3d_mesh_upload(t3d_ctx* ctx, const t3d_mesh_desc* desc, uint64_t* out_handle);
+/* Free a mesh.  T3D_ERR_BUSY while a submitted job still references it. */
+t3d_status t3d_mesh_free(t3d_ctx* ctx, uint64_t handle);
+/* Upload LightVolume cell colours: [cell_count] RGBA float32, cell order
+ * x + y*cx + z*cx*cy (getLightCell, light.go:1267-1272). */
+t3d_status t3d_volume_upload(t3d_ctx* ctx, const float* cells_rgba, uint32_t cell_count, uint64_t* out_handle);
+t3d_status t3d_volume_free(t3d_ctx* ctx, uint64_t handle);
+
+/* ---- frames ---------------------------------------------------------- */
+
+/* A frame owns the device output arrays of one Camera.Render call and the
+ * per-frame scratch.  It is reused across renders; outputs stay valid until
+ * the next t3d_render into the same frame. */
+t3d_status t3d_frame_create(t3d_ctx* ctx, t3d_frame** out);
+t3d_status t3d_frame_destroy(t3d_ctx* ctx, t3d_frame* frame);
+
+/* Camera.Render (camera.go:1269): runs the whole geometry path on the device
+ * and waits for it.  On return the frame holds, per render pair in the
+ * reference's draw order (solids in traversal order, then transparents
+ * sorted by RenderOrder and distance, camera.go:2050-2109), the colour
+ * vertices, depth vertices and indices that flush() would draw. */
+t3d_status t3d_render(t3d_ctx* ctx, t3d_frame* frame, const t3d_render_desc* desc);
+
+/* Number of render pairs of the last render and their results. */
+t3d_status t3d_frame_pairs(const t3d_frame* frame, uint32_t* out_count, const t3d_pair_result** out_pairs);
+/* Device pointers of the frame's output arrays (valid until the next
+ * render into this frame).  Pair p occupies vertices
+ * [vertex_offset, vertex_offset + 3*survivor_count). */
+t3d_status t3d_frame_device_outputs(const t3d_frame* frame, const t3d_vertex** color_vertices,
+                                    const t3d_vertex** depth_vertices, const uint32_t** indices);
+/* Copy all pairs' outputs, concatenated in pair order, to host memory.
+ * Buffers must hold 3*sum(survivor_count) entries each; any may be NULL. */
+t3d_status t3d_frame_download(t3d_ctx* ctx, const t3d_frame* frame, t3d_vertex* color_vertices,
+                              t3d_vertex* depth_vertices, uint32_t* indices);
+/* Mesh triangle index of every drawn triangle, pair-concatenated in draw
+ * order: the order sortingTriangleBucket.ForEach visits them
+ * (sortingTriangles.go:93-125).  Holds sum(survivor_count) entries. */
+t3d_status t3d_frame_download_triangles(t3d_ctx* ctx, const t3d_frame* frame, uint32_t* triangle_ids);
+/* Wall-clock milliseconds of the last t3d_render (host prologue + device
+ * work + survivor readback). */
+double t3d_frame_last_ms(const t3d_frame* frame);
+/* Duration in milliseconds of the last render's device work, measured with
+ * HIP events on the context's stream; per_kernel receives up to
+ * max_kernels (name, total ms, launches) rows. */
+typedef struct {
+  char name[32];
+  double total_ms;
+  uint32_t launches;
+  uint64_t algorithmic_bytes;        /* algorithmic HBM bytes the launches of this kernel moved */
+} t3d_kernel_time;
+t3d_status t3d_profile_enable(t3d_ctx* ctx, int32_t on);
+t3d_status t3d_profile_read(t3d_ctx* ctx, t3d_kernel_time* per_kernel, uint32_t max_kernels,
+                            uint32_t* out_count, int32_t reset);
+
+/* ---- tuning ---------------------------------------------------------- */
+
+/* Launch shape of the batched form: chosen by t3d_autotune or set by hand. */
+typedef struct {
+  uint32_t vertex_block;             /* threads per block, vertex kernel */
+  uint32_t vertices_per_thread;
+  uint32_t triangle_tile;            /* triangles per tile in the cull/sort kernels */
+  uint32_t small_part_triangles;     /* parts up to this size take the fused one-block path */
+  uint32_t use_graph;                /* replay the batched chain from a HIP graph */
+} t3d_tuning;
+t3d_status t3d_get_tuning(const t3d_ctx* ctx, t3d_tuning* out);
+t3d_status t3d_set_tuning(t3d_ctx* ctx, const t3d_tuning* tuning);
+/* Time candidate launch shapes on `desc` and keep the fastest; writes the
+ * winner to *out (may be NULL). */
+t3d_status t3d_autotune(t3d_ctx* ctx, t3d_frame* frame, const t3d_render_desc* desc, uint32_t reps,
+                        t3d_tuning* out);
+
+/* ---- self checks ------------------------------------------------------ */
+
+/* Runs the device's float32 primitives on n inputs: out[0..n) = x/y,
+ * out[n..2n) = sqrt(|x|), out[2n..3n) = x*y + x (unfused), out[3n..4n) =
+ * round-half-away(x).  Lets a test confirm the device rounds exactly as the
+ * host does.  Host pointers; n <= 1<<20. */
+t3d_status t3d_selftest_fp(t3d_ctx* ctx, const float* x, const float* y, float* out, uint32_t n);
+/* sizeof of each public struct, for binding checks: fills up to max entries
+ * in the order vertex, mesh_desc, material_desc, light_desc, model_desc,
+ * camera_desc, world_desc, render_desc, pair_result, kernel_time, tuning. */
+uint32_t t3d_struct_sizes(uint32_t* sizes, uint32_t max);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* T3D_H */

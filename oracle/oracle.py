@@ -1,0 +1,152 @@
+"""ctypes binding of the CPU oracle (oracle/libt3d_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by bench.py's CPU-baseline leg, always as the checker.  The product package
+never imports this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from tetra3d_amd import _abi as A
+from tetra3d_amd.scene import RenderInputs
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libt3d_oracle.so")
+
+
+class OracleResult(C.Structure):
+    _fields_ = [
+        ("pair_count", C.c_uint32),
+        ("pairs", C.POINTER(A.PairResult)),
+        ("vertex_count", C.c_uint64),
+        ("color", C.POINTER(A.Vertex)),
+        ("depth", C.POINTER(A.Vertex)),
+        ("indices", C.POINTER(C.c_uint32)),
+        ("triangle_ids", C.POINTER(C.c_uint32)),
+        ("depth_keys", C.POINTER(C.c_float)),
+    ]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        M4 = C.POINTER(A.Mat4)
+        FP = C.POINTER(C.c_float)
+        L.t3do_create.restype = C.c_void_p
+        L.t3do_destroy.argtypes = [C.c_void_p]
+        L.t3do_mesh_register.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(A.MeshDesc)]
+        L.t3do_volume_register.argtypes = [C.c_void_p, C.c_uint64, FP, C.c_uint32]
+        L.t3do_render.argtypes = [C.c_void_p, C.POINTER(A.RenderDesc), C.POINTER(OracleResult)]
+        L.t3do_result_free.argtypes = [C.POINTER(OracleResult)]
+        L.t3do_last_error.restype = C.c_char_p
+        L.t3do_last_error.argtypes = [C.c_void_p]
+        for n in ("t3do_mat4_mult",):
+            getattr(L, n).argtypes = [M4, M4, M4]
+        L.t3do_mat4_inverted.argtypes = [M4, M4]
+        L.t3do_mat4_transposed.argtypes = [M4, M4]
+        L.t3do_mat4_decompose.argtypes = [M4, FP, FP, M4]
+        L.t3do_mat4_mult_vec.argtypes = [M4, FP, FP]
+        L.t3do_mat4_mult_vec_w.argtypes = [M4, FP, FP]
+        L.t3do_projection_perspective.argtypes = [C.c_float] * 5 + [M4]
+        L.t3do_view_matrix.argtypes = [C.POINTER(A.CameraDesc), M4]
+        L.t3do_vp_matrix.argtypes = [C.POINTER(A.CameraDesc), M4]
+        L.t3do_vec3_unit.argtypes = [FP, FP]
+        L.t3do_sphere_in_frustum.argtypes = [C.POINTER(A.CameraDesc), FP, C.c_float]
+        L.t3do_model_frustum_sphere.argtypes = [M4, C.c_int, FP, FP, FP, FP]
+        L.t3do_inverted_cam_pos.argtypes = [M4, FP, FP]
+        L.t3do_bucket_sort.argtypes = [FP, C.c_uint32, C.c_float, C.c_float, C.c_int32, C.POINTER(C.c_uint32)]
+        L.t3do_bucket_sort.restype = C.c_uint32
+        L.t3do_bucket_bin.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int32]
+        L.t3do_bucket_bin.restype = C.c_uint32
+        L.t3do_go_stable_sort_table.argtypes = [C.POINTER(C.c_int32), C.c_uint32, C.POINTER(C.c_uint8)]
+        _lib = L
+    return _lib
+
+
+def _vertices(ptr, n):
+    if n == 0:
+        return np.zeros((0, 12), dtype=np.float32)
+    return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_float)), shape=(n * 12,)).reshape(n, 12).copy()
+
+
+class Frame:
+    """Outputs of one oracle render, pair-concatenated like the GPU download."""
+
+    def __init__(self, res):
+        n = int(res.vertex_count)
+        self.pairs = [(p.model_index, p.part_index, p.triangle_count, p.survivor_count, p.vertex_offset)
+                      for p in res.pairs[:res.pair_count]]
+        self.color = _vertices(res.color, n)
+        self.depth = _vertices(res.depth, n)
+        self.indices = np.ctypeslib.as_array(res.indices, shape=(n,)).copy() if n else np.zeros(0, np.uint32)
+        t = n // 3
+        self.triangle_ids = np.ctypeslib.as_array(res.triangle_ids, shape=(t,)).copy() if t else np.zeros(0, np.uint32)
+        self.depth_keys = np.ctypeslib.as_array(res.depth_keys, shape=(t,)).copy() if t else np.zeros(0, np.float32)
+
+
+class Oracle:
+    """An oracle context holding registered meshes and volumes."""
+
+    def __init__(self):
+        self.L = lib()
+        self.ctx = self.L.t3do_create()
+        self._handles = {}
+
+    def close(self):
+        if self.ctx:
+            self.L.t3do_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        self.close()
+
+    def register_mesh(self, mesh, handle):
+        d = mesh.desc()
+        rc = self.L.t3do_mesh_register(self.ctx, handle, C.byref(d))
+        if rc != 0:
+            raise RuntimeError("oracle mesh register: %s" % self.L.t3do_last_error(self.ctx).decode())
+        self._handles[id(mesh)] = handle
+
+    def register_volume(self, vol, handle):
+        cells = np.ascontiguousarray(vol.cells, dtype=np.float32)
+        self.L.t3do_volume_register(self.ctx, handle, cells.ctypes.data_as(C.POINTER(C.c_float)), len(cells))
+
+    def handle_of(self, mesh):
+        return self._handles[id(mesh)]
+
+    def render(self, camera, scene, lights, models):
+        inp = RenderInputs(camera, scene, lights, models, self.handle_of)
+        res = OracleResult()
+        rc = self.L.t3do_render(self.ctx, C.byref(inp.render), C.byref(res))
+        if rc != 0:
+            raise RuntimeError("oracle render (%d): %s" % (rc, self.L.t3do_last_error(self.ctx).decode()))
+        try:
+            return Frame(res)
+        finally:
+            self.L.t3do_result_free(C.byref(res))
+
+    def render_workload(self, wl):
+        """Register the workload's meshes/volumes (handles = the ones the GPU
+        side assigned, stored on each object) and render it."""
+        for i, m in enumerate(wl.meshes):
+            if id(m) not in self._handles:
+                h = m._handles.get("gpu", 1000000 + i)
+                self.register_mesh(m, h)
+        for v in wl.volumes:
+            if v.handle == 0:
+                v.handle = 2000000 + wl.volumes.index(v)
+            self.register_volume(v, v.handle)
+        return self.render(wl.camera, wl.scene, wl.lights, wl.models)

@@ -1,0 +1,1450 @@
+/*
+ * t3d_oracle.c — CPU restatement of Tetra3D's per-frame geometry path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Only the tests,
+ * the smoke check and the benchmark's CPU-baseline leg load this code.
+ *
+ * Build: gcc -O2 -ffp-contract=off -fno-fast-math (see oracle/Makefile).  Go on
+ * amd64 (GOAMD64=v1, the default) evaluates float32 expressions left to right,
+ * rounds every operation to float32 and never fuses a*b+c; the same holds for
+ * C on x86-64 with contraction off.  math32 functions are float64 round trips
+ * (math32/math32.go:114-149, 476-478): Sqrt and Round are exact in both
+ * languages; Tan/Cos/Atan only feed camera-level constants and use the C
+ * library's double routines here (Go's differ by at most one double ulp,
+ * which survives rounding to float32 only in rare boundary cases).
+ *
+ * Every function cites the reference file:line it restates.  Parity status of
+ * each part is recorded next to its tests.
+ */
+#include "t3d_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MAXF32 3.40282346638528859811704183484516925440e+38f /* math.MaxFloat32 */
+#define NBINS 512                                          /* sortingTriangles.go:166-168 */
+
+typedef struct { float x, y, z; } v3;
+typedef struct { float x, y, z, w; } v4;
+typedef t3d_mat4 m4;
+
+/* ------------------------------------------------------------------------ */
+/* math32 (math32/math32.go)                                                 */
+/* ------------------------------------------------------------------------ */
+
+static float sqrt32(float x) { return (float)sqrt((double)x); }   /* math32.go:121-123 */
+static float round32(float x) { return (float)round((double)x); } /* math32.go:109-111, half away from zero */
+static float tan32(float x) { return (float)tan((double)x); }     /* math32.go:476-478 */
+static float cos32(float x) { return (float)cos((double)x); }     /* math32.go:143-145 */
+static float atan32(float x) { return (float)atan((double)x); }
+static float abs32(float x) { return (float)fabs((double)x); }    /* math32.go:227-229 */
+static float clamp32(float v, float lo, float hi) {               /* math32.go:38-45 */
+  if (v < lo) return lo;
+  else if (v > hi) return hi;
+  return v;
+}
+static int isnan32(float x) { return x != x; }
+static int isinf32(float x) { return x == INFINITY || x == -INFINITY; }
+static const float PI32 = (float)3.14159265358979323846264338327950288419716939937510582097494459; /* math32.go Pi */
+
+/* Go's int(float32) on amd64 (CVTTSS2SQ): truncation, and the integer
+ * indefinite value (minimum int64) for NaN or out-of-range inputs. */
+static int64_t go_int(float f) {
+  if (!(f >= -9.2233720368547758e18f && f < 9.2233720368547758e18f)) return INT64_MIN;
+  return (int64_t)f;
+}
+
+/* ------------------------------------------------------------------------ */
+/* vectors (vector.go)                                                       */
+/* ------------------------------------------------------------------------ */
+
+static v3 v3_sub(v3 a, v3 b) { v3 r = {a.x - b.x, a.y - b.y, a.z - b.z}; return r; }
+static v3 v3_add(v3 a, v3 b) { v3 r = {a.x + b.x, a.y + b.y, a.z + b.z}; return r; }
+static v3 v3_mul(v3 a, v3 b) { v3 r = {a.x * b.x, a.y * b.y, a.z * b.z}; return r; }     /* vector.go:250-255 */
+static float v3_dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }           /* vector.go:526-528 */
+static float v3_mag(v3 a) { return sqrt32(a.x * a.x + a.y * a.y + a.z * a.z); }          /* vector.go:188-190 */
+static float v3_dist_sq(v3 a, v3 b) {                                                    /* vector.go:242-247 */
+  a.x -= b.x;
+  a.y -= b.y;
+  a.z -= b.z;
+  return a.x * a.x + a.y * a.y + a.z * a.z;
+}
+static v3 v3_unit(v3 v) {                                                                /* vector.go:258-266 */
+  float l = v3_mag(v);
+  if (l < 1e-8f || l == 1) return v;
+  v.x = v.x / l;
+  v.y = v.y / l;
+  v.z = v.z / l;
+  return v;
+}
+static v4 v4_unit(v4 v) {                                                                /* vector.go:1869-1877, W kept */
+  float l = sqrt32(v.x * v.x + v.y * v.y + v.z * v.z);
+  if (l < 1e-8f || l == 1) return v;
+  v.x = v.x / l;
+  v.y = v.y / l;
+  v.z = v.z / l;
+  return v;
+}
+static float v4_mag(v4 v) { return sqrt32(v.x * v.x + v.y * v.y + v.z * v.z); }         /* vector.go:1880-1882 */
+static v3 v3_from(const float* f) { v3 r = {f[0], f[1], f[2]}; return r; }
+
+/* ------------------------------------------------------------------------ */
+/* matrices (matrix.go, matrix_none.go, matrix_amd64.s)                      */
+/* ------------------------------------------------------------------------ */
+
+static m4 m4_identity(void) {                                                            /* matrix.go:13-24 */
+  m4 r;
+  memset(&r, 0, sizeof r);
+  r.m[0][0] = r.m[1][1] = r.m[2][2] = r.m[3][3] = 1;
+  return r;
+}
+static m4 m4_zero(void) { m4 r; memset(&r, 0, sizeof r); return r; }
+
+/* Mult: matrix_none.go:6-30; matrix_amd64.s:4-81 accumulates
+ * ((a0·r0 + a1·r1) + a2·r2) + a3·r3 with MULPS/ADDPS, the same order. */
+static m4 m4_mult(const m4* a, const m4* b) {
+  m4 r;
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++)
+      r.m[i][j] = a->m[i][0] * b->m[0][j] + a->m[i][1] * b->m[1][j] + a->m[i][2] * b->m[2][j] +
+                  a->m[i][3] * b->m[3][j];
+  return r;
+}
+static m4 m4_transposed(const m4* a) {                                                   /* matrix.go:218-230 */
+  m4 r = m4_identity();
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) r.m[i][j] = a->m[j][i];
+  return r;
+}
+static m4 m4_translate(float x, float y, float z) {                                     /* matrix.go:82-88 */
+  m4 r = m4_identity();
+  r.m[3][0] = x;
+  r.m[3][1] = y;
+  r.m[3][2] = z;
+  return r;
+}
+/* Inverted: matrix.go:255-307 (cofactor form), operation order as written. */
+static m4 m4_inverted(const m4* mat) {
+  const float(*m)[4] = mat->m;
+  float A2323 = m[2][2] * m[3][3] - m[2][3] * m[3][2];
+  float A1323 = m[2][1] * m[3][3] - m[2][3] * m[3][1];
+  float A1223 = m[2][1] * m[3][2] - m[2][2] * m[3][1];
+  float A0323 = m[2][0] * m[3][3] - m[2][3] * m[3][0];
+  float A0223 = m[2][0] * m[3][2] - m[2][2] * m[3][0];
+  float A0123 = m[2][0] * m[3][1] - m[2][1] * m[3][0];
+  float A2313 = m[1][2] * m[3][3] - m[1][3] * m[3][2];
+  float A1313 = m[1][1] * m[3][3] - m[1][3] * m[3][1];
+  float A1213 = m[1][1] * m[3][2] - m[1][2] * m[3][1];
+  float A2312 = m[1][2] * m[2][3] - m[1][3] * m[2][2];
+  float A1312 = m[1][1] * m[2][3] - m[1][3] * m[2][1];
+  float A1212 = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+  float A0313 = m[1][0] * m[3][3] - m[1][3] * m[3][0];
+  float A0213 = m[1][0] * m[3][2] - m[1][2] * m[3][0];
+  float A0312 = m[1][0] * m[2][3] - m[1][3] * m[2][0];
+  float A0212 = m[1][0] * m[2][2] - m[1][2] * m[2][0];
+  float A0113 = m[1][0] * m[3][1] - m[1][1] * m[3][0];
+  float A0112 = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+
+  float det = m[0][0] * (m[1][1] * A2323 - m[1][2] * A1323 + m[1][3] * A1223) -
+              m[0][1] * (m[1][0] * A2323 - m[1][2] * A0323 + m[1][3] * A0223) +
+              m[0][2] * (m[1][0] * A1323 - m[1][1] * A0323 + m[1][3] * A0123) -
+              m[0][3] * (m[1][0] * A1223 - m[1][1] * A0223 + m[1][2] * A0123);
+  det = 1 / det;
+
+  m4 r = m4_identity();
+  r.m[0][0] = det * (m[1][1] * A2323 - m[1][2] * A1323 + m[1][3] * A1223);
+  r.m[0][1] = det * -(m[0][1] * A2323 - m[0][2] * A1323 + m[0][3] * A1223);
+  r.m[0][2] = det * (m[0][1] * A2313 - m[0][2] * A1313 + m[0][3] * A1213);
+  r.m[0][3] = det * -(m[0][1] * A2312 - m[0][2] * A1312 + m[0][3] * A1212);
+  r.m[1][0] = det * -(m[1][0] * A2323 - m[1][2] * A0323 + m[1][3] * A0223);
+  r.m[1][1] = det * (m[0][0] * A2323 - m[0][2] * A0323 + m[0][3] * A0223);
+  r.m[1][2] = det * -(m[0][0] * A2313 - m[0][2] * A0313 + m[0][3] * A0213);
+  r.m[1][3] = det * (m[0][0] * A2312 - m[0][2] * A0312 + m[0][3] * A0212);
+  r.m[2][0] = det * (m[1][0] * A1323 - m[1][1] * A0323 + m[1][3] * A0123);
+  r.m[2][1] = det * -(m[0][0] * A1323 - m[0][1] * A0323 + m[0][3] * A0123);
+  r.m[2][2] = det * (m[0][0] * A1313 - m[0][1] * A0313 + m[0][3] * A0113);
+  r.m[2][3] = det * -(m[0][0] * A1312 - m[0][1] * A0312 + m[0][3] * A0112);
+  r.m[3][0] = det * -(m[1][0] * A1223 - m[1][1] * A0223 + m[1][2] * A0123);
+  r.m[3][1] = det * (m[0][0] * A1223 - m[0][1] * A0223 + m[0][2] * A0123);
+  r.m[3][2] = det * -(m[0][0] * A1213 - m[0][1] * A0213 + m[0][2] * A0113);
+  r.m[3][3] = det * (m[0][0] * A1212 - m[0][1] * A0212 + m[0][2] * A0112);
+  return r;
+}
+static v4 m4_row(const m4* a, int i) { v4 r = {a->m[i][0], a->m[i][1], a->m[i][2], a->m[i][3]}; return r; }
+static void m4_set_row(m4* a, int i, v4 v) {
+  a->m[i][0] = v.x;
+  a->m[i][1] = v.y;
+  a->m[i][2] = v.z;
+  a->m[i][3] = v.w;
+}
+/* Decompose: matrix.go:181-196. */
+static void m4_decompose(const m4* a, v3* pos, v3* scale, m4* rot) {
+  pos->x = a->m[3][0];
+  pos->y = a->m[3][1];
+  pos->z = a->m[3][2];
+  *rot = m4_identity();
+  m4_set_row(rot, 0, v4_unit(m4_row(a, 0)));
+  m4_set_row(rot, 1, v4_unit(m4_row(a, 1)));
+  m4_set_row(rot, 2, v4_unit(m4_row(a, 2)));
+  m4 rt = m4_transposed(rot);
+  m4 in = m4_mult(a, &rt);
+  scale->x = v4_mag(m4_row(&in, 0));
+  scale->y = v4_mag(m4_row(&in, 1));
+  scale->z = v4_mag(m4_row(&in, 2));
+}
+/* MultVec: matrix.go:619-627. */
+static v3 m4_mult_vec(const m4* a, v3 v) {
+  v3 r;
+  r.x = a->m[0][0] * v.x + a->m[1][0] * v.y + a->m[2][0] * v.z + a->m[3][0];
+  r.y = a->m[0][1] * v.x + a->m[1][1] * v.y + a->m[2][1] * v.z + a->m[3][1];
+  r.z = a->m[0][2] * v.x + a->m[1][2] * v.y + a->m[2][2] * v.z + a->m[3][2];
+  return r;
+}
+/* MultVecW: matrix.go:630-637; also the inlined form in model.go:772-775, 795-798. */
+static v4 m4_mult_vec_w(const m4* a, v3 v) {
+  v4 r;
+  r.x = a->m[0][0] * v.x + a->m[1][0] * v.y + a->m[2][0] * v.z + a->m[3][0];
+  r.y = a->m[0][1] * v.x + a->m[1][1] * v.y + a->m[2][1] * v.z + a->m[3][1];
+  r.z = a->m[0][2] * v.x + a->m[1][2] * v.y + a->m[2][2] * v.z + a->m[3][2];
+  r.w = a->m[0][3] * v.x + a->m[1][3] * v.y + a->m[2][3] * v.z + a->m[3][3];
+  return r;
+}
+/* Forward/Right/Up: matrix.go:142-176. */
+static v3 m4_forward(const m4* a) { v3 r = {a->m[2][0], a->m[2][1], a->m[2][2]}; return v3_unit(r); }
+static v3 m4_right(const m4* a) { v3 r = {a->m[0][0], a->m[0][1], a->m[0][2]}; return v3_unit(r); }
+static v3 m4_up(const m4* a) { v3 r = {a->m[1][0], a->m[1][1], a->m[1][2]}; return v3_unit(r); }
+static v3 m4_row3(const m4* a) { v3 r = {a->m[3][0], a->m[3][1], a->m[3][2]}; return r; }  /* WorldPosition, node.go:737-739 */
+
+/* NewProjectionMatrix4Perspective: matrix.go:588-604. */
+static m4 projection_perspective(float fovy, float near, float far, float vw, float vh) {
+  float aspect = vw / vh;
+  float t = tan32(fovy * PI32 / 360);
+  float b = -t;
+  float r = t * aspect;
+  float l = -r;
+  m4 p = m4_zero();
+  p.m[0][0] = (2 * near) / (r - l);
+  p.m[0][2] = (r + l) / (r - l);
+  p.m[1][1] = (2 * near) / (t - b);
+  p.m[1][2] = (t + b) / (t - b);
+  p.m[2][2] = -((far + near) / (far - near));
+  p.m[2][3] = -((2 * far * near) / (far - near));
+  p.m[3][2] = -1;
+  return p;
+}
+/* NewProjectionMatrix4Orthographic: matrix.go:609-616. */
+static m4 projection_orthographic(float near, float far, float right, float left, float top, float bottom) {
+  m4 p = m4_zero();
+  p.m[0][0] = 2 / (right - left);
+  p.m[1][1] = 2 / (top - bottom);
+  p.m[2][2] = -2 / (far - near);
+  p.m[3][3] = 1;
+  return p;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Go sort.SliceStable (Go 1.24 sort package: stable_func, insertionSort_func, */
+/* symMerge_func, rotate_func, swapRange_func).  Needed only where the       */
+/* reference passes a comparator that is not a strict weak order            */
+/* (camera.go:1458-1467, 1483-1495); elsewhere any stable sort agrees.       */
+/* ------------------------------------------------------------------------ */
+
+typedef int (*less_fn)(void* env, int32_t a, int32_t b);
+typedef struct { int32_t* d; less_fn less; void* env; } go_sorter;
+
+static int gs_less(go_sorter* s, int i, int j) { return s->less(s->env, s->d[i], s->d[j]); }
+static void gs_swap(go_sorter* s, int i, int j) { int32_t t = s->d[i]; s->d[i] = s->d[j]; s->d[j] = t; }
+static void gs_insertion(go_sorter* s, int a, int b) {
+  for (int i = a + 1; i < b; i++)
+    for (int j = i; j > a && gs_less(s, j, j - 1); j--) gs_swap(s, j, j - 1);
+}
+static void gs_swap_range(go_sorter* s, int a, int b, int n) {
+  for (int i = 0; i < n; i++) gs_swap(s, a + i, b + i);
+}
+static void gs_rotate(go_sorter* s, int a, int m, int b) {
+  int i = m - a, j = b - m;
+  while (i != j) {
+    if (i > j) {
+      gs_swap_range(s, m - i, m, j);
+      i -= j;
+    } else {
+      gs_swap_range(s, m - i, m + j - i, i);
+      j -= i;
+    }
+  }
+  gs_swap_range(s, m - i, m, i);
+}
+static void gs_sym_merge(go_sorter* s, int a, int m, int b) {
+  if (m - a == 1) {
+    int i = m, j = b;
+    while (i < j) {
+      int h = (int)((unsigned)(i + j) >> 1);
+      if (gs_less(s, h, a)) i = h + 1;
+      else j = h;
+    }
+    for (int k = a; k < i - 1; k++) gs_swap(s, k, k + 1);
+    return;
+  }
+  if (b - m == 1) {
+    int i = a, j = m;
+    while (i < j) {
+      int h = (int)((unsigned)(i + j) >> 1);
+      if (!gs_less(s, m, h)) i = h + 1;
+      else j = h;
+    }
+    for (int k = m; k > i; k--) gs_swap(s, k, k - 1);
+    return;
+  }
+  int mid = (int)((unsigned)(a + b) >> 1);
+  int n = mid + m;
+  int start, r;
+  if (m > mid) {
+    start = n - b;
+    r = mid;
+  } else {
+    start = a;
+    r = m;
+  }
+  int p = n - 1;
+  while (start < r) {
+    int c = (int)((unsigned)(start + r) >> 1);
+    if (!gs_less(s, p - c, c)) start = c + 1;
+    else r = c;
+  }
+  int end = n - start;
+  if (start < m && m < end) gs_rotate(s, start, m, end);
+  if (a < start && start < mid) gs_sym_merge(s, a, start, mid);
+  if (mid < end && end < b) gs_sym_merge(s, mid, end, b);
+}
+static void go_stable_sort(int32_t* d, int n, less_fn less, void* env) {
+  go_sorter s = {d, less, env};
+  int block = 20, a = 0, b = block;
+  while (b <= n) {
+    gs_insertion(&s, a, b);
+    a = b;
+    b += block;
+  }
+  gs_insertion(&s, a, n);
+  while (block < n) {
+    a = 0;
+    b = 2 * block;
+    while (b <= n) {
+      gs_sym_merge(&s, a, a + block, b);
+      a = b;
+      b += 2 * block;
+    }
+    int m = a + block;
+    if (m < n) gs_sym_merge(&s, a, m, n);
+    block *= 2;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* oracle context: registered meshes and light volumes                       */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  uint64_t handle;
+  t3d_mesh_desc d;   /* pointers below own copies */
+  float* pos;
+  float* nrm;
+  float* col;
+  float* uv;
+  uint16_t* bones;
+  float* weights;
+  float* sk_pos;
+  float* sk_nrm;
+  uint32_t* idx;
+  t3d_part_range* parts;
+  v3* tri_center;    /* Triangle.Center (mesh.go:1980-1982) */
+} omesh;
+
+typedef struct {
+  uint64_t handle;
+  float* cells;
+  uint32_t n;
+} ovolume;
+
+struct t3do_ctx {
+  omesh* meshes;
+  int nmeshes, cap_meshes;
+  ovolume* vols;
+  int nvols, cap_vols;
+  char err[512];
+};
+
+static int set_err(t3do_ctx* c, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(c->err, sizeof c->err, fmt, ap);
+  va_end(ap);
+  return T3D_ERR_INVALID_ARGUMENT;
+}
+
+t3do_ctx* t3do_create(void) { return (t3do_ctx*)calloc(1, sizeof(t3do_ctx)); }
+
+static void omesh_free(omesh* m) {
+  free(m->pos); free(m->nrm); free(m->col); free(m->uv); free(m->bones); free(m->weights);
+  free(m->sk_pos); free(m->sk_nrm); free(m->idx); free(m->parts); free(m->tri_center);
+}
+
+void t3do_destroy(t3do_ctx* c) {
+  if (!c) return;
+  for (int i = 0; i < c->nmeshes; i++) omesh_free(&c->meshes[i]);
+  for (int i = 0; i < c->nvols; i++) free(c->vols[i].cells);
+  free(c->meshes);
+  free(c->vols);
+  free(c);
+}
+
+const char* t3do_last_error(const t3do_ctx* c) { return c ? c->err : "null context"; }
+
+static void* dup_mem(const void* p, size_t n) {
+  if (!p || n == 0) return NULL;
+  void* r = malloc(n);
+  memcpy(r, p, n);
+  return r;
+}
+
+int t3do_mesh_register(t3do_ctx* c, uint64_t handle, const t3d_mesh_desc* d) {
+  if (!d || !d->positions || !d->normals || (!d->indices && d->triangle_count)) return set_err(c, "mesh: missing arrays");
+  for (uint32_t t = 0; t < 3u * d->triangle_count; t++)
+    if (d->indices[t] >= d->vertex_count) return set_err(c, "mesh: index out of range");
+  if (c->nmeshes == c->cap_meshes) {
+    c->cap_meshes = c->cap_meshes ? 2 * c->cap_meshes : 16;
+    c->meshes = (omesh*)realloc(c->meshes, sizeof(omesh) * c->cap_meshes);
+  }
+  omesh* m = &c->meshes[c->nmeshes++];
+  memset(m, 0, sizeof *m);
+  m->handle = handle;
+  m->d = *d;
+  size_t V = d->vertex_count, T = d->triangle_count, K = d->shape_key_count;
+  m->pos = (float*)dup_mem(d->positions, V * 12);
+  m->nrm = (float*)dup_mem(d->normals, V * 12);
+  m->col = (float*)dup_mem(d->colors, V * 16);
+  m->uv = (float*)dup_mem(d->uvs, V * 8);
+  m->bones = (uint16_t*)dup_mem(d->bone_indices, V * 8);
+  m->weights = (float*)dup_mem(d->bone_weights, V * 16);
+  m->sk_pos = (float*)dup_mem(d->shape_key_positions, K * V * 12);
+  m->sk_nrm = (float*)dup_mem(d->shape_key_normals, K * V * 12);
+  m->idx = (uint32_t*)dup_mem(d->indices, T * 12);
+  m->parts = (t3d_part_range*)dup_mem(d->parts, sizeof(t3d_part_range) * d->part_count);
+  m->tri_center = (v3*)malloc(sizeof(v3) * (T ? T : 1));
+  /* RecalculateCenter: mesh.go:1980-1982 */
+  for (size_t t = 0; t < T; t++) {
+    const float* a = m->pos + 3 * (size_t)m->idx[3 * t + 0];
+    const float* b = m->pos + 3 * (size_t)m->idx[3 * t + 1];
+    const float* cc = m->pos + 3 * (size_t)m->idx[3 * t + 2];
+    m->tri_center[t].x = (a[0] + b[0] + cc[0]) / 3;
+    m->tri_center[t].y = (a[1] + b[1] + cc[1]) / 3;
+    m->tri_center[t].z = (a[2] + b[2] + cc[2]) / 3;
+  }
+  return T3D_OK;
+}
+
+int t3do_volume_register(t3do_ctx* c, uint64_t handle, const float* cells, uint32_t n) {
+  if (c->nvols == c->cap_vols) {
+    c->cap_vols = c->cap_vols ? 2 * c->cap_vols : 8;
+    c->vols = (ovolume*)realloc(c->vols, sizeof(ovolume) * c->cap_vols);
+  }
+  ovolume* v = &c->vols[c->nvols++];
+  v->handle = handle;
+  v->n = n;
+  v->cells = (float*)dup_mem(cells, (size_t)n * 16);
+  return T3D_OK;
+}
+
+static omesh* find_mesh(t3do_ctx* c, uint64_t h) {
+  for (int i = 0; i < c->nmeshes; i++)
+    if (c->meshes[i].handle == h) return &c->meshes[i];
+  return NULL;
+}
+static ovolume* find_vol(t3do_ctx* c, uint64_t h) {
+  for (int i = 0; i < c->nvols; i++)
+    if (c->vols[i].handle == h) return &c->vols[i];
+  return NULL;
+}
+
+/* ------------------------------------------------------------------------ */
+/* camera                                                                    */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  const t3d_camera_desc* d;
+  v3 pos;                 /* WorldPosition */
+  m4 view, proj, vp;
+  v3 forward, right, up;  /* cached at Clear (camera.go:1085-1088) */
+  float sf_x, sf_y, sf_tang;
+  float aspect;
+} ocam;
+
+/* ViewMatrix: camera.go:638-648. */
+static m4 view_matrix(const t3d_camera_desc* cam) {
+  v3 cp = m4_row3(&cam->transform);
+  m4 t = m4_translate(-cp.x, -cp.y, -cp.z);
+  m4 rt = m4_transposed(&cam->world_rotation);
+  return m4_mult(&t, &rt);
+}
+
+/* Projection + sphere factors: camera.go:651-679 (3.1415 quirk at 660). */
+static void cam_setup(ocam* c, const t3d_camera_desc* d) {
+  c->d = d;
+  c->pos = m4_row3(&d->transform);
+  c->aspect = (float)d->width / (float)d->height;                   /* camera.go:986-992 */
+  float angle = d->fov * 3.1415f / 360;
+  c->sf_tang = tan32(angle);
+  c->sf_y = 1.0f / cos32(angle);
+  c->sf_x = 1.0f / cos32(atan32(c->sf_tang * c->aspect));
+  if (d->perspective) {
+    c->proj = projection_perspective(d->fov, d->near, d->far, (float)d->width, (float)d->height);
+  } else {
+    float asr = (float)d->height / (float)d->width;
+    c->proj = projection_orthographic(d->near, d->far, 1 * d->ortho_scale, -1 * d->ortho_scale,
+                                      asr * d->ortho_scale, -asr * d->ortho_scale);
+  }
+  c->view = view_matrix(d);
+  c->vp = m4_mult(&c->view, &c->proj);                              /* camera.go:1329 */
+  c->forward = m4_forward(&d->world_rotation);
+  c->right = m4_right(&d->world_rotation);
+  c->up = m4_up(&d->world_rotation);
+}
+
+/* SphereInFrustum: camera.go:933-983. */
+static int sphere_in_frustum(const ocam* c, v3 position, float radius) {
+  v3 diff = v3_sub(position, c->pos);
+  v3 nf = {-c->forward.x, -c->forward.y, -c->forward.z};
+  float pcZ = v3_dot(diff, nf);
+  const t3d_camera_desc* d = c->d;
+  if (pcZ > d->far + radius || pcZ < d->near - radius) return 0;
+  if (d->perspective) {
+    float dd = c->sf_y * radius;
+    pcZ *= c->sf_tang;
+    float pcY = v3_dot(diff, c->up);
+    if (pcY > pcZ + dd || pcY < -pcZ - dd) return 0;
+    pcZ *= c->aspect;
+    dd = c->sf_x * radius;
+    float pcX = v3_dot(diff, c->right);
+    if (pcX > pcZ + dd || pcX < -pcZ - dd) return 0;
+  } else {
+    float width = d->ortho_scale;
+    float height = width / c->aspect;
+    float pcY = v3_dot(diff, c->up);
+    if (-height / 2 - radius > pcY || pcY > height / 2 + radius) return 0;
+    float pcX = v3_dot(diff, c->right);
+    if (-width / 2 - radius > pcX || pcX > width / 2 + radius) return 0;
+  }
+  return 1;
+}
+
+/* Model.onTransformUpdate frustum sphere: model.go:142-182. */
+static void model_sphere(const m4* transform, int skinned, const float* dmin, const float* dmax, v3* pos,
+                         float* radius) {
+  v3 position, scale;
+  m4 rotation;
+  m4_decompose(transform, &position, &scale, &rotation);
+  v3 center = {0, 0, 0};
+  if (!skinned) {                                                   /* Dimensions.Center, mesh.go:46-52 */
+    center.x = (dmax[0] + dmin[0]) / 2;
+    center.y = (dmax[1] + dmin[1]) / 2;
+    center.z = (dmax[2] + dmin[2]) / 2;
+  }
+  center = m4_mult_vec(&rotation, center);
+  position.x += center.x * scale.x;
+  position.y += center.y * scale.y;
+  position.z += center.z * scale.z;
+  v3 mn = {dmin[0] * scale.x, dmin[1] * scale.y, dmin[2] * scale.z};
+  v3 mx = {dmax[0] * scale.x, dmax[1] * scale.y, dmax[2] * scale.z};
+  *pos = position;
+  *radius = v3_mag(v3_sub(mx, mn)) / 2;                             /* Dimensions.MaxSpan, mesh.go:41-43 */
+}
+
+/* invertedCamPos: model.go:629-630 (also PointLight.beginModel, light.go:175-189). */
+static v3 inverted_pos(const m4* model_transform, v3 world) {
+  m4 inv = m4_inverted(model_transform);
+  v3 p, s;
+  m4 r;
+  m4_decompose(&inv, &p, &s, &r);
+  v3 rs = m4_mult_vec(&r, world);
+  v3 is = {1 / s.x, 1 / s.y, 1 / s.z};
+  return v3_add(rs, v3_mul(p, is));
+}
+
+/* ------------------------------------------------------------------------ */
+/* lights (light.go)                                                         */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  int type;
+  int visible;
+  float color[3];
+  float energy;
+  float range, range_sq;    /* SetRange: light.go:273-276 */
+  v3 world_pos;
+  float amb[3];             /* AmbientLight.result (light.go:110-112) */
+  v3 forward;               /* DirectionalLight.workingForward (light.go:322-324) */
+  /* per model (beginModel) */
+  v3 working_pos;           /* PointLight.workingPosition */
+  m4 working_rot;           /* DirectionalLight.workingModelRotation */
+  /* volume */
+  const float* cells;
+  uint32_t ncells;
+  v3 vmin, vmax, cell, offset, bias;
+  int cx, cy, cz;
+} olight;
+
+static void light_from_desc(t3do_ctx* c, olight* l, const t3d_light_desc* d) {
+  memset(l, 0, sizeof *l);
+  l->type = d->type;
+  l->visible = d->visible;
+  l->color[0] = d->color[0];
+  l->color[1] = d->color[1];
+  l->color[2] = d->color[2];
+  l->energy = d->energy;
+  l->range = d->range;
+  l->range_sq = d->range * d->range;
+  l->world_pos = m4_row3(&d->transform);
+  if (d->type == T3D_LIGHT_VOLUME) {
+    ovolume* v = find_vol(c, d->volume);
+    l->cells = v ? v->cells : NULL;
+    l->ncells = v ? v->n : 0;
+    l->vmin = v3_from(d->volume_min);
+    l->vmax = v3_from(d->volume_max);
+    l->cell = v3_from(d->cell_size);
+    l->offset = v3_from(d->offset);
+    l->bias = v3_from(d->light_bias);
+    l->cx = d->cell_count[0];
+    l->cy = d->cell_count[1];
+    l->cz = d->cell_count[2];
+  }
+}
+
+/* beginRender: light.go:110-112 (ambient), 322-324 (directional). */
+static void light_begin_render(olight* l, const t3d_light_desc* d) {
+  if (l->type == T3D_LIGHT_AMBIENT) {
+    l->amb[0] = l->color[0] * l->energy;
+    l->amb[1] = l->color[1] * l->energy;
+    l->amb[2] = l->color[2] * l->energy;
+  } else if (l->type == T3D_LIGHT_DIRECTIONAL) {
+    l->forward = m4_forward(&d->world_rotation);
+  }
+}
+
+/* beginModel: light.go:175-189 (point), 326-330 (directional). */
+static void light_begin_model(olight* l, const t3d_model_desc* model) {
+  if (l->type == T3D_LIGHT_POINT) {
+    if (model->skinned) l->working_pos = l->world_pos;
+    else l->working_pos = inverted_pos(&model->transform, l->world_pos);
+  } else if (l->type == T3D_LIGHT_DIRECTIONAL) {
+    if (!model->skinned) {
+      m4 inv = m4_inverted(&model->world_rotation);
+      l->working_rot = m4_transposed(&inv);
+    }
+  }
+}
+
+typedef struct {
+  v3* alt_pos;   /* globalMeshAlteredVertexPositions */
+  v3* alt_nrm;   /* globalMeshAlteredVertexNormals */
+  v4* clip;      /* globalVertexTransforms */
+  float* lights; /* mesh.vertexLights.colors (RGBA) */
+} oscratch;
+
+/* getLightCell with convertToXYZIndices: light.go:717-724, 1267-1272. */
+static const float* volume_cell(const olight* l, v3 p) {
+  int64_t x = go_int(round32((p.x - l->offset.x - l->vmin.x - (l->cell.x / 2)) / l->cell.x));
+  int64_t y = go_int(round32((p.y - l->offset.y - l->vmin.y - (l->cell.y / 2)) / l->cell.y));
+  int64_t z = go_int(round32((p.z - l->offset.z - l->vmin.z - (l->cell.z / 2)) / l->cell.z));
+  if (x < 0 || y < 0 || z < 0 || x >= l->cx || y >= l->cy || z >= l->cz) return NULL;
+  int64_t i = x + (y * l->cx) + (z * l->cx * l->cy);
+  if (!l->cells || i >= (int64_t)l->ncells) return NULL;
+  return l->cells + 4 * i;
+}
+
+static int dims_contains(const olight* l, v3 p) {                  /* mesh.go:80-82 */
+  return p.x >= l->vmin.x && p.y >= l->vmin.y && p.z >= l->vmin.z && p.x <= l->vmax.x && p.y <= l->vmax.y &&
+         p.z <= l->vmax.z;
+}
+
+/* VertexLight for one vertex and one light: light.go:116-120 (ambient),
+ * 191-258 (point), 332-360 (directional), 583-663 (volume). */
+static void vertex_light(const olight* l, uint32_t v, const t3d_model_desc* model, const t3d_material_desc* mat,
+                         const oscratch* s, float* out) {
+  int lighting_mode = (mat && mat->present) ? mat->lighting_mode : T3D_LIGHTING_DEFAULT;
+  if (l->type == T3D_LIGHT_AMBIENT) {
+    out[0] += l->amb[0];
+    out[1] += l->amb[1];
+    out[2] += l->amb[2];
+  } else if (l->type == T3D_LIGHT_POINT) {
+    v3 vp = s->alt_pos[v];
+    v3 vn = s->alt_nrm[v];
+    float distance = v3_dist_sq(l->working_pos, vp);
+    if (l->range > 0) {
+      if (distance > l->range_sq) return;
+    } else {
+      if (1 / distance * l->energy < 0.001f) return;
+    }
+    v3 lv = v3_unit(v3_sub(l->working_pos, vp));
+    if (lighting_mode == T3D_LIGHTING_FIXED_NORMALS) vn = lv;
+    float diffuse = v3_dot(vn, lv);
+    if (lighting_mode == T3D_LIGHTING_DOUBLE_SIDED) diffuse = abs32(diffuse);
+    if (diffuse > 0) {
+      float df = diffuse * (1.0f / (1.0f + (0.1f * distance))) * 2;
+      if (l->range_sq > 0) {
+        float dc = clamp32((l->range_sq - distance) / distance, 0, 1);
+        df *= dc;
+      }
+      out[0] += l->color[0] * df * l->energy;
+      out[1] += l->color[1] * df * l->energy;
+      out[2] += l->color[2] * df * l->energy;
+    }
+  } else if (l->type == T3D_LIGHT_DIRECTIONAL) {
+    v3 n;
+    if (lighting_mode == T3D_LIGHTING_FIXED_NORMALS) n = l->forward;
+    else if (model->skinned) n = s->alt_nrm[v];
+    else n = m4_mult_vec(&l->working_rot, s->alt_nrm[v]);
+    float df = v3_dot(n, l->forward);
+    if (lighting_mode == T3D_LIGHTING_DOUBLE_SIDED) df = abs32(df);
+    if (df <= 0) return;
+    out[0] += l->color[0] * df * l->energy;
+    out[1] += l->color[1] * df * l->energy;
+    out[2] += l->color[2] * df * l->energy;
+  } else if (l->type == T3D_LIGHT_VOLUME) {
+    int mode = (mat && mat->present) ? mat->light_volume_shading_mode : 0;
+    if (mode == T3D_LV_OFF) return;                                 /* LightVolume.Light, light.go:671-673 */
+    float cc[4] = {0, 0, 0, 1};
+    v3 halfc = {l->cell.x / 2, l->cell.y / 2, l->cell.z / 2};
+    if (mode == T3D_LV_PER_OBJECT) {
+      const float* cell = volume_cell(l, m4_row3(&model->transform));
+      if (cell) {
+        cc[0] = cell[0];
+        cc[1] = cell[1];
+        cc[2] = cell[2];
+      }
+    } else {
+      v3 vpos = {0, 0, 0}, vpn = {0, 0, 0};
+      if (model->skinned) {
+        if (mode == T3D_LV_PER_VERTEX_WITH_NORMAL) vpn = v3_add(v3_add(vpos, v3_mul(s->alt_nrm[v], halfc)), l->bias);
+        else vpos = v3_add(s->alt_pos[v], l->bias);
+      } else {
+        if (mode == T3D_LV_PER_VERTEX_WITH_NORMAL)
+          vpn = v3_add(m4_mult_vec(&model->transform, v3_add(s->alt_pos[v], v3_mul(s->alt_nrm[v], halfc))), l->bias);
+        else vpos = v3_add(m4_mult_vec(&model->transform, s->alt_pos[v]), l->bias);
+      }
+      if (!dims_contains(l, vpos)) return;
+      cc[0] = cc[1] = cc[2] = 0;
+      const float* cell = volume_cell(l, mode == T3D_LV_PER_VERTEX_WITH_NORMAL ? vpn : vpos);
+      if (cell) {
+        cc[0] = cell[0];
+        cc[1] = cell[1];
+        cc[2] = cell[2];
+        cc[3] = cell[3];
+      }
+    }
+    out[0] += l->color[0] * cc[0] * l->energy;
+    out[1] += l->color[1] * cc[1] * l->energy;
+    out[2] += l->color[2] * cc[2] * l->energy;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* bucket sort (sortingTriangles.go)                                         */
+/* ------------------------------------------------------------------------ */
+
+/* Sort's bin formula: sortingTriangles.go:38-64. */
+static uint32_t bucket_bin(float depth, float minRange, float maxRange, int sort_mode) {
+  float rangeDiff = maxRange - minRange;
+  if (rangeDiff == 0) {
+    maxRange += 0.001f;
+    rangeDiff += 0.001f;
+  }
+  int64_t target = 0;
+  if (sort_mode != T3D_SORT_NONE) {
+    float d = (depth - minRange) / rangeDiff * (float)NBINS;
+    float t = clamp32(d, 0, (float)(NBINS - 1));
+    target = go_int(t);
+  }
+  return (uint32_t)target;
+}
+
+/* Sort + ForEach (sortingTriangles.go:38-64, 93-125): returns the visit order. */
+static void bucket_order(const float* depth, uint32_t n, float mn, float mx, int sort_mode, uint32_t* order,
+                         uint32_t* bin_scratch, uint32_t* counts) {
+  memset(counts, 0, sizeof(uint32_t) * (NBINS + 1));
+  for (uint32_t i = 0; i < n; i++) {
+    bin_scratch[i] = bucket_bin(depth[i], mn, mx, sort_mode);
+    counts[bin_scratch[i]]++;
+  }
+  /* bins are appended in insertion order; ForEach walks them 511→0 for
+   * BackToFront, 0→511 otherwise */
+  uint32_t start[NBINS];
+  uint32_t acc = 0;
+  if (sort_mode == T3D_SORT_BACK_TO_FRONT) {
+    for (int b = NBINS - 1; b >= 0; b--) {
+      start[b] = acc;
+      acc += counts[b];
+    }
+  } else {
+    for (int b = 0; b < NBINS; b++) {
+      start[b] = acc;
+      acc += counts[b];
+    }
+  }
+  for (uint32_t i = 0; i < n; i++) order[start[bin_scratch[i]]++] = i;
+}
+
+/* ------------------------------------------------------------------------ */
+/* render                                                                    */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  uint32_t model, part;
+  float depth;
+  int32_t order;
+} opair;
+
+typedef struct {
+  t3d_vertex* color;
+  t3d_vertex* depth;
+  uint32_t* idx;
+  uint32_t* tri_ids;
+  float* keys;
+  uint64_t n, cap;          /* vertices */
+} obuf;
+
+static void obuf_reserve(obuf* b, uint64_t extra) {
+  if (b->n + extra <= b->cap) return;
+  uint64_t nc = b->cap ? b->cap : 1024;
+  while (nc < b->n + extra) nc *= 2;
+  b->color = (t3d_vertex*)realloc(b->color, nc * sizeof(t3d_vertex));
+  b->depth = (t3d_vertex*)realloc(b->depth, nc * sizeof(t3d_vertex));
+  b->idx = (uint32_t*)realloc(b->idx, nc * sizeof(uint32_t));
+  b->tri_ids = (uint32_t*)realloc(b->tri_ids, (nc / 3 + 1) * sizeof(uint32_t));
+  b->keys = (float*)realloc(b->keys, (nc / 3 + 1) * sizeof(float));
+  b->cap = nc;
+}
+
+/* Model.isTransparent: model.go:1274-1282. */
+static int is_transparent(const t3d_model_desc* model, const t3d_material_desc* mat) {
+  if (mat && mat->present) {
+    int matT = mat->transparency_mode == T3D_TRANSPARENCY_TRANSPARENT || !mat->blend_source_over ||
+               (mat->transparency_mode == T3D_TRANSPARENCY_AUTO && mat->color[3] < 0.999f);
+    int modelT = mat->transparency_mode != T3D_TRANSPARENCY_OPAQUE && model->color[3] < 0.999f;
+    return matT || modelT;
+  }
+  return model->color[3] < 0.999f;
+}
+
+static const t3d_material_desc* part_mat(const t3d_model_desc* model, uint32_t p) {
+  if (!model->materials) return NULL;
+  const t3d_material_desc* m = &model->materials[p];
+  return m->present ? m : NULL;
+}
+
+typedef struct { const opair* pairs; } pair_env;
+/* transparent pass order: camera.go:2050-2055 */
+static int less_transparent(void* env, int32_t a, int32_t b) {
+  const opair* p = ((pair_env*)env)->pairs;
+  if (p[a].order == p[b].order) return p[a].depth > p[b].depth;
+  return p[a].order < p[b].order;
+}
+/* solid order when RenderDepth is off: camera.go:1458-1467 */
+static int less_solid_nodepth(void* env, int32_t a, int32_t b) {
+  const opair* p = ((pair_env*)env)->pairs;
+  if (p[a].order < p[b].order) return 1;
+  return p[a].depth > p[b].depth;
+}
+
+typedef struct {
+  const olight* lights;
+  v3 pos;
+} light_env;
+/* MaxLightCount comparator: camera.go:1483-1495 */
+static int less_light(void* env, int32_t a, int32_t b) {
+  light_env* e = (light_env*)env;
+  const olight* la = &e->lights[a];
+  const olight* lb = &e->lights[b];
+  if (la->type == T3D_LIGHT_AMBIENT) return 1;
+  else if (lb->type == T3D_LIGHT_AMBIENT) return 0;
+  else if (la->type == T3D_LIGHT_DIRECTIONAL) return 1;
+  else if (lb->type == T3D_LIGHT_DIRECTIONAL) return 0;
+  return v3_dist_sq(e->pos, la->world_pos) < v3_dist_sq(e->pos, lb->world_pos);
+}
+
+static v3 shape_pos(const omesh* m, const t3d_model_desc* model, uint32_t v) {   /* mesh.go:796-808 */
+  v3 out = v3_from(m->pos + 3 * (size_t)v);
+  for (uint32_t k = 0; k < m->d.shape_key_count; k++) {
+    float w = model->shape_key_weights ? model->shape_key_weights[k] : 0.0f;
+    const float* kp = m->sk_pos + ((size_t)k * m->d.vertex_count + v) * 3;
+    out.x += kp[0] * w;
+    out.y += kp[1] * w;
+    out.z += kp[2] * w;
+  }
+  return out;
+}
+static v3 shape_nrm(const omesh* m, const t3d_model_desc* model, uint32_t v) {   /* mesh.go:812-822 */
+  v3 out = v3_from(m->nrm + 3 * (size_t)v);
+  for (uint32_t k = 0; k < m->d.shape_key_count; k++) {
+    float w = model->shape_key_weights ? model->shape_key_weights[k] : 0.0f;
+    const float* kn = m->sk_nrm + ((size_t)k * m->d.vertex_count + v) * 3;
+    out.x += kn[0] * w;
+    out.y += kn[1] * w;
+    out.z += kn[2] * w;
+  }
+  return out;
+}
+
+/* Model.skinVertex: model.go:559-596. */
+static void skin_vertex(const omesh* m, const t3d_model_desc* model, uint32_t v, v3* op, v3* on) {
+  m4 S = m4_zero();
+  for (int k = 0; k < 4; k++) {
+    float w = m->weights[4 * (size_t)v + k];
+    if (w == 0) continue;
+    const m4* B = &model->bone_influences[m->bones[4 * (size_t)v + k]];
+    if (w == 1) {
+      S = *B;
+      break;
+    }
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++) S.m[i][j] += B->m[i][j] * w;   /* Add(Scale(w)), matrix.go:639-661 */
+  }
+  *op = m4_mult_vec(&S, shape_pos(m, model, v));
+  S.m[3][0] = 0;
+  S.m[3][1] = 0;
+  S.m[3][2] = 0;
+  S.m[3][3] = 1;
+  *on = m4_mult_vec(&S, shape_nrm(m, model, v));
+}
+
+static int validate_model(t3do_ctx* c, const t3d_model_desc* model, const omesh* m, const t3d_camera_desc* cam) {
+  if (model->has_vertex_transform_function || model->has_vertex_clip_function)
+    return set_err(c, "model uses a Go vertex callback") , T3D_ERR_UNSUPPORTED;
+  if (model->skinned) {
+    if (!m->bones || !m->weights || !model->bone_influences) return set_err(c, "skinned model without skin data");
+    for (size_t i = 0; i < 4 * (size_t)m->d.vertex_count; i++)
+      if (m->weights[i] != 0 && m->bones[i] >= model->bone_count) return set_err(c, "bone index out of range");
+  }
+  (void)cam;
+  return T3D_OK;
+}
+
+/* render closure for one pair: camera.go:1507-1827 with ProcessVertices
+ * (model.go:600-995). Appends the flush's arrays to *ob. */
+static int render_pair(t3do_ctx* c, const ocam* cam, const t3d_world_desc* world, const t3d_model_desc* model,
+                       uint32_t model_index, uint32_t part_index, const omesh* m, olight* scene_lights, int nscene,
+                       olight* group_lights, const t3d_light_desc* group_descs, int ngroup, oscratch* s, obuf* ob,
+                       t3do_result* res, uint32_t* pair_cap) {
+  const t3d_camera_desc* cd = cam->d;
+  const t3d_material_desc* mat = part_mat(model, part_index);
+  const t3d_part_range* pr = &m->parts[part_index];
+
+  int lighting = 0;                                                 /* camera.go:1521-1528 */
+  if (world && world->present) {
+    if (mat) lighting = world->lighting_on && !mat->shadeless && !model->shadeless;
+    else lighting = world->lighting_on && !model->shadeless;
+  }
+  int sort_mode = mat ? mat->triangle_sort_mode : T3D_SORT_BACK_TO_FRONT;
+
+  /* pair record (emitted even when empty, survivor_count 0) */
+  if (res->pair_count == *pair_cap) {
+    *pair_cap = *pair_cap ? 2 * *pair_cap : 64;
+    res->pairs = (t3d_pair_result*)realloc(res->pairs, sizeof(t3d_pair_result) * *pair_cap);
+  }
+  t3d_pair_result* out_pair = &res->pairs[res->pair_count++];
+  out_pair->model_index = model_index;
+  out_pair->part_index = part_index;
+  out_pair->triangle_count = pr->triangle_count;
+  out_pair->survivor_count = 0;
+  out_pair->vertex_offset = ob->n;
+
+  /* ProcessVertices early out: model.go:604-606 */
+  if ((model->color[3] == 0 && is_transparent(model, mat)) || !model->visible) return T3D_OK;
+  if (mat && (mat->billboard || mat->custom_depth_function))
+    return set_err(c, "billboarded material / custom depth function"), T3D_ERR_UNSUPPORTED;
+
+  const m4* M = &model->transform;
+  v3 camPos = cam->pos;
+  v3 icp = inverted_pos(M, camPos);                                 /* model.go:629-630 */
+  m4 mvp = m4_mult(M, &cam->vp);                                    /* model.go:695 */
+  float minDepth = MAXF32, maxDepth = -MAXF32;
+  float camNear = cd->near, camFar = cd->far;
+  int skinned = model->skinned;
+  int snapping = cd->vertex_snapping > 0;
+  int backface = mat && mat->backface_culling;
+
+  uint32_t T = pr->triangle_count;
+  uint32_t* bucket_tri = (uint32_t*)malloc(sizeof(uint32_t) * (T ? T : 1));
+  float* bucket_depth = (float*)malloc(sizeof(float) * (T ? T : 1));
+  uint32_t nb = 0;
+
+  for (uint32_t t = pr->triangle_start; t < pr->triangle_start + T; t++) {   /* forEachTri, mesh.go:2238-2256 */
+    const uint32_t* tri = m->idx + 3 * (size_t)t;
+    for (int vi = 0; vi < 3; vi++) {                                         /* model.go:751-822 */
+      uint32_t v = tri[vi];
+      v3 vert, normal;
+      if (skinned) {
+        skin_vertex(m, model, v, &vert, &normal);
+        s->alt_pos[v] = vert;
+        s->alt_nrm[v] = normal;
+        s->clip[v] = m4_mult_vec_w(&cam->vp, vert);
+      } else {
+        vert = shape_pos(m, model, v);
+        normal = shape_nrm(m, model, v);
+        s->alt_pos[v] = vert;
+        s->alt_nrm[v] = normal;
+        s->clip[v] = m4_mult_vec_w(&mvp, vert);
+      }
+      if (snapping) {                                                        /* model.go:807-812 */
+        float vs = cd->vertex_snapping;
+        s->clip[v].x = round32(s->clip[v].x * vs) / vs;
+        s->clip[v].y = round32(s->clip[v].y * vs) / vs;
+        s->clip[v].z = round32(s->clip[v].z * vs) / vs;
+      }
+    }
+    v3 stc = {0, 0, 0};
+    int outOfBounds = 1;
+    v3 tvp[3];
+    for (int i = 0; i < 3; i++) {                                            /* model.go:855-886 */
+      uint32_t v = tri[i];
+      if (skinned) {
+        stc.x += s->alt_pos[v].x;
+        stc.y += s->alt_pos[v].y;
+        stc.z += s->alt_pos[v].z;
+      }
+      float w = s->clip[v].w;
+      if (w < 0) w = 0.000001f;
+      tvp[i].x = s->clip[v].x / w;
+      tvp[i].y = s->clip[v].y / w;
+      if (s->clip[v].z + 1 >= camNear && s->clip[v].z < camFar) outOfBounds = 0;
+    }
+    if (outOfBounds) continue;
+    if ((tvp[0].x < -0.5f && tvp[1].x < -0.5f && tvp[2].x < -0.5f) ||          /* model.go:890-898 */
+        (tvp[0].x > 0.5f && tvp[1].x > 0.5f && tvp[2].x > 0.5f) ||
+        (tvp[0].y < -0.5f && tvp[1].y < -0.5f && tvp[2].y < -0.5f) ||
+        (tvp[0].y > 0.5f && tvp[1].y > 0.5f && tvp[2].y > 0.5f))
+      continue;
+    if (backface) {                                                          /* model.go:902-925 */
+      float n0x = tvp[0].x - tvp[1].x;
+      float n0y = tvp[0].y - tvp[1].y;
+      float n1x = tvp[1].x - tvp[2].x;
+      float n1y = tvp[1].y - tvp[2].y;
+      float nor = (n0x * n1y) - (n1x * n0y);
+      if (nor < 0) continue;
+    }
+    float depth;
+    if (skinned) {                                                           /* model.go:941-955 */
+      float dx = camPos.x - (stc.x / 3);
+      float dy = camPos.y - (stc.y / 3);
+      float dz = camPos.z - (stc.z / 3);
+      depth = dx * dx + dy * dy + dz * dz;
+    } else {
+      v3 ctr = m->tri_center[t];
+      float dx = icp.x - ctr.x;
+      float dy = icp.y - ctr.y;
+      float dz = icp.z - ctr.z;
+      depth = dx * dx + dy * dy + dz * dz;
+    }
+    if (isnan32(depth) || isinf32(depth)) continue;                          /* model.go:957-960 */
+    if (depth < minDepth) minDepth = depth;
+    if (depth > maxDepth) maxDepth = depth;
+    bucket_tri[nb] = t;                                                      /* AddTriangle, sortingTriangles.go:32-36 */
+    bucket_depth[nb] = depth;
+    nb++;
+  }
+
+  if (nb == 0) {                                                             /* camera.go:1556-1558 */
+    free(bucket_tri);
+    free(bucket_depth);
+    return T3D_OK;
+  }
+  uint32_t* order = (uint32_t*)malloc(sizeof(uint32_t) * nb);
+  uint32_t* bins = (uint32_t*)malloc(sizeof(uint32_t) * nb);
+  uint32_t counts[NBINS + 1];
+  bucket_order(bucket_depth, nb, minDepth, maxDepth, sort_mode, order, bins, counts);
+
+  float srcW = 0, srcH = 0;                                                  /* camera.go:1560-1566 */
+  if (mat && mat->use_texture) {
+    srcW = mat->texture_size[0];
+    srcH = mat->texture_size[1];
+  }
+
+  olight* lights = scene_lights;
+  int nlights = nscene;
+  if (lighting) {                                                            /* camera.go:1573-1588 */
+    if (model->light_group_active) {
+      lights = group_lights;
+      nlights = ngroup;
+      for (int i = 0; i < nlights; i++) light_begin_render(&lights[i], &group_descs[i]);
+    }
+    for (int i = 0; i < nlights; i++) light_begin_model(&lights[i], model);
+  }
+
+  v3 mdmin = v3_from(m->d.dimensions_min), mdmax = v3_from(m->d.dimensions_max);
+  float maxSpan = v3_mag(v3_sub(mdmax, mdmin));                              /* camera.go:1592 */
+  v3 modelPos = m4_row3(M);
+  float mp[4] = {model->color[0], model->color[1], model->color[2], model->color[3]};   /* camera.go:1597-1601 */
+  if (mat) {
+    mp[0] *= mat->color[0];
+    mp[1] *= mat->color[1];
+    mp[2] *= mat->color[2];
+    mp[3] *= mat->color[3];
+  }
+  float camWidth = (float)cd->width, camHeight = (float)cd->height;
+  float halfW = camWidth / 2, halfH = camHeight / 2;
+  float dmp = (cd->far - cd->near) * cd->depth_margin;                       /* camera.go:1339-1341 */
+  float camSpread = cd->far - cd->near + (dmp * 2);
+
+  if (lighting) {                                                            /* camera.go:1610-1641 */
+    for (uint32_t k = 0; k < nb; k++) {
+      const uint32_t* tri = m->idx + 3 * (size_t)bucket_tri[order[k]];
+      for (int vi = 0; vi < 3; vi++) {
+        float* L = s->lights + 4 * (size_t)tri[vi];
+        L[0] = 0;
+        L[1] = 0;
+        L[2] = 0;
+        L[3] = 1;
+      }
+      for (int li = 0; li < nlights; li++) {
+        const olight* l = &lights[li];
+        if (l->type == T3D_LIGHT_POINT && l->range > 0) {
+          float dist = maxSpan + l->range;
+          if (v3_dist_sq(modelPos, l->world_pos) > dist * dist) continue;
+        }
+        for (int vi = 0; vi < 3; vi++) vertex_light(l, tri[vi], model, mat, s, s->lights + 4 * (size_t)tri[vi]);
+      }
+    }
+  }
+
+  /* vertex assembly: camera.go:1646-1820 */
+  obuf_reserve(ob, 3 * (uint64_t)nb);
+  uint64_t base = ob->n;
+  const float* vcol = m->col;
+  for (uint32_t k = 0; k < nb; k++) {
+    uint32_t t = bucket_tri[order[k]];
+    const uint32_t* tri = m->idx + 3 * (size_t)t;
+    ob->tri_ids[(base / 3) + k] = t;
+    ob->keys[(base / 3) + k] = bucket_depth[order[k]];
+    for (int vi = 0; vi < 3; vi++) {
+      uint32_t v = tri[vi];
+      uint64_t o = base + 3 * (uint64_t)k + vi;
+      t3d_vertex* cv = &ob->color[o];
+      t3d_vertex* dv = &ob->depth[o];
+      memset(cv, 0, sizeof *cv);
+      memset(dv, 0, sizeof *dv);
+      v4 target = s->clip[v];
+      float w = target.w;
+      if (!cd->perspective) w = 1.0f;
+      if (w < 0) w = 0.001f;
+      float dx = (target.x / w) * camWidth + halfW;
+      float dy = (target.y / -w) * camHeight + halfH;
+      cv->dst_x = dx;
+      cv->dst_y = dy;
+      dv->dst_x = dx;
+      dv->dst_y = dy;
+      float U = m->uv ? m->uv[2 * (size_t)v] : 0.0f;
+      float Vv = m->uv ? m->uv[2 * (size_t)v + 1] : 0.0f;
+      float uvU, uvV;
+      if (cd->perspective_corrected_texture_mapping) {
+        uvU = (U / w) * srcW;
+        uvV = ((1 - Vv) / w) * srcH;
+      } else {
+        uvU = U * srcW;
+        uvV = (1 - Vv) * srcH;
+      }
+      cv->src_x = uvU;
+      cv->src_y = uvV;
+      if (cd->perspective_corrected_texture_mapping) {
+        float d = 1.0f / w;
+        cv->custom0 = d;
+        dv->custom0 = d;
+      }
+      dv->src_x = uvU;
+      dv->src_y = uvV;
+      if (vcol) {
+        cv->color_r = vcol[4 * (size_t)v + 0] * mp[0];
+        cv->color_g = vcol[4 * (size_t)v + 1] * mp[1];
+        cv->color_b = vcol[4 * (size_t)v + 2] * mp[2];
+        cv->color_a = vcol[4 * (size_t)v + 3] * mp[3];
+      } else {
+        cv->color_r = mp[0];
+        cv->color_g = mp[1];
+        cv->color_b = mp[2];
+        cv->color_a = mp[3];
+      }
+      if (lighting) {
+        const float* L = s->lights + 4 * (size_t)v;
+        cv->color_r *= L[0];
+        cv->color_g *= L[1];
+        cv->color_b *= L[2];
+      }
+      if (cd->render_depth) {
+        float vd = target.z;
+        float depth = (vd + dmp) / camSpread;
+        if (depth < 0) depth = 0;
+        else if (depth > 1) depth = 1;
+        dv->color_r = depth;
+        dv->color_g = depth;
+        dv->color_b = depth;
+        dv->color_a = 1;
+      } else if (world && world->present && world->fog_on) {
+        float vd = target.z;
+        float depth = (vd + dmp) / camSpread;
+        if (depth < 0) depth = 0;
+        else if (depth > 1) depth = 1;
+        depth = world->fog_range[0] + ((world->fog_range[1] - world->fog_range[0]) * 1 - depth);
+        if (world->fog_mode == T3D_FOG_ADD) {
+          cv->color_r += world->fog_color[0] * depth;
+          cv->color_g += world->fog_color[1] * depth;
+          cv->color_b += world->fog_color[2] * depth;
+        } else if (world->fog_mode == T3D_FOG_SUB) {
+          cv->color_r *= world->fog_color[0] * depth;
+          cv->color_g *= world->fog_color[1] * depth;
+          cv->color_b *= world->fog_color[2] * depth;
+        }
+      }
+      ob->idx[o] = (uint32_t)(3 * k + vi);                                   /* indexList, camera.go:1816 */
+    }
+  }
+  ob->n += 3 * (uint64_t)nb;
+  out_pair->survivor_count = nb;
+  free(order);
+  free(bins);
+  free(bucket_tri);
+  free(bucket_depth);
+  return T3D_OK;
+}
+
+int t3do_render(t3do_ctx* c, const t3d_render_desc* desc, t3do_result* res) {
+  memset(res, 0, sizeof *res);
+  if (!desc || !desc->camera) return set_err(c, "render: missing camera");
+  const t3d_camera_desc* cd = desc->camera;
+  if (cd->width <= 0 || cd->height <= 0) return set_err(c, "render: camera size");
+  if (cd->render_normals) return set_err(c, "RenderNormals unsupported"), T3D_ERR_UNSUPPORTED;
+  const t3d_world_desc* world = (desc->world && desc->world->present) ? desc->world : NULL;
+
+  ocam cam;
+  cam_setup(&cam, cd);
+
+  /* scene light list: camera.go:1275-1303 */
+  int nl = 0;
+  olight* lights = (olight*)calloc(desc->light_count + 1, sizeof(olight));
+  if (world) {
+    if (world->lighting_on && world->ambient_visible) {
+      olight* a = &lights[nl++];
+      a->type = T3D_LIGHT_AMBIENT;
+      a->visible = 1;
+      a->color[0] = world->ambient_color[0];
+      a->color[1] = world->ambient_color[1];
+      a->color[2] = world->ambient_color[2];
+      a->energy = world->ambient_energy;
+      light_begin_render(a, NULL);
+    }
+  }
+  for (uint32_t i = 0; i < desc->light_count; i++) {
+    const t3d_light_desc* ld = &desc->lights[i];
+    if ((!world || world->lighting_on) && ld->visible) {
+      light_from_desc(c, &lights[nl], ld);
+      light_begin_render(&lights[nl], ld);
+      nl++;
+    }
+  }
+
+  /* render pairs: camera.go:1343-1457 */
+  opair* pairs = (opair*)malloc(sizeof(opair) * 16);
+  int np = 0, capp = 16;
+  int32_t* solids = NULL;
+  int32_t* transp = NULL;
+  int ns = 0, nt = 0;
+  int rc = T3D_OK;
+  for (uint32_t mi = 0; mi < desc->model_count; mi++) {
+    const t3d_model_desc* model = &desc->models[mi];
+    if (!model->visible || model->mesh == 0) continue;             /* camera.go:1346-1349, 1371 */
+    omesh* m = find_mesh(c, model->mesh);
+    if (!m) { rc = set_err(c, "render: unknown mesh handle"); goto done; }
+    rc = validate_model(c, model, m, cd);
+    if (rc) goto done;
+    if (model->frustum_culling) {
+      v3 sp;
+      float sr;
+      model_sphere(&model->transform, model->skinned, m->d.dimensions_min, m->d.dimensions_max, &sp, &sr);
+      if (!sphere_in_frustum(&cam, sp, sr)) continue;
+    }
+    for (uint32_t p = 0; p < m->d.part_count; p++) {
+      const t3d_material_desc* mat = part_mat(model, p);
+      if (mat && !mat->visible) continue;
+      if (np == capp) {
+        capp *= 2;
+        pairs = (opair*)realloc(pairs, sizeof(opair) * capp);
+      }
+      opair* rp = &pairs[np];
+      rp->model = mi;
+      rp->part = p;
+      rp->depth = v3_dist_sq(cam.pos, m4_row3(&model->transform));
+      rp->order = mat ? mat->render_order : 0;
+      if (is_transparent(model, mat)) {
+        transp = (int32_t*)realloc(transp, sizeof(int32_t) * (nt + 1));
+        transp[nt++] = np;
+      } else {
+        solids = (int32_t*)realloc(solids, sizeof(int32_t) * (ns + 1));
+        solids[ns++] = np;
+      }
+      np++;
+    }
+  }
+  pair_env penv = {pairs};
+  if (!cd->render_depth) go_stable_sort(solids, ns, less_solid_nodepth, &penv);  /* camera.go:1458-1467 */
+
+  if (cd->max_light_count > 0) {                                                  /* camera.go:1474-1497 */
+    light_env lenv;
+    lenv.lights = lights;
+    lenv.pos = cd->has_max_light_count_focus ? v3_from(cd->max_light_count_focus) : cam.pos;
+    int32_t* ord = (int32_t*)malloc(sizeof(int32_t) * (nl ? nl : 1));
+    for (int i = 0; i < nl; i++) ord[i] = i;
+    go_stable_sort(ord, nl, less_light, &lenv);
+    olight* sorted = (olight*)calloc(nl + 1, sizeof(olight));
+    for (int i = 0; i < nl; i++) sorted[i] = lights[ord[i]];
+    free(ord);
+    free(lights);
+    lights = sorted;
+    if (nl > cd->max_light_count) nl = cd->max_light_count;
+  }
+
+  go_stable_sort(transp, nt, less_transparent, &penv);                            /* camera.go:2050-2055 */
+
+  {
+    uint32_t maxV = 1;
+    for (int i = 0; i < c->nmeshes; i++)
+      if (c->meshes[i].d.vertex_count > maxV) maxV = c->meshes[i].d.vertex_count;
+    oscratch s;
+    s.alt_pos = (v3*)calloc(maxV, sizeof(v3));
+    s.alt_nrm = (v3*)calloc(maxV, sizeof(v3));
+    s.clip = (v4*)calloc(maxV, sizeof(v4));
+    s.lights = (float*)calloc((size_t)maxV * 4, sizeof(float));
+    obuf ob;
+    memset(&ob, 0, sizeof ob);
+    uint32_t pair_cap = 0;
+    for (int pass = 0; pass < 2 && rc == T3D_OK; pass++) {                        /* camera.go:2057-2109 */
+      int32_t* list = pass == 0 ? solids : transp;
+      int n = pass == 0 ? ns : nt;
+      for (int i = 0; i < n && rc == T3D_OK; i++) {
+        const opair* rp = &pairs[list[i]];
+        const t3d_model_desc* model = &desc->models[rp->model];
+        if (!model->visible || model->auto_batch_static) continue;
+        omesh* m = find_mesh(c, model->mesh);
+        olight* glights = NULL;
+        int ng = 0;
+        if (model->light_group_active) {
+          ng = (int)model->light_group_count;
+          glights = (olight*)calloc(ng + 1, sizeof(olight));
+          for (int g = 0; g < ng; g++) light_from_desc(c, &glights[g], &model->light_group[g]);
+        }
+        rc = render_pair(c, &cam, world, model, rp->model, rp->part, m, lights, nl, glights, model->light_group, ng,
+                         &s, &ob, res, &pair_cap);
+        free(glights);
+      }
+    }
+    free(s.alt_pos);
+    free(s.alt_nrm);
+    free(s.clip);
+    free(s.lights);
+    res->vertex_count = ob.n;
+    res->color = ob.color;
+    res->depth = ob.depth;
+    res->indices = ob.idx;
+    res->triangle_ids = ob.tri_ids;
+    res->depth_keys = ob.keys;
+  }
+
+done:
+  free(pairs);
+  free(solids);
+  free(transp);
+  free(lights);
+  if (rc != T3D_OK) t3do_result_free(res);
+  return rc;
+}
+
+void t3do_result_free(t3do_result* r) {
+  if (!r) return;
+  free(r->pairs);
+  free(r->color);
+  free(r->depth);
+  free(r->indices);
+  free(r->triangle_ids);
+  free(r->depth_keys);
+  memset(r, 0, sizeof *r);
+}
+
+/* ------------------------------------------------------------------------ */
+/* unit-level entry points                                                   */
+/* ------------------------------------------------------------------------ */
+
+void t3do_mat4_mult(const t3d_mat4* a, const t3d_mat4* b, t3d_mat4* out) { *out = m4_mult(a, b); }
+void t3do_mat4_inverted(const t3d_mat4* a, t3d_mat4* out) { *out = m4_inverted(a); }
+void t3do_mat4_transposed(const t3d_mat4* a, t3d_mat4* out) { *out = m4_transposed(a); }
+void t3do_mat4_decompose(const t3d_mat4* a, float pos[3], float scale[3], t3d_mat4* rot) {
+  v3 p, s;
+  m4_decompose(a, &p, &s, rot);
+  pos[0] = p.x; pos[1] = p.y; pos[2] = p.z;
+  scale[0] = s.x; scale[1] = s.y; scale[2] = s.z;
+}
+void t3do_mat4_mult_vec(const t3d_mat4* a, const float v[3], float out[3]) {
+  v3 r = m4_mult_vec(a, v3_from(v));
+  out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+void t3do_mat4_mult_vec_w(const t3d_mat4* a, const float v[3], float out[4]) {
+  v4 r = m4_mult_vec_w(a, v3_from(v));
+  out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = r.w;
+}
+void t3do_projection_perspective(float fovy, float near, float far, float w, float h, t3d_mat4* out) {
+  *out = projection_perspective(fovy, near, far, w, h);
+}
+void t3do_view_matrix(const t3d_camera_desc* cam, t3d_mat4* out) { *out = view_matrix(cam); }
+void t3do_vp_matrix(const t3d_camera_desc* cam, t3d_mat4* out) {
+  ocam c;
+  cam_setup(&c, cam);
+  *out = c.vp;
+}
+void t3do_vec3_unit(const float v[3], float out[3]) {
+  v3 r = v3_unit(v3_from(v));
+  out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+int t3do_sphere_in_frustum(const t3d_camera_desc* cam, const float pos[3], float radius) {
+  ocam c;
+  cam_setup(&c, cam);
+  return sphere_in_frustum(&c, v3_from(pos), radius);
+}
+void t3do_model_frustum_sphere(const t3d_mat4* transform, int skinned, const float dims_min[3],
+                               const float dims_max[3], float pos[3], float* radius) {
+  v3 p;
+  model_sphere(transform, skinned, dims_min, dims_max, &p, radius);
+  pos[0] = p.x; pos[1] = p.y; pos[2] = p.z;
+}
+void t3do_inverted_cam_pos(const t3d_mat4* model_transform, const float cam_pos[3], float out[3]) {
+  v3 r = inverted_pos(model_transform, v3_from(cam_pos));
+  out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+uint32_t t3do_bucket_bin(float depth, float min_depth, float max_depth, int32_t sort_mode) {
+  return bucket_bin(depth, min_depth, max_depth, sort_mode);
+}
+uint32_t t3do_bucket_sort(const float* depths, uint32_t n, float min_depth, float max_depth, int32_t sort_mode,
+                          uint32_t* order) {
+  uint32_t* bins = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+  uint32_t counts[NBINS + 1];
+  bucket_order(depths, n, min_depth, max_depth, sort_mode, order, bins, counts);
+  free(bins);
+  return n;
+}
+
+typedef struct { const uint8_t* t; uint32_t n; } table_env;
+static int less_table(void* env, int32_t a, int32_t b) {
+  table_env* e = (table_env*)env;
+  return e->t[(size_t)a * e->n + b];
+}
+void t3do_go_stable_sort_table(int32_t* items, uint32_t n, const uint8_t* less_table_) {
+  table_env e = {less_table_, n};
+  go_stable_sort(items, (int)n, less_table, &e);
+}

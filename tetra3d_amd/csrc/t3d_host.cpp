@@ -1,0 +1,1373 @@
+// t3d_host.cpp — host runtime behind include/t3d.h.
+//
+// Owns the HIP context state (stream, resident mesh pool, light volumes,
+// budget accounting), computes the per-frame uniforms that Camera.Render
+// computes on the CPU (camera.go:1269-1601, model.go:142-182, 600-695,
+// light.go beginRender/beginModel), decides the render-pair order exactly as
+// the reference does, and drives the kernel chain of t3d_kernels.hip in the
+// plain (one chain per render pair) or batched (one chain per frame, replayed
+// from a HIP graph) form.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "t3d_internal.h"
+
+using namespace t3d;
+
+namespace {
+
+constexpr float kMaxF32 = 3.40282346638528859811704183484516925440e+38f;
+
+std::string vfmt(const char* fmt, va_list ap) {
+  char buf[1024];
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  return buf;
+}
+
+uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+// Go's int(float32) on amd64 (CVTTSS2SQ): truncation, min int64 for NaN/overflow.
+int64_t go_int(float f) {
+  if (!(f >= -9.2233720368547758e18f && f < 9.2233720368547758e18f)) return INT64_MIN;
+  return (int64_t)f;
+}
+float round32(float x) { return (float)std::round((double)x); }  // math32.Round, half away from zero
+
+// Go 1.24 sort.SliceStable restated (insertion sort over blocks of 20, then
+// SymMerge passes).  The reference passes comparators that are not strict
+// weak orders (camera.go:1458-1467, 1483-1495); with those, only this exact
+// algorithm reproduces its order.
+template <class Less>
+struct GoStable {
+  std::vector<int32_t>& d;
+  Less less;
+  bool lt(int i, int j) { return less(d[i], d[j]); }
+  void swap(int i, int j) { std::swap(d[i], d[j]); }
+  void insertion(int a, int b) {
+    for (int i = a + 1; i < b; i++)
+      for (int j = i; j > a && lt(j, j - 1); j--) swap(j, j - 1);
+  }
+  void swap_range(int a, int b, int n) {
+    for (int i = 0; i < n; i++) swap(a + i, b + i);
+  }
+  void rotate(int a, int m, int b) {
+    int i = m - a, j = b - m;
+    while (i != j) {
+      if (i > j) {
+        swap_range(m - i, m, j);
+        i -= j;
+      } else {
+        swap_range(m - i, m + j - i, i);
+        j -= i;
+      }
+    }
+    swap_range(m - i, m, i);
+  }
+  void sym_merge(int a, int m, int b) {
+    if (m - a == 1) {
+      int i = m, j = b;
+      while (i < j) {
+        int h = (int)((unsigned)(i + j) >> 1);
+        if (lt(h, a)) i = h + 1;
+        else j = h;
+      }
+      for (int k = a; k < i - 1; k++) swap(k, k + 1);
+      return;
+    }
+    if (b - m == 1) {
+      int i = a, j = m;
+      while (i < j) {
+        int h = (int)((unsigned)(i + j) >> 1);
+        if (!lt(m, h)) i = h + 1;
+        else j = h;
+      }
+      for (int k = m; k > i; k--) swap(k, k - 1);
+      return;
+    }
+    int mid = (int)((unsigned)(a + b) >> 1);
+    int n = mid + m;
+    int start, r;
+    if (m > mid) {
+      start = n - b;
+      r = mid;
+    } else {
+      start = a;
+      r = m;
+    }
+    int p = n - 1;
+    while (start < r) {
+      int c = (int)((unsigned)(start + r) >> 1);
+      if (!lt(p - c, c)) start = c + 1;
+      else r = c;
+    }
+    int end = n - start;
+    if (start < m && m < end) rotate(start, m, end);
+    if (a < start && start < mid) sym_merge(a, start, mid);
+    if (mid < end && end < b) sym_merge(mid, end, b);
+  }
+  void run() {
+    int n = (int)d.size();
+    int block = 20, a = 0, b = block;
+    while (b <= n) {
+      insertion(a, b);
+      a = b;
+      b += block;
+    }
+    insertion(a, n);
+    while (block < n) {
+      a = 0;
+      b = 2 * block;
+      while (b <= n) {
+        sym_merge(a, a + block, b);
+        a = b;
+        b += 2 * block;
+      }
+      int m = a + block;
+      if (m < n) sym_merge(a, m, n);
+      block *= 2;
+    }
+  }
+};
+template <class Less>
+void go_slice_stable(std::vector<int32_t>& d, Less less) {
+  GoStable<Less> s{d, less};
+  s.run();
+}
+
+// A device allocation accounted against the context budget.
+struct DevBuf {
+  void* ptr = nullptr;
+  uint64_t bytes = 0;
+};
+
+struct Mesh {
+  uint64_t handle = 0;
+  DevBuf buf;
+  const float* pos = nullptr;
+  const float* nrm = nullptr;
+  const float* col = nullptr;
+  const float* uv = nullptr;
+  const uint16_t* bones = nullptr;
+  const float* weights = nullptr;
+  const float* sk_pos = nullptr;
+  const float* sk_nrm = nullptr;
+  const uint32_t* idx = nullptr;
+  uint32_t V = 0, T = 0, K = 0;
+  std::vector<t3d_part_range> parts;
+  std::vector<uint32_t> part_vmin, part_vcount;
+  float dmin[3], dmax[3];
+  std::vector<uint16_t> max_bone;  // not used on the device; host check of palettes
+  uint32_t max_bone_index = 0;
+  bool has_skin = false;
+  int refs = 0;                    // in-flight jobs referencing the mesh
+};
+
+struct Volume {
+  uint64_t handle = 0;
+  DevBuf buf;
+  uint32_t n = 0;
+  std::vector<float> host;         // for per-object lookups on the host
+};
+
+}  // namespace
+
+struct t3d_frame {
+  // device tables and scratch
+  DevBuf tables, scratch, outputs;
+  uint64_t tables_cap = 0, scratch_cap = 0, outputs_cap = 0;
+  void* pinned = nullptr;          // host staging of the tables
+  uint64_t pinned_cap = 0;
+  uint32_t* pinned_stats = nullptr;
+  uint64_t pinned_stats_cap = 0;
+  FrameArgs args{};
+  // per-render bookkeeping
+  std::vector<t3d_pair_result> pairs;
+  std::vector<PartGPU> parts;
+  std::vector<uint32_t> part_first_tri;  // mesh triangle_start per pair
+  std::vector<uint64_t> bytes_vertex, bytes_cull;  // algorithmic bytes per part
+  uint32_t nparts = 0;
+  uint32_t total_vchunks = 0, total_tiles = 0, total_sblocks = 0;
+  std::vector<uint32_t> vchunk_prefix, tile_prefix, sblock_prefix;
+  uint32_t vertex_threads = 256;
+  uint64_t total_survivors = 0;
+  double last_ms = 0;
+  // graph cache of the batched chain
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<uint8_t> graph_key;
+};
+
+struct t3d_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t budget = 0, used = 0;
+  uint64_t next_handle = 1;
+  std::map<uint64_t, std::unique_ptr<Mesh>> meshes;
+  std::map<uint64_t, std::unique_ptr<Volume>> volumes;
+  std::string err;
+  t3d_tuning tuning{256, 1, 1024, 0, 1};
+  // profiling
+  bool profiling = false;
+  struct Prof {
+    double ms = 0;
+    uint32_t launches = 0;
+    uint64_t bytes = 0;
+  };
+  std::map<std::string, Prof> prof;
+};
+
+namespace {
+
+t3d_status fail(t3d_ctx* c, t3d_status st, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  if (c) c->err = vfmt(fmt, ap);
+  va_end(ap);
+  return st;
+}
+
+#define HIP_TRY(ctx, expr)                                                                   \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) return fail(ctx, T3D_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+t3d_status dev_alloc(t3d_ctx* c, DevBuf& b, uint64_t bytes) {
+  bytes = align_up(std::max<uint64_t>(bytes, 256), 256);
+  if (c->budget && c->used + bytes > c->budget)
+    return fail(c, T3D_ERR_BUDGET, "allocation of %llu bytes exceeds the HBM budget (%llu of %llu used)",
+                (unsigned long long)bytes, (unsigned long long)c->used, (unsigned long long)c->budget);
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) return fail(c, T3D_ERR_OUT_OF_MEMORY, "hipMalloc(%llu): %s", (unsigned long long)bytes, hipGetErrorString(e));
+  b.ptr = p;
+  b.bytes = bytes;
+  c->used += bytes;
+  return T3D_OK;
+}
+
+void dev_free(t3d_ctx* c, DevBuf& b) {
+  if (b.ptr) {
+    (void)hipFree(b.ptr);
+    c->used -= b.bytes;
+  }
+  b.ptr = nullptr;
+  b.bytes = 0;
+}
+
+// Grow `b` to at least `bytes` (discarding contents), with 25% headroom.
+t3d_status ensure(t3d_ctx* c, DevBuf& b, uint64_t& cap, uint64_t bytes) {
+  if (bytes <= cap && b.ptr) return T3D_OK;
+  dev_free(c, b);
+  uint64_t want = align_up(bytes + bytes / 4 + 4096, 1 << 16);
+  t3d_status st = dev_alloc(c, b, want);
+  if (st) {
+    cap = 0;
+    return st;
+  }
+  cap = b.bytes;
+  return T3D_OK;
+}
+
+// Bump allocator over one device block (256-byte aligned slices).
+struct Carve {
+  uint8_t* base;
+  uint64_t off = 0;
+  template <class T>
+  T* take(uint64_t count) {
+    T* p = reinterpret_cast<T*>(base + off);
+    off = align_up(off + std::max<uint64_t>(count, 1) * sizeof(T), 256);
+    return p;
+  }
+};
+template <class T>
+uint64_t carve_size(uint64_t count) {
+  return align_up(std::max<uint64_t>(count, 1) * sizeof(T), 256);
+}
+
+// ---------------------------------------------------------------------------
+// camera (camera.go:638-679, 933-983, 1085-1088)
+// ---------------------------------------------------------------------------
+
+struct Cam {
+  const t3d_camera_desc* d;
+  V3 pos;
+  M4 vp;
+  V3 fwd, right, up;
+  float sf_x, sf_y, sf_tang, aspect;
+};
+
+M4 projection(const t3d_camera_desc* d) {
+  if (d->perspective) {                                              // matrix.go:588-604
+    const float pi32 = (float)3.14159265358979323846;
+    float aspect = (float)d->width / (float)d->height;
+    float t = (float)std::tan((double)(d->fov * pi32 / 360.0f));
+    float b = -t, r = t * aspect, l = -r;
+    M4 p = zero4();
+    p.m[0][0] = (2 * d->near) / (r - l);
+    p.m[0][2] = (r + l) / (r - l);
+    p.m[1][1] = (2 * d->near) / (t - b);
+    p.m[1][2] = (t + b) / (t - b);
+    p.m[2][2] = -((d->far + d->near) / (d->far - d->near));
+    p.m[2][3] = -((2 * d->far * d->near) / (d->far - d->near));
+    p.m[3][2] = -1;
+    return p;
+  }
+  float asr = (float)d->height / (float)d->width;                    // camera.go:669-674
+  float right = 1 * d->ortho_scale, left = -1 * d->ortho_scale;
+  float top = asr * d->ortho_scale, bottom = -asr * d->ortho_scale;
+  M4 p = zero4();                                                    // matrix.go:609-616
+  p.m[0][0] = 2 / (right - left);
+  p.m[1][1] = 2 / (top - bottom);
+  p.m[2][2] = -2 / (d->far - d->near);
+  p.m[3][3] = 1;
+  return p;
+}
+
+void cam_setup(Cam& c, const t3d_camera_desc* d) {
+  c.d = d;
+  c.pos = row3(d->transform);
+  c.aspect = (float)d->width / (float)d->height;
+  float angle = d->fov * 3.1415f / 360;                              // camera.go:659-665
+  c.sf_tang = (float)std::tan((double)angle);
+  c.sf_y = 1.0f / (float)std::cos((double)angle);
+  c.sf_x = 1.0f / (float)std::cos(std::atan((double)(c.sf_tang * c.aspect)));
+  M4 t = translate(-c.pos.x, -c.pos.y, -c.pos.z);                    // ViewMatrix, camera.go:638-648
+  M4 view = mult(t, transposed(d->world_rotation));
+  c.vp = mult(view, projection(d));                                  // camera.go:1329
+  c.fwd = forward(d->world_rotation);
+  c.right = right(d->world_rotation);
+  c.up = up(d->world_rotation);
+}
+
+bool sphere_in_frustum(const Cam& c, V3 position, float radius) {    // camera.go:933-983
+  V3 diff = sub(position, c.pos);
+  float pcZ = dot(diff, v3(-c.fwd.x, -c.fwd.y, -c.fwd.z));
+  const t3d_camera_desc* d = c.d;
+  if (pcZ > d->far + radius || pcZ < d->near - radius) return false;
+  if (d->perspective) {
+    float dd = c.sf_y * radius;
+    pcZ *= c.sf_tang;
+    float pcY = dot(diff, c.up);
+    if (pcY > pcZ + dd || pcY < -pcZ - dd) return false;
+    pcZ *= c.aspect;
+    dd = c.sf_x * radius;
+    float pcX = dot(diff, c.right);
+    if (pcX > pcZ + dd || pcX < -pcZ - dd) return false;
+  } else {
+    float width = d->ortho_scale;
+    float height = width / c.aspect;
+    float pcY = dot(diff, c.up);
+    if (-height / 2 - radius > pcY || pcY > height / 2 + radius) return false;
+    float pcX = dot(diff, c.right);
+    if (-width / 2 - radius > pcX || pcX > width / 2 + radius) return false;
+  }
+  return true;
+}
+
+// Model.onTransformUpdate sphere (model.go:142-182).
+void model_sphere(const M4& transform, bool skinned, const float* dmin, const float* dmax, V3& pos, float& radius) {
+  V3 position, scale;
+  M4 rotation;
+  decompose(transform, position, scale, rotation);
+  V3 center = v3(0, 0, 0);
+  if (!skinned) center = v3((dmax[0] + dmin[0]) / 2, (dmax[1] + dmin[1]) / 2, (dmax[2] + dmin[2]) / 2);
+  center = mult_vec(rotation, center);
+  position.x += center.x * scale.x;
+  position.y += center.y * scale.y;
+  position.z += center.z * scale.z;
+  V3 mn = v3(dmin[0] * scale.x, dmin[1] * scale.y, dmin[2] * scale.z);
+  V3 mx = v3(dmax[0] * scale.x, dmax[1] * scale.y, dmax[2] * scale.z);
+  pos = position;
+  radius = magnitude(sub(mx, mn)) / 2;
+}
+
+bool is_transparent(const t3d_model_desc& m, const t3d_material_desc* mat) {   // model.go:1274-1282
+  if (mat) {
+    bool matT = mat->transparency_mode == T3D_TRANSPARENCY_TRANSPARENT || !mat->blend_source_over ||
+                (mat->transparency_mode == T3D_TRANSPARENCY_AUTO && mat->color[3] < 0.999f);
+    bool modelT = mat->transparency_mode != T3D_TRANSPARENCY_OPAQUE && m.color[3] < 0.999f;
+    return matT || modelT;
+  }
+  return m.color[3] < 0.999f;
+}
+
+const t3d_material_desc* part_material(const t3d_model_desc& m, uint32_t p) {
+  if (!m.materials) return nullptr;
+  return m.materials[p].present ? &m.materials[p] : nullptr;
+}
+
+// beginRender (light.go:110-112, 322-324) into a frame light record.
+LightGPU make_light(t3d_ctx* c, const t3d_light_desc& d) {
+  LightGPU l{};
+  l.type = d.type;
+  l.energy = d.energy;
+  l.range = d.range;
+  l.range_sq = d.range * d.range;
+  for (int i = 0; i < 3; i++) l.color[i] = d.color[i];
+  if (d.type == T3D_LIGHT_AMBIENT) {
+    for (int i = 0; i < 3; i++) l.amb[i] = d.color[i] * d.energy;
+  } else if (d.type == T3D_LIGHT_DIRECTIONAL) {
+    V3 f = forward(d.world_rotation);
+    l.fwd[0] = f.x;
+    l.fwd[1] = f.y;
+    l.fwd[2] = f.z;
+  } else if (d.type == T3D_LIGHT_VOLUME) {
+    auto it = c->volumes.find(d.volume);
+    if (it != c->volumes.end()) {
+      l.cells = reinterpret_cast<const float*>(it->second->buf.ptr);
+      l.ncells = it->second->n;
+    }
+    l.cx = d.cell_count[0];
+    l.cy = d.cell_count[1];
+    l.cz = d.cell_count[2];
+    for (int i = 0; i < 3; i++) {
+      l.vmin[i] = d.volume_min[i];
+      l.vmax[i] = d.volume_max[i];
+      l.cell[i] = d.cell_size[i];
+      l.offset[i] = d.offset[i];
+      l.bias[i] = d.light_bias[i];
+    }
+  }
+  return l;
+}
+
+// A scene light entry: the descriptor (nullptr for the world ambient light).
+struct SceneLight {
+  const t3d_light_desc* desc;
+  t3d_light_desc ambient;  // used when desc == nullptr
+  const t3d_light_desc& get() const { return desc ? *desc : ambient; }
+};
+
+struct Pair {
+  uint32_t model, part;
+  float depth;
+  int32_t order;
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int32_t t3d_abi_version(void) { return T3D_ABI_VERSION; }
+
+uint32_t t3d_struct_sizes(uint32_t* sizes, uint32_t max) {
+  const uint32_t s[] = {(uint32_t)sizeof(t3d_vertex),        (uint32_t)sizeof(t3d_mesh_desc),
+                        (uint32_t)sizeof(t3d_material_desc), (uint32_t)sizeof(t3d_light_desc),
+                        (uint32_t)sizeof(t3d_model_desc),    (uint32_t)sizeof(t3d_camera_desc),
+                        (uint32_t)sizeof(t3d_world_desc),    (uint32_t)sizeof(t3d_render_desc),
+                        (uint32_t)sizeof(t3d_pair_result),   (uint32_t)sizeof(t3d_kernel_time),
+                        (uint32_t)sizeof(t3d_tuning)};
+  uint32_t n = (uint32_t)(sizeof s / sizeof s[0]);
+  for (uint32_t i = 0; i < n && i < max; i++) sizes[i] = s[i];
+  return n;
+}
+
+t3d_status t3d_ctx_create(int32_t device, uint64_t hbm_budget_bytes, t3d_ctx** out) {
+  if (!out) return T3D_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return T3D_ERR_DEVICE;
+  if (device < 0 || device >= n) return T3D_ERR_INVALID_ARGUMENT;
+  if (hipSetDevice(device) != hipSuccess) return T3D_ERR_DEVICE;
+  auto* c = new t3d_ctx();
+  c->device = device;
+  c->budget = hbm_budget_bytes;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return T3D_ERR_DEVICE;
+  }
+  *out = c;
+  return T3D_OK;
+}
+
+t3d_status t3d_ctx_destroy(t3d_ctx* c) {
+  if (!c) return T3D_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& kv : c->meshes) dev_free(c, kv.second->buf);
+  for (auto& kv : c->volumes) dev_free(c, kv.second->buf);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return T3D_OK;
+}
+
+const char* t3d_last_error(const t3d_ctx* c) { return c ? c->err.c_str() : "null context"; }
+uint64_t t3d_ctx_memory_used(const t3d_ctx* c) { return c ? c->used : 0; }
+
+t3d_status t3d_mesh_upload(t3d_ctx* c, const t3d_mesh_desc* d, uint64_t* out_handle) {
+  if (!c || !d || !out_handle) return fail(c, T3D_ERR_INVALID_ARGUMENT, "mesh_upload: null argument");
+  *out_handle = 0;
+  if (d->vertex_count == 0 || !d->positions || !d->normals)          // Mesh.AddVertices panics on 0 vertices (mesh.go:680-682)
+    return fail(c, T3D_ERR_INVALID_ARGUMENT, "mesh_upload: no vertices");
+  if (d->triangle_count && !d->indices) return fail(c, T3D_ERR_INVALID_ARGUMENT, "mesh_upload: no indices");
+  if (d->part_count && !d->parts) return fail(c, T3D_ERR_INVALID_ARGUMENT, "mesh_upload: no parts");
+  if ((d->bone_indices == nullptr) != (d->bone_weights == nullptr))
+    return fail(c, T3D_ERR_INVALID_ARGUMENT, "mesh_upload: bone indices and weights must come together");
+  if (d->shape_key_count && (!d->shape_key_positions || !d->shape_key_normals))
+    return fail(c, T3D_ERR_INVALID_ARGUMENT, "mesh_upload: shape keys without data");
+  const uint64_t V = d->vertex_count, T = d->triangle_count, K = d->shape_key_count;
+  auto m = std::make_unique<Mesh>();
+  m->V = d->vertex_count;
+  m->T = d->triangle_count;
+  m->K = d->shape_key_count;
+  for (int i = 0; i < 3; i++) {
+    m->dmin[i] = d->dimensions_min[i];
+    m->dmax[i] = d->dimensions_max[i];
+  }
+  // validate indices and part ranges; referenced vertex range per part
+  for (uint64_t i = 0; i < 3 * T; i++)
+    if (d->indices[i] >= d->vertex_count)
+      return fail(c, T3D_ERR_INVALID_ARGUMENT, "mesh_upload: triangle %llu references vertex %u of %u",
+                  (unsigned long long)(i / 3), d->indices[i], d->vertex_count);
+  for (uint32_t p = 0; p < d->part_count; p++) {
+    const t3d_part_range& r = d->parts[p];
+    if ((uint64_t)r.triangle_start + r.triangle_count > T)
+      return fail(c, T3D_ERR_INVALID_ARGUMENT, "mesh_upload: part %u exceeds the triangle count", p);
+    uint32_t lo = UINT32_MAX, hi = 0;
+    for (uint64_t t = r.triangle_start; t < (uint64_t)r.triangle_start + r.triangle_count; t++)
+      for (int k = 0; k < 3; k++) {
+        lo = std::min(lo, d->indices[3 * t + k]);
+        hi = std::max(hi, d->indices[3 * t + k]);
+      }
+    if (r.triangle_count == 0) lo = hi = 0;
+    m->parts.push_back(r);
+    m->part_vmin.push_back(lo);
+    m->part_vcount.push_back(r.triangle_count ? hi - lo + 1 : 0);
+  }
+  if (d->bone_indices) {
+    m->has_skin = true;
+    for (uint64_t i = 0; i < 4 * V; i++)
+      if (d->bone_weights[i] != 0) m->max_bone_index = std::max<uint32_t>(m->max_bone_index, d->bone_indices[i]);
+  }
+  uint64_t bytes = carve_size<float>(3 * V) * 2 + (d->colors ? carve_size<float>(4 * V) : 0) +
+                   (d->uvs ? carve_size<float>(2 * V) : 0) + (d->bone_indices ? carve_size<uint16_t>(4 * V) + carve_size<float>(4 * V) : 0) +
+                   (K ? carve_size<float>(3 * K * V) * 2 : 0) + carve_size<uint32_t>(3 * T);
+  (void)hipSetDevice(c->device);
+  t3d_status st = dev_alloc(c, m->buf, bytes);
+  if (st) return st;
+  Carve cv{reinterpret_cast<uint8_t*>(m->buf.ptr)};
+  std::vector<std::pair<void*, std::pair<const void*, uint64_t>>> copies;
+  auto put = [&](const void* src, uint64_t n, auto* typed) {
+    using TT = std::remove_pointer_t<decltype(typed)>;
+    TT* dst = cv.take<TT>(n);
+    copies.push_back({dst, {src, n * sizeof(TT)}});
+    return dst;
+  };
+  m->pos = put(d->positions, 3 * V, (float*)nullptr);
+  m->nrm = put(d->normals, 3 * V, (float*)nullptr);
+  if (d->colors) m->col = put(d->colors, 4 * V, (float*)nullptr);
+  if (d->uvs) m->uv = put(d->uvs, 2 * V, (float*)nullptr);
+  if (d->bone_indices) {
+    m->bones = put(d->bone_indices, 4 * V, (uint16_t*)nullptr);
+    m->weights = put(d->bone_weights, 4 * V, (float*)nullptr);
+  }
+  if (K) {
+    m->sk_pos = put(d->shape_key_positions, 3 * K * V, (float*)nullptr);
+    m->sk_nrm = put(d->shape_key_normals, 3 * K * V, (float*)nullptr);
+  }
+  m->idx = put(d->indices, 3 * T, (uint32_t*)nullptr);
+  for (auto& cp : copies) {
+    if (!cp.second.second) continue;
+    hipError_t e = hipMemcpy(cp.first, cp.second.first, cp.second.second, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      dev_free(c, m->buf);
+      return fail(c, T3D_ERR_DEVICE, "mesh upload copy: %s", hipGetErrorString(e));
+    }
+  }
+  m->handle = c->next_handle++;
+  *out_handle = m->handle;
+  c->meshes[m->handle] = std::move(m);
+  return T3D_OK;
+}
+
+t3d_status t3d_mesh_free(t3d_ctx* c, uint64_t handle) {
+  if (!c) return T3D_ERR_INVALID_ARGUMENT;
+  auto it = c->meshes.find(handle);
+  if (it == c->meshes.end()) return fail(c, T3D_ERR_NOT_FOUND, "mesh_free: unknown handle %llu", (unsigned long long)handle);
+  if (it->second->refs > 0) return fail(c, T3D_ERR_BUSY, "mesh_free: mesh %llu is referenced by in-flight work", (unsigned long long)handle);
+  (void)hipStreamSynchronize(c->stream);
+  dev_free(c, it->second->buf);
+  c->meshes.erase(it);
+  return T3D_OK;
+}
+
+t3d_status t3d_volume_upload(t3d_ctx* c, const float* cells, uint32_t n, uint64_t* out_handle) {
+  if (!c || !out_handle || (!cells && n)) return fail(c, T3D_ERR_INVALID_ARGUMENT, "volume_upload: null argument");
+  auto v = std::make_unique<Volume>();
+  v->n = n;
+  v->host.assign(cells, cells + 4 * (size_t)n);
+  t3d_status st = dev_alloc(c, v->buf, (uint64_t)n * 16);
+  if (st) return st;
+  if (n) HIP_TRY(c, hipMemcpy(v->buf.ptr, cells, (size_t)n * 16, hipMemcpyHostToDevice));
+  v->handle = c->next_handle++;
+  *out_handle = v->handle;
+  c->volumes[v->handle] = std::move(v);
+  return T3D_OK;
+}
+
+t3d_status t3d_volume_free(t3d_ctx* c, uint64_t handle) {
+  if (!c) return T3D_ERR_INVALID_ARGUMENT;
+  auto it = c->volumes.find(handle);
+  if (it == c->volumes.end()) return fail(c, T3D_ERR_NOT_FOUND, "volume_free: unknown handle");
+  (void)hipStreamSynchronize(c->stream);
+  dev_free(c, it->second->buf);
+  c->volumes.erase(it);
+  return T3D_OK;
+}
+
+t3d_status t3d_frame_create(t3d_ctx* c, t3d_frame** out) {
+  if (!c || !out) return T3D_ERR_INVALID_ARGUMENT;
+  *out = new t3d_frame();
+  return T3D_OK;
+}
+
+t3d_status t3d_frame_destroy(t3d_ctx* c, t3d_frame* f) {
+  if (!c || !f) return T3D_ERR_INVALID_ARGUMENT;
+  (void)hipStreamSynchronize(c->stream);
+  dev_free(c, f->tables);
+  dev_free(c, f->scratch);
+  dev_free(c, f->outputs);
+  if (f->pinned) (void)hipHostFree(f->pinned);
+  if (f->pinned_stats) (void)hipHostFree(f->pinned_stats);
+  if (f->graph_exec) (void)hipGraphExecDestroy(f->graph_exec);
+  delete f;
+  return T3D_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Records a kernel launch between two events when profiling is on.
+struct ProfScope {
+  t3d_ctx* c;
+  const char* name;
+  uint64_t bytes;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ProfScope(t3d_ctx* c_, const char* n, uint64_t b) : c(c_), name(n), bytes(b) {
+    if (c->profiling) {
+      (void)hipEventCreate(&e0);
+      (void)hipEventCreate(&e1);
+      (void)hipEventRecord(e0, c->stream);
+    }
+  }
+  ~ProfScope() {
+    if (c->profiling) {
+      (void)hipEventRecord(e1, c->stream);
+      (void)hipEventSynchronize(e1);
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      auto& p = c->prof[name];
+      p.ms += ms;
+      p.launches++;
+      p.bytes += bytes;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+    }
+  }
+};
+
+// The kernel chain of one frame over parts [p0, p1) (all of them for the
+// batched form).  Block offsets come from the frame's prefix arrays.  The
+// profile charges each kernel its algorithmic bytes: mesh streams read once
+// (k_vertex), triangle indices once (k_cull), outputs (k_assemble, added
+// after the survivor counts are known).
+t3d_status run_chain(t3d_ctx* c, t3d_frame* f, uint32_t p0, uint32_t p1) {
+  const FrameArgs& a = f->args;
+  struct {
+    uint64_t vertex = 0, cull = 0;
+  } tr;
+  if (c->profiling)
+    for (uint32_t i = p0; i < p1; i++) {
+      tr.vertex += f->bytes_vertex[i];
+      tr.cull += f->bytes_cull[i];
+    }
+  const uint32_t vb0 = f->vchunk_prefix[p0], vb1 = f->vchunk_prefix[p1];
+  const uint32_t tb0 = f->tile_prefix[p0], tb1 = f->tile_prefix[p1];
+  const uint32_t sb0 = f->sblock_prefix[p0], sb1 = f->sblock_prefix[p1];
+  {
+    ProfScope ps(c, "k_vertex", tr.vertex);
+    HIP_TRY(c, launch_vertex(a, vb0, vb1 - vb0, f->vertex_threads, c->stream));
+  }
+  {
+    ProfScope ps(c, "k_cull", tr.cull);
+    HIP_TRY(c, launch_cull(a, tb0, tb1 - tb0, c->stream));
+  }
+  if (sb1 > sb0) {
+    {
+      ProfScope ps(c, "k_hist", 0);
+      HIP_TRY(c, launch_hist(a, tb0, tb1 - tb0, c->stream));
+    }
+    ProfScope ps(c, "k_scan", 0);
+    HIP_TRY(c, launch_scan(a, sb0, sb1 - sb0, p0, p1 - p0, c->stream));
+  }
+  {
+    ProfScope ps(c, "k_scatter", 0);
+    HIP_TRY(c, launch_scatter(a, tb0, tb1 - tb0, c->stream));
+  }
+  {
+    ProfScope ps(c, "k_assemble", 0);
+    HIP_TRY(c, launch_assemble(a, tb0, tb1 - tb0, c->stream));
+  }
+  return T3D_OK;
+}
+
+}  // namespace
+
+extern "C" t3d_status t3d_render(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc) {
+  auto wall0 = std::chrono::steady_clock::now();
+  if (!c || !f || !desc || !desc->camera) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: null argument");
+  const t3d_camera_desc* cd = desc->camera;
+  if (cd->width <= 0 || cd->height <= 0) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: camera size %dx%d", cd->width, cd->height);
+  if (cd->render_normals) return fail(c, T3D_ERR_UNSUPPORTED, "render: Camera.RenderNormals is outside the GPU path");
+  if (desc->light_count && !desc->lights) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: lights missing");
+  if (desc->model_count && !desc->models) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: models missing");
+  const t3d_world_desc* world = (desc->world && desc->world->present) ? desc->world : nullptr;
+  (void)hipSetDevice(c->device);
+
+  Cam cam;
+  cam_setup(cam, cd);
+
+  // scene light list (camera.go:1275-1303)
+  std::vector<SceneLight> scene_lights;
+  if (world && world->lighting_on && world->ambient_visible) {
+    SceneLight s{};
+    s.desc = nullptr;
+    s.ambient.type = T3D_LIGHT_AMBIENT;
+    s.ambient.visible = 1;
+    for (int i = 0; i < 3; i++) s.ambient.color[i] = world->ambient_color[i];
+    s.ambient.energy = world->ambient_energy;
+    scene_lights.push_back(s);
+  }
+  for (uint32_t i = 0; i < desc->light_count; i++) {
+    const t3d_light_desc& l = desc->lights[i];
+    if (l.type < T3D_LIGHT_AMBIENT || l.type > T3D_LIGHT_VOLUME)
+      return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: light %u has unknown type %d", i, l.type);
+    if (l.type == T3D_LIGHT_VOLUME && !c->volumes.count(l.volume))
+      return fail(c, T3D_ERR_NOT_FOUND, "render: light %u references unknown volume", i);
+    if ((!world || world->lighting_on) && l.visible) scene_lights.push_back(SceneLight{&l, {}});
+  }
+
+  // render pairs (camera.go:1343-1457)
+  std::vector<Pair> pairs;
+  std::vector<int32_t> solids, transparents;
+  for (uint32_t mi = 0; mi < desc->model_count; mi++) {
+    const t3d_model_desc& md = desc->models[mi];
+    if (!md.visible || md.mesh == 0) continue;
+    auto it = c->meshes.find(md.mesh);
+    if (it == c->meshes.end()) return fail(c, T3D_ERR_NOT_FOUND, "render: model %u references unknown mesh %llu", mi, (unsigned long long)md.mesh);
+    const Mesh& mesh = *it->second;
+    if (md.has_vertex_transform_function || md.has_vertex_clip_function)
+      return fail(c, T3D_ERR_UNSUPPORTED, "render: model %u uses a Go vertex callback", mi);
+    if (md.skinned) {
+      if (!mesh.has_skin || !md.bone_influences) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: skinned model %u without skin data", mi);
+      if (mesh.max_bone_index >= md.bone_count) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u bone index out of palette", mi);
+    }
+    if (md.light_group_active && md.light_group_count && !md.light_group)
+      return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u light group missing", mi);
+    if (md.frustum_culling) {
+      V3 sp;
+      float sr;
+      model_sphere(md.transform, md.skinned != 0, mesh.dmin, mesh.dmax, sp, sr);
+      if (!sphere_in_frustum(cam, sp, sr)) continue;
+    }
+    for (uint32_t p = 0; p < mesh.parts.size(); p++) {
+      const t3d_material_desc* mat = part_material(md, p);
+      if (mat && !mat->visible) continue;
+      Pair rp{mi, p, dist_sq(cam.pos, row3(md.transform)), mat ? mat->render_order : 0};
+      (is_transparent(md, mat) ? transparents : solids).push_back((int32_t)pairs.size());
+      pairs.push_back(rp);
+    }
+  }
+  if (!cd->render_depth)                                             // camera.go:1458-1467
+    go_slice_stable(solids, [&](int32_t a, int32_t b) {
+      if (pairs[a].order < pairs[b].order) return true;
+      return pairs[a].depth > pairs[b].depth;
+    });
+  if (cd->max_light_count > 0) {                                     // camera.go:1474-1497
+    V3 fp = cd->has_max_light_count_focus ? v3(cd->max_light_count_focus[0], cd->max_light_count_focus[1], cd->max_light_count_focus[2]) : cam.pos;
+    std::vector<int32_t> ord(scene_lights.size());
+    for (size_t i = 0; i < ord.size(); i++) ord[i] = (int32_t)i;
+    go_slice_stable(ord, [&](int32_t a, int32_t b) {
+      const t3d_light_desc& la = scene_lights[a].get();
+      const t3d_light_desc& lb = scene_lights[b].get();
+      if (la.type == T3D_LIGHT_AMBIENT) return true;
+      else if (lb.type == T3D_LIGHT_AMBIENT) return false;
+      else if (la.type == T3D_LIGHT_DIRECTIONAL) return true;
+      else if (lb.type == T3D_LIGHT_DIRECTIONAL) return false;
+      return dist_sq(fp, row3(la.transform)) < dist_sq(fp, row3(lb.transform));
+    });
+    std::vector<SceneLight> sorted;
+    for (int32_t i : ord) sorted.push_back(scene_lights[i]);
+    if ((int)sorted.size() > cd->max_light_count) sorted.resize(cd->max_light_count);
+    scene_lights = sorted;
+  }
+  go_slice_stable(transparents, [&](int32_t a, int32_t b) {           // camera.go:2050-2055
+    if (pairs[a].order == pairs[b].order) return pairs[a].depth > pairs[b].depth;
+    return pairs[a].order < pairs[b].order;
+  });
+  std::vector<int32_t> draw;                                          // camera.go:2057-2109
+  for (int32_t i : solids) draw.push_back(i);
+  for (int32_t i : transparents) draw.push_back(i);
+
+  // frame lights: scene lights first, then light-group lights as needed
+  std::vector<LightGPU> lights;
+  for (auto& s : scene_lights) lights.push_back(make_light(c, s.get()));
+  const uint32_t n_scene = (uint32_t)lights.size();
+  std::map<uint32_t, uint32_t> group_base;  // model -> first frame light of its group
+
+  // per part uniforms
+  FrameConst fc{};
+  fc.vp = cam.vp;
+  fc.campos[0] = cam.pos.x;
+  fc.campos[1] = cam.pos.y;
+  fc.campos[2] = cam.pos.z;
+  fc.near_ = cd->near;
+  fc.far_ = cd->far;
+  fc.cam_w = (float)cd->width;
+  fc.cam_h = (float)cd->height;
+  fc.half_w = (float)cd->width / 2;
+  fc.half_h = (float)cd->height / 2;
+  fc.dmp = (cd->far - cd->near) * cd->depth_margin;                   // camera.go:1339-1341
+  fc.spread = cd->far - cd->near + (fc.dmp * 2);
+  fc.snap = cd->vertex_snapping;
+  fc.perspective = cd->perspective;
+  fc.pc_tex = cd->perspective_corrected_texture_mapping;
+  fc.render_depth = cd->render_depth;
+  fc.fog_active = (!cd->render_depth && world && world->fog_on) ? 1 : 0;
+  fc.fog_mode = world ? world->fog_mode : 0;
+  if (world) {
+    for (int i = 0; i < 4; i++) fc.fog_color[i] = world->fog_color[i];
+    fc.fog_range[0] = world->fog_range[0];
+    fc.fog_range[1] = world->fog_range[1];
+  }
+
+  const t3d_tuning& tu = c->tuning;
+  const uint32_t vthreads = tu.vertex_block ? tu.vertex_block : 256;
+  const uint32_t vchunk = vthreads * std::max<uint32_t>(1, tu.vertices_per_thread);
+  const uint32_t tile = std::max<uint32_t>(64, tu.triangle_tile);
+
+  f->pairs.clear();
+  f->parts.clear();
+  f->part_first_tri.clear();
+  std::vector<PartLight> plights;
+  std::vector<const float*> palettes_host;   // host palettes to upload, per part (nullptr if none)
+  std::vector<uint32_t> palette_count;
+  std::vector<const float*> skw_host;
+  std::vector<uint32_t> skw_count;
+  uint64_t sumV = 0, sumT = 0, sumHist = 0;
+  uint32_t sblocks = 0;
+  f->vchunk_prefix.assign(1, 0);
+  f->tile_prefix.assign(1, 0);
+  f->sblock_prefix.assign(1, 0);
+  f->bytes_vertex.clear();
+  f->bytes_cull.clear();
+  std::map<uint32_t, M4> rd_cache;
+
+  for (int32_t di : draw) {
+    const Pair& rp = pairs[di];
+    const t3d_model_desc& md = desc->models[rp.model];
+    if (!md.visible || md.auto_batch_static) continue;               // camera.go:2067-2071
+    const Mesh& mesh = *c->meshes[md.mesh];
+    const t3d_material_desc* mat = part_material(md, rp.part);
+    if (mat && (mat->billboard || mat->custom_depth_function))
+      return fail(c, T3D_ERR_UNSUPPORTED, "render: model %u part %u uses billboarding or a custom depth function", rp.model, rp.part);
+    const t3d_part_range& pr = mesh.parts[rp.part];
+    bool lighting = false;                                           // camera.go:1521-1528
+    if (world) lighting = mat ? (world->lighting_on && !mat->shadeless && !md.shadeless) : (world->lighting_on && !md.shadeless);
+    const bool early_out = (md.color[3] == 0 && is_transparent(md, mat)) || !md.visible;   // model.go:604-606
+
+    PartGPU P{};
+    const M4& M = md.transform;
+    P.mvp = mult(M, cam.vp);                                          // model.go:695
+    P.model = M;
+    V3 icp = inverted_position(M, cam.pos);                           // model.go:629-630
+    P.icp[0] = icp.x;
+    P.icp[1] = icp.y;
+    P.icp[2] = icp.z;
+    float mp[4] = {md.color[0], md.color[1], md.color[2], md.color[3]};   // camera.go:1597-1601
+    if (mat)
+      for (int i = 0; i < 4; i++) mp[i] *= mat->color[i];
+    for (int i = 0; i < 4; i++) P.mp[i] = mp[i];
+    if (mat && mat->use_texture) {
+      P.src_w = mat->texture_size[0];
+      P.src_h = mat->texture_size[1];
+    }
+    uint32_t flags = 0;
+    if (md.skinned) flags |= PF_SKINNED;
+    if (mat && mat->backface_culling) flags |= PF_BACKFACE;
+    if (lighting) flags |= PF_LIGHTING;
+    if (mesh.col) flags |= PF_HAS_COLOR;
+    if (mesh.uv) flags |= PF_HAS_UV;
+    if (mesh.K) flags |= PF_SHAPE_KEYS;
+    const uint32_t sort_mode = mat ? (uint32_t)mat->triangle_sort_mode : T3D_SORT_BACK_TO_FRONT;
+    if (sort_mode > T3D_SORT_NONE) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: triangle sort mode %u", sort_mode);
+    flags |= sort_mode << PF_SORT_SHIFT;
+    flags |= ((uint32_t)(mat ? mat->lighting_mode : 0) & 3u) << PF_LMODE_SHIFT;
+    const uint32_t lv_mode = (uint32_t)(mat ? mat->light_volume_shading_mode : 0) & 3u;
+    flags |= lv_mode << PF_LVMODE_SHIFT;
+    P.flags = flags;
+
+    // lights of this part: beginModel (light.go:175-189, 326-330) and the
+    // per-model point-light skip (camera.go:1592, 1625-1629)
+    if (lighting) {
+      uint32_t lbase = 0, lcount = n_scene;
+      if (md.light_group_active) {
+        auto g = group_base.find(rp.model);
+        if (g == group_base.end()) {
+          uint32_t b = (uint32_t)lights.size();
+          for (uint32_t k = 0; k < md.light_group_count; k++) {
+            const t3d_light_desc& l = md.light_group[k];
+            if (l.type == T3D_LIGHT_VOLUME && !c->volumes.count(l.volume))
+              return fail(c, T3D_ERR_NOT_FOUND, "render: light group references unknown volume");
+            lights.push_back(make_light(c, l));
+          }
+          g = group_base.emplace(rp.model, b).first;
+        }
+        lbase = g->second;
+        lcount = md.light_group_count;
+      }
+      if (lcount > kMaxLights) return fail(c, T3D_ERR_UNSUPPORTED, "render: %u lights on one model (max %u)", lcount, kMaxLights);
+      P.light_base = (uint32_t)plights.size();
+      P.light_count = lcount;
+      const float maxSpan = magnitude(sub(v3(mesh.dmax[0], mesh.dmax[1], mesh.dmax[2]), v3(mesh.dmin[0], mesh.dmin[1], mesh.dmin[2])));
+      const V3 modelPos = row3(M);
+      bool need_rd = false;
+      for (uint32_t k = 0; k < lcount; k++) {
+        const uint32_t li = lbase + k;
+        const t3d_light_desc& ld = (li < n_scene) ? scene_lights[li].get() : md.light_group[k];
+        PartLight pl{};
+        pl.light = li;
+        pl.enabled = 1;
+        if (ld.type == T3D_LIGHT_POINT) {
+          V3 wp = row3(ld.transform);
+          V3 w = md.skinned ? wp : inverted_position(M, wp);
+          pl.v[0] = w.x;
+          pl.v[1] = w.y;
+          pl.v[2] = w.z;
+          if (ld.range > 0) {
+            float dist = maxSpan + ld.range;
+            if (dist_sq(modelPos, wp) > dist * dist) pl.enabled = 0;
+          }
+        } else if (ld.type == T3D_LIGHT_DIRECTIONAL) {
+          need_rd = true;
+        } else if (ld.type == T3D_LIGHT_VOLUME && lv_mode == T3D_LV_PER_OBJECT) {
+          const Volume& vol = *c->volumes[ld.volume];
+          V3 p = modelPos;
+          int64_t x = go_int(round32((p.x - ld.offset[0] - ld.volume_min[0] - (ld.cell_size[0] / 2)) / ld.cell_size[0]));
+          int64_t y = go_int(round32((p.y - ld.offset[1] - ld.volume_min[1] - (ld.cell_size[1] / 2)) / ld.cell_size[1]));
+          int64_t z = go_int(round32((p.z - ld.offset[2] - ld.volume_min[2] - (ld.cell_size[2] / 2)) / ld.cell_size[2]));
+          const int cx = ld.cell_count[0], cy = ld.cell_count[1], cz = ld.cell_count[2];
+          if (!(x < 0 || y < 0 || z < 0 || x >= cx || y >= cy || z >= cz)) {
+            int64_t ci = x + (y * cx) + (z * cx * cy);
+            if (ci < (int64_t)vol.n) {
+              pl.cell_found = 1;
+              for (int q = 0; q < 3; q++) pl.v[q] = vol.host[4 * ci + q];
+            }
+          }
+        }
+        plights.push_back(pl);
+      }
+      if (need_rd && !md.skinned) {
+        auto r = rd_cache.find(rp.model);
+        if (r == rd_cache.end()) r = rd_cache.emplace(rp.model, transposed(inverted(md.world_rotation))).first;
+        P.rd = r->second;
+      }
+    }
+
+    // mesh arrays and frame slices
+    P.pos = mesh.pos;
+    P.nrm = mesh.nrm;
+    P.col = mesh.col;
+    P.uv = mesh.uv;
+    P.bones = mesh.bones;
+    P.weights = mesh.weights;
+    P.idx = mesh.idx + 3 * (size_t)pr.triangle_start;
+    P.sk_pos = mesh.sk_pos;
+    P.sk_nrm = mesh.sk_nrm;
+    P.shape_keys = mesh.K;
+    P.mesh_vcount = mesh.V;
+    P.bone_count = md.skinned ? md.bone_count : 0;
+    P.vmin = mesh.part_vmin[rp.part];
+    P.vcount = early_out ? 0 : mesh.part_vcount[rp.part];
+    P.tcount = early_out ? 0 : pr.triangle_count;
+    P.vbase = (uint32_t)sumV;
+    P.tbase = (uint32_t)sumT;
+    P.out_base = 3 * sumT;
+    const uint32_t nvchunks = (P.vcount + vchunk - 1) / vchunk;
+    const uint32_t ntiles = (P.tcount + tile - 1) / tile;
+    P.ntiles = ntiles;
+    if (ntiles > 1) {
+      P.hbase = (uint32_t)sumHist;
+      P.sblock0 = sblocks;
+      P.nsblocks = (kBins * ntiles + kScanBlock - 1) / kScanBlock;
+      sumHist += (uint64_t)kBins * ntiles;
+      sblocks += P.nsblocks;
+    }
+    sumV += P.vcount;
+    sumT += P.tcount;
+    f->vchunk_prefix.push_back(f->vchunk_prefix.back() + nvchunks);
+    f->tile_prefix.push_back(f->tile_prefix.back() + ntiles);
+    f->sblock_prefix.push_back(sblocks);
+    palettes_host.push_back(md.skinned ? &md.bone_influences[0].m[0][0] : nullptr);
+    palette_count.push_back(md.skinned ? md.bone_count : 0);
+    skw_host.push_back(mesh.K ? md.shape_key_weights : nullptr);
+    skw_count.push_back(mesh.K);
+    // algorithmic bytes: mesh streams read once, triangle indices once, outputs
+    const uint64_t vb = 24 + (mesh.col ? 16 : 0) + (md.skinned ? 24 : 0);
+    f->bytes_vertex.push_back((uint64_t)P.vcount * vb + (md.skinned ? 64ull * md.bone_count : 0) + sizeof(PartGPU));
+    f->bytes_cull.push_back(12ull * P.tcount);
+    f->parts.push_back(P);
+    f->part_first_tri.push_back(pr.triangle_start);
+    t3d_pair_result r{};
+    r.model_index = rp.model;
+    r.part_index = rp.part;
+    r.triangle_count = pr.triangle_count;
+    r.vertex_offset = P.out_base;
+    f->pairs.push_back(r);
+  }
+  const uint32_t P_n = (uint32_t)f->parts.size();
+  f->nparts = P_n;
+
+  // palettes and shape-key weights live in the table block
+  uint64_t pal_floats = 0, skw_floats = 0;
+  for (uint32_t i = 0; i < P_n; i++) {
+    pal_floats += 16ull * palette_count[i];
+    skw_floats += skw_count[i];
+  }
+
+  // device tables: FrameConst, parts, lights, plights, prefixes, palettes, shape-key weights
+  const uint64_t tbytes = carve_size<FrameConst>(1) + carve_size<PartGPU>(P_n) + carve_size<LightGPU>(lights.size()) +
+                          carve_size<PartLight>(plights.size()) + 3 * carve_size<uint32_t>(P_n + 1) +
+                          carve_size<float>(pal_floats) + carve_size<float>(skw_floats);
+  t3d_status st = ensure(c, f->tables, f->tables_cap, tbytes);
+  if (st) return st;
+  if (tbytes > f->pinned_cap) {
+    if (f->pinned) (void)hipHostFree(f->pinned);
+    f->pinned = nullptr;
+    f->pinned_cap = align_up(tbytes + tbytes / 4, 1 << 16);
+    HIP_TRY(c, hipHostMalloc(&f->pinned, f->pinned_cap, hipHostMallocDefault));
+  }
+  // scratch: vertex streams, keys, sorted ids, stats, histograms, block sums
+  const uint64_t sbytes = 3 * carve_size<float4>(sumV) + 2 * carve_size<uint32_t>(sumT) + carve_size<uint32_t>(4ull * P_n) +
+                          carve_size<uint32_t>(sumHist) + carve_size<uint32_t>(sblocks);
+  st = ensure(c, f->scratch, f->scratch_cap, sbytes);
+  if (st) return st;
+  const uint64_t obytes = 2 * carve_size<t3d_vertex>(3 * sumT) + carve_size<uint32_t>(3 * sumT);
+  st = ensure(c, f->outputs, f->outputs_cap, obytes);
+  if (st) return st;
+
+  // fill the pinned staging block with the same carve layout as the device block
+  Carve dev{reinterpret_cast<uint8_t*>(f->tables.ptr)};
+  Carve host{reinterpret_cast<uint8_t*>(f->pinned)};
+  FrameConst* d_fc = dev.take<FrameConst>(1);
+  *host.take<FrameConst>(1) = fc;
+  PartGPU* d_parts = dev.take<PartGPU>(P_n);
+  PartGPU* h_parts = host.take<PartGPU>(P_n);
+  LightGPU* d_lights = dev.take<LightGPU>(lights.size());
+  LightGPU* h_lights = host.take<LightGPU>(lights.size());
+  PartLight* d_pl = dev.take<PartLight>(plights.size());
+  PartLight* h_pl = host.take<PartLight>(plights.size());
+  uint32_t* d_vpre = dev.take<uint32_t>(P_n + 1);
+  uint32_t* h_vpre = host.take<uint32_t>(P_n + 1);
+  uint32_t* d_tpre = dev.take<uint32_t>(P_n + 1);
+  uint32_t* h_tpre = host.take<uint32_t>(P_n + 1);
+  uint32_t* d_spre = dev.take<uint32_t>(P_n + 1);
+  uint32_t* h_spre = host.take<uint32_t>(P_n + 1);
+  float* d_pal = dev.take<float>(pal_floats);
+  float* h_pal = host.take<float>(pal_floats);
+  float* d_skw = dev.take<float>(skw_floats);
+  float* h_skw = host.take<float>(skw_floats);
+  uint64_t pal_off = 0, skw_off = 0;
+  for (uint32_t i = 0; i < P_n; i++) {
+    PartGPU& P = f->parts[i];
+    if (palette_count[i]) {
+      std::memcpy(h_pal + pal_off, palettes_host[i], 64ull * palette_count[i]);
+      P.palette = d_pal + pal_off;
+      pal_off += 16ull * palette_count[i];
+    }
+    if (skw_count[i]) {
+      for (uint32_t k = 0; k < skw_count[i]; k++) h_skw[skw_off + k] = skw_host[i] ? skw_host[i][k] : 0.0f;
+      P.sk_w = d_skw + skw_off;
+      skw_off += skw_count[i];
+    }
+  }
+  if (P_n) std::memcpy(h_parts, f->parts.data(), sizeof(PartGPU) * P_n);
+  if (!lights.empty()) std::memcpy(h_lights, lights.data(), sizeof(LightGPU) * lights.size());
+  if (!plights.empty()) std::memcpy(h_pl, plights.data(), sizeof(PartLight) * plights.size());
+  std::memcpy(h_vpre, f->vchunk_prefix.data(), 4 * (P_n + 1));
+  std::memcpy(h_tpre, f->tile_prefix.data(), 4 * (P_n + 1));
+  std::memcpy(h_spre, f->sblock_prefix.data(), 4 * (P_n + 1));
+
+  Carve sc{reinterpret_cast<uint8_t*>(f->scratch.ptr)};
+  Carve oc{reinterpret_cast<uint8_t*>(f->outputs.ptr)};
+  FrameArgs& a = f->args;
+  a.parts = d_parts;
+  a.nparts = P_n;
+  a.vchunk = vchunk;
+  a.tile = tile;
+  a.lights = d_lights;
+  a.plights = d_pl;
+  a.vchunk_prefix = d_vpre;
+  a.tile_prefix = d_tpre;
+  a.sblock_prefix = d_spre;
+  a.vclip = sc.take<float4>(sumV);
+  a.vcol = sc.take<float4>(sumV);
+  a.vpos = sc.take<float4>(sumV);
+  a.key = sc.take<float>(sumT);
+  a.sorted = sc.take<uint32_t>(sumT);
+  a.stats = sc.take<uint32_t>(4ull * P_n);
+  a.hist = sc.take<uint32_t>(sumHist);
+  a.bsum = sc.take<uint32_t>(sblocks);
+  a.out_color = reinterpret_cast<float4*>(oc.take<t3d_vertex>(3 * sumT));
+  a.out_depth = reinterpret_cast<float4*>(oc.take<t3d_vertex>(3 * sumT));
+  a.out_idx = oc.take<uint32_t>(3 * sumT);
+  a.fc = d_fc;
+  f->vertex_threads = vthreads;
+  f->total_vchunks = f->vchunk_prefix.back();
+  f->total_tiles = f->tile_prefix.back();
+  f->total_sblocks = sblocks;
+
+  HIP_TRY(c, hipMemcpyAsync(f->tables.ptr, f->pinned, host.off, hipMemcpyHostToDevice, c->stream));
+
+  if (4ull * P_n > f->pinned_stats_cap) {
+    if (f->pinned_stats) (void)hipHostFree(f->pinned_stats);
+    f->pinned_stats = nullptr;
+    f->pinned_stats_cap = align_up(4ull * P_n + 1024, 1024);
+    HIP_TRY(c, hipHostMalloc((void**)&f->pinned_stats, f->pinned_stats_cap * 4, hipHostMallocDefault));
+  }
+
+  // output bytes per survivor are known only after the cull; the profile
+  // attributes them to k_assemble after readback (below)
+  if (P_n) {
+    {
+      ProfScope ps(c, "k_init_stats", 0);
+      HIP_TRY(c, launch_init_stats(a, P_n, c->stream));
+    }
+    if (desc->exec_mode == T3D_EXEC_PLAIN) {
+      for (uint32_t i = 0; i < P_n; i++) {
+        st = run_chain(c, f, i, i + 1);
+        if (st) return st;
+      }
+    } else if (tu.use_graph && !c->profiling) {
+      // graph of the whole chain; reused while the launch shape is unchanged
+      std::vector<uint8_t> key(sizeof(FrameArgs) + 4 * sizeof(uint32_t));
+      uint32_t shape[4] = {f->total_vchunks, f->total_tiles, f->total_sblocks, vthreads};
+      std::memcpy(key.data(), &a, sizeof(FrameArgs));
+      std::memcpy(key.data() + sizeof(FrameArgs), shape, sizeof shape);
+      if (!f->graph_exec || key != f->graph_key) {
+        if (f->graph_exec) {
+          (void)hipGraphExecDestroy(f->graph_exec);
+          f->graph_exec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        st = run_chain(c, f, 0, P_n);
+        hipError_t ce = hipStreamEndCapture(c->stream, &g);
+        if (st) {
+          if (g) (void)hipGraphDestroy(g);
+          return st;
+        }
+        if (ce != hipSuccess) return fail(c, T3D_ERR_DEVICE, "graph capture: %s", hipGetErrorString(ce));
+        hipError_t ie = hipGraphInstantiate(&f->graph_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) return fail(c, T3D_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(ie));
+        f->graph_key = key;
+      }
+      HIP_TRY(c, hipGraphLaunch(f->graph_exec, c->stream));
+    } else {
+      st = run_chain(c, f, 0, P_n);
+      if (st) return st;
+    }
+    HIP_TRY(c, hipMemcpyAsync(f->pinned_stats, a.stats, 16ull * P_n, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < P_n; i++) {
+    f->pairs[i].survivor_count = f->pinned_stats[4 * i + 2];
+    total += f->pairs[i].survivor_count;
+  }
+  f->total_survivors = total;
+  if (c->profiling) {
+    auto& p = c->prof["k_assemble"];
+    p.bytes += 300ull * total;   // 3 x (48 B colour + 48 B depth + 4 B index) per drawn triangle
+  }
+  f->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+  return T3D_OK;
+}
+
+extern "C" {
+
+t3d_status t3d_frame_pairs(const t3d_frame* f, uint32_t* out_count, const t3d_pair_result** out_pairs) {
+  if (!f || !out_count || !out_pairs) return T3D_ERR_INVALID_ARGUMENT;
+  *out_count = (uint32_t)f->pairs.size();
+  *out_pairs = f->pairs.data();
+  return T3D_OK;
+}
+
+t3d_status t3d_frame_device_outputs(const t3d_frame* f, const t3d_vertex** cv, const t3d_vertex** dv, const uint32_t** idx) {
+  if (!f) return T3D_ERR_INVALID_ARGUMENT;
+  if (cv) *cv = reinterpret_cast<const t3d_vertex*>(f->args.out_color);
+  if (dv) *dv = reinterpret_cast<const t3d_vertex*>(f->args.out_depth);
+  if (idx) *idx = f->args.out_idx;
+  return T3D_OK;
+}
+
+double t3d_frame_last_ms(const t3d_frame* f) { return f ? f->last_ms : 0.0; }
+
+t3d_status t3d_frame_download(t3d_ctx* c, const t3d_frame* f, t3d_vertex* cv, t3d_vertex* dv, uint32_t* idx) {
+  if (!c || !f) return T3D_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(c->device);
+  uint64_t off = 0;
+  for (const auto& p : f->pairs) {
+    const uint64_t n = 3ull * p.survivor_count;
+    if (!n) continue;
+    if (cv) HIP_TRY(c, hipMemcpyAsync(cv + off, reinterpret_cast<const t3d_vertex*>(f->args.out_color) + p.vertex_offset, n * sizeof(t3d_vertex), hipMemcpyDeviceToHost, c->stream));
+    if (dv) HIP_TRY(c, hipMemcpyAsync(dv + off, reinterpret_cast<const t3d_vertex*>(f->args.out_depth) + p.vertex_offset, n * sizeof(t3d_vertex), hipMemcpyDeviceToHost, c->stream));
+    if (idx) HIP_TRY(c, hipMemcpyAsync(idx + off, f->args.out_idx + p.vertex_offset, n * 4, hipMemcpyDeviceToHost, c->stream));
+    off += n;
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return T3D_OK;
+}
+
+t3d_status t3d_frame_download_triangles(t3d_ctx* c, const t3d_frame* f, uint32_t* ids) {
+  if (!c || !f || !ids) return T3D_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(c->device);
+  uint64_t off = 0;
+  for (size_t i = 0; i < f->pairs.size(); i++) {
+    const auto& p = f->pairs[i];
+    if (!p.survivor_count) continue;
+    HIP_TRY(c, hipMemcpyAsync(ids + off, f->args.sorted + f->parts[i].tbase, 4ull * p.survivor_count, hipMemcpyDeviceToHost, c->stream));
+    off += p.survivor_count;
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  off = 0;
+  for (size_t i = 0; i < f->pairs.size(); i++) {
+    const auto& p = f->pairs[i];
+    for (uint32_t k = 0; k < p.survivor_count; k++) ids[off + k] += f->part_first_tri[i];
+    off += p.survivor_count;
+  }
+  return T3D_OK;
+}
+
+t3d_status t3d_profile_enable(t3d_ctx* c, int32_t on) {
+  if (!c) return T3D_ERR_INVALID_ARGUMENT;
+  c->profiling = on != 0;
+  return T3D_OK;
+}
+
+t3d_status t3d_profile_read(t3d_ctx* c, t3d_kernel_time* out, uint32_t max, uint32_t* count, int32_t reset) {
+  if (!c || !count) return T3D_ERR_INVALID_ARGUMENT;
+  uint32_t i = 0;
+  for (auto& kv : c->prof) {
+    if (out && i < max) {
+      std::memset(&out[i], 0, sizeof(t3d_kernel_time));
+      std::snprintf(out[i].name, sizeof out[i].name, "%s", kv.first.c_str());
+      out[i].total_ms = kv.second.ms;
+      out[i].launches = kv.second.launches;
+      out[i].algorithmic_bytes = kv.second.bytes;
+    }
+    i++;
+  }
+  *count = i;
+  if (reset) c->prof.clear();
+  return T3D_OK;
+}
+
+t3d_status t3d_get_tuning(const t3d_ctx* c, t3d_tuning* out) {
+  if (!c || !out) return T3D_ERR_INVALID_ARGUMENT;
+  *out = c->tuning;
+  return T3D_OK;
+}
+
+t3d_status t3d_set_tuning(t3d_ctx* c, const t3d_tuning* t) {
+  if (!c || !t) return T3D_ERR_INVALID_ARGUMENT;
+  if (t->vertex_block == 0 || t->vertex_block > 256 || t->vertex_block % 64)
+    return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: vertex_block must be 64, 128, 192 or 256");
+  if (t->vertices_per_thread == 0 || t->vertices_per_thread > 64)
+    return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: vertices_per_thread in [1, 64]");
+  if (t->triangle_tile < 64 || t->triangle_tile > (1u << 20) || t->triangle_tile % 64)
+    return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: triangle_tile must be a multiple of 64 in [64, 2^20]");
+  c->tuning = *t;
+  return T3D_OK;
+}
+
+t3d_status t3d_autotune(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc, uint32_t reps, t3d_tuning* out) {
+  if (!c || !f || !desc) return T3D_ERR_INVALID_ARGUMENT;
+  reps = std::max<uint32_t>(reps, 1);
+  const t3d_tuning base = c->tuning;
+  t3d_tuning best = base;
+  double best_ms = 1e30;
+  const uint32_t vblocks[] = {128, 256};
+  const uint32_t vpts[] = {1, 2, 4};
+  const uint32_t tiles[] = {512, 1024, 2048, 4096};
+  t3d_render_desc d = *desc;
+  d.exec_mode = T3D_EXEC_BATCHED;
+  for (uint32_t vb : vblocks)
+    for (uint32_t vpt : vpts)
+      for (uint32_t tl : tiles) {
+        t3d_tuning t = base;
+        t.vertex_block = vb;
+        t.vertices_per_thread = vpt;
+        t.triangle_tile = tl;
+        c->tuning = t;
+        t3d_status st = t3d_render(c, f, &d);   // warm: builds the graph for this shape
+        if (st) {
+          c->tuning = base;
+          return st;
+        }
+        double acc = 0;
+        for (uint32_t r = 0; r < reps; r++) {
+          st = t3d_render(c, f, &d);
+          if (st) {
+            c->tuning = base;
+            return st;
+          }
+          acc += f->last_ms;
+        }
+        acc /= reps;
+        if (acc < best_ms) {
+          best_ms = acc;
+          best = t;
+        }
+      }
+  c->tuning = best;
+  if (out) *out = best;
+  return T3D_OK;
+}
+
+t3d_status t3d_selftest_fp(t3d_ctx* c, const float* x, const float* y, float* out, uint32_t n) {
+  if (!c || !x || !y || !out || n == 0 || n > (1u << 20)) return fail(c, T3D_ERR_INVALID_ARGUMENT, "selftest_fp: bad arguments");
+  (void)hipSetDevice(c->device);
+  float *dx = nullptr, *dy = nullptr, *dout = nullptr;
+  HIP_TRY(c, hipMalloc(&dx, 4ull * n));
+  HIP_TRY(c, hipMalloc(&dy, 4ull * n));
+  HIP_TRY(c, hipMalloc(&dout, 16ull * n));
+  hipError_t e = hipMemcpyAsync(dx, x, 4ull * n, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dy, y, 4ull * n, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = launch_fp_selftest(dx, dy, dout, n, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout, 16ull * n, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(dx);
+  (void)hipFree(dy);
+  (void)hipFree(dout);
+  if (e != hipSuccess) return fail(c, T3D_ERR_DEVICE, "selftest_fp: %s", hipGetErrorString(e));
+  return T3D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,147 @@
+// t3d_internal.h — device-side layout of one frame of the geometry path.
+//
+// HBM layout (all arrays 256-byte aligned):
+//   mesh pool (resident, per mesh): pos[V]xyz f32, nrm[V]xyz f32, col[V]rgba f32,
+//     uv[V] f32x2, bones[V] u16x4, weights[V] f32x4, shape keys [K][V]xyz,
+//     idx[T] u32x3 — attribute-level SoA, so every per-vertex stream is a
+//     unit-stride coalesced load.
+//   frame tables (uploaded per render): PartGPU[P], LightGPU[L], PartLight[sum],
+//     block->part prefix arrays.
+//   frame scratch: vclip[sumV] f32x4, vcol[sumV] f32x4, vpos[sumV] f32x4 (skinned
+//     positions), key[sumT] f32, sorted[sumT] u32, stats[P] u32x4, hist, bsum.
+//   outputs: color[3*sumT] t3d_vertex, depth[3*sumT] t3d_vertex, idx[3*sumT] u32;
+//     render pair p owns vertices [out_base, out_base + 3*T_p) and fills the
+//     first 3*S_p of them.
+#pragma once
+
+#include <stdint.h>
+
+#include "t3d_math.h"
+
+namespace t3d {
+
+constexpr uint32_t kBins = 512;        // sortingTriangles.go:166-168
+constexpr uint32_t kMaxLights = 64;    // lights one part may see on the GPU path
+constexpr uint32_t kMaxLdsBones = 64;  // bone palettes staged in LDS up to this size
+constexpr uint32_t kScanBlock = 2048;  // elements per scan block (256 threads x 8)
+
+// PartGPU.flags
+enum : uint32_t {
+  PF_SKINNED = 1u << 0,
+  PF_BACKFACE = 1u << 1,
+  PF_LIGHTING = 1u << 2,
+  PF_HAS_COLOR = 1u << 3,
+  PF_HAS_UV = 1u << 4,
+  PF_SHAPE_KEYS = 1u << 5,
+  PF_SORT_SHIFT = 8,     // 2 bits: T3D_SORT_*
+  PF_LMODE_SHIFT = 10,   // 2 bits: T3D_LIGHTING_*
+  PF_LVMODE_SHIFT = 12,  // 2 bits: T3D_LV_*
+};
+
+// Per render pair: uniforms (camera.go:1507-1601, model.go:600-695) and the
+// pair's slices of the frame arrays.
+struct alignas(16) PartGPU {
+  M4 mvp;              // Model.Transform()·VP (model.go:695)
+  M4 rd;               // DirectionalLight working rotation: Transposed(Inverted(WorldRotation)) (light.go:326-330)
+  M4 model;            // Model.Transform() (LightVolume lookups, light.go:610-628)
+  float mp[4];         // mpColor = Model.Color ⊙ Material.Color (camera.go:1597-1601)
+  float icp[3];        // invertedCamPos (model.go:629-630)
+  uint32_t flags;
+  float src_w, src_h;  // texture size when the material textures (camera.go:1560-1566)
+  uint32_t light_base, light_count;
+  const float* pos;
+  const float* nrm;
+  const float* col;
+  const float* uv;
+  const uint16_t* bones;
+  const float* weights;
+  const float* palette;  // bone_count x 16 floats
+  const uint32_t* idx;   // this part's first triangle in the mesh index array
+  const float* sk_pos;
+  const float* sk_nrm;
+  const float* sk_w;     // per-frame shape key weights
+  uint32_t shape_keys, mesh_vcount;
+  uint32_t vmin, vcount;      // vertex range the part's triangles reference
+  uint32_t tcount, vbase;     // triangles; first slot in the vertex scratch
+  uint32_t tbase, ntiles;     // first slot in the triangle scratch; tiles of the part
+  uint32_t hbase, sblock0;    // histogram offset and first scan block (multi-tile parts)
+  uint32_t nsblocks, bone_count;
+  uint64_t out_base;          // first output vertex
+};
+
+// Frame-level light constants (light.go; beginRender results).
+struct alignas(16) LightGPU {
+  int32_t type;
+  float energy;
+  float range;
+  float range_sq;       // SetRange: range*range (light.go:273-276)
+  float color[4];
+  float amb[4];         // AmbientLight.result = color·energy (light.go:110-112)
+  float fwd[4];         // DirectionalLight.workingForward (light.go:322-324)
+  const float* cells;   // LightVolume cell colours, RGBA
+  uint32_t ncells;
+  int32_t cx, cy, cz;
+  float vmin[3], vmax[3], cell[3], offset[3], bias[3];
+  float pad[1];
+};
+
+// Per (render pair, light) values from beginModel and the per-model skip.
+struct alignas(16) PartLight {
+  float v[3];           // point: workingPosition; volume per-object mode: cell colour
+  uint32_t light;       // index into the frame's LightGPU array
+  uint32_t enabled;     // 0: the point-light distance skip removes it (camera.go:1625-1629)
+  uint32_t cell_found;  // volume per-object mode: a cell exists at the model position
+  uint32_t pad[2];
+};
+
+// Camera-level constants (camera.go:1329-1341, 1591-1606).
+struct FrameConst {
+  M4 vp;
+  float campos[4];
+  float near_, far_, cam_w, cam_h;
+  float half_w, half_h, dmp, spread;
+  float snap;
+  int32_t perspective, pc_tex, render_depth;
+  int32_t fog_active, fog_mode;
+  float fog_color[4];
+  float fog_range[2];
+};
+
+struct FrameArgs {
+  const PartGPU* parts;
+  uint32_t nparts;
+  uint32_t vchunk;              // vertices per vertex-kernel block
+  uint32_t tile;                // triangles per tile
+  uint32_t pad0;
+  const LightGPU* lights;
+  const PartLight* plights;
+  const uint32_t* vchunk_prefix;  // [P+1]
+  const uint32_t* tile_prefix;    // [P+1]
+  const uint32_t* sblock_prefix;  // [P+1]
+  float4* vclip;
+  float4* vcol;
+  float4* vpos;
+  float* key;
+  uint32_t* sorted;
+  uint32_t* stats;              // [P][4]: min bits, max bits, survivors, 0
+  uint32_t* hist;
+  uint32_t* bsum;
+  float4* out_color;            // t3d_vertex as 3 float4
+  float4* out_depth;
+  uint32_t* out_idx;
+  const FrameConst* fc;         // device copy, so graph kernel arguments stay fixed across frames
+};
+
+// Kernel launchers (t3d_kernels.hip).  blk0 offsets blockIdx so the plain
+// form can launch one part's blocks at a time.
+hipError_t launch_init_stats(const FrameArgs& a, uint32_t nparts, hipStream_t s);
+hipError_t launch_vertex(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t threads, hipStream_t s);
+hipError_t launch_cull(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s);
+hipError_t launch_hist(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s);
+hipError_t launch_scan(const FrameArgs& a, uint32_t sblk0, uint32_t nsblk, uint32_t part0, uint32_t nparts,
+                       hipStream_t s);
+hipError_t launch_scatter(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s);
+hipError_t launch_assemble(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s);
+hipError_t launch_fp_selftest(const float* a, const float* b, float* out, uint32_t n, hipStream_t s);
+
+}  // namespace t3d

@@ -1,0 +1,770 @@
+// t3d_kernels.hip — CDNA4 (gfx950) kernels of the per-frame geometry path.
+//
+// One frame is a chain of launches over all render pairs ("parts") at once;
+// every block works on exactly one part, found by binary search over a
+// per-part prefix of block counts, so per-part uniforms are wave-uniform
+// (scalar loads) and no block straddles two parts.
+//
+//   k_vertex    per vertex: shape keys, skinning or MVP transform, snapping,
+//               lighting (lights staged in LDS), vertex colour, fog
+//               (model.go:751-822, camera.go:1610-1644, 1740-1810)
+//   k_cull      per triangle: w clamp, z-range / screen / backface rejects,
+//               depth key, per-part survivor count and min/max depth
+//               (model.go:828-969)
+//   k_hist      per tile of a multi-tile part: 512-bin histogram
+//   k_scan*     exclusive scan of each part's [bin][tile] histogram matrix
+//   k_scatter   stable counting sort: output slot of every survivor
+//               (sortingTriangleBucket Sort + ForEach, sortingTriangles.go:38-125)
+//   k_assemble  per drawn triangle, in output order: ebiten colour/depth
+//               vertices and indices staged in LDS and written as full
+//               coalesced float4 rows (camera.go:1646-1820)
+//
+// All arithmetic is float32 in the reference's order; the build uses
+// -ffp-contract=off and correctly rounded division and square root.
+#include <hip/hip_runtime.h>
+
+#include "t3d_internal.h"
+
+namespace t3d {
+
+constexpr float kMaxF32 = 3.40282346638528859811704183484516925440e+38f;
+
+__device__ __forceinline__ uint32_t find_part(const uint32_t* __restrict__ prefix, uint32_t n, uint32_t b) {
+  // Largest p in [0, n) with prefix[p] <= b (parts with zero blocks are skipped).
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (prefix[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ float clampf(float v, float lo, float hi) {   // math32.go:38-45
+  if (v < lo) return lo;
+  else if (v > hi) return hi;
+  return v;
+}
+
+// Go's int(float32) on amd64: truncation; NaN / out of range give min int64.
+__device__ __forceinline__ int64_t go_int(float f) {
+  if (!(f >= -9.2233720368547758e18f && f < 9.2233720368547758e18f)) return INT64_MIN;
+  return (int64_t)f;
+}
+
+__device__ __forceinline__ V3 ld3(const float* __restrict__ p, uint32_t i) {
+  return v3(p[3 * (size_t)i + 0], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2]);
+}
+
+// ---------------------------------------------------------------------------
+// lighting
+// ---------------------------------------------------------------------------
+
+// LightVolume cell lookup: convertToXYZIndices + getLightCell (light.go:717-724, 1267-1272).
+__device__ __forceinline__ const float* volume_cell(const LightGPU& l, V3 p) {
+  int64_t x = go_int(roundf(fdiv(p.x - l.offset[0] - l.vmin[0] - fdiv(l.cell[0], 2.0f), l.cell[0])));
+  int64_t y = go_int(roundf(fdiv(p.y - l.offset[1] - l.vmin[1] - fdiv(l.cell[1], 2.0f), l.cell[1])));
+  int64_t z = go_int(roundf(fdiv(p.z - l.offset[2] - l.vmin[2] - fdiv(l.cell[2], 2.0f), l.cell[2])));
+  if (x < 0 || y < 0 || z < 0 || x >= l.cx || y >= l.cy || z >= l.cz) return nullptr;
+  int64_t i = x + (y * l.cx) + (z * l.cx * l.cy);
+  if (!l.cells || i >= (int64_t)l.ncells) return nullptr;
+  return l.cells + 4 * i;
+}
+
+// One light's VertexLight on one vertex (light.go:116-120, 191-258, 332-360,
+// 583-663), accumulating into rgb in the reference's order.
+__device__ __forceinline__ void light_vertex(const LightGPU& l, const PartLight& pl, const PartGPU& P, uint32_t flags,
+                                             V3 vp, V3 vn, float& r, float& g, float& b) {
+  const uint32_t lmode = (flags >> PF_LMODE_SHIFT) & 3u;
+  if (l.type == T3D_LIGHT_AMBIENT) {
+    r += l.amb[0];
+    g += l.amb[1];
+    b += l.amb[2];
+  } else if (l.type == T3D_LIGHT_POINT) {
+    V3 wp = v3(pl.v[0], pl.v[1], pl.v[2]);
+    float distance = dist_sq(wp, vp);
+    if (l.range > 0) {
+      if (distance > l.range_sq) return;
+    } else {
+      if (fdiv(1.0f, distance) * l.energy < 0.001f) return;
+    }
+    V3 lv = unit(sub(wp, vp));
+    V3 n = lmode == T3D_LIGHTING_FIXED_NORMALS ? lv : vn;
+    float diffuse = dot(n, lv);
+    if (lmode == T3D_LIGHTING_DOUBLE_SIDED) diffuse = fabsf(diffuse);
+    if (diffuse > 0) {
+      float df = diffuse * fdiv(1.0f, 1.0f + (0.1f * distance)) * 2.0f;
+      if (l.range_sq > 0) {
+        float dc = clampf(fdiv(l.range_sq - distance, distance), 0.0f, 1.0f);
+        df *= dc;
+      }
+      r += l.color[0] * df * l.energy;
+      g += l.color[1] * df * l.energy;
+      b += l.color[2] * df * l.energy;
+    }
+  } else if (l.type == T3D_LIGHT_DIRECTIONAL) {
+    V3 fwd = v3(l.fwd[0], l.fwd[1], l.fwd[2]);
+    V3 n;
+    if (lmode == T3D_LIGHTING_FIXED_NORMALS) n = fwd;
+    else if (flags & PF_SKINNED) n = vn;
+    else n = mult_vec(P.rd, vn);
+    float df = dot(n, fwd);
+    if (lmode == T3D_LIGHTING_DOUBLE_SIDED) df = fabsf(df);
+    if (df <= 0) return;
+    r += l.color[0] * df * l.energy;
+    g += l.color[1] * df * l.energy;
+    b += l.color[2] * df * l.energy;
+  } else if (l.type == T3D_LIGHT_VOLUME) {
+    const uint32_t mode = (flags >> PF_LVMODE_SHIFT) & 3u;
+    if (mode == T3D_LV_OFF) return;                                // LightVolume.Light, light.go:671-673
+    float c0 = 0, c1 = 0, c2 = 0;
+    if (mode == T3D_LV_PER_OBJECT) {
+      if (pl.cell_found) {
+        c0 = pl.v[0];
+        c1 = pl.v[1];
+        c2 = pl.v[2];
+      }
+    } else {
+      V3 halfc = v3(fdiv(l.cell[0], 2.0f), fdiv(l.cell[1], 2.0f), fdiv(l.cell[2], 2.0f));
+      V3 bias = v3(l.bias[0], l.bias[1], l.bias[2]);
+      V3 vpos = v3(0, 0, 0), vpn = v3(0, 0, 0);
+      if (flags & PF_SKINNED) {
+        if (mode == T3D_LV_PER_VERTEX_WITH_NORMAL) vpn = add(add(vpos, hmul(vn, halfc)), bias);
+        else vpos = add(vp, bias);
+      } else {
+        if (mode == T3D_LV_PER_VERTEX_WITH_NORMAL) vpn = add(mult_vec(P.model, add(vp, hmul(vn, halfc))), bias);
+        else vpos = add(mult_vec(P.model, vp), bias);
+      }
+      if (!(vpos.x >= l.vmin[0] && vpos.y >= l.vmin[1] && vpos.z >= l.vmin[2] && vpos.x <= l.vmax[0] &&
+            vpos.y <= l.vmax[1] && vpos.z <= l.vmax[2]))
+        return;                                                      // Dimensions.Contains, mesh.go:80-82
+      const float* cell = volume_cell(l, mode == T3D_LV_PER_VERTEX_WITH_NORMAL ? vpn : vpos);
+      if (cell) {
+        c0 = cell[0];
+        c1 = cell[1];
+        c2 = cell[2];
+      }
+    }
+    r += l.color[0] * c0 * l.energy;
+    g += l.color[1] * c1 * l.energy;
+    b += l.color[2] * c2 * l.energy;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_vertex
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
+  const uint32_t b = blk0 + blockIdx.x;
+  const uint32_t p = find_part(a.vchunk_prefix, a.nparts, b);
+  const PartGPU& P = a.parts[p];
+  const uint32_t flags = P.flags;
+
+  __shared__ LightGPU s_light[kMaxLights];
+  __shared__ PartLight s_pl[kMaxLights];
+  __shared__ M4 s_bones[kMaxLdsBones];
+
+  const uint32_t nl = (flags & PF_LIGHTING) ? P.light_count : 0u;
+  for (uint32_t i = threadIdx.x; i < nl; i += blockDim.x) {
+    s_pl[i] = a.plights[P.light_base + i];
+    s_light[i] = a.lights[s_pl[i].light];
+  }
+  const bool skinned = (flags & PF_SKINNED) != 0;
+  const bool lds_bones = skinned && P.bone_count <= kMaxLdsBones;
+  if (lds_bones) {
+    const float* src = P.palette;
+    float* dst = &s_bones[0].m[0][0];
+    for (uint32_t i = threadIdx.x; i < 16u * P.bone_count; i += blockDim.x) dst[i] = src[i];
+  }
+  __syncthreads();
+
+  const FrameConst& fc = *a.fc;
+  const uint32_t first = (b - a.vchunk_prefix[p]) * a.vchunk;
+  const uint32_t last = min(first + a.vchunk, P.vcount);
+  for (uint32_t i = first + threadIdx.x; i < last; i += blockDim.x) {
+    const uint32_t v = P.vmin + i;
+    // VertexPositionWithShapeKeys / VertexNormalWithShapeKeys (mesh.go:796-822)
+    V3 pos = ld3(P.pos, v);
+    V3 nrm = ld3(P.nrm, v);
+    if (flags & PF_SHAPE_KEYS) {
+      for (uint32_t k = 0; k < P.shape_keys; k++) {
+        const float w = P.sk_w[k];
+        const size_t o = ((size_t)k * P.mesh_vcount + v) * 3;
+        pos.x += P.sk_pos[o + 0] * w;
+        pos.y += P.sk_pos[o + 1] * w;
+        pos.z += P.sk_pos[o + 2] * w;
+      }
+      for (uint32_t k = 0; k < P.shape_keys; k++) {
+        const float w = P.sk_w[k];
+        const size_t o = ((size_t)k * P.mesh_vcount + v) * 3;
+        nrm.x += P.sk_nrm[o + 0] * w;
+        nrm.y += P.sk_nrm[o + 1] * w;
+        nrm.z += P.sk_nrm[o + 2] * w;
+      }
+    }
+    V3 vp, vn;
+    V4 clip;
+    if (skinned) {
+      // Model.skinVertex (model.go:559-596)
+      const ushort4 bi = reinterpret_cast<const ushort4*>(P.bones)[v];
+      const float4 bw = reinterpret_cast<const float4*>(P.weights)[v];
+      const uint16_t bidx[4] = {bi.x, bi.y, bi.z, bi.w};
+      const float bwt[4] = {bw.x, bw.y, bw.z, bw.w};
+      M4 S = zero4();
+      for (int k = 0; k < 4; k++) {
+        const float w = bwt[k];
+        if (w == 0) continue;
+        const M4& B = lds_bones ? s_bones[bidx[k]] : reinterpret_cast<const M4*>(P.palette)[bidx[k]];
+        if (w == 1) {
+          S = B;
+          break;
+        }
+        for (int r = 0; r < 4; r++)
+          for (int c = 0; c < 4; c++) S.m[r][c] += B.m[r][c] * w;
+      }
+      vp = mult_vec(S, pos);
+      S.m[3][0] = 0;
+      S.m[3][1] = 0;
+      S.m[3][2] = 0;
+      S.m[3][3] = 1;
+      vn = mult_vec(S, nrm);
+      clip = mult_vec_w(fc.vp, vp);                                 // model.go:772-775
+    } else {
+      vp = pos;
+      vn = nrm;
+      clip = mult_vec_w(P.mvp, pos);                                // model.go:795-798
+    }
+    if (fc.snap > 0) {                                              // model.go:807-812
+      clip.x = fdiv(roundf(clip.x * fc.snap), fc.snap);
+      clip.y = fdiv(roundf(clip.y * fc.snap), fc.snap);
+      clip.z = fdiv(roundf(clip.z * fc.snap), fc.snap);
+    }
+    // vertex colour (camera.go:1750-1768)
+    float cr, cg, cb, ca;
+    if (flags & PF_HAS_COLOR) {
+      const float4 vc = reinterpret_cast<const float4*>(P.col)[v];
+      cr = vc.x * P.mp[0];
+      cg = vc.y * P.mp[1];
+      cb = vc.z * P.mp[2];
+      ca = vc.w * P.mp[3];
+    } else {
+      cr = P.mp[0];
+      cg = P.mp[1];
+      cb = P.mp[2];
+      ca = P.mp[3];
+    }
+    if (flags & PF_LIGHTING) {
+      // vertexLights reset to (0,0,0,1) then every light in order (camera.go:1613-1641)
+      float lr = 0, lg = 0, lb = 0;
+      for (uint32_t k = 0; k < nl; k++) {
+        if (!s_pl[k].enabled) continue;
+        light_vertex(s_light[k], s_pl[k], P, flags, vp, vn, lr, lg, lb);
+      }
+      cr *= lr;
+      cg *= lg;
+      cb *= lb;
+    }
+    if (fc.fog_active) {                                            // camera.go:1786-1810
+      float depth = fdiv(clip.z + fc.dmp, fc.spread);
+      if (depth < 0) depth = 0;
+      else if (depth > 1) depth = 1;
+      depth = fc.fog_range[0] + ((fc.fog_range[1] - fc.fog_range[0]) * 1.0f - depth);
+      if (fc.fog_mode == T3D_FOG_ADD) {
+        cr += fc.fog_color[0] * depth;
+        cg += fc.fog_color[1] * depth;
+        cb += fc.fog_color[2] * depth;
+      } else if (fc.fog_mode == T3D_FOG_SUB) {
+        cr *= fc.fog_color[0] * depth;
+        cg *= fc.fog_color[1] * depth;
+        cb *= fc.fog_color[2] * depth;
+      }
+    }
+    const uint32_t o = P.vbase + i;
+    a.vclip[o] = make_float4(clip.x, clip.y, clip.z, clip.w);
+    a.vcol[o] = make_float4(cr, cg, cb, ca);
+    if (skinned) a.vpos[o] = make_float4(vp.x, vp.y, vp.z, 0.0f);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_cull
+// ---------------------------------------------------------------------------
+
+__global__ void k_init_stats(uint32_t* stats, uint32_t nparts) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nparts) {
+    stats[4 * i + 0] = __float_as_uint(kMaxF32);  // minDepth = MaxFloat32 (model.go:702)
+    stats[4 * i + 1] = 0u;                        // max over non-negative keys
+    stats[4 * i + 2] = 0u;
+    stats[4 * i + 3] = 0u;
+  }
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_cull(FrameArgs a, uint32_t blk0) {
+  const uint32_t b = blk0 + blockIdx.x;
+  const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
+  const PartGPU& P = a.parts[p];
+  const uint32_t flags = P.flags;
+  const FrameConst& fc = *a.fc;
+  const uint32_t t0 = (b - a.tile_prefix[p]) * a.tile;
+  const uint32_t t1 = min(t0 + a.tile, P.tcount);
+  const bool skinned = (flags & PF_SKINNED) != 0;
+  const bool backface = (flags & PF_BACKFACE) != 0;
+  const V3 camPos = v3(fc.campos[0], fc.campos[1], fc.campos[2]);
+  const V3 icp = v3(P.icp[0], P.icp[1], P.icp[2]);
+
+  float mn = kMaxF32, mx = 0.0f;
+  uint32_t cnt = 0;
+  for (uint32_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+    const uint32_t i0 = P.idx[3 * (size_t)t + 0], i1 = P.idx[3 * (size_t)t + 1], i2 = P.idx[3 * (size_t)t + 2];
+    const uint32_t vi[3] = {i0, i1, i2};
+    float nx[3], ny[3];
+    bool outOfBounds = true;
+    V3 stc = v3(0, 0, 0);
+    for (int k = 0; k < 3; k++) {                                   // model.go:855-886
+      const uint32_t o = P.vbase + (vi[k] - P.vmin);
+      const float4 c = a.vclip[o];
+      if (skinned) {
+        const float4 sp = a.vpos[o];
+        stc.x += sp.x;
+        stc.y += sp.y;
+        stc.z += sp.z;
+      }
+      float w = c.w;
+      if (w < 0) w = 0.000001f;
+      nx[k] = fdiv(c.x, w);
+      ny[k] = fdiv(c.y, w);
+      if (c.z + 1.0f >= fc.near_ && c.z < fc.far_) outOfBounds = false;
+    }
+    bool keep = !outOfBounds;
+    if (keep && ((nx[0] < -0.5f && nx[1] < -0.5f && nx[2] < -0.5f) ||      // model.go:890-898
+                 (nx[0] > 0.5f && nx[1] > 0.5f && nx[2] > 0.5f) ||
+                 (ny[0] < -0.5f && ny[1] < -0.5f && ny[2] < -0.5f) ||
+                 (ny[0] > 0.5f && ny[1] > 0.5f && ny[2] > 0.5f)))
+      keep = false;
+    if (keep && backface) {                                          // model.go:902-925
+      const float n0x = nx[0] - nx[1];
+      const float n0y = ny[0] - ny[1];
+      const float n1x = nx[1] - nx[2];
+      const float n1y = ny[1] - ny[2];
+      const float nor = (n0x * n1y) - (n1x * n0y);
+      if (nor < 0) keep = false;
+    }
+    float depth = -1.0f;
+    if (keep) {                                                       // model.go:941-960
+      if (skinned) {
+        const float dx = camPos.x - fdiv(stc.x, 3.0f);
+        const float dy = camPos.y - fdiv(stc.y, 3.0f);
+        const float dz = camPos.z - fdiv(stc.z, 3.0f);
+        depth = dx * dx + dy * dy + dz * dz;
+      } else {
+        // Triangle.Center from the base positions (mesh.go:1980-1982)
+        const V3 pa = ld3(P.pos, i0), pb = ld3(P.pos, i1), pc = ld3(P.pos, i2);
+        const float cx = fdiv(pa.x + pb.x + pc.x, 3.0f);
+        const float cy = fdiv(pa.y + pb.y + pc.y, 3.0f);
+        const float cz = fdiv(pa.z + pb.z + pc.z, 3.0f);
+        const float dx = icp.x - cx;
+        const float dy = icp.y - cy;
+        const float dz = icp.z - cz;
+        depth = dx * dx + dy * dy + dz * dz;
+      }
+      if (isnan(depth) || isinf(depth)) {
+        keep = false;
+        depth = -1.0f;
+      }
+    }
+    a.key[P.tbase + t] = depth;
+    if (keep) {
+      mn = fminf(mn, depth);
+      mx = fmaxf(mx, depth);
+      cnt++;
+    }
+  }
+  // block reduction, one atomic of each kind per block
+  __shared__ float s_mn[8], s_mx[8];
+  __shared__ uint32_t s_cnt[8];
+  mn = wave_min(mn);
+  mx = wave_max(mx);
+  cnt = wave_sum(cnt);
+  const uint32_t wv = threadIdx.x / warpSize, ln = threadIdx.x % warpSize;
+  if (ln == 0) {
+    s_mn[wv] = mn;
+    s_mx[wv] = mx;
+    s_cnt[wv] = cnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t nw = (blockDim.x + warpSize - 1) / warpSize;
+    for (uint32_t w = 1; w < nw; w++) {
+      mn = fminf(mn, s_mn[w]);
+      mx = fmaxf(mx, s_mx[w]);
+      cnt += s_cnt[w];
+    }
+    if (cnt) {
+      atomicMin(&a.stats[4 * p + 0], __float_as_uint(mn));   // keys are >= +0: uint order = float order
+      atomicMax(&a.stats[4 * p + 1], __float_as_uint(mx));
+      atomicAdd(&a.stats[4 * p + 2], cnt);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// sort: bins, histograms, scan, stable scatter
+// ---------------------------------------------------------------------------
+
+// Bin of a survivor in visit order: sortingTriangleBucket.Sort's
+// int(clamp(((d - min) / (max - min)) * 512, 0, 511)) (sortingTriangles.go:38-64),
+// reversed for BackToFront so ascending keys follow ForEach (sortingTriangles.go:93-125).
+__device__ __forceinline__ uint32_t visit_bin(float d, float mn, float rd, uint32_t sort_mode) {
+  int64_t bin = 0;
+  if (sort_mode != T3D_SORT_NONE) {
+    const float x = fdiv(d - mn, rd) * (float)kBins;
+    bin = go_int(clampf(x, 0.0f, (float)(kBins - 1)));
+  }
+  return sort_mode == T3D_SORT_BACK_TO_FRONT ? (kBins - 1) - (uint32_t)bin : (uint32_t)bin;
+}
+
+__device__ __forceinline__ void part_range(const uint32_t* stats, uint32_t p, float& mn, float& rd) {
+  mn = __uint_as_float(stats[4 * p + 0]);
+  const float mx = __uint_as_float(stats[4 * p + 1]);
+  rd = mx - mn;
+  if (rd == 0) rd += 0.001f;                                      // sortingTriangles.go:43-46
+}
+
+__global__ __launch_bounds__(64) void k_hist(FrameArgs a, uint32_t blk0) {
+  const uint32_t b = blk0 + blockIdx.x;
+  const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
+  const PartGPU& P = a.parts[p];
+  if (P.ntiles <= 1) return;
+  __shared__ uint32_t h[kBins];
+  for (uint32_t i = threadIdx.x; i < kBins; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  const uint32_t tl = b - a.tile_prefix[p];
+  const uint32_t t0 = tl * a.tile, t1 = min(t0 + a.tile, P.tcount);
+  float mn, rd;
+  part_range(a.stats, p, mn, rd);
+  const uint32_t mode = (P.flags >> PF_SORT_SHIFT) & 3u;
+  for (uint32_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+    const float d = a.key[P.tbase + t];
+    if (d >= 0) atomicAdd(&h[visit_bin(d, mn, rd, mode)], 1u);
+  }
+  __syncthreads();
+  uint32_t* col = a.hist + P.hbase + tl;
+  for (uint32_t i = threadIdx.x; i < kBins; i += blockDim.x) col[(size_t)i * P.ntiles] = h[i];
+}
+
+// Scan of each multi-tile part's histogram matrix, laid out bin-major
+// (element bin*ntiles + tile), which is exactly the output order.
+__device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_t* s_tmp, uint32_t& total) {
+  const uint32_t ln = threadIdx.x % warpSize, wv = threadIdx.x / warpSize;
+  uint32_t x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (ln >= (uint32_t)o) x += y;
+  }
+  if (ln == 63) s_tmp[wv] = x;
+  __syncthreads();
+  uint32_t wbase = 0, tot = 0;
+  for (uint32_t w = 0; w < blockDim.x / 64; w++) {
+    if (w < wv) wbase += s_tmp[w];
+    tot += s_tmp[w];
+  }
+  __syncthreads();
+  total = tot;
+  return wbase + x - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_reduce(FrameArgs a, uint32_t sblk0) {
+  const uint32_t b = sblk0 + blockIdx.x;
+  const uint32_t p = find_part(a.sblock_prefix, a.nparts, b);
+  const PartGPU& P = a.parts[p];
+  const uint32_t len = kBins * P.ntiles;
+  const uint32_t e0 = (b - a.sblock_prefix[p]) * kScanBlock;
+  uint32_t s = 0;
+  for (uint32_t e = e0 + threadIdx.x; e < min(e0 + kScanBlock, len); e += blockDim.x) s += a.hist[P.hbase + e];
+  __shared__ uint32_t s_tmp[8];
+  s = wave_sum(s);
+  if (threadIdx.x % warpSize == 0) s_tmp[threadIdx.x / warpSize] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (uint32_t w = 0; w < blockDim.x / warpSize; w++) t += s_tmp[w];
+    a.bsum[b] = t;
+  }
+}
+
+__global__ void k_scan_bsums(FrameArgs a, uint32_t part0, uint32_t nparts) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nparts) return;
+  const PartGPU& P = a.parts[part0 + i];
+  uint32_t run = 0;
+  for (uint32_t k = 0; k < P.nsblocks; k++) {
+    const uint32_t c = a.bsum[P.sblock0 + k];
+    a.bsum[P.sblock0 + k] = run;
+    run += c;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scan_down(FrameArgs a, uint32_t sblk0) {
+  const uint32_t b = sblk0 + blockIdx.x;
+  const uint32_t p = find_part(a.sblock_prefix, a.nparts, b);
+  const PartGPU& P = a.parts[p];
+  const uint32_t len = kBins * P.ntiles;
+  const uint32_t e0 = (b - a.sblock_prefix[p]) * kScanBlock;
+  __shared__ uint32_t s_tmp[8];
+  uint32_t base = a.bsum[b];
+  constexpr uint32_t per = kScanBlock / 256;
+  uint32_t v[per];
+  uint32_t local = 0;
+  const uint32_t my0 = e0 + threadIdx.x * per;
+  for (uint32_t k = 0; k < per; k++) {
+    const uint32_t e = my0 + k;
+    v[k] = e < len ? a.hist[P.hbase + e] : 0u;
+    local += v[k];
+  }
+  uint32_t total;
+  uint32_t ex = block_exclusive_scan_256(local, s_tmp, total) + base;
+  for (uint32_t k = 0; k < per; k++) {
+    const uint32_t e = my0 + k;
+    if (e < len) a.hist[P.hbase + e] = ex;
+    ex += v[k];
+  }
+}
+
+// Stable counting sort: one wavefront per tile walks its triangles in index
+// order, 64 at a time, ranking equal bins with ballots; run[bin] is the next
+// output slot of that bin (from the scanned histogram, or from a local scan
+// when the whole part is one tile).
+__global__ __launch_bounds__(64) void k_scatter(FrameArgs a, uint32_t blk0) {
+  const uint32_t b = blk0 + blockIdx.x;
+  const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
+  const PartGPU& P = a.parts[p];
+  const uint32_t tl = b - a.tile_prefix[p];
+  const uint32_t t0 = tl * a.tile, t1 = min(t0 + a.tile, P.tcount);
+  const uint32_t lane = threadIdx.x;
+  __shared__ uint32_t run[kBins];
+  float mn, rd;
+  part_range(a.stats, p, mn, rd);
+  const uint32_t mode = (P.flags >> PF_SORT_SHIFT) & 3u;
+  if (a.stats[4 * p + 2] == 0) return;
+
+  if (P.ntiles > 1) {
+    const uint32_t* col = a.hist + P.hbase + tl;
+    for (uint32_t i = lane; i < kBins; i += 64) run[i] = col[(size_t)i * P.ntiles];
+    __syncthreads();
+  } else {
+    for (uint32_t i = lane; i < kBins; i += 64) run[i] = 0;
+    __syncthreads();
+    for (uint32_t t = t0 + lane; t < t1; t += 64) {
+      const float d = a.key[P.tbase + t];
+      if (d >= 0) atomicAdd(&run[visit_bin(d, mn, rd, mode)], 1u);
+    }
+    __syncthreads();
+    // exclusive scan of 512 counters: 8 consecutive per lane
+    uint32_t c[8], s = 0;
+    for (int k = 0; k < 8; k++) {
+      c[k] = run[lane * 8 + k];
+      s += c[k];
+    }
+    uint32_t x = s;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if (lane >= (uint32_t)o) x += y;
+    }
+    uint32_t ex = x - s;
+    __syncthreads();
+    for (int k = 0; k < 8; k++) {
+      run[lane * 8 + k] = ex;
+      ex += c[k];
+    }
+    __syncthreads();
+  }
+
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (uint32_t base_t = t0; base_t < t1; base_t += 64) {
+    const uint32_t t = base_t + lane;
+    float d = -1.0f;
+    if (t < t1) d = a.key[P.tbase + t];
+    const bool valid = d >= 0;
+    const uint32_t bin = valid ? visit_bin(d, mn, rd, mode) : 0xFFFFFFFFu;
+    uint64_t todo = __ballot(valid);
+    uint64_t peers = 0;
+    while (todo) {
+      const int leader = __ffsll((unsigned long long)todo) - 1;
+      const uint32_t lb = __shfl(bin, leader);
+      const uint64_t m = __ballot(valid && bin == lb);
+      if (valid && bin == lb) peers = m;
+      todo &= ~m;
+    }
+    uint32_t slot = 0;
+    if (valid) slot = run[bin] + (uint32_t)__popcll(peers & lt_mask);
+    __syncthreads();
+    if (valid) {
+      a.sorted[P.tbase + slot] = t;
+      if ((peers & lt_mask) == 0) run[bin] = slot + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_assemble
+// ---------------------------------------------------------------------------
+
+constexpr uint32_t kAsmThreads = 256;
+constexpr uint32_t kFloatsPerTri = 36;   // 3 ebiten vertices of 12 floats
+
+__global__ __launch_bounds__(kAsmThreads) void k_assemble(FrameArgs a, uint32_t blk0) {
+  const uint32_t b = blk0 + blockIdx.x;
+  const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
+  const PartGPU& P = a.parts[p];
+  const uint32_t S = a.stats[4 * p + 2];
+  const uint32_t k0 = (b - a.tile_prefix[p]) * a.tile;
+  if (k0 >= S) return;
+  const uint32_t k1 = min(k0 + a.tile, S);
+  const FrameConst& fc = *a.fc;
+  const uint32_t wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+
+  __shared__ float4 s_out[kAsmThreads / 64][64 * kFloatsPerTri / 4];   // 9 KB per wave
+  float4* my = s_out[wv];
+  float4* out_c = a.out_color + P.out_base * 3;   // 3 float4 per vertex
+  float4* out_d = a.out_depth + P.out_base * 3;
+  const bool has_uv = (P.flags & PF_HAS_UV) != 0;
+
+  for (uint32_t kb = k0; kb < k1; kb += kAsmThreads) {
+    const uint32_t wk0 = kb + wv * 64;                  // this wave's first triangle
+    const uint32_t k = wk0 + lane;
+    const bool active = k < k1;
+    float4 cq[9], dq[9];
+    if (active) {
+      const uint32_t t = a.sorted[P.tbase + k];
+      for (int vi = 0; vi < 3; vi++) {
+        const uint32_t v = P.idx[3 * (size_t)t + vi];
+        const uint32_t o = P.vbase + (v - P.vmin);
+        const float4 c = a.vclip[o];
+        const float4 col = a.vcol[o];
+        float w = c.w;                                            // camera.go:1655-1667
+        if (!fc.perspective) w = 1.0f;
+        if (w < 0) w = 0.001f;
+        const float dx = fdiv(c.x, w) * fc.cam_w + fc.half_w;
+        const float dy = fdiv(c.y, -w) * fc.cam_h + fc.half_h;
+        float U = 0, V = 0;
+        if (has_uv) {
+          const float2 uv = reinterpret_cast<const float2*>(P.uv)[v];
+          U = uv.x;
+          V = uv.y;
+        }
+        float uvU, uvV, c0 = 0.0f;
+        if (fc.pc_tex) {                                          // camera.go:1688-1711
+          uvU = fdiv(U, w) * P.src_w;
+          uvV = fdiv(1.0f - V, w) * P.src_h;
+          c0 = fdiv(1.0f, w);
+        } else {
+          uvU = U * P.src_w;
+          uvV = (1.0f - V) * P.src_h;
+        }
+        float dr = 0, dg = 0, db = 0, da = 0;
+        if (fc.render_depth) {                                    // camera.go:1772-1784
+          float depth = fdiv(c.z + fc.dmp, fc.spread);
+          if (depth < 0) depth = 0;
+          else if (depth > 1) depth = 1;
+          dr = dg = db = depth;
+          da = 1.0f;
+        }
+        cq[3 * vi + 0] = make_float4(dx, dy, uvU, uvV);
+        cq[3 * vi + 1] = col;
+        cq[3 * vi + 2] = make_float4(c0, 0.0f, 0.0f, 0.0f);
+        dq[3 * vi + 0] = make_float4(dx, dy, uvU, uvV);
+        dq[3 * vi + 1] = make_float4(dr, dg, db, da);
+        dq[3 * vi + 2] = make_float4(c0, 0.0f, 0.0f, 0.0f);
+      }
+    }
+    const uint32_t wcount = wk0 < k1 ? min(64u, k1 - wk0) : 0u;   // triangles this wave writes
+    // colour vertices: registers -> LDS -> coalesced float4 rows
+    if (active)
+      for (int q = 0; q < 9; q++) my[lane * 9 + q] = cq[q];
+    __syncthreads();
+    for (uint32_t f = lane; f < wcount * 9; f += 64) out_c[(size_t)wk0 * 9 + f] = my[f];
+    __syncthreads();
+    if (active)
+      for (int q = 0; q < 9; q++) my[lane * 9 + q] = dq[q];
+    __syncthreads();
+    for (uint32_t f = lane; f < wcount * 9; f += 64) out_d[(size_t)wk0 * 9 + f] = my[f];
+    // indices: pair-local iota, as the reference's indexList (camera.go:1816)
+    uint32_t* oi = a.out_idx + P.out_base;
+    for (uint32_t f = lane; f < wcount * 3; f += 64) oi[(size_t)wk0 * 3 + f] = wk0 * 3 + f;
+    __syncthreads();
+  }
+}
+
+__global__ void k_fp_selftest(const float* x, const float* y, float* out, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = fdiv(x[i], y[i]);
+  out[n + i] = fsqrt(fabsf(x[i]));
+  out[2 * n + i] = x[i] * y[i] + x[i];
+  out[3 * n + i] = roundf(x[i]);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+hipError_t launch_init_stats(const FrameArgs& a, uint32_t nparts, hipStream_t s) {
+  if (!nparts) return hipSuccess;
+  hipLaunchKernelGGL(k_init_stats, dim3((nparts + 255) / 256), dim3(256), 0, s, a.stats, nparts);
+  return hipGetLastError();
+}
+hipError_t launch_vertex(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t threads, hipStream_t s) {
+  if (!nblk) return hipSuccess;
+  hipLaunchKernelGGL(k_vertex, dim3(nblk), dim3(threads), 0, s, a, blk0);
+  return hipGetLastError();
+}
+hipError_t launch_cull(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s) {
+  if (!nblk) return hipSuccess;
+  hipLaunchKernelGGL(k_cull, dim3(nblk), dim3(256), 0, s, a, blk0);
+  return hipGetLastError();
+}
+hipError_t launch_hist(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s) {
+  if (!nblk) return hipSuccess;
+  hipLaunchKernelGGL(k_hist, dim3(nblk), dim3(64), 0, s, a, blk0);
+  return hipGetLastError();
+}
+hipError_t launch_scan(const FrameArgs& a, uint32_t sblk0, uint32_t nsblk, uint32_t part0, uint32_t nparts,
+                       hipStream_t s) {
+  if (!nsblk) return hipSuccess;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nsblk), dim3(256), 0, s, a, sblk0);
+  hipLaunchKernelGGL(k_scan_bsums, dim3((nparts + 63) / 64), dim3(64), 0, s, a, part0, nparts);
+  hipLaunchKernelGGL(k_scan_down, dim3(nsblk), dim3(256), 0, s, a, sblk0);
+  return hipGetLastError();
+}
+hipError_t launch_scatter(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s) {
+  if (!nblk) return hipSuccess;
+  hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(64), 0, s, a, blk0);
+  return hipGetLastError();
+}
+hipError_t launch_assemble(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s) {
+  if (!nblk) return hipSuccess;
+  hipLaunchKernelGGL(k_assemble, dim3(nblk), dim3(kAsmThreads), 0, s, a, blk0);
+  return hipGetLastError();
+}
+hipError_t launch_fp_selftest(const float* x, const float* y, float* out, uint32_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_fp_selftest, dim3((n + 255) / 256), dim3(256), 0, s, x, y, out, n);
+  return hipGetLastError();
+}
+
+}  // namespace t3d

@@ -1,0 +1,489 @@
+"""Host-side mirror of the Tetra3D scene API that Camera.Render consumes.
+
+Names follow the reference (Node, Model, Mesh, MeshPart, Material, Camera,
+AmbientLight, PointLight, DirectionalLight, LightVolume, World, Scene) so a
+caller of the reference finds the same objects.  The scene graph stays on the
+host, as in the reference; Camera.render() turns it into the C-ABI
+descriptors of include/t3d.h and runs the geometry path on the GPU.
+
+Node.transform() restates node.go:572-608 (Scale·Rotation·Translate·parent)
+in float32 so world matrices are bit-identical to what the Go host computes.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi as A
+
+f32 = np.float32
+
+
+# ---------------------------------------------------------------------------
+# float32 matrix helpers (matrix.go), exact operation order
+# ---------------------------------------------------------------------------
+
+def identity():
+    return np.eye(4, dtype=f32)
+
+
+def mat_mult(a, b):
+    """Matrix4.Mult (matrix_none.go:6-30): ((a0·b0 + a1·b1) + a2·b2) + a3·b3 per element."""
+    a = np.asarray(a, dtype=f32)
+    b = np.asarray(b, dtype=f32)
+    return (((a[:, 0:1] * b[0:1, :] + a[:, 1:2] * b[1:2, :]) + a[:, 2:3] * b[2:3, :]) + a[:, 3:4] * b[3:4, :]).astype(f32)
+
+
+def mat_translate(x, y, z):
+    m = identity()
+    m[3, 0], m[3, 1], m[3, 2] = f32(x), f32(y), f32(z)
+    return m
+
+
+def mat_scale(x, y, z):
+    m = identity()
+    m[0, 0], m[1, 1], m[2, 2] = f32(x), f32(y), f32(z)
+    return m
+
+
+def quat_to_matrix(x, y, z, w):
+    """Quaternion.ToMatrix4 (quaternion.go:135-183): m1.Mult(m2)."""
+    x, y, z, w = f32(x), f32(y), f32(z), f32(w)
+    m1 = np.array([[w, z, -y, x], [-z, w, x, y], [y, -x, w, z], [-x, -y, -z, w]], dtype=f32)
+    m2 = np.array([[w, z, -y, -x], [-z, w, x, -y], [y, -x, w, -z], [x, y, z, w]], dtype=f32)
+    return mat_mult(m1, m2)
+
+
+def vec3_unit(v):
+    """Vector3.Unit (vector.go:258-266): divide by the float32 magnitude unless it is < 1e-8 or 1."""
+    v = np.asarray(v, dtype=f32)
+    l = f32(np.sqrt(np.float64(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])))
+    if l < f32(1e-8) or l == f32(1):
+        return v.copy()
+    return np.array([v[0] / l, v[1] / l, v[2] / l], dtype=f32)
+
+
+def rotation_from_forward(direction):
+    """A rotation Matrix4 whose row 2 (Forward) is the given unit direction."""
+    fwd = vec3_unit(direction)
+    up = np.array([0, 1, 0], dtype=f32)
+    if abs(float(np.dot(fwd, up))) > 0.99:
+        up = np.array([1, 0, 0], dtype=f32)
+    right = np.cross(up, fwd).astype(f32)
+    right = vec3_unit(right)
+    up2 = np.cross(fwd, right).astype(f32)
+    m = identity()
+    m[0, :3], m[1, :3], m[2, :3] = right, vec3_unit(up2), fwd
+    return m
+
+
+# ---------------------------------------------------------------------------
+# scene graph
+# ---------------------------------------------------------------------------
+
+class Node:
+    """Node (node.go:406-436): local position, scale, rotation and parent."""
+
+    def __init__(self, name=""):
+        self.name = name
+        self.position = np.zeros(3, dtype=f32)
+        self.scale = np.ones(3, dtype=f32)
+        self.rotation = identity()
+        self.parent = None
+        self.visible = True
+
+    def set_local_position(self, x, y, z):
+        self.position = np.array([x, y, z], dtype=f32)
+
+    def set_local_scale(self, x, y, z):
+        self.scale = np.array([x, y, z], dtype=f32)
+
+    def set_local_rotation(self, rot):
+        self.rotation = np.asarray(rot, dtype=f32).copy()
+
+    def transform(self):
+        """Node.Transform (node.go:572-608): Scale·Rotation·Translate·parent."""
+        t = mat_scale(*self.scale)
+        t = mat_mult(t, self.rotation)
+        t = mat_mult(t, mat_translate(*self.position))
+        if self.parent is not None:
+            t = mat_mult(t, self.parent.transform())
+        return t
+
+    def world_rotation(self):
+        """Node.WorldRotation (node.go:916-938) for parent chains without rotation."""
+        p = self.parent
+        while p is not None:
+            if not np.allclose(p.rotation, identity(), atol=1e-4):
+                t = self.transform()
+                r = identity()
+                for i in range(3):
+                    row = t[i, :3]
+                    r[i, :3] = vec3_unit(row)
+                    r[i, 3] = t[i, 3]
+                return r
+            p = p.parent
+        return self.rotation.copy()
+
+    def world_position(self):
+        return self.transform()[3, :3].copy()
+
+
+class Material:
+    """Material (material.go:66-131) with NewMaterial defaults (material.go:148-172)."""
+
+    def __init__(self, name=""):
+        self.name = name
+        self.color = np.array([1, 1, 1, 1], dtype=f32)
+        self.backface_culling = True
+        self.triangle_sort_mode = A.SORT_BACK_TO_FRONT
+        self.shadeless = False
+        self.fogless = False
+        self.lighting_mode = A.LIGHTING_DEFAULT
+        self.light_volume_shading_mode = A.LV_PER_VERTEX_WITHOUT_NORMAL
+        self.transparency_mode = A.TRANSPARENCY_AUTO
+        self.blend_source_over = True
+        self.render_order = 0
+        self.visible = True
+        self.texture_size = None  # (w, h) when a texture is set and UseTexture is on
+        self.billboard = False
+        self.custom_depth_function = False
+        self.depth_mode = 0
+
+    def desc(self):
+        d = A.MaterialDesc()
+        d.present = 1
+        d.visible = int(self.visible)
+        d.color = A.F4(*[float(c) for c in self.color])
+        d.backface_culling = int(self.backface_culling)
+        d.triangle_sort_mode = self.triangle_sort_mode
+        d.shadeless = int(self.shadeless)
+        d.fogless = int(self.fogless)
+        d.lighting_mode = self.lighting_mode
+        d.light_volume_shading_mode = self.light_volume_shading_mode
+        d.transparency_mode = self.transparency_mode
+        d.blend_source_over = int(self.blend_source_over)
+        d.render_order = self.render_order
+        if self.texture_size is not None:
+            d.use_texture = 1
+            d.texture_size = A.F2(float(self.texture_size[0]), float(self.texture_size[1]))
+        d.billboard = int(self.billboard)
+        d.custom_depth_function = int(self.custom_depth_function)
+        d.depth_mode = self.depth_mode
+        return d
+
+
+class MeshPart:
+    """MeshPart (mesh.go:2183-2190): a triangle range drawn with one material."""
+
+    def __init__(self, mesh, material, triangle_start, triangle_count):
+        self.mesh = mesh
+        self.material = material
+        self.triangle_start = triangle_start
+        self.triangle_count = triangle_count
+
+
+class Mesh:
+    """Mesh (mesh.go:260-302): SoA vertex arrays, triangles and parts."""
+
+    def __init__(self, name, positions, normals, indices, colors=None, uvs=None,
+                 bone_indices=None, bone_weights=None, parts=None, dimensions=None):
+        self.name = name
+        self.positions = np.ascontiguousarray(positions, dtype=f32).reshape(-1, 3)
+        self.normals = np.ascontiguousarray(normals, dtype=f32).reshape(-1, 3)
+        self.indices = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3)
+        self.colors = None if colors is None else np.ascontiguousarray(colors, dtype=f32).reshape(-1, 4)
+        self.uvs = None if uvs is None else np.ascontiguousarray(uvs, dtype=f32).reshape(-1, 2)
+        self.bone_indices = None if bone_indices is None else np.ascontiguousarray(bone_indices, dtype=np.uint16).reshape(-1, 4)
+        self.bone_weights = None if bone_weights is None else np.ascontiguousarray(bone_weights, dtype=f32).reshape(-1, 4)
+        self.shape_key_positions = None
+        self.shape_key_normals = None
+        if parts is None:
+            parts = [MeshPart(self, Material(name), 0, len(self.indices))]
+        self.mesh_parts = parts
+        if dimensions is None:
+            dimensions = (self.positions.min(axis=0), self.positions.max(axis=0))  # UpdateDimensions, mesh.go:729-764
+        self.dimensions = (np.asarray(dimensions[0], dtype=f32), np.asarray(dimensions[1], dtype=f32))
+        self._handles = {}
+
+    @property
+    def vertex_count(self):
+        return len(self.positions)
+
+    @property
+    def triangle_count(self):
+        return len(self.indices)
+
+    def desc(self):
+        """A t3d_mesh_desc pointing into this mesh's arrays (keep the mesh alive)."""
+        d = A.MeshDesc()
+        fp = C.POINTER(C.c_float)
+        d.vertex_count = self.vertex_count
+        d.positions = self.positions.ctypes.data_as(fp)
+        d.normals = self.normals.ctypes.data_as(fp)
+        if self.colors is not None:
+            d.colors = self.colors.ctypes.data_as(fp)
+        if self.uvs is not None:
+            d.uvs = self.uvs.ctypes.data_as(fp)
+        if self.bone_indices is not None:
+            d.bone_indices = self.bone_indices.ctypes.data_as(C.POINTER(C.c_uint16))
+            d.bone_weights = self.bone_weights.ctypes.data_as(fp)
+        if self.shape_key_positions is not None:
+            d.shape_key_count = len(self.shape_key_positions)
+            d.shape_key_positions = self.shape_key_positions.ctypes.data_as(fp)
+            d.shape_key_normals = self.shape_key_normals.ctypes.data_as(fp)
+        d.triangle_count = self.triangle_count
+        d.indices = self.indices.ctypes.data_as(C.POINTER(C.c_uint32))
+        parts = (A.PartRange * len(self.mesh_parts))()
+        for i, p in enumerate(self.mesh_parts):
+            parts[i].triangle_start = p.triangle_start
+            parts[i].triangle_count = p.triangle_count
+        self._parts_keepalive = parts
+        d.part_count = len(self.mesh_parts)
+        d.parts = parts
+        d.dimensions_min = A.F3(*[float(x) for x in self.dimensions[0]])
+        d.dimensions_max = A.F3(*[float(x) for x in self.dimensions[1]])
+        return d
+
+
+class Model(Node):
+    """Model (model.go:20-61) with NewModel defaults (model.go:66-90)."""
+
+    def __init__(self, name, mesh):
+        super().__init__(name)
+        self.mesh = mesh
+        self.color = np.array([1, 1, 1, 1], dtype=f32)
+        self.shadeless = False
+        self.frustum_culling = True
+        self.auto_batch_static = False
+        self.skinned = False
+        self.bone_influences = None  # (B, 4, 4) float32: boneInfluence per bone
+        self.shape_key_weights = None
+        self.light_group = None      # list of lights when a LightGroup is active
+
+    def desc(self, mesh_handle, keep):
+        d = A.ModelDesc()
+        d.mesh = mesh_handle
+        d.transform = A.mat4(self.transform())
+        d.world_rotation = A.mat4(self.world_rotation())
+        d.color = A.F4(*[float(c) for c in self.color])
+        d.visible = int(self.visible)
+        d.shadeless = int(self.shadeless)
+        d.frustum_culling = int(self.frustum_culling)
+        d.auto_batch_static = int(self.auto_batch_static)
+        d.skinned = int(self.skinned)
+        if self.skinned:
+            bones = np.ascontiguousarray(self.bone_influences, dtype=f32).reshape(-1, 4, 4)
+            keep.append(bones)
+            d.bone_count = len(bones)
+            d.bone_influences = bones.ctypes.data_as(C.POINTER(A.Mat4))
+        if self.shape_key_weights is not None:
+            w = np.ascontiguousarray(self.shape_key_weights, dtype=f32)
+            keep.append(w)
+            d.shape_key_weights = w.ctypes.data_as(C.POINTER(C.c_float))
+        mats = (A.MaterialDesc * len(self.mesh.mesh_parts))()
+        for i, p in enumerate(self.mesh.mesh_parts):
+            if p.material is not None:
+                mats[i] = p.material.desc()
+        keep.append(mats)
+        d.materials = mats
+        if self.light_group is not None:
+            lg = (A.LightDesc * max(1, len(self.light_group)))()
+            for i, l in enumerate(self.light_group):
+                lg[i] = l.desc()
+            keep.append(lg)
+            d.light_group_active = 1
+            d.light_group_count = len(self.light_group)
+            d.light_group = lg
+        return d
+
+
+class _Light(Node):
+    TYPE = -1
+
+    def __init__(self, name, r, g, b, energy):
+        super().__init__(name)
+        self.color = np.array([r, g, b], dtype=f32)
+        self.energy = f32(energy)
+
+    def desc(self):
+        d = A.LightDesc()
+        d.type = self.TYPE
+        d.visible = int(self.visible)
+        d.color = A.F3(*[float(c) for c in self.color])
+        d.energy = float(self.energy)
+        d.transform = A.mat4(self.transform())
+        d.world_rotation = A.mat4(self.world_rotation())
+        return d
+
+
+class AmbientLight(_Light):
+    """AmbientLight (light.go:80-132)."""
+    TYPE = A.LIGHT_AMBIENT
+
+
+class PointLight(_Light):
+    """PointLight (light.go:137-285); range 0 selects inverse-square falloff."""
+    TYPE = A.LIGHT_POINT
+
+    def __init__(self, name, r, g, b, energy):
+        super().__init__(name, r, g, b, energy)
+        self.range = f32(0)
+
+    def desc(self):
+        d = super().desc()
+        d.range = float(self.range)
+        return d
+
+
+class DirectionalLight(_Light):
+    """DirectionalLight (light.go:290-374): shines along its rotation's row 2."""
+    TYPE = A.LIGHT_DIRECTIONAL
+
+
+class LightVolume(_Light):
+    """LightVolume (light.go:376-680): a grid of cell colours."""
+    TYPE = A.LIGHT_VOLUME
+
+    def __init__(self, name, dims_min, dims_max, cell_size):
+        super().__init__(name, 1, 1, 1, 1)
+        self.dims_min = np.asarray(dims_min, dtype=f32)
+        self.dims_max = np.asarray(dims_max, dtype=f32)
+        self.cell_size = np.asarray(cell_size, dtype=f32)
+        self.cell_count = self._count_cells()
+        n = int(np.prod(self.cell_count))
+        self.cells = np.zeros((n, 4), dtype=f32)
+        self.cells[:, 3] = 1
+        self.offset = np.zeros(3, dtype=f32)
+        self.light_bias = np.zeros(3, dtype=f32)
+        self.handle = 0
+
+    def _count_cells(self):
+        """Cell counts as LightVolumeResize's float32 loops produce them (light.go:467-494)."""
+        counts = []
+        for a in range(3):
+            cs = f32(self.cell_size[a])
+            half = f32(cs / f32(2))
+            v = f32(self.dims_min[a] + half)
+            hi = f32(self.dims_max[a] - half)
+            n = 0
+            while v <= hi:
+                n += 1
+                v = f32(v + cs)
+            counts.append(n)
+        return tuple(counts)
+
+    def desc(self):
+        d = super().desc()
+        d.volume = self.handle
+        d.volume_min = A.F3(*[float(x) for x in self.dims_min])
+        d.volume_max = A.F3(*[float(x) for x in self.dims_max])
+        d.cell_size = A.F3(*[float(x) for x in self.cell_size])
+        d.cell_count = A.I3(*self.cell_count)
+        d.offset = A.F3(*[float(x) for x in self.offset])
+        d.light_bias = A.F3(*[float(x) for x in self.light_bias])
+        return d
+
+
+class World:
+    """World (world.go:25-63); the reference's default ambient energy is 0."""
+
+    def __init__(self, name="world"):
+        self.name = name
+        self.lighting_on = True
+        self.fog_on = True
+        self.fog_mode = A.FOG_ADD
+        self.fog_color = np.zeros(4, dtype=f32)
+        self.fog_range = np.array([0, 1], dtype=f32)
+        self.ambient_light = AmbientLight("ambient light", 1, 1, 1, 0)
+
+    def desc(self):
+        d = A.WorldDesc()
+        d.present = 1
+        d.lighting_on = int(self.lighting_on)
+        d.fog_on = int(self.fog_on)
+        d.fog_mode = self.fog_mode
+        d.fog_color = A.F4(*[float(x) for x in self.fog_color])
+        d.fog_range = A.F2(*[float(x) for x in self.fog_range])
+        a = self.ambient_light
+        d.ambient_color = A.F3(*[float(x) for x in a.color])
+        d.ambient_energy = float(a.energy)
+        d.ambient_visible = int(a.visible)
+        return d
+
+
+class Scene:
+    """Scene (scene.go:12-31): a World plus the node lists handed to Render."""
+
+    def __init__(self, name="scene", world=None):
+        self.name = name
+        self.world = world
+
+
+class Camera(Node):
+    """Camera (camera.go:256-321) with NewCamera defaults (camera.go:324-547)."""
+
+    def __init__(self, name, w, h):
+        super().__init__(name)
+        self.width, self.height = int(w), int(h)
+        self.perspective = True
+        self.fov = f32(60)
+        self.near = f32(0.1)
+        self.far = f32(100)
+        self.ortho_scale = f32(20)
+        self.render_depth = True
+        self.render_normals = False
+        self.perspective_corrected_texture_mapping = False
+        self.depth_margin = f32(0.04)
+        self.vertex_snapping = f32(0)
+        self.max_light_count = 0
+        self.max_light_count_focus = None
+
+    def desc(self):
+        d = A.CameraDesc()
+        d.width, d.height = self.width, self.height
+        d.perspective = int(self.perspective)
+        d.fov = float(self.fov)
+        d.near = float(self.near)
+        d.far = float(self.far)
+        d.ortho_scale = float(self.ortho_scale)
+        d.transform = A.mat4(self.transform())
+        d.world_rotation = A.mat4(self.world_rotation())
+        d.render_depth = int(self.render_depth)
+        d.render_normals = int(self.render_normals)
+        d.perspective_corrected_texture_mapping = int(self.perspective_corrected_texture_mapping)
+        d.depth_margin = float(self.depth_margin)
+        d.vertex_snapping = float(self.vertex_snapping)
+        d.max_light_count = int(self.max_light_count)
+        if self.max_light_count_focus is not None:
+            d.has_max_light_count_focus = 1
+            d.max_light_count_focus = A.F3(*[float(x) for x in self.max_light_count_focus])
+        return d
+
+
+class RenderInputs:
+    """The descriptors of one Camera.Render(scene, lights, models) call.
+
+    `mesh_handle` maps a Mesh to the handle the executing side registered it
+    under; `volume_handle` does the same for LightVolumes.  The object keeps
+    every array the descriptors point into alive.
+    """
+
+    def __init__(self, camera, scene, lights, models, mesh_handle, exec_mode=A.EXEC_BATCHED):
+        self.keep = []
+        self.cam = camera.desc()
+        self.world = scene.world.desc() if (scene is not None and scene.world is not None) else A.WorldDesc()
+        self.lights = (A.LightDesc * max(1, len(lights)))()
+        for i, l in enumerate(lights):
+            self.lights[i] = l.desc()
+        self.models = (A.ModelDesc * max(1, len(models)))()
+        for i, m in enumerate(models):
+            h = mesh_handle(m.mesh) if m.mesh is not None else 0
+            self.models[i] = m.desc(h, self.keep)
+        self.render = A.RenderDesc()
+        self.render.camera = C.pointer(self.cam)
+        self.render.world = C.pointer(self.world)
+        self.render.light_count = len(lights)
+        self.render.lights = self.lights
+        self.render.model_count = len(models)
+        self.render.models = self.models
+        self.render.exec_mode = exec_mode

@@ -1,0 +1,471 @@
+"""Seeded synthetic scenes for the geometry path.
+
+One PRNG (SplitMix64, seed 0x7e7a3d unless given) drives every random choice;
+floats come from the top 24 bits of each draw.  The same arrays feed the GPU
+path and the oracle, so both see byte-identical inputs.
+
+Scenes (sizes as in the measurement plan):
+  icosphere_scene  — 642-vertex shared-edge icosphere, 640x360, 3 lights
+  cubes_scene      — 4,096 unit cubes in a 200^3 box, 1920x1080, 10 lights
+  heightfield_scene— 1024x1024 heightfield, camera inside the terrain, 18 lights
+  mixed_scene      — 512 models, V log-uniform in [24, 65536], 128 skinned
+  serving_pool     — 10,000 models of the mixed distribution + a Poisson job trace
+"""
+import math
+
+import numpy as np
+
+from . import _abi as A
+from .scene import (AmbientLight, Camera, DirectionalLight, LightVolume, Material, Mesh, MeshPart, Model,
+                    PointLight, Scene, World, identity, mat_mult, mat_scale, mat_translate, quat_to_matrix,
+                    rotation_from_forward)
+
+f32 = np.float32
+DEFAULT_SEED = 0x7E7A3D
+_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+class SplitMix64:
+    """SplitMix64: output k (1-based) mixes seed + k*0x9E3779B97F4A7C15."""
+
+    def __init__(self, seed=DEFAULT_SEED):
+        self.seed = np.uint64(seed & 0xFFFFFFFFFFFFFFFF)
+        self.counter = 0
+
+    def u64(self, n):
+        k = np.arange(self.counter + 1, self.counter + n + 1, dtype=np.uint64)
+        self.counter += n
+        with np.errstate(over="ignore"):
+            z = self.seed + k * _GAMMA
+            z = (z ^ (z >> np.uint64(30))) * _M1
+            z = (z ^ (z >> np.uint64(27))) * _M2
+            z = z ^ (z >> np.uint64(31))
+        return z
+
+    def floats(self, n):
+        """n floats in [0, 1) from the top 24 bits."""
+        return ((self.u64(n) >> np.uint64(40)).astype(np.float64) / float(1 << 24)).astype(f32)
+
+    def uniform(self, lo, hi, n):
+        return (np.float64(lo) + (np.float64(hi) - np.float64(lo)) * self.floats(n).astype(np.float64)).astype(f32)
+
+    def ints(self, lo, hi, n):
+        """n integers in [lo, hi)."""
+        return (lo + (self.u64(n) % np.uint64(hi - lo)).astype(np.int64)).astype(np.int64)
+
+    def child(self, tag):
+        """An independent stream derived from this one's seed and a tag."""
+        return SplitMix64(int(self.seed) ^ ((tag * 0xD1B54A32D192ED03) & 0xFFFFFFFFFFFFFFFF))
+
+
+def uniform_quaternions(rng, n):
+    """Shoemake's uniform random unit quaternions, as (x, y, z, w) float32."""
+    u = rng.floats(3 * n).astype(np.float64).reshape(n, 3)
+    a, b = np.sqrt(1 - u[:, 0]), np.sqrt(u[:, 0])
+    t1, t2 = 2 * np.pi * u[:, 1], 2 * np.pi * u[:, 2]
+    return np.stack([a * np.sin(t1), a * np.cos(t1), b * np.sin(t2), b * np.cos(t2)], axis=1).astype(f32)
+
+
+def unit_vectors(rng, n):
+    u = rng.floats(2 * n).astype(np.float64).reshape(n, 2)
+    z = 2 * u[:, 0] - 1
+    phi = 2 * np.pi * u[:, 1]
+    r = np.sqrt(np.maximum(0, 1 - z * z))
+    return np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1).astype(f32)
+
+
+def axis_angle_matrix(axis, angle):
+    """Rotation Matrix4 about a unit axis (the NewMatrix4Rotate form, matrix.go:101-128)."""
+    x, y, z = (float(a) for a in axis)
+    s, c = math.sin(angle), math.cos(angle)
+    m = 1 - c
+    r = np.eye(4, dtype=np.float64)
+    r[0, 0] = m * x * x + c
+    r[0, 1] = m * x * y + z * s
+    r[0, 2] = m * z * x - y * s
+    r[1, 0] = m * x * y - z * s
+    r[1, 1] = m * y * y + c
+    r[1, 2] = m * y * z + x * s
+    r[2, 0] = m * z * x + y * s
+    r[2, 1] = m * y * z - x * s
+    r[2, 2] = m * z * z + c
+    return r.astype(f32)
+
+
+# ---------------------------------------------------------------------------
+# meshes
+# ---------------------------------------------------------------------------
+
+def icosphere(level, radius=1.0, name="icosphere", color=None):
+    """Shared-edge icosphere: 10*4^level + 2 vertices, 20*4^level triangles,
+    smooth normals, outward counter-clockwise winding (front faces survive the
+    screen-space backface test)."""
+    t = (1 + math.sqrt(5)) / 2
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(level):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    p = np.array(verts)
+    idx = np.array(faces, dtype=np.int64)
+    n = np.cross(p[idx[:, 1]] - p[idx[:, 0]], p[idx[:, 2]] - p[idx[:, 0]])
+    flip = np.einsum("ij,ij->i", n, p[idx].sum(axis=1)) < 0
+    idx[flip] = idx[flip][:, [0, 2, 1]]
+    pos = (p * radius).astype(f32)
+    nrm = p.astype(f32)
+    cols = None
+    if color is not None:
+        cols = np.tile(np.asarray(color, dtype=f32), (len(pos), 1))
+    uv = np.stack([0.5 + np.arctan2(p[:, 2], p[:, 0]) / (2 * np.pi), 0.5 + np.arcsin(np.clip(p[:, 1], -1, 1)) / np.pi],
+                  axis=1).astype(f32)
+    return Mesh(name, pos, nrm, idx.astype(np.uint32), colors=cols, uvs=uv)
+
+
+def cube_mesh(w=1.0, h=1.0, d=1.0):
+    """The 24-vertex / 12-triangle cube of NewCubeMesh (mesh.go:1251-1342),
+    with AutoNormal's last-writer face normals (mesh.go:755-764)."""
+    w, h, d = f32(w) / f32(2), f32(h) / f32(2), f32(d) / f32(2)
+    v = [(-w, h, -d, 0, 0), (w, h, d, 1, 1), (w, h, -d, 1, 0), (-w, h, d, 0, 1),
+         (w, -h, -d, 1, 0), (w, -h, d, 1, 1), (-w, -h, -d, 0, 0), (-w, -h, d, 0, 1),
+         (-w, h, d, 0, 0), (w, -h, d, 1, 1), (w, h, d, 1, 0), (-w, -h, d, 0, 1),
+         (w, h, -d, 1, 0), (w, -h, -d, 1, 1), (-w, h, -d, 0, 0), (-w, -h, -d, 0, 1),
+         (w, h, -d, 1, 0), (w, h, d, 1, 1), (w, -h, -d, 0, 0), (w, -h, d, 0, 1),
+         (-w, -h, -d, 0, 0), (-w, h, d, 1, 1), (-w, h, -d, 1, 0), (-w, -h, d, 0, 1)]
+    idx = np.array([0, 1, 2, 3, 1, 0, 4, 5, 6, 6, 5, 7, 8, 9, 10, 11, 9, 8, 12, 13, 14, 14, 13, 15,
+                    16, 17, 18, 18, 17, 19, 20, 21, 22, 23, 21, 20], dtype=np.uint32).reshape(-1, 3)
+    pos = np.array([x[:3] for x in v], dtype=f32)
+    uv = np.array([x[3:] for x in v], dtype=f32)
+    nrm = np.zeros_like(pos)
+    for a, b, c in idx:                       # calculateNormal (mesh.go:2171-2179), Cross (vector.go:164-180)
+        v0 = pos[b] - pos[a]
+        v1 = pos[c] - pos[b]
+        cr = np.array([v0[1] * v1[2] - v1[1] * v0[2], v0[2] * v1[0] - v1[2] * v0[0], v0[0] * v1[1] - v1[0] * v0[1]],
+                      dtype=f32)
+        l = f32(np.sqrt(np.float64(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2])))
+        if not (l < f32(1e-8) or l == f32(1)):
+            cr = (cr / l).astype(f32)
+        for k in (a, b, c):
+            nrm[k] = cr
+    mesh = Mesh("Cube", pos, nrm, idx, uvs=uv)
+    mesh.mesh_parts[0].material = Material("Cube")
+    return mesh
+
+
+def value_noise(rng, xs, zs, cell, octaves=3):
+    """Smooth value noise in [0, 1) on a grid of sample coordinates."""
+    total = np.zeros((len(zs), len(xs)), dtype=np.float64)
+    amp, norm = 1.0, 0.0
+    for o in range(octaves):
+        c = cell / (2 ** o)
+        gx = xs / c
+        gz = zs / c
+        nx, nz = int(np.floor(gx.max())) + 2, int(np.floor(gz.max())) + 2
+        lattice = rng.floats(nx * nz).astype(np.float64).reshape(nz, nx)
+        ix, iz = np.floor(gx).astype(np.int64), np.floor(gz).astype(np.int64)
+        fx, fz = gx - ix, gz - iz
+        sx, sz = fx * fx * (3 - 2 * fx), fz * fz * (3 - 2 * fz)
+        a = lattice[np.ix_(iz, ix)]
+        b = lattice[np.ix_(iz, ix + 1)]
+        cc = lattice[np.ix_(iz + 1, ix)]
+        d = lattice[np.ix_(iz + 1, ix + 1)]
+        top = a + (b - a) * sx[None, :]
+        bot = cc + (d - cc) * sx[None, :]
+        total += amp * (top + (bot - top) * sz[:, None])
+        norm += amp
+        amp *= 0.5
+    return total / norm
+
+
+def grid_mesh(rng, nx, nz, spacing, amplitude, name="grid", colors=True, noise_cell=None):
+    """nx*nz heightfield grid with value-noise heights, finite-difference
+    normals and (optionally) random vertex colours; faces point +Y."""
+    xs = np.arange(nx, dtype=np.float64) * spacing
+    zs = np.arange(nz, dtype=np.float64) * spacing
+    cell = noise_cell if noise_cell is not None else max(spacing * 4, (nx * spacing) / 4)
+    h = value_noise(rng, xs, zs, cell) * amplitude
+    hx = np.gradient(h, spacing, axis=1)
+    hz = np.gradient(h, spacing, axis=0)
+    n = np.stack([-hx, np.ones_like(h), -hz], axis=-1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    X, Z = np.meshgrid(xs, zs)
+    pos = np.stack([X, h, Z], axis=-1).reshape(-1, 3).astype(f32)
+    nrm = n.reshape(-1, 3).astype(f32)
+    uv = np.stack([X / max(xs[-1], 1e-9), Z / max(zs[-1], 1e-9)], axis=-1).reshape(-1, 2).astype(f32)
+    j, i = np.meshgrid(np.arange(nz - 1), np.arange(nx - 1), indexing="ij")
+    v00 = (j * nx + i).ravel()
+    v10 = (j * nx + i + 1).ravel()
+    v01 = ((j + 1) * nx + i).ravel()
+    v11 = ((j + 1) * nx + i + 1).ravel()
+    # (x0,z0),(x0,z1),(x1,z0) and (x1,z0),(x0,z1),(x1,z1): (b-a)x(c-a) points +Y
+    tris = np.empty((len(v00) * 2, 3), dtype=np.uint32)
+    tris[0::2] = np.stack([v00, v01, v10], axis=1)
+    tris[1::2] = np.stack([v10, v01, v11], axis=1)
+    cols = None
+    if colors:
+        cols = np.ones((nx * nz, 4), dtype=f32)
+        cols[:, :3] = rng.floats(nx * nz * 3).reshape(-1, 3)
+    return Mesh(name, pos, nrm, tris, colors=cols, uvs=uv)
+
+
+# ---------------------------------------------------------------------------
+# scenes
+# ---------------------------------------------------------------------------
+
+class Workload:
+    """One Camera.Render call's inputs: camera, scene, light list, model list."""
+
+    def __init__(self, name, camera, scene, lights, models):
+        self.name = name
+        self.camera = camera
+        self.scene = scene
+        self.lights = lights
+        self.models = models
+        seen, meshes = set(), []
+        for m in models:
+            if m.mesh is not None and id(m.mesh) not in seen:
+                seen.add(id(m.mesh))
+                meshes.append(m.mesh)
+        self.meshes = meshes
+        self.volumes = [l for l in lights if isinstance(l, LightVolume)]
+
+    @property
+    def triangle_count(self):
+        return int(sum(m.mesh.triangle_count for m in self.models if m.mesh is not None))
+
+    @property
+    def vertex_count(self):
+        return int(sum(m.mesh.vertex_count for m in self.models if m.mesh is not None))
+
+
+def _world(ambient):
+    w = World()
+    w.ambient_light.energy = f32(ambient)
+    return w
+
+
+def _point(name, pos, rng_range, energy=1.0, color=(1, 1, 1)):
+    p = PointLight(name, *color, energy)
+    p.set_local_position(*pos)
+    p.range = f32(rng_range)
+    return p
+
+
+def _directional(name, direction, energy=1.0, color=(1, 1, 1)):
+    d = DirectionalLight(name, *color, energy)
+    d.set_local_rotation(rotation_from_forward(direction))
+    return d
+
+
+def icosphere_scene(seed=DEFAULT_SEED, level=3):
+    """Single model: 642-vertex icosphere at the origin with a seeded rotation,
+    white vertex colour; 640x360 camera at z=+4, FOV 75; ambient 0.2, one
+    directional light, one point light at (2,2,2) with range 10."""
+    rng = SplitMix64(seed)
+    mesh = icosphere(level, 1.0, color=(1, 1, 1, 1))
+    model = Model("icosphere", mesh)
+    q = uniform_quaternions(rng.child(1), 1)[0]
+    model.set_local_rotation(quat_to_matrix(*q))
+    cam = Camera("camera", 640, 360)
+    cam.fov = f32(75)
+    cam.set_local_position(0, 0, 4)
+    d = _directional("sun", unit_vectors(rng.child(2), 1)[0])
+    p = _point("point", (2, 2, 2), 10)
+    return Workload("icosphere_642v", cam, Scene("icosphere", _world(0.2)), [d, p], [model])
+
+
+def cubes_scene(seed=DEFAULT_SEED, count=4096, box=200.0):
+    """4,096 shared unit cubes: positions uniform in a 200^3 box, uniform
+    rotations, isotropic scale in [0.5, 2]; 1920x1080 camera at the centre of
+    the +Z face looking -Z, FOV 60, far 400; ambient 0.2, one directional and
+    8 point lights (range 30)."""
+    rng = SplitMix64(seed)
+    mesh = cube_mesh(1, 1, 1)
+    half = box / 2
+    pos = rng.child(1).uniform(-half, half, 3 * count).reshape(count, 3)
+    quats = uniform_quaternions(rng.child(2), count)
+    scales = rng.child(3).uniform(0.5, 2.0, count)
+    models = []
+    for i in range(count):
+        m = Model("cube%d" % i, mesh)
+        m.set_local_position(*pos[i])
+        m.set_local_rotation(quat_to_matrix(*quats[i]))
+        m.set_local_scale(scales[i], scales[i], scales[i])
+        models.append(m)
+    cam = Camera("camera", 1920, 1080)
+    cam.fov = f32(60)
+    cam.far = f32(400)
+    cam.set_local_position(0, 0, half)
+    lights = [_directional("sun", unit_vectors(rng.child(4), 1)[0])]
+    lp = rng.child(5).uniform(-half, half, 24).reshape(8, 3)
+    lights += [_point("point%d" % i, lp[i], 30) for i in range(8)]
+    return Workload("cubes_4096", cam, Scene("cubes", _world(0.2)), lights, models)
+
+
+def heightfield_scene(seed=DEFAULT_SEED, n=1024, amplitude=20.0):
+    """One 1024x1024-vertex heightfield (2,093,058 triangles, one MeshPart):
+    value-noise heights with amplitude 20, random vertex colours,
+    finite-difference normals; camera inside the terrain's bounding box at
+    height 5; 16 point lights (range 15-60), one directional light and one
+    LightVolume (cell 32) over the bounding box."""
+    rng = SplitMix64(seed)
+    mesh = grid_mesh(rng.child(1), n, n, 1.0, amplitude, name="heightfield", noise_cell=64.0)
+    model = Model("terrain", mesh)
+    cam = Camera("camera", 1920, 1080)
+    cam.fov = f32(75)
+    cam.far = f32(1000)
+    c = (n - 1) / 2.0
+    cam.set_local_position(c, 5.0, c)
+    yaw = float(rng.child(2).floats(1)[0]) * 2 * np.pi
+    cam.set_local_rotation(axis_angle_matrix((0, 1, 0), yaw))
+    lr = rng.child(3)
+    lights = [_directional("sun", unit_vectors(lr, 1)[0])]
+    lp = np.stack([lr.uniform(0, n - 1, 16), lr.uniform(0, 25, 16), lr.uniform(0, n - 1, 16)], axis=1)
+    ranges = lr.uniform(15, 60, 16)
+    lights += [_point("point%d" % i, lp[i], ranges[i]) for i in range(16)]
+    dmin, dmax = mesh.dimensions
+    vol = LightVolume("volume", dmin, dmax, (32, 32, 32))
+    vol.cells[:, :3] = lr.floats(len(vol.cells) * 3).reshape(-1, 3)
+    lights.append(vol)
+    return Workload("heightfield_1024", cam, Scene("terrain", _world(0.2)), lights, [model])
+
+
+def _mixed_mesh(rng, target_v, kind, idx):
+    if kind == "sphere":
+        level = int(np.clip(round(math.log(max(target_v - 2, 10) / 10.0, 4)), 1, 6))
+        return icosphere(level, 4.0, name="sphere%d" % idx, color=(1, 1, 1, 1))
+    nx = max(2, int(round(math.sqrt(target_v))))
+    nz = max(2, int(round(target_v / nx)))
+    return grid_mesh(rng, nx, nz, 8.0 / (nx - 1), 1.5, name="grid%d" % idx)
+
+
+def _skin(model, rng, bones=32, max_angle_deg=30.0):
+    """32-bone armature: 4 influences per vertex with Dirichlet(1) weights
+    (normalised in float32), bone matrices = random rotation of at most 30
+    degrees composed with the model's placement (identity inverse-bind)."""
+    mesh = model.mesh
+    V = mesh.vertex_count
+    keys = rng.floats(V * bones).reshape(V, bones)
+    bidx = np.argsort(keys, axis=1)[:, :4].astype(np.uint16)
+    e = -np.log(np.maximum(rng.floats(V * 4).astype(np.float64), 1e-12)).reshape(V, 4)
+    w = (e / e.sum(axis=1, keepdims=True)).astype(f32)
+    mesh.bone_indices = np.ascontiguousarray(bidx)
+    mesh.bone_weights = np.ascontiguousarray(w)
+    axes = unit_vectors(rng, bones)
+    angles = rng.floats(bones).astype(np.float64) * math.radians(max_angle_deg)
+    place = model.transform()
+    pal = np.stack([mat_mult(axis_angle_matrix(axes[k], angles[k]), place) for k in range(bones)])
+    model.skinned = True
+    model.bone_influences = pal.astype(f32)
+
+
+def mixed_models(rng, count, skinned_every=4, spread=(60.0, 60.0, 30.0, 250.0), vmin=24, vmax=65536):
+    """`count` models with vertex counts log-uniform in [vmin, vmax]: grids
+    (with random vertex colours) and, every fourth model, shared-size
+    icospheres; one model in `skinned_every` is skinned."""
+    sx, sy, z0, z1 = spread
+    lv = rng.child(11).floats(count).astype(np.float64)
+    targets = np.exp(np.log(vmin) + (np.log(vmax) - np.log(vmin)) * lv)
+    pos = np.stack([rng.child(12).uniform(-sx, sx, count), rng.child(13).uniform(-sy, sy, count),
+                    rng.child(14).uniform(-z1, -z0, count)], axis=1)
+    quats = uniform_quaternions(rng.child(15), count)
+    models = []
+    sphere_cache = {}
+    mesh_rng = rng.child(16)
+    skin_rng = rng.child(17)
+    for i in range(count):
+        kind = "sphere" if i % 4 == 3 else "grid"
+        skinned = (i % skinned_every) == 1
+        if kind == "sphere" and not skinned:
+            level = int(np.clip(round(math.log(max(targets[i] - 2, 10) / 10.0, 4)), 1, 6))
+            if level not in sphere_cache:
+                sphere_cache[level] = _mixed_mesh(mesh_rng, targets[i], kind, i)
+            mesh = sphere_cache[level]
+        else:
+            mesh = _mixed_mesh(mesh_rng, targets[i], kind, i)
+        m = Model("model%d" % i, mesh)
+        m.set_local_position(*pos[i])
+        m.set_local_rotation(quat_to_matrix(*quats[i]))
+        if skinned:
+            _skin(m, skin_rng)
+        models.append(m)
+    return models
+
+
+def mixed_scene(seed=DEFAULT_SEED, count=512):
+    """512 models of uneven size (V log-uniform in [24, 65536]), 128 skinned
+    with 32-bone armatures; ambient 0.2, 8 point lights and one directional
+    light; 1920x1080 camera at the origin looking -Z, FOV 60, far 400."""
+    rng = SplitMix64(seed)
+    models = mixed_models(rng, count)
+    cam = Camera("camera", 1920, 1080)
+    cam.fov = f32(60)
+    cam.far = f32(400)
+    lr = rng.child(21)
+    lights = [_directional("sun", unit_vectors(lr, 1)[0])]
+    lp = np.stack([lr.uniform(-60, 60, 8), lr.uniform(-60, 60, 8), lr.uniform(-250, -30, 8)], axis=1)
+    lights += [_point("point%d" % i, lp[i], 80) for i in range(8)]
+    return Workload("mixed_512", cam, Scene("mixed", _world(0.2)), lights, models)
+
+
+class Job:
+    """One serving request: a camera, a subset of the pool and a light list."""
+
+    def __init__(self, arrival_s, camera, model_ids, lights):
+        self.arrival_s = arrival_s
+        self.camera = camera
+        self.model_ids = model_ids
+        self.lights = lights
+
+
+def serving_trace(seed=DEFAULT_SEED, pool_size=10000, jobs=2000, rate_hz=2000.0, max_models=200, max_lights=16):
+    """A resident pool of `pool_size` models (the mixed distribution) and a
+    Poisson job trace at `rate_hz`: each job has a seeded camera (640x360 to
+    1920x1080), 1-`max_models` models and 1-`max_lights` lights."""
+    rng = SplitMix64(seed)
+    models = mixed_models(rng.child(31), pool_size)
+    jr = rng.child(32)
+    gaps = -np.log(np.maximum(jr.floats(jobs).astype(np.float64), 1e-12)) / rate_hz
+    arrivals = np.cumsum(gaps)
+    res = [(640, 360), (1280, 720), (1600, 900), (1920, 1080)]
+    out = []
+    for j in range(jobs):
+        w, h = res[int(jr.ints(0, len(res), 1)[0])]
+        cam = Camera("camera%d" % j, w, h)
+        cam.fov = f32(jr.uniform(50, 80, 1)[0])
+        cam.far = f32(400)
+        n = int(jr.ints(1, max_models + 1, 1)[0])
+        ids = np.sort(jr.ints(0, pool_size, n))
+        nl = int(jr.ints(1, max_lights + 1, 1)[0])
+        lights = [_directional("sun%d" % j, unit_vectors(jr, 1)[0])]
+        for k in range(nl - 1):
+            p = (jr.uniform(-60, 60, 1)[0], jr.uniform(-60, 60, 1)[0], jr.uniform(-250, -30, 1)[0])
+            lights.append(_point("p%d_%d" % (j, k), p, 80))
+        out.append(Job(float(arrivals[j]), cam, ids, lights))
+    return models, out
+
+
+SCENES = {
+    "icosphere": icosphere_scene,
+    "cubes": cubes_scene,
+    "heightfield": heightfield_scene,
+    "mixed": mixed_scene,
+}
