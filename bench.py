@@ -1,0 +1,231 @@
+"""Benchmark of the geometry path (Camera.Render: transform, skinning, lighting,
+cull, 512-bin depth sort, ebiten vertex assembly) on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload mixed]
+
+A step is one Camera.Render frame of the workload, in the batched form (one
+launch chain for all render pairs, replayed from a HIP graph), with all
+meshes, bone palettes and light volumes resident in HBM before timing.
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+METRIC = "tris/sec (xform+light+cull+sort), % HBM peak; p99 frame latency vs plain"
+
+WORKLOADS = {
+    "mixed": ("mixed_512", "512 models, vertex counts log-uniform in [24, 65536] (grids and icospheres), "
+              "128 skinned with 32-bone 4-influence armatures, ambient + 1 directional + 8 point lights, "
+              "1920x1080, per-part 512-bin back-to-front sort"),
+    "cubes": ("cubes_4096", "4,096 unit cubes in a 200^3 box, ambient + 1 directional + 8 point lights, 1920x1080"),
+    "heightfield": ("heightfield_1024", "1024x1024 heightfield (2,093,058 triangles), camera inside the terrain, "
+                    "16 point + 1 directional + 1 volume light"),
+    "icosphere": ("icosphere_642v", "642-vertex icosphere, 640x360, ambient + directional + point light"),
+}
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def frame_bytes(prof, frames):
+    """Algorithmic bytes per frame: mesh streams + indices + uniforms + outputs."""
+    tot = 0
+    for name, (ms, launches, b) in prof.items():
+        tot += b
+    return tot / max(frames, 1)
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
+    or None when no summary exists for this workload."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload)
+    try:
+        with open(path) as fh:
+            d = json.load(fh)
+        return d["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def cpu_baseline(wl, budget_s):
+    """The oracle (single-threaded restatement of the reference CPU path) on
+    whole frames of the same workload, for about budget_s seconds."""
+    from oracle.oracle import Oracle
+    o = Oracle()
+    for i, m in enumerate(wl.meshes):
+        o.register_mesh(m, 1 + i)
+    for i, v in enumerate(wl.volumes):
+        o.register_volume(v, 100000 + i)
+    saved = [v.handle for v in wl.volumes]
+    for i, v in enumerate(wl.volumes):
+        v.handle = 100000 + i
+    frames, t0 = 0, time.perf_counter()
+    while True:
+        o.render(wl.camera, wl.scene, wl.lights, wl.models)
+        frames += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or frames >= 20:
+            break
+    for v, h in zip(wl.volumes, saved):
+        v.handle = h
+    o.close()
+    return frames, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="mixed", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run the plain device copy of BYTES (counter calibration) and exit")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the geometry path runs on a single MI355X; "
+              "multi-GPU sharding is a design note, not a measured configuration); got --gpus %d" % args.gpus,
+              file=sys.stderr)
+        return 2
+
+    from tetra3d_amd import _abi as A
+    from tetra3d_amd import workloads as W
+    from tetra3d_amd.runtime import Context
+
+    if args.copy_calibration:
+        with Context(0) as ctx:
+            ms = ctx.selftest_copy(args.copy_calibration, reps=20)
+        print(json.dumps({"copy_bytes": args.copy_calibration, "kernel_ms": ms,
+                          "GBps": 2 * args.copy_calibration / (ms / 1e3) / 1e9}), flush=True)
+        return 0
+
+    wname, wdesc = WORKLOADS[args.workload]
+    t = time.perf_counter()
+    wl = W.SCENES[args.workload]()
+    log("workload %s: %d models, %d vertices, %d triangles (generated in %.1fs)"
+        % (wname, len(wl.models), wl.vertex_count, wl.triangle_count, time.perf_counter() - t))
+
+    ctx = Context(0)
+    ctx.prepare(wl)
+    inp = ctx.inputs(wl.camera, wl.scene, wl.lights, wl.models, A.EXEC_BATCHED)
+    tuning = None
+    if not args.no_autotune:
+        t = time.perf_counter()
+        tu = ctx.autotune(inp, reps=3)
+        tuning = {"vertex_block": tu.vertex_block, "vertices_per_thread": tu.vertices_per_thread,
+                  "triangle_tile": tu.triangle_tile, "emit_variant": tu.emit_variant, "store_mode": tu.store_mode}
+        log("autotune picked %s in %.1fs" % (tuning, time.perf_counter() - t))
+
+    for _ in range(args.warmup):
+        ctx.render_inputs(inp)
+    pairs = ctx.pairs()
+    survivors = int(sum(p[3] for p in pairs))
+    tris = wl.triangle_count
+    log("pairs %d, survivors %d of %d triangles" % (len(pairs), survivors, tris))
+
+    # timed region: K frames, batched form from the HIP graph
+    per = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        a = time.perf_counter()
+        ctx.render_inputs(inp)       # synchronous: returns when the frame's outputs are in HBM
+        per.append((time.perf_counter() - a) * 1e3)
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    value = tris / (ms / 1e3)
+    p50, p99 = float(np.percentile(per, 50)), float(np.percentile(per, 99))
+    log("batched: %.3f ms/frame, %.3g tris/s, p50 %.3f p99 %.3f ms" % (ms, value, p50, p99))
+
+    # per-kernel durations with HIP events on the library's stream
+    ctx.profile(True)
+    ctx.profile_read(reset=True)
+    for _ in range(args.profile_steps):
+        ctx.render_inputs(inp)
+    prof = ctx.profile_read(reset=True)
+    ctx.profile(False)
+    host_ms = prof.pop("host:prologue", (0.0, 1, 0))[0] / args.profile_steps
+    kern = {k: {"avg_ms": v[0] / max(v[1], 1), "launches_per_frame": v[1] / args.profile_steps,
+                "algorithmic_bytes_per_launch": v[2] / max(v[1], 1)} for k, v in prof.items()}
+    dom = max(kern, key=lambda k: kern[k]["avg_ms"] * kern[k]["launches_per_frame"])
+    dk = kern[dom]
+    achieved = dk["algorithmic_bytes_per_launch"] / (dk["avg_ms"] / 1e3) / 1e9
+    B = frame_bytes(prof, args.profile_steps)
+    device_ms = sum(v[0] for v in prof.values()) / args.profile_steps
+    log("kernels: " + ", ".join("%s %.3fms" % (k, v["avg_ms"]) for k, v in sorted(kern.items())))
+
+    # plain per-pair form of the same frame (one kernel chain per render pair)
+    plain = None
+    if args.plain_steps > 0:
+        pinp = ctx.inputs(wl.camera, wl.scene, wl.lights, wl.models, A.EXEC_PLAIN)
+        ctx.render_inputs(pinp)
+        pper = []
+        for _ in range(args.plain_steps):
+            a = time.perf_counter()
+            ctx.render_inputs(pinp)
+            pper.append((time.perf_counter() - a) * 1e3)
+        plain_ms = float(np.mean(pper))
+        plain_value = tris / (plain_ms / 1e3)
+        plain = {"value": plain_value, "ms_per_step": plain_ms, "p99_ms": float(np.percentile(pper, 99)),
+                 "speedup_batched_vs_plain": value / plain_value}
+        log("plain: %.3f ms/frame, %.3g tris/s" % (plain_ms, plain_value))
+
+    cpu = None
+    if not args.no_cpu:
+        frames, el = cpu_baseline(wl, args.cpu_seconds)
+        cpu = {"value": tris * frames / el, "unit": "tris/s", "cores": 1, "kind": "port",
+               "sample": "%d whole frame(s) of %s on the oracle (C restatement of camera.go/model.go/light.go/"
+                         "sortingTriangles.go), single-threaded, %.1f s" % (frames, wname, el),
+               "host_cpu": platform.processor() or platform.machine(), "nproc": os.cpu_count()}
+        log("cpu baseline: %.3g tris/s (%d frames in %.1fs)" % (cpu["value"], frames, el))
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "tris/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded SplitMix64 scene, seed 0x7e7a3d)",
+        "config": {"workload": wname, "description": wdesc, "triangles_per_frame": tris,
+                   "vertices_per_frame": wl.vertex_count, "render_pairs": len(pairs), "drawn_triangles": survivors,
+                   "exec": "batched, HIP graph", "tuning": tuning},
+        "latency_ms": {"p50": p50, "p99": p99, "mean": ms},
+        "hbm": {"algorithmic_bytes_per_frame": B, "frame_pct_of_peak": 100.0 * B / (ms / 1e3) / (HBM_PEAK_GBS * 1e9),
+                "device_ms_per_frame": device_ms, "host_prologue_ms": host_ms,
+                "device_pct_of_peak": 100.0 * B / (device_ms / 1e3) / (HBM_PEAK_GBS * 1e9)},
+        "plain": plain,
+        "kernels": kern,
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(wname, dom),
+                     "algorithmic_bytes_per_launch": dk["algorithmic_bytes_per_launch"], "avg_ms": dk["avg_ms"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

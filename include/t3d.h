@@ -312,9 +312,13 @@ t3d_status t3d_profile_read(t3d_ctx* ctx, t3d_kernel_time* per_kernel, uint32_t 
 typedef struct {
   uint32_t vertex_block;             /* threads per block, vertex kernel */
   uint32_t vertices_per_thread;
-  uint32_t triangle_tile;            /* triangles per tile in the cull/sort kernels */
+  uint32_t triangle_tile;            /* triangles per tile in the cull/sort kernels (64..2048) */
   uint32_t small_part_triangles;     /* parts up to this size take the fused one-block path */
   uint32_t use_graph;                /* replay the batched chain from a HIP graph */
+  uint32_t emit_variant;             /* 0: fused sort+assembly, LDS-staged runs; 1: fused, direct
+                                        stores; 2: sort, then assembly in output order */
+  uint32_t store_mode;               /* assembly stores (variant 2): bit 0 = direct per-lane stores
+                                        instead of LDS-staged rows, bit 1 = non-temporal stores */
 } t3d_tuning;
 t3d_status t3d_get_tuning(const t3d_ctx* ctx, t3d_tuning* out);
 t3d_status t3d_set_tuning(t3d_ctx* ctx, const t3d_tuning* tuning);
@@ -330,6 +334,10 @@ t3d_status t3d_autotune(t3d_ctx* ctx, t3d_frame* frame, const t3d_render_desc* d
  * round-half-away(x).  Lets a test confirm the device rounds exactly as the
  * host does.  Host pointers; n <= 1<<20. */
 t3d_status t3d_selftest_fp(t3d_ctx* ctx, const float* x, const float* y, float* out, uint32_t n);
+/* Copies `bytes` (a multiple of 16) between two device buffers `reps` times
+ * with a plain float4 kernel and returns the mean kernel time in *out_ms:
+ * the reference point that HBM byte counters are scaled against. */
+t3d_status t3d_selftest_copy(t3d_ctx* ctx, uint64_t bytes, uint32_t reps, double* out_ms);
 /* sizeof of each public struct, for binding checks: fills up to max entries
  * in the order vertex, mesh_desc, material_desc, light_desc, model_desc,
  * camera_desc, world_desc, render_desc, pair_result, kernel_time, tuning. */

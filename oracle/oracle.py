@@ -104,6 +104,7 @@ class Oracle:
         self.L = lib()
         self.ctx = self.L.t3do_create()
         self._handles = {}
+        self._refs = []   # registered meshes stay alive so their ids are not reused
 
     def close(self):
         if self.ctx:
@@ -119,6 +120,7 @@ class Oracle:
         if rc != 0:
             raise RuntimeError("oracle mesh register: %s" % self.L.t3do_last_error(self.ctx).decode())
         self._handles[id(mesh)] = handle
+        self._refs.append(mesh)
 
     def register_volume(self, vol, handle):
         cells = np.ascontiguousarray(vol.cells, dtype=np.float32)

@@ -16,12 +16,19 @@ SCENES = {
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("variant", [(0, 0), (1, 0), (2, 0), (2, 1), (2, 2), (2, 3)],
+                         ids=["fused_staged", "fused_direct", "split_staged", "split_direct", "split_staged_nt",
+                              "split_direct_nt"])
 @pytest.mark.parametrize("mode", [A.EXEC_BATCHED, A.EXEC_PLAIN], ids=["batched", "plain"])
 @pytest.mark.parametrize("scene", list(SCENES))
-def test_scene_parity(gpu_ctx, oracle, scene, mode):
+def test_scene_parity(gpu_ctx, oracle, scene, mode, variant):
     wl = SCENES[scene]()
-    print("parity", scene, "mode", mode, "tris", wl.triangle_count, flush=True)
-    g = gpu_ctx.render_workload(wl, exec_mode=mode)
+    print("parity", scene, "mode", mode, "variant", variant, "tris", wl.triangle_count, flush=True)
+    gpu_ctx.set_tuning(emit_variant=variant[0], store_mode=variant[1])
+    try:
+        g = gpu_ctx.render_workload(wl, exec_mode=mode)
+    finally:
+        gpu_ctx.set_tuning(emit_variant=2, store_mode=2)
     o = oracle.render_workload(wl)
     s = compare_frames(g, o)
     print(s, flush=True)

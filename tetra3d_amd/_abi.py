@@ -218,6 +218,8 @@ class Tuning(C.Structure):
         ("triangle_tile", C.c_uint32),
         ("small_part_triangles", C.c_uint32),
         ("use_graph", C.c_uint32),
+        ("emit_variant", C.c_uint32),
+        ("store_mode", C.c_uint32),
     ]
 
 

@@ -194,7 +194,7 @@ struct t3d_frame {
   std::vector<t3d_pair_result> pairs;
   std::vector<PartGPU> parts;
   std::vector<uint32_t> part_first_tri;  // mesh triangle_start per pair
-  std::vector<uint64_t> bytes_vertex, bytes_cull;  // algorithmic bytes per part
+  std::vector<uint64_t> bytes_vertex, bytes_cull, bytes_uv;  // algorithmic bytes per part
   uint32_t nparts = 0;
   uint32_t total_vchunks = 0, total_tiles = 0, total_sblocks = 0;
   std::vector<uint32_t> vchunk_prefix, tile_prefix, sblock_prefix;
@@ -214,7 +214,7 @@ struct t3d_ctx {
   std::map<uint64_t, std::unique_ptr<Mesh>> meshes;
   std::map<uint64_t, std::unique_ptr<Volume>> volumes;
   std::string err;
-  t3d_tuning tuning{256, 1, 1024, 0, 1};
+  t3d_tuning tuning{256, 1, 1024, 0, 1, 2, 2};
   // profiling
   bool profiling = false;
   struct Prof {
@@ -689,12 +689,13 @@ struct ProfScope {
 t3d_status run_chain(t3d_ctx* c, t3d_frame* f, uint32_t p0, uint32_t p1) {
   const FrameArgs& a = f->args;
   struct {
-    uint64_t vertex = 0, cull = 0;
+    uint64_t vertex = 0, cull = 0, uv = 0;
   } tr;
   if (c->profiling)
     for (uint32_t i = p0; i < p1; i++) {
       tr.vertex += f->bytes_vertex[i];
       tr.cull += f->bytes_cull[i];
+      tr.uv += f->bytes_uv[i];
     }
   const uint32_t vb0 = f->vchunk_prefix[p0], vb1 = f->vchunk_prefix[p1];
   const uint32_t tb0 = f->tile_prefix[p0], tb1 = f->tile_prefix[p1];
@@ -715,13 +716,14 @@ t3d_status run_chain(t3d_ctx* c, t3d_frame* f, uint32_t p0, uint32_t p1) {
     ProfScope ps(c, "k_scan", 0);
     HIP_TRY(c, launch_scan(a, sb0, sb1 - sb0, p0, p1 - p0, c->stream));
   }
+  const uint32_t variant = c->tuning.emit_variant;
   {
-    ProfScope ps(c, "k_scatter", 0);
-    HIP_TRY(c, launch_scatter(a, tb0, tb1 - tb0, c->stream));
+    ProfScope ps(c, "k_emit", variant == 2 ? 0 : tr.uv);
+    HIP_TRY(c, launch_emit(a, tb0, tb1 - tb0, variant, c->stream));
   }
-  {
-    ProfScope ps(c, "k_assemble", 0);
-    HIP_TRY(c, launch_assemble(a, tb0, tb1 - tb0, c->stream));
+  if (variant == 2) {
+    ProfScope ps(c, "k_assemble", tr.uv);
+    HIP_TRY(c, launch_assemble(a, tb0, tb1 - tb0, c->tuning.store_mode, c->stream));
   }
   return T3D_OK;
 }
@@ -859,7 +861,7 @@ extern "C" t3d_status t3d_render(t3d_ctx* c, t3d_frame* f, const t3d_render_desc
   const t3d_tuning& tu = c->tuning;
   const uint32_t vthreads = tu.vertex_block ? tu.vertex_block : 256;
   const uint32_t vchunk = vthreads * std::max<uint32_t>(1, tu.vertices_per_thread);
-  const uint32_t tile = std::max<uint32_t>(64, tu.triangle_tile);
+  const uint32_t tile = std::min<uint32_t>(kMaxTile, std::max<uint32_t>(64, tu.triangle_tile));
 
   f->pairs.clear();
   f->parts.clear();
@@ -876,7 +878,16 @@ extern "C" t3d_status t3d_render(t3d_ctx* c, t3d_frame* f, const t3d_render_desc
   f->sblock_prefix.assign(1, 0);
   f->bytes_vertex.clear();
   f->bytes_cull.clear();
+  f->bytes_uv.clear();
   std::map<uint32_t, M4> rd_cache;
+  // Decompose(Inverted(M)) per model: shared by invertedCamPos (model.go:629-630)
+  // and every point light's workingPosition (light.go:175-189)
+  struct InvDec {
+    uint32_t model = UINT32_MAX;
+    V3 p, is;
+    M4 r;
+    V3 apply(V3 x) const { return add(mult_vec(r, x), hmul(p, is)); }
+  } inv;
 
   for (int32_t di : draw) {
     const Pair& rp = pairs[di];
@@ -895,7 +906,14 @@ extern "C" t3d_status t3d_render(t3d_ctx* c, t3d_frame* f, const t3d_render_desc
     const M4& M = md.transform;
     P.mvp = mult(M, cam.vp);                                          // model.go:695
     P.model = M;
-    V3 icp = inverted_position(M, cam.pos);                           // model.go:629-630
+    if (inv.model != rp.model) {
+      M4 im = inverted(M);
+      V3 s;
+      decompose(im, inv.p, s, inv.r);
+      inv.is = v3(fdiv(1.0f, s.x), fdiv(1.0f, s.y), fdiv(1.0f, s.z));
+      inv.model = rp.model;
+    }
+    V3 icp = inv.apply(cam.pos);                                      // model.go:629-630
     P.icp[0] = icp.x;
     P.icp[1] = icp.y;
     P.icp[2] = icp.z;
@@ -955,7 +973,7 @@ extern "C" t3d_status t3d_render(t3d_ctx* c, t3d_frame* f, const t3d_render_desc
         pl.enabled = 1;
         if (ld.type == T3D_LIGHT_POINT) {
           V3 wp = row3(ld.transform);
-          V3 w = md.skinned ? wp : inverted_position(M, wp);
+          V3 w = md.skinned ? wp : inv.apply(wp);
           pl.v[0] = w.x;
           pl.v[1] = w.y;
           pl.v[2] = w.z;
@@ -1031,6 +1049,7 @@ extern "C" t3d_status t3d_render(t3d_ctx* c, t3d_frame* f, const t3d_render_desc
     const uint64_t vb = 24 + (mesh.col ? 16 : 0) + (md.skinned ? 24 : 0);
     f->bytes_vertex.push_back((uint64_t)P.vcount * vb + (md.skinned ? 64ull * md.bone_count : 0) + sizeof(PartGPU));
     f->bytes_cull.push_back(12ull * P.tcount);
+    f->bytes_uv.push_back(mesh.uv ? 8ull * P.vcount : 0);   // UVs are read by the assembly
     f->parts.push_back(P);
     f->part_first_tri.push_back(pr.triangle_start);
     t3d_pair_result r{};
@@ -1063,7 +1082,7 @@ extern "C" t3d_status t3d_render(t3d_ctx* c, t3d_frame* f, const t3d_render_desc
     HIP_TRY(c, hipHostMalloc(&f->pinned, f->pinned_cap, hipHostMallocDefault));
   }
   // scratch: vertex streams, keys, sorted ids, stats, histograms, block sums
-  const uint64_t sbytes = 3 * carve_size<float4>(sumV) + 2 * carve_size<uint32_t>(sumT) + carve_size<uint32_t>(4ull * P_n) +
+  const uint64_t sbytes = carve_size<float4>(2 * sumV) + carve_size<float4>(sumV) + 2 * carve_size<uint32_t>(sumT) + carve_size<uint4>(sumT) + carve_size<uint32_t>(4ull * P_n) +
                           carve_size<uint32_t>(sumHist) + carve_size<uint32_t>(sblocks);
   st = ensure(c, f->scratch, f->scratch_cap, sbytes);
   if (st) return st;
@@ -1125,11 +1144,11 @@ extern "C" t3d_status t3d_render(t3d_ctx* c, t3d_frame* f, const t3d_render_desc
   a.vchunk_prefix = d_vpre;
   a.tile_prefix = d_tpre;
   a.sblock_prefix = d_spre;
-  a.vclip = sc.take<float4>(sumV);
-  a.vcol = sc.take<float4>(sumV);
+  a.vrec = sc.take<float4>(2 * sumV);
   a.vpos = sc.take<float4>(sumV);
   a.key = sc.take<float>(sumT);
   a.sorted = sc.take<uint32_t>(sumT);
+  a.tri_vo = sc.take<uint4>(sumT);
   a.stats = sc.take<uint32_t>(4ull * P_n);
   a.hist = sc.take<uint32_t>(sumHist);
   a.bsum = sc.take<uint32_t>(sblocks);
@@ -1142,6 +1161,11 @@ extern "C" t3d_status t3d_render(t3d_ctx* c, t3d_frame* f, const t3d_render_desc
   f->total_tiles = f->tile_prefix.back();
   f->total_sblocks = sblocks;
 
+  if (c->profiling) {
+    auto& hp = c->prof["host:prologue"];
+    hp.ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    hp.launches++;
+  }
   HIP_TRY(c, hipMemcpyAsync(f->tables.ptr, f->pinned, host.off, hipMemcpyHostToDevice, c->stream));
 
   if (4ull * P_n > f->pinned_stats_cap) {
@@ -1152,7 +1176,7 @@ extern "C" t3d_status t3d_render(t3d_ctx* c, t3d_frame* f, const t3d_render_desc
   }
 
   // output bytes per survivor are known only after the cull; the profile
-  // attributes them to k_assemble after readback (below)
+  // attributes them to k_emit after readback (below)
   if (P_n) {
     {
       ProfScope ps(c, "k_init_stats", 0);
@@ -1203,7 +1227,7 @@ extern "C" t3d_status t3d_render(t3d_ctx* c, t3d_frame* f, const t3d_render_desc
   }
   f->total_survivors = total;
   if (c->profiling) {
-    auto& p = c->prof["k_assemble"];
+    auto& p = c->prof[c->tuning.emit_variant == 2 ? "k_assemble" : "k_emit"];
     p.bytes += 300ull * total;   // 3 x (48 B colour + 48 B depth + 4 B index) per drawn triangle
   }
   f->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
@@ -1301,8 +1325,10 @@ t3d_status t3d_set_tuning(t3d_ctx* c, const t3d_tuning* t) {
     return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: vertex_block must be 64, 128, 192 or 256");
   if (t->vertices_per_thread == 0 || t->vertices_per_thread > 64)
     return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: vertices_per_thread in [1, 64]");
-  if (t->triangle_tile < 64 || t->triangle_tile > (1u << 20) || t->triangle_tile % 64)
-    return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: triangle_tile must be a multiple of 64 in [64, 2^20]");
+  if (t->triangle_tile < 64 || t->triangle_tile > kMaxTile || t->triangle_tile % 64)
+    return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: triangle_tile must be a multiple of 64 in [64, %u]", kMaxTile);
+  if (t->store_mode > 3) return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: store_mode in [0, 3]");
+  if (t->emit_variant > 2) return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: emit_variant in [0, 2]");
   c->tuning = *t;
   return T3D_OK;
 }
@@ -1315,9 +1341,11 @@ t3d_status t3d_autotune(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc, u
   double best_ms = 1e30;
   const uint32_t vblocks[] = {128, 256};
   const uint32_t vpts[] = {1, 2, 4};
-  const uint32_t tiles[] = {512, 1024, 2048, 4096};
+  const uint32_t tiles[] = {512, 1024, 2048};
+  const uint32_t variants[] = {0, 1, 2, 3, 4, 5};   // 0, 1, then variant 2 with store modes 0..3
   t3d_render_desc d = *desc;
   d.exec_mode = T3D_EXEC_BATCHED;
+  for (uint32_t ev : variants)
   for (uint32_t vb : vblocks)
     for (uint32_t vpt : vpts)
       for (uint32_t tl : tiles) {
@@ -1325,6 +1353,8 @@ t3d_status t3d_autotune(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc, u
         t.vertex_block = vb;
         t.vertices_per_thread = vpt;
         t.triangle_tile = tl;
+        t.emit_variant = ev < 2 ? ev : 2;
+        t.store_mode = ev < 2 ? 0 : ev - 2;
         c->tuning = t;
         t3d_status st = t3d_render(c, f, &d);   // warm: builds the graph for this shape
         if (st) {
@@ -1348,6 +1378,36 @@ t3d_status t3d_autotune(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc, u
       }
   c->tuning = best;
   if (out) *out = best;
+  return T3D_OK;
+}
+
+t3d_status t3d_selftest_copy(t3d_ctx* c, uint64_t bytes, uint32_t reps, double* out_ms) {
+  if (!c || bytes == 0 || bytes % 16 || reps == 0) return fail(c, T3D_ERR_INVALID_ARGUMENT, "selftest_copy: bad arguments");
+  (void)hipSetDevice(c->device);
+  DevBuf src, dst;
+  t3d_status st = dev_alloc(c, src, bytes);
+  if (st) return st;
+  st = dev_alloc(c, dst, bytes);
+  if (st) {
+    dev_free(c, src);
+    return st;
+  }
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  hipError_t e = launch_copy(src.ptr, dst.ptr, bytes, c->stream);   // warm
+  if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+  for (uint32_t i = 0; i < reps && e == hipSuccess; i++) e = launch_copy(src.ptr, dst.ptr, bytes, c->stream);
+  if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  dev_free(c, src);
+  dev_free(c, dst);
+  if (e != hipSuccess) return fail(c, T3D_ERR_DEVICE, "selftest_copy: %s", hipGetErrorString(e));
+  if (out_ms) *out_ms = ms / reps;
   return T3D_OK;
 }
 

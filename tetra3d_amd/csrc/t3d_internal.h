@@ -7,8 +7,9 @@
 //     unit-stride coalesced load.
 //   frame tables (uploaded per render): PartGPU[P], LightGPU[L], PartLight[sum],
 //     block->part prefix arrays.
-//   frame scratch: vclip[sumV] f32x4, vcol[sumV] f32x4, vpos[sumV] f32x4 (skinned
-//     positions), key[sumT] f32, sorted[sumT] u32, stats[P] u32x4, hist, bsum.
+//   frame scratch: vrec[sumV] = {clip f32x4, colour f32x4} (one 32-byte record
+//     per vertex), vpos[sumV] f32x4 (skinned positions), key[sumT] f32,
+//     sorted[sumT] u32 (drawn triangle per slot), stats[P] u32x4, hist, bsum.
 //   outputs: color[3*sumT] t3d_vertex, depth[3*sumT] t3d_vertex, idx[3*sumT] u32;
 //     render pair p owns vertices [out_base, out_base + 3*T_p) and fills the
 //     first 3*S_p of them.
@@ -24,6 +25,7 @@ constexpr uint32_t kBins = 512;        // sortingTriangles.go:166-168
 constexpr uint32_t kMaxLights = 64;    // lights one part may see on the GPU path
 constexpr uint32_t kMaxLdsBones = 64;  // bone palettes staged in LDS up to this size
 constexpr uint32_t kScanBlock = 2048;  // elements per scan block (256 threads x 8)
+constexpr uint32_t kMaxTile = 2048;    // triangles per tile (keys of a tile are staged in LDS)
 
 // PartGPU.flags
 enum : uint32_t {
@@ -118,11 +120,11 @@ struct FrameArgs {
   const uint32_t* vchunk_prefix;  // [P+1]
   const uint32_t* tile_prefix;    // [P+1]
   const uint32_t* sblock_prefix;  // [P+1]
-  float4* vclip;
-  float4* vcol;
+  float4* vrec;                 // [sumV][2]: clip position, vertex colour
   float4* vpos;
   float* key;
-  uint32_t* sorted;
+  uint32_t* sorted;              // [sumT] drawn triangle (part-local) per output slot
+  uint4* tri_vo;                // [sumT] vertex-record offsets of the triangle in each slot
   uint32_t* stats;              // [P][4]: min bits, max bits, survivors, 0
   uint32_t* hist;
   uint32_t* bsum;
@@ -140,8 +142,9 @@ hipError_t launch_cull(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStre
 hipError_t launch_hist(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s);
 hipError_t launch_scan(const FrameArgs& a, uint32_t sblk0, uint32_t nsblk, uint32_t part0, uint32_t nparts,
                        hipStream_t s);
-hipError_t launch_scatter(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s);
-hipError_t launch_assemble(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s);
+hipError_t launch_emit(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t variant, hipStream_t s);
+hipError_t launch_assemble(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t mode, hipStream_t s);
+hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 hipError_t launch_fp_selftest(const float* a, const float* b, float* out, uint32_t n, hipStream_t s);
 
 }  // namespace t3d

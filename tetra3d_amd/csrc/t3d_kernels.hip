@@ -13,11 +13,11 @@
 //               (model.go:828-969)
 //   k_hist      per tile of a multi-tile part: 512-bin histogram
 //   k_scan*     exclusive scan of each part's [bin][tile] histogram matrix
-//   k_scatter   stable counting sort: output slot of every survivor
-//               (sortingTriangleBucket Sort + ForEach, sortingTriangles.go:38-125)
-//   k_assemble  per drawn triangle, in output order: ebiten colour/depth
-//               vertices and indices staged in LDS and written as full
-//               coalesced float4 rows (camera.go:1646-1820)
+//   k_emit      stable counting sort fused with vertex assembly: output slot of
+//               every survivor (sortingTriangleBucket Sort + ForEach,
+//               sortingTriangles.go:38-125) and its ebiten colour/depth vertices
+//               and indices, staged in LDS per bin run and written as
+//               contiguous float4 rows (camera.go:1646-1820)
 //
 // All arithmetic is float32 in the reference's order; the build uses
 // -ffp-contract=off and correctly rounded division and square root.
@@ -281,8 +281,8 @@ __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
       }
     }
     const uint32_t o = P.vbase + i;
-    a.vclip[o] = make_float4(clip.x, clip.y, clip.z, clip.w);
-    a.vcol[o] = make_float4(cr, cg, cb, ca);
+    a.vrec[2 * (size_t)o] = make_float4(clip.x, clip.y, clip.z, clip.w);
+    a.vrec[2 * (size_t)o + 1] = make_float4(cr, cg, cb, ca);
     if (skinned) a.vpos[o] = make_float4(vp.x, vp.y, vp.z, 0.0f);
   }
 }
@@ -337,7 +337,7 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a, uint32_t blk0) {
     V3 stc = v3(0, 0, 0);
     for (int k = 0; k < 3; k++) {                                   // model.go:855-886
       const uint32_t o = P.vbase + (vi[k] - P.vmin);
-      const float4 c = a.vclip[o];
+      const float4 c = a.vrec[2 * (size_t)o];
       if (skinned) {
         const float4 sp = a.vpos[o];
         stc.x += sp.x;
@@ -458,9 +458,14 @@ __global__ __launch_bounds__(64) void k_hist(FrameArgs a, uint32_t blk0) {
   float mn, rd;
   part_range(a.stats, p, mn, rd);
   const uint32_t mode = (P.flags >> PF_SORT_SHIFT) & 3u;
-  for (uint32_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
-    const float d = a.key[P.tbase + t];
-    if (d >= 0) atomicAdd(&h[visit_bin(d, mn, rd, mode)], 1u);
+  for (uint32_t tb = t0; tb < t1; tb += 8 * 64) {   // eight independent loads in flight per lane
+    float d[8];
+    for (int j = 0; j < 8; j++) {
+      const uint32_t t = tb + j * 64 + threadIdx.x;
+      d[j] = t < t1 ? a.key[P.tbase + t] : -1.0f;
+    }
+    for (int j = 0; j < 8; j++)
+      if (d[j] >= 0) atomicAdd(&h[visit_bin(d[j], mn, rd, mode)], 1u);
   }
   __syncthreads();
   uint32_t* col = a.hist + P.hbase + tl;
@@ -545,22 +550,45 @@ __global__ __launch_bounds__(256) void k_scan_down(FrameArgs a, uint32_t sblk0) 
   }
 }
 
-// Stable counting sort: one wavefront per tile walks its triangles in index
-// order, 64 at a time, ranking equal bins with ballots; run[bin] is the next
-// output slot of that bin (from the scanned histogram, or from a local scan
-// when the whole part is one tile).
-__global__ __launch_bounds__(64) void k_scatter(FrameArgs a, uint32_t blk0) {
+// Fused stable scatter + vertex assembly.  One wavefront per tile walks its
+// triangles in index order, 64 at a time (so vertex records and indices are
+// read coherently), ranks equal bins with ballots, and gives every survivor
+// its output slot: run[bin] + rank, where run[bin] comes from the scanned
+// [bin][tile] histogram (or a local scan when the part is a single tile).
+// Within a chunk the survivors of one bin occupy consecutive slots; their
+// ebiten vertices are staged in LDS grouped by bin and copied out as
+// contiguous float4 rows (camera.go:1646-1820 per corner).
+// Variants: 0 = bin runs staged in LDS and copied out as contiguous rows;
+// 1 = each lane stores its own triangle's vertices directly; 2 = slots only
+// (the drawn-triangle list), with k_assemble writing the vertices in slot order.
+template <int kVariant>
+__global__ __launch_bounds__(64) void k_emit(FrameArgs a, uint32_t blk0) {
   const uint32_t b = blk0 + blockIdx.x;
   const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
   const PartGPU& P = a.parts[p];
+  if (a.stats[4 * p + 2] == 0) return;
   const uint32_t tl = b - a.tile_prefix[p];
   const uint32_t t0 = tl * a.tile, t1 = min(t0 + a.tile, P.tcount);
   const uint32_t lane = threadIdx.x;
+  const FrameConst& fc = *a.fc;
+  const uint32_t* __restrict__ idx = P.idx;
+  const float* __restrict__ uvp = P.uv;
+  const uint32_t vbase = P.vbase, vmin = P.vmin, tbase = P.tbase;
+  const bool has_uv = (P.flags & PF_HAS_UV) != 0;
+  const float src_w = P.src_w, src_h = P.src_h;
+  const uint32_t mode = (P.flags >> PF_SORT_SHIFT) & 3u;
+  float4* out_c = a.out_color + P.out_base * 3;   // 3 float4 per vertex
+  float4* out_d = a.out_depth + P.out_base * 3;
+  uint32_t* out_i = a.out_idx + P.out_base;
+  uint32_t* sorted = a.sorted + tbase;
+
   __shared__ uint32_t run[kBins];
+  __shared__ float4 stage[kVariant == 0 ? 64 * 9 : 1];
+  __shared__ float s_key[kMaxTile];
   float mn, rd;
   part_range(a.stats, p, mn, rd);
-  const uint32_t mode = (P.flags >> PF_SORT_SHIFT) & 3u;
-  if (a.stats[4 * p + 2] == 0) return;
+  for (uint32_t t = t0 + lane; t < t1; t += 64) s_key[t - t0] = a.key[tbase + t];   // one burst of loads
+  __syncthreads();
 
   if (P.ntiles > 1) {
     const uint32_t* col = a.hist + P.hbase + tl;
@@ -570,12 +598,11 @@ __global__ __launch_bounds__(64) void k_scatter(FrameArgs a, uint32_t blk0) {
     for (uint32_t i = lane; i < kBins; i += 64) run[i] = 0;
     __syncthreads();
     for (uint32_t t = t0 + lane; t < t1; t += 64) {
-      const float d = a.key[P.tbase + t];
+      const float d = s_key[t - t0];
       if (d >= 0) atomicAdd(&run[visit_bin(d, mn, rd, mode)], 1u);
     }
     __syncthreads();
-    // exclusive scan of 512 counters: 8 consecutive per lane
-    uint32_t c[8], s = 0;
+    uint32_t c[8], s = 0;                       // exclusive scan of 512 counters, 8 per lane
     for (int k = 0; k < 8; k++) {
       c[k] = run[lane * 8 + k];
       s += c[k];
@@ -598,37 +625,167 @@ __global__ __launch_bounds__(64) void k_scatter(FrameArgs a, uint32_t blk0) {
   for (uint32_t base_t = t0; base_t < t1; base_t += 64) {
     const uint32_t t = base_t + lane;
     float d = -1.0f;
-    if (t < t1) d = a.key[P.tbase + t];
+    if (t < t1) d = s_key[t - t0];
     const bool valid = d >= 0;
+    const uint64_t vmask = __ballot(valid);
+    if (vmask == 0) continue;                   // uniform: no survivor in this chunk
     const uint32_t bin = valid ? visit_bin(d, mn, rd, mode) : 0xFFFFFFFFu;
-    uint64_t todo = __ballot(valid);
-    uint64_t peers = 0;
-    while (todo) {
+    uint64_t todo = vmask, peers = 0;
+    while (todo) {                              // peers = lanes with my bin
       const int leader = __ffsll((unsigned long long)todo) - 1;
       const uint32_t lb = __shfl(bin, leader);
       const uint64_t m = __ballot(valid && bin == lb);
       if (valid && bin == lb) peers = m;
       todo &= ~m;
     }
+    const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+    const uint32_t cnt = (uint32_t)__popcll(peers);
+    const bool is_leader = valid && rank == 0;
+    const uint64_t leaders = __ballot(is_leader);
+    // groups ordered by leader lane: exclusive prefix of group sizes
+    uint32_t x = is_leader ? cnt : 0u;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if (lane >= (uint32_t)o) x += y;
+    }
+    const uint32_t gstart_l = x - (is_leader ? cnt : 0u);   // valid at leader lanes
+    const int my_leader = valid ? __ffsll((unsigned long long)peers) - 1 : 0;
+    const uint32_t gstart = __shfl(gstart_l, my_leader);
+    const uint32_t cpos = gstart + rank;        // wave-local position, grouped by bin
     uint32_t slot = 0;
-    if (valid) slot = run[bin] + (uint32_t)__popcll(peers & lt_mask);
+    if (valid) slot = run[bin] + rank;
+    const uint32_t slot0_l = slot;              // slot of rank 0 at leader lanes
+    __syncthreads();
+    if (is_leader) run[bin] = slot + cnt;
+
+    if (kVariant == 2) {
+      if (valid) {
+        sorted[slot] = t;
+        a.tri_vo[tbase + slot] = make_uint4(vbase + (idx[3 * (size_t)t + 0] - vmin), vbase + (idx[3 * (size_t)t + 1] - vmin),
+                                            vbase + (idx[3 * (size_t)t + 2] - vmin), 0u);
+      }
+      __syncthreads();
+      continue;
+    }
+    // per-corner values of this survivor (camera.go:1655-1784)
+    float4 cq[9];
+    float dq[3][6];
+    if (valid) {
+      sorted[slot] = t;
+      const uint32_t vi[3] = {idx[3 * (size_t)t + 0], idx[3 * (size_t)t + 1], idx[3 * (size_t)t + 2]};
+      for (int k = 0; k < 3; k++) {
+        const uint32_t v = vi[k];
+        const uint32_t o = vbase + (v - vmin);
+        const float4 c = a.vrec[2 * (size_t)o];
+        const float4 col = a.vrec[2 * (size_t)o + 1];
+        float w = c.w;
+        if (!fc.perspective) w = 1.0f;
+        if (w < 0) w = 0.001f;
+        const float dx = fdiv(c.x, w) * fc.cam_w + fc.half_w;
+        const float dy = fdiv(c.y, -w) * fc.cam_h + fc.half_h;
+        float U = 0, V = 0;
+        if (has_uv) {
+          const float2 uv = reinterpret_cast<const float2*>(uvp)[v];
+          U = uv.x;
+          V = uv.y;
+        }
+        float uvU, uvV, c0 = 0.0f;
+        if (fc.pc_tex) {
+          uvU = fdiv(U, w) * src_w;
+          uvV = fdiv(1.0f - V, w) * src_h;
+          c0 = fdiv(1.0f, w);
+        } else {
+          uvU = U * src_w;
+          uvV = (1.0f - V) * src_h;
+        }
+        float depth = 0.0f;
+        if (fc.render_depth) {
+          depth = fdiv(c.z + fc.dmp, fc.spread);
+          if (depth < 0) depth = 0;
+          else if (depth > 1) depth = 1;
+        }
+        cq[3 * k + 0] = make_float4(dx, dy, uvU, uvV);
+        cq[3 * k + 1] = col;
+        cq[3 * k + 2] = make_float4(c0, 0.0f, 0.0f, 0.0f);
+        dq[k][0] = dx;
+        dq[k][1] = dy;
+        dq[k][2] = uvU;
+        dq[k][3] = uvV;
+        dq[k][4] = depth;
+        dq[k][5] = c0;
+      }
+    }
+    const float da = fc.render_depth ? 1.0f : 0.0f;
+    if (kVariant == 1) {
+      if (valid) {
+        float4* oc = out_c + (size_t)slot * 9;
+        float4* od = out_d + (size_t)slot * 9;
+        for (int q = 0; q < 9; q++) oc[q] = cq[q];
+        for (int k = 0; k < 3; k++) {
+          od[3 * k + 0] = make_float4(dq[k][0], dq[k][1], dq[k][2], dq[k][3]);
+          od[3 * k + 1] = make_float4(dq[k][4], dq[k][4], dq[k][4], da);
+          od[3 * k + 2] = make_float4(dq[k][5], 0.0f, 0.0f, 0.0f);
+        }
+        for (int k = 0; k < 3; k++) out_i[(size_t)slot * 3 + k] = slot * 3 + k;
+      }
+      __syncthreads();
+      continue;
+    }
+    if (valid)
+      for (int q = 0; q < 9; q++) stage[cpos * 9 + q] = cq[q];
+    __syncthreads();
+    // colour vertices: one contiguous copy per bin group
+    for (uint64_t lm = leaders; lm; lm &= lm - 1) {
+      const int L = __ffsll((unsigned long long)lm) - 1;
+      const uint32_t gs = __shfl(gstart_l, L), gc = __shfl(cnt, L), s0 = __shfl(slot0_l, L);
+      for (uint32_t j = lane; j < gc * 9; j += 64) out_c[(size_t)s0 * 9 + j] = stage[gs * 9 + j];
+    }
     __syncthreads();
     if (valid) {
-      a.sorted[P.tbase + slot] = t;
-      if ((peers & lt_mask) == 0) run[bin] = slot + (uint32_t)__popcll(peers);
+      for (int k = 0; k < 3; k++) {
+        stage[cpos * 9 + 3 * k + 0] = make_float4(dq[k][0], dq[k][1], dq[k][2], dq[k][3]);
+        stage[cpos * 9 + 3 * k + 1] = make_float4(dq[k][4], dq[k][4], dq[k][4], da);
+        stage[cpos * 9 + 3 * k + 2] = make_float4(dq[k][5], 0.0f, 0.0f, 0.0f);
+      }
+    }
+    __syncthreads();
+    for (uint64_t lm = leaders; lm; lm &= lm - 1) {
+      const int L = __ffsll((unsigned long long)lm) - 1;
+      const uint32_t gs = __shfl(gstart_l, L), gc = __shfl(cnt, L), s0 = __shfl(slot0_l, L);
+      for (uint32_t j = lane; j < gc * 9; j += 64) out_d[(size_t)s0 * 9 + j] = stage[gs * 9 + j];
+      // pair-local iota indices, as the reference's indexList (camera.go:1816)
+      for (uint32_t j = lane; j < gc * 3; j += 64) out_i[(size_t)s0 * 3 + j] = s0 * 3 + j;
     }
     __syncthreads();
   }
 }
 
-// ---------------------------------------------------------------------------
-// k_assemble
-// ---------------------------------------------------------------------------
 
-constexpr uint32_t kAsmThreads = 256;
-constexpr uint32_t kFloatsPerTri = 36;   // 3 ebiten vertices of 12 floats
+typedef float f4v __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(kAsmThreads) void k_assemble(FrameArgs a, uint32_t blk0) {
+template <bool kNT>
+__device__ __forceinline__ void store4(float4* p, float4 v) {
+  if (kNT) {
+    f4v x = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(x, reinterpret_cast<f4v*>(p));
+  } else {
+    *p = v;
+  }
+}
+template <bool kNT>
+__device__ __forceinline__ void store1(uint32_t* p, uint32_t v) {
+  if (kNT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+// Output-order assembly (emit variant 2): per drawn triangle in slot order,
+// the ebiten colour/depth vertices and indices (camera.go:1646-1820).  The
+// sort pass left each slot's three vertex-record offsets in tri_vo, so the
+// only gathers are the 32-byte vertex records and UVs.  kStage: stage each
+// wavefront's 64 triangles in LDS and write contiguous float4 rows (else each
+// lane stores its own 144-byte runs); kNT: non-temporal (streaming) stores.
+template <bool kStage, bool kNT>
+__global__ __launch_bounds__(64) void k_assemble(FrameArgs a, uint32_t blk0) {
   const uint32_t b = blk0 + blockIdx.x;
   const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
   const PartGPU& P = a.parts[p];
@@ -637,26 +794,31 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble(FrameArgs a, uint32_t 
   if (k0 >= S) return;
   const uint32_t k1 = min(k0 + a.tile, S);
   const FrameConst& fc = *a.fc;
-  const uint32_t wv = threadIdx.x / 64, lane = threadIdx.x % 64;
-
-  __shared__ float4 s_out[kAsmThreads / 64][64 * kFloatsPerTri / 4];   // 9 KB per wave
-  float4* my = s_out[wv];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t vbase = P.vbase, vmin = P.vmin, tbase = P.tbase;
+  const bool has_uv = (P.flags & PF_HAS_UV) != 0;
+  const float src_w = P.src_w, src_h = P.src_h;
+  const float* __restrict__ uvp = P.uv;
+  const uint4* __restrict__ tvo = a.tri_vo + tbase;
   float4* out_c = a.out_color + P.out_base * 3;   // 3 float4 per vertex
   float4* out_d = a.out_depth + P.out_base * 3;
-  const bool has_uv = (P.flags & PF_HAS_UV) != 0;
+  uint32_t* out_i = a.out_idx + P.out_base;
+  const float da = fc.render_depth ? 1.0f : 0.0f;
 
-  for (uint32_t kb = k0; kb < k1; kb += kAsmThreads) {
-    const uint32_t wk0 = kb + wv * 64;                  // this wave's first triangle
+  __shared__ float4 stage[kStage ? 64 * 9 : 1];
+  for (uint32_t wk0 = k0; wk0 < k1; wk0 += 64) {
     const uint32_t k = wk0 + lane;
     const bool active = k < k1;
-    float4 cq[9], dq[9];
+    const uint32_t wcount = min(64u, k1 - wk0);
+    float4 cq[9];
+    float dd[3];
     if (active) {
-      const uint32_t t = a.sorted[P.tbase + k];
+      const uint4 vo = tvo[k];
+      const uint32_t ov[3] = {vo.x, vo.y, vo.z};
       for (int vi = 0; vi < 3; vi++) {
-        const uint32_t v = P.idx[3 * (size_t)t + vi];
-        const uint32_t o = P.vbase + (v - P.vmin);
-        const float4 c = a.vclip[o];
-        const float4 col = a.vcol[o];
+        const uint32_t o = ov[vi];
+        const float4 c = a.vrec[2 * (size_t)o];
+        const float4 col = a.vrec[2 * (size_t)o + 1];
         float w = c.w;                                            // camera.go:1655-1667
         if (!fc.perspective) w = 1.0f;
         if (w < 0) w = 0.001f;
@@ -664,51 +826,63 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble(FrameArgs a, uint32_t 
         const float dy = fdiv(c.y, -w) * fc.cam_h + fc.half_h;
         float U = 0, V = 0;
         if (has_uv) {
-          const float2 uv = reinterpret_cast<const float2*>(P.uv)[v];
+          const float2 uv = reinterpret_cast<const float2*>(uvp)[o - vbase + vmin];
           U = uv.x;
           V = uv.y;
         }
         float uvU, uvV, c0 = 0.0f;
         if (fc.pc_tex) {                                          // camera.go:1688-1711
-          uvU = fdiv(U, w) * P.src_w;
-          uvV = fdiv(1.0f - V, w) * P.src_h;
+          uvU = fdiv(U, w) * src_w;
+          uvV = fdiv(1.0f - V, w) * src_h;
           c0 = fdiv(1.0f, w);
         } else {
-          uvU = U * P.src_w;
-          uvV = (1.0f - V) * P.src_h;
+          uvU = U * src_w;
+          uvV = (1.0f - V) * src_h;
         }
-        float dr = 0, dg = 0, db = 0, da = 0;
+        float depth = 0.0f;
         if (fc.render_depth) {                                    // camera.go:1772-1784
-          float depth = fdiv(c.z + fc.dmp, fc.spread);
+          depth = fdiv(c.z + fc.dmp, fc.spread);
           if (depth < 0) depth = 0;
           else if (depth > 1) depth = 1;
-          dr = dg = db = depth;
-          da = 1.0f;
         }
         cq[3 * vi + 0] = make_float4(dx, dy, uvU, uvV);
         cq[3 * vi + 1] = col;
         cq[3 * vi + 2] = make_float4(c0, 0.0f, 0.0f, 0.0f);
-        dq[3 * vi + 0] = make_float4(dx, dy, uvU, uvV);
-        dq[3 * vi + 1] = make_float4(dr, dg, db, da);
-        dq[3 * vi + 2] = make_float4(c0, 0.0f, 0.0f, 0.0f);
+        dd[vi] = depth;
       }
     }
-    const uint32_t wcount = wk0 < k1 ? min(64u, k1 - wk0) : 0u;   // triangles this wave writes
-    // colour vertices: registers -> LDS -> coalesced float4 rows
-    if (active)
-      for (int q = 0; q < 9; q++) my[lane * 9 + q] = cq[q];
-    __syncthreads();
-    for (uint32_t f = lane; f < wcount * 9; f += 64) out_c[(size_t)wk0 * 9 + f] = my[f];
-    __syncthreads();
-    if (active)
-      for (int q = 0; q < 9; q++) my[lane * 9 + q] = dq[q];
-    __syncthreads();
-    for (uint32_t f = lane; f < wcount * 9; f += 64) out_d[(size_t)wk0 * 9 + f] = my[f];
+    if (kStage) {
+      if (active)
+        for (int q = 0; q < 9; q++) stage[lane * 9 + q] = cq[q];
+      __syncthreads();
+      for (uint32_t f = lane; f < wcount * 9; f += 64) store4<kNT>(&out_c[(size_t)wk0 * 9 + f], stage[f]);
+      __syncthreads();
+      if (active)
+        for (int vi = 0; vi < 3; vi++) {
+          stage[lane * 9 + 3 * vi + 1] = make_float4(dd[vi], dd[vi], dd[vi], da);
+        }
+      __syncthreads();
+      for (uint32_t f = lane; f < wcount * 9; f += 64) store4<kNT>(&out_d[(size_t)wk0 * 9 + f], stage[f]);
+      __syncthreads();
+    } else if (active) {
+      float4* oc = out_c + (size_t)k * 9;
+      float4* od = out_d + (size_t)k * 9;
+      for (int q = 0; q < 9; q++) store4<kNT>(&oc[q], cq[q]);
+      for (int vi = 0; vi < 3; vi++) {
+        store4<kNT>(&od[3 * vi + 0], cq[3 * vi + 0]);
+        store4<kNT>(&od[3 * vi + 1], make_float4(dd[vi], dd[vi], dd[vi], da));
+        store4<kNT>(&od[3 * vi + 2], cq[3 * vi + 2]);
+      }
+    }
     // indices: pair-local iota, as the reference's indexList (camera.go:1816)
-    uint32_t* oi = a.out_idx + P.out_base;
-    for (uint32_t f = lane; f < wcount * 3; f += 64) oi[(size_t)wk0 * 3 + f] = wk0 * 3 + f;
-    __syncthreads();
+    for (uint32_t f = lane; f < wcount * 3; f += 64) store1<kNT>(&out_i[(size_t)wk0 * 3 + f], wk0 * 3 + f);
   }
+}
+
+// Plain float4 copy of a known size: the reference point for HBM byte counters.
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
 }
 
 __global__ void k_fp_selftest(const float* x, const float* y, float* out, uint32_t n) {
@@ -752,14 +926,26 @@ hipError_t launch_scan(const FrameArgs& a, uint32_t sblk0, uint32_t nsblk, uint3
   hipLaunchKernelGGL(k_scan_down, dim3(nsblk), dim3(256), 0, s, a, sblk0);
   return hipGetLastError();
 }
-hipError_t launch_scatter(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s) {
+hipError_t launch_emit(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t variant, hipStream_t s) {
   if (!nblk) return hipSuccess;
-  hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(64), 0, s, a, blk0);
+  if (variant == 0) hipLaunchKernelGGL(k_emit<0>, dim3(nblk), dim3(64), 0, s, a, blk0);
+  else if (variant == 1) hipLaunchKernelGGL(k_emit<1>, dim3(nblk), dim3(64), 0, s, a, blk0);
+  else hipLaunchKernelGGL(k_emit<2>, dim3(nblk), dim3(64), 0, s, a, blk0);
   return hipGetLastError();
 }
-hipError_t launch_assemble(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s) {
+hipError_t launch_assemble(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t mode, hipStream_t s) {
   if (!nblk) return hipSuccess;
-  hipLaunchKernelGGL(k_assemble, dim3(nblk), dim3(kAsmThreads), 0, s, a, blk0);
+  switch (mode & 3u) {
+    case 0: hipLaunchKernelGGL((k_assemble<true, false>), dim3(nblk), dim3(64), 0, s, a, blk0); break;
+    case 1: hipLaunchKernelGGL((k_assemble<false, false>), dim3(nblk), dim3(64), 0, s, a, blk0); break;
+    case 2: hipLaunchKernelGGL((k_assemble<true, true>), dim3(nblk), dim3(64), 0, s, a, blk0); break;
+    default: hipLaunchKernelGGL((k_assemble<false, true>), dim3(nblk), dim3(64), 0, s, a, blk0); break;
+  }
+  return hipGetLastError();
+}
+hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s) {
+  const uint64_t n = bytes / 16;
+  hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, s, reinterpret_cast<const float4*>(src), reinterpret_cast<float4*>(dst), n);
   return hipGetLastError();
 }
 hipError_t launch_fp_selftest(const float* x, const float* y, float* out, uint32_t n, hipStream_t s) {

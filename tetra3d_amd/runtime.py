@@ -71,11 +71,12 @@ def lib():
     L.t3d_set_tuning.argtypes = [vp, P(A.Tuning)]
     L.t3d_autotune.argtypes = [vp, vp, P(A.RenderDesc), C.c_uint32, P(A.Tuning)]
     L.t3d_selftest_fp.argtypes = [vp, P(C.c_float), P(C.c_float), P(C.c_float), C.c_uint32]
+    L.t3d_selftest_copy.argtypes = [vp, C.c_uint64, C.c_uint32, P(C.c_double)]
     for name in ("t3d_ctx_create", "t3d_ctx_destroy", "t3d_mesh_upload", "t3d_mesh_free", "t3d_volume_upload",
                  "t3d_volume_free", "t3d_frame_create", "t3d_frame_destroy", "t3d_render", "t3d_frame_pairs",
                  "t3d_frame_device_outputs", "t3d_frame_download", "t3d_frame_download_triangles",
                  "t3d_profile_enable", "t3d_profile_read", "t3d_get_tuning", "t3d_set_tuning", "t3d_autotune",
-                 "t3d_selftest_fp"):
+                 "t3d_selftest_fp", "t3d_selftest_copy"):
         getattr(L, name).restype = C.c_int32
     _lib = L
     return L
@@ -115,7 +116,8 @@ class Context:
             raise T3DError(rc, "t3d_ctx_create failed (is a HIP device visible?)")
         self.frame = C.c_void_p()
         self._check(self.L.t3d_frame_create(self.ctx, C.byref(self.frame)))
-        self._mesh_handles = {}
+        self._mesh_handles = {}   # id(mesh) -> handle
+        self._mesh_refs = {}      # id(mesh) -> mesh: keeps ids from being reused while uploaded
 
     def _check(self, rc):
         if rc != 0:
@@ -155,11 +157,13 @@ class Context:
         out = C.c_uint64()
         self._check(self.L.t3d_mesh_upload(self.ctx, C.byref(d), C.byref(out)))
         self._mesh_handles[id(mesh)] = out.value
+        self._mesh_refs[id(mesh)] = mesh
         mesh._handles["gpu"] = out.value
         return out.value
 
     def free_mesh(self, mesh):
         h = self._mesh_handles.pop(id(mesh))
+        self._mesh_refs.pop(id(mesh), None)
         self._check(self.L.t3d_mesh_free(self.ctx, h))
 
     def upload_volume(self, vol):
@@ -256,6 +260,11 @@ class Context:
     def render_workload(self, wl, exec_mode=A.EXEC_BATCHED, download=True):
         self.prepare(wl)
         return self.render(wl.camera, wl.scene, wl.lights, wl.models, exec_mode, download)
+
+    def selftest_copy(self, nbytes, reps=20):
+        ms = C.c_double()
+        self._check(self.L.t3d_selftest_copy(self.ctx, nbytes, reps, C.byref(ms)))
+        return ms.value
 
     def selftest_fp(self, x, y):
         x = np.ascontiguousarray(x, dtype=np.float32)
