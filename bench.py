@@ -84,6 +84,136 @@ def cpu_baseline(wl, budget_s):
     return frames, el
 
 
+class _JobInputs:
+    """Descriptors of one serving job, assembled from per-model descriptors
+    built once for the pool (a model's descriptor does not depend on the job)."""
+
+    def __init__(self, camera, world, lights, model_descs):
+        from tetra3d_amd import _abi as A
+        import ctypes as C
+        self.cam = camera.desc()
+        self.world = world
+        self.lights = (A.LightDesc * max(1, len(lights)))(*[l.desc() for l in lights])
+        self.models = (A.ModelDesc * max(1, len(model_descs)))(*model_descs)
+        self.render = A.RenderDesc()
+        self.render.camera = C.pointer(self.cam)
+        self.render.world = C.pointer(self.world)
+        self.render.light_count = len(lights)
+        self.render.lights = self.lights
+        self.render.model_count = len(model_descs)
+        self.render.models = self.models
+        self.render.exec_mode = A.EXEC_BATCHED
+
+
+def job_inputs(ctx, models, jobs, scene):
+    keep, cache = [], {}
+    world = scene.world.desc()
+    out = []
+    for j in jobs:
+        descs = []
+        for i in j.model_ids:
+            i = int(i)
+            if i not in cache:
+                cache[i] = models[i].desc(ctx.mesh_handle(models[i].mesh), keep)
+            descs.append(cache[i])
+        out.append(_JobInputs(j.camera, world, j.lights, descs))
+    for o in out:
+        o.keep = keep          # arrays the cached model descriptors point into
+    return out
+
+
+def serve_bench(args):
+    """Serving loop: a resident pool of mixed models, a Poisson job trace
+    replayed at its arrival times against the server (continuous batching on
+    several streams, admission by HBM budget); per-job latency is submit ->
+    outputs complete in HBM.  The plain per-job path (one kernel chain per
+    render pair, one job at a time) runs on a sample of the same jobs."""
+    from tetra3d_amd import _abi as A
+    from tetra3d_amd import workloads as W
+    from tetra3d_amd.runtime import Context
+    from tetra3d_amd.serving import Server
+
+    t = time.perf_counter()
+    models, jobs = W.serving_trace(pool_size=args.pool, jobs=args.jobs, rate_hz=args.rate,
+                                   workers=min(16, os.cpu_count() or 1))
+    scene = W.Scene("serving", W._world(0.2))
+    pool_v = sum(m.mesh.vertex_count for m in models)
+    pool_t = sum(m.mesh.triangle_count for m in models)
+    log("serving pool: %d models, %d vertices, %d triangles; %d jobs at %.0f/s (generated in %.1fs)"
+        % (len(models), pool_v, pool_t, len(jobs), args.rate, time.perf_counter() - t))
+    ctx = Context(0)
+    t = time.perf_counter()
+    for m in models:
+        ctx.upload_mesh(m.mesh)
+    resident = ctx.memory_used()
+    log("pool resident in HBM: %.2f GB (uploaded in %.1fs)" % (resident / 1e9, time.perf_counter() - t))
+    t = time.perf_counter()
+    inputs = job_inputs(ctx, models, jobs, scene)
+    job_tris = [sum(models[i].mesh.triangle_count for i in j.model_ids) for j in jobs]
+    log("descriptors built in %.1fs; mean %.3g triangles per job" % (time.perf_counter() - t, np.mean(job_tris)))
+
+    budget = int(args.serve_budget_gb * 1e9)
+    with Server(ctx, slots=args.slots, max_batch_jobs=args.max_batch_jobs, hbm_budget=budget,
+                auto_release=True) as srv:
+        # warm-up: grow every slot's frame buffers to a large batch
+        warm = [srv.submit_inputs(inputs[k % len(inputs)]) for k in range(min(len(inputs), 4 * args.slots))]
+        srv.drain()
+        srv.reset_stats()
+        # replay the trace at its arrival times
+        arrivals = np.array([j.arrival_s for j in jobs]) - jobs[0].arrival_s
+        t0 = time.perf_counter()
+        for k, inp in enumerate(inputs):
+            while time.perf_counter() - t0 < arrivals[k]:
+                pass
+            srv.submit_inputs(inp)
+        srv.drain(timeout_ms=600000)
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        del warm
+    span_s = (st.last_done_ms - st.first_submit_ms) / 1e3
+    served = st.triangles / span_s
+    mean_batch = st.completed / max(st.batches, 1)
+    log("served %d jobs in %d batches (mean %.1f jobs/batch), %.3g tris/s, p50 %.3f p99 %.3f ms"
+        % (st.completed, st.batches, mean_batch, served, st.p50_ms, st.p99_ms))
+
+    # plain per-job path on a sample: one kernel chain per render pair, jobs one after another
+    sample = list(range(0, len(jobs), max(1, len(jobs) // args.plain_jobs)))[:args.plain_jobs]
+    ctx.render(jobs[sample[0]].camera, scene, jobs[sample[0]].lights,
+               [models[i] for i in jobs[sample[0]].model_ids], A.EXEC_PLAIN, download=False)
+    lat, tt = [], 0
+    t0 = time.perf_counter()
+    for k in sample:
+        a = time.perf_counter()
+        ctx.render(jobs[k].camera, scene, jobs[k].lights, [models[i] for i in jobs[k].model_ids], A.EXEC_PLAIN,
+                   download=False)
+        lat.append((time.perf_counter() - a) * 1e3)
+        tt += job_tris[k]
+    plain_value = tt / (time.perf_counter() - t0)
+    log("plain per-job: %.3g tris/s, p50 %.3f ms per job" % (plain_value, np.percentile(lat, 50)))
+
+    out = {
+        "metric": "serving tris/sec with p50/p99 job latency (continuous batching vs plain per-job path)",
+        "value": served,
+        "unit": "tris/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f32",
+        "data": "synthetic (seeded SplitMix64 pool and Poisson trace, seed 0x7e7a3d)",
+        "config": {"workload": "serving_pool%d" % args.pool, "pool_models": len(models), "pool_vertices": pool_v,
+                   "pool_triangles": pool_t, "resident_bytes": resident, "jobs": len(jobs), "rate_hz": args.rate,
+                   "offered_tris_per_s": float(np.mean(job_tris)) * args.rate, "slots": args.slots,
+                   "max_batch_jobs": args.max_batch_jobs, "hbm_budget_bytes": budget},
+        "latency_ms": {"p50": st.p50_ms, "p99": st.p99_ms},
+        "batches": st.batches, "mean_jobs_per_batch": mean_batch, "completed": st.completed,
+        "rejected": st.rejected, "wall_s": wall,
+        "plain": {"value": plain_value, "jobs_sampled": len(sample), "p50_ms": float(np.percentile(lat, 50)),
+                  "p99_ms": float(np.percentile(lat, 99)), "speedup_served_vs_plain": served / plain_value},
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,12 +221,23 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="mixed", choices=sorted(WORKLOADS))
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", default="", metavar="K=V,...",
+                    help="launch shape to use instead of autotuning (fields of t3d_tuning), e.g. the config.tuning "
+                         "of an earlier run, so profiler runs time the same shape")
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain device copy of BYTES (counter calibration) and exit")
+    ap.add_argument("--serve", action="store_true", help="run the serving loop instead of the frame benchmark")
+    ap.add_argument("--pool", type=int, default=10000)
+    ap.add_argument("--jobs", type=int, default=2000)
+    ap.add_argument("--rate", type=float, default=2000.0)
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--max-batch-jobs", type=int, default=32)
+    ap.add_argument("--serve-budget-gb", type=float, default=24.0)
+    ap.add_argument("--plain-jobs", type=int, default=20)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -108,6 +249,9 @@ def main():
     from tetra3d_amd import _abi as A
     from tetra3d_amd import workloads as W
     from tetra3d_amd.runtime import Context
+
+    if args.serve:
+        return serve_bench(args)
 
     if args.copy_calibration:
         with Context(0) as ctx:
@@ -126,11 +270,16 @@ def main():
     ctx.prepare(wl)
     inp = ctx.inputs(wl.camera, wl.scene, wl.lights, wl.models, A.EXEC_BATCHED)
     tuning = None
-    if not args.no_autotune:
+    if args.tuning:
+        kv = dict(item.split("=") for item in args.tuning.split(",") if item)
+        ctx.set_tuning(**{k: int(v) for k, v in kv.items()})
+        tu = ctx.tuning()
+        tuning = {f: getattr(tu, f) for f, _ in A.Tuning._fields_}
+        log("tuning set to %s" % tuning)
+    elif not args.no_autotune:
         t = time.perf_counter()
         tu = ctx.autotune(inp, reps=3)
-        tuning = {"vertex_block": tu.vertex_block, "vertices_per_thread": tu.vertices_per_thread,
-                  "triangle_tile": tu.triangle_tile, "emit_variant": tu.emit_variant, "store_mode": tu.store_mode}
+        tuning = {f: getattr(tu, f) for f, _ in A.Tuning._fields_}
         log("autotune picked %s in %.1fs" % (tuning, time.perf_counter() - t))
 
     for _ in range(args.warmup):
@@ -140,7 +289,7 @@ def main():
     tris = wl.triangle_count
     log("pairs %d, survivors %d of %d triangles" % (len(pairs), survivors, tris))
 
-    # timed region: K frames, batched form from the HIP graph
+    # latency: K synchronous frames, each submit -> outputs complete in HBM
     per = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -148,10 +297,28 @@ def main():
         ctx.render_inputs(inp)       # synchronous: returns when the frame's outputs are in HBM
         per.append((time.perf_counter() - a) * 1e3)
     t1 = time.perf_counter()
+    sync_ms = (t1 - t0) * 1e3 / args.steps
+    p50, p99 = float(np.percentile(per, 50)), float(np.percentile(per, 99))
+    log("batched, synchronous: %.3f ms/frame, %.3g tris/s, p50 %.3f p99 %.3f ms"
+        % (sync_ms, tris / (sync_ms / 1e3), p50, p99))
+
+    # throughput (the timed region): K frames submitted asynchronously into
+    # two alternating frames, so frame k's host prologue overlaps frame k-1's
+    # kernels; every frame's outputs are complete in HBM before the clock stops
+    frames = [ctx.frame, ctx.new_frame()]
+    for k in range(max(2, args.warmup)):
+        ctx.render_async(inp, frames[k % 2])
+    for f in frames:
+        ctx.wait(f)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ctx.render_async(inp, frames[k % 2])   # waits for frame k-2 of the same buffers first
+    for f in frames:
+        ctx.wait(f)
+    t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
     value = tris / (ms / 1e3)
-    p50, p99 = float(np.percentile(per, 50)), float(np.percentile(per, 99))
-    log("batched: %.3f ms/frame, %.3g tris/s, p50 %.3f p99 %.3f ms" % (ms, value, p50, p99))
+    log("batched, pipelined: %.3f ms/frame, %.3g tris/s" % (ms, value))
 
     # per-kernel durations with HIP events on the library's stream
     ctx.profile(True)
@@ -210,8 +377,10 @@ def main():
         "data": "synthetic (seeded SplitMix64 scene, seed 0x7e7a3d)",
         "config": {"workload": wname, "description": wdesc, "triangles_per_frame": tris,
                    "vertices_per_frame": wl.vertex_count, "render_pairs": len(pairs), "drawn_triangles": survivors,
-                   "exec": "batched, HIP graph", "tuning": tuning},
-        "latency_ms": {"p50": p50, "p99": p99, "mean": ms},
+                   "exec": "batched, HIP graph, two frames in flight", "tuning": tuning},
+        "latency_ms": {"p50": p50, "p99": p99, "mean": sync_ms,
+                       "note": "synchronous frames: submit -> outputs complete in HBM"},
+        "pipelined": {"ms_per_step": ms, "frames_in_flight": 2},
         "hbm": {"algorithmic_bytes_per_frame": B, "frame_pct_of_peak": 100.0 * B / (ms / 1e3) / (HBM_PEAK_GBS * 1e9),
                 "device_ms_per_frame": device_ms, "host_prologue_ms": host_ms,
                 "device_pct_of_peak": 100.0 * B / (device_ms / 1e3) / (HBM_PEAK_GBS * 1e9)},

@@ -275,7 +275,19 @@ t3d_status t3d_frame_destroy(t3d_ctx* ctx, t3d_frame* frame);
  * vertices, depth vertices and indices that flush() would draw. */
 t3d_status t3d_render(t3d_ctx* ctx, t3d_frame* frame, const t3d_render_desc* desc);
 
-/* Number of render pairs of the last render and their results. */
+/* The same render without the wait: builds the frame's tables, enqueues the
+ * table upload (on the context's copy stream) and the kernel chain, and
+ * returns.  A caller that alternates two frames overlaps the host work of
+ * one Camera.Render with the device work of the previous one.  Rendering
+ * into a frame that is still in flight waits for it first; its earlier
+ * results are then replaced. */
+t3d_status t3d_render_async(t3d_ctx* ctx, t3d_frame* frame, const t3d_render_desc* desc);
+/* Waits for the frame's last t3d_render_async and publishes its results
+ * (pairs, survivor counts).  T3D_OK at once when nothing is in flight. */
+t3d_status t3d_frame_wait(t3d_ctx* ctx, t3d_frame* frame);
+
+/* Number of render pairs of the last render and their results
+ * (T3D_ERR_NOT_READY while an asynchronous render is in flight). */
 t3d_status t3d_frame_pairs(const t3d_frame* frame, uint32_t* out_count, const t3d_pair_result** out_pairs);
 /* Device pointers of the frame's output arrays (valid until the next
  * render into this frame).  Pair p occupies vertices
@@ -283,7 +295,9 @@ t3d_status t3d_frame_pairs(const t3d_frame* frame, uint32_t* out_count, const t3
 t3d_status t3d_frame_device_outputs(const t3d_frame* frame, const t3d_vertex** color_vertices,
                                     const t3d_vertex** depth_vertices, const uint32_t** indices);
 /* Copy all pairs' outputs, concatenated in pair order, to host memory.
- * Buffers must hold 3*sum(survivor_count) entries each; any may be NULL. */
+ * Buffers must hold 3*sum(survivor_count) entries each; any may be NULL.
+ * These accessors return T3D_ERR_NOT_READY while an asynchronous render of
+ * the frame is in flight (t3d_frame_wait first). */
 t3d_status t3d_frame_download(t3d_ctx* ctx, const t3d_frame* frame, t3d_vertex* color_vertices,
                               t3d_vertex* depth_vertices, uint32_t* indices);
 /* Mesh triangle index of every drawn triangle, pair-concatenated in draw
@@ -313,12 +327,11 @@ typedef struct {
   uint32_t vertex_block;             /* threads per block, vertex kernel */
   uint32_t vertices_per_thread;
   uint32_t triangle_tile;            /* triangles per tile in the cull/sort kernels (64..2048) */
-  uint32_t small_part_triangles;     /* parts up to this size take the fused one-block path */
+  uint32_t scatter_waves;            /* wavefronts per tile in the sort's scatter pass: 1 or 4 */
   uint32_t use_graph;                /* replay the batched chain from a HIP graph */
-  uint32_t emit_variant;             /* 0: fused sort+assembly, LDS-staged runs; 1: fused, direct
-                                        stores; 2: sort, then assembly in output order */
-  uint32_t store_mode;               /* assembly stores (variant 2): bit 0 = direct per-lane stores
-                                        instead of LDS-staged rows, bit 1 = non-temporal stores */
+  uint32_t store_mode;               /* assembly stores: bit 0 = direct per-lane stores instead of
+                                        LDS-staged rows, bit 1 = non-temporal (streaming) stores */
+  uint32_t assemble_split;           /* assembly blocks per tile: a power of two, tile/split >= 64 */
 } t3d_tuning;
 t3d_status t3d_get_tuning(const t3d_ctx* ctx, t3d_tuning* out);
 t3d_status t3d_set_tuning(t3d_ctx* ctx, const t3d_tuning* tuning);
@@ -326,6 +339,70 @@ t3d_status t3d_set_tuning(t3d_ctx* ctx, const t3d_tuning* tuning);
  * winner to *out (may be NULL). */
 t3d_status t3d_autotune(t3d_ctx* ctx, t3d_frame* frame, const t3d_render_desc* desc, uint32_t reps,
                         t3d_tuning* out);
+
+/* ---- serving ----------------------------------------------------------- */
+
+/* A render-request loop over a context's resident meshes.  Jobs (one
+ * Camera.Render each: camera + a subset of models + lights) are admitted by
+ * an HBM budget and batched continuously: whenever one of `slots` launch
+ * slots is free, every admitted job waiting at that moment (up to
+ * max_batch_jobs) goes into one launch chain on that slot's HIP stream, so
+ * batches overlap on the device while new requests queue.  Every job's
+ * results are identical to rendering it alone with t3d_render.  All
+ * functions below are thread-safe. */
+typedef struct t3d_server t3d_server;
+typedef struct {
+  uint32_t slots;                    /* batches in flight at once, each on its own stream (>= 1) */
+  uint32_t max_batch_jobs;           /* jobs merged into one launch chain (>= 1) */
+  uint64_t hbm_budget_bytes;         /* frame memory in-flight jobs may hold; 0 = unbounded */
+  int32_t auto_release;              /* release a job's outputs as soon as it completes */
+} t3d_server_config;
+enum { T3D_JOB_QUEUED = 0, T3D_JOB_RUNNING = 1, T3D_JOB_DONE = 2, T3D_JOB_FAILED = 3, T3D_JOB_CANCELLED = 4 };
+typedef struct {
+  int32_t state;                     /* T3D_JOB_* */
+  t3d_status status;                 /* error of a failed job */
+  double submit_ms, start_ms, done_ms; /* steady-clock times since server creation */
+  uint32_t pair_count;
+  uint64_t triangles;                /* input triangles of the job's models */
+  uint64_t survivors;                /* triangles drawn */
+  uint64_t batch;                    /* batch the job ran in */
+  uint32_t batch_jobs;               /* jobs in that batch */
+} t3d_job_info;
+typedef struct {
+  uint64_t submitted, completed, failed, rejected, cancelled, batches;
+  uint64_t triangles;                /* input triangles of completed jobs */
+  uint32_t queue_depth, running_batches;
+  uint64_t memory_in_flight;         /* admitted bytes of queued-for-launch and running jobs */
+  double p50_ms, p99_ms;             /* submit -> outputs in HBM, completed jobs */
+  double first_submit_ms, last_done_ms;
+} t3d_server_stats;
+
+t3d_status t3d_server_create(t3d_ctx* ctx, const t3d_server_config* cfg, t3d_server** out);
+/* Waits for running batches, cancels queued jobs, frees everything. */
+t3d_status t3d_server_destroy(t3d_server* s);
+/* Copies the descriptor (and everything it points to); the caller may reuse
+ * its memory on return.  T3D_ERR_BUDGET if the job alone exceeds the budget. */
+t3d_status t3d_job_submit(t3d_server* s, const t3d_render_desc* desc, uint64_t* out_job);
+/* T3D_OK when done, T3D_ERR_NOT_READY while queued or running, the job's
+ * error when it failed, T3D_ERR_CANCELLED when cancelled. */
+t3d_status t3d_job_poll(t3d_server* s, uint64_t job);
+t3d_status t3d_job_wait(t3d_server* s, uint64_t job, uint32_t timeout_ms);
+t3d_status t3d_job_info_get(t3d_server* s, uint64_t job, t3d_job_info* out);
+/* Results of a completed job, as t3d_frame_pairs + t3d_frame_download(_triangles)
+ * give them for t3d_render (vertex offsets count from the job's first vertex).
+ * pairs holds pair_count entries; the arrays 3*survivors (tri_ids: survivors). */
+t3d_status t3d_job_download(t3d_server* s, uint64_t job, t3d_pair_result* pairs, t3d_vertex* color_vertices,
+                            t3d_vertex* depth_vertices, uint32_t* indices, uint32_t* triangle_ids);
+/* Frees a finished job's outputs (its batch's memory is reused once all of
+ * the batch's jobs are released). */
+t3d_status t3d_job_release(t3d_server* s, uint64_t job);
+/* Cancels a queued job; T3D_ERR_BUSY once it is running or finished. */
+t3d_status t3d_job_cancel(t3d_server* s, uint64_t job);
+/* Waits until no job is queued or running. */
+t3d_status t3d_server_drain(t3d_server* s, uint32_t timeout_ms);
+t3d_status t3d_server_stats_get(t3d_server* s, t3d_server_stats* out);
+/* Forget latency samples and counters (e.g. after warm-up). */
+t3d_status t3d_server_stats_reset(t3d_server* s);
 
 /* ---- self checks ------------------------------------------------------ */
 

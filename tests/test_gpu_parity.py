@@ -16,23 +16,67 @@ SCENES = {
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("variant", [(0, 0), (1, 0), (2, 0), (2, 1), (2, 2), (2, 3)],
-                         ids=["fused_staged", "fused_direct", "split_staged", "split_direct", "split_staged_nt",
-                              "split_direct_nt"])
+@pytest.mark.parametrize("store_mode", [0, 1, 2, 3], ids=["staged", "direct", "staged_nt", "direct_nt"])
 @pytest.mark.parametrize("mode", [A.EXEC_BATCHED, A.EXEC_PLAIN], ids=["batched", "plain"])
 @pytest.mark.parametrize("scene", list(SCENES))
-def test_scene_parity(gpu_ctx, oracle, scene, mode, variant):
+def test_scene_parity(gpu_ctx, oracle, scene, mode, store_mode):
     wl = SCENES[scene]()
-    print("parity", scene, "mode", mode, "variant", variant, "tris", wl.triangle_count, flush=True)
-    gpu_ctx.set_tuning(emit_variant=variant[0], store_mode=variant[1])
+    print("parity", scene, "mode", mode, "store_mode", store_mode, "tris", wl.triangle_count, flush=True)
+    gpu_ctx.set_tuning(store_mode=store_mode)
     try:
         g = gpu_ctx.render_workload(wl, exec_mode=mode)
     finally:
-        gpu_ctx.set_tuning(emit_variant=2, store_mode=2)
+        gpu_ctx.set_tuning(store_mode=2)
     o = oracle.render_workload(wl)
     s = compare_frames(g, o)
     print(s, flush=True)
     assert s["survivors"] > 0
+
+
+SHAPES = [  # (triangle_tile, scatter_waves, assemble_split)
+    (512, 1, 1), (512, 4, 8), (1024, 4, 1), (1024, 4, 4), (2048, 1, 2), (2048, 4, 32), (64, 4, 1),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape", SHAPES, ids=["t%d_w%d_s%d" % s for s in SHAPES])
+@pytest.mark.parametrize("scene", ["heightfield_128", "mixed_48"])
+def test_launch_shapes_parity(gpu_ctx, oracle, scene, shape):
+    """Every tile size, scatter width and assembly split gives the same frame."""
+    wl = SCENES[scene]()
+    tile, waves, split = shape
+    print("shape parity", scene, shape, flush=True)
+    saved = gpu_ctx.tuning()
+    gpu_ctx.set_tuning(triangle_tile=tile, scatter_waves=waves, assemble_split=split)
+    try:
+        g = gpu_ctx.render_workload(wl)
+    finally:
+        gpu_ctx.L.t3d_set_tuning(gpu_ctx.ctx, saved)
+    s = compare_frames(g, oracle.render_workload(wl))
+    assert s["survivors"] > 0
+
+
+@pytest.mark.gpu
+def test_async_frames_overlap_and_match(gpu_ctx, oracle):
+    """Two frames rendered asynchronously back to back (the second's host
+    work overlapping the first's kernels) give the synchronous results."""
+    from tetra3d_amd.runtime import T3DError
+    wa, wb = SCENES["mixed_48"](), SCENES["cubes"]()
+    gpu_ctx.prepare(wa)
+    gpu_ctx.prepare(wb)
+    ia, ib = gpu_ctx.inputs(wa.camera, wa.scene, wa.lights, wa.models), gpu_ctx.inputs(wb.camera, wb.scene, wb.lights, wb.models)
+    fa, fb = gpu_ctx.new_frame(), gpu_ctx.new_frame()
+    for rep in range(3):
+        print("async round", rep, flush=True)
+        gpu_ctx.render_async(ia, fa)
+        gpu_ctx.render_async(ib, fb)
+        with pytest.raises(T3DError) as e:
+            gpu_ctx.pairs(fb)                    # in flight until waited for
+        assert e.value.status == A.T3D_ERR_NOT_READY
+        gpu_ctx.wait(fa)
+        gpu_ctx.wait(fb)
+        compare_frames(gpu_ctx.download(frame=fa), oracle.render_workload(wa))
+        compare_frames(gpu_ctx.download(frame=fb), oracle.render_workload(wb))
 
 
 @pytest.mark.gpu

@@ -216,10 +216,10 @@ class Tuning(C.Structure):
         ("vertex_block", C.c_uint32),
         ("vertices_per_thread", C.c_uint32),
         ("triangle_tile", C.c_uint32),
-        ("small_part_triangles", C.c_uint32),
+        ("scatter_waves", C.c_uint32),
         ("use_graph", C.c_uint32),
-        ("emit_variant", C.c_uint32),
         ("store_mode", C.c_uint32),
+        ("assemble_split", C.c_uint32),
     ]
 
 

@@ -20,13 +20,11 @@
 #include <string>
 #include <vector>
 
-#include "t3d_internal.h"
+#include "t3d_runtime.h"
 
 using namespace t3d;
 
 namespace {
-
-constexpr float kMaxF32 = 3.40282346638528859811704183484516925440e+38f;
 
 std::string vfmt(const char* fmt, va_list ap) {
   char buf[1024];
@@ -144,88 +142,9 @@ void go_slice_stable(std::vector<int32_t>& d, Less less) {
   s.run();
 }
 
-// A device allocation accounted against the context budget.
-struct DevBuf {
-  void* ptr = nullptr;
-  uint64_t bytes = 0;
-};
-
-struct Mesh {
-  uint64_t handle = 0;
-  DevBuf buf;
-  const float* pos = nullptr;
-  const float* nrm = nullptr;
-  const float* col = nullptr;
-  const float* uv = nullptr;
-  const uint16_t* bones = nullptr;
-  const float* weights = nullptr;
-  const float* sk_pos = nullptr;
-  const float* sk_nrm = nullptr;
-  const uint32_t* idx = nullptr;
-  uint32_t V = 0, T = 0, K = 0;
-  std::vector<t3d_part_range> parts;
-  std::vector<uint32_t> part_vmin, part_vcount;
-  float dmin[3], dmax[3];
-  std::vector<uint16_t> max_bone;  // not used on the device; host check of palettes
-  uint32_t max_bone_index = 0;
-  bool has_skin = false;
-  int refs = 0;                    // in-flight jobs referencing the mesh
-};
-
-struct Volume {
-  uint64_t handle = 0;
-  DevBuf buf;
-  uint32_t n = 0;
-  std::vector<float> host;         // for per-object lookups on the host
-};
-
 }  // namespace
 
-struct t3d_frame {
-  // device tables and scratch
-  DevBuf tables, scratch, outputs;
-  uint64_t tables_cap = 0, scratch_cap = 0, outputs_cap = 0;
-  void* pinned = nullptr;          // host staging of the tables
-  uint64_t pinned_cap = 0;
-  uint32_t* pinned_stats = nullptr;
-  uint64_t pinned_stats_cap = 0;
-  FrameArgs args{};
-  // per-render bookkeeping
-  std::vector<t3d_pair_result> pairs;
-  std::vector<PartGPU> parts;
-  std::vector<uint32_t> part_first_tri;  // mesh triangle_start per pair
-  std::vector<uint64_t> bytes_vertex, bytes_cull, bytes_uv;  // algorithmic bytes per part
-  uint32_t nparts = 0;
-  uint32_t total_vchunks = 0, total_tiles = 0, total_sblocks = 0;
-  std::vector<uint32_t> vchunk_prefix, tile_prefix, sblock_prefix;
-  uint32_t vertex_threads = 256;
-  uint64_t total_survivors = 0;
-  double last_ms = 0;
-  // graph cache of the batched chain
-  hipGraphExec_t graph_exec = nullptr;
-  std::vector<uint8_t> graph_key;
-};
-
-struct t3d_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  uint64_t budget = 0, used = 0;
-  uint64_t next_handle = 1;
-  std::map<uint64_t, std::unique_ptr<Mesh>> meshes;
-  std::map<uint64_t, std::unique_ptr<Volume>> volumes;
-  std::string err;
-  t3d_tuning tuning{256, 1, 1024, 0, 1, 2, 2};
-  // profiling
-  bool profiling = false;
-  struct Prof {
-    double ms = 0;
-    uint32_t launches = 0;
-    uint64_t bytes = 0;
-  };
-  std::map<std::string, Prof> prof;
-};
-
-namespace {
+namespace t3d {
 
 t3d_status fail(t3d_ctx* c, t3d_status st, const char* fmt, ...) {
   va_list ap;
@@ -470,7 +389,8 @@ uint32_t t3d_struct_sizes(uint32_t* sizes, uint32_t max) {
                         (uint32_t)sizeof(t3d_model_desc),    (uint32_t)sizeof(t3d_camera_desc),
                         (uint32_t)sizeof(t3d_world_desc),    (uint32_t)sizeof(t3d_render_desc),
                         (uint32_t)sizeof(t3d_pair_result),   (uint32_t)sizeof(t3d_kernel_time),
-                        (uint32_t)sizeof(t3d_tuning)};
+                        (uint32_t)sizeof(t3d_tuning),        (uint32_t)sizeof(t3d_server_config),
+                        (uint32_t)sizeof(t3d_job_info),      (uint32_t)sizeof(t3d_server_stats)};
   uint32_t n = (uint32_t)(sizeof s / sizeof s[0]);
   for (uint32_t i = 0; i < n && i < max; i++) sizes[i] = s[i];
   return n;
@@ -490,6 +410,11 @@ t3d_status t3d_ctx_create(int32_t device, uint64_t hbm_budget_bytes, t3d_ctx** o
     delete c;
     return T3D_ERR_DEVICE;
   }
+  if (hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return T3D_ERR_DEVICE;
+  }
   *out = c;
   return T3D_OK;
 }
@@ -498,9 +423,11 @@ t3d_status t3d_ctx_destroy(t3d_ctx* c) {
   if (!c) return T3D_ERR_INVALID_ARGUMENT;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  (void)hipStreamSynchronize(c->copy_stream);
   for (auto& kv : c->meshes) dev_free(c, kv.second->buf);
   for (auto& kv : c->volumes) dev_free(c, kv.second->buf);
   (void)hipStreamDestroy(c->stream);
+  (void)hipStreamDestroy(c->copy_stream);
   delete c;
   return T3D_OK;
 }
@@ -510,6 +437,7 @@ uint64_t t3d_ctx_memory_used(const t3d_ctx* c) { return c ? c->used : 0; }
 
 t3d_status t3d_mesh_upload(t3d_ctx* c, const t3d_mesh_desc* d, uint64_t* out_handle) {
   if (!c || !d || !out_handle) return fail(c, T3D_ERR_INVALID_ARGUMENT, "mesh_upload: null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
   *out_handle = 0;
   if (d->vertex_count == 0 || !d->positions || !d->normals)          // Mesh.AddVertices panics on 0 vertices (mesh.go:680-682)
     return fail(c, T3D_ERR_INVALID_ARGUMENT, "mesh_upload: no vertices");
@@ -596,6 +524,7 @@ t3d_status t3d_mesh_upload(t3d_ctx* c, const t3d_mesh_desc* d, uint64_t* out_han
 
 t3d_status t3d_mesh_free(t3d_ctx* c, uint64_t handle) {
   if (!c) return T3D_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(c->mu);
   auto it = c->meshes.find(handle);
   if (it == c->meshes.end()) return fail(c, T3D_ERR_NOT_FOUND, "mesh_free: unknown handle %llu", (unsigned long long)handle);
   if (it->second->refs > 0) return fail(c, T3D_ERR_BUSY, "mesh_free: mesh %llu is referenced by in-flight work", (unsigned long long)handle);
@@ -607,6 +536,7 @@ t3d_status t3d_mesh_free(t3d_ctx* c, uint64_t handle) {
 
 t3d_status t3d_volume_upload(t3d_ctx* c, const float* cells, uint32_t n, uint64_t* out_handle) {
   if (!c || !out_handle || (!cells && n)) return fail(c, T3D_ERR_INVALID_ARGUMENT, "volume_upload: null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
   auto v = std::make_unique<Volume>();
   v->n = n;
   v->host.assign(cells, cells + 4 * (size_t)n);
@@ -621,6 +551,7 @@ t3d_status t3d_volume_upload(t3d_ctx* c, const float* cells, uint32_t n, uint64_
 
 t3d_status t3d_volume_free(t3d_ctx* c, uint64_t handle) {
   if (!c) return T3D_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(c->mu);
   auto it = c->volumes.find(handle);
   if (it == c->volumes.end()) return fail(c, T3D_ERR_NOT_FOUND, "volume_free: unknown handle");
   (void)hipStreamSynchronize(c->stream);
@@ -637,13 +568,9 @@ t3d_status t3d_frame_create(t3d_ctx* c, t3d_frame** out) {
 
 t3d_status t3d_frame_destroy(t3d_ctx* c, t3d_frame* f) {
   if (!c || !f) return T3D_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(c->mu);
   (void)hipStreamSynchronize(c->stream);
-  dev_free(c, f->tables);
-  dev_free(c, f->scratch);
-  dev_free(c, f->outputs);
-  if (f->pinned) (void)hipHostFree(f->pinned);
-  if (f->pinned_stats) (void)hipHostFree(f->pinned_stats);
-  if (f->graph_exec) (void)hipGraphExecDestroy(f->graph_exec);
+  frame_release(c, f);
   delete f;
   return T3D_OK;
 }
@@ -655,19 +582,22 @@ namespace {
 // Records a kernel launch between two events when profiling is on.
 struct ProfScope {
   t3d_ctx* c;
+  hipStream_t stream;
   const char* name;
   uint64_t bytes;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  ProfScope(t3d_ctx* c_, const char* n, uint64_t b) : c(c_), name(n), bytes(b) {
-    if (c->profiling) {
+  bool on;
+  ProfScope(t3d_ctx* c_, hipStream_t s, const char* n, uint64_t b, bool profile)
+      : c(c_), stream(s), name(n), bytes(b), on(profile && c_->profiling) {
+    if (on) {
       (void)hipEventCreate(&e0);
       (void)hipEventCreate(&e1);
-      (void)hipEventRecord(e0, c->stream);
+      (void)hipEventRecord(e0, stream);
     }
   }
   ~ProfScope() {
-    if (c->profiling) {
-      (void)hipEventRecord(e1, c->stream);
+    if (on) {
+      (void)hipEventRecord(e1, stream);
       (void)hipEventSynchronize(e1);
       float ms = 0;
       (void)hipEventElapsedTime(&ms, e0, e1);
@@ -686,12 +616,12 @@ struct ProfScope {
 // profile charges each kernel its algorithmic bytes: mesh streams read once
 // (k_vertex), triangle indices once (k_cull), outputs (k_assemble, added
 // after the survivor counts are known).
-t3d_status run_chain(t3d_ctx* c, t3d_frame* f, uint32_t p0, uint32_t p1) {
+t3d_status run_chain(t3d_ctx* c, t3d_frame* f, uint32_t p0, uint32_t p1, hipStream_t st, bool prof) {
   const FrameArgs& a = f->args;
   struct {
     uint64_t vertex = 0, cull = 0, uv = 0;
   } tr;
-  if (c->profiling)
+  if (prof && c->profiling)
     for (uint32_t i = p0; i < p1; i++) {
       tr.vertex += f->bytes_vertex[i];
       tr.cull += f->bytes_cull[i];
@@ -701,163 +631,47 @@ t3d_status run_chain(t3d_ctx* c, t3d_frame* f, uint32_t p0, uint32_t p1) {
   const uint32_t tb0 = f->tile_prefix[p0], tb1 = f->tile_prefix[p1];
   const uint32_t sb0 = f->sblock_prefix[p0], sb1 = f->sblock_prefix[p1];
   {
-    ProfScope ps(c, "k_vertex", tr.vertex);
-    HIP_TRY(c, launch_vertex(a, vb0, vb1 - vb0, f->vertex_threads, c->stream));
+    ProfScope ps(c, st, "k_vertex", tr.vertex, prof);
+    HIP_TRY(c, launch_vertex(a, vb0, vb1 - vb0, f->vertex_threads, st));
   }
   {
-    ProfScope ps(c, "k_cull", tr.cull);
-    HIP_TRY(c, launch_cull(a, tb0, tb1 - tb0, c->stream));
+    ProfScope ps(c, st, "k_cull", tr.cull, prof);
+    HIP_TRY(c, launch_cull(a, tb0, tb1 - tb0, st));
   }
   if (sb1 > sb0) {
     {
-      ProfScope ps(c, "k_hist", 0);
-      HIP_TRY(c, launch_hist(a, tb0, tb1 - tb0, c->stream));
+      ProfScope ps(c, st, "k_hist", 0, prof);
+      HIP_TRY(c, launch_hist(a, tb0, tb1 - tb0, st));
     }
-    ProfScope ps(c, "k_scan", 0);
-    HIP_TRY(c, launch_scan(a, sb0, sb1 - sb0, p0, p1 - p0, c->stream));
+    ProfScope ps(c, st, "k_scan", 0, prof);
+    HIP_TRY(c, launch_scan(a, sb0, sb1 - sb0, p0, p1 - p0, st));
   }
-  const uint32_t variant = c->tuning.emit_variant;
   {
-    ProfScope ps(c, "k_emit", variant == 2 ? 0 : tr.uv);
-    HIP_TRY(c, launch_emit(a, tb0, tb1 - tb0, variant, c->stream));
+    ProfScope ps(c, st, "k_scatter", 0, prof);
+    HIP_TRY(c, launch_scatter(a, tb0, tb1 - tb0, c->tuning.scatter_waves, st));
   }
-  if (variant == 2) {
-    ProfScope ps(c, "k_assemble", tr.uv);
-    HIP_TRY(c, launch_assemble(a, tb0, tb1 - tb0, c->tuning.store_mode, c->stream));
+  {
+    ProfScope ps(c, st, "k_assemble", tr.uv, prof);
+    HIP_TRY(c, launch_assemble(a, tb0, tb1 - tb0, c->tuning.store_mode, c->tuning.assemble_split, st));
   }
   return T3D_OK;
 }
 
 }  // namespace
 
-extern "C" t3d_status t3d_render(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc) {
+// ---------------------------------------------------------------------------
+// frame build / launch / finish
+// ---------------------------------------------------------------------------
+
+// Builds the frame tables of one or more Camera.Render calls (a batch of
+// jobs shares one launch chain; every render pair carries its job's camera
+// index).  Pure host work: nothing is sent to the device yet.
+namespace t3d {
+
+t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* descs, uint32_t ndesc) {
   auto wall0 = std::chrono::steady_clock::now();
-  if (!c || !f || !desc || !desc->camera) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: null argument");
-  const t3d_camera_desc* cd = desc->camera;
-  if (cd->width <= 0 || cd->height <= 0) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: camera size %dx%d", cd->width, cd->height);
-  if (cd->render_normals) return fail(c, T3D_ERR_UNSUPPORTED, "render: Camera.RenderNormals is outside the GPU path");
-  if (desc->light_count && !desc->lights) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: lights missing");
-  if (desc->model_count && !desc->models) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: models missing");
-  const t3d_world_desc* world = (desc->world && desc->world->present) ? desc->world : nullptr;
+  if (!c || !f || !descs || ndesc == 0) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: null argument");
   (void)hipSetDevice(c->device);
-
-  Cam cam;
-  cam_setup(cam, cd);
-
-  // scene light list (camera.go:1275-1303)
-  std::vector<SceneLight> scene_lights;
-  if (world && world->lighting_on && world->ambient_visible) {
-    SceneLight s{};
-    s.desc = nullptr;
-    s.ambient.type = T3D_LIGHT_AMBIENT;
-    s.ambient.visible = 1;
-    for (int i = 0; i < 3; i++) s.ambient.color[i] = world->ambient_color[i];
-    s.ambient.energy = world->ambient_energy;
-    scene_lights.push_back(s);
-  }
-  for (uint32_t i = 0; i < desc->light_count; i++) {
-    const t3d_light_desc& l = desc->lights[i];
-    if (l.type < T3D_LIGHT_AMBIENT || l.type > T3D_LIGHT_VOLUME)
-      return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: light %u has unknown type %d", i, l.type);
-    if (l.type == T3D_LIGHT_VOLUME && !c->volumes.count(l.volume))
-      return fail(c, T3D_ERR_NOT_FOUND, "render: light %u references unknown volume", i);
-    if ((!world || world->lighting_on) && l.visible) scene_lights.push_back(SceneLight{&l, {}});
-  }
-
-  // render pairs (camera.go:1343-1457)
-  std::vector<Pair> pairs;
-  std::vector<int32_t> solids, transparents;
-  for (uint32_t mi = 0; mi < desc->model_count; mi++) {
-    const t3d_model_desc& md = desc->models[mi];
-    if (!md.visible || md.mesh == 0) continue;
-    auto it = c->meshes.find(md.mesh);
-    if (it == c->meshes.end()) return fail(c, T3D_ERR_NOT_FOUND, "render: model %u references unknown mesh %llu", mi, (unsigned long long)md.mesh);
-    const Mesh& mesh = *it->second;
-    if (md.has_vertex_transform_function || md.has_vertex_clip_function)
-      return fail(c, T3D_ERR_UNSUPPORTED, "render: model %u uses a Go vertex callback", mi);
-    if (md.skinned) {
-      if (!mesh.has_skin || !md.bone_influences) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: skinned model %u without skin data", mi);
-      if (mesh.max_bone_index >= md.bone_count) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u bone index out of palette", mi);
-    }
-    if (md.light_group_active && md.light_group_count && !md.light_group)
-      return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u light group missing", mi);
-    if (md.frustum_culling) {
-      V3 sp;
-      float sr;
-      model_sphere(md.transform, md.skinned != 0, mesh.dmin, mesh.dmax, sp, sr);
-      if (!sphere_in_frustum(cam, sp, sr)) continue;
-    }
-    for (uint32_t p = 0; p < mesh.parts.size(); p++) {
-      const t3d_material_desc* mat = part_material(md, p);
-      if (mat && !mat->visible) continue;
-      Pair rp{mi, p, dist_sq(cam.pos, row3(md.transform)), mat ? mat->render_order : 0};
-      (is_transparent(md, mat) ? transparents : solids).push_back((int32_t)pairs.size());
-      pairs.push_back(rp);
-    }
-  }
-  if (!cd->render_depth)                                             // camera.go:1458-1467
-    go_slice_stable(solids, [&](int32_t a, int32_t b) {
-      if (pairs[a].order < pairs[b].order) return true;
-      return pairs[a].depth > pairs[b].depth;
-    });
-  if (cd->max_light_count > 0) {                                     // camera.go:1474-1497
-    V3 fp = cd->has_max_light_count_focus ? v3(cd->max_light_count_focus[0], cd->max_light_count_focus[1], cd->max_light_count_focus[2]) : cam.pos;
-    std::vector<int32_t> ord(scene_lights.size());
-    for (size_t i = 0; i < ord.size(); i++) ord[i] = (int32_t)i;
-    go_slice_stable(ord, [&](int32_t a, int32_t b) {
-      const t3d_light_desc& la = scene_lights[a].get();
-      const t3d_light_desc& lb = scene_lights[b].get();
-      if (la.type == T3D_LIGHT_AMBIENT) return true;
-      else if (lb.type == T3D_LIGHT_AMBIENT) return false;
-      else if (la.type == T3D_LIGHT_DIRECTIONAL) return true;
-      else if (lb.type == T3D_LIGHT_DIRECTIONAL) return false;
-      return dist_sq(fp, row3(la.transform)) < dist_sq(fp, row3(lb.transform));
-    });
-    std::vector<SceneLight> sorted;
-    for (int32_t i : ord) sorted.push_back(scene_lights[i]);
-    if ((int)sorted.size() > cd->max_light_count) sorted.resize(cd->max_light_count);
-    scene_lights = sorted;
-  }
-  go_slice_stable(transparents, [&](int32_t a, int32_t b) {           // camera.go:2050-2055
-    if (pairs[a].order == pairs[b].order) return pairs[a].depth > pairs[b].depth;
-    return pairs[a].order < pairs[b].order;
-  });
-  std::vector<int32_t> draw;                                          // camera.go:2057-2109
-  for (int32_t i : solids) draw.push_back(i);
-  for (int32_t i : transparents) draw.push_back(i);
-
-  // frame lights: scene lights first, then light-group lights as needed
-  std::vector<LightGPU> lights;
-  for (auto& s : scene_lights) lights.push_back(make_light(c, s.get()));
-  const uint32_t n_scene = (uint32_t)lights.size();
-  std::map<uint32_t, uint32_t> group_base;  // model -> first frame light of its group
-
-  // per part uniforms
-  FrameConst fc{};
-  fc.vp = cam.vp;
-  fc.campos[0] = cam.pos.x;
-  fc.campos[1] = cam.pos.y;
-  fc.campos[2] = cam.pos.z;
-  fc.near_ = cd->near;
-  fc.far_ = cd->far;
-  fc.cam_w = (float)cd->width;
-  fc.cam_h = (float)cd->height;
-  fc.half_w = (float)cd->width / 2;
-  fc.half_h = (float)cd->height / 2;
-  fc.dmp = (cd->far - cd->near) * cd->depth_margin;                   // camera.go:1339-1341
-  fc.spread = cd->far - cd->near + (fc.dmp * 2);
-  fc.snap = cd->vertex_snapping;
-  fc.perspective = cd->perspective;
-  fc.pc_tex = cd->perspective_corrected_texture_mapping;
-  fc.render_depth = cd->render_depth;
-  fc.fog_active = (!cd->render_depth && world && world->fog_on) ? 1 : 0;
-  fc.fog_mode = world ? world->fog_mode : 0;
-  if (world) {
-    for (int i = 0; i < 4; i++) fc.fog_color[i] = world->fog_color[i];
-    fc.fog_range[0] = world->fog_range[0];
-    fc.fog_range[1] = world->fog_range[1];
-  }
-
   const t3d_tuning& tu = c->tuning;
   const uint32_t vthreads = tu.vertex_block ? tu.vertex_block : 256;
   const uint32_t vchunk = vthreads * std::max<uint32_t>(1, tu.vertices_per_thread);
@@ -866,6 +680,9 @@ extern "C" t3d_status t3d_render(t3d_ctx* c, t3d_frame* f, const t3d_render_desc
   f->pairs.clear();
   f->parts.clear();
   f->part_first_tri.clear();
+  f->job_pair_begin.assign(1, 0);
+  f->fcs.clear();
+  std::vector<LightGPU> lights;
   std::vector<PartLight> plights;
   std::vector<const float*> palettes_host;   // host palettes to upload, per part (nullptr if none)
   std::vector<uint32_t> palette_count;
@@ -879,185 +696,323 @@ extern "C" t3d_status t3d_render(t3d_ctx* c, t3d_frame* f, const t3d_render_desc
   f->bytes_vertex.clear();
   f->bytes_cull.clear();
   f->bytes_uv.clear();
-  std::map<uint32_t, M4> rd_cache;
-  // Decompose(Inverted(M)) per model: shared by invertedCamPos (model.go:629-630)
-  // and every point light's workingPosition (light.go:175-189)
-  struct InvDec {
-    uint32_t model = UINT32_MAX;
-    V3 p, is;
-    M4 r;
-    V3 apply(V3 x) const { return add(mult_vec(r, x), hmul(p, is)); }
-  } inv;
 
-  for (int32_t di : draw) {
-    const Pair& rp = pairs[di];
-    const t3d_model_desc& md = desc->models[rp.model];
-    if (!md.visible || md.auto_batch_static) continue;               // camera.go:2067-2071
-    const Mesh& mesh = *c->meshes[md.mesh];
-    const t3d_material_desc* mat = part_material(md, rp.part);
-    if (mat && (mat->billboard || mat->custom_depth_function))
-      return fail(c, T3D_ERR_UNSUPPORTED, "render: model %u part %u uses billboarding or a custom depth function", rp.model, rp.part);
-    const t3d_part_range& pr = mesh.parts[rp.part];
-    bool lighting = false;                                           // camera.go:1521-1528
-    if (world) lighting = mat ? (world->lighting_on && !mat->shadeless && !md.shadeless) : (world->lighting_on && !md.shadeless);
-    const bool early_out = (md.color[3] == 0 && is_transparent(md, mat)) || !md.visible;   // model.go:604-606
+  for (uint32_t job = 0; job < ndesc; job++) {
+    const t3d_render_desc* desc = descs[job];
+    if (!desc || !desc->camera) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: null camera");
+    const t3d_camera_desc* cd = desc->camera;
+    if (cd->width <= 0 || cd->height <= 0) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: camera size %dx%d", cd->width, cd->height);
+    if (cd->render_normals) return fail(c, T3D_ERR_UNSUPPORTED, "render: Camera.RenderNormals is outside the GPU path");
+    if (desc->light_count && !desc->lights) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: lights missing");
+    if (desc->model_count && !desc->models) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: models missing");
+    const t3d_world_desc* world = (desc->world && desc->world->present) ? desc->world : nullptr;
 
-    PartGPU P{};
-    const M4& M = md.transform;
-    P.mvp = mult(M, cam.vp);                                          // model.go:695
-    P.model = M;
-    if (inv.model != rp.model) {
-      M4 im = inverted(M);
-      V3 s;
-      decompose(im, inv.p, s, inv.r);
-      inv.is = v3(fdiv(1.0f, s.x), fdiv(1.0f, s.y), fdiv(1.0f, s.z));
-      inv.model = rp.model;
+    Cam cam;
+    cam_setup(cam, cd);
+
+    // scene light list (camera.go:1275-1303)
+    std::vector<SceneLight> scene_lights;
+    if (world && world->lighting_on && world->ambient_visible) {
+      SceneLight sl{};
+      sl.desc = nullptr;
+      sl.ambient.type = T3D_LIGHT_AMBIENT;
+      sl.ambient.visible = 1;
+      for (int i = 0; i < 3; i++) sl.ambient.color[i] = world->ambient_color[i];
+      sl.ambient.energy = world->ambient_energy;
+      scene_lights.push_back(sl);
     }
-    V3 icp = inv.apply(cam.pos);                                      // model.go:629-630
-    P.icp[0] = icp.x;
-    P.icp[1] = icp.y;
-    P.icp[2] = icp.z;
-    float mp[4] = {md.color[0], md.color[1], md.color[2], md.color[3]};   // camera.go:1597-1601
-    if (mat)
-      for (int i = 0; i < 4; i++) mp[i] *= mat->color[i];
-    for (int i = 0; i < 4; i++) P.mp[i] = mp[i];
-    if (mat && mat->use_texture) {
-      P.src_w = mat->texture_size[0];
-      P.src_h = mat->texture_size[1];
+    for (uint32_t i = 0; i < desc->light_count; i++) {
+      const t3d_light_desc& l = desc->lights[i];
+      if (l.type < T3D_LIGHT_AMBIENT || l.type > T3D_LIGHT_VOLUME)
+        return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: light %u has unknown type %d", i, l.type);
+      if (l.type == T3D_LIGHT_VOLUME && !c->volumes.count(l.volume))
+        return fail(c, T3D_ERR_NOT_FOUND, "render: light %u references unknown volume", i);
+      if ((!world || world->lighting_on) && l.visible) scene_lights.push_back(SceneLight{&l, {}});
     }
-    uint32_t flags = 0;
-    if (md.skinned) flags |= PF_SKINNED;
-    if (mat && mat->backface_culling) flags |= PF_BACKFACE;
-    if (lighting) flags |= PF_LIGHTING;
-    if (mesh.col) flags |= PF_HAS_COLOR;
-    if (mesh.uv) flags |= PF_HAS_UV;
-    if (mesh.K) flags |= PF_SHAPE_KEYS;
-    const uint32_t sort_mode = mat ? (uint32_t)mat->triangle_sort_mode : T3D_SORT_BACK_TO_FRONT;
-    if (sort_mode > T3D_SORT_NONE) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: triangle sort mode %u", sort_mode);
-    flags |= sort_mode << PF_SORT_SHIFT;
-    flags |= ((uint32_t)(mat ? mat->lighting_mode : 0) & 3u) << PF_LMODE_SHIFT;
-    const uint32_t lv_mode = (uint32_t)(mat ? mat->light_volume_shading_mode : 0) & 3u;
-    flags |= lv_mode << PF_LVMODE_SHIFT;
-    P.flags = flags;
 
-    // lights of this part: beginModel (light.go:175-189, 326-330) and the
-    // per-model point-light skip (camera.go:1592, 1625-1629)
-    if (lighting) {
-      uint32_t lbase = 0, lcount = n_scene;
-      if (md.light_group_active) {
-        auto g = group_base.find(rp.model);
-        if (g == group_base.end()) {
-          uint32_t b = (uint32_t)lights.size();
-          for (uint32_t k = 0; k < md.light_group_count; k++) {
-            const t3d_light_desc& l = md.light_group[k];
-            if (l.type == T3D_LIGHT_VOLUME && !c->volumes.count(l.volume))
-              return fail(c, T3D_ERR_NOT_FOUND, "render: light group references unknown volume");
-            lights.push_back(make_light(c, l));
-          }
-          g = group_base.emplace(rp.model, b).first;
-        }
-        lbase = g->second;
-        lcount = md.light_group_count;
+    // render pairs (camera.go:1343-1457)
+    std::vector<Pair> pairs;
+    std::vector<int32_t> solids, transparents;
+    for (uint32_t mi = 0; mi < desc->model_count; mi++) {
+      const t3d_model_desc& md = desc->models[mi];
+      if (!md.visible || md.mesh == 0) continue;
+      auto it = c->meshes.find(md.mesh);
+      if (it == c->meshes.end())
+        return fail(c, T3D_ERR_NOT_FOUND, "render: model %u references unknown mesh %llu", mi, (unsigned long long)md.mesh);
+      const Mesh& mesh = *it->second;
+      if (md.has_vertex_transform_function || md.has_vertex_clip_function)
+        return fail(c, T3D_ERR_UNSUPPORTED, "render: model %u uses a Go vertex callback", mi);
+      if (md.skinned) {
+        if (!mesh.has_skin || !md.bone_influences) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: skinned model %u without skin data", mi);
+        if (mesh.max_bone_index >= md.bone_count) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u bone index out of palette", mi);
       }
-      if (lcount > kMaxLights) return fail(c, T3D_ERR_UNSUPPORTED, "render: %u lights on one model (max %u)", lcount, kMaxLights);
-      P.light_base = (uint32_t)plights.size();
-      P.light_count = lcount;
-      const float maxSpan = magnitude(sub(v3(mesh.dmax[0], mesh.dmax[1], mesh.dmax[2]), v3(mesh.dmin[0], mesh.dmin[1], mesh.dmin[2])));
-      const V3 modelPos = row3(M);
-      bool need_rd = false;
-      for (uint32_t k = 0; k < lcount; k++) {
-        const uint32_t li = lbase + k;
-        const t3d_light_desc& ld = (li < n_scene) ? scene_lights[li].get() : md.light_group[k];
-        PartLight pl{};
-        pl.light = li;
-        pl.enabled = 1;
-        if (ld.type == T3D_LIGHT_POINT) {
-          V3 wp = row3(ld.transform);
-          V3 w = md.skinned ? wp : inv.apply(wp);
-          pl.v[0] = w.x;
-          pl.v[1] = w.y;
-          pl.v[2] = w.z;
-          if (ld.range > 0) {
-            float dist = maxSpan + ld.range;
-            if (dist_sq(modelPos, wp) > dist * dist) pl.enabled = 0;
+      if (md.light_group_active && md.light_group_count && !md.light_group)
+        return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u light group missing", mi);
+      if (md.frustum_culling) {
+        V3 sp;
+        float sr;
+        model_sphere(md.transform, md.skinned != 0, mesh.dmin, mesh.dmax, sp, sr);
+        if (!sphere_in_frustum(cam, sp, sr)) continue;
+      }
+      for (uint32_t p = 0; p < mesh.parts.size(); p++) {
+        const t3d_material_desc* mat = part_material(md, p);
+        if (mat && !mat->visible) continue;
+        Pair rp{mi, p, dist_sq(cam.pos, row3(md.transform)), mat ? mat->render_order : 0};
+        (is_transparent(md, mat) ? transparents : solids).push_back((int32_t)pairs.size());
+        pairs.push_back(rp);
+      }
+    }
+    if (!cd->render_depth)                                             // camera.go:1458-1467
+      go_slice_stable(solids, [&](int32_t x, int32_t y) {
+        if (pairs[x].order < pairs[y].order) return true;
+        return pairs[x].depth > pairs[y].depth;
+      });
+    if (cd->max_light_count > 0) {                                     // camera.go:1474-1497
+      V3 fp = cd->has_max_light_count_focus
+                  ? v3(cd->max_light_count_focus[0], cd->max_light_count_focus[1], cd->max_light_count_focus[2])
+                  : cam.pos;
+      std::vector<int32_t> ord(scene_lights.size());
+      for (size_t i = 0; i < ord.size(); i++) ord[i] = (int32_t)i;
+      go_slice_stable(ord, [&](int32_t x, int32_t y) {
+        const t3d_light_desc& la = scene_lights[x].get();
+        const t3d_light_desc& lb = scene_lights[y].get();
+        if (la.type == T3D_LIGHT_AMBIENT) return true;
+        else if (lb.type == T3D_LIGHT_AMBIENT) return false;
+        else if (la.type == T3D_LIGHT_DIRECTIONAL) return true;
+        else if (lb.type == T3D_LIGHT_DIRECTIONAL) return false;
+        return dist_sq(fp, row3(la.transform)) < dist_sq(fp, row3(lb.transform));
+      });
+      std::vector<SceneLight> sorted;
+      for (int32_t i : ord) sorted.push_back(scene_lights[i]);
+      if ((int)sorted.size() > cd->max_light_count) sorted.resize(cd->max_light_count);
+      scene_lights = sorted;
+    }
+    go_slice_stable(transparents, [&](int32_t x, int32_t y) {           // camera.go:2050-2055
+      if (pairs[x].order == pairs[y].order) return pairs[x].depth > pairs[y].depth;
+      return pairs[x].order < pairs[y].order;
+    });
+    std::vector<int32_t> draw;                                          // camera.go:2057-2109
+    draw.reserve(solids.size() + transparents.size());
+    for (int32_t i : solids) draw.push_back(i);
+    for (int32_t i : transparents) draw.push_back(i);
+
+    // this job's lights: scene lights first, then light-group lights as needed
+    const uint32_t lbase_scene = (uint32_t)lights.size();
+    for (auto& sl : scene_lights) lights.push_back(make_light(c, sl.get()));
+    const uint32_t n_scene = (uint32_t)scene_lights.size();
+    std::map<uint32_t, uint32_t> group_base;  // model -> first frame light of its group
+
+    // camera constants (camera.go:1329-1341, 1591-1606)
+    FrameConst fc{};
+    fc.vp = cam.vp;
+    fc.campos[0] = cam.pos.x;
+    fc.campos[1] = cam.pos.y;
+    fc.campos[2] = cam.pos.z;
+    fc.near_ = cd->near;
+    fc.far_ = cd->far;
+    fc.cam_w = (float)cd->width;
+    fc.cam_h = (float)cd->height;
+    fc.half_w = (float)cd->width / 2;
+    fc.half_h = (float)cd->height / 2;
+    fc.dmp = (cd->far - cd->near) * cd->depth_margin;
+    fc.spread = cd->far - cd->near + (fc.dmp * 2);
+    fc.snap = cd->vertex_snapping;
+    fc.perspective = cd->perspective;
+    fc.pc_tex = cd->perspective_corrected_texture_mapping;
+    fc.render_depth = cd->render_depth;
+    fc.fog_active = (!cd->render_depth && world && world->fog_on) ? 1 : 0;
+    fc.fog_mode = world ? world->fog_mode : 0;
+    if (world) {
+      for (int i = 0; i < 4; i++) fc.fog_color[i] = world->fog_color[i];
+      fc.fog_range[0] = world->fog_range[0];
+      fc.fog_range[1] = world->fog_range[1];
+    }
+    f->fcs.push_back(fc);
+
+    std::map<uint32_t, M4> rd_cache;
+    // Decompose(Inverted(M)) per model: shared by invertedCamPos (model.go:629-630)
+    // and every point light's workingPosition (light.go:175-189)
+    struct InvDec {
+      uint32_t model = UINT32_MAX;
+      V3 p, is;
+      M4 r;
+      V3 apply(V3 x) const { return add(mult_vec(r, x), hmul(p, is)); }
+    } inv;
+
+    for (int32_t di : draw) {
+      const Pair& rp = pairs[di];
+      const t3d_model_desc& md = desc->models[rp.model];
+      if (!md.visible || md.auto_batch_static) continue;               // camera.go:2067-2071
+      const Mesh& mesh = *c->meshes[md.mesh];
+      const t3d_material_desc* mat = part_material(md, rp.part);
+      if (mat && (mat->billboard || mat->custom_depth_function))
+        return fail(c, T3D_ERR_UNSUPPORTED, "render: model %u part %u uses billboarding or a custom depth function", rp.model, rp.part);
+      const t3d_part_range& pr = mesh.parts[rp.part];
+      bool lighting = false;                                           // camera.go:1521-1528
+      if (world) lighting = mat ? (world->lighting_on && !mat->shadeless && !md.shadeless) : (world->lighting_on && !md.shadeless);
+      const bool early_out = (md.color[3] == 0 && is_transparent(md, mat)) || !md.visible;   // model.go:604-606
+
+      PartGPU P{};
+      const M4& M = md.transform;
+      P.mvp = mult(M, cam.vp);                                          // model.go:695
+      P.model = M;
+      P.cam = job;
+      if (inv.model != rp.model) {
+        M4 im = inverted(M);
+        V3 sc;
+        decompose(im, inv.p, sc, inv.r);
+        inv.is = v3(fdiv(1.0f, sc.x), fdiv(1.0f, sc.y), fdiv(1.0f, sc.z));
+        inv.model = rp.model;
+      }
+      V3 icp = inv.apply(cam.pos);                                      // model.go:629-630
+      P.icp[0] = icp.x;
+      P.icp[1] = icp.y;
+      P.icp[2] = icp.z;
+      float mp[4] = {md.color[0], md.color[1], md.color[2], md.color[3]};   // camera.go:1597-1601
+      if (mat)
+        for (int i = 0; i < 4; i++) mp[i] *= mat->color[i];
+      for (int i = 0; i < 4; i++) P.mp[i] = mp[i];
+      if (mat && mat->use_texture) {
+        P.src_w = mat->texture_size[0];
+        P.src_h = mat->texture_size[1];
+      }
+      uint32_t flags = 0;
+      if (md.skinned) flags |= PF_SKINNED;
+      if (mat && mat->backface_culling) flags |= PF_BACKFACE;
+      if (lighting) flags |= PF_LIGHTING;
+      if (mesh.col) flags |= PF_HAS_COLOR;
+      if (mesh.uv) flags |= PF_HAS_UV;
+      if (mesh.K) flags |= PF_SHAPE_KEYS;
+      const uint32_t sort_mode = mat ? (uint32_t)mat->triangle_sort_mode : T3D_SORT_BACK_TO_FRONT;
+      if (sort_mode > T3D_SORT_NONE) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: triangle sort mode %u", sort_mode);
+      flags |= sort_mode << PF_SORT_SHIFT;
+      flags |= ((uint32_t)(mat ? mat->lighting_mode : 0) & 3u) << PF_LMODE_SHIFT;
+      const uint32_t lv_mode = (uint32_t)(mat ? mat->light_volume_shading_mode : 0) & 3u;
+      flags |= lv_mode << PF_LVMODE_SHIFT;
+      P.flags = flags;
+
+      // lights of this part: beginModel (light.go:175-189, 326-330) and the
+      // per-model point-light skip (camera.go:1592, 1625-1629)
+      if (lighting) {
+        uint32_t lbase = lbase_scene, lcount = n_scene;
+        if (md.light_group_active) {
+          auto g = group_base.find(rp.model);
+          if (g == group_base.end()) {
+            uint32_t gb = (uint32_t)lights.size();
+            for (uint32_t k = 0; k < md.light_group_count; k++) {
+              const t3d_light_desc& l = md.light_group[k];
+              if (l.type == T3D_LIGHT_VOLUME && !c->volumes.count(l.volume))
+                return fail(c, T3D_ERR_NOT_FOUND, "render: light group references unknown volume");
+              lights.push_back(make_light(c, l));
+            }
+            g = group_base.emplace(rp.model, gb).first;
           }
-        } else if (ld.type == T3D_LIGHT_DIRECTIONAL) {
-          need_rd = true;
-        } else if (ld.type == T3D_LIGHT_VOLUME && lv_mode == T3D_LV_PER_OBJECT) {
-          const Volume& vol = *c->volumes[ld.volume];
-          V3 p = modelPos;
-          int64_t x = go_int(round32((p.x - ld.offset[0] - ld.volume_min[0] - (ld.cell_size[0] / 2)) / ld.cell_size[0]));
-          int64_t y = go_int(round32((p.y - ld.offset[1] - ld.volume_min[1] - (ld.cell_size[1] / 2)) / ld.cell_size[1]));
-          int64_t z = go_int(round32((p.z - ld.offset[2] - ld.volume_min[2] - (ld.cell_size[2] / 2)) / ld.cell_size[2]));
-          const int cx = ld.cell_count[0], cy = ld.cell_count[1], cz = ld.cell_count[2];
-          if (!(x < 0 || y < 0 || z < 0 || x >= cx || y >= cy || z >= cz)) {
-            int64_t ci = x + (y * cx) + (z * cx * cy);
-            if (ci < (int64_t)vol.n) {
-              pl.cell_found = 1;
-              for (int q = 0; q < 3; q++) pl.v[q] = vol.host[4 * ci + q];
+          lbase = g->second;
+          lcount = md.light_group_count;
+        }
+        if (lcount > kMaxLights) return fail(c, T3D_ERR_UNSUPPORTED, "render: %u lights on one model (max %u)", lcount, kMaxLights);
+        P.light_base = (uint32_t)plights.size();
+        P.light_count = lcount;
+        const float maxSpan = magnitude(sub(v3(mesh.dmax[0], mesh.dmax[1], mesh.dmax[2]), v3(mesh.dmin[0], mesh.dmin[1], mesh.dmin[2])));
+        const V3 modelPos = row3(M);
+        bool need_rd = false;
+        for (uint32_t k = 0; k < lcount; k++) {
+          const uint32_t li = lbase + k;
+          const t3d_light_desc& ld = md.light_group_active ? md.light_group[k] : scene_lights[k].get();
+          PartLight pl{};
+          pl.light = li;
+          pl.enabled = 1;
+          if (ld.type == T3D_LIGHT_POINT) {
+            V3 wp = row3(ld.transform);
+            V3 w = md.skinned ? wp : inv.apply(wp);
+            pl.v[0] = w.x;
+            pl.v[1] = w.y;
+            pl.v[2] = w.z;
+            if (ld.range > 0) {
+              float dist = maxSpan + ld.range;
+              if (dist_sq(modelPos, wp) > dist * dist) pl.enabled = 0;
+            }
+          } else if (ld.type == T3D_LIGHT_DIRECTIONAL) {
+            need_rd = true;
+          } else if (ld.type == T3D_LIGHT_VOLUME && lv_mode == T3D_LV_PER_OBJECT) {
+            const Volume& vol = *c->volumes[ld.volume];
+            V3 q = modelPos;
+            int64_t x = go_int(round32((q.x - ld.offset[0] - ld.volume_min[0] - (ld.cell_size[0] / 2)) / ld.cell_size[0]));
+            int64_t y = go_int(round32((q.y - ld.offset[1] - ld.volume_min[1] - (ld.cell_size[1] / 2)) / ld.cell_size[1]));
+            int64_t z = go_int(round32((q.z - ld.offset[2] - ld.volume_min[2] - (ld.cell_size[2] / 2)) / ld.cell_size[2]));
+            const int cx = ld.cell_count[0], cy = ld.cell_count[1], cz = ld.cell_count[2];
+            if (!(x < 0 || y < 0 || z < 0 || x >= cx || y >= cy || z >= cz)) {
+              int64_t ci = x + (y * cx) + (z * cx * cy);
+              if (ci < (int64_t)vol.n) {
+                pl.cell_found = 1;
+                for (int qq = 0; qq < 3; qq++) pl.v[qq] = vol.host[4 * ci + qq];
+              }
             }
           }
+          plights.push_back(pl);
         }
-        plights.push_back(pl);
+        if (need_rd && !md.skinned) {
+          auto r = rd_cache.find(rp.model);
+          if (r == rd_cache.end()) r = rd_cache.emplace(rp.model, transposed(inverted(md.world_rotation))).first;
+          P.rd = r->second;
+        }
       }
-      if (need_rd && !md.skinned) {
-        auto r = rd_cache.find(rp.model);
-        if (r == rd_cache.end()) r = rd_cache.emplace(rp.model, transposed(inverted(md.world_rotation))).first;
-        P.rd = r->second;
-      }
-    }
 
-    // mesh arrays and frame slices
-    P.pos = mesh.pos;
-    P.nrm = mesh.nrm;
-    P.col = mesh.col;
-    P.uv = mesh.uv;
-    P.bones = mesh.bones;
-    P.weights = mesh.weights;
-    P.idx = mesh.idx + 3 * (size_t)pr.triangle_start;
-    P.sk_pos = mesh.sk_pos;
-    P.sk_nrm = mesh.sk_nrm;
-    P.shape_keys = mesh.K;
-    P.mesh_vcount = mesh.V;
-    P.bone_count = md.skinned ? md.bone_count : 0;
-    P.vmin = mesh.part_vmin[rp.part];
-    P.vcount = early_out ? 0 : mesh.part_vcount[rp.part];
-    P.tcount = early_out ? 0 : pr.triangle_count;
-    P.vbase = (uint32_t)sumV;
-    P.tbase = (uint32_t)sumT;
-    P.out_base = 3 * sumT;
-    const uint32_t nvchunks = (P.vcount + vchunk - 1) / vchunk;
-    const uint32_t ntiles = (P.tcount + tile - 1) / tile;
-    P.ntiles = ntiles;
-    if (ntiles > 1) {
-      P.hbase = (uint32_t)sumHist;
-      P.sblock0 = sblocks;
-      P.nsblocks = (kBins * ntiles + kScanBlock - 1) / kScanBlock;
-      sumHist += (uint64_t)kBins * ntiles;
-      sblocks += P.nsblocks;
+      // mesh arrays and frame slices
+      P.pos = mesh.pos;
+      P.nrm = mesh.nrm;
+      P.col = mesh.col;
+      P.uv = mesh.uv;
+      P.bones = mesh.bones;
+      P.weights = mesh.weights;
+      P.idx = mesh.idx + 3 * (size_t)pr.triangle_start;
+      P.sk_pos = mesh.sk_pos;
+      P.sk_nrm = mesh.sk_nrm;
+      P.shape_keys = mesh.K;
+      P.mesh_vcount = mesh.V;
+      P.bone_count = md.skinned ? md.bone_count : 0;
+      P.vmin = mesh.part_vmin[rp.part];
+      P.vcount = early_out ? 0 : mesh.part_vcount[rp.part];
+      P.tcount = early_out ? 0 : pr.triangle_count;
+      P.vbase = (uint32_t)sumV;
+      P.tbase = (uint32_t)sumT;
+      P.out_base = 3 * sumT;
+      const uint32_t nvchunks = (P.vcount + vchunk - 1) / vchunk;
+      const uint32_t ntiles = (P.tcount + tile - 1) / tile;
+      P.ntiles = ntiles;
+      if (ntiles > 1) {
+        P.hbase = (uint32_t)sumHist;
+        P.sblock0 = sblocks;
+        P.nsblocks = (kBins * ntiles + kScanBlock - 1) / kScanBlock;
+        sumHist += (uint64_t)kBins * ntiles;
+        sblocks += P.nsblocks;
+      }
+      sumV += P.vcount;
+      sumT += P.tcount;
+      if (sumV > 0xFFFFFFFFull || sumT > 0x3FFFFFFFull)
+        return fail(c, T3D_ERR_UNSUPPORTED, "render: frame exceeds 2^32 vertices or 2^30 triangles");
+      f->vchunk_prefix.push_back(f->vchunk_prefix.back() + nvchunks);
+      f->tile_prefix.push_back(f->tile_prefix.back() + ntiles);
+      f->sblock_prefix.push_back(sblocks);
+      palettes_host.push_back(md.skinned ? &md.bone_influences[0].m[0][0] : nullptr);
+      palette_count.push_back(md.skinned ? md.bone_count : 0);
+      skw_host.push_back(mesh.K ? md.shape_key_weights : nullptr);
+      skw_count.push_back(mesh.K);
+      // algorithmic bytes: mesh streams read once, triangle indices once, outputs
+      const uint64_t vb = 24 + (mesh.col ? 16 : 0) + (md.skinned ? 24 : 0);
+      f->bytes_vertex.push_back((uint64_t)P.vcount * vb + (md.skinned ? 64ull * md.bone_count : 0) + sizeof(PartGPU));
+      f->bytes_cull.push_back(12ull * P.tcount);
+      f->bytes_uv.push_back(mesh.uv ? 8ull * P.vcount : 0);   // UVs are read by the assembly
+      f->parts.push_back(P);
+      f->part_first_tri.push_back(pr.triangle_start);
+      t3d_pair_result r{};
+      r.model_index = rp.model;
+      r.part_index = rp.part;
+      r.triangle_count = pr.triangle_count;
+      r.vertex_offset = P.out_base;
+      f->pairs.push_back(r);
     }
-    sumV += P.vcount;
-    sumT += P.tcount;
-    f->vchunk_prefix.push_back(f->vchunk_prefix.back() + nvchunks);
-    f->tile_prefix.push_back(f->tile_prefix.back() + ntiles);
-    f->sblock_prefix.push_back(sblocks);
-    palettes_host.push_back(md.skinned ? &md.bone_influences[0].m[0][0] : nullptr);
-    palette_count.push_back(md.skinned ? md.bone_count : 0);
-    skw_host.push_back(mesh.K ? md.shape_key_weights : nullptr);
-    skw_count.push_back(mesh.K);
-    // algorithmic bytes: mesh streams read once, triangle indices once, outputs
-    const uint64_t vb = 24 + (mesh.col ? 16 : 0) + (md.skinned ? 24 : 0);
-    f->bytes_vertex.push_back((uint64_t)P.vcount * vb + (md.skinned ? 64ull * md.bone_count : 0) + sizeof(PartGPU));
-    f->bytes_cull.push_back(12ull * P.tcount);
-    f->bytes_uv.push_back(mesh.uv ? 8ull * P.vcount : 0);   // UVs are read by the assembly
-    f->parts.push_back(P);
-    f->part_first_tri.push_back(pr.triangle_start);
-    t3d_pair_result r{};
-    r.model_index = rp.model;
-    r.part_index = rp.part;
-    r.triangle_count = pr.triangle_count;
-    r.vertex_offset = P.out_base;
-    f->pairs.push_back(r);
+    f->job_pair_begin.push_back((uint32_t)f->pairs.size());
   }
   const uint32_t P_n = (uint32_t)f->parts.size();
   f->nparts = P_n;
@@ -1069,8 +1024,8 @@ extern "C" t3d_status t3d_render(t3d_ctx* c, t3d_frame* f, const t3d_render_desc
     skw_floats += skw_count[i];
   }
 
-  // device tables: FrameConst, parts, lights, plights, prefixes, palettes, shape-key weights
-  const uint64_t tbytes = carve_size<FrameConst>(1) + carve_size<PartGPU>(P_n) + carve_size<LightGPU>(lights.size()) +
+  // device tables: FrameConst[jobs], parts, lights, plights, prefixes, palettes, shape-key weights
+  const uint64_t tbytes = carve_size<FrameConst>(ndesc) + carve_size<PartGPU>(P_n) + carve_size<LightGPU>(lights.size()) +
                           carve_size<PartLight>(plights.size()) + 3 * carve_size<uint32_t>(P_n + 1) +
                           carve_size<float>(pal_floats) + carve_size<float>(skw_floats);
   t3d_status st = ensure(c, f->tables, f->tables_cap, tbytes);
@@ -1081,20 +1036,28 @@ extern "C" t3d_status t3d_render(t3d_ctx* c, t3d_frame* f, const t3d_render_desc
     f->pinned_cap = align_up(tbytes + tbytes / 4, 1 << 16);
     HIP_TRY(c, hipHostMalloc(&f->pinned, f->pinned_cap, hipHostMallocDefault));
   }
-  // scratch: vertex streams, keys, sorted ids, stats, histograms, block sums
-  const uint64_t sbytes = carve_size<float4>(2 * sumV) + carve_size<float4>(sumV) + 2 * carve_size<uint32_t>(sumT) + carve_size<uint4>(sumT) + carve_size<uint32_t>(4ull * P_n) +
-                          carve_size<uint32_t>(sumHist) + carve_size<uint32_t>(sblocks);
+  // scratch: vertex records, skinned positions, keys, survivor records, stats, histograms, block sums
+  const uint64_t sbytes = carve_size<float4>(2 * sumV) + carve_size<float4>(sumV) + carve_size<uint32_t>(sumT) +
+                          2 * carve_size<uint4>(sumT) + carve_size<uint32_t>(4ull * P_n) + carve_size<uint32_t>(sumHist) +
+                          carve_size<uint32_t>(sblocks);
   st = ensure(c, f->scratch, f->scratch_cap, sbytes);
   if (st) return st;
   const uint64_t obytes = 2 * carve_size<t3d_vertex>(3 * sumT) + carve_size<uint32_t>(3 * sumT);
   st = ensure(c, f->outputs, f->outputs_cap, obytes);
   if (st) return st;
+  if (4ull * P_n > f->pinned_stats_cap) {
+    if (f->pinned_stats) (void)hipHostFree(f->pinned_stats);
+    f->pinned_stats = nullptr;
+    f->pinned_stats_cap = align_up(4ull * P_n + 1024, 1024);
+    HIP_TRY(c, hipHostMalloc((void**)&f->pinned_stats, f->pinned_stats_cap * 4, hipHostMallocDefault));
+  }
 
   // fill the pinned staging block with the same carve layout as the device block
   Carve dev{reinterpret_cast<uint8_t*>(f->tables.ptr)};
   Carve host{reinterpret_cast<uint8_t*>(f->pinned)};
-  FrameConst* d_fc = dev.take<FrameConst>(1);
-  *host.take<FrameConst>(1) = fc;
+  FrameConst* d_fc = dev.take<FrameConst>(ndesc);
+  FrameConst* h_fc = host.take<FrameConst>(ndesc);
+  std::memcpy(h_fc, f->fcs.data(), sizeof(FrameConst) * ndesc);
   PartGPU* d_parts = dev.take<PartGPU>(P_n);
   PartGPU* h_parts = host.take<PartGPU>(P_n);
   LightGPU* d_lights = dev.take<LightGPU>(lights.size());
@@ -1131,6 +1094,7 @@ extern "C" t3d_status t3d_render(t3d_ctx* c, t3d_frame* f, const t3d_render_desc
   std::memcpy(h_vpre, f->vchunk_prefix.data(), 4 * (P_n + 1));
   std::memcpy(h_tpre, f->tile_prefix.data(), 4 * (P_n + 1));
   std::memcpy(h_spre, f->sblock_prefix.data(), 4 * (P_n + 1));
+  f->table_bytes = host.off;
 
   Carve sc{reinterpret_cast<uint8_t*>(f->scratch.ptr)};
   Carve oc{reinterpret_cast<uint8_t*>(f->outputs.ptr)};
@@ -1147,7 +1111,7 @@ extern "C" t3d_status t3d_render(t3d_ctx* c, t3d_frame* f, const t3d_render_desc
   a.vrec = sc.take<float4>(2 * sumV);
   a.vpos = sc.take<float4>(sumV);
   a.key = sc.take<float>(sumT);
-  a.sorted = sc.take<uint32_t>(sumT);
+  a.tri_rec = sc.take<uint4>(sumT);
   a.tri_vo = sc.take<uint4>(sumT);
   a.stats = sc.take<uint32_t>(4ull * P_n);
   a.hist = sc.take<uint32_t>(sumHist);
@@ -1160,84 +1124,209 @@ extern "C" t3d_status t3d_render(t3d_ctx* c, t3d_frame* f, const t3d_render_desc
   f->total_vchunks = f->vchunk_prefix.back();
   f->total_tiles = f->tile_prefix.back();
   f->total_sblocks = sblocks;
-
-  if (c->profiling) {
+  f->total_triangles = sumT;
+  if (c->profiling) {   // the caller holds ctx->mu
     auto& hp = c->prof["host:prologue"];
     hp.ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
     hp.launches++;
   }
-  HIP_TRY(c, hipMemcpyAsync(f->tables.ptr, f->pinned, host.off, hipMemcpyHostToDevice, c->stream));
+  return T3D_OK;
+}
 
-  if (4ull * P_n > f->pinned_stats_cap) {
-    if (f->pinned_stats) (void)hipHostFree(f->pinned_stats);
-    f->pinned_stats = nullptr;
-    f->pinned_stats_cap = align_up(4ull * P_n + 1024, 1024);
-    HIP_TRY(c, hipHostMalloc((void**)&f->pinned_stats, f->pinned_stats_cap * 4, hipHostMallocDefault));
+// Sends the tables and enqueues the chain (graph replay when allowed) and the
+// survivor-count readback on `stream`; returns without waiting.
+t3d_status frame_launch(t3d_ctx* c, t3d_frame* f, hipStream_t stream, hipStream_t copy_stream, int32_t exec_mode,
+                        bool allow_graph, bool profile) {
+  const uint32_t P_n = f->nparts;
+  const FrameArgs& a = f->args;
+  if (!f->done) HIP_TRY(c, hipEventCreateWithFlags(&f->done, hipEventDisableTiming));
+  if (!f->tables_ready) HIP_TRY(c, hipEventCreateWithFlags(&f->tables_ready, hipEventDisableTiming));
+  HIP_TRY(c, hipMemcpyAsync(f->tables.ptr, f->pinned, f->table_bytes, hipMemcpyHostToDevice, copy_stream));
+  if (copy_stream != stream) {
+    HIP_TRY(c, hipEventRecord(f->tables_ready, copy_stream));
+    HIP_TRY(c, hipStreamWaitEvent(stream, f->tables_ready, 0));
   }
-
-  // output bytes per survivor are known only after the cull; the profile
-  // attributes them to k_emit after readback (below)
-  if (P_n) {
-    {
-      ProfScope ps(c, "k_init_stats", 0);
-      HIP_TRY(c, launch_init_stats(a, P_n, c->stream));
-    }
-    if (desc->exec_mode == T3D_EXEC_PLAIN) {
-      for (uint32_t i = 0; i < P_n; i++) {
-        st = run_chain(c, f, i, i + 1);
-        if (st) return st;
-      }
-    } else if (tu.use_graph && !c->profiling) {
-      // graph of the whole chain; reused while the launch shape is unchanged
-      std::vector<uint8_t> key(sizeof(FrameArgs) + 4 * sizeof(uint32_t));
-      uint32_t shape[4] = {f->total_vchunks, f->total_tiles, f->total_sblocks, vthreads};
-      std::memcpy(key.data(), &a, sizeof(FrameArgs));
-      std::memcpy(key.data() + sizeof(FrameArgs), shape, sizeof shape);
-      if (!f->graph_exec || key != f->graph_key) {
-        if (f->graph_exec) {
-          (void)hipGraphExecDestroy(f->graph_exec);
-          f->graph_exec = nullptr;
-        }
-        hipGraph_t g = nullptr;
-        HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        st = run_chain(c, f, 0, P_n);
-        hipError_t ce = hipStreamEndCapture(c->stream, &g);
-        if (st) {
-          if (g) (void)hipGraphDestroy(g);
-          return st;
-        }
-        if (ce != hipSuccess) return fail(c, T3D_ERR_DEVICE, "graph capture: %s", hipGetErrorString(ce));
-        hipError_t ie = hipGraphInstantiate(&f->graph_exec, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (ie != hipSuccess) return fail(c, T3D_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(ie));
-        f->graph_key = key;
-      }
-      HIP_TRY(c, hipGraphLaunch(f->graph_exec, c->stream));
-    } else {
-      st = run_chain(c, f, 0, P_n);
+  if (!P_n) {
+    HIP_TRY(c, hipEventRecord(f->done, stream));
+    return T3D_OK;
+  }
+  {
+    ProfScope ps(c, stream, "k_init_stats", 0, profile);
+    HIP_TRY(c, launch_init_stats(a, P_n, stream));
+  }
+  t3d_status st = T3D_OK;
+  if (exec_mode == T3D_EXEC_PLAIN) {
+    for (uint32_t i = 0; i < P_n; i++) {
+      st = run_chain(c, f, i, i + 1, stream, profile);
       if (st) return st;
     }
-    HIP_TRY(c, hipMemcpyAsync(f->pinned_stats, a.stats, 16ull * P_n, hipMemcpyDeviceToHost, c->stream));
+  } else if (allow_graph && c->tuning.use_graph && !(profile && c->profiling)) {
+    // graph of the whole chain; reused while the launch shape is unchanged
+    std::vector<uint8_t> key(sizeof(FrameArgs) + 7 * sizeof(uint32_t));
+    uint32_t shape[7] = {f->total_vchunks, f->total_tiles,    f->total_sblocks,          f->vertex_threads,
+                         c->tuning.store_mode, c->tuning.scatter_waves, c->tuning.assemble_split};
+    std::memcpy(key.data(), &a, sizeof(FrameArgs));
+    std::memcpy(key.data() + sizeof(FrameArgs), shape, sizeof shape);
+    if (!f->graph_exec || key != f->graph_key) {
+      if (f->graph_exec) {
+        (void)hipGraphExecDestroy(f->graph_exec);
+        f->graph_exec = nullptr;
+      }
+      hipGraph_t g = nullptr;
+      HIP_TRY(c, hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      st = run_chain(c, f, 0, P_n, stream, profile);
+      hipError_t ce = hipStreamEndCapture(stream, &g);
+      if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+      }
+      if (ce != hipSuccess) return fail(c, T3D_ERR_DEVICE, "graph capture: %s", hipGetErrorString(ce));
+      hipError_t ie = hipGraphInstantiate(&f->graph_exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) return fail(c, T3D_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(ie));
+      f->graph_key = key;
+    }
+    HIP_TRY(c, hipGraphLaunch(f->graph_exec, stream));
+  } else {
+    st = run_chain(c, f, 0, P_n, stream, profile);
+    if (st) return st;
   }
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemcpyAsync(f->pinned_stats, a.stats, 16ull * P_n, hipMemcpyDeviceToHost, stream));
+  HIP_TRY(c, hipEventRecord(f->done, stream));
+  return T3D_OK;
+}
+
+// After the launch's work is complete: survivor counts into the pair results.
+void frame_collect(t3d_ctx* c, t3d_frame* f, bool profile) {
   uint64_t total = 0;
-  for (uint32_t i = 0; i < P_n; i++) {
+  for (uint32_t i = 0; i < f->nparts; i++) {
     f->pairs[i].survivor_count = f->pinned_stats[4 * i + 2];
     total += f->pairs[i].survivor_count;
   }
   f->total_survivors = total;
-  if (c->profiling) {
-    auto& p = c->prof[c->tuning.emit_variant == 2 ? "k_assemble" : "k_emit"];
-    p.bytes += 300ull * total;   // 3 x (48 B colour + 48 B depth + 4 B index) per drawn triangle
+  if (profile && c->profiling) c->prof["k_assemble"].bytes += 300ull * total;   // 3 x (48 B colour + 48 B depth + 4 B index)
+}
+
+t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uint32_t p1, hipStream_t stream,
+                                t3d_vertex* cv, t3d_vertex* dv, uint32_t* idx, uint32_t* tri_ids) {
+  uint64_t off = 0, toff = 0;
+  uint64_t nrec = 0;
+  for (uint32_t i = p0; i < p1; i++) nrec += f->pairs[i].survivor_count;
+  std::vector<uint4> recs(tri_ids ? std::max<uint64_t>(nrec, 1) : 0);
+  for (uint32_t i = p0; i < p1; i++) {
+    const auto& p = f->pairs[i];
+    const uint64_t n = 3ull * p.survivor_count;
+    if (!n) continue;
+    if (cv) HIP_TRY(c, hipMemcpyAsync(cv + off, reinterpret_cast<const t3d_vertex*>(f->args.out_color) + p.vertex_offset, n * sizeof(t3d_vertex), hipMemcpyDeviceToHost, stream));
+    if (dv) HIP_TRY(c, hipMemcpyAsync(dv + off, reinterpret_cast<const t3d_vertex*>(f->args.out_depth) + p.vertex_offset, n * sizeof(t3d_vertex), hipMemcpyDeviceToHost, stream));
+    if (idx) HIP_TRY(c, hipMemcpyAsync(idx + off, f->args.out_idx + p.vertex_offset, n * 4, hipMemcpyDeviceToHost, stream));
+    if (tri_ids) HIP_TRY(c, hipMemcpyAsync(recs.data() + toff, f->args.tri_vo + f->parts[i].tbase, 16ull * p.survivor_count, hipMemcpyDeviceToHost, stream));
+    off += n;
+    toff += p.survivor_count;
   }
-  f->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+  HIP_TRY(c, hipStreamSynchronize(stream));
+  if (tri_ids) {
+    toff = 0;
+    for (uint32_t i = p0; i < p1; i++) {
+      const auto& p = f->pairs[i];
+      for (uint32_t k = 0; k < p.survivor_count; k++) tri_ids[toff + k] = recs[toff + k].w + f->part_first_tri[i];
+      toff += p.survivor_count;
+    }
+  }
   return T3D_OK;
+}
+
+uint64_t estimate_job_bytes(t3d_ctx* c, const t3d_render_desc* desc) {
+  // scratch: vrec + vpos (48 B/V); key + two records (36 B/T); outputs: 3 x (48 + 48 + 4) = 300 B/T;
+  // tables: ~1 KB per model plus palettes
+  uint64_t V = 0, T = 0, extra = 64 * 1024;
+  for (uint32_t mi = 0; mi < desc->model_count; mi++) {
+    const t3d_model_desc& md = desc->models[mi];
+    auto it = c->meshes.find(md.mesh);
+    if (it == c->meshes.end()) continue;
+    const Mesh& m = *it->second;
+    for (size_t p = 0; p < m.parts.size(); p++) {
+      V += m.part_vcount[p];
+      T += m.parts[p].triangle_count;
+    }
+    extra += 1024 + 64ull * (md.skinned ? md.bone_count : 0);
+  }
+  return 48 * V + 336 * T + extra;
+}
+
+void frame_release(t3d_ctx* c, t3d_frame* f) {
+  dev_free(c, f->tables);
+  dev_free(c, f->scratch);
+  dev_free(c, f->outputs);
+  f->tables_cap = f->scratch_cap = f->outputs_cap = 0;
+  if (f->pinned) (void)hipHostFree(f->pinned);
+  if (f->pinned_stats) (void)hipHostFree(f->pinned_stats);
+  f->pinned = nullptr;
+  f->pinned_stats = nullptr;
+  f->pinned_cap = f->pinned_stats_cap = 0;
+  if (f->graph_exec) (void)hipGraphExecDestroy(f->graph_exec);
+  f->graph_exec = nullptr;
+  if (f->done) (void)hipEventDestroy(f->done);
+  if (f->tables_ready) (void)hipEventDestroy(f->tables_ready);
+  f->done = f->tables_ready = nullptr;
+  f->inflight = false;
+}
+
+}  // namespace t3d
+
+namespace {
+
+// Caller holds c->mu.  Waits for an outstanding launch of the frame and
+// publishes its results.
+t3d_status frame_finish(t3d_ctx* c, t3d_frame* f) {
+  if (!f->inflight) return T3D_OK;
+  f->inflight = false;
+  HIP_TRY(c, hipEventSynchronize(f->done));
+  frame_collect(c, f, true);
+  f->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - f->submitted).count();
+  return T3D_OK;
+}
+
+t3d_status render_submit(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc) {
+  const auto wall0 = std::chrono::steady_clock::now();
+  t3d_status st = frame_finish(c, f);
+  if (st) return st;
+  st = frame_build(c, f, &desc, 1);
+  if (st) return st;
+  st = frame_launch(c, f, c->stream, c->copy_stream, desc->exec_mode, true, true);
+  if (st) return st;
+  f->submitted = wall0;
+  f->inflight = true;
+  return T3D_OK;
+}
+
+}  // namespace
+
+extern "C" t3d_status t3d_render(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc) {
+  if (!c || !f || !desc) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  t3d_status st = render_submit(c, f, desc);
+  if (st) return st;
+  return frame_finish(c, f);
+}
+
+extern "C" t3d_status t3d_render_async(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc) {
+  if (!c || !f || !desc) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  return render_submit(c, f, desc);
+}
+
+extern "C" t3d_status t3d_frame_wait(t3d_ctx* c, t3d_frame* f) {
+  if (!c || !f) return T3D_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(c->mu);
+  return frame_finish(c, f);
 }
 
 extern "C" {
 
 t3d_status t3d_frame_pairs(const t3d_frame* f, uint32_t* out_count, const t3d_pair_result** out_pairs) {
   if (!f || !out_count || !out_pairs) return T3D_ERR_INVALID_ARGUMENT;
+  if (f->inflight) return T3D_ERR_NOT_READY;
   *out_count = (uint32_t)f->pairs.size();
   *out_pairs = f->pairs.data();
   return T3D_OK;
@@ -1245,6 +1334,7 @@ t3d_status t3d_frame_pairs(const t3d_frame* f, uint32_t* out_count, const t3d_pa
 
 t3d_status t3d_frame_device_outputs(const t3d_frame* f, const t3d_vertex** cv, const t3d_vertex** dv, const uint32_t** idx) {
   if (!f) return T3D_ERR_INVALID_ARGUMENT;
+  if (f->inflight) return T3D_ERR_NOT_READY;
   if (cv) *cv = reinterpret_cast<const t3d_vertex*>(f->args.out_color);
   if (dv) *dv = reinterpret_cast<const t3d_vertex*>(f->args.out_depth);
   if (idx) *idx = f->args.out_idx;
@@ -1255,38 +1345,18 @@ double t3d_frame_last_ms(const t3d_frame* f) { return f ? f->last_ms : 0.0; }
 
 t3d_status t3d_frame_download(t3d_ctx* c, const t3d_frame* f, t3d_vertex* cv, t3d_vertex* dv, uint32_t* idx) {
   if (!c || !f) return T3D_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (f->inflight) return fail(c, T3D_ERR_NOT_READY, "frame_download: render in flight (t3d_frame_wait first)");
   (void)hipSetDevice(c->device);
-  uint64_t off = 0;
-  for (const auto& p : f->pairs) {
-    const uint64_t n = 3ull * p.survivor_count;
-    if (!n) continue;
-    if (cv) HIP_TRY(c, hipMemcpyAsync(cv + off, reinterpret_cast<const t3d_vertex*>(f->args.out_color) + p.vertex_offset, n * sizeof(t3d_vertex), hipMemcpyDeviceToHost, c->stream));
-    if (dv) HIP_TRY(c, hipMemcpyAsync(dv + off, reinterpret_cast<const t3d_vertex*>(f->args.out_depth) + p.vertex_offset, n * sizeof(t3d_vertex), hipMemcpyDeviceToHost, c->stream));
-    if (idx) HIP_TRY(c, hipMemcpyAsync(idx + off, f->args.out_idx + p.vertex_offset, n * 4, hipMemcpyDeviceToHost, c->stream));
-    off += n;
-  }
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
-  return T3D_OK;
+  return frame_download_range(c, f, 0, (uint32_t)f->pairs.size(), c->stream, cv, dv, idx, nullptr);
 }
 
 t3d_status t3d_frame_download_triangles(t3d_ctx* c, const t3d_frame* f, uint32_t* ids) {
   if (!c || !f || !ids) return T3D_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (f->inflight) return fail(c, T3D_ERR_NOT_READY, "frame_download: render in flight (t3d_frame_wait first)");
   (void)hipSetDevice(c->device);
-  uint64_t off = 0;
-  for (size_t i = 0; i < f->pairs.size(); i++) {
-    const auto& p = f->pairs[i];
-    if (!p.survivor_count) continue;
-    HIP_TRY(c, hipMemcpyAsync(ids + off, f->args.sorted + f->parts[i].tbase, 4ull * p.survivor_count, hipMemcpyDeviceToHost, c->stream));
-    off += p.survivor_count;
-  }
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
-  off = 0;
-  for (size_t i = 0; i < f->pairs.size(); i++) {
-    const auto& p = f->pairs[i];
-    for (uint32_t k = 0; k < p.survivor_count; k++) ids[off + k] += f->part_first_tri[i];
-    off += p.survivor_count;
-  }
-  return T3D_OK;
+  return frame_download_range(c, f, 0, (uint32_t)f->pairs.size(), c->stream, nullptr, nullptr, nullptr, ids);
 }
 
 t3d_status t3d_profile_enable(t3d_ctx* c, int32_t on) {
@@ -1328,54 +1398,77 @@ t3d_status t3d_set_tuning(t3d_ctx* c, const t3d_tuning* t) {
   if (t->triangle_tile < 64 || t->triangle_tile > kMaxTile || t->triangle_tile % 64)
     return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: triangle_tile must be a multiple of 64 in [64, %u]", kMaxTile);
   if (t->store_mode > 3) return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: store_mode in [0, 3]");
-  if (t->emit_variant > 2) return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: emit_variant in [0, 2]");
+  if (t->scatter_waves != 1 && t->scatter_waves != 4)
+    return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: scatter_waves must be 1 or 4");
+  if (t->assemble_split == 0 || (t->assemble_split & (t->assemble_split - 1)) ||
+      t->triangle_tile % t->assemble_split || t->triangle_tile / t->assemble_split < 64)
+    return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: assemble_split must be a power of two with tile/split >= 64");
   c->tuning = *t;
   return T3D_OK;
 }
 
+// Coordinate descent over the launch shape: each parameter in turn takes the
+// value with the lowest mean frame time while the others stay fixed; two
+// sweeps.  Every candidate is timed on the caller's own frame.
 t3d_status t3d_autotune(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc, uint32_t reps, t3d_tuning* out) {
   if (!c || !f || !desc) return T3D_ERR_INVALID_ARGUMENT;
   reps = std::max<uint32_t>(reps, 1);
   const t3d_tuning base = c->tuning;
-  t3d_tuning best = base;
-  double best_ms = 1e30;
-  const uint32_t vblocks[] = {128, 256};
-  const uint32_t vpts[] = {1, 2, 4};
-  const uint32_t tiles[] = {512, 1024, 2048};
-  const uint32_t variants[] = {0, 1, 2, 3, 4, 5};   // 0, 1, then variant 2 with store modes 0..3
   t3d_render_desc d = *desc;
   d.exec_mode = T3D_EXEC_BATCHED;
-  for (uint32_t ev : variants)
-  for (uint32_t vb : vblocks)
-    for (uint32_t vpt : vpts)
-      for (uint32_t tl : tiles) {
-        t3d_tuning t = base;
-        t.vertex_block = vb;
-        t.vertices_per_thread = vpt;
-        t.triangle_tile = tl;
-        t.emit_variant = ev < 2 ? ev : 2;
-        t.store_mode = ev < 2 ? 0 : ev - 2;
-        c->tuning = t;
-        t3d_status st = t3d_render(c, f, &d);   // warm: builds the graph for this shape
-        if (st) {
-          c->tuning = base;
-          return st;
-        }
-        double acc = 0;
-        for (uint32_t r = 0; r < reps; r++) {
-          st = t3d_render(c, f, &d);
-          if (st) {
-            c->tuning = base;
-            return st;
-          }
-          acc += f->last_ms;
-        }
-        acc /= reps;
-        if (acc < best_ms) {
-          best_ms = acc;
+  auto valid = [](const t3d_tuning& t) {
+    return t.triangle_tile % t.assemble_split == 0 && t.triangle_tile / t.assemble_split >= 64;
+  };
+  auto time_of = [&](const t3d_tuning& t, double& ms) -> t3d_status {
+    c->tuning = t;
+    t3d_status st = t3d_render(c, f, &d);   // warm: builds the graph for this shape
+    if (st) return st;
+    std::vector<double> v;
+    for (uint32_t r = 0; r < reps; r++) {
+      st = t3d_render(c, f, &d);
+      if (st) return st;
+      v.push_back(f->last_ms);
+    }
+    std::sort(v.begin(), v.end());
+    ms = v[v.size() / 2];
+    return T3D_OK;
+  };
+  struct Axis {
+    uint32_t t3d_tuning::*field;
+    std::vector<uint32_t> values;
+  };
+  const Axis axes[] = {
+      {&t3d_tuning::triangle_tile, {512, 1024, 2048}},
+      {&t3d_tuning::vertex_block, {128, 256}},
+      {&t3d_tuning::vertices_per_thread, {1, 2, 4}},
+      {&t3d_tuning::scatter_waves, {1, 4}},
+      {&t3d_tuning::assemble_split, {1, 2, 4, 8}},
+      {&t3d_tuning::store_mode, {0, 2}},   // LDS-staged rows, plain or streaming stores
+  };
+  t3d_tuning best = base;
+  if (!valid(best)) best.assemble_split = 1;
+  double best_ms = 0;
+  t3d_status st = time_of(best, best_ms);
+  for (int sweep = 0; sweep < 2 && st == T3D_OK; sweep++)
+    for (const Axis& ax : axes) {
+      for (uint32_t v : ax.values) {
+        t3d_tuning t = best;
+        t.*(ax.field) = v;
+        if (t.*(ax.field) == best.*(ax.field) || !valid(t)) continue;
+        double ms = 0;
+        st = time_of(t, ms);
+        if (st) break;
+        if (ms < best_ms) {
+          best_ms = ms;
           best = t;
         }
       }
+      if (st) break;
+    }
+  if (st) {
+    c->tuning = base;
+    return st;
+  }
   c->tuning = best;
   if (out) *out = best;
   return T3D_OK;
@@ -1383,6 +1476,7 @@ t3d_status t3d_autotune(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc, u
 
 t3d_status t3d_selftest_copy(t3d_ctx* c, uint64_t bytes, uint32_t reps, double* out_ms) {
   if (!c || bytes == 0 || bytes % 16 || reps == 0) return fail(c, T3D_ERR_INVALID_ARGUMENT, "selftest_copy: bad arguments");
+  std::lock_guard<std::mutex> lk(c->mu);
   (void)hipSetDevice(c->device);
   DevBuf src, dst;
   t3d_status st = dev_alloc(c, src, bytes);

@@ -9,7 +9,8 @@
 //     block->part prefix arrays.
 //   frame scratch: vrec[sumV] = {clip f32x4, colour f32x4} (one 32-byte record
 //     per vertex), vpos[sumV] f32x4 (skinned positions), key[sumT] f32,
-//     sorted[sumT] u32 (drawn triangle per slot), stats[P] u32x4, hist, bsum.
+//     tri_rec/tri_vo[sumT] u32x4 (survivor records by triangle / by slot),
+//     stats[P] u32x4, hist, bsum.
 //   outputs: color[3*sumT] t3d_vertex, depth[3*sumT] t3d_vertex, idx[3*sumT] u32;
 //     render pair p owns vertices [out_base, out_base + 3*T_p) and fills the
 //     first 3*S_p of them.
@@ -68,6 +69,7 @@ struct alignas(16) PartGPU {
   uint32_t tbase, ntiles;     // first slot in the triangle scratch; tiles of the part
   uint32_t hbase, sblock0;    // histogram offset and first scan block (multi-tile parts)
   uint32_t nsblocks, bone_count;
+  uint32_t cam, pad1;         // camera (job) of the pair in FrameArgs.fc
   uint64_t out_base;          // first output vertex
 };
 
@@ -123,15 +125,15 @@ struct FrameArgs {
   float4* vrec;                 // [sumV][2]: clip position, vertex colour
   float4* vpos;
   float* key;
-  uint32_t* sorted;              // [sumT] drawn triangle (part-local) per output slot
-  uint4* tri_vo;                // [sumT] vertex-record offsets of the triangle in each slot
+  uint4* tri_rec;               // [sumT] survivors, triangle order: {vertex-record offsets x3, triangle}
+  uint4* tri_vo;                // [sumT] the same records in output-slot order
   uint32_t* stats;              // [P][4]: min bits, max bits, survivors, 0
   uint32_t* hist;
   uint32_t* bsum;
   float4* out_color;            // t3d_vertex as 3 float4
   float4* out_depth;
   uint32_t* out_idx;
-  const FrameConst* fc;         // device copy, so graph kernel arguments stay fixed across frames
+  const FrameConst* fc;         // [jobs] device copies, so graph kernel arguments stay fixed across frames
 };
 
 // Kernel launchers (t3d_kernels.hip).  blk0 offsets blockIdx so the plain
@@ -142,8 +144,9 @@ hipError_t launch_cull(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStre
 hipError_t launch_hist(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s);
 hipError_t launch_scan(const FrameArgs& a, uint32_t sblk0, uint32_t nsblk, uint32_t part0, uint32_t nparts,
                        hipStream_t s);
-hipError_t launch_emit(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t variant, hipStream_t s);
-hipError_t launch_assemble(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t mode, hipStream_t s);
+hipError_t launch_scatter(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t waves, hipStream_t s);
+hipError_t launch_assemble(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t mode, uint32_t split,
+                           hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 hipError_t launch_fp_selftest(const float* a, const float* b, float* out, uint32_t n, hipStream_t s);
 

@@ -13,11 +13,11 @@
 //               (model.go:828-969)
 //   k_hist      per tile of a multi-tile part: 512-bin histogram
 //   k_scan*     exclusive scan of each part's [bin][tile] histogram matrix
-//   k_emit      stable counting sort fused with vertex assembly: output slot of
-//               every survivor (sortingTriangleBucket Sort + ForEach,
-//               sortingTriangles.go:38-125) and its ebiten colour/depth vertices
-//               and indices, staged in LDS per bin run and written as
-//               contiguous float4 rows (camera.go:1646-1820)
+//   k_scatter   stable counting sort: output slot of every survivor
+//               (sortingTriangleBucket Sort + ForEach, sortingTriangles.go:38-125)
+//   k_assemble  per drawn triangle in slot order: ebiten colour/depth vertices
+//               and indices, staged in LDS and written as contiguous float4 rows
+//               with streaming stores (camera.go:1646-1820)
 //
 // All arithmetic is float32 in the reference's order; the build uses
 // -ffp-contract=off and correctly rounded division and square root.
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
   }
   __syncthreads();
 
-  const FrameConst& fc = *a.fc;
+  const FrameConst& fc = a.fc[P.cam];
   const uint32_t first = (b - a.vchunk_prefix[p]) * a.vchunk;
   const uint32_t last = min(first + a.vchunk, P.vcount);
   for (uint32_t i = first + threadIdx.x; i < last; i += blockDim.x) {
@@ -319,7 +319,7 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a, uint32_t blk0) {
   const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
   const PartGPU& P = a.parts[p];
   const uint32_t flags = P.flags;
-  const FrameConst& fc = *a.fc;
+  const FrameConst& fc = a.fc[P.cam];
   const uint32_t t0 = (b - a.tile_prefix[p]) * a.tile;
   const uint32_t t1 = min(t0 + a.tile, P.tcount);
   const bool skinned = (flags & PF_SKINNED) != 0;
@@ -333,10 +333,12 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a, uint32_t blk0) {
     const uint32_t i0 = P.idx[3 * (size_t)t + 0], i1 = P.idx[3 * (size_t)t + 1], i2 = P.idx[3 * (size_t)t + 2];
     const uint32_t vi[3] = {i0, i1, i2};
     float nx[3], ny[3];
+    uint32_t ov[3];
     bool outOfBounds = true;
     V3 stc = v3(0, 0, 0);
     for (int k = 0; k < 3; k++) {                                   // model.go:855-886
       const uint32_t o = P.vbase + (vi[k] - P.vmin);
+      ov[k] = o;
       const float4 c = a.vrec[2 * (size_t)o];
       if (skinned) {
         const float4 sp = a.vpos[o];
@@ -389,6 +391,7 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a, uint32_t blk0) {
     }
     a.key[P.tbase + t] = depth;
     if (keep) {
+      a.tri_rec[P.tbase + t] = make_uint4(ov[0], ov[1], ov[2], t);
       mn = fminf(mn, depth);
       mx = fmaxf(mx, depth);
       cnt++;
@@ -550,19 +553,15 @@ __global__ __launch_bounds__(256) void k_scan_down(FrameArgs a, uint32_t sblk0) 
   }
 }
 
-// Fused stable scatter + vertex assembly.  One wavefront per tile walks its
-// triangles in index order, 64 at a time (so vertex records and indices are
-// read coherently), ranks equal bins with ballots, and gives every survivor
-// its output slot: run[bin] + rank, where run[bin] comes from the scanned
-// [bin][tile] histogram (or a local scan when the part is a single tile).
-// Within a chunk the survivors of one bin occupy consecutive slots; their
-// ebiten vertices are staged in LDS grouped by bin and copied out as
-// contiguous float4 rows (camera.go:1646-1820 per corner).
-// Variants: 0 = bin runs staged in LDS and copied out as contiguous rows;
-// 1 = each lane stores its own triangle's vertices directly; 2 = slots only
-// (the drawn-triangle list), with k_assemble writing the vertices in slot order.
-template <int kVariant>
-__global__ __launch_bounds__(64) void k_emit(FrameArgs a, uint32_t blk0) {
+// Stable counting sort, scatter pass.  One wavefront per tile walks its
+// triangles in index order, 64 at a time (keys staged in LDS in one burst),
+// ranks equal bins with ballots and gives every survivor its output slot:
+// run[bin] + rank, where run[bin] comes from the scanned [bin][tile]
+// histogram, or from a local scan when the part is a single tile.  This is
+// sortingTriangleBucket.Sort + ForEach (sortingTriangles.go:38-125): bins
+// visited 511->0 (BackToFront) or 0->511, insertion order kept within a bin.
+// The survivor's record {vertex-record offsets, triangle} moves to its slot.
+__global__ __launch_bounds__(64) void k_scatter(FrameArgs a, uint32_t blk0) {
   const uint32_t b = blk0 + blockIdx.x;
   const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
   const PartGPU& P = a.parts[p];
@@ -570,25 +569,16 @@ __global__ __launch_bounds__(64) void k_emit(FrameArgs a, uint32_t blk0) {
   const uint32_t tl = b - a.tile_prefix[p];
   const uint32_t t0 = tl * a.tile, t1 = min(t0 + a.tile, P.tcount);
   const uint32_t lane = threadIdx.x;
-  const FrameConst& fc = *a.fc;
-  const uint32_t* __restrict__ idx = P.idx;
-  const float* __restrict__ uvp = P.uv;
-  const uint32_t vbase = P.vbase, vmin = P.vmin, tbase = P.tbase;
-  const bool has_uv = (P.flags & PF_HAS_UV) != 0;
-  const float src_w = P.src_w, src_h = P.src_h;
+  const uint32_t tbase = P.tbase;
   const uint32_t mode = (P.flags >> PF_SORT_SHIFT) & 3u;
-  float4* out_c = a.out_color + P.out_base * 3;   // 3 float4 per vertex
-  float4* out_d = a.out_depth + P.out_base * 3;
-  uint32_t* out_i = a.out_idx + P.out_base;
-  uint32_t* sorted = a.sorted + tbase;
+  const uint4* __restrict__ rec_in = a.tri_rec + tbase;
+  uint4* __restrict__ rec_out = a.tri_vo + tbase;
 
   __shared__ uint32_t run[kBins];
-  __shared__ float4 stage[kVariant == 0 ? 64 * 9 : 1];
   __shared__ float s_key[kMaxTile];
   float mn, rd;
   part_range(a.stats, p, mn, rd);
   for (uint32_t t = t0 + lane; t < t1; t += 64) s_key[t - t0] = a.key[tbase + t];   // one burst of loads
-  __syncthreads();
 
   if (P.ntiles > 1) {
     const uint32_t* col = a.hist + P.hbase + tl;
@@ -629,6 +619,8 @@ __global__ __launch_bounds__(64) void k_emit(FrameArgs a, uint32_t blk0) {
     const bool valid = d >= 0;
     const uint64_t vmask = __ballot(valid);
     if (vmask == 0) continue;                   // uniform: no survivor in this chunk
+    uint4 rec = make_uint4(0, 0, 0, 0);
+    if (valid) rec = rec_in[t];                 // issued before the ranking so its latency overlaps it
     const uint32_t bin = valid ? visit_bin(d, mn, rd, mode) : 0xFFFFFFFFu;
     uint64_t todo = vmask, peers = 0;
     while (todo) {                              // peers = lanes with my bin
@@ -639,127 +631,117 @@ __global__ __launch_bounds__(64) void k_emit(FrameArgs a, uint32_t blk0) {
       todo &= ~m;
     }
     const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
-    const uint32_t cnt = (uint32_t)__popcll(peers);
-    const bool is_leader = valid && rank == 0;
-    const uint64_t leaders = __ballot(is_leader);
-    // groups ordered by leader lane: exclusive prefix of group sizes
-    uint32_t x = is_leader ? cnt : 0u;
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = __shfl_up(x, o);
-      if (lane >= (uint32_t)o) x += y;
-    }
-    const uint32_t gstart_l = x - (is_leader ? cnt : 0u);   // valid at leader lanes
-    const int my_leader = valid ? __ffsll((unsigned long long)peers) - 1 : 0;
-    const uint32_t gstart = __shfl(gstart_l, my_leader);
-    const uint32_t cpos = gstart + rank;        // wave-local position, grouped by bin
     uint32_t slot = 0;
     if (valid) slot = run[bin] + rank;
-    const uint32_t slot0_l = slot;              // slot of rank 0 at leader lanes
-    __syncthreads();
-    if (is_leader) run[bin] = slot + cnt;
-
-    if (kVariant == 2) {
-      if (valid) {
-        sorted[slot] = t;
-        a.tri_vo[tbase + slot] = make_uint4(vbase + (idx[3 * (size_t)t + 0] - vmin), vbase + (idx[3 * (size_t)t + 1] - vmin),
-                                            vbase + (idx[3 * (size_t)t + 2] - vmin), 0u);
-      }
-      __syncthreads();
-      continue;
-    }
-    // per-corner values of this survivor (camera.go:1655-1784)
-    float4 cq[9];
-    float dq[3][6];
-    if (valid) {
-      sorted[slot] = t;
-      const uint32_t vi[3] = {idx[3 * (size_t)t + 0], idx[3 * (size_t)t + 1], idx[3 * (size_t)t + 2]};
-      for (int k = 0; k < 3; k++) {
-        const uint32_t v = vi[k];
-        const uint32_t o = vbase + (v - vmin);
-        const float4 c = a.vrec[2 * (size_t)o];
-        const float4 col = a.vrec[2 * (size_t)o + 1];
-        float w = c.w;
-        if (!fc.perspective) w = 1.0f;
-        if (w < 0) w = 0.001f;
-        const float dx = fdiv(c.x, w) * fc.cam_w + fc.half_w;
-        const float dy = fdiv(c.y, -w) * fc.cam_h + fc.half_h;
-        float U = 0, V = 0;
-        if (has_uv) {
-          const float2 uv = reinterpret_cast<const float2*>(uvp)[v];
-          U = uv.x;
-          V = uv.y;
-        }
-        float uvU, uvV, c0 = 0.0f;
-        if (fc.pc_tex) {
-          uvU = fdiv(U, w) * src_w;
-          uvV = fdiv(1.0f - V, w) * src_h;
-          c0 = fdiv(1.0f, w);
-        } else {
-          uvU = U * src_w;
-          uvV = (1.0f - V) * src_h;
-        }
-        float depth = 0.0f;
-        if (fc.render_depth) {
-          depth = fdiv(c.z + fc.dmp, fc.spread);
-          if (depth < 0) depth = 0;
-          else if (depth > 1) depth = 1;
-        }
-        cq[3 * k + 0] = make_float4(dx, dy, uvU, uvV);
-        cq[3 * k + 1] = col;
-        cq[3 * k + 2] = make_float4(c0, 0.0f, 0.0f, 0.0f);
-        dq[k][0] = dx;
-        dq[k][1] = dy;
-        dq[k][2] = uvU;
-        dq[k][3] = uvV;
-        dq[k][4] = depth;
-        dq[k][5] = c0;
-      }
-    }
-    const float da = fc.render_depth ? 1.0f : 0.0f;
-    if (kVariant == 1) {
-      if (valid) {
-        float4* oc = out_c + (size_t)slot * 9;
-        float4* od = out_d + (size_t)slot * 9;
-        for (int q = 0; q < 9; q++) oc[q] = cq[q];
-        for (int k = 0; k < 3; k++) {
-          od[3 * k + 0] = make_float4(dq[k][0], dq[k][1], dq[k][2], dq[k][3]);
-          od[3 * k + 1] = make_float4(dq[k][4], dq[k][4], dq[k][4], da);
-          od[3 * k + 2] = make_float4(dq[k][5], 0.0f, 0.0f, 0.0f);
-        }
-        for (int k = 0; k < 3; k++) out_i[(size_t)slot * 3 + k] = slot * 3 + k;
-      }
-      __syncthreads();
-      continue;
-    }
-    if (valid)
-      for (int q = 0; q < 9; q++) stage[cpos * 9 + q] = cq[q];
-    __syncthreads();
-    // colour vertices: one contiguous copy per bin group
-    for (uint64_t lm = leaders; lm; lm &= lm - 1) {
-      const int L = __ffsll((unsigned long long)lm) - 1;
-      const uint32_t gs = __shfl(gstart_l, L), gc = __shfl(cnt, L), s0 = __shfl(slot0_l, L);
-      for (uint32_t j = lane; j < gc * 9; j += 64) out_c[(size_t)s0 * 9 + j] = stage[gs * 9 + j];
-    }
     __syncthreads();
     if (valid) {
-      for (int k = 0; k < 3; k++) {
-        stage[cpos * 9 + 3 * k + 0] = make_float4(dq[k][0], dq[k][1], dq[k][2], dq[k][3]);
-        stage[cpos * 9 + 3 * k + 1] = make_float4(dq[k][4], dq[k][4], dq[k][4], da);
-        stage[cpos * 9 + 3 * k + 2] = make_float4(dq[k][5], 0.0f, 0.0f, 0.0f);
-      }
-    }
-    __syncthreads();
-    for (uint64_t lm = leaders; lm; lm &= lm - 1) {
-      const int L = __ffsll((unsigned long long)lm) - 1;
-      const uint32_t gs = __shfl(gstart_l, L), gc = __shfl(cnt, L), s0 = __shfl(slot0_l, L);
-      for (uint32_t j = lane; j < gc * 9; j += 64) out_d[(size_t)s0 * 9 + j] = stage[gs * 9 + j];
-      // pair-local iota indices, as the reference's indexList (camera.go:1816)
-      for (uint32_t j = lane; j < gc * 3; j += 64) out_i[(size_t)s0 * 3 + j] = s0 * 3 + j;
+      if (rank == 0) run[bin] = slot + (uint32_t)__popcll(peers);
+      rec_out[slot] = rec;
     }
     __syncthreads();
   }
 }
 
+// The same scatter with four wavefronts per tile.  Wave w owns a contiguous
+// quarter of the tile's 64-triangle chunks.  A first pass over the staged keys
+// counts each wave's survivors per bin; the base of wave w in bin k is
+// run[k] plus the counts of waves 0..w-1 in bin k, so the four quarters rank
+// independently and the result is the single-wave order exactly.
+constexpr uint32_t kScatterWaves = 4;
+__global__ __launch_bounds__(256) void k_scatter4(FrameArgs a, uint32_t blk0) {
+  const uint32_t b = blk0 + blockIdx.x;
+  const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
+  const PartGPU& P = a.parts[p];
+  if (a.stats[4 * p + 2] == 0) return;
+  const uint32_t tl = b - a.tile_prefix[p];
+  const uint32_t t0 = tl * a.tile, t1 = min(t0 + a.tile, P.tcount);
+  const uint32_t n = t1 - t0;
+  const uint32_t tid = threadIdx.x, lane = tid % 64u, wv = tid / 64u;
+  const uint32_t tbase = P.tbase;
+  const uint32_t mode = (P.flags >> PF_SORT_SHIFT) & 3u;
+  const uint4* __restrict__ rec_in = a.tri_rec + tbase;
+  uint4* __restrict__ rec_out = a.tri_vo + tbase;
+
+  __shared__ uint16_t s_bin[kMaxTile];            // visit bin, or 0xFFFF for a culled triangle
+  __shared__ uint32_t s_off[kScatterWaves][kBins];
+  __shared__ uint32_t s_tmp[8];
+  float mn, rd;
+  part_range(a.stats, p, mn, rd);
+  for (uint32_t i = tid; i < kScatterWaves * kBins; i += blockDim.x) (&s_off[0][0])[i] = 0;
+  const uint32_t nchunks = (n + 63) / 64;
+  const uint32_t c0 = (wv * nchunks) / kScatterWaves, c1 = ((wv + 1) * nchunks) / kScatterWaves;
+  __syncthreads();
+  for (uint32_t i = c0 * 64 + lane; i < min(c1 * 64, n); i += 64) {
+    const float d = a.key[tbase + t0 + i];
+    uint16_t bn = 0xFFFFu;
+    if (d >= 0) {
+      const uint32_t k = visit_bin(d, mn, rd, mode);
+      bn = (uint16_t)k;
+      atomicAdd(&s_off[wv][k], 1u);
+    }
+    s_bin[i] = bn;
+  }
+  __syncthreads();
+  // per bin: base of each wave (two bins per thread)
+  if (P.ntiles > 1) {
+    const uint32_t* col = a.hist + P.hbase + tl;
+    for (uint32_t k = tid; k < kBins; k += blockDim.x) {
+      uint32_t run = col[(size_t)k * P.ntiles];
+      for (uint32_t w = 0; w < kScatterWaves; w++) {
+        const uint32_t c = s_off[w][k];
+        s_off[w][k] = run;
+        run += c;
+      }
+    }
+  } else {
+    uint32_t cnt[2], sum = 0;
+    for (int j = 0; j < 2; j++) {
+      cnt[j] = 0;
+      for (uint32_t w = 0; w < kScatterWaves; w++) cnt[j] += s_off[w][2 * tid + j];
+      sum += cnt[j];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan_256(sum, s_tmp, total);
+    for (int j = 0; j < 2; j++) {
+      const uint32_t k = 2 * tid + j;
+      for (uint32_t w = 0; w < kScatterWaves; w++) {
+        const uint32_t c = s_off[w][k];
+        s_off[w][k] = run;
+        run += c;
+      }
+    }
+  }
+  __syncthreads();
+
+  // each wave ranks its own chunks; only this wave touches s_off[wv]
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint32_t* off = s_off[wv];
+  for (uint32_t c = c0; c < c1; c++) {
+    const uint32_t i = c * 64 + lane;
+    const uint32_t bin = i < n ? (uint32_t)s_bin[i] : 0xFFFFu;
+    const bool valid = bin != 0xFFFFu;
+    const uint64_t vmask = __ballot(valid);
+    if (vmask == 0) continue;                     // uniform across the wave
+    uint4 rec = make_uint4(0, 0, 0, 0);
+    if (valid) rec = rec_in[t0 + i];
+    uint64_t todo = vmask, peers = 0;
+    while (todo) {
+      const int leader = __ffsll((unsigned long long)todo) - 1;
+      const uint32_t lb = __shfl(bin, leader);
+      const uint64_t m = __ballot(valid && bin == lb);
+      if (valid && bin == lb) peers = m;
+      todo &= ~m;
+    }
+    const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+    uint32_t slot = 0;
+    if (valid) slot = off[bin] + rank;
+    __builtin_amdgcn_wave_barrier();
+    if (valid && rank == 0) off[bin] = slot + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (valid) rec_out[slot] = rec;
+  }
+}
 
 typedef float f4v __attribute__((ext_vector_type(4)));
 
@@ -778,22 +760,25 @@ __device__ __forceinline__ void store1(uint32_t* p, uint32_t v) {
   else *p = v;
 }
 
-// Output-order assembly (emit variant 2): per drawn triangle in slot order,
+// Output-order assembly: per drawn triangle in slot order,
 // the ebiten colour/depth vertices and indices (camera.go:1646-1820).  The
 // sort pass left each slot's three vertex-record offsets in tri_vo, so the
 // only gathers are the 32-byte vertex records and UVs.  kStage: stage each
 // wavefront's 64 triangles in LDS and write contiguous float4 rows (else each
 // lane stores its own 144-byte runs); kNT: non-temporal (streaming) stores.
+// `split` blocks share one tile's slot range (a power of two dividing the
+// tile), so the grid follows the survivors more finely than the sort's tiles.
 template <bool kStage, bool kNT>
-__global__ __launch_bounds__(64) void k_assemble(FrameArgs a, uint32_t blk0) {
-  const uint32_t b = blk0 + blockIdx.x;
+__global__ __launch_bounds__(64) void k_assemble(FrameArgs a, uint32_t blk0, uint32_t split) {
+  const uint32_t b = blk0 + blockIdx.x / split;
   const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
   const PartGPU& P = a.parts[p];
   const uint32_t S = a.stats[4 * p + 2];
-  const uint32_t k0 = (b - a.tile_prefix[p]) * a.tile;
+  const uint32_t sub = a.tile / split;
+  const uint32_t k0 = (b - a.tile_prefix[p]) * a.tile + (blockIdx.x % split) * sub;
   if (k0 >= S) return;
-  const uint32_t k1 = min(k0 + a.tile, S);
-  const FrameConst& fc = *a.fc;
+  const uint32_t k1 = min(k0 + sub, S);
+  const FrameConst& fc = a.fc[P.cam];
   const uint32_t lane = threadIdx.x;
   const uint32_t vbase = P.vbase, vmin = P.vmin, tbase = P.tbase;
   const bool has_uv = (P.flags & PF_HAS_UV) != 0;
@@ -926,20 +911,22 @@ hipError_t launch_scan(const FrameArgs& a, uint32_t sblk0, uint32_t nsblk, uint3
   hipLaunchKernelGGL(k_scan_down, dim3(nsblk), dim3(256), 0, s, a, sblk0);
   return hipGetLastError();
 }
-hipError_t launch_emit(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t variant, hipStream_t s) {
+hipError_t launch_scatter(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t waves, hipStream_t s) {
   if (!nblk) return hipSuccess;
-  if (variant == 0) hipLaunchKernelGGL(k_emit<0>, dim3(nblk), dim3(64), 0, s, a, blk0);
-  else if (variant == 1) hipLaunchKernelGGL(k_emit<1>, dim3(nblk), dim3(64), 0, s, a, blk0);
-  else hipLaunchKernelGGL(k_emit<2>, dim3(nblk), dim3(64), 0, s, a, blk0);
+  if (waves == kScatterWaves) hipLaunchKernelGGL(k_scatter4, dim3(nblk), dim3(64 * kScatterWaves), 0, s, a, blk0);
+  else hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(64), 0, s, a, blk0);
   return hipGetLastError();
 }
-hipError_t launch_assemble(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t mode, hipStream_t s) {
+hipError_t launch_assemble(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t mode, uint32_t split,
+                           hipStream_t s) {
   if (!nblk) return hipSuccess;
+  if (split == 0 || (split & (split - 1)) || a.tile % split) return hipErrorInvalidValue;
+  const dim3 g(nblk * split);
   switch (mode & 3u) {
-    case 0: hipLaunchKernelGGL((k_assemble<true, false>), dim3(nblk), dim3(64), 0, s, a, blk0); break;
-    case 1: hipLaunchKernelGGL((k_assemble<false, false>), dim3(nblk), dim3(64), 0, s, a, blk0); break;
-    case 2: hipLaunchKernelGGL((k_assemble<true, true>), dim3(nblk), dim3(64), 0, s, a, blk0); break;
-    default: hipLaunchKernelGGL((k_assemble<false, true>), dim3(nblk), dim3(64), 0, s, a, blk0); break;
+    case 0: hipLaunchKernelGGL((k_assemble<true, false>), g, dim3(64), 0, s, a, blk0, split); break;
+    case 1: hipLaunchKernelGGL((k_assemble<false, false>), g, dim3(64), 0, s, a, blk0, split); break;
+    case 2: hipLaunchKernelGGL((k_assemble<true, true>), g, dim3(64), 0, s, a, blk0, split); break;
+    default: hipLaunchKernelGGL((k_assemble<false, true>), g, dim3(64), 0, s, a, blk0, split); break;
   }
   return hipGetLastError();
 }

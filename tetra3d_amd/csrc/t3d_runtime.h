@@ -1,0 +1,135 @@
+// t3d_runtime.h — host-side runtime state shared by the render path
+// (t3d_host.cpp) and the serving loop (t3d_serve.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "t3d_internal.h"
+
+namespace t3d {
+
+// A device allocation accounted against the context budget.
+struct DevBuf {
+  void* ptr = nullptr;
+  uint64_t bytes = 0;
+};
+
+// A resident mesh (mesh.go:260-302) and its MeshPart ranges.
+struct Mesh {
+  uint64_t handle = 0;
+  DevBuf buf;
+  const float* pos = nullptr;
+  const float* nrm = nullptr;
+  const float* col = nullptr;
+  const float* uv = nullptr;
+  const uint16_t* bones = nullptr;
+  const float* weights = nullptr;
+  const float* sk_pos = nullptr;
+  const float* sk_nrm = nullptr;
+  const uint32_t* idx = nullptr;
+  uint32_t V = 0, T = 0, K = 0;
+  std::vector<t3d_part_range> parts;
+  std::vector<uint32_t> part_vmin, part_vcount;
+  float dmin[3], dmax[3];
+  uint32_t max_bone_index = 0;
+  bool has_skin = false;
+  int refs = 0;                    // submitted jobs referencing the mesh
+};
+
+struct Volume {
+  uint64_t handle = 0;
+  DevBuf buf;
+  uint32_t n = 0;
+  std::vector<float> host;         // for per-object lookups on the host
+};
+
+}  // namespace t3d
+
+struct t3d_frame {
+  // device tables and scratch
+  t3d::DevBuf tables, scratch, outputs;
+  uint64_t tables_cap = 0, scratch_cap = 0, outputs_cap = 0;
+  void* pinned = nullptr;          // host staging of the tables
+  uint64_t pinned_cap = 0;
+  uint32_t* pinned_stats = nullptr;
+  uint64_t pinned_stats_cap = 0;
+  t3d::FrameArgs args{};
+  // per-render bookkeeping
+  std::vector<t3d_pair_result> pairs;
+  std::vector<t3d::PartGPU> parts;
+  std::vector<uint32_t> part_first_tri;  // mesh triangle_start per pair
+  std::vector<uint64_t> bytes_vertex, bytes_cull, bytes_uv;  // algorithmic bytes per part
+  std::vector<t3d::FrameConst> fcs;      // one camera per job of the frame
+  std::vector<uint32_t> job_pair_begin;  // [jobs + 1] first render pair of each job
+  uint64_t table_bytes = 0;
+  uint64_t total_triangles = 0;
+  uint32_t nparts = 0;
+  uint32_t total_vchunks = 0, total_tiles = 0, total_sblocks = 0;
+  std::vector<uint32_t> vchunk_prefix, tile_prefix, sblock_prefix;
+  uint32_t vertex_threads = 256;
+  uint64_t total_survivors = 0;
+  double last_ms = 0;
+  // graph cache of the batched chain
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<uint8_t> graph_key;
+  // asynchronous renders: tables uploaded / whole frame done, and whether a
+  // launch is outstanding
+  hipEvent_t tables_ready = nullptr, done = nullptr;
+  bool inflight = false;
+  std::chrono::steady_clock::time_point submitted;
+};
+
+struct t3d_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;        // kernel chains of t3d_render / t3d_render_async
+  hipStream_t copy_stream = nullptr;   // table uploads, so they overlap the previous frame's kernels
+  uint64_t budget = 0, used = 0;
+  uint64_t next_handle = 1;
+  std::map<uint64_t, std::unique_ptr<t3d::Mesh>> meshes;
+  std::map<uint64_t, std::unique_ptr<t3d::Volume>> volumes;
+  std::string err;
+  t3d_tuning tuning{256, 1, 1024, 4, 1, 2, 4};
+  // Guards resources, the budget and err between callers and the serving
+  // thread; frame builds hold it, device work runs outside it.
+  std::mutex mu;
+  // profiling (synchronous render path)
+  bool profiling = false;
+  struct Prof {
+    double ms = 0;
+    uint32_t launches = 0;
+    uint64_t bytes = 0;
+  };
+  std::map<std::string, Prof> prof;
+};
+
+namespace t3d {
+
+t3d_status fail(t3d_ctx* c, t3d_status st, const char* fmt, ...);
+t3d_status dev_alloc(t3d_ctx* c, DevBuf& b, uint64_t bytes);
+void dev_free(t3d_ctx* c, DevBuf& b);
+void frame_release(t3d_ctx* c, t3d_frame* f);
+// Device bytes a frame holds between renders (tables, scratch, outputs).
+inline uint64_t frame_capacity(const t3d_frame* f) { return f->tables.bytes + f->scratch.bytes + f->outputs.bytes; }
+
+// One frame of one or more Camera.Render calls: build (host work, caller
+// holds ctx->mu), launch (enqueue on `stream`), collect (after completion).
+t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* descs, uint32_t ndesc);
+// The table upload goes on `copy_stream` (may equal `stream`); `stream` waits
+// for it, runs the chain and the survivor readback, then records f->done.
+t3d_status frame_launch(t3d_ctx* c, t3d_frame* f, hipStream_t stream, hipStream_t copy_stream, int32_t exec_mode,
+                        bool allow_graph, bool profile);
+void frame_collect(t3d_ctx* c, t3d_frame* f, bool profile);
+// Copy pairs [p0, p1) of a frame, concatenated, to host memory (any output may be null).
+t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uint32_t p1, hipStream_t stream,
+                                t3d_vertex* cv, t3d_vertex* dv, uint32_t* idx, uint32_t* tri_ids);
+// Frame memory a job may need (scratch + outputs + tables), before frustum culling.
+uint64_t estimate_job_bytes(t3d_ctx* c, const t3d_render_desc* desc);
+
+}  // namespace t3d

@@ -59,6 +59,8 @@ def lib():
     L.t3d_frame_create.argtypes = [vp, P(vp)]
     L.t3d_frame_destroy.argtypes = [vp, vp]
     L.t3d_render.argtypes = [vp, vp, P(A.RenderDesc)]
+    L.t3d_render_async.argtypes = [vp, vp, P(A.RenderDesc)]
+    L.t3d_frame_wait.argtypes = [vp, vp]
     L.t3d_frame_pairs.argtypes = [vp, P(C.c_uint32), P(P(A.PairResult))]
     L.t3d_frame_device_outputs.argtypes = [vp, P(vp), P(vp), P(vp)]
     L.t3d_frame_download.argtypes = [vp, vp, P(A.Vertex), P(A.Vertex), P(C.c_uint32)]
@@ -116,6 +118,7 @@ class Context:
             raise T3DError(rc, "t3d_ctx_create failed (is a HIP device visible?)")
         self.frame = C.c_void_p()
         self._check(self.L.t3d_frame_create(self.ctx, C.byref(self.frame)))
+        self._extra_frames = []
         self._mesh_handles = {}   # id(mesh) -> handle
         self._mesh_refs = {}      # id(mesh) -> mesh: keeps ids from being reused while uploaded
 
@@ -126,6 +129,9 @@ class Context:
 
     def close(self):
         if self.ctx:
+            for f in self._extra_frames:
+                self.L.t3d_frame_destroy(self.ctx, f)
+            self._extra_frames = []
             if self.frame:
                 self.L.t3d_frame_destroy(self.ctx, self.frame)
                 self.frame = None
@@ -162,9 +168,11 @@ class Context:
         return out.value
 
     def free_mesh(self, mesh):
-        h = self._mesh_handles.pop(id(mesh))
+        h = self._mesh_handles[id(mesh)]
+        self._check(self.L.t3d_mesh_free(self.ctx, h))   # BUSY while a server job references it
+        del self._mesh_handles[id(mesh)]
         self._mesh_refs.pop(id(mesh), None)
-        self._check(self.L.t3d_mesh_free(self.ctx, h))
+        mesh._handles.pop("gpu", None)
 
     def upload_volume(self, vol):
         cells = np.ascontiguousarray(vol.cells, dtype=np.float32)
@@ -214,6 +222,20 @@ class Context:
         """Run one Camera.Render on the device; returns nothing (outputs stay in HBM)."""
         self._check(self.L.t3d_render(self.ctx, self.frame, C.byref(inp.render)))
 
+    def new_frame(self):
+        """An extra frame (outputs + scratch) of this context, for asynchronous renders."""
+        f = C.c_void_p()
+        self._check(self.L.t3d_frame_create(self.ctx, C.byref(f)))
+        self._extra_frames.append(f)
+        return f
+
+    def render_async(self, inp, frame=None):
+        """Enqueue one Camera.Render into `frame` (default: the context's frame) and return."""
+        self._check(self.L.t3d_render_async(self.ctx, frame or self.frame, C.byref(inp.render)))
+
+    def wait(self, frame=None):
+        self._check(self.L.t3d_frame_wait(self.ctx, frame or self.frame))
+
     def autotune(self, inp, reps=3):
         t = A.Tuning()
         self._check(self.L.t3d_autotune(self.ctx, self.frame, C.byref(inp.render), reps, C.byref(t)))
@@ -222,26 +244,27 @@ class Context:
     def last_ms(self):
         return float(self.L.t3d_frame_last_ms(self.frame))
 
-    def pairs(self):
+    def pairs(self, frame=None):
         n = C.c_uint32()
         ptr = C.POINTER(A.PairResult)()
-        self._check(self.L.t3d_frame_pairs(self.frame, C.byref(n), C.byref(ptr)))
+        self._check(self.L.t3d_frame_pairs(frame or self.frame, C.byref(n), C.byref(ptr)))
         return [(p.model_index, p.part_index, p.triangle_count, p.survivor_count, p.vertex_offset)
                 for p in ptr[:n.value]]
 
-    def download(self, triangles=True):
-        pairs = self.pairs()
+    def download(self, triangles=True, frame=None):
+        frame = frame or self.frame
+        pairs = self.pairs(frame)
         S = int(sum(p[3] for p in pairs))
         color = np.zeros((3 * S, 12), dtype=np.float32)
         depth = np.zeros((3 * S, 12), dtype=np.float32)
         idx = np.zeros(3 * S, dtype=np.uint32)
         VP = C.POINTER(A.Vertex)
-        self._check(self.L.t3d_frame_download(self.ctx, self.frame, color.ctypes.data_as(VP), depth.ctypes.data_as(VP),
+        self._check(self.L.t3d_frame_download(self.ctx, frame, color.ctypes.data_as(VP), depth.ctypes.data_as(VP),
                                               idx.ctypes.data_as(C.POINTER(C.c_uint32))))
         tri = None
         if triangles:
             tri = np.zeros(S, dtype=np.uint32)
-            self._check(self.L.t3d_frame_download_triangles(self.ctx, self.frame,
+            self._check(self.L.t3d_frame_download_triangles(self.ctx, frame,
                                                             tri.ctypes.data_as(C.POINTER(C.c_uint32))))
         return Frame(pairs, color, depth, idx, tri, self.last_ms())
 

@@ -98,10 +98,32 @@ def axis_angle_matrix(axis, angle):
 # meshes
 # ---------------------------------------------------------------------------
 
+_ICO_CACHE = {}
+
+
+def _icosphere_topology(level):
+    """Unit-sphere vertices (float64) and outward-wound faces of a level."""
+    if level not in _ICO_CACHE:
+        _ICO_CACHE[level] = _build_icosphere_topology(level)
+    return _ICO_CACHE[level]
+
+
 def icosphere(level, radius=1.0, name="icosphere", color=None):
     """Shared-edge icosphere: 10*4^level + 2 vertices, 20*4^level triangles,
     smooth normals, outward counter-clockwise winding (front faces survive the
     screen-space backface test)."""
+    p, idx = _icosphere_topology(level)
+    pos = (p * radius).astype(f32)
+    nrm = p.astype(f32)
+    cols = None
+    if color is not None:
+        cols = np.tile(np.asarray(color, dtype=f32), (len(pos), 1))
+    uv = np.stack([0.5 + np.arctan2(p[:, 2], p[:, 0]) / (2 * np.pi), 0.5 + np.arcsin(np.clip(p[:, 1], -1, 1)) / np.pi],
+                  axis=1).astype(f32)
+    return Mesh(name, pos, nrm, idx.astype(np.uint32), colors=cols, uvs=uv)
+
+
+def _build_icosphere_topology(level):
     t = (1 + math.sqrt(5)) / 2
     verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
              (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
@@ -130,14 +152,9 @@ def icosphere(level, radius=1.0, name="icosphere", color=None):
     n = np.cross(p[idx[:, 1]] - p[idx[:, 0]], p[idx[:, 2]] - p[idx[:, 0]])
     flip = np.einsum("ij,ij->i", n, p[idx].sum(axis=1)) < 0
     idx[flip] = idx[flip][:, [0, 2, 1]]
-    pos = (p * radius).astype(f32)
-    nrm = p.astype(f32)
-    cols = None
-    if color is not None:
-        cols = np.tile(np.asarray(color, dtype=f32), (len(pos), 1))
-    uv = np.stack([0.5 + np.arctan2(p[:, 2], p[:, 0]) / (2 * np.pi), 0.5 + np.arcsin(np.clip(p[:, 1], -1, 1)) / np.pi],
-                  axis=1).astype(f32)
-    return Mesh(name, pos, nrm, idx.astype(np.uint32), colors=cols, uvs=uv)
+    p.setflags(write=False)
+    idx.setflags(write=False)
+    return p, idx
 
 
 def cube_mesh(w=1.0, h=1.0, d=1.0):
@@ -350,7 +367,7 @@ def heightfield_scene(seed=DEFAULT_SEED, n=1024, amplitude=20.0):
 
 def _mixed_mesh(rng, target_v, kind, idx):
     if kind == "sphere":
-        level = int(np.clip(round(math.log(max(target_v - 2, 10) / 10.0, 4)), 1, 6))
+        level = _sphere_level(target_v)
         return icosphere(level, 4.0, name="sphere%d" % idx, color=(1, 1, 1, 1))
     nx = max(2, int(round(math.sqrt(target_v))))
     nz = max(2, int(round(target_v / nx)))
@@ -363,8 +380,15 @@ def _skin(model, rng, bones=32, max_angle_deg=30.0):
     degrees composed with the model's placement (identity inverse-bind)."""
     mesh = model.mesh
     V = mesh.vertex_count
-    keys = rng.floats(V * bones).reshape(V, bones)
-    bidx = np.argsort(keys, axis=1)[:, :4].astype(np.uint16)
+    # four distinct bones per vertex: the first four steps of a Fisher-Yates
+    # shuffle of 0..bones-1
+    perm = np.tile(np.arange(bones, dtype=np.int64), (V, 1))
+    rows = np.arange(V)
+    for k in range(4):
+        j = k + rng.ints(0, bones - k, V)
+        pk, pj = perm[rows, k].copy(), perm[rows, j].copy()
+        perm[rows, k], perm[rows, j] = pj, pk
+    bidx = perm[:, :4].astype(np.uint16)
     e = -np.log(np.maximum(rng.floats(V * 4).astype(np.float64), 1e-12)).reshape(V, 4)
     w = (e / e.sum(axis=1, keepdims=True)).astype(f32)
     mesh.bone_indices = np.ascontiguousarray(bidx)
@@ -377,37 +401,62 @@ def _skin(model, rng, bones=32, max_angle_deg=30.0):
     model.bone_influences = pal.astype(f32)
 
 
-def mixed_models(rng, count, skinned_every=4, spread=(60.0, 60.0, 30.0, 250.0), vmin=24, vmax=65536):
+def _sphere_level(target_v):
+    return int(np.clip(round(math.log(max(target_v - 2, 10) / 10.0, 4)), 1, 6))
+
+
+def _mixed_one(args):
+    """Model i of a mixed set; its mesh and skin come from its own streams."""
+    i, target, pos, quat, seed, skinned, sphere = args
+    base = SplitMix64(seed)
+    mesh = sphere if sphere is not None else _mixed_mesh(base.child(16).child(i), target,
+                                                         "sphere" if i % 4 == 3 else "grid", i)
+    m = Model("model%d" % i, mesh)
+    m.set_local_position(*pos)
+    m.set_local_rotation(quat_to_matrix(*quat))
+    if skinned:
+        _skin(m, base.child(17).child(i))
+    return m
+
+
+def _mixed_chunk(items):
+    return [_mixed_one(a) for a in items]
+
+
+def mixed_models(rng, count, skinned_every=4, spread=(60.0, 60.0, 30.0, 250.0), vmin=24, vmax=65536, workers=1):
     """`count` models with vertex counts log-uniform in [vmin, vmax]: grids
     (with random vertex colours) and, every fourth model, shared-size
-    icospheres; one model in `skinned_every` is skinned."""
+    icospheres; one model in `skinned_every` is skinned.  Model i draws its
+    mesh and skin from streams of its own, so `workers` processes build the
+    same models as one."""
     sx, sy, z0, z1 = spread
     lv = rng.child(11).floats(count).astype(np.float64)
     targets = np.exp(np.log(vmin) + (np.log(vmax) - np.log(vmin)) * lv)
     pos = np.stack([rng.child(12).uniform(-sx, sx, count), rng.child(13).uniform(-sy, sy, count),
                     rng.child(14).uniform(-z1, -z0, count)], axis=1)
     quats = uniform_quaternions(rng.child(15), count)
-    models = []
     sphere_cache = {}
-    mesh_rng = rng.child(16)
-    skin_rng = rng.child(17)
+    items = []
     for i in range(count):
-        kind = "sphere" if i % 4 == 3 else "grid"
         skinned = (i % skinned_every) == 1
-        if kind == "sphere" and not skinned:
-            level = int(np.clip(round(math.log(max(targets[i] - 2, 10) / 10.0, 4)), 1, 6))
+        sphere = None
+        if i % 4 == 3 and not skinned:        # unskinned icospheres share one mesh per level
+            level = _sphere_level(targets[i])
             if level not in sphere_cache:
-                sphere_cache[level] = _mixed_mesh(mesh_rng, targets[i], kind, i)
-            mesh = sphere_cache[level]
-        else:
-            mesh = _mixed_mesh(mesh_rng, targets[i], kind, i)
-        m = Model("model%d" % i, mesh)
-        m.set_local_position(*pos[i])
-        m.set_local_rotation(quat_to_matrix(*quats[i]))
-        if skinned:
-            _skin(m, skin_rng)
-        models.append(m)
-    return models
+                sphere_cache[level] = _mixed_mesh(None, targets[i], "sphere", i)
+            sphere = sphere_cache[level]
+        items.append((i, float(targets[i]), pos[i], quats[i], int(rng.seed), skinned, sphere))
+    if workers <= 1 or count < 64:
+        return _mixed_chunk(items)
+    import multiprocessing as mp
+    work = [it for it in items if it[6] is None]
+    chunks = [work[k::workers] for k in range(workers)]
+    with mp.get_context("fork").Pool(workers) as pool:
+        built = {}
+        for res in pool.imap_unordered(_mixed_chunk, chunks):
+            for m in res:
+                built[int(m.name[5:])] = m
+    return [built[it[0]] if it[6] is None else _mixed_one(it) for it in items]
 
 
 def mixed_scene(seed=DEFAULT_SEED, count=512):
@@ -436,12 +485,13 @@ class Job:
         self.lights = lights
 
 
-def serving_trace(seed=DEFAULT_SEED, pool_size=10000, jobs=2000, rate_hz=2000.0, max_models=200, max_lights=16):
+def serving_trace(seed=DEFAULT_SEED, pool_size=10000, jobs=2000, rate_hz=2000.0, max_models=200, max_lights=16,
+                  workers=1):
     """A resident pool of `pool_size` models (the mixed distribution) and a
     Poisson job trace at `rate_hz`: each job has a seeded camera (640x360 to
     1920x1080), 1-`max_models` models and 1-`max_lights` lights."""
     rng = SplitMix64(seed)
-    models = mixed_models(rng.child(31), pool_size)
+    models = mixed_models(rng.child(31), pool_size, workers=workers)
     jr = rng.child(32)
     gaps = -np.log(np.maximum(jr.floats(jobs).astype(np.float64), 1e-12)) / rate_hz
     arrivals = np.cumsum(gaps)
