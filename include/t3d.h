@@ -332,6 +332,7 @@ typedef struct {
   uint32_t store_mode;               /* assembly stores: bit 0 = direct per-lane stores instead of
                                         LDS-staged rows, bit 1 = non-temporal (streaming) stores */
   uint32_t assemble_split;           /* assembly blocks per tile: a power of two, tile/split >= 64 */
+  uint32_t xcd_swizzle;              /* 1: each XCD takes a contiguous range of a launch's blocks */
 } t3d_tuning;
 t3d_status t3d_get_tuning(const t3d_ctx* ctx, t3d_tuning* out);
 t3d_status t3d_set_tuning(t3d_ctx* ctx, const t3d_tuning* tuning);

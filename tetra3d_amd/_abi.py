@@ -220,6 +220,7 @@ class Tuning(C.Structure):
         ("use_graph", C.c_uint32),
         ("store_mode", C.c_uint32),
         ("assemble_split", C.c_uint32),
+        ("xcd_swizzle", C.c_uint32),
     ]
 
 

@@ -1103,6 +1103,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   a.nparts = P_n;
   a.vchunk = vchunk;
   a.tile = tile;
+  a.xcd_swizzle = tu.xcd_swizzle ? 1u : 0u;
   a.lights = d_lights;
   a.plights = d_pl;
   a.vchunk_prefix = d_vpre;
@@ -1403,6 +1404,7 @@ t3d_status t3d_set_tuning(t3d_ctx* c, const t3d_tuning* t) {
   if (t->assemble_split == 0 || (t->assemble_split & (t->assemble_split - 1)) ||
       t->triangle_tile % t->assemble_split || t->triangle_tile / t->assemble_split < 64)
     return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: assemble_split must be a power of two with tile/split >= 64");
+  if (t->xcd_swizzle > 1) return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: xcd_swizzle is 0 or 1");
   c->tuning = *t;
   return T3D_OK;
 }
@@ -1444,6 +1446,7 @@ t3d_status t3d_autotune(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc, u
       {&t3d_tuning::scatter_waves, {1, 4}},
       {&t3d_tuning::assemble_split, {1, 2, 4, 8}},
       {&t3d_tuning::store_mode, {0, 2}},   // LDS-staged rows, plain or streaming stores
+      {&t3d_tuning::xcd_swizzle, {0, 1}},
   };
   t3d_tuning best = base;
   if (!valid(best)) best.assemble_split = 1;

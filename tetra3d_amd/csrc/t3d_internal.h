@@ -116,7 +116,7 @@ struct FrameArgs {
   uint32_t nparts;
   uint32_t vchunk;              // vertices per vertex-kernel block
   uint32_t tile;                // triangles per tile
-  uint32_t pad0;
+  uint32_t xcd_swizzle;         // 1: give each XCD a contiguous range of blocks (see block_index)
   const LightGPU* lights;
   const PartLight* plights;
   const uint32_t* vchunk_prefix;  // [P+1]

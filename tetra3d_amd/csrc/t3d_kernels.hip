@@ -40,6 +40,19 @@ __device__ __forceinline__ uint32_t find_part(const uint32_t* __restrict__ prefi
   return lo;
 }
 
+// Workgroups of a launch are dealt round-robin to the 8 XCDs (workgroup i to
+// XCD i % 8), each with its own L2.  With the swizzle on, the workgroups an
+// XCD receives take consecutive logical block numbers, so the blocks of one
+// render pair (which gather the same vertex records) share an L2.  The map
+// is a bijection of [0, n).
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t block_index(uint32_t swizzle) {
+  const uint32_t i = blockIdx.x, n = gridDim.x;
+  if (!swizzle) return i;
+  const uint32_t per = n / kXcds, rem = n % kXcds, x = i % kXcds;
+  return x * per + min(x, rem) + i / kXcds;
+}
+
 __device__ __forceinline__ float clampf(float v, float lo, float hi) {   // math32.go:38-45
   if (v < lo) return lo;
   else if (v > hi) return hi;
@@ -156,7 +169,7 @@ __device__ __forceinline__ void light_vertex(const LightGPU& l, const PartLight&
 // ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
-  const uint32_t b = blk0 + blockIdx.x;
+  const uint32_t b = blk0 + block_index(a.xcd_swizzle);
   const uint32_t p = find_part(a.vchunk_prefix, a.nparts, b);
   const PartGPU& P = a.parts[p];
   const uint32_t flags = P.flags;
@@ -315,7 +328,7 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 }
 
 __global__ __launch_bounds__(256) void k_cull(FrameArgs a, uint32_t blk0) {
-  const uint32_t b = blk0 + blockIdx.x;
+  const uint32_t b = blk0 + block_index(a.xcd_swizzle);
   const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
   const PartGPU& P = a.parts[p];
   const uint32_t flags = P.flags;
@@ -449,7 +462,7 @@ __device__ __forceinline__ void part_range(const uint32_t* stats, uint32_t p, fl
 }
 
 __global__ __launch_bounds__(64) void k_hist(FrameArgs a, uint32_t blk0) {
-  const uint32_t b = blk0 + blockIdx.x;
+  const uint32_t b = blk0 + block_index(a.xcd_swizzle);
   const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
   const PartGPU& P = a.parts[p];
   if (P.ntiles <= 1) return;
@@ -562,7 +575,7 @@ __global__ __launch_bounds__(256) void k_scan_down(FrameArgs a, uint32_t sblk0) 
 // visited 511->0 (BackToFront) or 0->511, insertion order kept within a bin.
 // The survivor's record {vertex-record offsets, triangle} moves to its slot.
 __global__ __launch_bounds__(64) void k_scatter(FrameArgs a, uint32_t blk0) {
-  const uint32_t b = blk0 + blockIdx.x;
+  const uint32_t b = blk0 + block_index(a.xcd_swizzle);
   const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
   const PartGPU& P = a.parts[p];
   if (a.stats[4 * p + 2] == 0) return;
@@ -649,7 +662,7 @@ __global__ __launch_bounds__(64) void k_scatter(FrameArgs a, uint32_t blk0) {
 // independently and the result is the single-wave order exactly.
 constexpr uint32_t kScatterWaves = 4;
 __global__ __launch_bounds__(256) void k_scatter4(FrameArgs a, uint32_t blk0) {
-  const uint32_t b = blk0 + blockIdx.x;
+  const uint32_t b = blk0 + block_index(a.xcd_swizzle);
   const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
   const PartGPU& P = a.parts[p];
   if (a.stats[4 * p + 2] == 0) return;
@@ -770,12 +783,13 @@ __device__ __forceinline__ void store1(uint32_t* p, uint32_t v) {
 // tile), so the grid follows the survivors more finely than the sort's tiles.
 template <bool kStage, bool kNT>
 __global__ __launch_bounds__(64) void k_assemble(FrameArgs a, uint32_t blk0, uint32_t split) {
-  const uint32_t b = blk0 + blockIdx.x / split;
+  const uint32_t li = block_index(a.xcd_swizzle);
+  const uint32_t b = blk0 + li / split;
   const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
   const PartGPU& P = a.parts[p];
   const uint32_t S = a.stats[4 * p + 2];
   const uint32_t sub = a.tile / split;
-  const uint32_t k0 = (b - a.tile_prefix[p]) * a.tile + (blockIdx.x % split) * sub;
+  const uint32_t k0 = (b - a.tile_prefix[p]) * a.tile + (li % split) * sub;
   if (k0 >= S) return;
   const uint32_t k1 = min(k0 + sub, S);
   const FrameConst& fc = a.fc[P.cam];
