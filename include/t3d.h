@@ -368,6 +368,7 @@ typedef struct {
   uint64_t survivors;                /* triangles drawn */
   uint64_t batch;                    /* batch the job ran in */
   uint32_t batch_jobs;               /* jobs in that batch */
+  uint64_t admitted_bytes;           /* frame memory the job counts against the budget */
 } t3d_job_info;
 typedef struct {
   uint64_t submitted, completed, failed, rejected, cancelled, batches;

@@ -54,6 +54,36 @@ def test_jobs_same_alone_and_in_mix(gpu_ctx, oracle, pool):
     compare_frames(alone[0], ref)
 
 
+def test_budget_limits_admission(gpu_ctx, pool):
+    """With room for about two jobs, jobs still all complete, in more and
+    smaller batches, and the bytes admitted at once never pass the budget."""
+    from tetra3d_amd.serving import Server
+    models, jobs, scene = pool
+    for m in models:
+        gpu_ctx.upload_mesh(m.mesh)
+    inputs = [gpu_ctx.inputs(j.camera, scene, j.lights, job_models(models, j)) for j in jobs]
+    with Server(gpu_ctx, slots=2, max_batch_jobs=8) as srv:
+        ids = [srv.submit_inputs(inp) for inp in inputs]
+        for i in ids:
+            srv.wait(i)
+        est = [srv.info(i).admitted_bytes for i in ids]
+        free = [srv.download(i) for i in ids]
+        for i in ids:
+            srv.release(i)
+    budget = 2 * max(est)
+    with Server(gpu_ctx, slots=2, max_batch_jobs=8, hbm_budget=budget) as srv:
+        ids = [srv.submit_inputs(inp) for inp in inputs]
+        peak = 0
+        for k, i in enumerate(ids):              # release as results are consumed
+            srv.wait(i)
+            peak = max(peak, srv.stats().memory_in_flight)
+            compare_frames(srv.download(i), free[k])
+            srv.release(i)
+        st = srv.stats()
+        print("budget", budget, "batches", st.batches, "peak admitted", peak, flush=True)
+        assert st.completed == len(jobs) and peak <= budget
+
+
 def test_budget_rejects_oversized_job(gpu_ctx, pool):
     from tetra3d_amd.runtime import T3DError
     from tetra3d_amd.serving import Server

@@ -3,14 +3,17 @@
 //
 // A job is one Camera.Render (camera.go:1269): a camera, a subset of the
 // resident models and a light list.  A worker thread owns `slots` launch
-// slots, each a frame (t3d_frame) with its own HIP stream.  Whenever a slot is
-// free, every admitted job queued at that moment (FIFO, up to max_batch_jobs,
+// slots, each a HIP stream with a completion event.  Whenever a slot is free,
+// every admitted job queued at that moment (FIFO, up to max_batch_jobs,
 // within the budget) becomes one batch: one launch chain whose render pairs
 // carry their job's camera index, so the batch's jobs share the kernels of
 // the geometry path.  Batches on different slots overlap on the device; jobs
-// arriving meanwhile wait for the next free slot.  A batched job's results
-// are bit-identical to rendering it alone: the render pairs, uniforms and
-// arithmetic of each job do not depend on the other jobs of its batch.
+// arriving meanwhile wait for the next free slot.  A batch renders into a
+// frame taken from a pool; when it completes the slot is free at once and
+// the frame stays with the batch until every job of the batch is released,
+// then returns to the pool.  A batched job's results are bit-identical to
+// rendering it alone: the render pairs, uniforms and arithmetic of each job
+// do not depend on the other jobs of its batch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -44,6 +47,8 @@ struct OwnedDesc {
   t3d_render_desc desc{};
 };
 
+struct Batch;
+
 struct Job {
   uint64_t id = 0;
   OwnedDesc d;
@@ -54,9 +59,9 @@ struct Job {
   t3d_status status = T3D_OK;
   std::string error;
   double submit_ms = 0, start_ms = 0, done_ms = 0;
-  int slot = -1;
+  Batch* batch = nullptr;            // while admitted and not released
   uint32_t index_in_batch = 0;
-  uint64_t batch = 0;
+  uint64_t batch_id = 0;
   uint32_t batch_jobs = 0;
   uint64_t survivors = 0;
   uint32_t pairs = 0;
@@ -64,18 +69,21 @@ struct Job {
   int pins = 0;                      // downloads in progress
 };
 
-// FREE: idle; RUNNING: batch on the device; HELD: finished, outputs kept until
-// every job of the batch is released; TRIM: idle, buffers being given back.
-enum SlotState { SLOT_FREE, SLOT_RUNNING, SLOT_HELD, SLOT_TRIM };
-
-struct Slot {
+// One launch chain: its frame holds the outputs of all its jobs until the
+// last of them is released.
+struct Batch {
+  uint64_t id = 0;
   t3d_frame* frame = nullptr;
-  hipStream_t stream = nullptr;
-  hipEvent_t done = nullptr;
-  SlotState state = SLOT_FREE;
   std::vector<uint64_t> jobs;
   uint32_t unreleased = 0;
-  uint64_t batch = 0;
+  bool running = true;
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  Batch* batch = nullptr;            // the batch on the device, or null when free
+  bool reserved = false;             // being launched outside the server lock
 };
 
 }  // namespace
@@ -89,6 +97,10 @@ struct t3d_server {
   std::deque<uint64_t> queue;
   std::unordered_map<uint64_t, std::unique_ptr<Job>> jobs;
   std::vector<Slot> slots;
+  std::unordered_map<uint64_t, std::unique_ptr<Batch>> batches_alive;
+  std::vector<t3d_frame*> free_frames;   // frames of finished, fully released batches
+  hipStream_t dl_stream = nullptr;       // downloads
+  std::mutex dl_mu;
   std::thread worker;
   bool stop = false;
   uint64_t next_id = 1, next_batch = 1;
@@ -179,32 +191,50 @@ void drop_job(t3d_server* s, Job& j) {
   j.meshes.clear();
 }
 
-// Called with s->mu held: frees a finished job's slot share.
+uint64_t frames_held(const t3d_server* s) {
+  uint64_t b = 0;
+  for (const auto& kv : s->batches_alive) b += frame_capacity(kv.second->frame);
+  for (const t3d_frame* f : s->free_frames) b += frame_capacity(f);
+  return b;
+}
+
+// Called with s->mu held: a batch whose jobs are all released gives its frame
+// back to the pool.
+void retire_batch(t3d_server* s, Batch* b) {
+  if (b->running || b->unreleased) return;
+  s->free_frames.push_back(b->frame);
+  const uint64_t id = b->id;
+  s->batches_alive.erase(id);   // deletes b
+}
+
+// Called with s->mu held: frees a finished job's share of its batch.
 void release_job(t3d_server* s, Job& j) {
   if (j.released) return;
   j.released = true;
   drop_job(s, j);
-  if (j.slot >= 0) {
-    Slot& sl = s->slots[j.slot];
-    s->inflight_bytes -= j.est_bytes;
-    if (sl.unreleased > 0 && --sl.unreleased == 0 && sl.state == SLOT_HELD) {
-      sl.state = SLOT_FREE;
-      sl.jobs.clear();
-    }
+  s->inflight_bytes -= j.est_bytes;
+  Batch* b = j.batch;
+  j.batch = nullptr;
+  if (b && b->unreleased > 0) {
+    b->unreleased--;
+    retire_batch(s, b);
   }
 }
 
 // Called with s->mu held, after the slot's event completed.
 void complete_slot(t3d_server* s, int si) {
   Slot& sl = s->slots[si];
-  frame_collect(s->ctx, sl.frame, false);
+  Batch* b = sl.batch;
+  sl.batch = nullptr;
+  b->running = false;
+  frame_collect(s->ctx, b->frame, false);
   const double now = s->now_ms();
-  for (uint32_t k = 0; k < sl.jobs.size(); k++) {
-    Job& j = *s->jobs[sl.jobs[k]];
-    const uint32_t p0 = sl.frame->job_pair_begin[k], p1 = sl.frame->job_pair_begin[k + 1];
+  for (uint32_t k = 0; k < b->jobs.size(); k++) {
+    Job& j = *s->jobs[b->jobs[k]];
+    const uint32_t p0 = b->frame->job_pair_begin[k], p1 = b->frame->job_pair_begin[k + 1];
     j.pairs = p1 - p0;
     j.survivors = 0;
-    for (uint32_t p = p0; p < p1; p++) j.survivors += sl.frame->pairs[p].survivor_count;
+    for (uint32_t p = p0; p < p1; p++) j.survivors += b->frame->pairs[p].survivor_count;
     j.state = T3D_JOB_DONE;
     j.done_ms = now;
     s->completed++;
@@ -212,15 +242,16 @@ void complete_slot(t3d_server* s, int si) {
     s->latencies.push_back(j.done_ms - j.submit_ms);
     s->last_done_ms = now;
   }
-  sl.state = SLOT_HELD;
   if (s->cfg.auto_release)
-    for (uint64_t id : std::vector<uint64_t>(sl.jobs)) release_job(s, *s->jobs[id]);
+    for (uint64_t id : std::vector<uint64_t>(b->jobs)) release_job(s, *s->jobs[id]);
+  else
+    retire_batch(s, b);
   s->cv_done.notify_all();
 }
 
-void fail_batch(t3d_server* s, int si, t3d_status st, const std::string& err) {
-  Slot& sl = s->slots[si];
-  for (uint64_t id : sl.jobs) {
+// Called with s->mu held: the batch never ran (or its launch failed).
+void fail_batch(t3d_server* s, Batch* b, t3d_status st, const std::string& err) {
+  for (uint64_t id : b->jobs) {
     Job& j = *s->jobs[id];
     j.state = T3D_JOB_FAILED;
     j.status = st;
@@ -228,13 +259,13 @@ void fail_batch(t3d_server* s, int si, t3d_status st, const std::string& err) {
     j.done_ms = s->now_ms();
     s->failed++;
     s->inflight_bytes -= j.est_bytes;
-    j.slot = -1;
+    j.batch = nullptr;
     j.released = true;
     drop_job(s, j);
   }
-  sl.jobs.clear();
-  sl.unreleased = 0;
-  sl.state = SLOT_FREE;
+  b->running = false;
+  b->unreleased = 0;
+  retire_batch(s, b);
   s->cv_done.notify_all();
 }
 
@@ -246,7 +277,7 @@ void worker_main(t3d_server* s) {
     bool running = false;
     for (int i = 0; i < (int)s->slots.size(); i++) {
       Slot& sl = s->slots[i];
-      if (sl.state != SLOT_RUNNING) continue;
+      if (!sl.batch || sl.reserved) continue;
       hipError_t q = hipEventQuery(sl.done);
       if (q == hipSuccess) {
         complete_slot(s, i);
@@ -254,7 +285,9 @@ void worker_main(t3d_server* s) {
       } else if (q == hipErrorNotReady) {
         running = true;
       } else {
-        fail_batch(s, i, T3D_ERR_DEVICE, hipGetErrorString(q));
+        Batch* b = sl.batch;
+        sl.batch = nullptr;
+        fail_batch(s, b, T3D_ERR_DEVICE, hipGetErrorString(q));
         progressed = true;
       }
     }
@@ -262,62 +295,75 @@ void worker_main(t3d_server* s) {
     // launch: every admitted job queued now goes into one batch on a free slot
     int free_slot = -1;
     for (int i = 0; i < (int)s->slots.size(); i++)
-      if (s->slots[i].state == SLOT_FREE) {
+      if (!s->slots[i].batch && !s->slots[i].reserved) {
         free_slot = i;
         break;
       }
     if (free_slot >= 0 && !s->queue.empty() && !s->stop) {
-      Slot& sl = s->slots[free_slot];
-      sl.jobs.clear();
-      while (!s->queue.empty() && sl.jobs.size() < s->cfg.max_batch_jobs) {
+      std::vector<uint64_t> take;
+      uint64_t need = 0;
+      while (!s->queue.empty() && take.size() < s->cfg.max_batch_jobs) {
         Job& j = *s->jobs[s->queue.front()];
         if (s->cfg.hbm_budget_bytes && s->inflight_bytes + j.est_bytes > s->cfg.hbm_budget_bytes) break;
         s->inflight_bytes += j.est_bytes;
-        sl.jobs.push_back(j.id);
+        need += j.est_bytes;
+        take.push_back(j.id);
         s->queue.pop_front();
       }
-      if (!sl.jobs.empty()) {
-        const uint64_t batch = s->next_batch++;
+      if (!take.empty()) {
+        Slot& sl = s->slots[free_slot];
+        auto bp = std::make_unique<Batch>();
+        Batch* b = bp.get();
+        b->id = s->next_batch++;
+        b->jobs = take;
+        b->unreleased = (uint32_t)take.size();
+        // the frame with the most capacity from the pool, else a new one
+        if (!s->free_frames.empty()) {
+          auto it = std::max_element(s->free_frames.begin(), s->free_frames.end(),
+                                     [](const t3d_frame* x, const t3d_frame* y) { return frame_capacity(x) < frame_capacity(y); });
+          b->frame = *it;
+          s->free_frames.erase(it);
+        } else {
+          b->frame = new t3d_frame();
+        }
+        // Pooled frames keep their buffers.  When the frames held plus this
+        // batch's need would pass the budget, idle pooled frames are freed.
+        std::vector<t3d_frame*> trim;
+        if (s->cfg.hbm_budget_bytes) {
+          uint64_t held = frames_held(s);
+          while (!s->free_frames.empty() && held + need + need / 4 > s->cfg.hbm_budget_bytes) {
+            held -= frame_capacity(s->free_frames.back());
+            trim.push_back(s->free_frames.back());
+            s->free_frames.pop_back();
+          }
+        }
         std::vector<const t3d_render_desc*> descs;
         const double now = s->now_ms();
-        for (uint32_t k = 0; k < sl.jobs.size(); k++) {
-          Job& j = *s->jobs[sl.jobs[k]];
+        for (uint32_t k = 0; k < take.size(); k++) {
+          Job& j = *s->jobs[take[k]];
           j.state = T3D_JOB_RUNNING;
           j.start_ms = now;
-          j.slot = free_slot;
+          j.batch = b;
           j.index_in_batch = k;
-          j.batch = batch;
-          j.batch_jobs = (uint32_t)sl.jobs.size();
+          j.batch_id = b->id;
+          j.batch_jobs = (uint32_t)take.size();
           descs.push_back(&j.d.desc);
         }
-        sl.state = SLOT_RUNNING;   // reserved while building outside the server lock
-        sl.unreleased = (uint32_t)sl.jobs.size();
-        sl.batch = batch;
+        s->batches_alive[b->id] = std::move(bp);
+        sl.batch = b;
+        sl.reserved = true;                // launched outside the server lock
         s->batches++;
-        // Frames keep their buffers between batches.  When the frames held by
-        // the other slots plus this batch's need would pass the budget, the
-        // buffers of idle slots are given back first.
-        std::vector<int> trim;
-        if (s->cfg.hbm_budget_bytes) {
-          uint64_t need = 0, held = 0;
-          for (uint64_t id : sl.jobs) need += s->jobs[id]->est_bytes;
-          for (int i = 0; i < (int)s->slots.size(); i++)
-            if (i != free_slot) held += frame_capacity(s->slots[i].frame);
-          for (int i = 0; i < (int)s->slots.size() && held + need + need / 4 > s->cfg.hbm_budget_bytes; i++)
-            if (i != free_slot && s->slots[i].state == SLOT_FREE && frame_capacity(s->slots[i].frame)) {
-              held -= frame_capacity(s->slots[i].frame);
-              trim.push_back(i);
-              s->slots[i].state = SLOT_TRIM;   // kept out of launches while trimmed outside the lock
-            }
-        }
         lk.unlock();
         t3d_status st;
         std::string err;
         {
           std::lock_guard<std::mutex> cl(s->ctx->mu);
-          for (int i : trim) frame_release(s->ctx, s->slots[i].frame);
-          st = frame_build(s->ctx, sl.frame, descs.data(), (uint32_t)descs.size());
-          if (st == T3D_OK) st = frame_launch(s->ctx, sl.frame, sl.stream, sl.stream, T3D_EXEC_BATCHED, false, false);
+          for (t3d_frame* f : trim) {
+            frame_release(s->ctx, f);
+            delete f;
+          }
+          st = frame_build(s->ctx, b->frame, descs.data(), (uint32_t)descs.size());
+          if (st == T3D_OK) st = frame_launch(s->ctx, b->frame, sl.stream, sl.stream, T3D_EXEC_BATCHED, false, false);
           if (st != T3D_OK) err = s->ctx->err;
         }
         if (st == T3D_OK && hipEventRecord(sl.done, sl.stream) != hipSuccess) {
@@ -325,8 +371,11 @@ void worker_main(t3d_server* s) {
           err = "hipEventRecord failed";
         }
         lk.lock();
-        for (int i : trim) s->slots[i].state = SLOT_FREE;
-        if (st != T3D_OK) fail_batch(s, free_slot, st, err);
+        sl.reserved = false;
+        if (st != T3D_OK) {
+          sl.batch = nullptr;
+          fail_batch(s, b, st, err);
+        }
         continue;
       }
     }
@@ -359,18 +408,18 @@ t3d_status t3d_server_create(t3d_ctx* c, const t3d_server_config* cfg, t3d_serve
   s->cfg = *cfg;
   s->t0 = Clock::now();
   s->slots.resize(cfg->slots);
-  for (auto& sl : s->slots) {
-    sl.frame = new t3d_frame();
-    if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) {
-      for (auto& x : s->slots) {
-        if (x.stream) (void)hipStreamDestroy(x.stream);
-        if (x.done) (void)hipEventDestroy(x.done);
-        delete x.frame;
-      }
-      delete s;
-      return fail(c, T3D_ERR_DEVICE, "server: stream/event creation failed");
+  bool ok = hipStreamCreateWithFlags(&s->dl_stream, hipStreamNonBlocking) == hipSuccess;
+  for (auto& sl : s->slots)
+    ok = ok && hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    for (auto& x : s->slots) {
+      if (x.stream) (void)hipStreamDestroy(x.stream);
+      if (x.done) (void)hipEventDestroy(x.done);
     }
+    if (s->dl_stream) (void)hipStreamDestroy(s->dl_stream);
+    delete s;
+    return fail(c, T3D_ERR_DEVICE, "server: stream/event creation failed");
   }
   s->worker = std::thread(worker_main, s);
   *out = s;
@@ -392,18 +441,27 @@ t3d_status t3d_server_destroy(t3d_server* s) {
   }
   s->cv_work.notify_all();
   s->worker.join();
+  std::vector<t3d_frame*> frames;
   {
     std::lock_guard<std::mutex> lk(s->mu);
     for (auto& kv : s->jobs) release_job(s, *kv.second);
+    for (auto& kv : s->batches_alive) frames.push_back(kv.second->frame);   // none once all are released
+    s->batches_alive.clear();
+    frames.insert(frames.end(), s->free_frames.begin(), s->free_frames.end());
+    s->free_frames.clear();
   }
   {
     std::lock_guard<std::mutex> cl(s->ctx->mu);
     for (auto& sl : s->slots) {
       (void)hipStreamSynchronize(sl.stream);
-      frame_release(s->ctx, sl.frame);
-      delete sl.frame;
       (void)hipEventDestroy(sl.done);
       (void)hipStreamDestroy(sl.stream);
+    }
+    (void)hipStreamSynchronize(s->dl_stream);
+    (void)hipStreamDestroy(s->dl_stream);
+    for (t3d_frame* f : frames) {
+      frame_release(s->ctx, f);
+      delete f;
     }
   }
   delete s;
@@ -495,15 +553,16 @@ t3d_status t3d_job_info_get(t3d_server* s, uint64_t job, t3d_job_info* out) {
   out->pair_count = j.pairs;
   out->triangles = j.triangles;
   out->survivors = j.survivors;
-  out->batch = j.batch;
+  out->batch = j.batch_id;
   out->batch_jobs = j.batch_jobs;
+  out->admitted_bytes = j.est_bytes;
   return T3D_OK;
 }
 
 t3d_status t3d_job_download(t3d_server* s, uint64_t job, t3d_pair_result* pairs, t3d_vertex* cv, t3d_vertex* dv,
                             uint32_t* idx, uint32_t* tri_ids) {
   if (!s) return T3D_ERR_INVALID_ARGUMENT;
-  Slot* sl = nullptr;
+  const t3d_frame* f = nullptr;
   uint32_t p0 = 0, p1 = 0;
   Job* jp = nullptr;
   {
@@ -512,14 +571,14 @@ t3d_status t3d_job_download(t3d_server* s, uint64_t job, t3d_pair_result* pairs,
     if (it == s->jobs.end()) return T3D_ERR_NOT_FOUND;
     Job& j = *it->second;
     if (j.state != T3D_JOB_DONE) return job_status(j) == T3D_OK ? T3D_ERR_NOT_READY : job_status(j);
-    if (j.released) return sfail(s, T3D_ERR_NOT_FOUND, "job_download: the job's outputs were released");
-    sl = &s->slots[j.slot];
-    p0 = sl->frame->job_pair_begin[j.index_in_batch];
-    p1 = sl->frame->job_pair_begin[j.index_in_batch + 1];
+    if (j.released || !j.batch) return sfail(s, T3D_ERR_NOT_FOUND, "job_download: the job's outputs were released");
+    f = j.batch->frame;   // stays with the batch while this job is unreleased (pins block release)
+    p0 = f->job_pair_begin[j.index_in_batch];
+    p1 = f->job_pair_begin[j.index_in_batch + 1];
     if (pairs) {
       uint64_t off = 0;
       for (uint32_t p = p0; p < p1; p++) {
-        pairs[p - p0] = sl->frame->pairs[p];
+        pairs[p - p0] = f->pairs[p];
         pairs[p - p0].vertex_offset = off;
         off += 3ull * pairs[p - p0].survivor_count;
       }
@@ -530,7 +589,8 @@ t3d_status t3d_job_download(t3d_server* s, uint64_t job, t3d_pair_result* pairs,
   t3d_status st = T3D_OK;
   if (cv || dv || idx || tri_ids) {
     (void)hipSetDevice(s->ctx->device);
-    st = frame_download_range(s->ctx, sl->frame, p0, p1, sl->stream, cv, dv, idx, tri_ids);
+    std::lock_guard<std::mutex> dl(s->dl_mu);   // one download at a time on the download stream
+    st = frame_download_range(s->ctx, f, p0, p1, s->dl_stream, cv, dv, idx, tri_ids);
   }
   std::lock_guard<std::mutex> lk(s->mu);
   jp->pins--;
@@ -574,7 +634,7 @@ t3d_status t3d_server_drain(t3d_server* s, uint32_t timeout_ms) {
   auto deadline = Clock::now() + std::chrono::milliseconds(timeout_ms);
   for (;;) {
     bool busy = !s->queue.empty();
-    for (auto& sl : s->slots) busy |= sl.state == SLOT_RUNNING;
+    for (auto& sl : s->slots) busy |= sl.batch != nullptr;
     if (!busy) return T3D_OK;
     if (s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout) return T3D_ERR_NOT_READY;
   }
@@ -592,7 +652,7 @@ t3d_status t3d_server_stats_get(t3d_server* s, t3d_server_stats* out) {
   out->batches = s->batches;
   out->triangles = s->tris_done;
   out->queue_depth = (uint32_t)s->queue.size();
-  for (auto& sl : s->slots) out->running_batches += sl.state == SLOT_RUNNING;
+  for (auto& sl : s->slots) out->running_batches += sl.batch != nullptr;
   out->memory_in_flight = s->inflight_bytes;
   if (!s->latencies.empty()) {
     std::vector<double> v = s->latencies;

@@ -35,6 +35,7 @@ class JobInfo(C.Structure):
         ("survivors", C.c_uint64),
         ("batch", C.c_uint64),
         ("batch_jobs", C.c_uint32),
+        ("admitted_bytes", C.c_uint64),
     ]
 
 
