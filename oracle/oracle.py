@@ -5,6 +5,7 @@ by bench.py's CPU-baseline leg, always as the checker.  The product package
 never imports this module.
 """
 import ctypes as C
+import time
 import os
 import subprocess
 
@@ -139,6 +140,19 @@ class Oracle:
             return Frame(res)
         finally:
             self.L.t3do_result_free(C.byref(res))
+
+    def render_seconds(self, camera, scene, lights, models):
+        """Seconds the C restatement takes for one Camera.Render (descriptor
+        building and the conversion of its outputs to numpy excluded)."""
+        inp = RenderInputs(camera, scene, lights, models, self.handle_of)
+        res = OracleResult()
+        t0 = time.perf_counter()
+        rc = self.L.t3do_render(self.ctx, C.byref(inp.render), C.byref(res))
+        el = time.perf_counter() - t0
+        self.L.t3do_result_free(C.byref(res))
+        if rc != 0:
+            raise RuntimeError("oracle render (%d): %s" % (rc, self.L.t3do_last_error(self.ctx).decode()))
+        return el
 
     def render_workload(self, wl):
         """Register the workload's meshes/volumes (handles = the ones the GPU

@@ -80,6 +80,34 @@ def test_async_frames_overlap_and_match(gpu_ctx, oracle):
         compare_frames(gpu_ctx.download(frame=fb), oracle.render_workload(wb))
 
 
+FULL = {
+    "icosphere_642v": lambda: W.icosphere_scene(),
+    "cubes_4096": lambda: W.cubes_scene(),
+    "heightfield_1024": lambda: W.heightfield_scene(),
+    "mixed_512": lambda: W.mixed_scene(),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scene", list(FULL))
+def test_full_size_parity(gpu_ctx, oracle, scene):
+    """The measured configurations at their full sizes, autotuned launch
+    shape included, bit-exact against the oracle (a few seconds each)."""
+    wl = FULL[scene]()
+    print("full-size parity", scene, "tris", wl.triangle_count, flush=True)
+    saved = gpu_ctx.tuning()
+    try:
+        if scene == "mixed_512":
+            gpu_ctx.prepare(wl)
+            gpu_ctx.autotune(gpu_ctx.inputs(wl.camera, wl.scene, wl.lights, wl.models), reps=1)
+        g = gpu_ctx.render_workload(wl)
+    finally:
+        gpu_ctx.L.t3d_set_tuning(gpu_ctx.ctx, saved)
+    s = compare_frames(g, oracle.render_workload(wl))
+    print(s, flush=True)
+    assert s["survivors"] > 0
+
+
 @pytest.mark.gpu
 def test_device_float_primitives_match_host(gpu_ctx):
     """Division, sqrt and round on the device round exactly as IEEE float32 on the host."""
