@@ -330,7 +330,8 @@ typedef struct {
   uint32_t scatter_waves;            /* wavefronts per tile in the sort's scatter pass: 1 or 4 */
   uint32_t use_graph;                /* replay the batched chain from a HIP graph */
   uint32_t store_mode;               /* assembly stores: bit 0 = direct per-lane stores instead of
-                                        LDS-staged rows, bit 1 = non-temporal (streaming) stores */
+                                        LDS-staged rows, bit 1 = non-temporal (streaming) stores,
+                                        bit 2 = stage 32 rows at a time instead of 64 (0-4 or 6) */
   uint32_t assemble_split;           /* assembly blocks per tile: a power of two, tile/split >= 64 */
   uint32_t xcd_swizzle;              /* 1: each XCD takes a contiguous range of a launch's blocks */
 } t3d_tuning;

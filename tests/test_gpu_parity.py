@@ -16,7 +16,8 @@ SCENES = {
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("store_mode", [0, 1, 2, 3], ids=["staged", "direct", "staged_nt", "direct_nt"])
+@pytest.mark.parametrize("store_mode", [0, 1, 2, 3, 4, 6],
+                         ids=["staged", "direct", "staged_nt", "direct_nt", "half", "half_nt"])
 @pytest.mark.parametrize("mode", [A.EXEC_BATCHED, A.EXEC_PLAIN], ids=["batched", "plain"])
 @pytest.mark.parametrize("scene", list(SCENES))
 def test_scene_parity(gpu_ctx, oracle, scene, mode, store_mode):

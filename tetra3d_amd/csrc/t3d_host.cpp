@@ -1398,7 +1398,8 @@ t3d_status t3d_set_tuning(t3d_ctx* c, const t3d_tuning* t) {
     return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: vertices_per_thread in [1, 64]");
   if (t->triangle_tile < 64 || t->triangle_tile > kMaxTile || t->triangle_tile % 64)
     return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: triangle_tile must be a multiple of 64 in [64, %u]", kMaxTile);
-  if (t->store_mode > 3) return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: store_mode in [0, 3]");
+  if (t->store_mode > 6 || t->store_mode == 5)
+    return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: store_mode is 0, 1, 2, 3, 4 or 6");
   if (t->scatter_waves != 1 && t->scatter_waves != 4)
     return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: scatter_waves must be 1 or 4");
   if (t->assemble_split == 0 || (t->assemble_split & (t->assemble_split - 1)) ||
@@ -1445,7 +1446,7 @@ t3d_status t3d_autotune(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc, u
       {&t3d_tuning::vertices_per_thread, {1, 2, 4}},
       {&t3d_tuning::scatter_waves, {1, 4}},
       {&t3d_tuning::assemble_split, {1, 2, 4, 8}},
-      {&t3d_tuning::store_mode, {0, 2}},   // LDS-staged rows, plain or streaming stores
+      {&t3d_tuning::store_mode, {0, 2, 4, 6}},   // LDS-staged rows (64 or 32 at a time), plain or streaming stores
       {&t3d_tuning::xcd_swizzle, {0, 1}},
   };
   t3d_tuning best = base;

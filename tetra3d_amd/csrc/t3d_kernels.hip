@@ -781,7 +781,7 @@ __device__ __forceinline__ void store1(uint32_t* p, uint32_t v) {
 // lane stores its own 144-byte runs); kNT: non-temporal (streaming) stores.
 // `split` blocks share one tile's slot range (a power of two dividing the
 // tile), so the grid follows the survivors more finely than the sort's tiles.
-template <bool kStage, bool kNT>
+template <int kStage, bool kNT>
 __global__ __launch_bounds__(64) void k_assemble(FrameArgs a, uint32_t blk0, uint32_t split) {
   const uint32_t li = block_index(a.xcd_swizzle);
   const uint32_t b = blk0 + li / split;
@@ -804,7 +804,10 @@ __global__ __launch_bounds__(64) void k_assemble(FrameArgs a, uint32_t blk0, uin
   uint32_t* out_i = a.out_idx + P.out_base;
   const float da = fc.render_depth ? 1.0f : 0.0f;
 
-  __shared__ float4 stage[kStage ? 64 * 9 : 1];
+  // kStage 1: all 64 triangles staged at once; 2: 32 at a time, halving the
+  // LDS per wavefront so twice as many wavefronts fit on a CU
+  constexpr uint32_t kRows = kStage == 2 ? 32 : 64;
+  __shared__ float4 stage[kStage ? kRows * 9 : 1];
   for (uint32_t wk0 = k0; wk0 < k1; wk0 += 64) {
     const uint32_t k = wk0 + lane;
     const bool active = k < k1;
@@ -851,18 +854,22 @@ __global__ __launch_bounds__(64) void k_assemble(FrameArgs a, uint32_t blk0, uin
       }
     }
     if (kStage) {
-      if (active)
-        for (int q = 0; q < 9; q++) stage[lane * 9 + q] = cq[q];
-      __syncthreads();
-      for (uint32_t f = lane; f < wcount * 9; f += 64) store4<kNT>(&out_c[(size_t)wk0 * 9 + f], stage[f]);
-      __syncthreads();
-      if (active)
-        for (int vi = 0; vi < 3; vi++) {
-          stage[lane * 9 + 3 * vi + 1] = make_float4(dd[vi], dd[vi], dd[vi], da);
-        }
-      __syncthreads();
-      for (uint32_t f = lane; f < wcount * 9; f += 64) store4<kNT>(&out_d[(size_t)wk0 * 9 + f], stage[f]);
-      __syncthreads();
+      for (uint32_t h = 0; h < 64 / kRows; h++) {
+        const uint32_t r0 = h * kRows;                              // first row of this round
+        const uint32_t rows = wcount > r0 ? min(kRows, wcount - r0) : 0u;
+        const bool mine = active && lane >= r0 && lane < r0 + kRows;
+        const uint32_t row = lane - r0;
+        if (mine)
+          for (int q = 0; q < 9; q++) stage[row * 9 + q] = cq[q];
+        __syncthreads();
+        for (uint32_t f = lane; f < rows * 9; f += 64) store4<kNT>(&out_c[(size_t)(wk0 + r0) * 9 + f], stage[f]);
+        __syncthreads();
+        if (mine)
+          for (int vi = 0; vi < 3; vi++) stage[row * 9 + 3 * vi + 1] = make_float4(dd[vi], dd[vi], dd[vi], da);
+        __syncthreads();
+        for (uint32_t f = lane; f < rows * 9; f += 64) store4<kNT>(&out_d[(size_t)(wk0 + r0) * 9 + f], stage[f]);
+        __syncthreads();
+      }
     } else if (active) {
       float4* oc = out_c + (size_t)k * 9;
       float4* od = out_d + (size_t)k * 9;
@@ -936,11 +943,14 @@ hipError_t launch_assemble(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uin
   if (!nblk) return hipSuccess;
   if (split == 0 || (split & (split - 1)) || a.tile % split) return hipErrorInvalidValue;
   const dim3 g(nblk * split);
-  switch (mode & 3u) {
-    case 0: hipLaunchKernelGGL((k_assemble<true, false>), g, dim3(64), 0, s, a, blk0, split); break;
-    case 1: hipLaunchKernelGGL((k_assemble<false, false>), g, dim3(64), 0, s, a, blk0, split); break;
-    case 2: hipLaunchKernelGGL((k_assemble<true, true>), g, dim3(64), 0, s, a, blk0, split); break;
-    default: hipLaunchKernelGGL((k_assemble<false, true>), g, dim3(64), 0, s, a, blk0, split); break;
+  switch (mode) {
+    case 0: hipLaunchKernelGGL((k_assemble<1, false>), g, dim3(64), 0, s, a, blk0, split); break;
+    case 1: hipLaunchKernelGGL((k_assemble<0, false>), g, dim3(64), 0, s, a, blk0, split); break;
+    case 2: hipLaunchKernelGGL((k_assemble<1, true>), g, dim3(64), 0, s, a, blk0, split); break;
+    case 3: hipLaunchKernelGGL((k_assemble<0, true>), g, dim3(64), 0, s, a, blk0, split); break;
+    case 4: hipLaunchKernelGGL((k_assemble<2, false>), g, dim3(64), 0, s, a, blk0, split); break;
+    case 6: hipLaunchKernelGGL((k_assemble<2, true>), g, dim3(64), 0, s, a, blk0, split); break;
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
