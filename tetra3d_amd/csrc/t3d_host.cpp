@@ -1014,6 +1014,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
     }
     f->job_pair_begin.push_back((uint32_t)f->pairs.size());
   }
+  const auto wall1 = std::chrono::steady_clock::now();   // render pairs and uniforms done
   const uint32_t P_n = (uint32_t)f->parts.size();
   f->nparts = P_n;
 
@@ -1127,9 +1128,13 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   f->total_sblocks = sblocks;
   f->total_triangles = sumT;
   if (c->profiling) {   // the caller holds ctx->mu
+    const auto wall2 = std::chrono::steady_clock::now();
     auto& hp = c->prof["host:prologue"];
-    hp.ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    hp.ms += std::chrono::duration<double, std::milli>(wall2 - wall0).count();
     hp.launches++;
+    auto& hu = c->prof["host:pairs+uniforms"];
+    hu.ms += std::chrono::duration<double, std::milli>(wall1 - wall0).count();
+    hu.launches++;
   }
   return T3D_OK;
 }

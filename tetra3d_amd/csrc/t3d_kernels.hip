@@ -808,14 +808,17 @@ __global__ __launch_bounds__(64) void k_assemble(FrameArgs a, uint32_t blk0, uin
   // LDS per wavefront so twice as many wavefronts fit on a CU
   constexpr uint32_t kRows = kStage == 2 ? 32 : 64;
   __shared__ float4 stage[kStage ? kRows * 9 : 1];
+  uint4 vo_next = make_uint4(0, 0, 0, 0);   // slot records one chunk ahead
+  if (k0 + lane < k1) vo_next = tvo[k0 + lane];
   for (uint32_t wk0 = k0; wk0 < k1; wk0 += 64) {
     const uint32_t k = wk0 + lane;
     const bool active = k < k1;
     const uint32_t wcount = min(64u, k1 - wk0);
+    const uint4 vo = vo_next;
+    if (k + 64 < k1) vo_next = tvo[k + 64];
     float4 cq[9];
     float dd[3];
     if (active) {
-      const uint4 vo = tvo[k];
       const uint32_t ov[3] = {vo.x, vo.y, vo.z};
       for (int vi = 0; vi < 3; vi++) {
         const uint32_t o = ov[vi];
