@@ -1038,9 +1038,10 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
     HIP_TRY(c, hipHostMalloc(&f->pinned, f->pinned_cap, hipHostMallocDefault));
   }
   // scratch: vertex records, skinned positions, keys, survivor records, stats, histograms, block sums
+  const uint32_t ntiles_total = f->tile_prefix.back();
   const uint64_t sbytes = carve_size<float4>(2 * sumV) + carve_size<float4>(sumV) + carve_size<uint32_t>(sumT) +
                           2 * carve_size<uint4>(sumT) + carve_size<uint32_t>(4ull * P_n) + carve_size<uint32_t>(sumHist) +
-                          carve_size<uint32_t>(sblocks);
+                          carve_size<uint32_t>(sblocks) + carve_size<uint32_t>(ntiles_total);
   st = ensure(c, f->scratch, f->scratch_cap, sbytes);
   if (st) return st;
   const uint64_t obytes = 2 * carve_size<t3d_vertex>(3 * sumT) + carve_size<uint32_t>(3 * sumT);
@@ -1118,6 +1119,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   a.stats = sc.take<uint32_t>(4ull * P_n);
   a.hist = sc.take<uint32_t>(sumHist);
   a.bsum = sc.take<uint32_t>(sblocks);
+  a.tile_count = sc.take<uint32_t>(ntiles_total);
   a.out_color = reinterpret_cast<float4*>(oc.take<t3d_vertex>(3 * sumT));
   a.out_depth = reinterpret_cast<float4*>(oc.take<t3d_vertex>(3 * sumT));
   a.out_idx = oc.take<uint32_t>(3 * sumT);
@@ -1243,9 +1245,11 @@ t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uin
 }
 
 uint64_t estimate_job_bytes(t3d_ctx* c, const t3d_render_desc* desc) {
-  // scratch: vrec + vpos (48 B/V); key + two records (36 B/T); outputs: 3 x (48 + 48 + 4) = 300 B/T;
+  // scratch: vrec + vpos (48 B/V); key + two records (36 B/T); per tile a 512-bin histogram column,
+  // a scan block sum share and a survivor count (~2 KB); outputs: 3 x (48 + 48 + 4) = 300 B/T;
   // tables: ~1 KB per model plus palettes
-  uint64_t V = 0, T = 0, extra = 64 * 1024;
+  const uint64_t tile = std::max<uint32_t>(64, c->tuning.triangle_tile);
+  uint64_t V = 0, T = 0, tiles = 0, extra = 64 * 1024;
   for (uint32_t mi = 0; mi < desc->model_count; mi++) {
     const t3d_model_desc& md = desc->models[mi];
     auto it = c->meshes.find(md.mesh);
@@ -1254,10 +1258,11 @@ uint64_t estimate_job_bytes(t3d_ctx* c, const t3d_render_desc* desc) {
     for (size_t p = 0; p < m.parts.size(); p++) {
       V += m.part_vcount[p];
       T += m.parts[p].triangle_count;
+      tiles += (m.parts[p].triangle_count + tile - 1) / tile;
     }
     extra += 1024 + 64ull * (md.skinned ? md.bone_count : 0);
   }
-  return 48 * V + 336 * T + extra;
+  return 48 * V + 336 * T + (4ull * kBins + 16) * tiles + extra;
 }
 
 void frame_release(t3d_ctx* c, t3d_frame* f) {

@@ -128,6 +128,7 @@ struct FrameArgs {
   uint4* tri_rec;               // [sumT] survivors, triangle order: {vertex-record offsets x3, triangle}
   uint4* tri_vo;                // [sumT] the same records in output-slot order
   uint32_t* stats;              // [P][4]: min bits, max bits, survivors, 0
+  uint32_t* tile_count;         // [tiles] survivors of each tile (compacted at the tile's front)
   uint32_t* hist;
   uint32_t* bsum;
   float4* out_color;            // t3d_vertex as 3 float4

@@ -327,6 +327,7 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   return v;
 }
 
+constexpr uint32_t kCullWaves = 4;   // k_cull runs 256-thread blocks
 __global__ __launch_bounds__(256) void k_cull(FrameArgs a, uint32_t blk0) {
   const uint32_t b = blk0 + block_index(a.xcd_swizzle);
   const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
@@ -340,100 +341,114 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a, uint32_t blk0) {
   const V3 camPos = v3(fc.campos[0], fc.campos[1], fc.campos[2]);
   const V3 icp = v3(P.icp[0], P.icp[1], P.icp[2]);
 
+  // Survivors are compacted in triangle order to the front of the tile's
+  // range of key[] and tri_rec[]; tile_count[b] says how many there are.
   float mn = kMaxF32, mx = 0.0f;
-  uint32_t cnt = 0;
-  for (uint32_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
-    const uint32_t i0 = P.idx[3 * (size_t)t + 0], i1 = P.idx[3 * (size_t)t + 1], i2 = P.idx[3 * (size_t)t + 2];
-    const uint32_t vi[3] = {i0, i1, i2};
-    float nx[3], ny[3];
-    uint32_t ov[3];
-    bool outOfBounds = true;
-    V3 stc = v3(0, 0, 0);
-    for (int k = 0; k < 3; k++) {                                   // model.go:855-886
-      const uint32_t o = P.vbase + (vi[k] - P.vmin);
-      ov[k] = o;
-      const float4 c = a.vrec[2 * (size_t)o];
-      if (skinned) {
-        const float4 sp = a.vpos[o];
-        stc.x += sp.x;
-        stc.y += sp.y;
-        stc.z += sp.z;
-      }
-      float w = c.w;
-      if (w < 0) w = 0.000001f;
-      nx[k] = fdiv(c.x, w);
-      ny[k] = fdiv(c.y, w);
-      if (c.z + 1.0f >= fc.near_ && c.z < fc.far_) outOfBounds = false;
-    }
-    bool keep = !outOfBounds;
-    if (keep && ((nx[0] < -0.5f && nx[1] < -0.5f && nx[2] < -0.5f) ||      // model.go:890-898
-                 (nx[0] > 0.5f && nx[1] > 0.5f && nx[2] > 0.5f) ||
-                 (ny[0] < -0.5f && ny[1] < -0.5f && ny[2] < -0.5f) ||
-                 (ny[0] > 0.5f && ny[1] > 0.5f && ny[2] > 0.5f)))
-      keep = false;
-    if (keep && backface) {                                          // model.go:902-925
-      const float n0x = nx[0] - nx[1];
-      const float n0y = ny[0] - ny[1];
-      const float n1x = nx[1] - nx[2];
-      const float n1y = ny[1] - ny[2];
-      const float nor = (n0x * n1y) - (n1x * n0y);
-      if (nor < 0) keep = false;
-    }
+  const uint32_t wv = threadIdx.x / warpSize, ln = threadIdx.x % warpSize;
+  const uint64_t lt_mask = (ln == 0) ? 0ull : (~0ull >> (64 - ln));
+  __shared__ uint32_t s_wc[kCullWaves];
+  uint32_t run = 0;                                 // survivors so far, block-uniform
+  for (uint32_t base = t0; base < t1; base += blockDim.x) {
+    const uint32_t t = base + threadIdx.x;
+    bool keep = false;
     float depth = -1.0f;
-    if (keep) {                                                       // model.go:941-960
-      if (skinned) {
-        const float dx = camPos.x - fdiv(stc.x, 3.0f);
-        const float dy = camPos.y - fdiv(stc.y, 3.0f);
-        const float dz = camPos.z - fdiv(stc.z, 3.0f);
-        depth = dx * dx + dy * dy + dz * dz;
-      } else {
-        // Triangle.Center from the base positions (mesh.go:1980-1982)
-        const V3 pa = ld3(P.pos, i0), pb = ld3(P.pos, i1), pc = ld3(P.pos, i2);
-        const float cx = fdiv(pa.x + pb.x + pc.x, 3.0f);
-        const float cy = fdiv(pa.y + pb.y + pc.y, 3.0f);
-        const float cz = fdiv(pa.z + pb.z + pc.z, 3.0f);
-        const float dx = icp.x - cx;
-        const float dy = icp.y - cy;
-        const float dz = icp.z - cz;
-        depth = dx * dx + dy * dy + dz * dz;
+    uint32_t ov[3] = {0, 0, 0};
+    if (t < t1) {
+      const uint32_t i0 = P.idx[3 * (size_t)t + 0], i1 = P.idx[3 * (size_t)t + 1], i2 = P.idx[3 * (size_t)t + 2];
+      const uint32_t vi[3] = {i0, i1, i2};
+      float nx[3], ny[3];
+      bool outOfBounds = true;
+      V3 stc = v3(0, 0, 0);
+      for (int k = 0; k < 3; k++) {                                   // model.go:855-886
+        const uint32_t o = P.vbase + (vi[k] - P.vmin);
+        ov[k] = o;
+        const float4 c = a.vrec[2 * (size_t)o];
+        if (skinned) {
+          const float4 sp = a.vpos[o];
+          stc.x += sp.x;
+          stc.y += sp.y;
+          stc.z += sp.z;
+        }
+        float w = c.w;
+        if (w < 0) w = 0.000001f;
+        nx[k] = fdiv(c.x, w);
+        ny[k] = fdiv(c.y, w);
+        if (c.z + 1.0f >= fc.near_ && c.z < fc.far_) outOfBounds = false;
       }
-      if (isnan(depth) || isinf(depth)) {
+      keep = !outOfBounds;
+      if (keep && ((nx[0] < -0.5f && nx[1] < -0.5f && nx[2] < -0.5f) ||      // model.go:890-898
+                   (nx[0] > 0.5f && nx[1] > 0.5f && nx[2] > 0.5f) ||
+                   (ny[0] < -0.5f && ny[1] < -0.5f && ny[2] < -0.5f) ||
+                   (ny[0] > 0.5f && ny[1] > 0.5f && ny[2] > 0.5f)))
         keep = false;
-        depth = -1.0f;
+      if (keep && backface) {                                          // model.go:902-925
+        const float n0x = nx[0] - nx[1];
+        const float n0y = ny[0] - ny[1];
+        const float n1x = nx[1] - nx[2];
+        const float n1y = ny[1] - ny[2];
+        const float nor = (n0x * n1y) - (n1x * n0y);
+        if (nor < 0) keep = false;
+      }
+      if (keep) {                                                       // model.go:941-960
+        if (skinned) {
+          const float dx = camPos.x - fdiv(stc.x, 3.0f);
+          const float dy = camPos.y - fdiv(stc.y, 3.0f);
+          const float dz = camPos.z - fdiv(stc.z, 3.0f);
+          depth = dx * dx + dy * dy + dz * dz;
+        } else {
+          // Triangle.Center from the base positions (mesh.go:1980-1982)
+          const V3 pa = ld3(P.pos, i0), pb = ld3(P.pos, i1), pc = ld3(P.pos, i2);
+          const float cx = fdiv(pa.x + pb.x + pc.x, 3.0f);
+          const float cy = fdiv(pa.y + pb.y + pc.y, 3.0f);
+          const float cz = fdiv(pa.z + pb.z + pc.z, 3.0f);
+          const float dx = icp.x - cx;
+          const float dy = icp.y - cy;
+          const float dz = icp.z - cz;
+          depth = dx * dx + dy * dy + dz * dz;
+        }
+        if (isnan(depth) || isinf(depth)) {
+          keep = false;
+          depth = -1.0f;
+        }
       }
     }
-    a.key[P.tbase + t] = depth;
+    const uint64_t m = __ballot(keep);
+    if (ln == 0) s_wc[wv] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t wbase = 0, tot = 0;
+    for (uint32_t w = 0; w < kCullWaves; w++) {
+      if (w < wv) wbase += s_wc[w];
+      tot += s_wc[w];
+    }
     if (keep) {
-      a.tri_rec[P.tbase + t] = make_uint4(ov[0], ov[1], ov[2], t);
+      const uint32_t pos = P.tbase + t0 + run + wbase + (uint32_t)__popcll(m & lt_mask);
+      a.key[pos] = depth;
+      a.tri_rec[pos] = make_uint4(ov[0], ov[1], ov[2], t);
       mn = fminf(mn, depth);
       mx = fmaxf(mx, depth);
-      cnt++;
     }
+    run += tot;
+    __syncthreads();                              // s_wc is rewritten next round
   }
   // block reduction, one atomic of each kind per block
-  __shared__ float s_mn[8], s_mx[8];
-  __shared__ uint32_t s_cnt[8];
+  __shared__ float s_mn[kCullWaves], s_mx[kCullWaves];
   mn = wave_min(mn);
   mx = wave_max(mx);
-  cnt = wave_sum(cnt);
-  const uint32_t wv = threadIdx.x / warpSize, ln = threadIdx.x % warpSize;
   if (ln == 0) {
     s_mn[wv] = mn;
     s_mx[wv] = mx;
-    s_cnt[wv] = cnt;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const uint32_t nw = (blockDim.x + warpSize - 1) / warpSize;
-    for (uint32_t w = 1; w < nw; w++) {
+    for (uint32_t w = 1; w < kCullWaves; w++) {
       mn = fminf(mn, s_mn[w]);
       mx = fmaxf(mx, s_mx[w]);
-      cnt += s_cnt[w];
     }
-    if (cnt) {
+    a.tile_count[b] = run;
+    if (run) {
       atomicMin(&a.stats[4 * p + 0], __float_as_uint(mn));   // keys are >= +0: uint order = float order
       atomicMax(&a.stats[4 * p + 1], __float_as_uint(mx));
-      atomicAdd(&a.stats[4 * p + 2], cnt);
+      atomicAdd(&a.stats[4 * p + 2], run);
     }
   }
 }
@@ -470,15 +485,16 @@ __global__ __launch_bounds__(64) void k_hist(FrameArgs a, uint32_t blk0) {
   for (uint32_t i = threadIdx.x; i < kBins; i += blockDim.x) h[i] = 0;
   __syncthreads();
   const uint32_t tl = b - a.tile_prefix[p];
-  const uint32_t t0 = tl * a.tile, t1 = min(t0 + a.tile, P.tcount);
+  const uint32_t n = a.tile_count[b];                // survivors, compacted at the tile's front
+  const float* __restrict__ keys = a.key + P.tbase + tl * a.tile;
   float mn, rd;
   part_range(a.stats, p, mn, rd);
   const uint32_t mode = (P.flags >> PF_SORT_SHIFT) & 3u;
-  for (uint32_t tb = t0; tb < t1; tb += 8 * 64) {   // eight independent loads in flight per lane
+  for (uint32_t ib = 0; ib < n; ib += 8 * 64) {     // eight independent loads in flight per lane
     float d[8];
     for (int j = 0; j < 8; j++) {
-      const uint32_t t = tb + j * 64 + threadIdx.x;
-      d[j] = t < t1 ? a.key[P.tbase + t] : -1.0f;
+      const uint32_t i = ib + j * 64 + threadIdx.x;
+      d[j] = i < n ? keys[i] : -1.0f;
     }
     for (int j = 0; j < 8; j++)
       if (d[j] >= 0) atomicAdd(&h[visit_bin(d[j], mn, rd, mode)], 1u);
@@ -580,18 +596,18 @@ __global__ __launch_bounds__(64) void k_scatter(FrameArgs a, uint32_t blk0) {
   const PartGPU& P = a.parts[p];
   if (a.stats[4 * p + 2] == 0) return;
   const uint32_t tl = b - a.tile_prefix[p];
-  const uint32_t t0 = tl * a.tile, t1 = min(t0 + a.tile, P.tcount);
+  const uint32_t n = a.tile_count[b];                // survivors, compacted at the tile's front
   const uint32_t lane = threadIdx.x;
   const uint32_t tbase = P.tbase;
   const uint32_t mode = (P.flags >> PF_SORT_SHIFT) & 3u;
-  const uint4* __restrict__ rec_in = a.tri_rec + tbase;
+  const uint4* __restrict__ rec_in = a.tri_rec + tbase + tl * a.tile;
   uint4* __restrict__ rec_out = a.tri_vo + tbase;
 
   __shared__ uint32_t run[kBins];
   __shared__ float s_key[kMaxTile];
   float mn, rd;
   part_range(a.stats, p, mn, rd);
-  for (uint32_t t = t0 + lane; t < t1; t += 64) s_key[t - t0] = a.key[tbase + t];   // one burst of loads
+  for (uint32_t i = lane; i < n; i += 64) s_key[i] = a.key[tbase + tl * a.tile + i];   // one burst of loads
 
   if (P.ntiles > 1) {
     const uint32_t* col = a.hist + P.hbase + tl;
@@ -600,10 +616,7 @@ __global__ __launch_bounds__(64) void k_scatter(FrameArgs a, uint32_t blk0) {
   } else {
     for (uint32_t i = lane; i < kBins; i += 64) run[i] = 0;
     __syncthreads();
-    for (uint32_t t = t0 + lane; t < t1; t += 64) {
-      const float d = s_key[t - t0];
-      if (d >= 0) atomicAdd(&run[visit_bin(d, mn, rd, mode)], 1u);
-    }
+    for (uint32_t i = lane; i < n; i += 64) atomicAdd(&run[visit_bin(s_key[i], mn, rd, mode)], 1u);
     __syncthreads();
     uint32_t c[8], s = 0;                       // exclusive scan of 512 counters, 8 per lane
     for (int k = 0; k < 8; k++) {
@@ -625,15 +638,13 @@ __global__ __launch_bounds__(64) void k_scatter(FrameArgs a, uint32_t blk0) {
   }
 
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (uint32_t base_t = t0; base_t < t1; base_t += 64) {
-    const uint32_t t = base_t + lane;
-    float d = -1.0f;
-    if (t < t1) d = s_key[t - t0];
-    const bool valid = d >= 0;
+  for (uint32_t base_i = 0; base_i < n; base_i += 64) {
+    const uint32_t i = base_i + lane;
+    const bool valid = i < n;
+    const float d = valid ? s_key[i] : -1.0f;
     const uint64_t vmask = __ballot(valid);
-    if (vmask == 0) continue;                   // uniform: no survivor in this chunk
     uint4 rec = make_uint4(0, 0, 0, 0);
-    if (valid) rec = rec_in[t];                 // issued before the ranking so its latency overlaps it
+    if (valid) rec = rec_in[i];                 // issued before the ranking so its latency overlaps it
     const uint32_t bin = valid ? visit_bin(d, mn, rd, mode) : 0xFFFFFFFFu;
     uint64_t todo = vmask, peers = 0;
     while (todo) {                              // peers = lanes with my bin
@@ -667,8 +678,8 @@ __global__ __launch_bounds__(256) void k_scatter4(FrameArgs a, uint32_t blk0) {
   const PartGPU& P = a.parts[p];
   if (a.stats[4 * p + 2] == 0) return;
   const uint32_t tl = b - a.tile_prefix[p];
-  const uint32_t t0 = tl * a.tile, t1 = min(t0 + a.tile, P.tcount);
-  const uint32_t n = t1 - t0;
+  const uint32_t t0 = tl * a.tile;
+  const uint32_t n = a.tile_count[b];                // survivors, compacted at the tile's front
   const uint32_t tid = threadIdx.x, lane = tid % 64u, wv = tid / 64u;
   const uint32_t tbase = P.tbase;
   const uint32_t mode = (P.flags >> PF_SORT_SHIFT) & 3u;
