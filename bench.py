@@ -144,7 +144,7 @@ def serve_bench(args):
     t = time.perf_counter()
     for m in models:
         ctx.upload_mesh(m.mesh)
-    resident = ctx.memory_used()
+    resident = ctx.memory_used
     log("pool resident in HBM: %.2f GB (uploaded in %.1fs)" % (resident / 1e9, time.perf_counter() - t))
     t = time.perf_counter()
     inputs = job_inputs(ctx, models, jobs, scene)

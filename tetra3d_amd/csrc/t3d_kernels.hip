@@ -469,6 +469,20 @@ __device__ __forceinline__ uint32_t visit_bin(float d, float mn, float rd, uint3
   return sort_mode == T3D_SORT_BACK_TO_FRONT ? (kBins - 1) - (uint32_t)bin : (uint32_t)bin;
 }
 
+// Lanes of `valid` (a ballot) whose 9-bit bin equals this lane's: one
+// ballot per bit, so the cost does not depend on how many distinct bins the
+// wavefront holds.  Meaningful for valid lanes only.
+__device__ __forceinline__ uint64_t bin_peers(uint32_t bin, uint64_t valid) {
+  static_assert(kBins == 512, "nine bin bits");
+  uint64_t peers = valid;
+  for (int bit = 0; bit < 9; bit++) {
+    const bool set = (bin >> bit) & 1u;
+    const uint64_t m = __ballot(set);
+    peers &= set ? m : ~m;
+  }
+  return peers;
+}
+
 __device__ __forceinline__ void part_range(const uint32_t* stats, uint32_t p, float& mn, float& rd) {
   mn = __uint_as_float(stats[4 * p + 0]);
   const float mx = __uint_as_float(stats[4 * p + 1]);
@@ -646,14 +660,7 @@ __global__ __launch_bounds__(64) void k_scatter(FrameArgs a, uint32_t blk0) {
     uint4 rec = make_uint4(0, 0, 0, 0);
     if (valid) rec = rec_in[i];                 // issued before the ranking so its latency overlaps it
     const uint32_t bin = valid ? visit_bin(d, mn, rd, mode) : 0xFFFFFFFFu;
-    uint64_t todo = vmask, peers = 0;
-    while (todo) {                              // peers = lanes with my bin
-      const int leader = __ffsll((unsigned long long)todo) - 1;
-      const uint32_t lb = __shfl(bin, leader);
-      const uint64_t m = __ballot(valid && bin == lb);
-      if (valid && bin == lb) peers = m;
-      todo &= ~m;
-    }
+    const uint64_t peers = bin_peers(bin, vmask);
     const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
     uint32_t slot = 0;
     if (valid) slot = run[bin] + rank;
@@ -748,14 +755,7 @@ __global__ __launch_bounds__(256) void k_scatter4(FrameArgs a, uint32_t blk0) {
     if (vmask == 0) continue;                     // uniform across the wave
     uint4 rec = make_uint4(0, 0, 0, 0);
     if (valid) rec = rec_in[t0 + i];
-    uint64_t todo = vmask, peers = 0;
-    while (todo) {
-      const int leader = __ffsll((unsigned long long)todo) - 1;
-      const uint32_t lb = __shfl(bin, leader);
-      const uint64_t m = __ballot(valid && bin == lb);
-      if (valid && bin == lb) peers = m;
-      todo &= ~m;
-    }
+    const uint64_t peers = bin_peers(bin, vmask);
     const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
     uint32_t slot = 0;
     if (valid) slot = off[bin] + rank;
