@@ -334,6 +334,8 @@ typedef struct {
                                         bit 2 = stage 32 rows at a time instead of 64 (0-4 or 6) */
   uint32_t assemble_split;           /* assembly blocks per tile: a power of two, tile/split >= 64 */
   uint32_t xcd_swizzle;              /* 1: each XCD takes a contiguous range of a launch's blocks */
+  uint32_t vertex_split;             /* 1: clip positions and colours in two arrays instead of one
+                                        interleaved 32-byte record per vertex */
 } t3d_tuning;
 t3d_status t3d_get_tuning(const t3d_ctx* ctx, t3d_tuning* out);
 t3d_status t3d_set_tuning(t3d_ctx* ctx, const t3d_tuning* tuning);

@@ -221,6 +221,7 @@ class Tuning(C.Structure):
         ("store_mode", C.c_uint32),
         ("assemble_split", C.c_uint32),
         ("xcd_swizzle", C.c_uint32),
+        ("vertex_split", C.c_uint32),
     ]
 
 

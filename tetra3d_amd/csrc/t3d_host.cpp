@@ -528,7 +528,7 @@ t3d_status t3d_mesh_free(t3d_ctx* c, uint64_t handle) {
   auto it = c->meshes.find(handle);
   if (it == c->meshes.end()) return fail(c, T3D_ERR_NOT_FOUND, "mesh_free: unknown handle %llu", (unsigned long long)handle);
   if (it->second->refs > 0) return fail(c, T3D_ERR_BUSY, "mesh_free: mesh %llu is referenced by in-flight work", (unsigned long long)handle);
-  (void)hipStreamSynchronize(c->stream);
+  (void)hipDeviceSynchronize();   // frames of any stream may still read it
   dev_free(c, it->second->buf);
   c->meshes.erase(it);
   return T3D_OK;
@@ -554,7 +554,7 @@ t3d_status t3d_volume_free(t3d_ctx* c, uint64_t handle) {
   std::lock_guard<std::mutex> lk(c->mu);
   auto it = c->volumes.find(handle);
   if (it == c->volumes.end()) return fail(c, T3D_ERR_NOT_FOUND, "volume_free: unknown handle");
-  (void)hipStreamSynchronize(c->stream);
+  (void)hipDeviceSynchronize();
   dev_free(c, it->second->buf);
   c->volumes.erase(it);
   return T3D_OK;
@@ -562,15 +562,25 @@ t3d_status t3d_volume_free(t3d_ctx* c, uint64_t handle) {
 
 t3d_status t3d_frame_create(t3d_ctx* c, t3d_frame** out) {
   if (!c || !out) return T3D_ERR_INVALID_ARGUMENT;
-  *out = new t3d_frame();
+  (void)hipSetDevice(c->device);
+  auto* f = new t3d_frame();
+  // each frame has a stream of its own, so frames in flight together overlap
+  // on the device (one's arithmetic-bound passes beside another's stores)
+  if (hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete f;
+    return fail(c, T3D_ERR_DEVICE, "frame_create: stream creation failed");
+  }
+  *out = f;
   return T3D_OK;
 }
 
 t3d_status t3d_frame_destroy(t3d_ctx* c, t3d_frame* f) {
   if (!c || !f) return T3D_ERR_INVALID_ARGUMENT;
   std::lock_guard<std::mutex> lk(c->mu);
+  if (f->stream) (void)hipStreamSynchronize(f->stream);
   (void)hipStreamSynchronize(c->stream);
   frame_release(c, f);
+  if (f->stream) (void)hipStreamDestroy(f->stream);
   delete f;
   return T3D_OK;
 }
@@ -1111,7 +1121,12 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   a.vchunk_prefix = d_vpre;
   a.tile_prefix = d_tpre;
   a.sblock_prefix = d_spre;
-  a.vrec = sc.take<float4>(2 * sumV);
+  {
+    float4* v = sc.take<float4>(2 * sumV);
+    a.vstride = tu.vertex_split ? 1u : 2u;
+    a.vclip = v;
+    a.vcol = tu.vertex_split ? v + sumV : v + 1;
+  }
   a.vpos = sc.take<float4>(sumV);
   a.key = sc.take<float>(sumT);
   a.tri_rec = sc.take<uint4>(sumT);
@@ -1245,7 +1260,7 @@ t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uin
 }
 
 uint64_t estimate_job_bytes(t3d_ctx* c, const t3d_render_desc* desc) {
-  // scratch: vrec + vpos (48 B/V); key + two records (36 B/T); per tile a 512-bin histogram column,
+  // scratch: clip + colour + skinned position (48 B/V); key + two records (36 B/T); per tile a 512-bin histogram column,
   // a scan block sum share and a survivor count (~2 KB); outputs: 3 x (48 + 48 + 4) = 300 B/T;
   // tables: ~1 KB per model plus palettes
   const uint64_t tile = std::max<uint32_t>(64, c->tuning.triangle_tile);
@@ -1304,7 +1319,7 @@ t3d_status render_submit(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc) 
   if (st) return st;
   st = frame_build(c, f, &desc, 1);
   if (st) return st;
-  st = frame_launch(c, f, c->stream, c->copy_stream, desc->exec_mode, true, true);
+  st = frame_launch(c, f, f->stream ? f->stream : c->stream, c->copy_stream, desc->exec_mode, true, true);
   if (st) return st;
   f->submitted = wall0;
   f->inflight = true;
@@ -1416,6 +1431,7 @@ t3d_status t3d_set_tuning(t3d_ctx* c, const t3d_tuning* t) {
       t->triangle_tile % t->assemble_split || t->triangle_tile / t->assemble_split < 64)
     return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: assemble_split must be a power of two with tile/split >= 64");
   if (t->xcd_swizzle > 1) return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: xcd_swizzle is 0 or 1");
+  if (t->vertex_split > 1) return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: vertex_split is 0 or 1");
   c->tuning = *t;
   return T3D_OK;
 }
@@ -1458,6 +1474,7 @@ t3d_status t3d_autotune(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc, u
       {&t3d_tuning::assemble_split, {1, 2, 4, 8}},
       {&t3d_tuning::store_mode, {0, 2, 4, 6}},   // LDS-staged rows (64 or 32 at a time), plain or streaming stores
       {&t3d_tuning::xcd_swizzle, {0, 1}},
+      {&t3d_tuning::vertex_split, {0, 1}},
   };
   t3d_tuning best = base;
   if (!valid(best)) best.assemble_split = 1;

@@ -7,8 +7,10 @@
 //     unit-stride coalesced load.
 //   frame tables (uploaded per render): PartGPU[P], LightGPU[L], PartLight[sum],
 //     block->part prefix arrays.
-//   frame scratch: vrec[sumV] = {clip f32x4, colour f32x4} (one 32-byte record
-//     per vertex), vpos[sumV] f32x4 (skinned positions), key[sumT] f32,
+//   frame scratch: per vertex a clip f32x4 and a colour f32x4, interleaved
+//     (one 32-byte record per vertex) or as two arrays (the cull pass then
+//     reads only clip positions), vpos[sumV] f32x4 (skinned positions),
+//     key[sumT] f32,
 //     tri_rec/tri_vo[sumT] u32x4 (survivor records by triangle / by slot),
 //     stats[P] u32x4, hist, bsum.
 //   outputs: color[3*sumT] t3d_vertex, depth[3*sumT] t3d_vertex, idx[3*sumT] u32;
@@ -122,7 +124,10 @@ struct FrameArgs {
   const uint32_t* vchunk_prefix;  // [P+1]
   const uint32_t* tile_prefix;    // [P+1]
   const uint32_t* sblock_prefix;  // [P+1]
-  float4* vrec;                 // [sumV][2]: clip position, vertex colour
+  float4* vclip;                // clip position of vertex record o at vclip[o * vstride]
+  float4* vcol;                 // its colour at vcol[o * vstride]
+  uint32_t vstride;             // 2: interleaved records; 1: two arrays
+  uint32_t pad1;
   float4* vpos;
   float* key;
   uint4* tri_rec;               // [sumT] survivors, triangle order: {vertex-record offsets x3, triangle}

@@ -294,8 +294,8 @@ __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
       }
     }
     const uint32_t o = P.vbase + i;
-    a.vrec[2 * (size_t)o] = make_float4(clip.x, clip.y, clip.z, clip.w);
-    a.vrec[2 * (size_t)o + 1] = make_float4(cr, cg, cb, ca);
+    a.vclip[(size_t)o * a.vstride] = make_float4(clip.x, clip.y, clip.z, clip.w);
+    a.vcol[(size_t)o * a.vstride] = make_float4(cr, cg, cb, ca);
     if (skinned) a.vpos[o] = make_float4(vp.x, vp.y, vp.z, 0.0f);
   }
 }
@@ -362,7 +362,7 @@ __global__ __launch_bounds__(256) void k_cull(FrameArgs a, uint32_t blk0) {
       for (int k = 0; k < 3; k++) {                                   // model.go:855-886
         const uint32_t o = P.vbase + (vi[k] - P.vmin);
         ov[k] = o;
-        const float4 c = a.vrec[2 * (size_t)o];
+        const float4 c = a.vclip[(size_t)o * a.vstride];
         if (skinned) {
           const float4 sp = a.vpos[o];
           stc.x += sp.x;
@@ -833,8 +833,8 @@ __global__ __launch_bounds__(64) void k_assemble(FrameArgs a, uint32_t blk0, uin
       const uint32_t ov[3] = {vo.x, vo.y, vo.z};
       for (int vi = 0; vi < 3; vi++) {
         const uint32_t o = ov[vi];
-        const float4 c = a.vrec[2 * (size_t)o];
-        const float4 col = a.vrec[2 * (size_t)o + 1];
+        const float4 c = a.vclip[(size_t)o * a.vstride];
+        const float4 col = a.vcol[(size_t)o * a.vstride];
         float w = c.w;                                            // camera.go:1655-1667
         if (!fc.perspective) w = 1.0f;
         if (w < 0) w = 0.001f;

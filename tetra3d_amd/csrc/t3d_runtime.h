@@ -82,6 +82,7 @@ struct t3d_frame {
   // asynchronous renders: tables uploaded / whole frame done, and whether a
   // launch is outstanding
   hipEvent_t tables_ready = nullptr, done = nullptr;
+  hipStream_t stream = nullptr;    // the frame's own chain stream (null: the caller's)
   bool inflight = false;
   std::chrono::steady_clock::time_point submitted;
 };
@@ -95,7 +96,7 @@ struct t3d_ctx {
   std::map<uint64_t, std::unique_ptr<t3d::Mesh>> meshes;
   std::map<uint64_t, std::unique_ptr<t3d::Volume>> volumes;
   std::string err;
-  t3d_tuning tuning{256, 1, 1024, 4, 1, 2, 4, 1};
+  t3d_tuning tuning{256, 2, 2048, 4, 1, 2, 8, 1, 0};
   // Guards resources, the budget and err between callers and the serving
   // thread; frame builds hold it, device work runs outside it.
   std::mutex mu;
