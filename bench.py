@@ -169,6 +169,23 @@ def serve_bench(args):
         wall = time.perf_counter() - t0
         st = srv.stats()
         del warm
+        # latency under heavier load: the same trace replayed faster
+        sweep = []
+        for rate in [float(r) for r in args.extra_rates.split(",") if r]:
+            srv.reset_stats()
+            arr = arrivals * (args.rate / rate)
+            t0 = time.perf_counter()
+            for k, inp in enumerate(inputs):
+                while time.perf_counter() - t0 < arr[k]:
+                    pass
+                srv.submit_inputs(inp)
+            srv.drain(timeout_ms=600000)
+            sw = srv.stats()
+            sweep.append({"rate_hz": rate, "offered_tris_per_s": float(np.mean(job_tris)) * rate,
+                          "value": sw.triangles / ((sw.last_done_ms - sw.first_submit_ms) / 1e3),
+                          "p50_ms": sw.p50_ms, "p99_ms": sw.p99_ms, "batches": sw.batches})
+            log("rate %.0f/s: %.3g tris/s, p50 %.3f p99 %.3f ms, %d batches"
+                % (rate, sweep[-1]["value"], sw.p50_ms, sw.p99_ms, sw.batches))
         # capacity: the same jobs submitted all at once (closed loop)
         srv.reset_stats()
         for inp in inputs:
@@ -216,9 +233,11 @@ def serve_bench(args):
         "rejected": st.rejected, "wall_s": wall,
         "saturated": {"value": capacity, "batches": sat.batches, "p50_ms": sat.p50_ms, "p99_ms": sat.p99_ms,
                       "note": "all jobs submitted at once: the server's capacity"},
+        "load_sweep": sweep,
         "plain": {"value": plain_value, "jobs_sampled": len(sample), "p50_ms": float(np.percentile(lat, 50)),
                   "p99_ms": float(np.percentile(lat, 99)), "speedup_served_vs_plain": served / plain_value,
-                  "speedup_capacity_vs_plain": capacity / plain_value},
+                  "speedup_capacity_vs_plain": capacity / plain_value,
+                  "speedup_at_rates_vs_plain": {str(int(x["rate_hz"])): x["value"] / plain_value for x in sweep}},
     }
     print(json.dumps(out), flush=True)
     ctx.close()
@@ -245,6 +264,8 @@ def main():
     ap.add_argument("--pool", type=int, default=10000)
     ap.add_argument("--jobs", type=int, default=2000)
     ap.add_argument("--rate", type=float, default=2000.0)
+    ap.add_argument("--extra-rates", default="4000,6000",
+                    help="further Poisson rates (jobs/s) replayed after the headline one, for latency under load")
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--max-batch-jobs", type=int, default=32)
     ap.add_argument("--serve-budget-gb", type=float, default=24.0)
