@@ -264,7 +264,7 @@ def main():
     ap.add_argument("--pool", type=int, default=10000)
     ap.add_argument("--jobs", type=int, default=2000)
     ap.add_argument("--rate", type=float, default=2000.0)
-    ap.add_argument("--extra-rates", default="4000,6000",
+    ap.add_argument("--extra-rates", default="3000,4000,5000",
                     help="further Poisson rates (jobs/s) replayed after the headline one, for latency under load")
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--max-batch-jobs", type=int, default=32)

@@ -300,11 +300,16 @@ void worker_main(t3d_server* s) {
         break;
       }
     if (free_slot >= 0 && !s->queue.empty() && !s->stop) {
+      // A batch holds at most budget/slots of estimated frame memory (a
+      // larger job still runs alone), so every slot's frame settles at a
+      // fixed size instead of being reallocated as batch sizes vary.
+      const uint64_t batch_cap = s->cfg.hbm_budget_bytes ? s->cfg.hbm_budget_bytes / s->slots.size() : UINT64_MAX;
       std::vector<uint64_t> take;
       uint64_t need = 0;
       while (!s->queue.empty() && take.size() < s->cfg.max_batch_jobs) {
         Job& j = *s->jobs[s->queue.front()];
         if (s->cfg.hbm_budget_bytes && s->inflight_bytes + j.est_bytes > s->cfg.hbm_budget_bytes) break;
+        if (!take.empty() && need + j.est_bytes > batch_cap) break;
         s->inflight_bytes += j.est_bytes;
         need += j.est_bytes;
         take.push_back(j.id);
@@ -326,12 +331,14 @@ void worker_main(t3d_server* s) {
         } else {
           b->frame = new t3d_frame();
         }
-        // Pooled frames keep their buffers.  When the frames held plus this
-        // batch's need would pass the budget, idle pooled frames are freed.
+        // Pooled frames keep their buffers (sized with 25 % headroom).  When
+        // the frames held plus this batch's need would pass that much over
+        // the budget, idle pooled frames are freed.
         std::vector<t3d_frame*> trim;
         if (s->cfg.hbm_budget_bytes) {
           uint64_t held = frames_held(s);
-          while (!s->free_frames.empty() && held + need + need / 4 > s->cfg.hbm_budget_bytes) {
+          const uint64_t limit = s->cfg.hbm_budget_bytes + s->cfg.hbm_budget_bytes / 4;
+          while (!s->free_frames.empty() && held + need + need / 4 > limit) {
             held -= frame_capacity(s->free_frames.back());
             trim.push_back(s->free_frames.back());
             s->free_frames.pop_back();
