@@ -154,9 +154,12 @@ def serve_bench(args):
     budget = int(args.serve_budget_gb * 1e9)
     with Server(ctx, slots=args.slots, max_batch_jobs=args.max_batch_jobs, hbm_budget=budget,
                 auto_release=True) as srv:
-        # warm-up: grow every slot's frame buffers to a large batch
-        warm = [srv.submit_inputs(inputs[k % len(inputs)]) for k in range(min(len(inputs), 4 * args.slots))]
-        srv.drain()
+        # warm-up: bursts of queued jobs form full-size batches on every slot,
+        # so each pooled frame reaches its final size before timing
+        warm = []
+        for _ in range(3):
+            warm += [srv.submit_inputs(inputs[k % len(inputs)]) for k in range(min(len(inputs), 16 * args.slots))]
+            srv.drain()
         srv.reset_stats()
         # replay the trace at its arrival times
         arrivals = np.array([j.arrival_s for j in jobs]) - jobs[0].arrival_s
@@ -255,6 +258,9 @@ def main():
                     help="launch shape to use instead of autotuning (fields of t3d_tuning), e.g. the config.tuning "
                          "of an earlier run, so profiler runs time the same shape")
     ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--sync-only", action="store_true",
+                    help="time synchronous frames only (no two frames in flight): for profiler runs, where "
+                         "overlapping frames would stretch each dispatch")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=10)
@@ -337,20 +343,25 @@ def main():
     # throughput (the timed region): K frames submitted asynchronously into
     # two alternating frames, so frame k's host prologue overlaps frame k-1's
     # kernels; every frame's outputs are complete in HBM before the clock stops
-    frames = [ctx.frame, ctx.new_frame()]
-    for k in range(max(2, args.warmup)):
-        ctx.render_async(inp, frames[k % 2])
-    for f in frames:
-        ctx.wait(f)
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        ctx.render_async(inp, frames[k % 2])   # waits for frame k-2 of the same buffers first
-    for f in frames:
-        ctx.wait(f)
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1e3 / args.steps
+    if args.sync_only:
+        # profiler runs: every kernel runs alone, so per-dispatch durations
+        # match the HIP-event timings below
+        ms = sync_ms
+    else:
+        frames = [ctx.frame, ctx.new_frame()]
+        for k in range(max(2, args.warmup)):
+            ctx.render_async(inp, frames[k % 2])
+        for f in frames:
+            ctx.wait(f)
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            ctx.render_async(inp, frames[k % 2])   # waits for frame k-2 of the same buffers first
+        for f in frames:
+            ctx.wait(f)
+        t1 = time.perf_counter()
+        ms = (t1 - t0) * 1e3 / args.steps
     value = tris / (ms / 1e3)
-    log("batched, pipelined: %.3f ms/frame, %.3g tris/s" % (ms, value))
+    log("batched, %s: %.3f ms/frame, %.3g tris/s" % ("synchronous" if args.sync_only else "pipelined", ms, value))
 
     # per-kernel durations with HIP events on the library's stream
     ctx.profile(True)
@@ -413,7 +424,7 @@ def main():
                    "exec": "batched, HIP graph, two frames in flight", "tuning": tuning},
         "latency_ms": {"p50": p50, "p99": p99, "mean": sync_ms,
                        "note": "synchronous frames: submit -> outputs complete in HBM"},
-        "pipelined": {"ms_per_step": ms, "frames_in_flight": 2},
+        "pipelined": None if args.sync_only else {"ms_per_step": ms, "frames_in_flight": 2},
         "hbm": {"algorithmic_bytes_per_frame": B, "frame_pct_of_peak": 100.0 * B / (ms / 1e3) / (HBM_PEAK_GBS * 1e9),
                 "device_ms_per_frame": device_ms, "host_prologue_ms": host_ms,
                 "host_pairs_uniforms_ms": host_uniforms_ms,

@@ -336,6 +336,7 @@ typedef struct {
   uint32_t xcd_swizzle;              /* 1: each XCD takes a contiguous range of a launch's blocks */
   uint32_t vertex_split;             /* 1: clip positions and colours in two arrays instead of one
                                         interleaved 32-byte record per vertex */
+  uint32_t cull_block;               /* threads per tile block in the cull pass: 256, 512 or 1024 */
 } t3d_tuning;
 t3d_status t3d_get_tuning(const t3d_ctx* ctx, t3d_tuning* out);
 t3d_status t3d_set_tuning(t3d_ctx* ctx, const t3d_tuning* tuning);

@@ -34,24 +34,24 @@ def test_scene_parity(gpu_ctx, oracle, scene, mode, store_mode):
     assert s["survivors"] > 0
 
 
-SHAPES = [  # (triangle_tile, scatter_waves, assemble_split, xcd_swizzle, vertex_split)
-    (512, 1, 1, 0, 0), (512, 4, 8, 1, 1), (1024, 4, 1, 1, 0), (1024, 4, 4, 0, 1), (2048, 1, 2, 1, 1),
-    (2048, 4, 32, 1, 0), (64, 4, 1, 1, 1),
+SHAPES = [  # (triangle_tile, scatter_waves, assemble_split, xcd_swizzle, vertex_split, cull_block)
+    (512, 1, 1, 0, 0, 256), (512, 4, 8, 1, 1, 1024), (1024, 4, 1, 1, 0, 512), (1024, 4, 4, 0, 1, 256),
+    (2048, 1, 2, 1, 1, 512), (2048, 4, 32, 1, 0, 1024), (64, 4, 1, 1, 1, 1024),
 ]
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("shape", SHAPES, ids=["t%d_w%d_s%d_x%d_v%d" % s for s in SHAPES])
+@pytest.mark.parametrize("shape", SHAPES, ids=["t%d_w%d_s%d_x%d_v%d_c%d" % s for s in SHAPES])
 @pytest.mark.parametrize("scene", ["heightfield_128", "mixed_48"])
 def test_launch_shapes_parity(gpu_ctx, oracle, scene, shape):
     """Every tile size, scatter width, assembly split, block order and vertex
     record layout gives the same frame."""
     wl = SCENES[scene]()
-    tile, waves, split, swz, vsplit = shape
+    tile, waves, split, swz, vsplit, cblock = shape
     print("shape parity", scene, shape, flush=True)
     saved = gpu_ctx.tuning()
     gpu_ctx.set_tuning(triangle_tile=tile, scatter_waves=waves, assemble_split=split, xcd_swizzle=swz,
-                       vertex_split=vsplit)
+                       vertex_split=vsplit, cull_block=cblock)
     try:
         g = gpu_ctx.render_workload(wl)
     finally:

@@ -222,6 +222,7 @@ class Tuning(C.Structure):
         ("assemble_split", C.c_uint32),
         ("xcd_swizzle", C.c_uint32),
         ("vertex_split", C.c_uint32),
+        ("cull_block", C.c_uint32),
     ]
 
 

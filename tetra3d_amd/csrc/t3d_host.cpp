@@ -646,7 +646,7 @@ t3d_status run_chain(t3d_ctx* c, t3d_frame* f, uint32_t p0, uint32_t p1, hipStre
   }
   {
     ProfScope ps(c, st, "k_cull", tr.cull, prof);
-    HIP_TRY(c, launch_cull(a, tb0, tb1 - tb0, st));
+    HIP_TRY(c, launch_cull(a, tb0, tb1 - tb0, c->tuning.cull_block, st));
   }
   if (sb1 > sb0) {
     {
@@ -1185,9 +1185,10 @@ t3d_status frame_launch(t3d_ctx* c, t3d_frame* f, hipStream_t stream, hipStream_
     }
   } else if (allow_graph && c->tuning.use_graph && !(profile && c->profiling)) {
     // graph of the whole chain; reused while the launch shape is unchanged
-    std::vector<uint8_t> key(sizeof(FrameArgs) + 7 * sizeof(uint32_t));
-    uint32_t shape[7] = {f->total_vchunks, f->total_tiles,    f->total_sblocks,          f->vertex_threads,
-                         c->tuning.store_mode, c->tuning.scatter_waves, c->tuning.assemble_split};
+    std::vector<uint8_t> key(sizeof(FrameArgs) + 8 * sizeof(uint32_t));
+    uint32_t shape[8] = {f->total_vchunks,       f->total_tiles,           f->total_sblocks,
+                         f->vertex_threads,      c->tuning.store_mode,     c->tuning.scatter_waves,
+                         c->tuning.assemble_split, c->tuning.cull_block};
     std::memcpy(key.data(), &a, sizeof(FrameArgs));
     std::memcpy(key.data() + sizeof(FrameArgs), shape, sizeof shape);
     if (!f->graph_exec || key != f->graph_key) {
@@ -1432,6 +1433,8 @@ t3d_status t3d_set_tuning(t3d_ctx* c, const t3d_tuning* t) {
     return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: assemble_split must be a power of two with tile/split >= 64");
   if (t->xcd_swizzle > 1) return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: xcd_swizzle is 0 or 1");
   if (t->vertex_split > 1) return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: vertex_split is 0 or 1");
+  if (t->cull_block != 256 && t->cull_block != 512 && t->cull_block != 1024)
+    return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: cull_block is 256, 512 or 1024");
   c->tuning = *t;
   return T3D_OK;
 }
@@ -1475,6 +1478,7 @@ t3d_status t3d_autotune(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc, u
       {&t3d_tuning::store_mode, {0, 2, 4, 6}},   // LDS-staged rows (64 or 32 at a time), plain or streaming stores
       {&t3d_tuning::xcd_swizzle, {0, 1}},
       {&t3d_tuning::vertex_split, {0, 1}},
+      {&t3d_tuning::cull_block, {256, 512, 1024}},
   };
   t3d_tuning best = base;
   if (!valid(best)) best.assemble_split = 1;

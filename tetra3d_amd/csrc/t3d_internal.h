@@ -146,7 +146,7 @@ struct FrameArgs {
 // form can launch one part's blocks at a time.
 hipError_t launch_init_stats(const FrameArgs& a, uint32_t nparts, hipStream_t s);
 hipError_t launch_vertex(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t threads, hipStream_t s);
-hipError_t launch_cull(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s);
+hipError_t launch_cull(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t threads, hipStream_t s);
 hipError_t launch_hist(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s);
 hipError_t launch_scan(const FrameArgs& a, uint32_t sblk0, uint32_t nsblk, uint32_t part0, uint32_t nparts,
                        hipStream_t s);

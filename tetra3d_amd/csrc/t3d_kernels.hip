@@ -327,8 +327,10 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   return v;
 }
 
-constexpr uint32_t kCullWaves = 4;   // k_cull runs 256-thread blocks
-__global__ __launch_bounds__(256) void k_cull(FrameArgs a, uint32_t blk0) {
+// kThreads: 256, 512 or 1024 threads per tile block (tunable).
+template <uint32_t kThreads>
+__global__ __launch_bounds__(kThreads) void k_cull(FrameArgs a, uint32_t blk0) {
+  constexpr uint32_t kCullWaves = kThreads / 64;
   const uint32_t b = blk0 + block_index(a.xcd_swizzle);
   const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
   const PartGPU& P = a.parts[p];
@@ -928,9 +930,14 @@ hipError_t launch_vertex(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint3
   hipLaunchKernelGGL(k_vertex, dim3(nblk), dim3(threads), 0, s, a, blk0);
   return hipGetLastError();
 }
-hipError_t launch_cull(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s) {
+hipError_t launch_cull(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t threads, hipStream_t s) {
   if (!nblk) return hipSuccess;
-  hipLaunchKernelGGL(k_cull, dim3(nblk), dim3(256), 0, s, a, blk0);
+  switch (threads) {
+    case 256: hipLaunchKernelGGL(k_cull<256>, dim3(nblk), dim3(256), 0, s, a, blk0); break;
+    case 512: hipLaunchKernelGGL(k_cull<512>, dim3(nblk), dim3(512), 0, s, a, blk0); break;
+    case 1024: hipLaunchKernelGGL(k_cull<1024>, dim3(nblk), dim3(1024), 0, s, a, blk0); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 hipError_t launch_hist(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s) {

@@ -96,7 +96,7 @@ struct t3d_ctx {
   std::map<uint64_t, std::unique_ptr<t3d::Mesh>> meshes;
   std::map<uint64_t, std::unique_ptr<t3d::Volume>> volumes;
   std::string err;
-  t3d_tuning tuning{256, 2, 2048, 4, 1, 2, 8, 1, 0};
+  t3d_tuning tuning{256, 1, 2048, 4, 1, 2, 8, 1, 0, 256};
   // Guards resources, the budget and err between callers and the serving
   // thread; frame builds hold it, device work runs outside it.
   std::mutex mu;
