@@ -106,6 +106,7 @@ class Oracle:
         self.ctx = self.L.t3do_create()
         self._handles = {}
         self._refs = []   # registered meshes stay alive so their ids are not reused
+        self._own_handle = 1 << 40   # above any handle a device context hands out
 
     def close(self):
         if self.ctx:
@@ -157,12 +158,16 @@ class Oracle:
     def render_workload(self, wl):
         """Register the workload's meshes/volumes (handles = the ones the GPU
         side assigned, stored on each object) and render it."""
-        for i, m in enumerate(wl.meshes):
+        for m in wl.meshes:
             if id(m) not in self._handles:
-                h = m._handles.get("gpu", 1000000 + i)
+                h = m._handles.get("gpu")
+                if h is None:               # not uploaded to a device: a handle of the oracle's own
+                    self._own_handle += 1
+                    h = self._own_handle
                 self.register_mesh(m, h)
         for v in wl.volumes:
             if v.handle == 0:
-                v.handle = 2000000 + wl.volumes.index(v)
+                self._own_handle += 1
+                v.handle = self._own_handle
             self.register_volume(v, v.handle)
         return self.render(wl.camera, wl.scene, wl.lights, wl.models)

@@ -1,0 +1,171 @@
+"""Seeded small scenes that turn the path's options on and off together, for
+GPU-vs-oracle parity beyond the measured workloads: camera projection and
+depth settings, snapping, perspective-corrected UVs, fog modes, every light
+type (LightVolume shading modes included), light groups, sort modes,
+backface culling, transparency modes and render order, lighting modes,
+shadeless and hidden parts, skinning, shape keys, non-uniform scale and
+frustum culling on and off."""
+import math
+
+import numpy as np
+
+from tetra3d_amd import _abi as A
+from tetra3d_amd import workloads as W
+from tetra3d_amd.scene import (AmbientLight, Camera, DirectionalLight, LightVolume, Material, Mesh, MeshPart, Model,
+                               PointLight, Scene, World, quat_to_matrix)
+
+f32 = np.float32
+
+
+def _rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    return quat_to_matrix(*q.astype(f32))
+
+
+def _two_part_mesh(rng, base, name):
+    """Split a mesh's triangles into two MeshParts with their own materials."""
+    T = base.triangle_count
+    cut = int(rng.integers(1, T)) if T > 1 else T
+    parts = [MeshPart(base, Material(name + "_a"), 0, cut)]
+    if cut < T:
+        parts.append(MeshPart(base, Material(name + "_b"), cut, T - cut))
+    base.mesh_parts = parts
+    return base
+
+
+def _mesh(rng, kind, name):
+    if kind == "sphere":
+        m = W.icosphere(int(rng.integers(1, 3)), float(rng.uniform(0.5, 2)), name=name,
+                        color=None if rng.random() < 0.3 else (1, 1, 1, 1))
+        if rng.random() < 0.5:   # per-vertex colours
+            m.colors = np.ascontiguousarray(np.c_[rng.random((m.vertex_count, 3)), np.ones(m.vertex_count)], dtype=f32)
+    elif kind == "grid":
+        n = int(rng.integers(3, 12))
+        m = W.grid_mesh(W.SplitMix64(int(rng.integers(1 << 30))), n, n, 0.5, 0.7, name=name)
+    elif kind == "cube":
+        m = W.cube_mesh(*rng.uniform(0.5, 2, 3))
+    else:                     # one triangle: every key equal, the range==0 quirk of the sort
+        m = Mesh(name, [[-1, -1, 0], [1, -1, 0], [0, 1, 0]], [[0, 0, 1]] * 3, [[0, 1, 2]],
+                 uvs=[[0, 0], [1, 0], [0.5, 1]])
+    if m.uvs is None and rng.random() < 0.5:
+        m.uvs = np.ascontiguousarray(rng.random((m.vertex_count, 2)), dtype=f32)
+    if rng.random() < 0.4 and m.triangle_count > 1:
+        m = _two_part_mesh(rng, m, name)
+    return m
+
+
+def _material(rng, mat):
+    mat.color = np.array([*rng.uniform(0.3, 1, 3), rng.choice([1.0, 1.0, 0.5, 0.0])], dtype=f32)
+    mat.backface_culling = bool(rng.random() < 0.7)
+    mat.triangle_sort_mode = int(rng.choice([A.SORT_BACK_TO_FRONT, A.SORT_FRONT_TO_BACK, A.SORT_NONE]))
+    mat.shadeless = bool(rng.random() < 0.15)
+    mat.fogless = bool(rng.random() < 0.2)
+    mat.lighting_mode = int(rng.choice([A.LIGHTING_DEFAULT, A.LIGHTING_FIXED_NORMALS, A.LIGHTING_DOUBLE_SIDED]))
+    mat.light_volume_shading_mode = int(rng.integers(0, 4))
+    mat.transparency_mode = int(rng.integers(0, 4))
+    mat.render_order = int(rng.integers(-1, 2))
+    mat.visible = bool(rng.random() < 0.95)
+    if rng.random() < 0.5:
+        mat.texture_size = (float(rng.choice([16, 64, 256])), float(rng.choice([16, 128])))
+
+
+def _skin(rng, model, bones=6):
+    mesh = model.mesh
+    V = mesh.vertex_count
+    idx = np.stack([rng.permutation(bones)[:4] for _ in range(V)]).astype(np.uint16)
+    w = rng.random((V, 4))
+    w[rng.random((V, 4)) < 0.3] = 0
+    w[:, 0] += 0.05
+    w = (w / w.sum(axis=1, keepdims=True)).astype(f32)
+    if rng.random() < 0.3:
+        w[: V // 3] = np.array([1, 0, 0, 0], dtype=f32)   # single full-weight influence
+    mesh.bone_indices, mesh.bone_weights = idx, w
+    place = model.transform()
+    pal = []
+    for _ in range(bones):
+        r = W.axis_angle_matrix(W.unit_vectors(W.SplitMix64(int(rng.integers(1 << 30))), 1)[0],
+                                float(rng.uniform(0, 0.6)))
+        pal.append(W.mat_mult(r, place))
+    model.skinned = True
+    model.bone_influences = np.stack(pal).astype(f32)
+
+
+def _lights(rng, lo, hi):
+    lights = []
+    for k in range(int(rng.integers(1, 6))):
+        t = rng.choice(["point", "point", "dir", "amb", "vol"])
+        if t == "point":
+            l = PointLight("p%d" % k, *rng.uniform(0.2, 1, 3), float(rng.uniform(0.5, 2)))
+            l.set_local_position(*rng.uniform(lo - 2, hi + 2))
+            l.range = f32(rng.choice([0.0, rng.uniform(2, 15)]))
+        elif t == "dir":
+            l = DirectionalLight("d%d" % k, *rng.uniform(0.2, 1, 3), float(rng.uniform(0.5, 1.5)))
+            l.set_local_rotation(_rotation(rng))
+        elif t == "amb":
+            l = AmbientLight("a%d" % k, *rng.uniform(0.2, 1, 3), float(rng.uniform(0.1, 0.5)))
+        else:
+            l = LightVolume("v%d" % k, lo - 1, hi + 1, (2.0, 2.0, 2.0))
+            l.cells[:, :3] = rng.random((len(l.cells), 3)).astype(f32)
+            l.light_bias = rng.uniform(-0.3, 0.3, 3).astype(f32)
+        l.visible = bool(rng.random() < 0.9)
+        lights.append(l)
+    return lights
+
+
+def feature_scene(seed):
+    rng = np.random.default_rng(seed)
+    lo, hi = np.array([-6, -4, -14], dtype=f32), np.array([6, 4, -3], dtype=f32)
+    cam = Camera("cam", int(rng.choice([320, 640, 1280])), int(rng.choice([240, 360, 720])))
+    cam.perspective = bool(rng.random() < 0.8)
+    cam.fov = f32(rng.uniform(40, 90))
+    cam.near = f32(rng.choice([0.05, 0.1, 1.0]))
+    cam.far = f32(rng.choice([30, 60, 300]))
+    cam.ortho_scale = f32(rng.uniform(8, 30))
+    cam.render_depth = bool(rng.random() < 0.7)
+    cam.perspective_corrected_texture_mapping = bool(rng.random() < 0.5)
+    cam.depth_margin = f32(rng.choice([0.0, 0.04, 0.2]))
+    cam.vertex_snapping = f32(rng.choice([0, 0, 0, 0.25, 8]))
+    if rng.random() < 0.3:
+        cam.set_local_rotation(_rotation(rng) if rng.random() < 0.2 else W.axis_angle_matrix((0, 1, 0), 0.3))
+    world = None
+    if rng.random() < 0.9:
+        world = World()
+        world.lighting_on = bool(rng.random() < 0.9)
+        world.fog_on = bool(rng.random() < 0.5)
+        world.fog_mode = int(rng.integers(0, 4))
+        world.fog_color = np.array([*rng.random(3), 1], dtype=f32)
+        world.fog_range = np.array(sorted(rng.uniform(0, 1, 2)), dtype=f32)
+        world.ambient_light.energy = f32(rng.uniform(0, 0.5))
+        world.ambient_light.visible = bool(rng.random() < 0.9)
+    lights = _lights(rng, lo, hi)
+    models = []
+    for i in range(int(rng.integers(4, 10))):
+        kind = rng.choice(["sphere", "grid", "cube", "tri"])
+        mesh = _mesh(rng, kind, "m%d" % i)
+        if kind != "tri" and rng.random() < 0.3:        # two shape keys
+            K = 2
+            mesh.shape_key_positions = np.ascontiguousarray(rng.normal(0, 0.1, (K, mesh.vertex_count, 3)), dtype=f32)
+            mesh.shape_key_normals = np.ascontiguousarray(rng.normal(0, 0.1, (K, mesh.vertex_count, 3)), dtype=f32)
+        for p in mesh.mesh_parts:
+            if p.material is None or rng.random() < 0.9:
+                p.material = p.material or Material("mat%d" % i)
+                _material(rng, p.material)
+            else:
+                p.material = None
+        m = Model("model%d" % i, mesh)
+        m.set_local_position(*rng.uniform(lo, hi))
+        m.set_local_rotation(_rotation(rng))
+        s = rng.uniform(0.5, 1.5, 3) if rng.random() < 0.3 else np.full(3, rng.uniform(0.5, 1.5))
+        m.set_local_scale(*s)
+        m.color = np.array([*rng.uniform(0.5, 1, 3), rng.choice([1.0, 0.7])], dtype=f32)
+        m.shadeless = bool(rng.random() < 0.1)
+        m.frustum_culling = bool(rng.random() < 0.8)
+        if mesh.shape_key_positions is not None:
+            m.shape_key_weights = rng.uniform(0, 1, len(mesh.shape_key_positions)).astype(f32)
+        if kind in ("sphere", "grid") and mesh.shape_key_positions is None and rng.random() < 0.35:
+            _skin(rng, m)
+        if rng.random() < 0.2 and lights:
+            m.light_group = [l for l in lights if not isinstance(l, LightVolume) and rng.random() < 0.6]
+        models.append(m)
+    return W.Workload("features_%d" % seed, cam, Scene("features", world), lights, models)
