@@ -171,7 +171,7 @@ __device__ __forceinline__ void light_vertex(const LightGPU& l, const PartLight&
 __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
   const uint32_t b = blk0 + block_index(a.xcd_swizzle);
   const uint32_t p = find_part(a.vchunk_prefix, a.nparts, b);
-  const PartGPU& P = a.parts[p];
+  const PartGPU P = a.parts[p];        // by value: read once, before any store
   const uint32_t flags = P.flags;
 
   __shared__ LightGPU s_light[kMaxLights];
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
   }
   __syncthreads();
 
-  const FrameConst& fc = a.fc[P.cam];
+  const FrameConst fc = a.fc[P.cam];
   const uint32_t first = (b - a.vchunk_prefix[p]) * a.vchunk;
   const uint32_t last = min(first + a.vchunk, P.vcount);
   for (uint32_t i = first + threadIdx.x; i < last; i += blockDim.x) {
@@ -333,9 +333,9 @@ __global__ __launch_bounds__(kThreads) void k_cull(FrameArgs a, uint32_t blk0) {
   constexpr uint32_t kCullWaves = kThreads / 64;
   const uint32_t b = blk0 + block_index(a.xcd_swizzle);
   const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
-  const PartGPU& P = a.parts[p];
+  const PartGPU P = a.parts[p];        // by value: read once, before any store
   const uint32_t flags = P.flags;
-  const FrameConst& fc = a.fc[P.cam];
+  const FrameConst fc = a.fc[P.cam];
   const uint32_t t0 = (b - a.tile_prefix[p]) * a.tile;
   const uint32_t t1 = min(t0 + a.tile, P.tcount);
   const bool skinned = (flags & PF_SKINNED) != 0;
@@ -361,15 +361,24 @@ __global__ __launch_bounds__(kThreads) void k_cull(FrameArgs a, uint32_t blk0) {
       float nx[3], ny[3];
       bool outOfBounds = true;
       V3 stc = v3(0, 0, 0);
+      // every gather of the triangle first, so their latencies overlap: clip
+      // positions, and the positions its depth key needs
+      // (the skinned position, or the mesh position for a static model: one
+      // branch-free load either way)
+      float4 cc[3];
+      V3 q[3];
+      for (int k = 0; k < 3; k++) {
+        ov[k] = P.vbase + (vi[k] - P.vmin);
+        cc[k] = a.vclip[(size_t)ov[k] * a.vstride];
+        const float* qs = skinned ? reinterpret_cast<const float*>(a.vpos + ov[k]) : P.pos + 3 * (size_t)vi[k];
+        q[k] = v3(qs[0], qs[1], qs[2]);
+      }
       for (int k = 0; k < 3; k++) {                                   // model.go:855-886
-        const uint32_t o = P.vbase + (vi[k] - P.vmin);
-        ov[k] = o;
-        const float4 c = a.vclip[(size_t)o * a.vstride];
+        const float4 c = cc[k];
         if (skinned) {
-          const float4 sp = a.vpos[o];
-          stc.x += sp.x;
-          stc.y += sp.y;
-          stc.z += sp.z;
+          stc.x += q[k].x;
+          stc.y += q[k].y;
+          stc.z += q[k].z;
         }
         float w = c.w;
         if (w < 0) w = 0.000001f;
@@ -399,7 +408,7 @@ __global__ __launch_bounds__(kThreads) void k_cull(FrameArgs a, uint32_t blk0) {
           depth = dx * dx + dy * dy + dz * dz;
         } else {
           // Triangle.Center from the base positions (mesh.go:1980-1982)
-          const V3 pa = ld3(P.pos, i0), pb = ld3(P.pos, i1), pc = ld3(P.pos, i2);
+          const V3 pa = q[0], pb = q[1], pc = q[2];
           const float cx = fdiv(pa.x + pb.x + pc.x, 3.0f);
           const float cy = fdiv(pa.y + pb.y + pc.y, 3.0f);
           const float cz = fdiv(pa.z + pb.z + pc.z, 3.0f);
@@ -495,7 +504,7 @@ __device__ __forceinline__ void part_range(const uint32_t* stats, uint32_t p, fl
 __global__ __launch_bounds__(64) void k_hist(FrameArgs a, uint32_t blk0) {
   const uint32_t b = blk0 + block_index(a.xcd_swizzle);
   const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
-  const PartGPU& P = a.parts[p];
+  const PartGPU P = a.parts[p];        // by value: read once, before any store
   if (P.ntiles <= 1) return;
   __shared__ uint32_t h[kBins];
   for (uint32_t i = threadIdx.x; i < kBins; i += blockDim.x) h[i] = 0;
@@ -544,7 +553,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_
 __global__ __launch_bounds__(256) void k_scan_reduce(FrameArgs a, uint32_t sblk0) {
   const uint32_t b = sblk0 + blockIdx.x;
   const uint32_t p = find_part(a.sblock_prefix, a.nparts, b);
-  const PartGPU& P = a.parts[p];
+  const PartGPU P = a.parts[p];        // by value: read once, before any store
   const uint32_t len = kBins * P.ntiles;
   const uint32_t e0 = (b - a.sblock_prefix[p]) * kScanBlock;
   uint32_t s = 0;
@@ -575,7 +584,7 @@ __global__ void k_scan_bsums(FrameArgs a, uint32_t part0, uint32_t nparts) {
 __global__ __launch_bounds__(256) void k_scan_down(FrameArgs a, uint32_t sblk0) {
   const uint32_t b = sblk0 + blockIdx.x;
   const uint32_t p = find_part(a.sblock_prefix, a.nparts, b);
-  const PartGPU& P = a.parts[p];
+  const PartGPU P = a.parts[p];        // by value: read once, before any store
   const uint32_t len = kBins * P.ntiles;
   const uint32_t e0 = (b - a.sblock_prefix[p]) * kScanBlock;
   __shared__ uint32_t s_tmp[8];
@@ -609,7 +618,7 @@ __global__ __launch_bounds__(256) void k_scan_down(FrameArgs a, uint32_t sblk0) 
 __global__ __launch_bounds__(64) void k_scatter(FrameArgs a, uint32_t blk0) {
   const uint32_t b = blk0 + block_index(a.xcd_swizzle);
   const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
-  const PartGPU& P = a.parts[p];
+  const PartGPU P = a.parts[p];        // by value: read once, before any store
   if (a.stats[4 * p + 2] == 0) return;
   const uint32_t tl = b - a.tile_prefix[p];
   const uint32_t n = a.tile_count[b];                // survivors, compacted at the tile's front
@@ -684,7 +693,7 @@ constexpr uint32_t kScatterWaves = 4;
 __global__ __launch_bounds__(256) void k_scatter4(FrameArgs a, uint32_t blk0) {
   const uint32_t b = blk0 + block_index(a.xcd_swizzle);
   const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
-  const PartGPU& P = a.parts[p];
+  const PartGPU P = a.parts[p];        // by value: read once, before any store
   if (a.stats[4 * p + 2] == 0) return;
   const uint32_t tl = b - a.tile_prefix[p];
   const uint32_t t0 = tl * a.tile;
@@ -799,23 +808,33 @@ __global__ __launch_bounds__(64) void k_assemble(FrameArgs a, uint32_t blk0, uin
   const uint32_t li = block_index(a.xcd_swizzle);
   const uint32_t b = blk0 + li / split;
   const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
-  const PartGPU& P = a.parts[p];
+  const PartGPU P = a.parts[p];        // by value: read once, before any store
   const uint32_t S = a.stats[4 * p + 2];
   const uint32_t sub = a.tile / split;
   const uint32_t k0 = (b - a.tile_prefix[p]) * a.tile + (li % split) * sub;
   if (k0 >= S) return;
   const uint32_t k1 = min(k0 + sub, S);
-  const FrameConst& fc = a.fc[P.cam];
+  const FrameConst fc = a.fc[P.cam];
   const uint32_t lane = threadIdx.x;
+  // Everything uniform is read once into registers here: the output stores
+  // below may alias these tables as far as the compiler knows, and would
+  // otherwise force a reload (a full memory round trip) per vertex.
   const uint32_t vbase = P.vbase, vmin = P.vmin, tbase = P.tbase;
   const bool has_uv = (P.flags & PF_HAS_UV) != 0;
   const float src_w = P.src_w, src_h = P.src_h;
   const float* __restrict__ uvp = P.uv;
+  const uint64_t out_base = P.out_base;
+  const bool persp = fc.perspective != 0, pc_tex = fc.pc_tex != 0, rdepth = fc.render_depth != 0;
+  const float cam_w = fc.cam_w, half_w = fc.half_w, cam_h = fc.cam_h, half_h = fc.half_h;
+  const float dmp = fc.dmp, spread = fc.spread;
+  const float4* __restrict__ vclip = a.vclip;
+  const float4* __restrict__ vcol = a.vcol;
+  const uint32_t vstride = a.vstride;
   const uint4* __restrict__ tvo = a.tri_vo + tbase;
-  float4* out_c = a.out_color + P.out_base * 3;   // 3 float4 per vertex
-  float4* out_d = a.out_depth + P.out_base * 3;
-  uint32_t* out_i = a.out_idx + P.out_base;
-  const float da = fc.render_depth ? 1.0f : 0.0f;
+  float4* __restrict__ out_c = a.out_color + out_base * 3;   // 3 float4 per vertex
+  float4* __restrict__ out_d = a.out_depth + out_base * 3;
+  uint32_t* __restrict__ out_i = a.out_idx + out_base;
+  const float da = rdepth ? 1.0f : 0.0f;
 
   // kStage 1: all 64 triangles staged at once; 2: 32 at a time, halving the
   // LDS per wavefront so twice as many wavefronts fit on a CU
@@ -833,23 +852,23 @@ __global__ __launch_bounds__(64) void k_assemble(FrameArgs a, uint32_t blk0, uin
     float dd[3];
     if (active) {
       const uint32_t ov[3] = {vo.x, vo.y, vo.z};
+      // all nine gathers of the triangle first, so their latencies overlap
+      float4 c[3], col[3];
+      float2 uv[3];
       for (int vi = 0; vi < 3; vi++) {
-        const uint32_t o = ov[vi];
-        const float4 c = a.vclip[(size_t)o * a.vstride];
-        const float4 col = a.vcol[(size_t)o * a.vstride];
-        float w = c.w;                                            // camera.go:1655-1667
-        if (!fc.perspective) w = 1.0f;
+        c[vi] = vclip[(size_t)ov[vi] * vstride];
+        col[vi] = vcol[(size_t)ov[vi] * vstride];
+        uv[vi] = has_uv ? reinterpret_cast<const float2*>(uvp)[ov[vi] - vbase + vmin] : make_float2(0.0f, 0.0f);
+      }
+      for (int vi = 0; vi < 3; vi++) {
+        float w = c[vi].w;                                        // camera.go:1655-1667
+        if (!persp) w = 1.0f;
         if (w < 0) w = 0.001f;
-        const float dx = fdiv(c.x, w) * fc.cam_w + fc.half_w;
-        const float dy = fdiv(c.y, -w) * fc.cam_h + fc.half_h;
-        float U = 0, V = 0;
-        if (has_uv) {
-          const float2 uv = reinterpret_cast<const float2*>(uvp)[o - vbase + vmin];
-          U = uv.x;
-          V = uv.y;
-        }
+        const float dx = fdiv(c[vi].x, w) * cam_w + half_w;
+        const float dy = fdiv(c[vi].y, -w) * cam_h + half_h;
+        const float U = uv[vi].x, V = uv[vi].y;
         float uvU, uvV, c0 = 0.0f;
-        if (fc.pc_tex) {                                          // camera.go:1688-1711
+        if (pc_tex) {                                             // camera.go:1688-1711
           uvU = fdiv(U, w) * src_w;
           uvV = fdiv(1.0f - V, w) * src_h;
           c0 = fdiv(1.0f, w);
@@ -858,13 +877,13 @@ __global__ __launch_bounds__(64) void k_assemble(FrameArgs a, uint32_t blk0, uin
           uvV = (1.0f - V) * src_h;
         }
         float depth = 0.0f;
-        if (fc.render_depth) {                                    // camera.go:1772-1784
-          depth = fdiv(c.z + fc.dmp, fc.spread);
+        if (rdepth) {                                             // camera.go:1772-1784
+          depth = fdiv(c[vi].z + dmp, spread);
           if (depth < 0) depth = 0;
           else if (depth > 1) depth = 1;
         }
         cq[3 * vi + 0] = make_float4(dx, dy, uvU, uvV);
-        cq[3 * vi + 1] = col;
+        cq[3 * vi + 1] = col[vi];
         cq[3 * vi + 2] = make_float4(c0, 0.0f, 0.0f, 0.0f);
         dd[vi] = depth;
       }
@@ -878,12 +897,22 @@ __global__ __launch_bounds__(64) void k_assemble(FrameArgs a, uint32_t blk0, uin
         if (mine)
           for (int q = 0; q < 9; q++) stage[row * 9 + q] = cq[q];
         __syncthreads();
-        for (uint32_t f = lane; f < rows * 9; f += 64) store4<kNT>(&out_c[(size_t)(wk0 + r0) * 9 + f], stage[f]);
+        // a fixed number of (predicated) stores per lane, so the compiler
+        // can count them and wait only for the loads it needs
+#pragma unroll
+        for (uint32_t q = 0; q < kRows * 9 / 64; q++) {
+          const uint32_t f = q * 64 + lane;
+          if (f < rows * 9) store4<kNT>(&out_c[(size_t)(wk0 + r0) * 9 + f], stage[f]);
+        }
         __syncthreads();
         if (mine)
           for (int vi = 0; vi < 3; vi++) stage[row * 9 + 3 * vi + 1] = make_float4(dd[vi], dd[vi], dd[vi], da);
         __syncthreads();
-        for (uint32_t f = lane; f < rows * 9; f += 64) store4<kNT>(&out_d[(size_t)(wk0 + r0) * 9 + f], stage[f]);
+#pragma unroll
+        for (uint32_t q = 0; q < kRows * 9 / 64; q++) {
+          const uint32_t f = q * 64 + lane;
+          if (f < rows * 9) store4<kNT>(&out_d[(size_t)(wk0 + r0) * 9 + f], stage[f]);
+        }
         __syncthreads();
       }
     } else if (active) {
@@ -897,7 +926,11 @@ __global__ __launch_bounds__(64) void k_assemble(FrameArgs a, uint32_t blk0, uin
       }
     }
     // indices: pair-local iota, as the reference's indexList (camera.go:1816)
-    for (uint32_t f = lane; f < wcount * 3; f += 64) store1<kNT>(&out_i[(size_t)wk0 * 3 + f], wk0 * 3 + f);
+#pragma unroll
+    for (uint32_t q = 0; q < 3; q++) {
+      const uint32_t f = q * 64 + lane;
+      if (f < wcount * 3) store1<kNT>(&out_i[(size_t)wk0 * 3 + f], wk0 * 3 + f);
+    }
   }
 }
 
