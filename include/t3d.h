@@ -331,7 +331,8 @@ typedef struct {
   uint32_t use_graph;                /* replay the batched chain from a HIP graph */
   uint32_t store_mode;               /* assembly stores: bit 0 = direct per-lane stores instead of
                                         LDS-staged rows, bit 1 = non-temporal (streaming) stores,
-                                        bit 2 = stage 32 rows at a time instead of 64 (0-4 or 6) */
+                                        bit 2 = stage 32 rows at a time instead of 64, bit 3 =
+                                        software-pipelined gathers (0-4, 6, 8, 10, 12, 14) */
   uint32_t assemble_split;           /* assembly blocks per tile: a power of two, tile/split >= 64 */
   uint32_t xcd_swizzle;              /* 1: each XCD takes a contiguous range of a launch's blocks */
   uint32_t vertex_split;             /* 1: clip positions and colours in two arrays instead of one
@@ -411,6 +412,12 @@ t3d_status t3d_server_stats_get(t3d_server* s, t3d_server_stats* out);
 t3d_status t3d_server_stats_reset(t3d_server* s);
 
 /* ---- self checks ------------------------------------------------------ */
+
+/* word != 0: before every render of this context, fill the frame's output
+ * arrays with `word`, so a vertex or index the path fails to write shows up
+ * instead of keeping an earlier render's value (tests pass a NaN pattern).
+ * 0 turns it off.  No counterpart in the reference: a debugging aid. */
+t3d_status t3d_debug_poison_outputs(t3d_ctx* ctx, uint32_t word);
 
 /* Runs the device's float32 primitives on n inputs: out[0..n) = x/y,
  * out[n..2n) = sqrt(|x|), out[2n..3n) = x*y + x (unfused), out[3n..4n) =

@@ -1051,7 +1051,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   const uint32_t ntiles_total = f->tile_prefix.back();
   const uint64_t sbytes = carve_size<float4>(2 * sumV) + carve_size<float4>(sumV) + carve_size<uint32_t>(sumT) +
                           2 * carve_size<uint4>(sumT) + carve_size<uint32_t>(4ull * P_n) + carve_size<uint32_t>(sumHist) +
-                          carve_size<uint32_t>(sblocks) + carve_size<uint32_t>(ntiles_total);
+                          carve_size<uint32_t>(sblocks) + carve_size<uint32_t>(ntiles_total) + carve_size<float4>(128);
   st = ensure(c, f->scratch, f->scratch_cap, sbytes);
   if (st) return st;
   const uint64_t obytes = 2 * carve_size<t3d_vertex>(3 * sumT) + carve_size<uint32_t>(3 * sumT);
@@ -1135,6 +1135,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   a.hist = sc.take<uint32_t>(sumHist);
   a.bsum = sc.take<uint32_t>(sblocks);
   a.tile_count = sc.take<uint32_t>(ntiles_total);
+  a.sink = sc.take<float4>(128);
   a.out_color = reinterpret_cast<float4*>(oc.take<t3d_vertex>(3 * sumT));
   a.out_depth = reinterpret_cast<float4*>(oc.take<t3d_vertex>(3 * sumT));
   a.out_idx = oc.take<uint32_t>(3 * sumT);
@@ -1169,6 +1170,7 @@ t3d_status frame_launch(t3d_ctx* c, t3d_frame* f, hipStream_t stream, hipStream_
     HIP_TRY(c, hipEventRecord(f->tables_ready, copy_stream));
     HIP_TRY(c, hipStreamWaitEvent(stream, f->tables_ready, 0));
   }
+  if (c->poison) HIP_TRY(c, launch_fill(f->outputs.ptr, f->outputs.bytes, c->poison, stream));
   if (!P_n) {
     HIP_TRY(c, hipEventRecord(f->done, stream));
     return T3D_OK;
@@ -1386,6 +1388,13 @@ t3d_status t3d_frame_download_triangles(t3d_ctx* c, const t3d_frame* f, uint32_t
   return frame_download_range(c, f, 0, (uint32_t)f->pairs.size(), c->stream, nullptr, nullptr, nullptr, ids);
 }
 
+t3d_status t3d_debug_poison_outputs(t3d_ctx* c, uint32_t word) {
+  if (!c) return T3D_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->poison = word;
+  return T3D_OK;
+}
+
 t3d_status t3d_profile_enable(t3d_ctx* c, int32_t on) {
   if (!c) return T3D_ERR_INVALID_ARGUMENT;
   c->profiling = on != 0;
@@ -1424,8 +1433,9 @@ t3d_status t3d_set_tuning(t3d_ctx* c, const t3d_tuning* t) {
     return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: vertices_per_thread in [1, 64]");
   if (t->triangle_tile < 64 || t->triangle_tile > kMaxTile || t->triangle_tile % 64)
     return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: triangle_tile must be a multiple of 64 in [64, %u]", kMaxTile);
-  if (t->store_mode > 6 || t->store_mode == 5)
-    return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: store_mode is 0, 1, 2, 3, 4 or 6");
+  if (!(t->store_mode <= 4 || t->store_mode == 6 || t->store_mode == 8 || t->store_mode == 10 || t->store_mode == 12 ||
+        t->store_mode == 14))
+    return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: store_mode is 0-4, 6, 8, 10, 12 or 14");
   if (t->scatter_waves != 1 && t->scatter_waves != 4)
     return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: scatter_waves must be 1 or 4");
   if (t->assemble_split == 0 || (t->assemble_split & (t->assemble_split - 1)) ||
@@ -1475,7 +1485,7 @@ t3d_status t3d_autotune(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc, u
       {&t3d_tuning::vertices_per_thread, {1, 2, 4}},
       {&t3d_tuning::scatter_waves, {1, 4}},
       {&t3d_tuning::assemble_split, {1, 2, 4, 8}},
-      {&t3d_tuning::store_mode, {0, 2, 4, 6}},   // LDS-staged rows (64 or 32 at a time), plain or streaming stores
+      {&t3d_tuning::store_mode, {0, 2, 4, 6, 8, 10, 12, 14}},   // staged rows (64 or 32), streaming or not, pipelined or not
       {&t3d_tuning::xcd_swizzle, {0, 1}},
       {&t3d_tuning::vertex_split, {0, 1}},
       {&t3d_tuning::cull_block, {256, 512, 1024}},

@@ -134,6 +134,7 @@ struct FrameArgs {
   uint4* tri_vo;                // [sumT] the same records in output-slot order
   uint32_t* stats;              // [P][4]: min bits, max bits, survivors, 0
   uint32_t* tile_count;         // [tiles] survivors of each tile (compacted at the tile's front)
+  float4* sink;                 // [128] store target of lanes past the end of a chunk (never read)
   uint32_t* hist;
   uint32_t* bsum;
   float4* out_color;            // t3d_vertex as 3 float4
@@ -154,6 +155,7 @@ hipError_t launch_scatter(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint
 hipError_t launch_assemble(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t mode, uint32_t split,
                            hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
+hipError_t launch_fill(void* dst, uint64_t bytes, uint32_t word, hipStream_t s);
 hipError_t launch_fp_selftest(const float* a, const float* b, float* out, uint32_t n, hipStream_t s);
 
 }  // namespace t3d

@@ -900,7 +900,7 @@ __global__ __launch_bounds__(64) void k_assemble(FrameArgs a, uint32_t blk0, uin
         // a fixed number of (predicated) stores per lane, so the compiler
         // can count them and wait only for the loads it needs
 #pragma unroll
-        for (uint32_t q = 0; q < kRows * 9 / 64; q++) {
+        for (uint32_t q = 0; q < (kRows * 9 + 63) / 64; q++) {
           const uint32_t f = q * 64 + lane;
           if (f < rows * 9) store4<kNT>(&out_c[(size_t)(wk0 + r0) * 9 + f], stage[f]);
         }
@@ -909,7 +909,7 @@ __global__ __launch_bounds__(64) void k_assemble(FrameArgs a, uint32_t blk0, uin
           for (int vi = 0; vi < 3; vi++) stage[row * 9 + 3 * vi + 1] = make_float4(dd[vi], dd[vi], dd[vi], da);
         __syncthreads();
 #pragma unroll
-        for (uint32_t q = 0; q < kRows * 9 / 64; q++) {
+        for (uint32_t q = 0; q < (kRows * 9 + 63) / 64; q++) {
           const uint32_t f = q * 64 + lane;
           if (f < rows * 9) store4<kNT>(&out_d[(size_t)(wk0 + r0) * 9 + f], stage[f]);
         }
@@ -932,6 +932,152 @@ __global__ __launch_bounds__(64) void k_assemble(FrameArgs a, uint32_t blk0, uin
       if (f < wcount * 3) store1<kNT>(&out_i[(size_t)wk0 * 3 + f], wk0 * 3 + f);
     }
   }
+}
+
+// The same assembly, software-pipelined: the nine gathers of chunk i+1 are
+// issued before chunk i's stores, so their latency hides behind the stores.
+// Every memory instruction runs on all 64 lanes (lanes past the end read a
+// clamped slot and write to a sink), which keeps the number of stores per
+// chunk fixed: the wait for the next chunk's gathers can then let the
+// current chunk's stores stay in flight instead of draining them.
+template <uint32_t kRows, bool kNT>
+__global__ __launch_bounds__(64) void k_assemble_pipe(FrameArgs a, uint32_t blk0, uint32_t split) {
+  const uint32_t li = block_index(a.xcd_swizzle);
+  const uint32_t b = blk0 + li / split;
+  const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
+  const PartGPU P = a.parts[p];
+  const uint32_t S = a.stats[4 * p + 2];
+  const uint32_t sub = a.tile / split;
+  const uint32_t k0 = (b - a.tile_prefix[p]) * a.tile + (li % split) * sub;
+  if (k0 >= S) return;
+  const uint32_t k1 = min(k0 + sub, S), last = k1 - 1;
+  const FrameConst fc = a.fc[P.cam];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t vbase = P.vbase, vmin = P.vmin, tbase = P.tbase;
+  const bool has_uv = (P.flags & PF_HAS_UV) != 0;
+  const float src_w = P.src_w, src_h = P.src_h;
+  const float2* __restrict__ uvp = reinterpret_cast<const float2*>(P.uv);
+  const uint64_t out_base = P.out_base;
+  const bool persp = fc.perspective != 0, pc_tex = fc.pc_tex != 0, rdepth = fc.render_depth != 0;
+  const float cam_w = fc.cam_w, half_w = fc.half_w, cam_h = fc.cam_h, half_h = fc.half_h;
+  const float dmp = fc.dmp, spread = fc.spread;
+  const float4* __restrict__ vclip = a.vclip;
+  const float4* __restrict__ vcol = a.vcol;
+  const uint32_t vstride = a.vstride;
+  const uint4* __restrict__ tvo = a.tri_vo + tbase;
+  float4* __restrict__ out_c = a.out_color + out_base * 3;
+  float4* __restrict__ out_d = a.out_depth + out_base * 3;
+  uint32_t* __restrict__ out_i = a.out_idx + out_base;
+  float4* __restrict__ sink = a.sink + lane;                 // per-lane sink, never read
+  uint32_t* __restrict__ sink_i = reinterpret_cast<uint32_t*>(a.sink + 64) + lane;
+  const float da = rdepth ? 1.0f : 0.0f;
+  __shared__ float4 stage[kRows * 9];
+
+  struct Gathered {
+    float4 c[3], col[3];
+    float2 uv[3];
+  };
+  auto gather = [&](Gathered& g, const uint4 vo) {
+    const uint32_t ov[3] = {vo.x, vo.y, vo.z};
+    for (int vi = 0; vi < 3; vi++) {
+      g.c[vi] = vclip[(size_t)ov[vi] * vstride];
+      g.col[vi] = vcol[(size_t)ov[vi] * vstride];
+      g.uv[vi] = has_uv ? uvp[ov[vi] - vbase + vmin] : make_float2(0.0f, 0.0f);
+    }
+  };
+  auto compute = [&](const Gathered& g, float4 (&cq)[9], float (&dd)[3]) {
+    for (int vi = 0; vi < 3; vi++) {
+      float w = g.c[vi].w;                                        // camera.go:1655-1667
+      if (!persp) w = 1.0f;
+      if (w < 0) w = 0.001f;
+      const float dx = fdiv(g.c[vi].x, w) * cam_w + half_w;
+      const float dy = fdiv(g.c[vi].y, -w) * cam_h + half_h;
+      const float U = g.uv[vi].x, V = g.uv[vi].y;
+      float uvU, uvV, c0 = 0.0f;
+      if (pc_tex) {                                               // camera.go:1688-1711
+        uvU = fdiv(U, w) * src_w;
+        uvV = fdiv(1.0f - V, w) * src_h;
+        c0 = fdiv(1.0f, w);
+      } else {
+        uvU = U * src_w;
+        uvV = (1.0f - V) * src_h;
+      }
+      float depth = 0.0f;
+      if (rdepth) {                                               // camera.go:1772-1784
+        depth = fdiv(g.c[vi].z + dmp, spread);
+        if (depth < 0) depth = 0;
+        else if (depth > 1) depth = 1;
+      }
+      cq[3 * vi + 0] = make_float4(dx, dy, uvU, uvV);
+      cq[3 * vi + 1] = g.col[vi];
+      cq[3 * vi + 2] = make_float4(c0, 0.0f, 0.0f, 0.0f);
+      dd[vi] = depth;
+    }
+  };
+  auto store_chunk = [&](const float4 (&cq)[9], const float (&dd)[3], uint32_t wk0) {
+    const uint32_t wcount = min(64u, k1 - wk0);
+#pragma unroll
+    for (uint32_t h = 0; h < 64 / kRows; h++) {
+      const uint32_t r0 = h * kRows;
+      const uint32_t rows = wcount > r0 ? min(kRows, wcount - r0) : 0u;
+      const bool mine = lane >= r0 && lane < r0 + kRows;
+      if (mine)
+        for (int q = 0; q < 9; q++) stage[(lane - r0) * 9 + q] = cq[q];
+      __syncthreads();
+#pragma unroll
+      for (uint32_t q = 0; q < (kRows * 9 + 63) / 64; q++) {
+        const uint32_t f = q * 64 + lane;
+        store4<kNT>(f < rows * 9 ? &out_c[(size_t)(wk0 + r0) * 9 + f] : sink, stage[min(f, kRows * 9 - 1)]);
+      }
+      __syncthreads();
+      if (mine)
+        for (int vi = 0; vi < 3; vi++) stage[(lane - r0) * 9 + 3 * vi + 1] = make_float4(dd[vi], dd[vi], dd[vi], da);
+      __syncthreads();
+#pragma unroll
+      for (uint32_t q = 0; q < (kRows * 9 + 63) / 64; q++) {
+        const uint32_t f = q * 64 + lane;
+        store4<kNT>(f < rows * 9 ? &out_d[(size_t)(wk0 + r0) * 9 + f] : sink, stage[min(f, kRows * 9 - 1)]);
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (uint32_t q = 0; q < 3; q++) {                            // indices (camera.go:1816)
+      const uint32_t f = q * 64 + lane;
+      store1<kNT>(f < wcount * 3 ? &out_i[(size_t)wk0 * 3 + f] : sink_i, wk0 * 3 + f);
+    }
+  };
+
+  // two register sets, used in turn: no copies between chunks
+  Gathered ga, gb;
+  gather(ga, tvo[min(k0 + lane, last)]);
+  uint4 vo_next = tvo[min(k0 + 64 + lane, last)];
+  float4 cq[9];
+  float dd[3];
+  for (uint32_t wk0 = k0;;) {
+    compute(ga, cq, dd);
+    bool more = wk0 + 64 < k1;                                    // uniform
+    if (more) {
+      gather(gb, vo_next);
+      vo_next = tvo[min(wk0 + 128 + lane, last)];
+    }
+    store_chunk(cq, dd, wk0);
+    wk0 += 64;
+    if (!more) break;
+    compute(gb, cq, dd);
+    more = wk0 + 64 < k1;
+    if (more) {
+      gather(ga, vo_next);
+      vo_next = tvo[min(wk0 + 128 + lane, last)];
+    }
+    store_chunk(cq, dd, wk0);
+    wk0 += 64;
+    if (!more) break;
+  }
+}
+
+// Fill of a word array (test poisoning of outputs: unwritten entries show up).
+__global__ __launch_bounds__(256) void k_fill(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
 // Plain float4 copy of a known size: the reference point for HBM byte counters.
@@ -998,6 +1144,10 @@ hipError_t launch_assemble(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uin
   if (split == 0 || (split & (split - 1)) || a.tile % split) return hipErrorInvalidValue;
   const dim3 g(nblk * split);
   switch (mode) {
+    case 8: hipLaunchKernelGGL((k_assemble_pipe<64, false>), g, dim3(64), 0, s, a, blk0, split); break;
+    case 10: hipLaunchKernelGGL((k_assemble_pipe<64, true>), g, dim3(64), 0, s, a, blk0, split); break;
+    case 12: hipLaunchKernelGGL((k_assemble_pipe<32, false>), g, dim3(64), 0, s, a, blk0, split); break;
+    case 14: hipLaunchKernelGGL((k_assemble_pipe<32, true>), g, dim3(64), 0, s, a, blk0, split); break;
     case 0: hipLaunchKernelGGL((k_assemble<1, false>), g, dim3(64), 0, s, a, blk0, split); break;
     case 1: hipLaunchKernelGGL((k_assemble<0, false>), g, dim3(64), 0, s, a, blk0, split); break;
     case 2: hipLaunchKernelGGL((k_assemble<1, true>), g, dim3(64), 0, s, a, blk0, split); break;
@@ -1006,6 +1156,11 @@ hipError_t launch_assemble(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uin
     case 6: hipLaunchKernelGGL((k_assemble<2, true>), g, dim3(64), 0, s, a, blk0, split); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+hipError_t launch_fill(void* dst, uint64_t bytes, uint32_t word, hipStream_t s) {
+  if (bytes < 4) return hipSuccess;
+  hipLaunchKernelGGL(k_fill, dim3(2048), dim3(256), 0, s, reinterpret_cast<uint32_t*>(dst), bytes / 4, word);
   return hipGetLastError();
 }
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s) {

@@ -100,6 +100,8 @@ struct t3d_ctx {
   // Guards resources, the budget and err between callers and the serving
   // thread; frame builds hold it, device work runs outside it.
   std::mutex mu;
+  // nonzero: fill every frame's outputs with this word before its render
+  uint32_t poison = 0;
   // profiling (synchronous render path)
   bool profiling = false;
   struct Prof {

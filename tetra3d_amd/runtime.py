@@ -60,6 +60,7 @@ def lib():
     L.t3d_frame_destroy.argtypes = [vp, vp]
     L.t3d_render.argtypes = [vp, vp, P(A.RenderDesc)]
     L.t3d_render_async.argtypes = [vp, vp, P(A.RenderDesc)]
+    L.t3d_debug_poison_outputs.argtypes = [vp, C.c_uint32]
     L.t3d_frame_wait.argtypes = [vp, vp]
     L.t3d_frame_pairs.argtypes = [vp, P(C.c_uint32), P(P(A.PairResult))]
     L.t3d_frame_device_outputs.argtypes = [vp, P(vp), P(vp), P(vp)]
@@ -221,6 +222,12 @@ class Context:
     def render_inputs(self, inp):
         """Run one Camera.Render on the device; returns nothing (outputs stay in HBM)."""
         self._check(self.L.t3d_render(self.ctx, self.frame, C.byref(inp.render)))
+
+    def poison_outputs(self, word=0x7FC0DEAD):
+        """Fill frame outputs with `word` before every render (0: off); tests
+        use a NaN pattern so unwritten entries cannot pass as an earlier
+        render's values."""
+        self._check(self.L.t3d_debug_poison_outputs(self.ctx, word))
 
     def new_frame(self):
         """An extra frame (outputs + scratch) of this context, for asynchronous renders."""
