@@ -640,6 +640,7 @@ t3d_status run_chain(t3d_ctx* c, t3d_frame* f, uint32_t p0, uint32_t p1, hipStre
   const uint32_t vb0 = f->vchunk_prefix[p0], vb1 = f->vchunk_prefix[p1];
   const uint32_t tb0 = f->tile_prefix[p0], tb1 = f->tile_prefix[p1];
   const uint32_t sb0 = f->sblock_prefix[p0], sb1 = f->sblock_prefix[p1];
+  const uint32_t ps0 = f->pscan_prefix[p0], ps1 = f->pscan_prefix[p1];
   {
     ProfScope ps(c, st, "k_vertex", tr.vertex, prof);
     HIP_TRY(c, launch_vertex(a, vb0, vb1 - vb0, f->vertex_threads, st));
@@ -648,12 +649,13 @@ t3d_status run_chain(t3d_ctx* c, t3d_frame* f, uint32_t p0, uint32_t p1, hipStre
     ProfScope ps(c, st, "k_cull", tr.cull, prof);
     HIP_TRY(c, launch_cull(a, tb0, tb1 - tb0, c->tuning.cull_block, st));
   }
-  if (sb1 > sb0) {
+  if (sb1 > sb0 || ps1 > ps0) {
     {
       ProfScope ps(c, st, "k_hist", 0, prof);
       HIP_TRY(c, launch_hist(a, tb0, tb1 - tb0, st));
     }
     ProfScope ps(c, st, "k_scan", 0, prof);
+    HIP_TRY(c, launch_scan_part(a, ps0, ps1 - ps0, st));
     HIP_TRY(c, launch_scan(a, sb0, sb1 - sb0, p0, p1 - p0, st));
   }
   {
@@ -703,6 +705,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   f->vchunk_prefix.assign(1, 0);
   f->tile_prefix.assign(1, 0);
   f->sblock_prefix.assign(1, 0);
+  f->pscan_prefix.assign(1, 0);
   f->bytes_vertex.clear();
   f->bytes_cull.clear();
   f->bytes_uv.clear();
@@ -990,10 +993,16 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
       const uint32_t nvchunks = (P.vcount + vchunk - 1) / vchunk;
       const uint32_t ntiles = (P.tcount + tile - 1) / tile;
       P.ntiles = ntiles;
+      uint32_t pscan = 0;
       if (ntiles > 1) {
         P.hbase = (uint32_t)sumHist;
         P.sblock0 = sblocks;
-        P.nsblocks = (kBins * ntiles + kScanBlock - 1) / kScanBlock;
+        if (kBins * ntiles <= kScanPartMax) {
+          P.nsblocks = 0;          // scanned whole by one block of k_scan_part
+          pscan = 1;
+        } else {
+          P.nsblocks = (kBins * ntiles + kScanBlock - 1) / kScanBlock;
+        }
         sumHist += (uint64_t)kBins * ntiles;
         sblocks += P.nsblocks;
       }
@@ -1004,6 +1013,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
       f->vchunk_prefix.push_back(f->vchunk_prefix.back() + nvchunks);
       f->tile_prefix.push_back(f->tile_prefix.back() + ntiles);
       f->sblock_prefix.push_back(sblocks);
+      f->pscan_prefix.push_back(f->pscan_prefix.back() + pscan);
       palettes_host.push_back(md.skinned ? &md.bone_influences[0].m[0][0] : nullptr);
       palette_count.push_back(md.skinned ? md.bone_count : 0);
       skw_host.push_back(mesh.K ? md.shape_key_weights : nullptr);
@@ -1037,7 +1047,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
 
   // device tables: FrameConst[jobs], parts, lights, plights, prefixes, palettes, shape-key weights
   const uint64_t tbytes = carve_size<FrameConst>(ndesc) + carve_size<PartGPU>(P_n) + carve_size<LightGPU>(lights.size()) +
-                          carve_size<PartLight>(plights.size()) + 3 * carve_size<uint32_t>(P_n + 1) +
+                          carve_size<PartLight>(plights.size()) + 4 * carve_size<uint32_t>(P_n + 1) +
                           carve_size<float>(pal_floats) + carve_size<float>(skw_floats);
   t3d_status st = ensure(c, f->tables, f->tables_cap, tbytes);
   if (st) return st;
@@ -1082,6 +1092,8 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   uint32_t* h_tpre = host.take<uint32_t>(P_n + 1);
   uint32_t* d_spre = dev.take<uint32_t>(P_n + 1);
   uint32_t* h_spre = host.take<uint32_t>(P_n + 1);
+  uint32_t* d_ppre = dev.take<uint32_t>(P_n + 1);
+  uint32_t* h_ppre = host.take<uint32_t>(P_n + 1);
   float* d_pal = dev.take<float>(pal_floats);
   float* h_pal = host.take<float>(pal_floats);
   float* d_skw = dev.take<float>(skw_floats);
@@ -1106,6 +1118,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   std::memcpy(h_vpre, f->vchunk_prefix.data(), 4 * (P_n + 1));
   std::memcpy(h_tpre, f->tile_prefix.data(), 4 * (P_n + 1));
   std::memcpy(h_spre, f->sblock_prefix.data(), 4 * (P_n + 1));
+  std::memcpy(h_ppre, f->pscan_prefix.data(), 4 * (P_n + 1));
   f->table_bytes = host.off;
 
   Carve sc{reinterpret_cast<uint8_t*>(f->scratch.ptr)};
@@ -1121,6 +1134,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   a.vchunk_prefix = d_vpre;
   a.tile_prefix = d_tpre;
   a.sblock_prefix = d_spre;
+  a.pscan_prefix = d_ppre;
   {
     float4* v = sc.take<float4>(2 * sumV);
     a.vstride = tu.vertex_split ? 1u : 2u;
@@ -1144,6 +1158,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   f->total_vchunks = f->vchunk_prefix.back();
   f->total_tiles = f->tile_prefix.back();
   f->total_sblocks = sblocks;
+  f->total_pscan = f->pscan_prefix.back();
   f->total_triangles = sumT;
   if (c->profiling) {   // the caller holds ctx->mu
     const auto wall2 = std::chrono::steady_clock::now();
@@ -1187,10 +1202,10 @@ t3d_status frame_launch(t3d_ctx* c, t3d_frame* f, hipStream_t stream, hipStream_
     }
   } else if (allow_graph && c->tuning.use_graph && !(profile && c->profiling)) {
     // graph of the whole chain; reused while the launch shape is unchanged
-    std::vector<uint8_t> key(sizeof(FrameArgs) + 8 * sizeof(uint32_t));
-    uint32_t shape[8] = {f->total_vchunks,       f->total_tiles,           f->total_sblocks,
+    std::vector<uint8_t> key(sizeof(FrameArgs) + 9 * sizeof(uint32_t));
+    uint32_t shape[9] = {f->total_vchunks,       f->total_tiles,           f->total_sblocks,
                          f->vertex_threads,      c->tuning.store_mode,     c->tuning.scatter_waves,
-                         c->tuning.assemble_split, c->tuning.cull_block};
+                         c->tuning.assemble_split, c->tuning.cull_block,   f->total_pscan};
     std::memcpy(key.data(), &a, sizeof(FrameArgs));
     std::memcpy(key.data() + sizeof(FrameArgs), shape, sizeof shape);
     if (!f->graph_exec || key != f->graph_key) {
@@ -1484,7 +1499,7 @@ t3d_status t3d_autotune(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc, u
       {&t3d_tuning::vertex_block, {128, 256}},
       {&t3d_tuning::vertices_per_thread, {1, 2, 4}},
       {&t3d_tuning::scatter_waves, {1, 4}},
-      {&t3d_tuning::assemble_split, {1, 2, 4, 8}},
+      {&t3d_tuning::assemble_split, {1, 2, 4, 8, 16, 32}},
       {&t3d_tuning::store_mode, {0, 2, 4, 6, 8, 10, 12, 14}},   // staged rows (64 or 32), streaming or not, pipelined or not
       {&t3d_tuning::xcd_swizzle, {0, 1}},
       {&t3d_tuning::vertex_split, {0, 1}},

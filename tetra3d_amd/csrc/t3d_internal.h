@@ -29,6 +29,7 @@ constexpr uint32_t kMaxLights = 64;    // lights one part may see on the GPU pat
 constexpr uint32_t kMaxLdsBones = 64;  // bone palettes staged in LDS up to this size
 constexpr uint32_t kScanBlock = 2048;  // elements per scan block (256 threads x 8)
 constexpr uint32_t kMaxTile = 2048;    // triangles per tile (keys of a tile are staged in LDS)
+constexpr uint32_t kScanPartMax = 65536;  // histogram matrices up to this size are scanned by one block
 
 // PartGPU.flags
 enum : uint32_t {
@@ -124,6 +125,7 @@ struct FrameArgs {
   const uint32_t* vchunk_prefix;  // [P+1]
   const uint32_t* tile_prefix;    // [P+1]
   const uint32_t* sblock_prefix;  // [P+1]
+  const uint32_t* pscan_prefix;   // [P+1] one k_scan_part block per multi-tile part with a small matrix
   float4* vclip;                // clip position of vertex record o at vclip[o * vstride]
   float4* vcol;                 // its colour at vcol[o * vstride]
   uint32_t vstride;             // 2: interleaved records; 1: two arrays
@@ -151,6 +153,7 @@ hipError_t launch_cull(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_
 hipError_t launch_hist(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s);
 hipError_t launch_scan(const FrameArgs& a, uint32_t sblk0, uint32_t nsblk, uint32_t part0, uint32_t nparts,
                        hipStream_t s);
+hipError_t launch_scan_part(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s);
 hipError_t launch_scatter(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t waves, hipStream_t s);
 hipError_t launch_assemble(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t mode, uint32_t split,
                            hipStream_t s);

@@ -607,6 +607,39 @@ __global__ __launch_bounds__(256) void k_scan_down(FrameArgs a, uint32_t sblk0) 
   }
 }
 
+// Exclusive scan of a whole [bin][tile] matrix by one 1024-thread block, in
+// coalesced chunks of 1024 entries with a running carry: the one launch
+// that replaces the three of the general scan for the (common) parts whose
+// matrix has at most kScanPartMax entries.
+__global__ __launch_bounds__(1024) void k_scan_part(FrameArgs a, uint32_t blk0) {
+  const uint32_t b = blk0 + blockIdx.x;
+  const uint32_t p = find_part(a.pscan_prefix, a.nparts, b);
+  const uint32_t len = kBins * a.parts[p].ntiles;
+  uint32_t* __restrict__ h = a.hist + a.parts[p].hbase;
+  const uint32_t tid = threadIdx.x, lane = tid % 64u, wv = tid / 64u;
+  __shared__ uint32_t s_w[16];
+  uint32_t carry = 0;
+  for (uint32_t c0 = 0; c0 < len; c0 += 1024) {
+    const uint32_t e = c0 + tid;
+    const uint32_t v = e < len ? h[e] : 0u;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) s_w[wv] = x;
+    __syncthreads();
+    uint32_t wbase = 0, tot = 0;
+    for (uint32_t w = 0; w < 16; w++) {
+      if (w < wv) wbase += s_w[w];
+      tot += s_w[w];
+    }
+    if (e < len) h[e] = carry + wbase + x - v;
+    carry += tot;
+    __syncthreads();
+  }
+}
+
 // Stable counting sort, scatter pass.  One wavefront per tile walks its
 // triangles in index order, 64 at a time (keys staged in LDS in one burst),
 // ranks equal bins with ballots and gives every survivor its output slot:
@@ -1130,6 +1163,11 @@ hipError_t launch_scan(const FrameArgs& a, uint32_t sblk0, uint32_t nsblk, uint3
   hipLaunchKernelGGL(k_scan_reduce, dim3(nsblk), dim3(256), 0, s, a, sblk0);
   hipLaunchKernelGGL(k_scan_bsums, dim3((nparts + 63) / 64), dim3(64), 0, s, a, part0, nparts);
   hipLaunchKernelGGL(k_scan_down, dim3(nsblk), dim3(256), 0, s, a, sblk0);
+  return hipGetLastError();
+}
+hipError_t launch_scan_part(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s) {
+  if (!nblk) return hipSuccess;
+  hipLaunchKernelGGL(k_scan_part, dim3(nblk), dim3(1024), 0, s, a, blk0);
   return hipGetLastError();
 }
 hipError_t launch_scatter(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t waves, hipStream_t s) {

@@ -72,7 +72,8 @@ struct t3d_frame {
   uint64_t total_triangles = 0;
   uint32_t nparts = 0;
   uint32_t total_vchunks = 0, total_tiles = 0, total_sblocks = 0;
-  std::vector<uint32_t> vchunk_prefix, tile_prefix, sblock_prefix;
+  std::vector<uint32_t> vchunk_prefix, tile_prefix, sblock_prefix, pscan_prefix;
+  uint32_t total_pscan = 0;
   uint32_t vertex_threads = 256;
   uint64_t total_survivors = 0;
   double last_ms = 0;
