@@ -607,36 +607,43 @@ __global__ __launch_bounds__(256) void k_scan_down(FrameArgs a, uint32_t sblk0) 
   }
 }
 
-// Exclusive scan of a whole [bin][tile] matrix by one 1024-thread block, in
-// coalesced chunks of 1024 entries with a running carry: the one launch
-// that replaces the three of the general scan for the (common) parts whose
-// matrix has at most kScanPartMax entries.
+// Exclusive scan of a whole [bin][tile] matrix by one 1024-thread block: each
+// thread sums a contiguous segment (16-byte loads), one block-wide scan of the
+// 1024 sums, then each thread rewrites its segment.  One launch and two
+// barriers replace the three launches of the general scan for the (common)
+// parts whose matrix has at most kScanPartMax entries.
 __global__ __launch_bounds__(1024) void k_scan_part(FrameArgs a, uint32_t blk0) {
   const uint32_t b = blk0 + blockIdx.x;
   const uint32_t p = find_part(a.pscan_prefix, a.nparts, b);
-  const uint32_t len = kBins * a.parts[p].ntiles;
-  uint32_t* __restrict__ h = a.hist + a.parts[p].hbase;
+  const uint32_t len = kBins * a.parts[p].ntiles;            // a multiple of 512
+  uint4* __restrict__ h = reinterpret_cast<uint4*>(a.hist + a.parts[p].hbase);   // 2 KB aligned
   const uint32_t tid = threadIdx.x, lane = tid % 64u, wv = tid / 64u;
+  const uint32_t per = (len / 4 + 1023) / 1024;              // uint4 per thread
+  const uint32_t q0 = min(tid * per, len / 4), q1 = min(q0 + per, len / 4);
+  uint32_t sum = 0;
+  for (uint32_t q = q0; q < q1; q++) {
+    const uint4 v = h[q];
+    sum += v.x + v.y + v.z + v.w;
+  }
   __shared__ uint32_t s_w[16];
-  uint32_t carry = 0;
-  for (uint32_t c0 = 0; c0 < len; c0 += 1024) {
-    const uint32_t e = c0 + tid;
-    const uint32_t v = e < len ? h[e] : 0u;
-    uint32_t x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = __shfl_up(x, o);
-      if (lane >= (uint32_t)o) x += y;
-    }
-    if (lane == 63) s_w[wv] = x;
-    __syncthreads();
-    uint32_t wbase = 0, tot = 0;
-    for (uint32_t w = 0; w < 16; w++) {
-      if (w < wv) wbase += s_w[w];
-      tot += s_w[w];
-    }
-    if (e < len) h[e] = carry + wbase + x - v;
-    carry += tot;
-    __syncthreads();
+  uint32_t x = sum;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == 63) s_w[wv] = x;
+  __syncthreads();
+  uint32_t run = x - sum;
+  for (uint32_t w = 0; w < wv; w++) run += s_w[w];
+  for (uint32_t q = q0; q < q1; q++) {
+    const uint4 v = h[q];
+    uint4 o;
+    o.x = run;
+    o.y = o.x + v.x;
+    o.z = o.y + v.y;
+    o.w = o.z + v.z;
+    run = o.w + v.w;
+    h[q] = o;
   }
 }
 
