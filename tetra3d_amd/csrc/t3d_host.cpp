@@ -333,6 +333,10 @@ LightGPU make_light(t3d_ctx* c, const t3d_light_desc& d) {
   l.range = d.range;
   l.range_sq = d.range * d.range;
   for (int i = 0; i < 3; i++) l.color[i] = d.color[i];
+  const V3 wp = row3(d.transform);
+  l.pos[0] = wp.x;
+  l.pos[1] = wp.y;
+  l.pos[2] = wp.z;
   if (d.type == T3D_LIGHT_AMBIENT) {
     for (int i = 0; i < 3; i++) l.amb[i] = d.color[i] * d.energy;
   } else if (d.type == T3D_LIGHT_DIRECTIONAL) {
@@ -456,6 +460,7 @@ t3d_status t3d_mesh_upload(t3d_ctx* c, const t3d_mesh_desc* d, uint64_t* out_han
     m->dmin[i] = d->dimensions_min[i];
     m->dmax[i] = d->dimensions_max[i];
   }
+  m->max_span = magnitude(sub(v3(m->dmax[0], m->dmax[1], m->dmax[2]), v3(m->dmin[0], m->dmin[1], m->dmin[2])));
   // validate indices and part ranges; referenced vertex range per part
   for (uint64_t i = 0; i < 3 * T; i++)
     if (d->indices[i] >= d->vertex_count)
@@ -642,6 +647,10 @@ t3d_status run_chain(t3d_ctx* c, t3d_frame* f, uint32_t p0, uint32_t p1, hipStre
   const uint32_t sb0 = f->sblock_prefix[p0], sb1 = f->sblock_prefix[p1];
   const uint32_t ps0 = f->pscan_prefix[p0], ps1 = f->pscan_prefix[p1];
   {
+    ProfScope ps(c, st, "k_uniforms", 0, prof);
+    HIP_TRY(c, launch_uniforms(a, p0, p1 - p0, st));
+  }
+  {
     ProfScope ps(c, st, "k_vertex", tr.vertex, prof);
     HIP_TRY(c, launch_vertex(a, vb0, vb1 - vb0, f->vertex_threads, st));
   }
@@ -695,7 +704,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   f->job_pair_begin.assign(1, 0);
   f->fcs.clear();
   std::vector<LightGPU> lights;
-  std::vector<PartLight> plights;
+  uint64_t n_plights = 0;   // (pair, light) entries, filled on the device
   std::vector<const float*> palettes_host;   // host palettes to upload, per part (nullptr if none)
   std::vector<uint32_t> palette_count;
   std::vector<const float*> skw_host;
@@ -842,16 +851,6 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
     }
     f->fcs.push_back(fc);
 
-    std::map<uint32_t, M4> rd_cache;
-    // Decompose(Inverted(M)) per model: shared by invertedCamPos (model.go:629-630)
-    // and every point light's workingPosition (light.go:175-189)
-    struct InvDec {
-      uint32_t model = UINT32_MAX;
-      V3 p, is;
-      M4 r;
-      V3 apply(V3 x) const { return add(mult_vec(r, x), hmul(p, is)); }
-    } inv;
-
     for (int32_t di : draw) {
       const Pair& rp = pairs[di];
       const t3d_model_desc& md = desc->models[rp.model];
@@ -865,22 +864,12 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
       if (world) lighting = mat ? (world->lighting_on && !mat->shadeless && !md.shadeless) : (world->lighting_on && !md.shadeless);
       const bool early_out = (md.color[3] == 0 && is_transparent(md, mat)) || !md.visible;   // model.go:604-606
 
+      // MVP, invertedCamPos, the directional working rotation and the
+      // per-light values are computed on the device (k_uniforms)
       PartGPU P{};
-      const M4& M = md.transform;
-      P.mvp = mult(M, cam.vp);                                          // model.go:695
-      P.model = M;
+      P.model = md.transform;
       P.cam = job;
-      if (inv.model != rp.model) {
-        M4 im = inverted(M);
-        V3 sc;
-        decompose(im, inv.p, sc, inv.r);
-        inv.is = v3(fdiv(1.0f, sc.x), fdiv(1.0f, sc.y), fdiv(1.0f, sc.z));
-        inv.model = rp.model;
-      }
-      V3 icp = inv.apply(cam.pos);                                      // model.go:629-630
-      P.icp[0] = icp.x;
-      P.icp[1] = icp.y;
-      P.icp[2] = icp.z;
+      P.max_span = mesh.max_span;
       float mp[4] = {md.color[0], md.color[1], md.color[2], md.color[3]};   // camera.go:1597-1601
       if (mat)
         for (int i = 0; i < 4; i++) mp[i] *= mat->color[i];
@@ -924,50 +913,16 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
           lcount = md.light_group_count;
         }
         if (lcount > kMaxLights) return fail(c, T3D_ERR_UNSUPPORTED, "render: %u lights on one model (max %u)", lcount, kMaxLights);
-        P.light_base = (uint32_t)plights.size();
+        P.light_base = (uint32_t)n_plights;
         P.light_count = lcount;
-        const float maxSpan = magnitude(sub(v3(mesh.dmax[0], mesh.dmax[1], mesh.dmax[2]), v3(mesh.dmin[0], mesh.dmin[1], mesh.dmin[2])));
-        const V3 modelPos = row3(M);
+        P.light0 = lbase;
+        n_plights += lcount;
         bool need_rd = false;
-        for (uint32_t k = 0; k < lcount; k++) {
-          const uint32_t li = lbase + k;
-          const t3d_light_desc& ld = md.light_group_active ? md.light_group[k] : scene_lights[k].get();
-          PartLight pl{};
-          pl.light = li;
-          pl.enabled = 1;
-          if (ld.type == T3D_LIGHT_POINT) {
-            V3 wp = row3(ld.transform);
-            V3 w = md.skinned ? wp : inv.apply(wp);
-            pl.v[0] = w.x;
-            pl.v[1] = w.y;
-            pl.v[2] = w.z;
-            if (ld.range > 0) {
-              float dist = maxSpan + ld.range;
-              if (dist_sq(modelPos, wp) > dist * dist) pl.enabled = 0;
-            }
-          } else if (ld.type == T3D_LIGHT_DIRECTIONAL) {
-            need_rd = true;
-          } else if (ld.type == T3D_LIGHT_VOLUME && lv_mode == T3D_LV_PER_OBJECT) {
-            const Volume& vol = *c->volumes[ld.volume];
-            V3 q = modelPos;
-            int64_t x = go_int(round32((q.x - ld.offset[0] - ld.volume_min[0] - (ld.cell_size[0] / 2)) / ld.cell_size[0]));
-            int64_t y = go_int(round32((q.y - ld.offset[1] - ld.volume_min[1] - (ld.cell_size[1] / 2)) / ld.cell_size[1]));
-            int64_t z = go_int(round32((q.z - ld.offset[2] - ld.volume_min[2] - (ld.cell_size[2] / 2)) / ld.cell_size[2]));
-            const int cx = ld.cell_count[0], cy = ld.cell_count[1], cz = ld.cell_count[2];
-            if (!(x < 0 || y < 0 || z < 0 || x >= cx || y >= cy || z >= cz)) {
-              int64_t ci = x + (y * cx) + (z * cx * cy);
-              if (ci < (int64_t)vol.n) {
-                pl.cell_found = 1;
-                for (int qq = 0; qq < 3; qq++) pl.v[qq] = vol.host[4 * ci + qq];
-              }
-            }
-          }
-          plights.push_back(pl);
-        }
-        if (need_rd && !md.skinned) {
-          auto r = rd_cache.find(rp.model);
-          if (r == rd_cache.end()) r = rd_cache.emplace(rp.model, transposed(inverted(md.world_rotation))).first;
-          P.rd = r->second;
+        for (uint32_t k = 0; k < lcount && !need_rd; k++)
+          need_rd = lights[lbase + k].type == T3D_LIGHT_DIRECTIONAL;
+        if (need_rd && !md.skinned) {       // k_uniforms inverts it (light.go:326-330)
+          P.rd = md.world_rotation;
+          P.flags |= PF_NEED_RD;
         }
       }
 
@@ -1045,9 +1000,9 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
     skw_floats += skw_count[i];
   }
 
-  // device tables: FrameConst[jobs], parts, lights, plights, prefixes, palettes, shape-key weights
+  // device tables: FrameConst[jobs], parts, lights, prefixes, palettes, shape-key weights
   const uint64_t tbytes = carve_size<FrameConst>(ndesc) + carve_size<PartGPU>(P_n) + carve_size<LightGPU>(lights.size()) +
-                          carve_size<PartLight>(plights.size()) + 4 * carve_size<uint32_t>(P_n + 1) +
+                          4 * carve_size<uint32_t>(P_n + 1) +
                           carve_size<float>(pal_floats) + carve_size<float>(skw_floats);
   t3d_status st = ensure(c, f->tables, f->tables_cap, tbytes);
   if (st) return st;
@@ -1061,7 +1016,8 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   const uint32_t ntiles_total = f->tile_prefix.back();
   const uint64_t sbytes = carve_size<float4>(2 * sumV) + carve_size<float4>(sumV) + carve_size<uint32_t>(sumT) +
                           2 * carve_size<uint4>(sumT) + carve_size<uint32_t>(4ull * P_n) + carve_size<uint32_t>(sumHist) +
-                          carve_size<uint32_t>(sblocks) + carve_size<uint32_t>(ntiles_total) + carve_size<float4>(128);
+                          carve_size<uint32_t>(sblocks) + carve_size<uint32_t>(ntiles_total) + carve_size<float4>(128) +
+                          carve_size<PartLight>(n_plights);
   st = ensure(c, f->scratch, f->scratch_cap, sbytes);
   if (st) return st;
   const uint64_t obytes = 2 * carve_size<t3d_vertex>(3 * sumT) + carve_size<uint32_t>(3 * sumT);
@@ -1084,8 +1040,6 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   PartGPU* h_parts = host.take<PartGPU>(P_n);
   LightGPU* d_lights = dev.take<LightGPU>(lights.size());
   LightGPU* h_lights = host.take<LightGPU>(lights.size());
-  PartLight* d_pl = dev.take<PartLight>(plights.size());
-  PartLight* h_pl = host.take<PartLight>(plights.size());
   uint32_t* d_vpre = dev.take<uint32_t>(P_n + 1);
   uint32_t* h_vpre = host.take<uint32_t>(P_n + 1);
   uint32_t* d_tpre = dev.take<uint32_t>(P_n + 1);
@@ -1114,7 +1068,6 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   }
   if (P_n) std::memcpy(h_parts, f->parts.data(), sizeof(PartGPU) * P_n);
   if (!lights.empty()) std::memcpy(h_lights, lights.data(), sizeof(LightGPU) * lights.size());
-  if (!plights.empty()) std::memcpy(h_pl, plights.data(), sizeof(PartLight) * plights.size());
   std::memcpy(h_vpre, f->vchunk_prefix.data(), 4 * (P_n + 1));
   std::memcpy(h_tpre, f->tile_prefix.data(), 4 * (P_n + 1));
   std::memcpy(h_spre, f->sblock_prefix.data(), 4 * (P_n + 1));
@@ -1130,7 +1083,6 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   a.tile = tile;
   a.xcd_swizzle = tu.xcd_swizzle ? 1u : 0u;
   a.lights = d_lights;
-  a.plights = d_pl;
   a.vchunk_prefix = d_vpre;
   a.tile_prefix = d_tpre;
   a.sblock_prefix = d_spre;
@@ -1150,6 +1102,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   a.bsum = sc.take<uint32_t>(sblocks);
   a.tile_count = sc.take<uint32_t>(ntiles_total);
   a.sink = sc.take<float4>(128);
+  a.plights = sc.take<PartLight>(n_plights);
   a.out_color = reinterpret_cast<float4*>(oc.take<t3d_vertex>(3 * sumT));
   a.out_depth = reinterpret_cast<float4*>(oc.take<t3d_vertex>(3 * sumT));
   a.out_idx = oc.take<uint32_t>(3 * sumT);

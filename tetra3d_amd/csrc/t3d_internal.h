@@ -39,6 +39,7 @@ enum : uint32_t {
   PF_HAS_COLOR = 1u << 3,
   PF_HAS_UV = 1u << 4,
   PF_SHAPE_KEYS = 1u << 5,
+  PF_NEED_RD = 1u << 6,  // a directional light lights this static part: k_uniforms inverts its rotation
   PF_SORT_SHIFT = 8,     // 2 bits: T3D_SORT_*
   PF_LMODE_SHIFT = 10,   // 2 bits: T3D_LIGHTING_*
   PF_LVMODE_SHIFT = 12,  // 2 bits: T3D_LV_*
@@ -47,11 +48,12 @@ enum : uint32_t {
 // Per render pair: uniforms (camera.go:1507-1601, model.go:600-695) and the
 // pair's slices of the frame arrays.
 struct alignas(16) PartGPU {
-  M4 mvp;              // Model.Transform()·VP (model.go:695)
-  M4 rd;               // DirectionalLight working rotation: Transposed(Inverted(WorldRotation)) (light.go:326-330)
+  M4 mvp;              // Model.Transform()·VP (model.go:695), computed by k_uniforms
+  M4 rd;               // DirectionalLight working rotation: Transposed(Inverted(WorldRotation)) (light.go:326-330);
+                       // uploaded as the model's WorldRotation, k_uniforms turns it into this
   M4 model;            // Model.Transform() (LightVolume lookups, light.go:610-628)
   float mp[4];         // mpColor = Model.Color ⊙ Material.Color (camera.go:1597-1601)
-  float icp[3];        // invertedCamPos (model.go:629-630)
+  float icp[3];        // invertedCamPos (model.go:629-630), computed by k_uniforms
   uint32_t flags;
   float src_w, src_h;  // texture size when the material textures (camera.go:1560-1566)
   uint32_t light_base, light_count;
@@ -72,7 +74,9 @@ struct alignas(16) PartGPU {
   uint32_t tbase, ntiles;     // first slot in the triangle scratch; tiles of the part
   uint32_t hbase, sblock0;    // histogram offset and first scan block (multi-tile parts)
   uint32_t nsblocks, bone_count;
-  uint32_t cam, pad1;         // camera (job) of the pair in FrameArgs.fc
+  uint32_t cam;               // camera (job) of the pair in FrameArgs.fc
+  float max_span;             // MaxSpan of the mesh dimensions (mesh.go:41-43), for the point-light skip
+  uint32_t light0, pad2;      // first FrameArgs.lights entry of the pair's light list
   uint64_t out_base;          // first output vertex
 };
 
@@ -89,10 +93,12 @@ struct alignas(16) LightGPU {
   uint32_t ncells;
   int32_t cx, cy, cz;
   float vmin[3], vmax[3], cell[3], offset[3], bias[3];
-  float pad[1];
+  float pos[3];         // world position (row 3 of the light's transform), for beginModel
+  float pad[2];
 };
 
-// Per (render pair, light) values from beginModel and the per-model skip.
+// Per (render pair, light) values from beginModel and the per-model skip,
+// written by k_uniforms (frame scratch, never uploaded).
 struct alignas(16) PartLight {
   float v[3];           // point: workingPosition; volume per-object mode: cell colour
   uint32_t light;       // index into the frame's LightGPU array
@@ -121,7 +127,7 @@ struct FrameArgs {
   uint32_t tile;                // triangles per tile
   uint32_t xcd_swizzle;         // 1: give each XCD a contiguous range of blocks (see block_index)
   const LightGPU* lights;
-  const PartLight* plights;
+  PartLight* plights;
   const uint32_t* vchunk_prefix;  // [P+1]
   const uint32_t* tile_prefix;    // [P+1]
   const uint32_t* sblock_prefix;  // [P+1]
@@ -148,6 +154,7 @@ struct FrameArgs {
 // Kernel launchers (t3d_kernels.hip).  blk0 offsets blockIdx so the plain
 // form can launch one part's blocks at a time.
 hipError_t launch_init_stats(const FrameArgs& a, uint32_t nparts, hipStream_t s);
+hipError_t launch_uniforms(const FrameArgs& a, uint32_t part0, uint32_t nparts, hipStream_t s);
 hipError_t launch_vertex(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t threads, hipStream_t s);
 hipError_t launch_cull(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t threads, hipStream_t s);
 hipError_t launch_hist(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s);

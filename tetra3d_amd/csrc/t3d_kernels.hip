@@ -304,6 +304,62 @@ __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
 // k_cull
 // ---------------------------------------------------------------------------
 
+// Per-pair uniforms, one thread per render pair: MVP (model.go:695),
+// invertedCamPos (model.go:629-630), the directional working rotation
+// (light.go:326-330) and, per light of the pair, beginModel's working
+// position (light.go:175-189), the per-model point-light skip
+// (camera.go:1592, 1625-1629) and the per-object LightVolume cell
+// (light.go:610-628).  The same float32 functions the host would run, so the
+// values are the ones the reference computes on the CPU.
+__global__ __launch_bounds__(64) void k_uniforms(FrameArgs a, uint32_t part0, uint32_t nparts) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nparts) return;
+  PartGPU& P = const_cast<PartGPU*>(a.parts)[part0 + i];
+  const FrameConst& fc = a.fc[P.cam];
+  const uint32_t flags = P.flags;
+  const M4 M = P.model;
+  P.mvp = mult(M, fc.vp);
+  V3 dp, ds;
+  M4 dr;
+  decompose(inverted(M), dp, ds, dr);
+  const V3 is = v3(fdiv(1.0f, ds.x), fdiv(1.0f, ds.y), fdiv(1.0f, ds.z));
+  const V3 icp = add(mult_vec(dr, v3(fc.campos[0], fc.campos[1], fc.campos[2])), hmul(dp, is));
+  P.icp[0] = icp.x;
+  P.icp[1] = icp.y;
+  P.icp[2] = icp.z;
+  if (flags & PF_NEED_RD) P.rd = transposed(inverted(P.rd));
+  if (!(flags & PF_LIGHTING)) return;
+  const bool skinned = (flags & PF_SKINNED) != 0;
+  const uint32_t lvmode = (flags >> PF_LVMODE_SHIFT) & 3u;
+  const V3 modelPos = row3(M);
+  for (uint32_t k = 0; k < P.light_count; k++) {
+    PartLight pl{};
+    pl.light = P.light0 + k;
+    pl.enabled = 1;
+    const LightGPU& L = a.lights[pl.light];
+    if (L.type == T3D_LIGHT_POINT) {
+      const V3 wp = v3(L.pos[0], L.pos[1], L.pos[2]);
+      const V3 w = skinned ? wp : add(mult_vec(dr, wp), hmul(dp, is));
+      pl.v[0] = w.x;
+      pl.v[1] = w.y;
+      pl.v[2] = w.z;
+      if (L.range > 0) {
+        const float dist = P.max_span + L.range;
+        if (dist_sq(modelPos, wp) > dist * dist) pl.enabled = 0;
+      }
+    } else if (L.type == T3D_LIGHT_VOLUME && lvmode == T3D_LV_PER_OBJECT) {
+      const float* cell = volume_cell(L, modelPos);
+      if (cell) {
+        pl.cell_found = 1;
+        pl.v[0] = cell[0];
+        pl.v[1] = cell[1];
+        pl.v[2] = cell[2];
+      }
+    }
+    a.plights[P.light_base + k] = pl;
+  }
+}
+
 __global__ void k_init_stats(uint32_t* stats, uint32_t nparts) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nparts) {
@@ -1142,6 +1198,11 @@ __global__ void k_fp_selftest(const float* x, const float* y, float* out, uint32
 hipError_t launch_init_stats(const FrameArgs& a, uint32_t nparts, hipStream_t s) {
   if (!nparts) return hipSuccess;
   hipLaunchKernelGGL(k_init_stats, dim3((nparts + 255) / 256), dim3(256), 0, s, a.stats, nparts);
+  return hipGetLastError();
+}
+hipError_t launch_uniforms(const FrameArgs& a, uint32_t part0, uint32_t nparts, hipStream_t s) {
+  if (!nparts) return hipSuccess;
+  hipLaunchKernelGGL(k_uniforms, dim3((nparts + 63) / 64), dim3(64), 0, s, a, part0, nparts);
   return hipGetLastError();
 }
 hipError_t launch_vertex(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t threads, hipStream_t s) {

@@ -38,6 +38,7 @@ struct Mesh {
   std::vector<t3d_part_range> parts;
   std::vector<uint32_t> part_vmin, part_vcount;
   float dmin[3], dmax[3];
+  float max_span = 0;              // MaxSpan(Dimensions) (mesh.go:41-43)
   uint32_t max_bone_index = 0;
   bool has_skin = false;
   int refs = 0;                    // submitted jobs referencing the mesh
