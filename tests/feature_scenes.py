@@ -169,3 +169,32 @@ def feature_scene(seed):
             m.light_group = [l for l in lights if not isinstance(l, LightVolume) and rng.random() < 0.6]
         models.append(m)
     return W.Workload("features_%d" % seed, cam, Scene("features", world), lights, models)
+
+
+def pair_order_scene(seed, n_models=24):
+    """Models stacked on a few shared positions (equal camera distances), one
+    render order for the solids and several for the transparent parts: the
+    order of render pairs then rests on the stable-sort ties."""
+    rng = np.random.default_rng(seed)
+    cam = Camera("cam", 640, 360)
+    cam.render_depth = bool(seed % 2)
+    spots = rng.uniform([-4, -3, -12], [4, 3, -5], (4, 3)).astype(f32)
+    lights = [DirectionalLight("sun", 1, 1, 1, 1.0)]
+    lights[0].set_local_rotation(W.axis_angle_matrix((1, 0, 0), -0.6))
+    models = []
+    for i in range(n_models):
+        mesh = W.icosphere(1, float(rng.uniform(0.5, 1.5)), name="s%d" % i)
+        mat = Material("mat%d" % i)
+        transparent = rng.random() < 0.4
+        mat.color = np.array([*rng.uniform(0.3, 1, 3), 0.5 if transparent else 1.0], dtype=f32)
+        mat.render_order = int(rng.integers(-1, 2)) if transparent else 0
+        mat.backface_culling = True
+        for p in mesh.mesh_parts:
+            p.material = mat
+        m = Model("model%d" % i, mesh)
+        m.set_local_position(*spots[rng.integers(0, len(spots))])
+        m.set_local_rotation(_rotation(rng))
+        models.append(m)
+    world = World()
+    world.ambient_light.energy = f32(0.3)
+    return W.Workload("pair_order_%d" % seed, cam, Scene("pair_order", world), lights, models)

@@ -3,7 +3,7 @@ options (see tests/feature_scenes.py), in both execution forms."""
 import pytest
 
 from tetra3d_amd import _abi as A
-from tests.feature_scenes import feature_scene
+from tests.feature_scenes import feature_scene, pair_order_scene
 from tests.parity import compare_frames
 
 SEEDS = list(range(40))
@@ -20,3 +20,14 @@ def test_feature_scenes(gpu_ctx, oracle, mode):
         s = compare_frames(g, oracle.render_workload(wl))
         drawn += s["survivors"]
     assert drawn > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [A.EXEC_BATCHED, A.EXEC_PLAIN], ids=["batched", "plain"])
+def test_pair_order_ties(gpu_ctx, oracle, mode):
+    """Render pairs at equal camera distances keep the reference's stable
+    order (camera.go:1458-1467, 2050-2055)."""
+    for seed in range(6):
+        wl = pair_order_scene(seed)
+        print("pair-order scene", seed, "mode", mode, flush=True)
+        compare_frames(gpu_ctx.render_workload(wl, exec_mode=mode), oracle.render_workload(wl))

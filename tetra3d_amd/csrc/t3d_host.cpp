@@ -375,7 +375,35 @@ struct Pair {
   uint32_t model, part;
   float depth;
   int32_t order;
+  const Mesh* mesh;
 };
+
+// Sorts render-pair indices by (order ascending, depth descending), ties in
+// their current position order.  Where that comparator is a strict weak
+// order, any stable sort, Go's sort.SliceStable included, yields exactly this
+// permutation, and a plain sort on (order, -depth, position) gets it in
+// O(n log n) instead of SliceStable's insertion + SymMerge passes.  Returns
+// false (d untouched) when a NaN depth breaks the strict weak order.
+bool sort_pairs_keyed(std::vector<int32_t>& d, const std::vector<Pair>& pairs) {
+  struct Key {
+    int32_t order;
+    float neg_depth;
+    int32_t pos, idx;
+  };
+  std::vector<Key> kv(d.size());
+  for (size_t i = 0; i < d.size(); i++) {
+    const Pair& p = pairs[d[i]];
+    if (std::isnan(p.depth)) return false;
+    kv[i] = Key{p.order, -p.depth, (int32_t)i, d[i]};
+  }
+  std::sort(kv.begin(), kv.end(), [](const Key& a, const Key& b) {
+    if (a.order != b.order) return a.order < b.order;
+    if (a.neg_depth != b.neg_depth) return a.neg_depth < b.neg_depth;
+    return a.pos < b.pos;
+  });
+  for (size_t i = 0; i < d.size(); i++) d[i] = kv[i].idx;
+  return true;
+}
 
 }  // namespace
 
@@ -779,16 +807,22 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
       for (uint32_t p = 0; p < mesh.parts.size(); p++) {
         const t3d_material_desc* mat = part_material(md, p);
         if (mat && !mat->visible) continue;
-        Pair rp{mi, p, dist_sq(cam.pos, row3(md.transform)), mat ? mat->render_order : 0};
+        Pair rp{mi, p, dist_sq(cam.pos, row3(md.transform)), mat ? mat->render_order : 0, &mesh};
         (is_transparent(md, mat) ? transparents : solids).push_back((int32_t)pairs.size());
         pairs.push_back(rp);
       }
     }
-    if (!cd->render_depth)                                             // camera.go:1458-1467
-      go_slice_stable(solids, [&](int32_t x, int32_t y) {
-        if (pairs[x].order < pairs[y].order) return true;
-        return pairs[x].depth > pairs[y].depth;
-      });
+    if (!cd->render_depth) {                                           // camera.go:1458-1467
+      // with one render order the comparator is depth-descending, a strict
+      // weak order; mixed orders need SliceStable's own steps
+      bool one_order = true;
+      for (int32_t i : solids) one_order = one_order && pairs[i].order == pairs[solids[0]].order;
+      if (!one_order || !sort_pairs_keyed(solids, pairs))
+        go_slice_stable(solids, [&](int32_t x, int32_t y) {
+          if (pairs[x].order < pairs[y].order) return true;
+          return pairs[x].depth > pairs[y].depth;
+        });
+    }
     if (cd->max_light_count > 0) {                                     // camera.go:1474-1497
       V3 fp = cd->has_max_light_count_focus
                   ? v3(cd->max_light_count_focus[0], cd->max_light_count_focus[1], cd->max_light_count_focus[2])
@@ -809,10 +843,11 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
       if ((int)sorted.size() > cd->max_light_count) sorted.resize(cd->max_light_count);
       scene_lights = sorted;
     }
-    go_slice_stable(transparents, [&](int32_t x, int32_t y) {           // camera.go:2050-2055
-      if (pairs[x].order == pairs[y].order) return pairs[x].depth > pairs[y].depth;
-      return pairs[x].order < pairs[y].order;
-    });
+    if (!sort_pairs_keyed(transparents, pairs))                         // camera.go:2050-2055
+      go_slice_stable(transparents, [&](int32_t x, int32_t y) {
+        if (pairs[x].order == pairs[y].order) return pairs[x].depth > pairs[y].depth;
+        return pairs[x].order < pairs[y].order;
+      });
     std::vector<int32_t> draw;                                          // camera.go:2057-2109
     draw.reserve(solids.size() + transparents.size());
     for (int32_t i : solids) draw.push_back(i);
@@ -855,7 +890,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
       const Pair& rp = pairs[di];
       const t3d_model_desc& md = desc->models[rp.model];
       if (!md.visible || md.auto_batch_static) continue;               // camera.go:2067-2071
-      const Mesh& mesh = *c->meshes[md.mesh];
+      const Mesh& mesh = *rp.mesh;
       const t3d_material_desc* mat = part_material(md, rp.part);
       if (mat && (mat->billboard || mat->custom_depth_function))
         return fail(c, T3D_ERR_UNSUPPORTED, "render: model %u part %u uses billboarding or a custom depth function", rp.model, rp.part);
@@ -1142,10 +1177,6 @@ t3d_status frame_launch(t3d_ctx* c, t3d_frame* f, hipStream_t stream, hipStream_
   if (!P_n) {
     HIP_TRY(c, hipEventRecord(f->done, stream));
     return T3D_OK;
-  }
-  {
-    ProfScope ps(c, stream, "k_init_stats", 0, profile);
-    HIP_TRY(c, launch_init_stats(a, P_n, stream));
   }
   t3d_status st = T3D_OK;
   if (exec_mode == T3D_EXEC_PLAIN) {

@@ -153,7 +153,6 @@ struct FrameArgs {
 
 // Kernel launchers (t3d_kernels.hip).  blk0 offsets blockIdx so the plain
 // form can launch one part's blocks at a time.
-hipError_t launch_init_stats(const FrameArgs& a, uint32_t nparts, hipStream_t s);
 hipError_t launch_uniforms(const FrameArgs& a, uint32_t part0, uint32_t nparts, hipStream_t s);
 hipError_t launch_vertex(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t threads, hipStream_t s);
 hipError_t launch_cull(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t threads, hipStream_t s);

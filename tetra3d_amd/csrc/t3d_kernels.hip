@@ -311,28 +311,26 @@ __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
 // (camera.go:1592, 1625-1629) and the per-object LightVolume cell
 // (light.go:610-628).  The same float32 functions the host would run, so the
 // values are the ones the reference computes on the CPU.
+// One wavefront per render pair; lane k handles the pair's light k (at most
+// kMaxLights = 64), lane 0 also the pair's own uniforms and the reset of its
+// depth range and survivor count, so the latency is one light's, not the sum
+// over lights.
 __global__ __launch_bounds__(64) void k_uniforms(FrameArgs a, uint32_t part0, uint32_t nparts) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nparts) return;
-  PartGPU& P = const_cast<PartGPU*>(a.parts)[part0 + i];
-  const FrameConst& fc = a.fc[P.cam];
+  PartGPU& P = const_cast<PartGPU*>(a.parts)[part0 + blockIdx.x];
+  const uint32_t k = threadIdx.x;
   const uint32_t flags = P.flags;
+  const uint32_t nl = (flags & PF_LIGHTING) ? P.light_count : 0u;
+  if (k > 0 && k >= nl) return;
+  const FrameConst& fc = a.fc[P.cam];
   const M4 M = P.model;
-  P.mvp = mult(M, fc.vp);
   V3 dp, ds;
   M4 dr;
   decompose(inverted(M), dp, ds, dr);
   const V3 is = v3(fdiv(1.0f, ds.x), fdiv(1.0f, ds.y), fdiv(1.0f, ds.z));
-  const V3 icp = add(mult_vec(dr, v3(fc.campos[0], fc.campos[1], fc.campos[2])), hmul(dp, is));
-  P.icp[0] = icp.x;
-  P.icp[1] = icp.y;
-  P.icp[2] = icp.z;
-  if (flags & PF_NEED_RD) P.rd = transposed(inverted(P.rd));
-  if (!(flags & PF_LIGHTING)) return;
-  const bool skinned = (flags & PF_SKINNED) != 0;
-  const uint32_t lvmode = (flags >> PF_LVMODE_SHIFT) & 3u;
-  const V3 modelPos = row3(M);
-  for (uint32_t k = 0; k < P.light_count; k++) {
+  if (k < nl) {
+    const bool skinned = (flags & PF_SKINNED) != 0;
+    const uint32_t lvmode = (flags >> PF_LVMODE_SHIFT) & 3u;
+    const V3 modelPos = row3(M);
     PartLight pl{};
     pl.light = P.light0 + k;
     pl.enabled = 1;
@@ -358,15 +356,18 @@ __global__ __launch_bounds__(64) void k_uniforms(FrameArgs a, uint32_t part0, ui
     }
     a.plights[P.light_base + k] = pl;
   }
-}
-
-__global__ void k_init_stats(uint32_t* stats, uint32_t nparts) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nparts) {
-    stats[4 * i + 0] = __float_as_uint(kMaxF32);  // minDepth = MaxFloat32 (model.go:702)
-    stats[4 * i + 1] = 0u;                        // max over non-negative keys
-    stats[4 * i + 2] = 0u;
-    stats[4 * i + 3] = 0u;
+  if (k == 0) {
+    uint32_t* st = a.stats + 4 * (part0 + blockIdx.x);
+    st[0] = __float_as_uint(kMaxF32);  // minDepth = MaxFloat32 (model.go:702)
+    st[1] = 0u;                        // max over non-negative keys
+    st[2] = 0u;                        // survivors
+    st[3] = 0u;
+    P.mvp = mult(M, fc.vp);
+    const V3 icp = add(mult_vec(dr, v3(fc.campos[0], fc.campos[1], fc.campos[2])), hmul(dp, is));
+    P.icp[0] = icp.x;
+    P.icp[1] = icp.y;
+    P.icp[2] = icp.z;
+    if (flags & PF_NEED_RD) P.rd = transposed(inverted(P.rd));
   }
 }
 
@@ -625,15 +626,22 @@ __global__ __launch_bounds__(256) void k_scan_reduce(FrameArgs a, uint32_t sblk0
   }
 }
 
-__global__ void k_scan_bsums(FrameArgs a, uint32_t part0, uint32_t nparts) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nparts) return;
-  const PartGPU& P = a.parts[part0 + i];
-  uint32_t run = 0;
-  for (uint32_t k = 0; k < P.nsblocks; k++) {
-    const uint32_t c = a.bsum[P.sblock0 + k];
-    a.bsum[P.sblock0 + k] = run;
-    run += c;
+// Exclusive scan of each part's scan-block sums, one 256-thread block per
+// part, 256 sums per step with the running total carried across steps (a
+// serial walk would pay one dependent memory round trip per scan block).
+__global__ __launch_bounds__(256) void k_scan_bsums(FrameArgs a, uint32_t part0) {
+  const uint32_t n = a.parts[part0 + blockIdx.x].nsblocks;
+  if (n == 0) return;
+  uint32_t* __restrict__ bs = a.bsum + a.parts[part0 + blockIdx.x].sblock0;
+  __shared__ uint32_t s_tmp[8];
+  uint32_t carry = 0;
+  for (uint32_t c = 0; c < n; c += 256) {
+    const uint32_t k = c + threadIdx.x;
+    const uint32_t v = k < n ? bs[k] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan_256(v, s_tmp, total);
+    if (k < n) bs[k] = carry + ex;
+    carry += total;
   }
 }
 
@@ -1195,14 +1203,9 @@ __global__ void k_fp_selftest(const float* x, const float* y, float* out, uint32
 // launchers
 // ---------------------------------------------------------------------------
 
-hipError_t launch_init_stats(const FrameArgs& a, uint32_t nparts, hipStream_t s) {
-  if (!nparts) return hipSuccess;
-  hipLaunchKernelGGL(k_init_stats, dim3((nparts + 255) / 256), dim3(256), 0, s, a.stats, nparts);
-  return hipGetLastError();
-}
 hipError_t launch_uniforms(const FrameArgs& a, uint32_t part0, uint32_t nparts, hipStream_t s) {
   if (!nparts) return hipSuccess;
-  hipLaunchKernelGGL(k_uniforms, dim3((nparts + 63) / 64), dim3(64), 0, s, a, part0, nparts);
+  hipLaunchKernelGGL(k_uniforms, dim3(nparts), dim3(64), 0, s, a, part0, nparts);
   return hipGetLastError();
 }
 hipError_t launch_vertex(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t threads, hipStream_t s) {
@@ -1229,7 +1232,7 @@ hipError_t launch_scan(const FrameArgs& a, uint32_t sblk0, uint32_t nsblk, uint3
                        hipStream_t s) {
   if (!nsblk) return hipSuccess;
   hipLaunchKernelGGL(k_scan_reduce, dim3(nsblk), dim3(256), 0, s, a, sblk0);
-  hipLaunchKernelGGL(k_scan_bsums, dim3((nparts + 63) / 64), dim3(64), 0, s, a, part0, nparts);
+  hipLaunchKernelGGL(k_scan_bsums, dim3(nparts), dim3(256), 0, s, a, part0);
   hipLaunchKernelGGL(k_scan_down, dim3(nsblk), dim3(256), 0, s, a, sblk0);
   return hipGetLastError();
 }
