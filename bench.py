@@ -372,6 +372,8 @@ def main():
     ctx.profile(False)
     host_ms = prof.pop("host:prologue", (0.0, 1, 0))[0] / args.profile_steps
     host_uniforms_ms = prof.pop("host:pairs+uniforms", (0.0, 1, 0))[0] / args.profile_steps
+    host_pair_list_ms = prof.pop("host:pair-list", (0.0, 1, 0))[0] / args.profile_steps
+    host_pair_sort_ms = prof.pop("host:pair-sort", (0.0, 1, 0))[0] / args.profile_steps
     kern = {k: {"avg_ms": v[0] / max(v[1], 1), "launches_per_frame": v[1] / args.profile_steps,
                 "algorithmic_bytes_per_launch": v[2] / max(v[1], 1)} for k, v in prof.items()}
     # the roofline is quoted for the longest kernel that moves algorithmic
@@ -432,7 +434,8 @@ def main():
         "pipelined": None if args.sync_only else {"ms_per_step": ms, "frames_in_flight": 2},
         "hbm": {"algorithmic_bytes_per_frame": B, "frame_pct_of_peak": 100.0 * B / (ms / 1e3) / (HBM_PEAK_GBS * 1e9),
                 "device_ms_per_frame": device_ms, "host_prologue_ms": host_ms,
-                "host_pairs_uniforms_ms": host_uniforms_ms,
+                "host_pairs_uniforms_ms": host_uniforms_ms, "host_pair_list_ms": host_pair_list_ms,
+                "host_pair_sort_ms": host_pair_sort_ms,
                 "device_pct_of_peak": 100.0 * B / (device_ms / 1e3) / (HBM_PEAK_GBS * 1e9)},
         "plain": plain,
         "kernels": kern,

@@ -747,7 +747,9 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   f->bytes_cull.clear();
   f->bytes_uv.clear();
 
+  double pair_list_ms = 0, pair_sort_ms = 0;   // frustum and render pairs; their sorts
   for (uint32_t job = 0; job < ndesc; job++) {
+    const auto job0 = std::chrono::steady_clock::now();
     const t3d_render_desc* desc = descs[job];
     if (!desc || !desc->camera) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: null camera");
     const t3d_camera_desc* cd = desc->camera;
@@ -812,6 +814,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
         pairs.push_back(rp);
       }
     }
+    const auto sort0 = std::chrono::steady_clock::now();
     if (!cd->render_depth) {                                           // camera.go:1458-1467
       // with one render order the comparator is depth-descending, a strict
       // weak order; mixed orders need SliceStable's own steps
@@ -852,6 +855,9 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
     draw.reserve(solids.size() + transparents.size());
     for (int32_t i : solids) draw.push_back(i);
     for (int32_t i : transparents) draw.push_back(i);
+    const auto sort1 = std::chrono::steady_clock::now();
+    pair_list_ms += std::chrono::duration<double, std::milli>(sort0 - job0).count();
+    pair_sort_ms += std::chrono::duration<double, std::milli>(sort1 - sort0).count();
 
     // this job's lights: scene lights first, then light-group lights as needed
     const uint32_t lbase_scene = (uint32_t)lights.size();
@@ -1156,6 +1162,12 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
     auto& hu = c->prof["host:pairs+uniforms"];
     hu.ms += std::chrono::duration<double, std::milli>(wall1 - wall0).count();
     hu.launches++;
+    auto& hl = c->prof["host:pair-list"];
+    hl.ms += pair_list_ms;
+    hl.launches++;
+    auto& hs = c->prof["host:pair-sort"];
+    hs.ms += pair_sort_ms;
+    hs.launches++;
   }
   return T3D_OK;
 }

@@ -672,23 +672,28 @@ __global__ __launch_bounds__(256) void k_scan_down(FrameArgs a, uint32_t sblk0) 
 }
 
 // Exclusive scan of a whole [bin][tile] matrix by one 1024-thread block: each
-// thread sums a contiguous segment (16-byte loads), one block-wide scan of the
-// 1024 sums, then each thread rewrites its segment.  One launch and two
-// barriers replace the three launches of the general scan for the (common)
-// parts whose matrix has at most kScanPartMax entries.
+// thread loads its contiguous segment (at most kPer 16-byte words, all loads
+// issued before the first use), one block-wide scan of the 1024 segment sums,
+// then each thread writes its segment back from registers.  One launch, two
+// barriers and one memory round trip each way replace the three launches of
+// the general scan for the (common) parts whose matrix has at most
+// kScanPartMax entries.
 __global__ __launch_bounds__(1024) void k_scan_part(FrameArgs a, uint32_t blk0) {
+  constexpr uint32_t kPer = kScanPartMax / 4 / 1024;
+  static_assert(kScanPartMax % 4096 == 0, "whole uint4 segments");
   const uint32_t b = blk0 + blockIdx.x;
   const uint32_t p = find_part(a.pscan_prefix, a.nparts, b);
-  const uint32_t len = kBins * a.parts[p].ntiles;            // a multiple of 512
+  const uint32_t n4 = kBins * a.parts[p].ntiles / 4;           // the matrix holds a multiple of 512 entries
   uint4* __restrict__ h = reinterpret_cast<uint4*>(a.hist + a.parts[p].hbase);   // 2 KB aligned
   const uint32_t tid = threadIdx.x, lane = tid % 64u, wv = tid / 64u;
-  const uint32_t per = (len / 4 + 1023) / 1024;              // uint4 per thread
-  const uint32_t q0 = min(tid * per, len / 4), q1 = min(q0 + per, len / 4);
+  const uint32_t per = (n4 + 1023) / 1024;                     // <= kPer
+  const uint32_t q0 = min(tid * per, n4), cnt = min(per, n4 - q0);
+  uint4 v[kPer];
+#pragma unroll
+  for (uint32_t j = 0; j < kPer; j++) v[j] = j < cnt ? h[q0 + j] : make_uint4(0, 0, 0, 0);
   uint32_t sum = 0;
-  for (uint32_t q = q0; q < q1; q++) {
-    const uint4 v = h[q];
-    sum += v.x + v.y + v.z + v.w;
-  }
+#pragma unroll
+  for (uint32_t j = 0; j < kPer; j++) sum += v[j].x + v[j].y + v[j].z + v[j].w;
   __shared__ uint32_t s_w[16];
   uint32_t x = sum;
   for (int o = 1; o < 64; o <<= 1) {
@@ -699,15 +704,15 @@ __global__ __launch_bounds__(1024) void k_scan_part(FrameArgs a, uint32_t blk0) 
   __syncthreads();
   uint32_t run = x - sum;
   for (uint32_t w = 0; w < wv; w++) run += s_w[w];
-  for (uint32_t q = q0; q < q1; q++) {
-    const uint4 v = h[q];
+#pragma unroll
+  for (uint32_t j = 0; j < kPer; j++) {
     uint4 o;
     o.x = run;
-    o.y = o.x + v.x;
-    o.z = o.y + v.y;
-    o.w = o.z + v.z;
-    run = o.w + v.w;
-    h[q] = o;
+    o.y = o.x + v[j].x;
+    o.z = o.y + v[j].y;
+    o.w = o.z + v[j].z;
+    run = o.w + v[j].w;
+    if (j < cnt) h[q0 + j] = o;
   }
 }
 
@@ -767,13 +772,14 @@ __global__ __launch_bounds__(64) void k_scatter(FrameArgs a, uint32_t blk0) {
   }
 
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint4 rec_next = lane < n ? rec_in[lane] : make_uint4(0, 0, 0, 0);   // records one chunk ahead
   for (uint32_t base_i = 0; base_i < n; base_i += 64) {
     const uint32_t i = base_i + lane;
     const bool valid = i < n;
     const float d = valid ? s_key[i] : -1.0f;
     const uint64_t vmask = __ballot(valid);
-    uint4 rec = make_uint4(0, 0, 0, 0);
-    if (valid) rec = rec_in[i];                 // issued before the ranking so its latency overlaps it
+    const uint4 rec = rec_next;
+    if (i + 64 < n) rec_next = rec_in[i + 64];
     const uint32_t bin = valid ? visit_bin(d, mn, rd, mode) : 0xFFFFFFFFu;
     const uint64_t peers = bin_peers(bin, vmask);
     const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
@@ -808,7 +814,6 @@ __global__ __launch_bounds__(256) void k_scatter4(FrameArgs a, uint32_t blk0) {
   const uint4* __restrict__ rec_in = a.tri_rec + tbase;
   uint4* __restrict__ rec_out = a.tri_vo + tbase;
 
-  __shared__ uint16_t s_bin[kMaxTile];            // visit bin, or 0xFFFF for a culled triangle
   __shared__ uint32_t s_off[kScatterWaves][kBins];
   __shared__ uint32_t s_tmp[8];
   float mn, rd;
@@ -816,16 +821,27 @@ __global__ __launch_bounds__(256) void k_scatter4(FrameArgs a, uint32_t blk0) {
   for (uint32_t i = tid; i < kScatterWaves * kBins; i += blockDim.x) (&s_off[0][0])[i] = 0;
   const uint32_t nchunks = (n + 63) / 64;
   const uint32_t c0 = (wv * nchunks) / kScatterWaves, c1 = ((wv + 1) * nchunks) / kScatterWaves;
+  // every key and record of this wave's chunks (at most kChunks of 64) is
+  // loaded up front, so the counting and ranking passes wait on memory once
+  constexpr uint32_t kChunks = kMaxTile / 64 / kScatterWaves;
+  float d[kChunks];
+  uint4 rec[kChunks];
+  uint32_t bin[kChunks];
+#pragma unroll
+  for (uint32_t j = 0; j < kChunks; j++) {
+    const uint32_t i = (c0 + j) * 64 + lane;
+    const bool in = c0 + j < c1 && i < n;
+    d[j] = in ? a.key[tbase + t0 + i] : -1.0f;
+    rec[j] = in ? rec_in[t0 + i] : make_uint4(0, 0, 0, 0);
+  }
   __syncthreads();
-  for (uint32_t i = c0 * 64 + lane; i < min(c1 * 64, n); i += 64) {
-    const float d = a.key[tbase + t0 + i];
-    uint16_t bn = 0xFFFFu;
-    if (d >= 0) {
-      const uint32_t k = visit_bin(d, mn, rd, mode);
-      bn = (uint16_t)k;
-      atomicAdd(&s_off[wv][k], 1u);
+#pragma unroll
+  for (uint32_t j = 0; j < kChunks; j++) {
+    bin[j] = 0xFFFFu;                             // no survivor in this lane
+    if (d[j] >= 0) {
+      bin[j] = visit_bin(d[j], mn, rd, mode);
+      atomicAdd(&s_off[wv][bin[j]], 1u);
     }
-    s_bin[i] = bn;
   }
   __syncthreads();
   // per bin: base of each wave (two bins per thread)
@@ -862,23 +878,20 @@ __global__ __launch_bounds__(256) void k_scatter4(FrameArgs a, uint32_t blk0) {
   // each wave ranks its own chunks; only this wave touches s_off[wv]
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint32_t* off = s_off[wv];
-  for (uint32_t c = c0; c < c1; c++) {
-    const uint32_t i = c * 64 + lane;
-    const uint32_t bin = i < n ? (uint32_t)s_bin[i] : 0xFFFFu;
-    const bool valid = bin != 0xFFFFu;
+#pragma unroll
+  for (uint32_t j = 0; j < kChunks; j++) {
+    const bool valid = bin[j] != 0xFFFFu;
     const uint64_t vmask = __ballot(valid);
     if (vmask == 0) continue;                     // uniform across the wave
-    uint4 rec = make_uint4(0, 0, 0, 0);
-    if (valid) rec = rec_in[t0 + i];
-    const uint64_t peers = bin_peers(bin, vmask);
+    const uint64_t peers = bin_peers(bin[j], vmask);
     const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
     uint32_t slot = 0;
-    if (valid) slot = off[bin] + rank;
+    if (valid) slot = off[bin[j]] + rank;
     __builtin_amdgcn_wave_barrier();
-    if (valid && rank == 0) off[bin] = slot + (uint32_t)__popcll(peers);
+    if (valid && rank == 0) off[bin[j]] = slot + (uint32_t)__popcll(peers);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    if (valid) rec_out[slot] = rec;
+    if (valid) rec_out[slot] = rec[j];
   }
 }
 
