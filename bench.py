@@ -247,6 +247,31 @@ def serve_bench(args):
     return 0
 
 
+def hbm_probe(nbytes=1 << 30, reps=20):
+    """Practical HBM ceilings of this part: a 1 GiB fill, zero and copy
+    (torch kernels, timed with events on the current stream), in GB/s of bytes
+    moved (a copy counts its read and its write)."""
+    import torch
+    a = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    b = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    out = {}
+    for name, fn, moved in [("fill", lambda: a.fill_(1), nbytes), ("zero", lambda: a.zero_(), nbytes),
+                            ("copy", lambda: b.copy_(a), 2 * nbytes)]:
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = {"ms": ms, "GBps": moved / (ms / 1e3) / 1e9}
+        log("%s: %.3f ms, %.0f GB/s" % (name, ms, out[name]["GBps"]))
+    print(json.dumps({"hbm_probe_bytes": nbytes, **out}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -266,6 +291,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain device copy of BYTES (counter calibration) and exit")
+    ap.add_argument("--hbm-probe", action="store_true",
+                    help="measure the HBM fill and copy ceilings (1 GiB, torch) instead of the frame benchmark")
     ap.add_argument("--serve", action="store_true", help="run the serving loop instead of the frame benchmark")
     ap.add_argument("--pool", type=int, default=10000)
     ap.add_argument("--jobs", type=int, default=2000)
@@ -290,6 +317,9 @@ def main():
 
     if args.serve:
         return serve_bench(args)
+
+    if args.hbm_probe:
+        return hbm_probe()
 
     if args.copy_calibration:
         with Context(0) as ctx:
