@@ -346,7 +346,7 @@ def main():
         log("tuning set to %s" % tuning)
     elif not args.no_autotune:
         t = time.perf_counter()
-        tu = ctx.autotune(inp, reps=3)
+        tu = ctx.autotune(inp, reps=5)
         tuning = {f: getattr(tu, f) for f, _ in A.Tuning._fields_}
         log("autotune picked %s in %.1fs" % (tuning, time.perf_counter() - t))
 

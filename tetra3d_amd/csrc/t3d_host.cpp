@@ -1057,7 +1057,8 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   const uint32_t ntiles_total = f->tile_prefix.back();
   const uint64_t sbytes = carve_size<float4>(2 * sumV) + carve_size<float4>(sumV) + carve_size<uint32_t>(sumT) +
                           2 * carve_size<uint4>(sumT) + carve_size<uint32_t>(4ull * P_n) + carve_size<uint32_t>(sumHist) +
-                          carve_size<uint32_t>(sblocks) + carve_size<uint32_t>(ntiles_total) + carve_size<float4>(128) +
+                          carve_size<uint32_t>(sblocks) + 2 * carve_size<uint32_t>(ntiles_total) +
+                          carve_size<uint32_t>(f->vchunk_prefix.back()) + carve_size<float4>(128) +
                           carve_size<PartLight>(n_plights);
   st = ensure(c, f->scratch, f->scratch_cap, sbytes);
   if (st) return st;
@@ -1142,6 +1143,8 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   a.hist = sc.take<uint32_t>(sumHist);
   a.bsum = sc.take<uint32_t>(sblocks);
   a.tile_count = sc.take<uint32_t>(ntiles_total);
+  a.tile_part = sc.take<uint32_t>(ntiles_total);
+  a.vchunk_part = sc.take<uint32_t>(f->vchunk_prefix.back());
   a.sink = sc.take<float4>(128);
   a.plights = sc.take<PartLight>(n_plights);
   a.out_color = reinterpret_cast<float4*>(oc.take<t3d_vertex>(3 * sumT));
@@ -1274,9 +1277,10 @@ t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uin
 }
 
 uint64_t estimate_job_bytes(t3d_ctx* c, const t3d_render_desc* desc) {
-  // scratch: clip + colour + skinned position (48 B/V); key + two records (36 B/T); per tile a 512-bin histogram column,
-  // a scan block sum share and a survivor count (~2 KB); outputs: 3 x (48 + 48 + 4) = 300 B/T;
-  // tables: ~1 KB per model plus palettes
+  // scratch: clip + colour + skinned position (48 B/V) and a block->pair entry per vertex block (at least 64
+  // vertices); key + two records (36 B/T); per tile a 512-bin histogram column, a scan block sum share, a survivor
+  // count and a tile->pair entry (~2 KB); outputs: 3 x (48 + 48 + 4) = 300 B/T; tables: ~1 KB per model plus
+  // palettes
   const uint64_t tile = std::max<uint32_t>(64, c->tuning.triangle_tile);
   uint64_t V = 0, T = 0, tiles = 0, extra = 64 * 1024;
   for (uint32_t mi = 0; mi < desc->model_count; mi++) {
@@ -1291,7 +1295,7 @@ uint64_t estimate_job_bytes(t3d_ctx* c, const t3d_render_desc* desc) {
     }
     extra += 1024 + 64ull * (md.skinned ? md.bone_count : 0);
   }
-  return 48 * V + 336 * T + (4ull * kBins + 16) * tiles + extra;
+  return 48 * V + 4 * (V / 64 + 1) + 336 * T + (4ull * kBins + 24) * tiles + extra;
 }
 
 void frame_release(t3d_ctx* c, t3d_frame* f) {
@@ -1514,7 +1518,7 @@ t3d_status t3d_autotune(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc, u
         double ms = 0;
         st = time_of(t, ms);
         if (st) break;
-        if (ms < best_ms) {
+        if (ms < best_ms * 0.99) {   // 1 % better, so timing noise does not pick between equals
           best_ms = ms;
           best = t;
         }

@@ -142,6 +142,8 @@ struct FrameArgs {
   uint4* tri_vo;                // [sumT] the same records in output-slot order
   uint32_t* stats;              // [P][4]: min bits, max bits, survivors, 0
   uint32_t* tile_count;         // [tiles] survivors of each tile (compacted at the tile's front)
+  uint32_t* tile_part;          // [tiles] render pair of each tile (written by k_uniforms)
+  uint32_t* vchunk_part;        // [vertex chunks] render pair of each vertex-kernel block (k_uniforms)
   float4* sink;                 // [128] store target of lanes past the end of a chunk (never read)
   uint32_t* hist;
   uint32_t* bsum;

@@ -170,7 +170,7 @@ __device__ __forceinline__ void light_vertex(const LightGPU& l, const PartLight&
 
 __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
   const uint32_t b = blk0 + block_index(a.xcd_swizzle);
-  const uint32_t p = find_part(a.vchunk_prefix, a.nparts, b);
+  const uint32_t p = a.vchunk_part[b];
   const PartGPU P = a.parts[p];        // by value: read once, before any store
   const uint32_t flags = P.flags;
 
@@ -318,6 +318,11 @@ __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
 __global__ __launch_bounds__(64) void k_uniforms(FrameArgs a, uint32_t part0, uint32_t nparts) {
   PartGPU& P = const_cast<PartGPU*>(a.parts)[part0 + blockIdx.x];
   const uint32_t k = threadIdx.x;
+  {  // block -> pair maps of the later kernels: one load where a search over the prefixes took nine
+    const uint32_t p = part0 + blockIdx.x;
+    for (uint32_t i = a.tile_prefix[p] + k; i < a.tile_prefix[p + 1]; i += 64) a.tile_part[i] = p;
+    for (uint32_t i = a.vchunk_prefix[p] + k; i < a.vchunk_prefix[p + 1]; i += 64) a.vchunk_part[i] = p;
+  }
   const uint32_t flags = P.flags;
   const uint32_t nl = (flags & PF_LIGHTING) ? P.light_count : 0u;
   if (k > 0 && k >= nl) return;
@@ -389,7 +394,7 @@ template <uint32_t kThreads>
 __global__ __launch_bounds__(kThreads) void k_cull(FrameArgs a, uint32_t blk0) {
   constexpr uint32_t kCullWaves = kThreads / 64;
   const uint32_t b = blk0 + block_index(a.xcd_swizzle);
-  const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
+  const uint32_t p = a.tile_part[b];
   const PartGPU P = a.parts[p];        // by value: read once, before any store
   const uint32_t flags = P.flags;
   const FrameConst fc = a.fc[P.cam];
@@ -560,7 +565,7 @@ __device__ __forceinline__ void part_range(const uint32_t* stats, uint32_t p, fl
 
 __global__ __launch_bounds__(64) void k_hist(FrameArgs a, uint32_t blk0) {
   const uint32_t b = blk0 + block_index(a.xcd_swizzle);
-  const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
+  const uint32_t p = a.tile_part[b];
   const PartGPU P = a.parts[p];        // by value: read once, before any store
   if (P.ntiles <= 1) return;
   __shared__ uint32_t h[kBins];
@@ -726,7 +731,7 @@ __global__ __launch_bounds__(1024) void k_scan_part(FrameArgs a, uint32_t blk0) 
 // The survivor's record {vertex-record offsets, triangle} moves to its slot.
 __global__ __launch_bounds__(64) void k_scatter(FrameArgs a, uint32_t blk0) {
   const uint32_t b = blk0 + block_index(a.xcd_swizzle);
-  const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
+  const uint32_t p = a.tile_part[b];
   const PartGPU P = a.parts[p];        // by value: read once, before any store
   if (a.stats[4 * p + 2] == 0) return;
   const uint32_t tl = b - a.tile_prefix[p];
@@ -802,7 +807,7 @@ __global__ __launch_bounds__(64) void k_scatter(FrameArgs a, uint32_t blk0) {
 constexpr uint32_t kScatterWaves = 4;
 __global__ __launch_bounds__(256) void k_scatter4(FrameArgs a, uint32_t blk0) {
   const uint32_t b = blk0 + block_index(a.xcd_swizzle);
-  const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
+  const uint32_t p = a.tile_part[b];
   const PartGPU P = a.parts[p];        // by value: read once, before any store
   if (a.stats[4 * p + 2] == 0) return;
   const uint32_t tl = b - a.tile_prefix[p];
@@ -924,7 +929,7 @@ template <int kStage, bool kNT>
 __global__ __launch_bounds__(64) void k_assemble(FrameArgs a, uint32_t blk0, uint32_t split) {
   const uint32_t li = block_index(a.xcd_swizzle);
   const uint32_t b = blk0 + li / split;
-  const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
+  const uint32_t p = a.tile_part[b];
   const PartGPU P = a.parts[p];        // by value: read once, before any store
   const uint32_t S = a.stats[4 * p + 2];
   const uint32_t sub = a.tile / split;
@@ -1061,7 +1066,7 @@ template <uint32_t kRows, bool kNT>
 __global__ __launch_bounds__(64) void k_assemble_pipe(FrameArgs a, uint32_t blk0, uint32_t split) {
   const uint32_t li = block_index(a.xcd_swizzle);
   const uint32_t b = blk0 + li / split;
-  const uint32_t p = find_part(a.tile_prefix, a.nparts, b);
+  const uint32_t p = a.tile_part[b];
   const PartGPU P = a.parts[p];
   const uint32_t S = a.stats[4 * p + 2];
   const uint32_t sub = a.tile / split;
