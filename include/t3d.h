@@ -421,6 +421,16 @@ t3d_status t3d_server_stats_reset(t3d_server* s);
  * 0 turns it off.  No counterpart in the reference: a debugging aid. */
 t3d_status t3d_debug_poison_outputs(t3d_ctx* ctx, uint32_t word);
 
+/* Bounding spheres of n models as the frustum test computes them
+ * (Model.onTransformUpdate, model.go:142-182): out_batched from the
+ * four-models-at-a-time path t3d_render uses, out_scalar from the one-model
+ * path, 4 floats per model (centre x, y, z, radius).  dims holds 6 floats per
+ * model (mesh dimension min xyz, max xyz), skinned one byte per model.  Needs
+ * no context or device.  Debugging aid with no reference counterpart: lets a
+ * host-only test check that the two paths agree bit for bit. */
+t3d_status t3d_debug_model_spheres(const t3d_mat4* transforms, const float* dims, const uint8_t* skinned, uint32_t n,
+                                   float* out_batched, float* out_scalar);
+
 /* Runs the device's float32 primitives on n inputs: out[0..n) = x/y,
  * out[n..2n) = sqrt(|x|), out[2n..3n) = x*y + x (unfused), out[3n..4n) =
  * round-half-away(x).  Lets a test confirm the device rounds exactly as the

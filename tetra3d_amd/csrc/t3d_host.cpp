@@ -310,6 +310,66 @@ void model_sphere(const M4& transform, bool skinned, const float* dmin, const fl
   radius = magnitude(sub(mx, mn)) / 2;
 }
 
+// model_sphere for four models at once: the same float32 operations in the
+// same order, lane by lane (SSE divisions and square roots are correctly
+// rounded, as the scalar code's are), so each lane's sphere is bit-identical
+// to model_sphere's.  The frustum pass of a frame with thousands of models
+// spends most of its time here.
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef int32_t i4 __attribute__((ext_vector_type(4)));
+struct Sphere4In {
+  const M4* t[4];
+  const float* dmin[4];
+  const float* dmax[4];
+  bool skinned[4];
+};
+static inline f4 sel4(i4 m, f4 a, f4 b) {   // m ? a : b, lane by lane
+  const i4 r = (__builtin_bit_cast(i4, a) & m) | (__builtin_bit_cast(i4, b) & ~m);
+  return __builtin_bit_cast(f4, r);
+}
+static inline f4 sqrt4(f4 x) { return __builtin_elementwise_sqrt(x); }
+void model_spheres4(const Sphere4In& in, V3* pos, float* radius) {
+  f4 a[4][4];
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) a[i][j] = f4{in.t[0]->m[i][j], in.t[1]->m[i][j], in.t[2]->m[i][j], in.t[3]->m[i][j]};
+  // Decompose (matrix.go:181-196): rot rows 0-2 = Vector4.Unit of a's rows (w kept), row 3 = (0,0,0,1)
+  f4 rot[4][4];
+  for (int i = 0; i < 3; i++) {
+    const f4 l = sqrt4(a[i][0] * a[i][0] + a[i][1] * a[i][1] + a[i][2] * a[i][2]);
+    const i4 keep = (l < 1e-8f) | (l == 1.0f);
+    for (int k = 0; k < 3; k++) rot[i][k] = sel4(keep, a[i][k], a[i][k] / l);
+    rot[i][3] = a[i][3];
+  }
+  for (int k = 0; k < 4; k++) rot[3][k] = f4(k == 3 ? 1.0f : 0.0f);
+  // scale_i = |xyz of row i of a * transpose(rot)|
+  f4 sc[3];
+  for (int i = 0; i < 3; i++) {
+    f4 r[3];
+    for (int j = 0; j < 3; j++) r[j] = a[i][0] * rot[j][0] + a[i][1] * rot[j][1] + a[i][2] * rot[j][2] + a[i][3] * rot[j][3];
+    sc[i] = sqrt4(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+  }
+  f4 mn[3], mx[3], c[3];
+  for (int k = 0; k < 3; k++) {
+    mn[k] = f4{in.dmin[0][k], in.dmin[1][k], in.dmin[2][k], in.dmin[3][k]};
+    mx[k] = f4{in.dmax[0][k], in.dmax[1][k], in.dmax[2][k], in.dmax[3][k]};
+    const f4 mid = (mx[k] + mn[k]) / 2.0f;
+    c[k] = f4{in.skinned[0] ? 0.0f : mid[0], in.skinned[1] ? 0.0f : mid[1], in.skinned[2] ? 0.0f : mid[2],
+              in.skinned[3] ? 0.0f : mid[3]};
+  }
+  // centre = MultVec(rot, c) (translation row of rot is 0), then the Model.onTransformUpdate sphere
+  f4 p[3], d[3];
+  for (int k = 0; k < 3; k++) {
+    const f4 ck = rot[0][k] * c[0] + rot[1][k] * c[1] + rot[2][k] * c[2] + rot[3][k];
+    p[k] = a[3][k] + ck * sc[k];
+    d[k] = mx[k] * sc[k] - mn[k] * sc[k];
+  }
+  const f4 r = sqrt4(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) / 2.0f;
+  for (int l = 0; l < 4; l++) {
+    pos[l] = v3(p[0][l], p[1][l], p[2][l]);
+    radius[l] = r[l];
+  }
+}
+
 bool is_transparent(const t3d_model_desc& m, const t3d_material_desc* mat) {   // model.go:1274-1282
   if (mat) {
     bool matT = mat->transparency_mode == T3D_TRANSPARENCY_TRANSPARENT || !mat->blend_source_over ||
@@ -748,6 +808,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   f->bytes_uv.clear();
 
   double pair_list_ms = 0, pair_sort_ms = 0;   // frustum and render pairs; their sorts
+  std::vector<uint8_t> in_frustum;
   for (uint32_t job = 0; job < ndesc; job++) {
     const auto job0 = std::chrono::steady_clock::now();
     const t3d_render_desc* desc = descs[job];
@@ -782,16 +843,64 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
       if ((!world || world->lighting_on) && l.visible) scene_lights.push_back(SceneLight{&l, {}});
     }
 
+    // consecutive models usually share a mesh: one map lookup per change
+    uint64_t last_mesh_id = 0;
+    const Mesh* last_mesh = nullptr;
+    auto find_mesh = [&](uint64_t id) -> const Mesh* {
+      if (id != last_mesh_id || !last_mesh) {
+        auto it = c->meshes.find(id);
+        last_mesh = it == c->meshes.end() ? nullptr : it->second.get();
+        last_mesh_id = id;
+      }
+      return last_mesh;
+    };
+
+    // frustum test of every culling model, spheres four at a time (model.go:142-182, camera.go:933-983)
+    in_frustum.assign(desc->model_count, 1);
+    {
+      Sphere4In q{};
+      uint32_t qi[4];
+      int nq = 0;
+      auto flush = [&]() {
+        for (int l = nq; l < 4; l++) {
+          q.t[l] = q.t[0];
+          q.dmin[l] = q.dmin[0];
+          q.dmax[l] = q.dmax[0];
+          q.skinned[l] = q.skinned[0];
+        }
+        V3 sp[4];
+        float sr[4];
+        model_spheres4(q, sp, sr);
+        for (int l = 0; l < nq; l++) in_frustum[qi[l]] = sphere_in_frustum(cam, sp[l], sr[l]) ? 1 : 0;
+        nq = 0;
+      };
+      for (uint32_t mi = 0; mi < desc->model_count; mi++) {
+        const t3d_model_desc& md = desc->models[mi];
+        if (!md.visible || md.mesh == 0 || !md.frustum_culling) continue;
+        const Mesh* m = find_mesh(md.mesh);
+        if (!m) continue;   // reported below, in model order
+        q.t[nq] = &md.transform;
+        q.dmin[nq] = m->dmin;
+        q.dmax[nq] = m->dmax;
+        q.skinned[nq] = md.skinned != 0;
+        qi[nq] = mi;
+        if (++nq == 4) flush();
+      }
+      if (nq) flush();
+    }
+
     // render pairs (camera.go:1343-1457)
     std::vector<Pair> pairs;
     std::vector<int32_t> solids, transparents;
+    pairs.reserve(desc->model_count);
+    solids.reserve(desc->model_count);
     for (uint32_t mi = 0; mi < desc->model_count; mi++) {
       const t3d_model_desc& md = desc->models[mi];
       if (!md.visible || md.mesh == 0) continue;
-      auto it = c->meshes.find(md.mesh);
-      if (it == c->meshes.end())
+      const Mesh* mp = find_mesh(md.mesh);
+      if (!mp)
         return fail(c, T3D_ERR_NOT_FOUND, "render: model %u references unknown mesh %llu", mi, (unsigned long long)md.mesh);
-      const Mesh& mesh = *it->second;
+      const Mesh& mesh = *mp;
       if (md.has_vertex_transform_function || md.has_vertex_clip_function)
         return fail(c, T3D_ERR_UNSUPPORTED, "render: model %u uses a Go vertex callback", mi);
       if (md.skinned) {
@@ -800,12 +909,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
       }
       if (md.light_group_active && md.light_group_count && !md.light_group)
         return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u light group missing", mi);
-      if (md.frustum_culling) {
-        V3 sp;
-        float sr;
-        model_sphere(md.transform, md.skinned != 0, mesh.dmin, mesh.dmax, sp, sr);
-        if (!sphere_in_frustum(cam, sp, sr)) continue;
-      }
+      if (md.frustum_culling && !in_frustum[mi]) continue;
       for (uint32_t p = 0; p < mesh.parts.size(); p++) {
         const t3d_material_desc* mat = part_material(md, p);
         if (mat && !mat->visible) continue;
@@ -1401,6 +1505,42 @@ t3d_status t3d_frame_download_triangles(t3d_ctx* c, const t3d_frame* f, uint32_t
   if (f->inflight) return fail(c, T3D_ERR_NOT_READY, "frame_download: render in flight (t3d_frame_wait first)");
   (void)hipSetDevice(c->device);
   return frame_download_range(c, f, 0, (uint32_t)f->pairs.size(), c->stream, nullptr, nullptr, nullptr, ids);
+}
+
+t3d_status t3d_debug_model_spheres(const t3d_mat4* transforms, const float* dims, const uint8_t* skinned, uint32_t n,
+                                   float* out_batched, float* out_scalar) {
+  if (!transforms || !dims || !skinned || !out_batched || !out_scalar) return T3D_ERR_INVALID_ARGUMENT;
+  for (uint32_t i = 0; i < n; i += 4) {
+    Sphere4In q{};
+    for (uint32_t l = 0; l < 4; l++) {
+      const uint32_t k = std::min(i + l, n - 1);
+      q.t[l] = &transforms[k];
+      q.dmin[l] = dims + 6ull * k;
+      q.dmax[l] = dims + 6ull * k + 3;
+      q.skinned[l] = skinned[k] != 0;
+    }
+    V3 sp[4];
+    float sr[4];
+    model_spheres4(q, sp, sr);
+    for (uint32_t l = 0; l < 4 && i + l < n; l++) {
+      float* o = out_batched + 4ull * (i + l);
+      o[0] = sp[l].x;
+      o[1] = sp[l].y;
+      o[2] = sp[l].z;
+      o[3] = sr[l];
+    }
+  }
+  for (uint32_t k = 0; k < n; k++) {
+    V3 sp;
+    float sr;
+    model_sphere(transforms[k], skinned[k] != 0, dims + 6ull * k, dims + 6ull * k + 3, sp, sr);
+    float* o = out_scalar + 4ull * k;
+    o[0] = sp.x;
+    o[1] = sp.y;
+    o[2] = sp.z;
+    o[3] = sr;
+  }
+  return T3D_OK;
 }
 
 t3d_status t3d_debug_poison_outputs(t3d_ctx* c, uint32_t word) {

@@ -75,11 +75,13 @@ def lib():
     L.t3d_autotune.argtypes = [vp, vp, P(A.RenderDesc), C.c_uint32, P(A.Tuning)]
     L.t3d_selftest_fp.argtypes = [vp, P(C.c_float), P(C.c_float), P(C.c_float), C.c_uint32]
     L.t3d_selftest_copy.argtypes = [vp, C.c_uint64, C.c_uint32, P(C.c_double)]
+    L.t3d_debug_model_spheres.argtypes = [P(A.Mat4), P(C.c_float), P(C.c_uint8), C.c_uint32, P(C.c_float),
+                                          P(C.c_float)]
     for name in ("t3d_ctx_create", "t3d_ctx_destroy", "t3d_mesh_upload", "t3d_mesh_free", "t3d_volume_upload",
                  "t3d_volume_free", "t3d_frame_create", "t3d_frame_destroy", "t3d_render", "t3d_frame_pairs",
                  "t3d_frame_device_outputs", "t3d_frame_download", "t3d_frame_download_triangles",
                  "t3d_profile_enable", "t3d_profile_read", "t3d_get_tuning", "t3d_set_tuning", "t3d_autotune",
-                 "t3d_selftest_fp", "t3d_selftest_copy"):
+                 "t3d_selftest_fp", "t3d_selftest_copy", "t3d_debug_model_spheres"):
         getattr(L, name).restype = C.c_int32
     _lib = L
     return L
@@ -305,3 +307,23 @@ class Context:
         self._check(self.L.t3d_selftest_fp(self.ctx, x.ctypes.data_as(FP), y.ctypes.data_as(FP),
                                            out.ctypes.data_as(FP), n))
         return out.reshape(4, n)
+
+
+def model_spheres(transforms, dims, skinned):
+    """Bounding spheres (centre xyz, radius) of models by the render's
+    four-at-a-time path and by the one-model path (t3d_debug_model_spheres);
+    host only, no device needed."""
+    L = lib()
+    t = np.ascontiguousarray(transforms, dtype=np.float32).reshape(-1, 4, 4)
+    n = len(t)
+    d = np.ascontiguousarray(dims, dtype=np.float32).reshape(n, 6)
+    sk = np.ascontiguousarray(skinned, dtype=np.uint8).reshape(n)
+    ob = np.zeros((n, 4), dtype=np.float32)
+    os_ = np.zeros((n, 4), dtype=np.float32)
+    FP = C.POINTER(C.c_float)
+    st = L.t3d_debug_model_spheres(t.ctypes.data_as(C.POINTER(A.Mat4)), d.ctypes.data_as(FP),
+                                   sk.ctypes.data_as(C.POINTER(C.c_uint8)), n, ob.ctypes.data_as(FP),
+                                   os_.ctypes.data_as(FP))
+    if st != 0:
+        raise RuntimeError("t3d_debug_model_spheres failed: %d" % st)
+    return ob, os_
