@@ -797,6 +797,10 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   std::vector<uint32_t> palette_count;
   std::vector<const float*> skw_host;
   std::vector<uint32_t> skw_count;
+  palettes_host.reserve(f->parts.capacity());
+  palette_count.reserve(f->parts.capacity());
+  skw_host.reserve(f->parts.capacity());
+  skw_count.reserve(f->parts.capacity());
   uint64_t sumV = 0, sumT = 0, sumHist = 0;
   uint32_t sblocks = 0;
   f->vchunk_prefix.assign(1, 0);
@@ -1011,7 +1015,8 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
 
       // MVP, invertedCamPos, the directional working rotation and the
       // per-light values are computed on the device (k_uniforms)
-      PartGPU P{};
+      f->parts.emplace_back();   // zeroed in place: no copy of the ~450-byte record per pair
+      PartGPU& P = f->parts.back();
       P.model = md.transform;
       P.cam = job;
       P.max_span = mesh.max_span;
@@ -1123,7 +1128,6 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
       f->bytes_vertex.push_back((uint64_t)P.vcount * vb + (md.skinned ? 64ull * md.bone_count : 0) + sizeof(PartGPU));
       f->bytes_cull.push_back(12ull * P.tcount);
       f->bytes_uv.push_back(mesh.uv ? 8ull * P.vcount : 0);   // UVs are read by the assembly
-      f->parts.push_back(P);
       f->part_first_tri.push_back(pr.triangle_start);
       t3d_pair_result r{};
       r.model_index = rp.model;
