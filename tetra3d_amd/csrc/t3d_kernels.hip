@@ -304,17 +304,16 @@ __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
 // k_cull
 // ---------------------------------------------------------------------------
 
-// Per-pair uniforms, one thread per render pair: MVP (model.go:695),
-// invertedCamPos (model.go:629-630), the directional working rotation
-// (light.go:326-330) and, per light of the pair, beginModel's working
-// position (light.go:175-189), the per-model point-light skip
-// (camera.go:1592, 1625-1629) and the per-object LightVolume cell
-// (light.go:610-628).  The same float32 functions the host would run, so the
-// values are the ones the reference computes on the CPU.
-// One wavefront per render pair; lane k handles the pair's light k (at most
-// kMaxLights = 64), lane 0 also the pair's own uniforms and the reset of its
-// depth range and survivor count, so the latency is one light's, not the sum
-// over lights.
+// Per-pair uniforms: MVP (model.go:695), invertedCamPos (model.go:629-630),
+// the directional working rotation (light.go:326-330) and, per light of the
+// pair, beginModel's working position (light.go:175-189), the per-model
+// point-light skip (camera.go:1592, 1625-1629) and the per-object LightVolume
+// cell (light.go:610-628).  The same float32 functions the host would run, so
+// the values are the ones the reference computes on the CPU.  One wavefront
+// per render pair: lane k handles the pair's light k (at most kMaxLights =
+// 64), lane 0 also the pair's own uniforms and the reset of its depth range
+// and survivor count, so the latency is one light's, not the sum over lights;
+// all lanes first write the pair's entries of the block->pair maps.
 __global__ __launch_bounds__(64) void k_uniforms(FrameArgs a, uint32_t part0, uint32_t nparts) {
   PartGPU& P = const_cast<PartGPU*>(a.parts)[part0 + blockIdx.x];
   const uint32_t k = threadIdx.x;
