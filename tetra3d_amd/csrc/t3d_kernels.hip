@@ -176,7 +176,11 @@ __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
 
   __shared__ LightGPU s_light[kMaxLights];
   __shared__ PartLight s_pl[kMaxLights];
-  __shared__ M4 s_bones[kMaxLdsBones];
+  // bone palette rows in LDS, five float4 per bone (four rows and one of
+  // padding): lanes blending different bones then start on different LDS
+  // banks instead of two bank groups (a 64-byte stride covers 16 of 32 banks)
+  constexpr uint32_t kBoneRows = 5;
+  __shared__ float4 s_bone_rows[kMaxLdsBones * kBoneRows];
 
   const uint32_t nl = (flags & PF_LIGHTING) ? P.light_count : 0u;
   for (uint32_t i = threadIdx.x; i < nl; i += blockDim.x) {
@@ -186,9 +190,8 @@ __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
   const bool skinned = (flags & PF_SKINNED) != 0;
   const bool lds_bones = skinned && P.bone_count <= kMaxLdsBones;
   if (lds_bones) {
-    const float* src = P.palette;
-    float* dst = &s_bones[0].m[0][0];
-    for (uint32_t i = threadIdx.x; i < 16u * P.bone_count; i += blockDim.x) dst[i] = src[i];
+    const float4* src = reinterpret_cast<const float4*>(P.palette);
+    for (uint32_t i = threadIdx.x; i < 4u * P.bone_count; i += blockDim.x) s_bone_rows[(i / 4) * kBoneRows + i % 4] = src[i];
   }
   __syncthreads();
 
@@ -228,13 +231,25 @@ __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
       for (int k = 0; k < 4; k++) {
         const float w = bwt[k];
         if (w == 0) continue;
-        const M4& B = lds_bones ? s_bones[bidx[k]] : reinterpret_cast<const M4*>(P.palette)[bidx[k]];
+        float4 B[4];
+        for (int r = 0; r < 4; r++)
+          B[r] = lds_bones ? s_bone_rows[bidx[k] * kBoneRows + r]
+                           : reinterpret_cast<const float4*>(P.palette)[(size_t)bidx[k] * 4 + r];
         if (w == 1) {
-          S = B;
+          for (int r = 0; r < 4; r++) {
+            S.m[r][0] = B[r].x;
+            S.m[r][1] = B[r].y;
+            S.m[r][2] = B[r].z;
+            S.m[r][3] = B[r].w;
+          }
           break;
         }
-        for (int r = 0; r < 4; r++)
-          for (int c = 0; c < 4; c++) S.m[r][c] += B.m[r][c] * w;
+        for (int r = 0; r < 4; r++) {
+          S.m[r][0] += B[r].x * w;
+          S.m[r][1] += B[r].y * w;
+          S.m[r][2] += B[r].z * w;
+          S.m[r][3] += B[r].w * w;
+        }
       }
       vp = mult_vec(S, pos);
       S.m[3][0] = 0;
