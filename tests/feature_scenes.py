@@ -198,3 +198,24 @@ def pair_order_scene(seed, n_models=24):
     world = World()
     world.ambient_light.energy = f32(0.3)
     return W.Workload("pair_order_%d" % seed, cam, Scene("pair_order", world), lights, models)
+
+
+def many_bones_scene(seed):
+    """Skinned models whose palettes straddle the LDS staging limit of the
+    vertex pass (64 bones): 63, 64, 65 and 96 bones, so both the staged and
+    the global-memory palette reads are exercised."""
+    rng = np.random.default_rng(seed)
+    cam = Camera("cam", 640, 360)
+    lights = [DirectionalLight("sun", 1, 1, 1, 1.0)]
+    lights[0].set_local_rotation(W.axis_angle_matrix((1, 0, 0), -0.6))
+    models = []
+    for i, bones in enumerate([63, 64, 65, 96]):
+        mesh = W.icosphere(2, 1.0, name="b%d" % i)
+        m = Model("skinned%d" % i, mesh)
+        m.set_local_position(-4.5 + 3 * i, 0, -8)
+        m.set_local_rotation(_rotation(rng))
+        _skin(rng, m, bones=bones)
+        models.append(m)
+    world = World()
+    world.ambient_light.energy = f32(0.2)
+    return W.Workload("many_bones_%d" % seed, cam, Scene("many_bones", world), lights, models)

@@ -3,7 +3,7 @@ options (see tests/feature_scenes.py), in both execution forms."""
 import pytest
 
 from tetra3d_amd import _abi as A
-from tests.feature_scenes import feature_scene, pair_order_scene
+from tests.feature_scenes import feature_scene, many_bones_scene, pair_order_scene
 from tests.parity import compare_frames
 
 SEEDS = list(range(40))
@@ -31,3 +31,15 @@ def test_pair_order_ties(gpu_ctx, oracle, mode):
         wl = pair_order_scene(seed)
         print("pair-order scene", seed, "mode", mode, flush=True)
         compare_frames(gpu_ctx.render_workload(wl, exec_mode=mode), oracle.render_workload(wl))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [A.EXEC_BATCHED, A.EXEC_PLAIN], ids=["batched", "plain"])
+def test_many_bone_palettes(gpu_ctx, oracle, mode):
+    """Palettes of 63-96 bones: staged in LDS up to 64 bones, read from
+    device memory beyond (model.go:559-596 either way)."""
+    for seed in range(2):
+        wl = many_bones_scene(seed)
+        print("many-bones scene", seed, "mode", mode, flush=True)
+        s = compare_frames(gpu_ctx.render_workload(wl, exec_mode=mode), oracle.render_workload(wl))
+        assert s["survivors"] > 0
