@@ -63,3 +63,33 @@ def test_size_boundaries(gpu_ctx, oracle, case, mode):
         gpu_ctx.L.t3d_set_tuning(gpu_ctx.ctx, saved)
     s = compare_frames(g, oracle.render_workload(wl))
     assert s["survivors"] > tris // 4
+
+
+def _empty_cases():
+    """(name, workload): no models; every model outside the frustum; a pair
+    whose every triangle is back-facing next to a visible one."""
+    base = _scene(512)
+    yield "no_models", W.Workload("no_models", base.camera, base.scene, base.lights, [])
+    away = _scene(512)
+    away.camera.set_local_rotation(W.axis_angle_matrix((1, 0, 0), math.pi / 2))   # looking up, away from the grid
+    yield "all_frustum_culled", away
+    vis = _scene(512)
+    flipped = _grid(18)
+    back = Mesh("back", flipped.positions, flipped.normals, flipped.indices[:, ::-1].copy(), colors=flipped.colors)
+    bm = Model("backfacing", back)
+    bm.set_local_position(0, 0.5, 0)
+    yield "backfacing_pair", W.Workload("backfacing_pair", vis.camera, vis.scene, vis.lights, vis.models + [bm])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [A.EXEC_BATCHED, A.EXEC_PLAIN], ids=["batched", "plain"])
+def test_empty_and_fully_culled(gpu_ctx, oracle, mode):
+    for name, wl in _empty_cases():
+        print("empty case", name, "mode", mode, flush=True)
+        g = gpu_ctx.render_workload(wl, exec_mode=mode)
+        o = oracle.render_workload(wl)
+        compare_frames(g, o)
+        if name == "backfacing_pair":
+            assert [p[3] for p in o.pairs][-1] == 0 and o.pairs[0][3] > 0
+        else:
+            assert g.survivors == 0
