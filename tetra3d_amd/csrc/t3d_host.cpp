@@ -576,7 +576,8 @@ t3d_status t3d_mesh_upload(t3d_ctx* c, const t3d_mesh_desc* d, uint64_t* out_han
   }
   uint64_t bytes = carve_size<float>(3 * V) * 2 + (d->colors ? carve_size<float>(4 * V) : 0) +
                    (d->uvs ? carve_size<float>(2 * V) : 0) + (d->bone_indices ? carve_size<uint16_t>(4 * V) + carve_size<float>(4 * V) : 0) +
-                   (K ? carve_size<float>(3 * K * V) * 2 : 0) + carve_size<uint32_t>(3 * T);
+                   (K ? carve_size<float>(3 * K * V) * 2 : 0) +
+                   (V <= 65536 ? carve_size<uint16_t>(3 * T) : carve_size<uint32_t>(3 * T));
   (void)hipSetDevice(c->device);
   t3d_status st = dev_alloc(c, m->buf, bytes);
   if (st) return st;
@@ -600,7 +601,16 @@ t3d_status t3d_mesh_upload(t3d_ctx* c, const t3d_mesh_desc* d, uint64_t* out_han
     m->sk_pos = put(d->shape_key_positions, 3 * K * V, (float*)nullptr);
     m->sk_nrm = put(d->shape_key_normals, 3 * K * V, (float*)nullptr);
   }
-  m->idx = put(d->indices, 3 * T, (uint32_t*)nullptr);
+  // meshes of at most 65,536 vertices keep 16-bit indices: half the index
+  // bytes the cull pass streams
+  std::vector<uint16_t> idx16;
+  if (V <= 65536) {
+    idx16.resize(3 * T);
+    for (uint64_t i = 0; i < 3 * T; i++) idx16[i] = (uint16_t)d->indices[i];
+    m->idx16 = put(idx16.data(), 3 * T, (uint16_t*)nullptr);
+  } else {
+    m->idx = put(d->indices, 3 * T, (uint32_t*)nullptr);
+  }
   for (auto& cp : copies) {
     if (!cp.second.second) continue;
     hipError_t e = hipMemcpy(cp.first, cp.second.first, cp.second.second, hipMemcpyHostToDevice);
@@ -1083,7 +1093,8 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
       P.uv = mesh.uv;
       P.bones = mesh.bones;
       P.weights = mesh.weights;
-      P.idx = mesh.idx + 3 * (size_t)pr.triangle_start;
+      P.idx = mesh.idx ? mesh.idx + 3 * (size_t)pr.triangle_start : nullptr;
+      P.idx16 = mesh.idx16 ? mesh.idx16 + 3 * (size_t)pr.triangle_start : nullptr;
       P.sk_pos = mesh.sk_pos;
       P.sk_nrm = mesh.sk_nrm;
       P.shape_keys = mesh.K;
@@ -1126,7 +1137,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
       // algorithmic bytes: mesh streams read once, triangle indices once, outputs
       const uint64_t vb = 24 + (mesh.col ? 16 : 0) + (md.skinned ? 24 : 0);
       f->bytes_vertex.push_back((uint64_t)P.vcount * vb + (md.skinned ? 64ull * md.bone_count : 0) + sizeof(PartGPU));
-      f->bytes_cull.push_back(12ull * P.tcount);
+      f->bytes_cull.push_back((mesh.idx16 ? 6ull : 12ull) * P.tcount);
       f->bytes_uv.push_back(mesh.uv ? 8ull * P.vcount : 0);   // UVs are read by the assembly
       f->part_first_tri.push_back(pr.triangle_start);
       t3d_pair_result r{};

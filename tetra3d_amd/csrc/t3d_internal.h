@@ -64,7 +64,8 @@ struct alignas(16) PartGPU {
   const uint16_t* bones;
   const float* weights;
   const float* palette;  // bone_count x 16 floats
-  const uint32_t* idx;   // this part's first triangle in the mesh index array
+  const uint32_t* idx;   // this part's first triangle in the mesh index array (32-bit indices)
+  const uint16_t* idx16; // the same for a mesh stored with 16-bit indices (idx is then null)
   const float* sk_pos;
   const float* sk_nrm;
   const float* sk_w;     // per-frame shape key weights

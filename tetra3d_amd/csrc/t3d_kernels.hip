@@ -432,7 +432,16 @@ __global__ __launch_bounds__(kThreads) void k_cull(FrameArgs a, uint32_t blk0) {
     float depth = -1.0f;
     uint32_t ov[3] = {0, 0, 0};
     if (t < t1) {
-      const uint32_t i0 = P.idx[3 * (size_t)t + 0], i1 = P.idx[3 * (size_t)t + 1], i2 = P.idx[3 * (size_t)t + 2];
+      uint32_t i0, i1, i2;
+      if (P.idx16) {                              // uniform: the pair's mesh index width
+        i0 = P.idx16[3 * (size_t)t + 0];
+        i1 = P.idx16[3 * (size_t)t + 1];
+        i2 = P.idx16[3 * (size_t)t + 2];
+      } else {
+        i0 = P.idx[3 * (size_t)t + 0];
+        i1 = P.idx[3 * (size_t)t + 1];
+        i2 = P.idx[3 * (size_t)t + 2];
+      }
       const uint32_t vi[3] = {i0, i1, i2};
       float nx[3], ny[3];
       bool outOfBounds = true;

@@ -33,7 +33,8 @@ struct Mesh {
   const float* weights = nullptr;
   const float* sk_pos = nullptr;
   const float* sk_nrm = nullptr;
-  const uint32_t* idx = nullptr;
+  const uint32_t* idx = nullptr;   // triangle indices, uint32 ...
+  const uint16_t* idx16 = nullptr; // ... or uint16 when the mesh has at most 65,536 vertices
   uint32_t V = 0, T = 0, K = 0;
   std::vector<t3d_part_range> parts;
   std::vector<uint32_t> part_vmin, part_vcount;
