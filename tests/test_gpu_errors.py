@@ -1,0 +1,74 @@
+"""Invalid inputs fail with a status and a message, allocate nothing that
+leaks and launch nothing, and the context renders correctly afterwards (the
+reference panics or errors where this library returns a status:
+camera.go:389-391, mesh.go:681, 2279).  Every case is refused by host-side
+validation, before any kernel."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from tetra3d_amd import _abi as A
+from tetra3d_amd import workloads as W
+from tetra3d_amd.runtime import T3DError
+from tests.feature_scenes import many_bones_scene
+from tests.parity import compare_frames
+
+
+def _refused(ctx, wl, mutate, status):
+    inp = ctx.inputs(wl.camera, wl.scene, wl.lights, wl.models)
+    mutate(inp.render)
+    with pytest.raises(T3DError) as e:
+        ctx.render_inputs(inp)
+    assert e.value.status == status, (e.value.status, status)
+
+
+@pytest.mark.gpu
+def test_invalid_render_inputs_are_refused(gpu_ctx, oracle):
+    wl = W.icosphere_scene()
+    gpu_ctx.prepare(wl)
+    print("invalid render inputs", flush=True)
+
+    def billboard(r):
+        r.models[0].materials[0].present = 1
+        r.models[0].materials[0].visible = 1
+        r.models[0].materials[0].billboard = 1
+
+    cases = [
+        (lambda r: setattr(r.models[0], "mesh", 0x7FFFFFFF), A.T3D_ERR_NOT_FOUND),
+        (lambda r: setattr(r.camera[0], "width", 0), A.T3D_ERR_INVALID_ARGUMENT),
+        (lambda r: setattr(r.camera[0], "render_normals", 1), A.T3D_ERR_UNSUPPORTED),
+        (lambda r: setattr(r.models[0], "has_vertex_transform_function", 1), A.T3D_ERR_UNSUPPORTED),
+        (lambda r: setattr(r.models[0], "has_vertex_clip_function", 1), A.T3D_ERR_UNSUPPORTED),
+        (lambda r: setattr(r.lights[0], "type", 99), A.T3D_ERR_INVALID_ARGUMENT),
+        (lambda r: setattr(r.models[0], "skinned", 1), A.T3D_ERR_INVALID_ARGUMENT),   # no skin data
+        (billboard, A.T3D_ERR_UNSUPPORTED),
+    ]
+    for mutate, status in cases:
+        _refused(gpu_ctx, wl, mutate, status)
+    # a bone index past the model's palette
+    sk = many_bones_scene(0)
+    gpu_ctx.prepare(sk)
+    _refused(gpu_ctx, sk, lambda r: setattr(r.models[3], "bone_count", 10), A.T3D_ERR_INVALID_ARGUMENT)
+    # more lights on one model than the vertex pass stages (64)
+    many = [W._point("p%d" % i, (0.1 * i, 2, 2), 10) for i in range(70)]
+    lights_wl = W.Workload("many_lights", wl.camera, wl.scene, many, wl.models)
+    _refused(gpu_ctx, lights_wl, lambda r: None, A.T3D_ERR_UNSUPPORTED)
+    # the context still renders both scenes exactly
+    compare_frames(gpu_ctx.render_workload(wl), oracle.render_workload(wl))
+    compare_frames(gpu_ctx.render_workload(sk), oracle.render_workload(sk))
+
+
+@pytest.mark.gpu
+def test_invalid_tuning_is_refused(gpu_ctx, oracle):
+    print("invalid tuning", flush=True)
+    before = gpu_ctx.tuning()
+    for kw in [dict(triangle_tile=100), dict(triangle_tile=4096), dict(store_mode=5), dict(scatter_waves=2),
+               dict(assemble_split=3), dict(vertex_block=320), dict(cull_block=128), dict(xcd_swizzle=2)]:
+        with pytest.raises(T3DError) as e:
+            gpu_ctx.set_tuning(**kw)
+        assert e.value.status == A.T3D_ERR_INVALID_ARGUMENT, kw
+    after = gpu_ctx.tuning()
+    assert bytes(before) == bytes(after)          # a refused tuning changes nothing
+    wl = W.icosphere_scene()
+    compare_frames(gpu_ctx.render_workload(wl), oracle.render_workload(wl))
