@@ -160,6 +160,22 @@ typedef struct {
   float light_bias[3];               /* LightVolumeLightBias */
 } t3d_light_desc;
 
+/* One bone of an armature as Node state (node.go:572-608): local position,
+ * scale and rotation matrix, the index of its parent bone (-1: the root
+ * bones' parent node, t3d_model_desc.armature_parent), and its inverse bind
+ * matrix (gltf.go:1499).  Bones are listed parents first.  The device
+ * computes Transform() = Scale(scale)·rotation·Translate(position)·parent
+ * Transform() and boneInfluence = inverse_bind·Transform() (node.go:592-594)
+ * for every bone, each frame. */
+typedef struct {
+  float position[3];
+  float scale[3];
+  int32_t parent;
+  uint32_t pad;
+  t3d_mat4 rotation;
+  t3d_mat4 inverse_bind;
+} t3d_bone_pose;
+
 /* Model (model.go:20-61) with the node state the path reads. */
 typedef struct {
   uint64_t mesh;                     /* handle from t3d_mesh_upload */
@@ -180,6 +196,11 @@ typedef struct {
   const t3d_light_desc* light_group; /* LightGroup.Lights */
   int32_t has_vertex_transform_function; /* Go callback: unsupported */
   int32_t has_vertex_clip_function;      /* Go callback: unsupported */
+  /* Skinned models may give their bones' poses instead of bone_influences:
+   * [bone_count] poses, parents first; the palette is then computed on the
+   * device each frame (bone_influences is ignored). */
+  const t3d_bone_pose* bone_poses;
+  const t3d_mat4* armature_parent;   /* Transform() of the root bones' parent node; NULL: roots have no parent */
 } t3d_model_desc;
 
 /* Camera state (camera.go:256-321, 324-547). */
@@ -442,7 +463,8 @@ t3d_status t3d_selftest_fp(t3d_ctx* ctx, const float* x, const float* y, float* 
 t3d_status t3d_selftest_copy(t3d_ctx* ctx, uint64_t bytes, uint32_t reps, double* out_ms);
 /* sizeof of each public struct, for binding checks: fills up to max entries
  * in the order vertex, mesh_desc, material_desc, light_desc, model_desc,
- * camera_desc, world_desc, render_desc, pair_result, kernel_time, tuning. */
+ * camera_desc, world_desc, render_desc, pair_result, kernel_time, tuning,
+ * server_config, job_info, server_stats, bone_pose. */
 uint32_t t3d_struct_sizes(uint32_t* sizes, uint32_t max);
 
 #ifdef __cplusplus

@@ -56,6 +56,8 @@ def lib():
         L.t3do_last_error.argtypes = [C.c_void_p]
         for n in ("t3do_mat4_mult",):
             getattr(L, n).argtypes = [M4, M4, M4]
+        L.t3do_bone_palettes.argtypes = [C.POINTER(A.BonePose), C.c_uint32, M4, M4]
+        L.t3do_bone_palettes.restype = C.c_int
         L.t3do_mat4_inverted.argtypes = [M4, M4]
         L.t3do_mat4_transposed.argtypes = [M4, M4]
         L.t3do_mat4_decompose.argtypes = [M4, FP, FP, M4]
@@ -77,6 +79,18 @@ def lib():
     return _lib
 
 
+def bone_palettes(poses, root=None):
+    """Palettes of an armature given as poses (ctypes array of BonePose,
+    parents first), as Node.Transform() computes them (node.go:572-608):
+    (n, 4, 4) float32, or None when a parent is not an earlier bone."""
+    n = len(poses)
+    out = (A.Mat4 * max(1, n))()
+    r = lib().t3do_bone_palettes(poses, n, C.byref(A.mat4(root)) if root is not None else None, out)
+    if r != 0:
+        return None
+    return np.frombuffer(out, dtype=np.float32, count=16 * n).reshape(n, 4, 4).copy()
+
+
 def _vertices(ptr, n):
     if n == 0:
         return np.zeros((0, 12), dtype=np.float32)
@@ -96,6 +110,7 @@ class Frame:
         t = n // 3
         self.triangle_ids = np.ctypeslib.as_array(res.triangle_ids, shape=(t,)).copy() if t else np.zeros(0, np.uint32)
         self.depth_keys = np.ctypeslib.as_array(res.depth_keys, shape=(t,)).copy() if t else np.zeros(0, np.float32)
+        self.survivors = t
 
 
 class Oracle:

@@ -127,6 +127,13 @@ static m4 m4_translate(float x, float y, float z) {                             
   r.m[3][2] = z;
   return r;
 }
+static m4 m4_scale(float x, float y, float z) {                                         /* matrix.go:89-96 */
+  m4 r = m4_identity();
+  r.m[0][0] = x;
+  r.m[1][1] = y;
+  r.m[2][2] = z;
+  return r;
+}
 /* Inverted: matrix.go:255-307 (cofactor form), operation order as written. */
 static m4 m4_inverted(const m4* mat) {
   const float(*m)[4] = mat->m;
@@ -897,6 +904,28 @@ static v3 shape_nrm(const omesh* m, const t3d_model_desc* model, uint32_t v) {  
   return out;
 }
 
+/* Node.Transform (node.go:572-608) down an armature, bones listed parents
+ * first: Scale(s).Mult(rotation).Mult(Translate(p)), then .Mult(parent's
+ * Transform()) (the armature parent's for a root bone, when it has one), and
+ * boneInfluence = inverseBindMatrix.Mult(transform) (node.go:592-594).
+ * world[] receives each bone's Transform().  Returns 0, or -1 for a parent
+ * index that is not an earlier bone. */
+static int bone_palettes(const t3d_bone_pose* poses, uint32_t n, const m4* root, m4* world, m4* out) {
+  for (uint32_t b = 0; b < n; b++) {
+    const t3d_bone_pose* B = &poses[b];
+    if (B->parent < -1 || B->parent >= (int32_t)b) return -1;
+    const m4 sc = m4_scale(B->scale[0], B->scale[1], B->scale[2]);
+    const m4 tr = m4_translate(B->position[0], B->position[1], B->position[2]);
+    m4 t = m4_mult(&sc, &B->rotation);
+    t = m4_mult(&t, &tr);
+    if (B->parent >= 0) t = m4_mult(&t, &world[B->parent]);
+    else if (root) t = m4_mult(&t, root);
+    world[b] = t;
+    out[b] = m4_mult(&B->inverse_bind, &t);
+  }
+  return 0;
+}
+
 /* Model.skinVertex: model.go:559-596. */
 static void skin_vertex(const omesh* m, const t3d_model_desc* model, uint32_t v, v3* op, v3* on) {
   m4 S = m4_zero();
@@ -923,7 +952,12 @@ static int validate_model(t3do_ctx* c, const t3d_model_desc* model, const omesh*
   if (model->has_vertex_transform_function || model->has_vertex_clip_function)
     return set_err(c, "model uses a Go vertex callback") , T3D_ERR_UNSUPPORTED;
   if (model->skinned) {
-    if (!m->bones || !m->weights || !model->bone_influences) return set_err(c, "skinned model without skin data");
+    if (!m->bones || !m->weights || (!model->bone_influences && !model->bone_poses))
+      return set_err(c, "skinned model without skin data");
+    if (model->bone_poses)
+      for (uint32_t b = 0; b < model->bone_count; b++)
+        if (model->bone_poses[b].parent < -1 || model->bone_poses[b].parent >= (int32_t)b)
+          return set_err(c, "bone pose parent is not an earlier bone");
     for (size_t i = 0; i < 4 * (size_t)m->d.vertex_count; i++)
       if (m->weights[i] != 0 && m->bones[i] >= model->bone_count) return set_err(c, "bone index out of range");
   }
@@ -1341,8 +1375,19 @@ int t3do_render(t3do_ctx* c, const t3d_render_desc* desc, t3do_result* res) {
           glights = (olight*)calloc(ng + 1, sizeof(olight));
           for (int g = 0; g < ng; g++) light_from_desc(c, &glights[g], &model->light_group[g]);
         }
+        /* bones given as poses: the palette from Node.Transform down the armature */
+        t3d_model_desc posed;
+        m4* pal = NULL;
+        if (model->skinned && model->bone_poses) {
+          pal = (m4*)malloc(sizeof(m4) * 2 * (model->bone_count ? model->bone_count : 1));
+          bone_palettes(model->bone_poses, model->bone_count, model->armature_parent, pal + model->bone_count, pal);
+          posed = *model;
+          posed.bone_influences = pal;
+          model = &posed;
+        }
         rc = render_pair(c, &cam, world, model, rp->model, rp->part, m, lights, nl, glights, model->light_group, ng,
                          &s, &ob, res, &pair_cap);
+        free(pal);
         free(glights);
       }
     }
@@ -1383,6 +1428,12 @@ void t3do_result_free(t3do_result* r) {
 /* ------------------------------------------------------------------------ */
 
 void t3do_mat4_mult(const t3d_mat4* a, const t3d_mat4* b, t3d_mat4* out) { *out = m4_mult(a, b); }
+int t3do_bone_palettes(const t3d_bone_pose* poses, uint32_t n, const t3d_mat4* root, t3d_mat4* out) {
+  m4* world = (m4*)malloc(sizeof(m4) * (n ? n : 1));
+  const int r = bone_palettes(poses, n, root, world, out);
+  free(world);
+  return r;
+}
 void t3do_mat4_inverted(const t3d_mat4* a, t3d_mat4* out) { *out = m4_inverted(a); }
 void t3do_mat4_transposed(const t3d_mat4* a, t3d_mat4* out) { *out = m4_transposed(a); }
 void t3do_mat4_decompose(const t3d_mat4* a, float pos[3], float scale[3], t3d_mat4* rot) {

@@ -46,6 +46,9 @@ const char* t3do_last_error(const t3do_ctx* ctx);
 
 /* ---- unit-level entry points (used by the tests that pin the oracle) ---- */
 void t3do_mat4_mult(const t3d_mat4* a, const t3d_mat4* b, t3d_mat4* out);
+/* boneInfluence of every bone of an armature from its poses (node.go:572-608);
+ * root may be NULL.  Returns 0, or -1 for a parent that is not an earlier bone. */
+int t3do_bone_palettes(const t3d_bone_pose* poses, uint32_t n, const t3d_mat4* root, t3d_mat4* out);
 void t3do_mat4_inverted(const t3d_mat4* a, t3d_mat4* out);
 void t3do_mat4_transposed(const t3d_mat4* a, t3d_mat4* out);
 void t3do_mat4_decompose(const t3d_mat4* a, float pos[3], float scale[3], t3d_mat4* rot);

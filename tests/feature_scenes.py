@@ -219,3 +219,66 @@ def many_bones_scene(seed):
     world = World()
     world.ambient_light.energy = f32(0.2)
     return W.Workload("many_bones_%d" % seed, cam, Scene("many_bones", world), lights, models)
+
+
+def random_poses(rng, n, tree=True):
+    """An armature of `n` bones as poses, parents first: small local offsets,
+    rotations and near-unit scales (one mirrored), parents drawn among the
+    earlier bones (tree) or each bone under the previous one (chain), with
+    inverse bind matrices of a rest pose."""
+    poses = (A.BonePose * max(1, n))()
+    for b in range(n):
+        p = poses[b]
+        p.position[:] = [float(x) for x in rng.uniform(-0.08, 0.08, 3).astype(f32)]
+        s = rng.uniform(0.9, 1.1, 3)
+        if b == 2:
+            s[1] = -s[1]
+        p.scale[:] = [float(x) for x in s.astype(f32)]
+        p.parent = -1 if b == 0 else (int(rng.integers(-1, b)) if tree else b - 1)
+        axis = W.unit_vectors(W.SplitMix64(int(rng.integers(1 << 30))), 1)[0]
+        p.rotation = A.mat4(W.axis_angle_matrix(axis, float(rng.uniform(0, 0.4))))
+        rest = W.axis_angle_matrix(axis, float(rng.uniform(0, 0.2)))
+        rest = np.array(rest, dtype=np.float64)
+        rest[3, :3] = rng.uniform(-0.05, 0.05, 3)
+        p.inverse_bind = A.mat4(np.linalg.inv(rest).astype(f32))
+    return poses
+
+
+def posed_scene(seed):
+    """Skinned models whose palettes are computed from bone poses: a tree of
+    12 bones on a two-part mesh (one palette job shared by two render pairs),
+    a 300-bone tree (past the 256 bones the palette pass keeps in LDS), a
+    40-bone chain without an armature parent, one bone, and a model with a
+    host palette beside them."""
+    rng = np.random.default_rng(seed)
+    cam = Camera("cam", 640, 360)
+    lights = [DirectionalLight("sun", 1, 1, 1, 1.0), PointLight("lamp", 1, 0.8, 0.6, 2.0)]
+    lights[0].set_local_rotation(W.axis_angle_matrix((1, 0, 0), -0.6))
+    lights[1].set_local_position(0, 3, -5)
+    models = []
+    specs = [("two_part", 12, True, True), ("big_tree", 300, True, True), ("chain", 40, False, False),
+             ("one_bone", 1, True, True), ("host_palette", 6, None, None)]
+    for i, (kind, bones, tree, rooted) in enumerate(specs):
+        if kind == "two_part":
+            mesh = _two_part_mesh(rng, W.icosphere(2, 1.0, name="pose_base"), "pose%d" % i)
+        else:
+            mesh = W.icosphere(2, 1.0, name="pose%d" % i)
+        m = Model("posed_%s" % kind, mesh)
+        m.set_local_position(-6 + 3 * i, 0, -9)
+        m.set_local_rotation(_rotation(rng))
+        _skin(rng, m, bones=max(4, bones))
+        if bones < 4:                                   # every vertex on bone 0
+            V = mesh.vertex_count
+            mesh.bone_indices = np.zeros((V, 4), dtype=np.uint16)
+            mesh.bone_weights = np.tile(np.array([1, 0, 0, 0], dtype=f32), (V, 1))
+        if tree is not None:
+            m.bone_poses = random_poses(rng, bones, tree)
+            if rooted:
+                m.armature_parent = np.array(m.transform(), dtype=f32)
+            else:
+                m.bone_poses[0].position[:] = [float(x) for x in np.asarray(m.transform(), dtype=f32)[3, :3]]
+            m.bone_influences = None
+        models.append(m)
+    world = World()
+    world.ambient_light.energy = f32(0.2)
+    return W.Workload("posed_%d" % seed, cam, Scene("posed", world), lights, models)

@@ -1,0 +1,87 @@
+"""Bone palettes computed on the device from poses (node.go:572-608) against
+the oracle, bit for bit: both execution forms, armatures in and past LDS,
+a palette shared by two render pairs, frames that change only the pose
+(the captured chain replayed with new poses), equality with host palettes,
+the parent check, and posed jobs in the serving mix."""
+import numpy as np
+import pytest
+
+from tetra3d_amd import _abi as A
+from tetra3d_amd import workloads as W
+from tetra3d_amd.runtime import T3DError
+from tests.feature_scenes import feature_scene, posed_scene
+from tests.parity import compare_frames
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("mode", [A.EXEC_BATCHED, A.EXEC_PLAIN], ids=["batched", "plain"])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_posed_matches_oracle(gpu_ctx, oracle, seed, mode):
+    wl = posed_scene(seed)
+    print("posed scene", seed, "mode", mode, flush=True)
+    s = compare_frames(gpu_ctx.render_workload(wl, exec_mode=mode), oracle.render_workload(wl))
+    assert s["survivors"] > 500
+
+
+def test_pose_changes_between_frames(gpu_ctx, oracle):
+    """Same launch shape every frame, so the batched form replays its graph;
+    only the poses and the armature parent move."""
+    wl = posed_scene(2)
+    rng = np.random.default_rng(5)
+    for frame in range(4):
+        print("pose frame", frame, flush=True)
+        for m in wl.models:
+            if m.bone_poses is None:
+                continue
+            for b in range(0, len(m.bone_poses), 3):
+                axis = W.unit_vectors(W.SplitMix64(int(rng.integers(1 << 30))), 1)[0]
+                m.bone_poses[b].rotation = A.mat4(W.axis_angle_matrix(axis, float(rng.uniform(0, 0.4))))
+            if m.armature_parent is not None:
+                m.armature_parent[3, 1] = np.float32(0.25 * frame)
+        compare_frames(gpu_ctx.render_workload(wl), oracle.render_workload(wl))
+
+
+def test_device_palettes_equal_host_palettes(gpu_ctx, oracle_lib):
+    wl = posed_scene(3)
+    posed = gpu_ctx.render_workload(wl)
+    for m in wl.models:
+        if m.bone_poses is not None:
+            m.bone_influences = oracle_lib.bone_palettes(m.bone_poses, m.armature_parent)
+            m.bone_poses = None
+    given = gpu_ctx.render_workload(wl)
+    compare_frames(posed, given)
+
+
+def test_parent_after_child_is_refused(gpu_ctx, oracle):
+    wl = posed_scene(4)
+    gpu_ctx.prepare(wl)
+    big = wl.models[1].bone_poses
+    for bad in (7, 200, -2):
+        old = big[7].parent
+        big[7].parent = bad
+        try:
+            with pytest.raises(T3DError) as e:
+                gpu_ctx.render_workload(wl)
+            assert e.value.status == A.T3D_ERR_INVALID_ARGUMENT
+        finally:
+            big[7].parent = old
+    compare_frames(gpu_ctx.render_workload(wl), oracle.render_workload(wl))
+
+
+def test_posed_jobs_same_alone_and_in_mix(gpu_ctx):
+    from tetra3d_amd.serving import Server
+    scenes = [posed_scene(5), feature_scene(6), posed_scene(7), posed_scene(8)]
+    alone = [gpu_ctx.render_workload(wl) for wl in scenes]
+    inputs = [gpu_ctx.inputs(wl.camera, wl.scene, wl.lights, wl.models) for wl in scenes]
+    with Server(gpu_ctx, slots=2, max_batch_jobs=4) as srv:
+        ids = [srv.submit_inputs(inp) for inp in inputs]
+        for i in ids:
+            srv.wait(i)
+        shared = 0
+        for k, i in enumerate(ids):
+            r = srv.download(i)
+            shared += r.info.batch_jobs > 1
+            compare_frames(r, alone[k])
+            srv.release(i)
+    print("posed jobs in shared batches", shared, flush=True)

@@ -121,6 +121,17 @@ class LightDesc(C.Structure):
     ]
 
 
+class BonePose(C.Structure):
+    _fields_ = [
+        ("position", C.c_float * 3),
+        ("scale", C.c_float * 3),
+        ("parent", C.c_int32),
+        ("pad", C.c_uint32),
+        ("rotation", Mat4),
+        ("inverse_bind", Mat4),
+    ]
+
+
 class ModelDesc(C.Structure):
     _fields_ = [
         ("mesh", C.c_uint64),
@@ -141,6 +152,8 @@ class ModelDesc(C.Structure):
         ("light_group", C.POINTER(LightDesc)),
         ("has_vertex_transform_function", C.c_int32),
         ("has_vertex_clip_function", C.c_int32),
+        ("bone_poses", C.POINTER(BonePose)),
+        ("armature_parent", C.POINTER(Mat4)),
     ]
 
 

@@ -482,7 +482,8 @@ uint32_t t3d_struct_sizes(uint32_t* sizes, uint32_t max) {
                         (uint32_t)sizeof(t3d_world_desc),    (uint32_t)sizeof(t3d_render_desc),
                         (uint32_t)sizeof(t3d_pair_result),   (uint32_t)sizeof(t3d_kernel_time),
                         (uint32_t)sizeof(t3d_tuning),        (uint32_t)sizeof(t3d_server_config),
-                        (uint32_t)sizeof(t3d_job_info),      (uint32_t)sizeof(t3d_server_stats)};
+                        (uint32_t)sizeof(t3d_job_info),      (uint32_t)sizeof(t3d_server_stats),
+                        (uint32_t)sizeof(t3d_bone_pose)};
   uint32_t n = (uint32_t)(sizeof s / sizeof s[0]);
   for (uint32_t i = 0; i < n && i < max; i++) sizes[i] = s[i];
   return n;
@@ -744,6 +745,10 @@ t3d_status run_chain(t3d_ctx* c, t3d_frame* f, uint32_t p0, uint32_t p1, hipStre
   const uint32_t tb0 = f->tile_prefix[p0], tb1 = f->tile_prefix[p1];
   const uint32_t sb0 = f->sblock_prefix[p0], sb1 = f->sblock_prefix[p1];
   const uint32_t ps0 = f->pscan_prefix[p0], ps1 = f->pscan_prefix[p1];
+  if (p0 == 0 && a.npal_jobs) {   // every armature of the frame, before the first pair's vertices
+    ProfScope ps(c, st, "k_palettes", f->bytes_palettes, prof);
+    HIP_TRY(c, launch_palettes(a, st));
+  }
   {
     ProfScope ps(c, st, "k_uniforms", 0, prof);
     HIP_TRY(c, launch_uniforms(a, p0, p1 - p0, st));
@@ -807,6 +812,15 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   std::vector<uint32_t> palette_count;
   std::vector<const float*> skw_host;
   std::vector<uint32_t> skw_count;
+  // models whose bones come as poses: one palette job each, shared by the model's pairs
+  struct PoseJob {
+    const t3d_bone_pose* poses;
+    const t3d_mat4* root;
+    uint32_t n;
+  };
+  std::vector<PoseJob> pose_jobs;
+  std::vector<int32_t> part_pose_job;   // per part: its palette job, or -1
+  part_pose_job.reserve(f->parts.capacity());
   palettes_host.reserve(f->parts.capacity());
   palette_count.reserve(f->parts.capacity());
   skw_host.reserve(f->parts.capacity());
@@ -918,7 +932,12 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
       if (md.has_vertex_transform_function || md.has_vertex_clip_function)
         return fail(c, T3D_ERR_UNSUPPORTED, "render: model %u uses a Go vertex callback", mi);
       if (md.skinned) {
-        if (!mesh.has_skin || !md.bone_influences) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: skinned model %u without skin data", mi);
+        if (!mesh.has_skin || (!md.bone_influences && !md.bone_poses))
+          return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: skinned model %u without skin data", mi);
+        if (md.bone_poses)
+          for (uint32_t b = 0; b < md.bone_count; b++)
+            if (md.bone_poses[b].parent < -1 || md.bone_poses[b].parent >= (int32_t)b)
+              return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u bone %u parent is not an earlier bone", mi, b);
         if (mesh.max_bone_index >= md.bone_count) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u bone index out of palette", mi);
       }
       if (md.light_group_active && md.light_group_count && !md.light_group)
@@ -982,6 +1001,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
     for (auto& sl : scene_lights) lights.push_back(make_light(c, sl.get()));
     const uint32_t n_scene = (uint32_t)scene_lights.size();
     std::map<uint32_t, uint32_t> group_base;  // model -> first frame light of its group
+    std::map<uint32_t, uint32_t> model_pose_job;  // posed model -> its palette job
 
     // camera constants (camera.go:1329-1341, 1591-1606)
     FrameConst fc{};
@@ -1130,8 +1150,18 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
       f->tile_prefix.push_back(f->tile_prefix.back() + ntiles);
       f->sblock_prefix.push_back(sblocks);
       f->pscan_prefix.push_back(f->pscan_prefix.back() + pscan);
-      palettes_host.push_back(md.skinned ? &md.bone_influences[0].m[0][0] : nullptr);
-      palette_count.push_back(md.skinned ? md.bone_count : 0);
+      int32_t pj = -1;
+      if (md.skinned && md.bone_poses) {
+        auto it = model_pose_job.find(rp.model);
+        if (it == model_pose_job.end()) {
+          it = model_pose_job.emplace(rp.model, (uint32_t)pose_jobs.size()).first;
+          pose_jobs.push_back({md.bone_poses, md.armature_parent, md.bone_count});
+        }
+        pj = (int32_t)it->second;
+      }
+      part_pose_job.push_back(pj);
+      palettes_host.push_back(md.skinned && pj < 0 ? &md.bone_influences[0].m[0][0] : nullptr);
+      palette_count.push_back(md.skinned && pj < 0 ? md.bone_count : 0);
       skw_host.push_back(mesh.K ? md.shape_key_weights : nullptr);
       skw_count.push_back(mesh.K);
       // algorithmic bytes: mesh streams read once, triangle indices once, outputs
@@ -1154,16 +1184,24 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   f->nparts = P_n;
 
   // palettes and shape-key weights live in the table block
-  uint64_t pal_floats = 0, skw_floats = 0;
+  uint64_t pal_floats = 0, skw_floats = 0, pose_bones = 0, pose_roots = 0;
   for (uint32_t i = 0; i < P_n; i++) {
     pal_floats += 16ull * palette_count[i];
     skw_floats += skw_count[i];
   }
+  for (const PoseJob& j : pose_jobs) {
+    pal_floats += 16ull * j.n;
+    pose_bones += j.n;
+    pose_roots += j.root ? 1 : 0;
+  }
+  const uint32_t npose = (uint32_t)pose_jobs.size();
 
   // device tables: FrameConst[jobs], parts, lights, prefixes, palettes, shape-key weights
   const uint64_t tbytes = carve_size<FrameConst>(ndesc) + carve_size<PartGPU>(P_n) + carve_size<LightGPU>(lights.size()) +
                           4 * carve_size<uint32_t>(P_n + 1) +
-                          carve_size<float>(pal_floats) + carve_size<float>(skw_floats);
+                          carve_size<float>(pal_floats) + carve_size<float>(skw_floats) +
+                          carve_size<t3d_bone_pose>(pose_bones) + carve_size<M4>(pose_roots) +
+                          carve_size<PaletteJob>(npose);
   t3d_status st = ensure(c, f->tables, f->tables_cap, tbytes);
   if (st) return st;
   if (tbytes > f->pinned_cap) {
@@ -1178,7 +1216,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
                           2 * carve_size<uint4>(sumT) + carve_size<uint32_t>(4ull * P_n) + carve_size<uint32_t>(sumHist) +
                           carve_size<uint32_t>(sblocks) + 2 * carve_size<uint32_t>(ntiles_total) +
                           carve_size<uint32_t>(f->vchunk_prefix.back()) + carve_size<float4>(128) +
-                          carve_size<PartLight>(n_plights);
+                          carve_size<PartLight>(n_plights) + carve_size<M4>(pose_bones);
   st = ensure(c, f->scratch, f->scratch_cap, sbytes);
   if (st) return st;
   const uint64_t obytes = 2 * carve_size<t3d_vertex>(3 * sumT) + carve_size<uint32_t>(3 * sumT);
@@ -1213,6 +1251,12 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   float* h_pal = host.take<float>(pal_floats);
   float* d_skw = dev.take<float>(skw_floats);
   float* h_skw = host.take<float>(skw_floats);
+  t3d_bone_pose* d_poses = dev.take<t3d_bone_pose>(pose_bones);
+  t3d_bone_pose* h_poses = host.take<t3d_bone_pose>(pose_bones);
+  M4* d_roots = dev.take<M4>(pose_roots);
+  M4* h_roots = host.take<M4>(pose_roots);
+  PaletteJob* d_pjobs = dev.take<PaletteJob>(npose);
+  PaletteJob* h_pjobs = host.take<PaletteJob>(npose);
   uint64_t pal_off = 0, skw_off = 0;
   for (uint32_t i = 0; i < P_n; i++) {
     PartGPU& P = f->parts[i];
@@ -1226,6 +1270,31 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
       P.sk_w = d_skw + skw_off;
       skw_off += skw_count[i];
     }
+  }
+  // palette jobs: poses and armature transforms uploaded, palettes after the per-pair ones (written by
+  // k_palettes), armature transforms of the bones in scratch
+  {
+    uint64_t bone_off = 0, root_off = 0;
+    std::vector<uint64_t> job_pal(npose);
+    for (uint32_t j = 0; j < npose; j++) {
+      const PoseJob& J = pose_jobs[j];
+      std::memcpy(h_poses + bone_off, J.poses, sizeof(t3d_bone_pose) * J.n);
+      PaletteJob& H = h_pjobs[j];
+      H.poses = d_poses + bone_off;
+      H.palette = d_pal + pal_off;
+      H.world_off = bone_off;
+      H.n = J.n;
+      H.root = nullptr;
+      if (J.root) {
+        h_roots[root_off] = *J.root;
+        H.root = d_roots + root_off++;
+      }
+      job_pal[j] = pal_off;
+      pal_off += 16ull * J.n;
+      bone_off += J.n;
+    }
+    for (uint32_t i = 0; i < P_n; i++)
+      if (part_pose_job[i] >= 0) f->parts[i].palette = d_pal + job_pal[part_pose_job[i]];
   }
   if (P_n) std::memcpy(h_parts, f->parts.data(), sizeof(PartGPU) * P_n);
   if (!lights.empty()) std::memcpy(h_lights, lights.data(), sizeof(LightGPU) * lights.size());
@@ -1266,6 +1335,11 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   a.vchunk_part = sc.take<uint32_t>(f->vchunk_prefix.back());
   a.sink = sc.take<float4>(128);
   a.plights = sc.take<PartLight>(n_plights);
+  a.pose_world = sc.take<M4>(pose_bones);
+  a.pal_jobs = d_pjobs;
+  a.npal_jobs = npose;
+  // poses and armature transforms read, palettes written
+  f->bytes_palettes = (sizeof(t3d_bone_pose) + 64) * pose_bones + 64 * pose_roots + sizeof(PaletteJob) * npose;
   a.out_color = reinterpret_cast<float4*>(oc.take<t3d_vertex>(3 * sumT));
   a.out_depth = reinterpret_cast<float4*>(oc.take<t3d_vertex>(3 * sumT));
   a.out_idx = oc.take<uint32_t>(3 * sumT);
@@ -1412,7 +1486,8 @@ uint64_t estimate_job_bytes(t3d_ctx* c, const t3d_render_desc* desc) {
       T += m.parts[p].triangle_count;
       tiles += (m.parts[p].triangle_count + tile - 1) / tile;
     }
-    extra += 1024 + 64ull * (md.skinned ? md.bone_count : 0);
+    // palettes; posed bones add their poses (160 B) and an armature transform (64 B) in scratch
+    extra += 1024 + (md.skinned ? (md.bone_poses ? 288ull : 64ull) * md.bone_count : 0);
   }
   return 48 * V + 4 * (V / 64 + 1) + 336 * T + (4ull * kBins + 24) * tiles + extra;
 }

@@ -121,6 +121,17 @@ struct FrameConst {
   float fog_range[2];
 };
 
+// One armature whose palette k_palettes computes from bone poses
+// (node.go:572-608): palette[b] = inverse_bind·Transform(), Transform() of
+// bone b written to FrameArgs.pose_world[world_off + b].
+struct PaletteJob {
+  const t3d_bone_pose* poses;   // [n], parents first
+  const M4* root;               // Transform() of the root bones' parent node, or null
+  float* palette;               // [n] x 16 floats
+  uint64_t world_off;
+  uint32_t n, pad;
+};
+
 struct FrameArgs {
   const PartGPU* parts;
   uint32_t nparts;
@@ -152,10 +163,14 @@ struct FrameArgs {
   float4* out_depth;
   uint32_t* out_idx;
   const FrameConst* fc;         // [jobs] device copies, so graph kernel arguments stay fixed across frames
+  const PaletteJob* pal_jobs;   // [npal_jobs] armatures posed on the device
+  M4* pose_world;               // bone Transform() scratch of the palette jobs
+  uint32_t npal_jobs, pad2;
 };
 
 // Kernel launchers (t3d_kernels.hip).  blk0 offsets blockIdx so the plain
 // form can launch one part's blocks at a time.
+hipError_t launch_palettes(const FrameArgs& a, hipStream_t s);
 hipError_t launch_uniforms(const FrameArgs& a, uint32_t part0, uint32_t nparts, hipStream_t s);
 hipError_t launch_vertex(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t threads, hipStream_t s);
 hipError_t launch_cull(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t threads, hipStream_t s);

@@ -329,6 +329,42 @@ __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
 // 64), lane 0 also the pair's own uniforms and the reset of its depth range
 // and survivor count, so the latency is one light's, not the sum over lights;
 // all lanes first write the pair's entries of the block->pair maps.
+// ---------------------------------------------------------------------------
+// k_palettes: the bone palettes of armatures given as poses, one block per
+// armature, computed as Node.Transform() does down the armature
+// (node.go:572-608).  Scale·rotation·Translate of every bone is independent
+// and spread over the block; the parent chain Transform() = local·parent's
+// Transform() is walked in bone order by one thread (a parent is always an
+// earlier bone), in LDS when the armature fits; boneInfluence =
+// inverseBindMatrix·Transform() (node.go:592-594) is again one bone per
+// thread.  Every product is Matrix4.Mult in its own order, so the palette
+// equals the host's bit for bit.
+constexpr uint32_t kLdsPoseBones = 256;   // 16 KB of LDS
+__global__ __launch_bounds__(256) void k_palettes(FrameArgs a) {
+  const PaletteJob J = a.pal_jobs[blockIdx.x];
+  __shared__ M4 s_world[kLdsPoseBones];
+  M4* W = J.n <= kLdsPoseBones ? s_world : a.pose_world + J.world_off;
+  for (uint32_t b = threadIdx.x; b < J.n; b += blockDim.x) {
+    const t3d_bone_pose& B = J.poses[b];
+    const M4 sr = mult(scale4(B.scale[0], B.scale[1], B.scale[2]), B.rotation);
+    W[b] = mult(sr, translate(B.position[0], B.position[1], B.position[2]));
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t b = 0; b < J.n; b++) {
+      const int32_t p = J.poses[b].parent;          // -1 <= p < b, checked by the host
+      if (p >= 0) W[b] = mult(W[b], W[p]);
+      else if (J.root) W[b] = mult(W[b], *J.root);
+    }
+  }
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < J.n; b += blockDim.x) {
+    const M4 r = mult(J.poses[b].inverse_bind, W[b]);
+    float4* out = reinterpret_cast<float4*>(J.palette + 16ull * b);
+    for (int i = 0; i < 4; i++) out[i] = make_float4(r.m[i][0], r.m[i][1], r.m[i][2], r.m[i][3]);
+  }
+}
+
 __global__ __launch_bounds__(64) void k_uniforms(FrameArgs a, uint32_t part0, uint32_t nparts) {
   PartGPU& P = const_cast<PartGPU*>(a.parts)[part0 + blockIdx.x];
   const uint32_t k = threadIdx.x;
@@ -1244,6 +1280,11 @@ __global__ void k_fp_selftest(const float* x, const float* y, float* out, uint32
 // launchers
 // ---------------------------------------------------------------------------
 
+hipError_t launch_palettes(const FrameArgs& a, hipStream_t s) {
+  if (!a.npal_jobs) return hipSuccess;
+  hipLaunchKernelGGL(k_palettes, dim3(a.npal_jobs), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
 hipError_t launch_uniforms(const FrameArgs& a, uint32_t part0, uint32_t nparts, hipStream_t s) {
   if (!nparts) return hipSuccess;
   hipLaunchKernelGGL(k_uniforms, dim3(nparts), dim3(64), 0, s, a, part0, nparts);

@@ -98,6 +98,13 @@ T3D_HD M4 translate(float x, float y, float z) {                           // ma
   r.m[3][2] = z;
   return r;
 }
+T3D_HD M4 scale4(float x, float y, float z) {                              // matrix.go:90-96
+  M4 r = identity();
+  r.m[0][0] = x;
+  r.m[1][1] = y;
+  r.m[2][2] = z;
+  return r;
+}
 // Matrix4.Inverted: matrix.go:255-307, the 2x2-minor cofactor expansion with
 // its exact grouping.
 T3D_HD M4 inverted(const M4& M) {

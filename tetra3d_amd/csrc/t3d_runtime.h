@@ -68,6 +68,7 @@ struct t3d_frame {
   std::vector<t3d::PartGPU> parts;
   std::vector<uint32_t> part_first_tri;  // mesh triangle_start per pair
   std::vector<uint64_t> bytes_vertex, bytes_cull, bytes_uv;  // algorithmic bytes per part
+  uint64_t bytes_palettes = 0;                               // of k_palettes, all armatures
   std::vector<t3d::FrameConst> fcs;      // one camera per job of the frame
   std::vector<uint32_t> job_pair_begin;  // [jobs + 1] first render pair of each job
   uint64_t table_bytes = 0;

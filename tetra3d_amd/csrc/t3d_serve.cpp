@@ -42,6 +42,8 @@ struct OwnedDesc {
   std::vector<t3d_model_desc> models;
   std::vector<t3d_material_desc> mats;
   std::vector<t3d_mat4> bones;
+  std::vector<t3d_bone_pose> poses;
+  std::vector<t3d_mat4> roots;     // armature parents of posed models
   std::vector<float> skw;
   std::vector<t3d_light_desc> group;
   t3d_render_desc desc{};
@@ -127,7 +129,7 @@ t3d_status copy_desc(t3d_ctx* c, const t3d_render_desc* in, Job& j) {
   o.lights.assign(in->lights, in->lights + in->light_count);
   o.models.assign(in->models, in->models + in->model_count);
   // sizes first so the vectors never reallocate while pointers are taken
-  size_t nm = 0, nb = 0, nk = 0, ng = 0;
+  size_t nm = 0, nb = 0, np = 0, nr = 0, nk = 0, ng = 0;
   for (const auto& m : o.models) {
     auto it = c->meshes.find(m.mesh);
     if (m.mesh && it == c->meshes.end()) return fail(c, T3D_ERR_NOT_FOUND, "submit: unknown mesh %llu", (unsigned long long)m.mesh);
@@ -135,11 +137,17 @@ t3d_status copy_desc(t3d_ctx* c, const t3d_render_desc* in, Job& j) {
     const Mesh& mesh = *it->second;
     if (m.materials) nm += mesh.parts.size();
     if (m.skinned && m.bone_influences) nb += m.bone_count;
+    if (m.skinned && m.bone_poses) {
+      np += m.bone_count;
+      if (m.armature_parent) nr++;
+    }
     if (m.shape_key_weights) nk += mesh.K;
     if (m.light_group_active && m.light_group) ng += m.light_group_count;
   }
   o.mats.reserve(nm);
   o.bones.reserve(nb);
+  o.poses.reserve(np);
+  o.roots.reserve(nr);
   o.skw.reserve(nk);
   o.group.reserve(ng);
   for (auto& m : o.models) {
@@ -154,6 +162,18 @@ t3d_status copy_desc(t3d_ctx* c, const t3d_render_desc* in, Job& j) {
       size_t at = o.bones.size();
       o.bones.insert(o.bones.end(), m.bone_influences, m.bone_influences + m.bone_count);
       m.bone_influences = o.bones.data() + at;
+    }
+    if (m.skinned && m.bone_poses) {
+      size_t at = o.poses.size();
+      o.poses.insert(o.poses.end(), m.bone_poses, m.bone_poses + m.bone_count);
+      m.bone_poses = o.poses.data() + at;
+      if (m.armature_parent) {
+        o.roots.push_back(*m.armature_parent);
+        m.armature_parent = &o.roots.back();
+      }
+    } else {
+      m.bone_poses = nullptr;
+      m.armature_parent = nullptr;
     }
     if (m.shape_key_weights) {
       size_t at = o.skw.size();

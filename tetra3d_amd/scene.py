@@ -257,6 +257,8 @@ class Model(Node):
         self.auto_batch_static = False
         self.skinned = False
         self.bone_influences = None  # (B, 4, 4) float32: boneInfluence per bone
+        self.bone_poses = None       # BonePose ctypes array: bones as node state, parents first (palette built on the device)
+        self.armature_parent = None  # 4x4: Transform() of the root bones' parent node
         self.shape_key_weights = None
         self.light_group = None      # list of lights when a LightGroup is active
 
@@ -271,7 +273,15 @@ class Model(Node):
         d.frustum_culling = int(self.frustum_culling)
         d.auto_batch_static = int(self.auto_batch_static)
         d.skinned = int(self.skinned)
-        if self.skinned:
+        if self.skinned and self.bone_poses is not None:
+            keep.append(self.bone_poses)
+            d.bone_count = len(self.bone_poses)
+            d.bone_poses = C.cast(self.bone_poses, C.POINTER(A.BonePose))
+            if self.armature_parent is not None:
+                root = A.mat4(self.armature_parent)
+                keep.append(root)
+                d.armature_parent = C.pointer(root)
+        elif self.skinned:
             bones = np.ascontiguousarray(self.bone_influences, dtype=f32).reshape(-1, 4, 4)
             keep.append(bones)
             d.bone_count = len(bones)
