@@ -30,6 +30,9 @@ WORKLOADS = {
     "cubes": ("cubes_4096", "4,096 unit cubes in a 200^3 box, ambient + 1 directional + 8 point lights, 1920x1080"),
     "heightfield": ("heightfield_1024", "1024x1024 heightfield (2,093,058 triangles), camera inside the terrain, "
                     "16 point + 1 directional + 1 volume light"),
+    "crowd": ("crowd_256", "256 characters sharing one 2,562-vertex mesh, each with its own 64-bone armature "
+              "given as poses (palettes built on the device every frame), ambient + 1 directional + 4 point "
+              "lights, 1920x1080"),
     "icosphere": ("icosphere_642v", "642-vertex icosphere, 640x360, ambient + directional + point light"),
 }
 

@@ -244,6 +244,27 @@ def random_poses(rng, n, tree=True):
     return poses
 
 
+def huge_armature_scene(seed, bones=5000):
+    """One skinned model whose armature has more bones than the palette pass
+    walks level by level (4096), so its parent chain is walked bone by bone,
+    beside a 12-bone one."""
+    rng = np.random.default_rng(seed)
+    cam = Camera("cam", 640, 360)
+    lights = [DirectionalLight("sun", 1, 1, 1, 1.0)]
+    models = []
+    for i, n in enumerate([bones, 12]):
+        m = Model("huge%d" % i, W.icosphere(2, 1.0, name="huge%d" % i))
+        m.set_local_position(-1.5 + 3 * i, 0, -6)
+        _skin(rng, m, bones=n)
+        m.bone_poses = random_poses(rng, n, True)
+        m.armature_parent = np.array(m.transform(), dtype=f32)
+        m.bone_influences = None
+        models.append(m)
+    world = World()
+    world.ambient_light.energy = f32(0.2)
+    return W.Workload("huge_armature_%d" % seed, cam, Scene("huge", world), lights, models)
+
+
 def posed_scene(seed):
     """Skinned models whose palettes are computed from bone poses: a tree of
     12 bones on a two-part mesh (one palette job shared by two render pairs),

@@ -1,15 +1,16 @@
 """Bone palettes computed on the device from poses (node.go:572-608) against
-the oracle, bit for bit: both execution forms, armatures in and past LDS,
-a palette shared by two render pairs, frames that change only the pose
-(the captured chain replayed with new poses), equality with host palettes,
-the parent check, and posed jobs in the serving mix."""
+the oracle, bit for bit: both execution forms, armatures in and past LDS
+and past the level-by-level walk (5,000 bones), a palette shared by two
+render pairs, frames that change only the pose (the captured chain replayed
+with new poses), equality with host palettes, the parent check, and posed
+jobs in the serving mix."""
 import numpy as np
 import pytest
 
 from tetra3d_amd import _abi as A
 from tetra3d_amd import workloads as W
 from tetra3d_amd.runtime import T3DError
-from tests.feature_scenes import feature_scene, posed_scene
+from tests.feature_scenes import feature_scene, huge_armature_scene, posed_scene
 from tests.parity import compare_frames
 
 pytestmark = pytest.mark.gpu
@@ -22,6 +23,14 @@ def test_posed_matches_oracle(gpu_ctx, oracle, seed, mode):
     print("posed scene", seed, "mode", mode, flush=True)
     s = compare_frames(gpu_ctx.render_workload(wl, exec_mode=mode), oracle.render_workload(wl))
     assert s["survivors"] > 500
+
+
+@pytest.mark.parametrize("mode", [A.EXEC_BATCHED, A.EXEC_PLAIN], ids=["batched", "plain"])
+def test_armature_past_level_walk(gpu_ctx, oracle, mode):
+    wl = huge_armature_scene(9)
+    print("5000-bone armature, mode", mode, flush=True)
+    s = compare_frames(gpu_ctx.render_workload(wl, exec_mode=mode), oracle.render_workload(wl))
+    assert s["survivors"] > 100
 
 
 def test_pose_changes_between_frames(gpu_ctx, oracle):

@@ -89,6 +89,7 @@ FULL = {
     "cubes_4096": lambda: W.cubes_scene(),
     "heightfield_1024": lambda: W.heightfield_scene(),
     "mixed_512": lambda: W.mixed_scene(),
+    "crowd_256": lambda: W.crowd_scene(),
 }
 
 

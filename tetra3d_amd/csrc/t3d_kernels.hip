@@ -316,8 +316,96 @@ __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
 }
 
 // ---------------------------------------------------------------------------
-// k_cull
+// k_palettes, k_uniforms
 // ---------------------------------------------------------------------------
+
+// k_palettes: the bone palettes of armatures given as poses, one block per
+// armature, computed as Node.Transform() does down the armature
+// (node.go:572-608):
+//   1. Scale·rotation·Translate of every bone, spread over the block;
+//   2. Transform() = local·parent's Transform() (the armature parent's for a
+//      root bone), level by level: each round, every bone whose parent is
+//      done takes its parent's product, so the rounds number the armature's
+//      depth, not its bones;
+//   3. boneInfluence = inverseBindMatrix·Transform() (node.go:592-594).
+// Every product is Matrix4.Mult in its own order, so the palette equals the
+// host's bit for bit.  Parents and round states live in LDS; the bone
+// transforms too up to kLdsPoseBones, in device memory past that.
+constexpr uint32_t kLdsPoseBones = 256;     // 16 KB of LDS
+constexpr uint32_t kLevelPoseBones = 4096;  // armatures walked level by level up to this size
+constexpr uint32_t kPoseThreads = 256;
+
+template <typename WPtr>
+__device__ __forceinline__ void pose_armature(const PaletteJob& J, WPtr W, int16_t* par, uint8_t* state) {
+  const uint32_t n = J.n;
+  for (uint32_t b = threadIdx.x; b < n; b += kPoseThreads) {
+    const t3d_bone_pose& B = J.poses[b];
+    const M4 sr = mult(scale4(B.scale[0], B.scale[1], B.scale[2]), B.rotation);
+    W[b] = mult(sr, translate(B.position[0], B.position[1], B.position[2]));
+    par[b] = (int16_t)B.parent;                      // -1 <= parent < b, checked by the host
+    state[b] = 0;                                    // 0 waiting, 1 ready this round, 2 done
+  }
+  __syncthreads();
+  // Each round readies at least the lowest waiting bone (its parent is earlier, so done): at most n rounds.
+  // A parent index outside [-1, b) (refused by the host) never readies, and the bound still ends the loop.
+  for (uint32_t round = 0; round < n; round++) {
+    bool waiting = false;
+    for (uint32_t b = threadIdx.x; b < n; b += kPoseThreads)
+      if (state[b] == 0) {
+        const int32_t p = par[b];
+        if (p < 0 || ((uint32_t)p < b && state[p] == 2)) state[b] = 1;   // a parent going 0 -> 1 now reads as not done
+        else waiting = true;
+      }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < n; b += kPoseThreads)
+      if (state[b] == 1) {
+        const int32_t p = par[b];
+        if (p >= 0) W[b] = mult(W[b], W[p]);         // W[p] was finished in an earlier round
+        else if (J.root) W[b] = mult(W[b], *J.root);
+        state[b] = 2;
+      }
+    if (!__syncthreads_or(waiting)) break;
+  }
+  for (uint32_t b = threadIdx.x; b < n; b += kPoseThreads) {
+    const M4 r = mult(J.poses[b].inverse_bind, W[b]);
+    float4* out = reinterpret_cast<float4*>(J.palette + 16ull * b);
+    for (int i = 0; i < 4; i++) out[i] = make_float4(r.m[i][0], r.m[i][1], r.m[i][2], r.m[i][3]);
+  }
+}
+
+// Armatures past kLevelPoseBones: the parent chain walked in bone order by
+// one thread (a parent is always an earlier bone).
+__device__ void pose_armature_serial(const PaletteJob& J, M4* W) {
+  for (uint32_t b = threadIdx.x; b < J.n; b += kPoseThreads) {
+    const t3d_bone_pose& B = J.poses[b];
+    const M4 sr = mult(scale4(B.scale[0], B.scale[1], B.scale[2]), B.rotation);
+    W[b] = mult(sr, translate(B.position[0], B.position[1], B.position[2]));
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (uint32_t b = 0; b < J.n; b++) {
+      const int32_t p = J.poses[b].parent;
+      if (p >= 0 && (uint32_t)p < b) W[b] = mult(W[b], W[p]);
+      else if (J.root) W[b] = mult(W[b], *J.root);
+    }
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < J.n; b += kPoseThreads) {
+    const M4 r = mult(J.poses[b].inverse_bind, W[b]);
+    float4* out = reinterpret_cast<float4*>(J.palette + 16ull * b);
+    for (int i = 0; i < 4; i++) out[i] = make_float4(r.m[i][0], r.m[i][1], r.m[i][2], r.m[i][3]);
+  }
+}
+
+__global__ __launch_bounds__(kPoseThreads) void k_palettes(FrameArgs a) {
+  const PaletteJob J = a.pal_jobs[blockIdx.x];
+  __shared__ M4 s_world[kLdsPoseBones];
+  __shared__ int16_t s_par[kLevelPoseBones];
+  __shared__ uint8_t s_state[kLevelPoseBones];
+  M4* gw = a.pose_world + J.world_off;
+  if (J.n <= kLdsPoseBones) pose_armature(J, s_world, s_par, s_state);
+  else if (J.n <= kLevelPoseBones) pose_armature(J, gw, s_par, s_state);
+  else pose_armature_serial(J, gw);
+}
 
 // Per-pair uniforms: MVP (model.go:695), invertedCamPos (model.go:629-630),
 // the directional working rotation (light.go:326-330) and, per light of the
@@ -329,42 +417,6 @@ __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
 // 64), lane 0 also the pair's own uniforms and the reset of its depth range
 // and survivor count, so the latency is one light's, not the sum over lights;
 // all lanes first write the pair's entries of the block->pair maps.
-// ---------------------------------------------------------------------------
-// k_palettes: the bone palettes of armatures given as poses, one block per
-// armature, computed as Node.Transform() does down the armature
-// (node.go:572-608).  Scale·rotation·Translate of every bone is independent
-// and spread over the block; the parent chain Transform() = local·parent's
-// Transform() is walked in bone order by one thread (a parent is always an
-// earlier bone), in LDS when the armature fits; boneInfluence =
-// inverseBindMatrix·Transform() (node.go:592-594) is again one bone per
-// thread.  Every product is Matrix4.Mult in its own order, so the palette
-// equals the host's bit for bit.
-constexpr uint32_t kLdsPoseBones = 256;   // 16 KB of LDS
-__global__ __launch_bounds__(256) void k_palettes(FrameArgs a) {
-  const PaletteJob J = a.pal_jobs[blockIdx.x];
-  __shared__ M4 s_world[kLdsPoseBones];
-  M4* W = J.n <= kLdsPoseBones ? s_world : a.pose_world + J.world_off;
-  for (uint32_t b = threadIdx.x; b < J.n; b += blockDim.x) {
-    const t3d_bone_pose& B = J.poses[b];
-    const M4 sr = mult(scale4(B.scale[0], B.scale[1], B.scale[2]), B.rotation);
-    W[b] = mult(sr, translate(B.position[0], B.position[1], B.position[2]));
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (uint32_t b = 0; b < J.n; b++) {
-      const int32_t p = J.poses[b].parent;          // -1 <= p < b, checked by the host
-      if (p >= 0) W[b] = mult(W[b], W[p]);
-      else if (J.root) W[b] = mult(W[b], *J.root);
-    }
-  }
-  __syncthreads();
-  for (uint32_t b = threadIdx.x; b < J.n; b += blockDim.x) {
-    const M4 r = mult(J.poses[b].inverse_bind, W[b]);
-    float4* out = reinterpret_cast<float4*>(J.palette + 16ull * b);
-    for (int i = 0; i < 4; i++) out[i] = make_float4(r.m[i][0], r.m[i][1], r.m[i][2], r.m[i][3]);
-  }
-}
-
 __global__ __launch_bounds__(64) void k_uniforms(FrameArgs a, uint32_t part0, uint32_t nparts) {
   PartGPU& P = const_cast<PartGPU*>(a.parts)[part0 + blockIdx.x];
   const uint32_t k = threadIdx.x;
@@ -438,6 +490,10 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
 }
+
+// ---------------------------------------------------------------------------
+// k_cull
+// ---------------------------------------------------------------------------
 
 // kThreads: 256, 512 or 1024 threads per tile block (tunable).
 template <uint32_t kThreads>
@@ -1282,7 +1338,7 @@ __global__ void k_fp_selftest(const float* x, const float* y, float* out, uint32
 
 hipError_t launch_palettes(const FrameArgs& a, hipStream_t s) {
   if (!a.npal_jobs) return hipSuccess;
-  hipLaunchKernelGGL(k_palettes, dim3(a.npal_jobs), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_palettes, dim3(a.npal_jobs), dim3(kPoseThreads), 0, s, a);
   return hipGetLastError();
 }
 hipError_t launch_uniforms(const FrameArgs& a, uint32_t part0, uint32_t nparts, hipStream_t s) {

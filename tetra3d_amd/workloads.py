@@ -9,6 +9,7 @@ Scenes (sizes as in the measurement plan):
   cubes_scene      — 4,096 unit cubes in a 200^3 box, 1920x1080, 10 lights
   heightfield_scene— 1024x1024 heightfield, camera inside the terrain, 18 lights
   mixed_scene      — 512 models, V log-uniform in [24, 65536], 128 skinned
+  crowd_scene      — 256 characters on one skinned mesh, 64-bone armatures as poses
   serving_pool     — 10,000 models of the mixed distribution + a Poisson job trace
 """
 import math
@@ -475,6 +476,58 @@ def mixed_scene(seed=DEFAULT_SEED, count=512):
     return Workload("mixed_512", cam, Scene("mixed", _world(0.2)), lights, models)
 
 
+def armature_poses(rng, bones, place, max_angle_deg=20.0):
+    """An armature of `bones` bones as node state, parents first: bone b > 0
+    hangs under a uniformly drawn earlier bone (bone 0 under the armature
+    node), local offsets in [-0.1, 0.1), rotations of at most `max_angle_deg`
+    about random axes, unit scale, identity inverse bind.  Returns the
+    BonePose array and the armature node's transform (`place`)."""
+    poses = (A.BonePose * bones)()
+    pos = rng.uniform(-0.1, 0.1, 3 * bones).reshape(bones, 3)
+    axes = unit_vectors(rng, bones)
+    angles = rng.floats(bones).astype(np.float64) * math.radians(max_angle_deg)
+    parents = [-1] + [int(rng.ints(0, b, 1)[0]) for b in range(1, bones)]
+    eye = A.mat4(identity())
+    for b in range(bones):
+        p = poses[b]
+        p.position[:] = [float(x) for x in pos[b]]
+        p.scale[:] = [1.0, 1.0, 1.0]
+        p.parent = parents[b]
+        p.rotation = A.mat4(axis_angle_matrix(axes[b], angles[b]))
+        p.inverse_bind = eye
+    return poses, np.asarray(place, dtype=f32).copy()
+
+
+def crowd_scene(seed=DEFAULT_SEED, count=256, bones=64, level=4):
+    """`count` characters sharing one 2,562-vertex icosphere mesh skinned to
+    `bones` bones (4 influences), each with its own armature given as poses,
+    so the palettes are built on the device every frame; a 16-wide grid 4
+    units apart in front of a 1920x1080 camera (FOV 60, far 400); ambient
+    0.2, one directional and four point lights."""
+    rng = SplitMix64(seed)
+    mesh = icosphere(level, 1.0, name="character")
+    models = []
+    for i in range(count):
+        m = Model("character%d" % i, mesh)
+        m.set_local_position(-30 + 4 * (i % 16), -6 + 4 * ((i // 16) % 4), -40 - 6 * (i // 64))
+        q = uniform_quaternions(rng.child(100 + i), 1)[0]
+        m.set_local_rotation(quat_to_matrix(*q))
+        if i == 0:
+            _skin(m, rng.child(7), bones=bones)      # the mesh's bone indices and weights
+        m.skinned = True
+        m.bone_influences = None
+        m.bone_poses, m.armature_parent = armature_poses(rng.child(1000 + i), bones, m.transform())
+        models.append(m)
+    cam = Camera("camera", 1920, 1080)
+    cam.fov = f32(60)
+    cam.far = f32(400)
+    lr = rng.child(21)
+    lights = [_directional("sun", unit_vectors(lr, 1)[0])]
+    lp = np.stack([lr.uniform(-30, 30, 4), lr.uniform(-8, 8, 4), lr.uniform(-60, -30, 4)], axis=1)
+    lights += [_point("point%d" % i, lp[i], 60) for i in range(4)]
+    return Workload("crowd_%d" % count, cam, Scene("crowd", _world(0.2)), lights, models)
+
+
 class Job:
     """One serving request: a camera, a subset of the pool and a light list."""
 
@@ -518,4 +571,5 @@ SCENES = {
     "cubes": cubes_scene,
     "heightfield": heightfield_scene,
     "mixed": mixed_scene,
+    "crowd": crowd_scene,
 }
