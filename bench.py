@@ -31,8 +31,9 @@ WORKLOADS = {
     "heightfield": ("heightfield_1024", "1024x1024 heightfield (2,093,058 triangles), camera inside the terrain, "
                     "16 point + 1 directional + 1 volume light"),
     "crowd": ("crowd_256", "256 characters sharing one 2,562-vertex mesh, each with its own 64-bone armature "
-              "given as poses (palettes built on the device every frame), ambient + 1 directional + 4 point "
-              "lights, 1920x1080"),
+              "given as poses and playing one of 8 one-second clips (24 rotation keys per bone) at its own "
+              "playhead: keyframe sampling and palettes on the device every frame; ambient + 1 directional + 4 "
+              "point lights, 1920x1080"),
     "icosphere": ("icosphere_642v", "642-vertex icosphere, 640x360, ambient + directional + point light"),
 }
 
@@ -74,6 +75,11 @@ def cpu_baseline(wl, budget_s):
     saved = [v.handle for v in wl.volumes]
     for i, v in enumerate(wl.volumes):
         v.handle = 100000 + i
+    anims = wl.animations
+    saved_anim = [an.handle for an in anims]
+    for i, an in enumerate(anims):
+        an.handle = an.handle or 200000 + i
+        o.register_animation(an, an.handle)
     frames, el = 0, 0.0
     while True:
         el += o.render_seconds(wl.camera, wl.scene, wl.lights, wl.models)
@@ -82,6 +88,8 @@ def cpu_baseline(wl, budget_s):
             break
     for v, h in zip(wl.volumes, saved):
         v.handle = h
+    for an, h in zip(anims, saved_anim):
+        an.handle = h
     o.close()
     return frames, el
 

@@ -176,6 +176,25 @@ typedef struct {
   t3d_mat4 inverse_bind;
 } t3d_bone_pose;
 
+/* AnimationTrack interpolation (animation.go:18-20).  Cubic is unimplemented
+ * in the reference and samples as linear there, and here. */
+enum { T3D_INTERPOLATION_LINEAR = 0, T3D_INTERPOLATION_CONSTANT = 1, T3D_INTERPOLATION_CUBIC = 2 };
+
+/* AnimationTrack (animation.go:40-54): keyframe times in seconds, in
+ * non-decreasing order, and per keyframe 3 floats (position, scale) or a
+ * quaternion x, y, z, w (rotation). */
+typedef struct {
+  uint32_t keyframe_count;           /* 0: the channel has no such track */
+  int32_t interpolation;             /* T3D_INTERPOLATION_* */
+  const float* times;                /* [keyframe_count] */
+  const float* values;               /* [keyframe_count] x 3 or x 4 */
+} t3d_anim_track;
+
+/* AnimationChannel (animation.go:188-200) of one bone. */
+typedef struct {
+  t3d_anim_track position, scale, rotation;
+} t3d_anim_channel;
+
 /* Model (model.go:20-61) with the node state the path reads. */
 typedef struct {
   uint64_t mesh;                     /* handle from t3d_mesh_upload */
@@ -201,6 +220,14 @@ typedef struct {
    * device each frame (bone_influences is ignored). */
   const t3d_bone_pose* bone_poses;
   const t3d_mat4* armature_parent;   /* Transform() of the root bones' parent node; NULL: roots have no parent */
+  /* With bone_poses: an animation from t3d_animation_upload (0: none) whose
+   * channel b animates bone b, sampled at animation_time (the player's
+   * playhead when AnimationPlayer.Update samples, animation.go:674-720):
+   * each present track replaces the bone's position, scale or rotation
+   * (animation.go:902-983, no blend, absolute motion). */
+  uint64_t animation;
+  float animation_time;
+  uint32_t pad_animation;
 } t3d_model_desc;
 
 /* Camera state (camera.go:256-321, 324-547). */
@@ -280,6 +307,14 @@ t3d_status t3d_mesh_free(t3d_ctx* ctx, uint64_t handle);
  * x + y*cx + z*cx*cy (getLightCell, light.go:1267-1272). */
 t3d_status t3d_volume_upload(t3d_ctx* ctx, const float* cells_rgba, uint32_t cell_count, uint64_t* out_handle);
 t3d_status t3d_volume_free(t3d_ctx* ctx, uint64_t handle);
+
+/* An Animation's channels (animation.go:188-260), resident on the device;
+ * replaces the per-frame AnimationTrack.ValueAsVector / ValueAsQuaternion
+ * calls of AnimationPlayer.updateValues (animation.go:63-160, 674-720).
+ * Times must be finite and non-decreasing within a track. */
+t3d_status t3d_animation_upload(t3d_ctx* ctx, const t3d_anim_channel* channels, uint32_t channel_count,
+                                uint64_t* out_handle);
+t3d_status t3d_animation_free(t3d_ctx* ctx, uint64_t handle);
 
 /* ---- frames ---------------------------------------------------------- */
 
@@ -464,7 +499,7 @@ t3d_status t3d_selftest_copy(t3d_ctx* ctx, uint64_t bytes, uint32_t reps, double
 /* sizeof of each public struct, for binding checks: fills up to max entries
  * in the order vertex, mesh_desc, material_desc, light_desc, model_desc,
  * camera_desc, world_desc, render_desc, pair_result, kernel_time, tuning,
- * server_config, job_info, server_stats, bone_pose. */
+ * server_config, job_info, server_stats, bone_pose, anim_track, anim_channel. */
 uint32_t t3d_struct_sizes(uint32_t* sizes, uint32_t max);
 
 #ifdef __cplusplus

@@ -58,6 +58,18 @@ def lib():
             getattr(L, n).argtypes = [M4, M4, M4]
         L.t3do_bone_palettes.argtypes = [C.POINTER(A.BonePose), C.c_uint32, M4, M4]
         L.t3do_bone_palettes.restype = C.c_int
+        L.t3do_animate_poses.argtypes = [C.POINTER(A.AnimChannel), C.c_uint32, C.c_float, C.POINTER(A.BonePose),
+                                         C.c_uint32]
+        L.t3do_animate_poses.restype = None
+        L.t3do_animation_register.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(A.AnimChannel), C.c_uint32]
+        L.t3do_go_sin.argtypes = [C.c_double]
+        L.t3do_go_sin.restype = C.c_double
+        L.t3do_go_atan2.argtypes = [C.c_double, C.c_double]
+        L.t3do_go_atan2.restype = C.c_double
+        L.t3do_quat_slerp.argtypes = [FP, FP, C.c_float, FP]
+        L.t3do_quat_slerp.restype = None
+        L.t3do_quat_to_mat4.argtypes = [FP, M4]
+        L.t3do_quat_to_mat4.restype = None
         L.t3do_mat4_inverted.argtypes = [M4, M4]
         L.t3do_mat4_transposed.argtypes = [M4, M4]
         L.t3do_mat4_decompose.argtypes = [M4, FP, FP, M4]
@@ -77,6 +89,16 @@ def lib():
         L.t3do_go_stable_sort_table.argtypes = [C.POINTER(C.c_int32), C.c_uint32, C.POINTER(C.c_uint8)]
         _lib = L
     return _lib
+
+
+def animate_poses(anim, time, poses):
+    """Copy of `poses` after sampling `anim` at `time` (animation.go:63-160,
+    902-983)."""
+    out = (A.BonePose * len(poses))()
+    C.memmove(out, poses, C.sizeof(A.BonePose) * len(poses))
+    arr, keep = anim.channels_desc()
+    lib().t3do_animate_poses(arr, len(anim.channels), time, out, len(poses))
+    return out
 
 
 def bone_palettes(poses, root=None):
@@ -122,6 +144,7 @@ class Oracle:
         self._handles = {}
         self._refs = []   # registered meshes stay alive so their ids are not reused
         self._own_handle = 1 << 40   # above any handle a device context hands out
+        self._anims = {}             # registered animations (copied by the oracle), kept alive by id
 
     def close(self):
         if self.ctx:
@@ -142,6 +165,11 @@ class Oracle:
     def register_volume(self, vol, handle):
         cells = np.ascontiguousarray(vol.cells, dtype=np.float32)
         self.L.t3do_volume_register(self.ctx, handle, cells.ctypes.data_as(C.POINTER(C.c_float)), len(cells))
+
+    def register_animation(self, anim, handle):
+        arr, keep = anim.channels_desc()
+        self.L.t3do_animation_register(self.ctx, handle, arr, len(anim.channels))
+        self._anims[id(anim)] = anim
 
     def handle_of(self, mesh):
         return self._handles[id(mesh)]
@@ -185,4 +213,10 @@ class Oracle:
                 self._own_handle += 1
                 v.handle = self._own_handle
             self.register_volume(v, v.handle)
+        for an in wl.animations:
+            if id(an) not in self._anims:
+                if an.handle == 0:
+                    self._own_handle += 1
+                    an.handle = self._own_handle
+                self.register_animation(an, an.handle)
         return self.render(wl.camera, wl.scene, wl.lights, wl.models)

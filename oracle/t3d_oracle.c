@@ -376,11 +376,20 @@ typedef struct {
   uint32_t n;
 } ovolume;
 
+typedef struct {
+  uint64_t handle;
+  t3d_anim_channel* ch;   /* tracks point into the copies below */
+  float* data;
+  uint32_t n;
+} oanim;
+
 struct t3do_ctx {
   omesh* meshes;
   int nmeshes, cap_meshes;
   ovolume* vols;
   int nvols, cap_vols;
+  oanim* anims;
+  int nanims, cap_anims;
   char err[512];
 };
 
@@ -403,8 +412,13 @@ void t3do_destroy(t3do_ctx* c) {
   if (!c) return;
   for (int i = 0; i < c->nmeshes; i++) omesh_free(&c->meshes[i]);
   for (int i = 0; i < c->nvols; i++) free(c->vols[i].cells);
+  for (int i = 0; i < c->nanims; i++) {
+    free(c->anims[i].ch);
+    free(c->anims[i].data);
+  }
   free(c->meshes);
   free(c->vols);
+  free(c->anims);
   free(c);
 }
 
@@ -463,6 +477,44 @@ int t3do_volume_register(t3do_ctx* c, uint64_t handle, const float* cells, uint3
   v->n = n;
   v->cells = (float*)dup_mem(cells, (size_t)n * 16);
   return T3D_OK;
+}
+
+int t3do_animation_register(t3do_ctx* c, uint64_t handle, const t3d_anim_channel* ch, uint32_t n) {
+  size_t floats = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    const t3d_anim_track* t[3] = {&ch[i].position, &ch[i].scale, &ch[i].rotation};
+    for (int k = 0; k < 3; k++) floats += (size_t)t[k]->keyframe_count * (k == 2 ? 5 : 4);
+  }
+  if (c->nanims == c->cap_anims) {
+    c->cap_anims = c->cap_anims ? 2 * c->cap_anims : 8;
+    c->anims = (oanim*)realloc(c->anims, sizeof(oanim) * c->cap_anims);
+  }
+  oanim* a = &c->anims[c->nanims++];
+  a->handle = handle;
+  a->n = n;
+  a->ch = (t3d_anim_channel*)calloc(n ? n : 1, sizeof(t3d_anim_channel));
+  a->data = (float*)malloc(sizeof(float) * (floats ? floats : 1));
+  float* d = a->data;
+  for (uint32_t i = 0; i < n; i++) {
+    a->ch[i] = ch[i];
+    t3d_anim_track* t[3] = {&a->ch[i].position, &a->ch[i].scale, &a->ch[i].rotation};
+    for (int k = 0; k < 3; k++) {
+      const uint32_t m = t[k]->keyframe_count, w = k == 2 ? 4 : 3;
+      if (!m) continue;
+      memcpy(d, t[k]->times, sizeof(float) * m);
+      t[k]->times = d;
+      d += m;
+      memcpy(d, t[k]->values, sizeof(float) * m * w);
+      t[k]->values = d;
+      d += (size_t)m * w;
+    }
+  }
+  return T3D_OK;
+}
+static oanim* find_anim(t3do_ctx* c, uint64_t h) {
+  for (int i = 0; i < c->nanims; i++)
+    if (c->anims[i].handle == h) return &c->anims[i];
+  return NULL;
 }
 
 static omesh* find_mesh(t3do_ctx* c, uint64_t h) {
@@ -904,6 +956,186 @@ static v3 shape_nrm(const omesh* m, const t3d_model_desc* model, uint32_t v) {  
   return out;
 }
 
+/* ------------------------------------------------------------------------
+ * Animation sampling.  math32's Sin and Atan2 are float32 wrappers of Go's
+ * float64 math.Sin and math.Atan2 (math32/math32.go:137-139, 204-206); Go's
+ * own math package is not in the reference, so its published algorithm is
+ * restated here: Go 1.2x src/math/sin.go and atan.go/atan2.go, which follow
+ * Cephes sin.c and atan.c (S. Moshier).  Constants are the Cephes ones with
+ * the IEEE bit patterns Go's source lists beside them.  amd64 Go does not fuse
+ * multiply-adds (GOAMD64=v1), and neither does this file.
+ * ------------------------------------------------------------------------ */
+static double go_sin(double x) {
+  const double PI4A = 7.85398125648498535156e-1, PI4B = 3.77489470793079817668e-8,
+               PI4C = 2.69515142907905952645e-15;
+  static const double S[6] = {1.58962301576546568060e-10, -2.50507477628578072866e-8, 2.75573136213857245213e-6,
+                              -1.98412698295895385996e-4, 8.33333333332211858878e-3, -1.66666666666666307295e-1};
+  static const double C[6] = {-1.13585365213876817300e-11, 2.08757008419747316778e-9, -2.75573141792967388112e-7,
+                              2.48015872888517045348e-5, -1.38888888888730564116e-3, 4.16666666666665929218e-2};
+  if (x == 0 || x != x) return x;
+  if (isinf(x)) return NAN;
+  int sign = 0;
+  if (x < 0) { x = -x; sign = 1; }
+  /* Go reduces x >= 2^29 with Payne-Hanek (trig_reduce.go); slerp's
+   * arguments lie in [0, pi/2], so that branch is never taken here. */
+  if (x >= 536870912.0) return NAN;
+  uint64_t j = (uint64_t)(x * 1.2732395447351628);   /* 4/Pi rounded to float64: 0x3ff45f306dc9c883 */
+  double y = (double)j;
+  if (j & 1) { j++; y++; }
+  j &= 7;
+  double z = ((x - y * PI4A) - y * PI4B) - y * PI4C;
+  if (j > 3) { sign = !sign; j -= 4; }
+  double zz = z * z;
+  if (j == 1 || j == 2)
+    y = 1.0 - 0.5 * zz + zz * zz * ((((((C[0] * zz) + C[1]) * zz + C[2]) * zz + C[3]) * zz + C[4]) * zz + C[5]);
+  else
+    y = z + z * zz * ((((((S[0] * zz) + S[1]) * zz + S[2]) * zz + S[3]) * zz + S[4]) * zz + S[5]);
+  return sign ? -y : y;
+}
+static double go_xatan(double x) {
+  const double P0 = -8.750608600031904122785e-01, P1 = -1.615753718733365076637e+01,
+               P2 = -7.500855792314704667340e+01, P3 = -1.228866684490136173410e+02,
+               P4 = -6.485021904942025371773e+01, Q0 = +2.485846490142306297962e+01,
+               Q1 = +1.650270098316988542046e+02, Q2 = +4.328810604912902668951e+02,
+               Q3 = +4.853903996359136964868e+02, Q4 = +1.945506571482613964425e+02;
+  double z = x * x;
+  z = z * ((((P0 * z + P1) * z + P2) * z + P3) * z + P4) / (((((z + Q0) * z + Q1) * z + Q2) * z + Q3) * z + Q4);
+  return x * z + x;
+}
+static double go_satan(double x) {
+  const double PIO2 = 1.5707963267948966, PIO4 = 0.7853981633974483;   /* Pi/2, Pi/4 rounded */
+  const double Morebits = 6.123233995736765886130e-17, Tan3pio8 = 2.41421356237309504880;
+  if (x <= 0.66) return go_xatan(x);
+  if (x > Tan3pio8) return PIO2 - go_xatan(1 / x) + Morebits;
+  return PIO4 + go_xatan((x - 1) / (x + 1)) + 0.5 * Morebits;
+}
+static double go_atan(double x) {
+  if (x == 0) return x;
+  return x > 0 ? go_satan(x) : -go_satan(-x);
+}
+static double go_atan2(double y, double x) {
+  const double PI = 3.141592653589793, PIO2 = 1.5707963267948966, PIO4 = 0.7853981633974483,
+               PI3O4 = 2.356194490192345;                                /* 3*Pi/4 rounded */
+  if (y != y || x != x) return NAN;
+  if (y == 0) return (x >= 0 && !signbit(x)) ? copysign(0.0, y) : copysign(PI, y);
+  if (x == 0) return copysign(PIO2, y);
+  if (isinf(x)) {
+    if (x > 0) return isinf(y) ? copysign(PIO4, y) : copysign(0.0, y);
+    return isinf(y) ? copysign(PI3O4, y) : copysign(PI, y);
+  }
+  if (isinf(y)) return copysign(PIO2, y);
+  double q = go_atan(y / x);
+  if (x < 0) return q <= 0 ? q + PI : q - PI;
+  return q;
+}
+static float m32_sin(float x) { return (float)go_sin((double)x); }                     /* math32.go:137-139 */
+static float m32_atan2(float y, float x) { return (float)go_atan2((double)y, (double)x); } /* math32.go:204-206 */
+
+typedef struct { float x, y, z, w; } oquat;
+/* Quaternion.Slerp: quaternion.go:26-59. */
+static oquat q_slerp(oquat q, oquat e, float percent) {
+  float cosOmega = q.x * e.x + q.y * e.y + q.z * e.z + q.w * e.w;   /* Dot, quaternion.go:82-84 */
+  if (cosOmega < 0) {
+    e.w *= -1; e.x *= -1; e.y *= -1; e.z *= -1;
+    cosOmega *= -1;
+  }
+  float k0, k1;
+  if (cosOmega > 0.9999f) {
+    k0 = 1 - percent;
+    k1 = percent;
+  } else {
+    float sinOmega = sqrt32(1 - (cosOmega * cosOmega));
+    float omega = m32_atan2(sinOmega, cosOmega);
+    float oneOverSinOmega = 1 / sinOmega;
+    k0 = m32_sin((1 - percent) * omega) * oneOverSinOmega;
+    k1 = m32_sin(percent * omega) * oneOverSinOmega;
+  }
+  q.w = q.w * k0 + e.w * k1;
+  q.x = q.x * k0 + e.x * k1;
+  q.y = q.y * k0 + e.y * k1;
+  q.z = q.z * k0 + e.z * k1;
+  return q;
+}
+/* Quaternion.ToMatrix4: quaternion.go:135-183, the product of two matrices. */
+static m4 q_to_m4(oquat q) {
+  m4 a = m4_identity(), b = m4_identity();
+  a.m[0][0] = q.w;  a.m[0][1] = q.z;  a.m[0][2] = -q.y; a.m[0][3] = q.x;
+  a.m[1][0] = -q.z; a.m[1][1] = q.w;  a.m[1][2] = q.x;  a.m[1][3] = q.y;
+  a.m[2][0] = q.y;  a.m[2][1] = -q.x; a.m[2][2] = q.w;  a.m[2][3] = q.z;
+  a.m[3][0] = -q.x; a.m[3][1] = -q.y; a.m[3][2] = -q.z; a.m[3][3] = q.w;
+  b.m[0][0] = q.w;  b.m[0][1] = q.z;  b.m[0][2] = -q.y; b.m[0][3] = -q.x;
+  b.m[1][0] = -q.z; b.m[1][1] = q.w;  b.m[1][2] = q.x;  b.m[1][3] = -q.y;
+  b.m[2][0] = q.y;  b.m[2][1] = -q.x; b.m[2][2] = q.w;  b.m[2][3] = -q.z;
+  b.m[3][0] = q.x;  b.m[3][1] = q.y;  b.m[3][2] = q.z;  b.m[3][3] = q.w;
+  return m4_mult(&a, &b);
+}
+/* Keyframe search shared by both samplers (animation.go:69-93, 122-147):
+ * returns 0 (no keyframes), 1 (use keyframe *k0 as is) or 2 (interpolate
+ * between *k0 and *k0 + 1 at *t). */
+static int track_find(const t3d_anim_track* tr, float time, uint32_t* k0, float* t) {
+  const uint32_t n = tr->keyframe_count;
+  if (n == 0) return 0;
+  if (time <= tr->times[0]) { *k0 = 0; return 1; }
+  if (time >= tr->times[n - 1]) { *k0 = n - 1; return 1; }
+  uint32_t last = 0;
+  while (tr->times[last] < time) last++;               /* first keyframe at or after time */
+  const uint32_t first = last - 1;
+  if (time == tr->times[first]) { *k0 = first; return 1; }
+  if (time == tr->times[last]) { *k0 = last; return 1; }
+  if (tr->interpolation == T3D_INTERPOLATION_CONSTANT) { *k0 = first; return 1; }
+  *k0 = first;
+  *t = (time - tr->times[first]) / (tr->times[last] - tr->times[first]);
+  return 2;
+}
+static int track_vec(const t3d_anim_track* tr, float time, v3* out) {        /* ValueAsVector, animation.go:63-110 */
+  uint32_t k;
+  float t;
+  const int r = track_find(tr, time, &k, &t);
+  if (!r) return 0;
+  const float* a = tr->values + 3 * (size_t)k;
+  v3 fd = {a[0], a[1], a[2]};
+  if (r == 1) { *out = fd; return 1; }
+  v3 ld = {a[3], a[4], a[5]};
+  v3 o = {fd.x + ((ld.x - fd.x) * t), fd.y + ((ld.y - fd.y) * t), fd.z + ((ld.z - fd.z) * t)};   /* vector.go:559-565 */
+  *out = o;
+  return 1;
+}
+static int track_quat(const t3d_anim_track* tr, float time, oquat* out) {    /* ValueAsQuaternion, animation.go:116-160 */
+  uint32_t k;
+  float t;
+  const int r = track_find(tr, time, &k, &t);
+  if (!r) return 0;
+  const float* a = tr->values + 4 * (size_t)k;
+  oquat fd = {a[0], a[1], a[2], a[3]};
+  if (r == 1) { *out = fd; return 1; }
+  oquat ld = {a[4], a[5], a[6], a[7]};
+  *out = q_slerp(fd, ld, t);
+  return 1;
+}
+/* AnimationPlayer.forceUpdate without a blend and with absolute motion
+ * (animation.go:902-983): each present track sets the bone's local
+ * position, scale or rotation (SetLocalRotation ignores an all-zero matrix,
+ * node.go:906-912). */
+static void animate_poses(const t3d_anim_channel* ch, uint32_t nch, float time, t3d_bone_pose* poses, uint32_t n) {
+  for (uint32_t b = 0; b < n && b < nch; b++) {
+    v3 v;
+    oquat q;
+    if (track_vec(&ch[b].position, time, &v)) {
+      poses[b].position[0] = v.x; poses[b].position[1] = v.y; poses[b].position[2] = v.z;
+    }
+    if (track_vec(&ch[b].scale, time, &v)) {
+      poses[b].scale[0] = v.x; poses[b].scale[1] = v.y; poses[b].scale[2] = v.z;
+    }
+    if (track_quat(&ch[b].rotation, time, &q)) {
+      const m4 r = q_to_m4(q);
+      int zero = 1;
+      for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) zero &= r.m[i][j] == 0;
+      if (!zero) poses[b].rotation = r;
+    }
+  }
+}
+
 /* Node.Transform (node.go:572-608) down an armature, bones listed parents
  * first: Scale(s).Mult(rotation).Mult(Translate(p)), then .Mult(parent's
  * Transform()) (the armature parent's for a root bone, when it has one), and
@@ -954,6 +1186,11 @@ static int validate_model(t3do_ctx* c, const t3d_model_desc* model, const omesh*
   if (model->skinned) {
     if (!m->bones || !m->weights || (!model->bone_influences && !model->bone_poses))
       return set_err(c, "skinned model without skin data");
+    if (model->animation) {
+      if (!model->bone_poses) return set_err(c, "animation without bone poses");
+      if (!find_anim(c, model->animation)) return set_err(c, "unknown animation");
+      if (model->animation_time != model->animation_time) return set_err(c, "animation time is NaN");
+    }
     if (model->bone_poses)
       for (uint32_t b = 0; b < model->bone_count; b++)
         if (model->bone_poses[b].parent < -1 || model->bone_poses[b].parent >= (int32_t)b)
@@ -1379,8 +1616,19 @@ int t3do_render(t3do_ctx* c, const t3d_render_desc* desc, t3do_result* res) {
         t3d_model_desc posed;
         m4* pal = NULL;
         if (model->skinned && model->bone_poses) {
-          pal = (m4*)malloc(sizeof(m4) * 2 * (model->bone_count ? model->bone_count : 1));
-          bone_palettes(model->bone_poses, model->bone_count, model->armature_parent, pal + model->bone_count, pal);
+          const uint32_t nb = model->bone_count;
+          pal = (m4*)malloc(sizeof(m4) * 2 * (nb ? nb : 1));
+          const t3d_bone_pose* poses = model->bone_poses;
+          t3d_bone_pose* animated = NULL;
+          if (model->animation) {
+            const oanim* an = find_anim(c, model->animation);
+            animated = (t3d_bone_pose*)malloc(sizeof(t3d_bone_pose) * (nb ? nb : 1));
+            memcpy(animated, poses, sizeof(t3d_bone_pose) * nb);
+            animate_poses(an->ch, an->n, model->animation_time, animated, nb);
+            poses = animated;
+          }
+          bone_palettes(poses, nb, model->armature_parent, pal + nb, pal);
+          free(animated);
           posed = *model;
           posed.bone_influences = pal;
           model = &posed;
@@ -1428,6 +1676,20 @@ void t3do_result_free(t3do_result* r) {
 /* ------------------------------------------------------------------------ */
 
 void t3do_mat4_mult(const t3d_mat4* a, const t3d_mat4* b, t3d_mat4* out) { *out = m4_mult(a, b); }
+void t3do_animate_poses(const t3d_anim_channel* ch, uint32_t nch, float time, t3d_bone_pose* poses, uint32_t n) {
+  animate_poses(ch, nch, time, poses, n);
+}
+double t3do_go_sin(double x) { return go_sin(x); }
+double t3do_go_atan2(double y, double x) { return go_atan2(y, x); }
+void t3do_quat_slerp(const float a[4], const float b[4], float percent, float out[4]) {
+  oquat p = {a[0], a[1], a[2], a[3]}, q = {b[0], b[1], b[2], b[3]};
+  oquat r = q_slerp(p, q, percent);
+  out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = r.w;
+}
+void t3do_quat_to_mat4(const float q[4], t3d_mat4* out) {
+  oquat p = {q[0], q[1], q[2], q[3]};
+  *out = q_to_m4(p);
+}
 int t3do_bone_palettes(const t3d_bone_pose* poses, uint32_t n, const t3d_mat4* root, t3d_mat4* out) {
   m4* world = (m4*)malloc(sizeof(m4) * (n ? n : 1));
   const int r = bone_palettes(poses, n, root, world, out);

@@ -39,6 +39,7 @@ void t3do_destroy(t3do_ctx* ctx);
 /* Register a mesh / light volume under a caller-chosen handle (copies data). */
 int t3do_mesh_register(t3do_ctx* ctx, uint64_t handle, const t3d_mesh_desc* desc);
 int t3do_volume_register(t3do_ctx* ctx, uint64_t handle, const float* cells_rgba, uint32_t cell_count);
+int t3do_animation_register(t3do_ctx* ctx, uint64_t handle, const t3d_anim_channel* channels, uint32_t channel_count);
 /* Camera.Render restated.  Fills *out (free with t3do_result_free). */
 int t3do_render(t3do_ctx* ctx, const t3d_render_desc* desc, t3do_result* out);
 void t3do_result_free(t3do_result* out);
@@ -49,6 +50,13 @@ void t3do_mat4_mult(const t3d_mat4* a, const t3d_mat4* b, t3d_mat4* out);
 /* boneInfluence of every bone of an armature from its poses (node.go:572-608);
  * root may be NULL.  Returns 0, or -1 for a parent that is not an earlier bone. */
 int t3do_bone_palettes(const t3d_bone_pose* poses, uint32_t n, const t3d_mat4* root, t3d_mat4* out);
+/* AnimationPlayer sampling at `time` applied to poses in place (animation.go:63-160, 902-983). */
+void t3do_animate_poses(const t3d_anim_channel* ch, uint32_t nch, float time, t3d_bone_pose* poses, uint32_t n);
+/* Go's float64 math.Sin and math.Atan2 as restated; Quaternion.Slerp and ToMatrix4. */
+double t3do_go_sin(double x);
+double t3do_go_atan2(double y, double x);
+void t3do_quat_slerp(const float a[4], const float b[4], float percent, float out[4]);
+void t3do_quat_to_mat4(const float q[4], t3d_mat4* out);
 void t3do_mat4_inverted(const t3d_mat4* a, t3d_mat4* out);
 void t3do_mat4_transposed(const t3d_mat4* a, t3d_mat4* out);
 void t3do_mat4_decompose(const t3d_mat4* a, float pos[3], float scale[3], t3d_mat4* rot);

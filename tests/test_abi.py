@@ -23,7 +23,7 @@ def test_struct_sizes_match_ctypes():
     from tetra3d_amd import serving as S
     mirror = [A.Vertex, A.MeshDesc, A.MaterialDesc, A.LightDesc, A.ModelDesc, A.CameraDesc, A.WorldDesc,
               A.RenderDesc, A.PairResult, A.KernelTime, A.Tuning, S.ServerConfig, S.JobInfo, S.ServerStats,
-              A.BonePose]
+              A.BonePose, A.AnimTrack, A.AnimChannel]
     assert sizes == [C.sizeof(s) for s in mirror]
     assert sizes[0] == 48  # ebiten.Vertex: 12 float32
 

@@ -2,8 +2,9 @@
 the oracle, bit for bit: both execution forms, armatures in and past LDS
 and past the level-by-level walk (5,000 bones), a palette shared by two
 render pairs, frames that change only the pose (the captured chain replayed
-with new poses), equality with host palettes, the parent check, and posed
-jobs in the serving mix."""
+with new poses), equality with host palettes, the parent check, animations
+sampled on the device (AnimationPlayer.updateValues, animation.go:674-720)
+and their refused inputs, and posed and animated jobs in the serving mix."""
 import numpy as np
 import pytest
 
@@ -78,9 +79,75 @@ def test_parent_after_child_is_refused(gpu_ctx, oracle):
     compare_frames(gpu_ctx.render_workload(wl), oracle.render_workload(wl))
 
 
+def animated_scene(seed):
+    """posed_scene with every posed armature playing an animation with fewer
+    channels than it has bones (so the last bones keep their poses)."""
+    wl = posed_scene(seed)
+    rng = W.SplitMix64(seed + 100)
+    for i, m in enumerate(wl.models):
+        if m.bone_poses is not None:
+            m.animation = W.random_animation(rng.child(i), max(1, len(m.bone_poses) - 3), keys=9 + i)
+            m.animation_time = 0.3 + 0.2 * i
+    return wl
+
+
+@pytest.mark.parametrize("mode", [A.EXEC_BATCHED, A.EXEC_PLAIN], ids=["batched", "plain"])
+def test_animated_matches_oracle(gpu_ctx, oracle, mode):
+    """Playheads before, inside, on and after the keyframes, frame after
+    frame (the batched form replays its graph with new playheads)."""
+    wl = animated_scene(10)
+    for frame, t in enumerate([-0.5, 0.0, 0.125, 0.37, 0.5, 0.99, 1.0, 2.0]):
+        for i, m in enumerate(wl.models):
+            if m.animation is not None:
+                m.animation_time = t + 0.01 * i * (frame % 2)
+        print("animated frame", frame, "t", t, "mode", mode, flush=True)
+        compare_frames(gpu_ctx.render_workload(wl, exec_mode=mode), oracle.render_workload(wl))
+
+
+def test_animation_inputs_refused(gpu_ctx, oracle):
+    from tetra3d_amd.scene import Animation
+    bad = []
+    an = Animation("descending", 1)
+    an.add_track(0, "position", [0.0, 1.0, 0.5], [[0, 0, 0]] * 3)
+    bad.append(an)
+    an = Animation("nan_time", 1)
+    an.add_track(0, "rotation", [0.0, float("nan")], [[0, 0, 0, 1]] * 2)
+    bad.append(an)
+    an = Animation("interpolation", 1)
+    an.add_track(0, "scale", [0.0, 1.0], [[1, 1, 1]] * 2, interpolation=7)
+    bad.append(an)
+    for an in bad:
+        print("refused upload", an.name, flush=True)
+        with pytest.raises(T3DError) as e:
+            gpu_ctx.upload_animation(an)
+        assert e.value.status == A.T3D_ERR_INVALID_ARGUMENT, an.name
+    wl = animated_scene(11)
+    gpu_ctx.prepare(wl)
+    m = wl.models[0]
+    m.animation_time = float("nan")
+    with pytest.raises(T3DError) as e:
+        gpu_ctx.render_workload(wl)
+    assert e.value.status == A.T3D_ERR_INVALID_ARGUMENT
+    m.animation_time = 0.5
+    poses, m.bone_poses = m.bone_poses, None          # an animation needs poses
+    m.bone_influences = np.tile(np.eye(4, dtype=np.float32), (12, 1, 1))
+    with pytest.raises(T3DError) as e:
+        gpu_ctx.render_workload(wl)
+    assert e.value.status == A.T3D_ERR_INVALID_ARGUMENT
+    m.bone_poses = poses
+    gone = m.animation
+    gpu_ctx.free_animation(gone)
+    gone.handle = 12345678                            # a handle the context does not know
+    with pytest.raises(T3DError) as e:
+        gpu_ctx.render_workload(wl)
+    assert e.value.status == A.T3D_ERR_NOT_FOUND
+    gone.handle = 0
+    compare_frames(gpu_ctx.render_workload(wl), oracle.render_workload(wl))
+
+
 def test_posed_jobs_same_alone_and_in_mix(gpu_ctx):
     from tetra3d_amd.serving import Server
-    scenes = [posed_scene(5), feature_scene(6), posed_scene(7), posed_scene(8)]
+    scenes = [posed_scene(5), feature_scene(6), animated_scene(7), posed_scene(8)]
     alone = [gpu_ctx.render_workload(wl) for wl in scenes]
     inputs = [gpu_ctx.inputs(wl.camera, wl.scene, wl.lights, wl.models) for wl in scenes]
     with Server(gpu_ctx, slots=2, max_batch_jobs=4) as srv:

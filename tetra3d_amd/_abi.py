@@ -132,6 +132,22 @@ class BonePose(C.Structure):
     ]
 
 
+INTERPOLATION_LINEAR, INTERPOLATION_CONSTANT, INTERPOLATION_CUBIC = 0, 1, 2
+
+
+class AnimTrack(C.Structure):
+    _fields_ = [
+        ("keyframe_count", C.c_uint32),
+        ("interpolation", C.c_int32),
+        ("times", C.POINTER(C.c_float)),
+        ("values", C.POINTER(C.c_float)),
+    ]
+
+
+class AnimChannel(C.Structure):
+    _fields_ = [("position", AnimTrack), ("scale", AnimTrack), ("rotation", AnimTrack)]
+
+
 class ModelDesc(C.Structure):
     _fields_ = [
         ("mesh", C.c_uint64),
@@ -154,6 +170,9 @@ class ModelDesc(C.Structure):
         ("has_vertex_clip_function", C.c_int32),
         ("bone_poses", C.POINTER(BonePose)),
         ("armature_parent", C.POINTER(Mat4)),
+        ("animation", C.c_uint64),
+        ("animation_time", C.c_float),
+        ("pad_animation", C.c_uint32),
     ]
 
 

@@ -483,7 +483,8 @@ uint32_t t3d_struct_sizes(uint32_t* sizes, uint32_t max) {
                         (uint32_t)sizeof(t3d_pair_result),   (uint32_t)sizeof(t3d_kernel_time),
                         (uint32_t)sizeof(t3d_tuning),        (uint32_t)sizeof(t3d_server_config),
                         (uint32_t)sizeof(t3d_job_info),      (uint32_t)sizeof(t3d_server_stats),
-                        (uint32_t)sizeof(t3d_bone_pose)};
+                        (uint32_t)sizeof(t3d_bone_pose), (uint32_t)sizeof(t3d_anim_track),
+                        (uint32_t)sizeof(t3d_anim_channel)};
   uint32_t n = (uint32_t)(sizeof s / sizeof s[0]);
   for (uint32_t i = 0; i < n && i < max; i++) sizes[i] = s[i];
   return n;
@@ -664,6 +665,72 @@ t3d_status t3d_volume_free(t3d_ctx* c, uint64_t handle) {
   return T3D_OK;
 }
 
+t3d_status t3d_animation_upload(t3d_ctx* c, const t3d_anim_channel* ch, uint32_t n, uint64_t* out_handle) {
+  if (!c || !out_handle || (!ch && n)) return fail(c, T3D_ERR_INVALID_ARGUMENT, "animation_upload: null argument");
+  uint64_t floats = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    const t3d_anim_track* t[3] = {&ch[i].position, &ch[i].scale, &ch[i].rotation};
+    for (int k = 0; k < 3; k++) {
+      const uint32_t m = t[k]->keyframe_count;
+      if (!m) continue;
+      if (!t[k]->times || !t[k]->values)
+        return fail(c, T3D_ERR_INVALID_ARGUMENT, "animation_upload: channel %u track %d without data", i, k);
+      if (t[k]->interpolation < T3D_INTERPOLATION_LINEAR || t[k]->interpolation > T3D_INTERPOLATION_CUBIC)
+        return fail(c, T3D_ERR_INVALID_ARGUMENT, "animation_upload: channel %u interpolation %d", i, t[k]->interpolation);
+      for (uint32_t j = 0; j < m; j++)
+        if (!std::isfinite(t[k]->times[j]) || (j && t[k]->times[j] < t[k]->times[j - 1]))
+          return fail(c, T3D_ERR_INVALID_ARGUMENT, "animation_upload: channel %u track %d times not finite and ascending", i, k);
+      floats += (uint64_t)m * (k == 2 ? 5 : 4);
+    }
+  }
+  std::lock_guard<std::mutex> lk(c->mu);
+  auto an = std::make_unique<Animation>();
+  const uint64_t head = align_up(sizeof(t3d_anim_channel) * (uint64_t)n, 256);
+  const uint64_t bytes = head + 4 * floats;
+  t3d_status st = dev_alloc(c, an->buf, bytes ? bytes : 256);
+  if (st) return st;
+  // staging image of the buffer: the channel table with device addresses, then times and values
+  std::vector<uint8_t> img(bytes);
+  auto* tab = reinterpret_cast<t3d_anim_channel*>(img.data());
+  uint8_t* dev = reinterpret_cast<uint8_t*>(an->buf.ptr);
+  uint64_t off = head;
+  for (uint32_t i = 0; i < n; i++) {
+    tab[i] = ch[i];
+    t3d_anim_track* t[3] = {&tab[i].position, &tab[i].scale, &tab[i].rotation};
+    for (int k = 0; k < 3; k++) {
+      const uint32_t m = t[k]->keyframe_count, w = k == 2 ? 4 : 3;
+      if (!m) {
+        t[k]->times = t[k]->values = nullptr;
+        continue;
+      }
+      std::memcpy(img.data() + off, t[k]->times, 4ull * m);
+      t[k]->times = reinterpret_cast<const float*>(dev + off);
+      off += 4ull * m;
+      std::memcpy(img.data() + off, t[k]->values, 4ull * m * w);
+      t[k]->values = reinterpret_cast<const float*>(dev + off);
+      off += 4ull * m * w;
+    }
+  }
+  if (bytes) HIP_TRY(c, hipMemcpy(an->buf.ptr, img.data(), bytes, hipMemcpyHostToDevice));
+  an->nchannels = n;
+  an->channels = reinterpret_cast<const t3d_anim_channel*>(an->buf.ptr);
+  an->handle = c->next_handle++;
+  *out_handle = an->handle;
+  c->animations[an->handle] = std::move(an);
+  return T3D_OK;
+}
+
+t3d_status t3d_animation_free(t3d_ctx* c, uint64_t handle) {
+  if (!c) return T3D_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(c->mu);
+  auto it = c->animations.find(handle);
+  if (it == c->animations.end()) return fail(c, T3D_ERR_NOT_FOUND, "animation_free: unknown handle");
+  (void)hipDeviceSynchronize();
+  dev_free(c, it->second->buf);
+  c->animations.erase(it);
+  return T3D_OK;
+}
+
 t3d_status t3d_frame_create(t3d_ctx* c, t3d_frame** out) {
   if (!c || !out) return T3D_ERR_INVALID_ARGUMENT;
   (void)hipSetDevice(c->device);
@@ -817,6 +884,8 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
     const t3d_bone_pose* poses;
     const t3d_mat4* root;
     uint32_t n;
+    const Animation* anim;
+    float time;
   };
   std::vector<PoseJob> pose_jobs;
   std::vector<int32_t> part_pose_job;   // per part: its palette job, or -1
@@ -938,6 +1007,13 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
           for (uint32_t b = 0; b < md.bone_count; b++)
             if (md.bone_poses[b].parent < -1 || md.bone_poses[b].parent >= (int32_t)b)
               return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u bone %u parent is not an earlier bone", mi, b);
+        if (md.animation) {
+          if (!md.bone_poses) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u animation without bone poses", mi);
+          if (!c->animations.count(md.animation))
+            return fail(c, T3D_ERR_NOT_FOUND, "render: model %u references unknown animation %llu", mi,
+                        (unsigned long long)md.animation);
+          if (std::isnan(md.animation_time)) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u animation time is NaN", mi);
+        }
         if (mesh.max_bone_index >= md.bone_count) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u bone index out of palette", mi);
       }
       if (md.light_group_active && md.light_group_count && !md.light_group)
@@ -1155,7 +1231,8 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
         auto it = model_pose_job.find(rp.model);
         if (it == model_pose_job.end()) {
           it = model_pose_job.emplace(rp.model, (uint32_t)pose_jobs.size()).first;
-          pose_jobs.push_back({md.bone_poses, md.armature_parent, md.bone_count});
+          const Animation* an = md.animation ? c->animations.at(md.animation).get() : nullptr;
+          pose_jobs.push_back({md.bone_poses, md.armature_parent, md.bone_count, an, md.animation_time});
         }
         pj = (int32_t)it->second;
       }
@@ -1284,6 +1361,10 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
       H.palette = d_pal + pal_off;
       H.world_off = bone_off;
       H.n = J.n;
+      H.time = J.time;
+      H.channels = J.anim ? J.anim->channels : nullptr;
+      H.nchannels = J.anim ? J.anim->nchannels : 0;
+      H.pad = 0;
       H.root = nullptr;
       if (J.root) {
         h_roots[root_off] = *J.root;

@@ -129,7 +129,10 @@ struct PaletteJob {
   const M4* root;               // Transform() of the root bones' parent node, or null
   float* palette;               // [n] x 16 floats
   uint64_t world_off;
-  uint32_t n, pad;
+  uint32_t n;
+  float time;                   // playhead the animation is sampled at
+  const t3d_anim_channel* channels;   // animation of the armature (channel b -> bone b), or null
+  uint32_t nchannels, pad;
 };
 
 struct FrameArgs {

@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include "t3d_internal.h"
+#include "t3d_anim.h"
 
 namespace t3d {
 
@@ -335,14 +336,23 @@ constexpr uint32_t kLdsPoseBones = 256;     // 16 KB of LDS
 constexpr uint32_t kLevelPoseBones = 4096;  // armatures walked level by level up to this size
 constexpr uint32_t kPoseThreads = 256;
 
+// Local transform of bone b: Scale·rotation·Translate of its pose, after the
+// armature's animation (if any) has set the channels it samples.
+__device__ __forceinline__ M4 bone_local(const PaletteJob& J, uint32_t b) {
+  const t3d_bone_pose& B = J.poses[b];
+  float pos[3] = {B.position[0], B.position[1], B.position[2]};
+  float scl[3] = {B.scale[0], B.scale[1], B.scale[2]};
+  M4 rot = B.rotation;
+  if (b < J.nchannels) animate_bone(J.channels[b], J.time, pos, scl, rot);
+  return mult(mult(scale4(scl[0], scl[1], scl[2]), rot), translate(pos[0], pos[1], pos[2]));
+}
+
 template <typename WPtr>
 __device__ __forceinline__ void pose_armature(const PaletteJob& J, WPtr W, int16_t* par, uint8_t* state) {
   const uint32_t n = J.n;
   for (uint32_t b = threadIdx.x; b < n; b += kPoseThreads) {
-    const t3d_bone_pose& B = J.poses[b];
-    const M4 sr = mult(scale4(B.scale[0], B.scale[1], B.scale[2]), B.rotation);
-    W[b] = mult(sr, translate(B.position[0], B.position[1], B.position[2]));
-    par[b] = (int16_t)B.parent;                      // -1 <= parent < b, checked by the host
+    W[b] = bone_local(J, b);
+    par[b] = (int16_t)J.poses[b].parent;            // -1 <= parent < b, checked by the host
     state[b] = 0;                                    // 0 waiting, 1 ready this round, 2 done
   }
   __syncthreads();
@@ -376,11 +386,7 @@ __device__ __forceinline__ void pose_armature(const PaletteJob& J, WPtr W, int16
 // Armatures past kLevelPoseBones: the parent chain walked in bone order by
 // one thread (a parent is always an earlier bone).
 __device__ void pose_armature_serial(const PaletteJob& J, M4* W) {
-  for (uint32_t b = threadIdx.x; b < J.n; b += kPoseThreads) {
-    const t3d_bone_pose& B = J.poses[b];
-    const M4 sr = mult(scale4(B.scale[0], B.scale[1], B.scale[2]), B.rotation);
-    W[b] = mult(sr, translate(B.position[0], B.position[1], B.position[2]));
-  }
+  for (uint32_t b = threadIdx.x; b < J.n; b += kPoseThreads) W[b] = bone_local(J, b);
   __syncthreads();
   if (threadIdx.x == 0)
     for (uint32_t b = 0; b < J.n; b++) {

@@ -52,6 +52,13 @@ struct Volume {
   std::vector<float> host;         // for per-object lookups on the host
 };
 
+struct Animation {
+  uint64_t handle = 0;
+  DevBuf buf;                       // channels (device pointers) followed by their times and values
+  uint32_t nchannels = 0;
+  const t3d_anim_channel* channels = nullptr;   // device address
+};
+
 }  // namespace t3d
 
 struct t3d_frame {
@@ -99,6 +106,7 @@ struct t3d_ctx {
   uint64_t next_handle = 1;
   std::map<uint64_t, std::unique_ptr<t3d::Mesh>> meshes;
   std::map<uint64_t, std::unique_ptr<t3d::Volume>> volumes;
+  std::map<uint64_t, std::unique_ptr<t3d::Animation>> animations;
   std::string err;
   t3d_tuning tuning{256, 1, 2048, 4, 1, 2, 8, 1, 0, 256};
   // Guards resources, the budget and err between callers and the serving

@@ -56,6 +56,8 @@ def lib():
     L.t3d_mesh_free.argtypes = [vp, C.c_uint64]
     L.t3d_volume_upload.argtypes = [vp, P(C.c_float), C.c_uint32, P(C.c_uint64)]
     L.t3d_volume_free.argtypes = [vp, C.c_uint64]
+    L.t3d_animation_upload.argtypes = [vp, P(A.AnimChannel), C.c_uint32, P(C.c_uint64)]
+    L.t3d_animation_free.argtypes = [vp, C.c_uint64]
     L.t3d_frame_create.argtypes = [vp, P(vp)]
     L.t3d_frame_destroy.argtypes = [vp, vp]
     L.t3d_render.argtypes = [vp, vp, P(A.RenderDesc)]
@@ -81,7 +83,8 @@ def lib():
                  "t3d_volume_free", "t3d_frame_create", "t3d_frame_destroy", "t3d_render", "t3d_frame_pairs",
                  "t3d_frame_device_outputs", "t3d_frame_download", "t3d_frame_download_triangles",
                  "t3d_profile_enable", "t3d_profile_read", "t3d_get_tuning", "t3d_set_tuning", "t3d_autotune",
-                 "t3d_selftest_fp", "t3d_selftest_copy", "t3d_debug_model_spheres"):
+                 "t3d_selftest_fp", "t3d_selftest_copy", "t3d_debug_model_spheres", "t3d_animation_upload",
+                 "t3d_animation_free"):
         getattr(L, name).restype = C.c_int32
     _lib = L
     return L
@@ -185,6 +188,17 @@ class Context:
         vol.handle = out.value
         return out.value
 
+    def upload_animation(self, anim):
+        arr, keep = anim.channels_desc()
+        out = C.c_uint64()
+        self._check(self.L.t3d_animation_upload(self.ctx, arr, len(anim.channels), C.byref(out)))
+        anim.handle = out.value
+        return out.value
+
+    def free_animation(self, anim):
+        self._check(self.L.t3d_animation_free(self.ctx, anim.handle))
+        anim.handle = 0
+
     def mesh_handle(self, mesh):
         return self._mesh_handles[id(mesh)]
 
@@ -194,6 +208,9 @@ class Context:
         for v in workload.volumes:
             if not v.handle:
                 self.upload_volume(v)
+        for an in workload.animations:
+            if not an.handle:
+                self.upload_animation(an)
 
     # -- tuning / profiling -------------------------------------------------
     def tuning(self):

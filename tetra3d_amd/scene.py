@@ -259,6 +259,8 @@ class Model(Node):
         self.bone_influences = None  # (B, 4, 4) float32: boneInfluence per bone
         self.bone_poses = None       # BonePose ctypes array: bones as node state, parents first (palette built on the device)
         self.armature_parent = None  # 4x4: Transform() of the root bones' parent node
+        self.animation = None        # Animation sampled on the device for the posed bones
+        self.animation_time = 0.0    # its playhead (AnimationPlayer.Playhead() when Update samples)
         self.shape_key_weights = None
         self.light_group = None      # list of lights when a LightGroup is active
 
@@ -286,6 +288,11 @@ class Model(Node):
             keep.append(bones)
             d.bone_count = len(bones)
             d.bone_influences = bones.ctypes.data_as(C.POINTER(A.Mat4))
+        if self.animation is not None:   # the library refuses one on a model without bone poses
+            if not self.animation.handle:
+                raise ValueError("animation %r has no handle: upload it first" % self.animation.name)
+            d.animation = self.animation.handle
+            d.animation_time = float(self.animation_time)
         if self.shape_key_weights is not None:
             w = np.ascontiguousarray(self.shape_key_weights, dtype=f32)
             keep.append(w)
@@ -305,6 +312,38 @@ class Model(Node):
             d.light_group_count = len(self.light_group)
             d.light_group = lg
         return d
+
+
+class Animation:
+    """Animation (animation.go:188-260) of an armature: channel b animates
+    bone b with up to three tracks (AnimationTrack, animation.go:40-160),
+    each (times, values, interpolation): times ascending, values (K, 3) for
+    "position" and "scale" or (K, 4) quaternions x, y, z, w for "rotation"."""
+
+    def __init__(self, name, channel_count):
+        self.name = name
+        self.channels = [dict() for _ in range(channel_count)]
+        self.handle = 0
+
+    def add_track(self, bone, kind, times, values, interpolation=A.INTERPOLATION_LINEAR):
+        width = 4 if kind == "rotation" else 3
+        t = np.ascontiguousarray(times, dtype=f32).reshape(-1)
+        v = np.ascontiguousarray(values, dtype=f32).reshape(len(t), width)
+        self.channels[bone][kind] = (t, v, int(interpolation))
+
+    def channels_desc(self):
+        """ctypes AnimChannel array (and the arrays it points into)."""
+        arr = (A.AnimChannel * max(1, len(self.channels)))()
+        keep = [arr]
+        for b, ch in enumerate(self.channels):
+            for kind, (t, v, ip) in ch.items():
+                tr = getattr(arr[b], kind)
+                tr.keyframe_count = len(t)
+                tr.interpolation = ip
+                tr.times = t.ctypes.data_as(C.POINTER(C.c_float))
+                tr.values = v.ctypes.data_as(C.POINTER(C.c_float))
+                keep += [t, v]
+        return arr, keep
 
 
 class _Light(Node):

@@ -9,7 +9,7 @@ Scenes (sizes as in the measurement plan):
   cubes_scene      — 4,096 unit cubes in a 200^3 box, 1920x1080, 10 lights
   heightfield_scene— 1024x1024 heightfield, camera inside the terrain, 18 lights
   mixed_scene      — 512 models, V log-uniform in [24, 65536], 128 skinned
-  crowd_scene      — 256 characters on one skinned mesh, 64-bone armatures as poses
+  crowd_scene      — 256 characters on one skinned mesh, 64-bone armatures as poses, animated
   serving_pool     — 10,000 models of the mixed distribution + a Poisson job trace
 """
 import math
@@ -266,6 +266,17 @@ class Workload:
         self.volumes = [l for l in lights if isinstance(l, LightVolume)]
 
     @property
+    def animations(self):
+        """The distinct animations the models play (read when rendered, so a
+        model may be given one after the workload is made)."""
+        anims = {}
+        for m in self.models:
+            an = getattr(m, "animation", None)
+            if an is not None:
+                anims[id(an)] = an
+        return list(anims.values())
+
+    @property
     def triangle_count(self):
         return int(sum(m.mesh.triangle_count for m in self.models if m.mesh is not None))
 
@@ -498,14 +509,46 @@ def armature_poses(rng, bones, place, max_angle_deg=20.0):
     return poses, np.asarray(place, dtype=f32).copy()
 
 
-def crowd_scene(seed=DEFAULT_SEED, count=256, bones=64, level=4):
+def random_animation(rng, bones, keys=24, length=1.0, name="clip"):
+    """An Animation of `bones` channels over `length` seconds: every bone
+    has a rotation track of `keys` keyframes (unit quaternions within 25
+    degrees of identity), every third bone a position track of 5 keyframes
+    (offsets within 0.05), every fifth bone a scale track of 3 keyframes; one
+    bone in seven steps (constant interpolation) and one in eleven has no
+    tracks at all."""
+    from .scene import Animation
+    an = Animation(name, bones)
+    times = np.linspace(0, length, keys).astype(f32)
+    for b in range(bones):
+        if b % 11 == 10:
+            continue
+        ip = A.INTERPOLATION_CONSTANT if b % 7 == 6 else A.INTERPOLATION_LINEAR
+        axes = unit_vectors(rng, keys).astype(np.float64)
+        ang = rng.floats(keys).astype(np.float64) * math.radians(25.0)
+        q = np.concatenate([axes * np.sin(ang / 2)[:, None], np.cos(ang / 2)[:, None]], axis=1)
+        if b % 4 == 3:
+            q[1::2] *= -1                                   # sign flips: the shortest-arc branch of Slerp
+        an.add_track(b, "rotation", times, q.astype(f32), ip)
+        if b % 3 == 0:
+            an.add_track(b, "position", np.linspace(0, length, 5).astype(f32),
+                         rng.uniform(-0.05, 0.05, 15).reshape(5, 3), ip)
+        if b % 5 == 0:
+            an.add_track(b, "scale", np.linspace(0, length, 3).astype(f32),
+                         rng.uniform(0.9, 1.1, 9).reshape(3, 3), ip)
+    return an
+
+
+def crowd_scene(seed=DEFAULT_SEED, count=256, bones=64, level=4, clips=8):
     """`count` characters sharing one 2,562-vertex icosphere mesh skinned to
-    `bones` bones (4 influences), each with its own armature given as poses,
-    so the palettes are built on the device every frame; a 16-wide grid 4
-    units apart in front of a 1920x1080 camera (FOV 60, far 400); ambient
-    0.2, one directional and four point lights."""
+    `bones` bones (4 influences), each with its own armature given as poses
+    and playing one of `clips` shared one-second animations at its own
+    playhead, so keyframe sampling and palettes run on the device every
+    frame; a 16-wide grid 4 units apart in front of a 1920x1080 camera (FOV
+    60, far 400); ambient 0.2, one directional and four point lights."""
     rng = SplitMix64(seed)
     mesh = icosphere(level, 1.0, name="character")
+    anims = [random_animation(rng.child(500 + k), bones, name="clip%d" % k) for k in range(clips)]
+    playheads = rng.uniform(-0.1, 1.1, count)          # before, inside and after the clips
     models = []
     for i in range(count):
         m = Model("character%d" % i, mesh)
@@ -517,6 +560,9 @@ def crowd_scene(seed=DEFAULT_SEED, count=256, bones=64, level=4):
         m.skinned = True
         m.bone_influences = None
         m.bone_poses, m.armature_parent = armature_poses(rng.child(1000 + i), bones, m.transform())
+        if clips:
+            m.animation = anims[i % clips]
+            m.animation_time = float(playheads[i])
         models.append(m)
     cam = Camera("camera", 1920, 1080)
     cam.fov = f32(60)
