@@ -90,6 +90,7 @@ FULL = {
     "heightfield_1024": lambda: W.heightfield_scene(),
     "mixed_512": lambda: W.mixed_scene(),
     "crowd_256": lambda: W.crowd_scene(),
+    "icosphere_ref_1272v": lambda: W.reference_icosphere_scene(),
 }
 
 

@@ -6,6 +6,7 @@ path and the oracle, so both see byte-identical inputs.
 
 Scenes (sizes as in the measurement plan):
   icosphere_scene  — 642-vertex shared-edge icosphere, 640x360, 3 lights
+  reference_icosphere_scene — the same on NewIcosphereMesh(2) (1,272 vertices)
   cubes_scene      — 4,096 unit cubes in a 200^3 box, 1920x1080, 10 lights
   heightfield_scene— 1024x1024 heightfield, camera inside the terrain, 18 lights
   mixed_scene      — 512 models, V log-uniform in [24, 65536], 128 skinned
@@ -122,6 +123,75 @@ def icosphere(level, radius=1.0, name="icosphere", color=None):
     uv = np.stack([0.5 + np.arctan2(p[:, 2], p[:, 0]) / (2 * np.pi), 0.5 + np.arcsin(np.clip(p[:, 1], -1, 1)) / np.pi],
                   axis=1).astype(f32)
     return Mesh(name, pos, nrm, idx.astype(np.uint32), colors=cols, uvs=uv)
+
+
+def reference_icosphere_mesh(detail_level=2, name="Icosphere", color=(1, 1, 1, 1)):
+    """NewIcosphereMesh(detail_level) as the reference builds it (mesh.go:1394-1525),
+    in float32: the 12 golden-ratio vertices, detail_level + 1 subdivisions
+    whose midpoint cache is keyed by the ordered index pair (so each edge's
+    midpoint is made once per direction: 1,272 vertices at level 2),
+    Unit() on every vertex, U/V set by index modulo 3, and AutoNormal
+    (mesh.go:755-763): each triangle's normal (mesh.go:2030-2037, 2171-2179)
+    assigned to its three vertices, the last triangle winning."""
+    def unit(x, y, z):                                  # Vector3.Unit, vector.go:258-266
+        l = f32(math.sqrt(float(f32(f32(x * x + y * y) + z * z))))
+        if l < f32(1e-8) or l == f32(1):
+            return x, y, z
+        return f32(x / l), f32(y / l), f32(z / l)
+
+    t = f32(f32(f32(1) + f32(math.sqrt(5.0))) / f32(2))
+    one, zero = f32(1), f32(0)
+    verts = [(-one, t, zero), (one, t, zero), (-one, -t, zero), (one, -t, zero),
+             (zero, -one, t), (zero, one, t), (zero, -one, -t), (zero, one, -t),
+             (t, zero, -one), (t, zero, one), (-t, zero, -one), (-t, zero, one)]
+    idx = [0, 11, 5, 0, 5, 1, 0, 1, 7, 0, 7, 10, 0, 10, 11, 1, 5, 9, 5, 11, 4, 11, 10, 2, 10, 7, 6, 7, 1, 8,
+           3, 9, 4, 3, 4, 2, 3, 2, 6, 3, 6, 8, 3, 8, 9, 4, 9, 5, 2, 4, 11, 6, 2, 10, 8, 6, 7, 9, 8, 1]
+    cache = {}
+
+    def middle(i0, i1):
+        if (i0, i1) in cache:
+            return cache[(i0, i1)]
+        a, b = verts[i0], verts[i1]
+        verts.append(tuple(f32(f32(a[k] + b[k]) / f32(2)) for k in range(3)))   # Add, then Divide(2)
+        cache[(i0, i1)] = len(verts) - 1
+        return len(verts) - 1
+
+    if detail_level > 0:
+        for _ in range(detail_level + 1):
+            new = []
+            for i in range(0, len(idx), 3):
+                ab, bc, ca = middle(idx[i], idx[i + 1]), middle(idx[i + 1], idx[i + 2]), middle(idx[i + 2], idx[i])
+                new += [idx[i], ab, ca, idx[i + 1], bc, ab, idx[i + 2], ca, bc, ab, bc, ca]
+            idx = new
+    pos = np.array([unit(*v) for v in verts], dtype=f32)
+    uv = np.zeros((len(pos), 2), dtype=f32)
+    uv[1::3, 0] = 1
+    uv[2::3, 1] = 1
+    nrm = np.zeros_like(pos)
+    tris = np.array(idx, dtype=np.uint32).reshape(-1, 3)
+    for a, b, c in tris:
+        p1, p2, p3 = pos[a], pos[b], pos[c]
+        v0 = [f32(p2[k] - p1[k]) for k in range(3)]
+        v1 = [f32(p3[k] - p2[k]) for k in range(3)]
+        cz = f32(f32(v0[0] * v1[1]) - f32(v1[0] * v0[1]))         # Vector3.Cross, vector.go:164-171
+        cy = f32(f32(v0[2] * v1[0]) - f32(v1[2] * v0[0]))
+        cx = f32(f32(v0[1] * v1[2]) - f32(v1[1] * v0[2]))
+        n = unit(cx, cy, cz)
+        if n == (0, 0, 0):
+            n = tuple(nrm[a])
+        nrm[a] = nrm[b] = nrm[c] = n
+    cols = np.tile(np.asarray(color, dtype=f32), (len(pos), 1)) if color is not None else None
+    return Mesh(name, pos, nrm, tris, colors=cols, uvs=uv)
+
+
+def reference_icosphere_scene(seed=DEFAULT_SEED, detail_level=2):
+    """The icosphere configuration on the reference's own generator,
+    NewIcosphereMesh(2): 1,272 vertices, 1,280 triangles, flat normals."""
+    wl = icosphere_scene(seed)
+    model = wl.models[0]
+    ref = Model("icosphere_ref", reference_icosphere_mesh(detail_level))
+    ref.set_local_rotation(model.rotation)
+    return Workload("icosphere_ref_1272v", wl.camera, wl.scene, wl.lights, [ref])
 
 
 def _build_icosphere_topology(level):
@@ -618,4 +688,5 @@ SCENES = {
     "heightfield": heightfield_scene,
     "mixed": mixed_scene,
     "crowd": crowd_scene,
+    "icosphere_ref": reference_icosphere_scene,
 }
