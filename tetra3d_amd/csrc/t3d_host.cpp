@@ -1522,22 +1522,78 @@ void frame_collect(t3d_ctx* c, t3d_frame* f, bool profile) {
 }
 
 t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uint32_t p1, hipStream_t stream,
-                                t3d_vertex* cv, t3d_vertex* dv, uint32_t* idx, uint32_t* tri_ids) {
+                                t3d_vertex* cv, t3d_vertex* dv, uint32_t* idx, uint32_t* tri_ids, DevBuf* pack,
+                                uint64_t* pack_cap) {
   uint64_t off = 0, toff = 0;
   uint64_t nrec = 0;
   for (uint32_t i = p0; i < p1; i++) nrec += f->pairs[i].survivor_count;
   std::vector<uint4> recs(tri_ids ? std::max<uint64_t>(nrec, 1) : 0);
-  for (uint32_t i = p0; i < p1; i++) {
-    const auto& p = f->pairs[i];
-    const uint64_t n = 3ull * p.survivor_count;
-    if (!n) continue;
-    if (cv) HIP_TRY(c, hipMemcpyAsync(cv + off, reinterpret_cast<const t3d_vertex*>(f->args.out_color) + p.vertex_offset, n * sizeof(t3d_vertex), hipMemcpyDeviceToHost, stream));
-    if (dv) HIP_TRY(c, hipMemcpyAsync(dv + off, reinterpret_cast<const t3d_vertex*>(f->args.out_depth) + p.vertex_offset, n * sizeof(t3d_vertex), hipMemcpyDeviceToHost, stream));
-    if (idx) HIP_TRY(c, hipMemcpyAsync(idx + off, f->args.out_idx + p.vertex_offset, n * 4, hipMemcpyDeviceToHost, stream));
-    if (tri_ids) HIP_TRY(c, hipMemcpyAsync(recs.data() + toff, f->args.tri_vo + f->parts[i].tbase, 16ull * p.survivor_count, hipMemcpyDeviceToHost, stream));
-    off += n;
-    toff += p.survivor_count;
+  // vertices and indices: the pairs' survivors packed on the device (k_pack), then one copy per stream
+  std::vector<uint32_t> blk(1, 0), nv;
+  std::vector<uint64_t> src, dst;
+  if (!pack) {   // one copy per pair and stream
+    for (uint32_t i = p0; i < p1; i++) {
+      const auto& p = f->pairs[i];
+      const uint64_t n = 3ull * p.survivor_count;
+      if (!n) continue;
+      const t3d_vertex* oc = reinterpret_cast<const t3d_vertex*>(f->args.out_color) + p.vertex_offset;
+      const t3d_vertex* od = reinterpret_cast<const t3d_vertex*>(f->args.out_depth) + p.vertex_offset;
+      if (cv) HIP_TRY(c, hipMemcpyAsync(cv + off, oc, n * sizeof(t3d_vertex), hipMemcpyDeviceToHost, stream));
+      if (dv) HIP_TRY(c, hipMemcpyAsync(dv + off, od, n * sizeof(t3d_vertex), hipMemcpyDeviceToHost, stream));
+      if (idx) HIP_TRY(c, hipMemcpyAsync(idx + off, f->args.out_idx + p.vertex_offset, n * 4, hipMemcpyDeviceToHost, stream));
+      off += n;
+    }
+    off = 0;
+  } else if (cv || dv || idx)
+    for (uint32_t i = p0; i < p1; i++) {
+      const uint64_t n = 3ull * f->pairs[i].survivor_count;
+      if (!n) continue;
+      src.push_back(f->pairs[i].vertex_offset);
+      dst.push_back(off);
+      nv.push_back((uint32_t)n);
+      blk.push_back(blk.back() + (uint32_t)((n + 255) / 256));
+      off += n;
+    }
+  if (off) {
+    const uint32_t np = (uint32_t)nv.size();
+    const uint64_t bytes = carve_size<uint32_t>(np + 1) + 2 * carve_size<uint64_t>(np) + carve_size<uint32_t>(np) +
+                           2 * carve_size<t3d_vertex>(off) + carve_size<uint32_t>(off);
+    t3d_status st = ensure(c, *pack, *pack_cap, bytes);
+    if (st) return st;
+    Carve pc{reinterpret_cast<uint8_t*>(pack->ptr)};
+    PackArgs pa{};
+    uint32_t* d_blk = pc.take<uint32_t>(np + 1);
+    uint64_t* d_src = pc.take<uint64_t>(np);
+    uint64_t* d_dst = pc.take<uint64_t>(np);
+    uint32_t* d_nv = pc.take<uint32_t>(np);
+    HIP_TRY(c, hipMemcpyAsync(d_blk, blk.data(), 4ull * (np + 1), hipMemcpyHostToDevice, stream));
+    HIP_TRY(c, hipMemcpyAsync(d_src, src.data(), 8ull * np, hipMemcpyHostToDevice, stream));
+    HIP_TRY(c, hipMemcpyAsync(d_dst, dst.data(), 8ull * np, hipMemcpyHostToDevice, stream));
+    HIP_TRY(c, hipMemcpyAsync(d_nv, nv.data(), 4ull * np, hipMemcpyHostToDevice, stream));
+    pa.blk_prefix = d_blk;
+    pa.src = d_src;
+    pa.dst = d_dst;
+    pa.nvert = d_nv;
+    pa.npairs = np;
+    pa.color = f->args.out_color;
+    pa.depth = f->args.out_depth;
+    pa.idx = f->args.out_idx;
+    pa.pcolor = reinterpret_cast<float4*>(pc.take<t3d_vertex>(off));
+    pa.pdepth = reinterpret_cast<float4*>(pc.take<t3d_vertex>(off));
+    pa.pidx = pc.take<uint32_t>(off);
+    HIP_TRY(c, launch_pack(pa, blk.back(), stream));
+    if (cv) HIP_TRY(c, hipMemcpyAsync(cv, pa.pcolor, off * sizeof(t3d_vertex), hipMemcpyDeviceToHost, stream));
+    if (dv) HIP_TRY(c, hipMemcpyAsync(dv, pa.pdepth, off * sizeof(t3d_vertex), hipMemcpyDeviceToHost, stream));
+    if (idx) HIP_TRY(c, hipMemcpyAsync(idx, pa.pidx, off * 4, hipMemcpyDeviceToHost, stream));
   }
+  if (tri_ids)
+    for (uint32_t i = p0; i < p1; i++) {
+      const auto& p = f->pairs[i];
+      if (!p.survivor_count) continue;
+      HIP_TRY(c, hipMemcpyAsync(recs.data() + toff, f->args.tri_vo + f->parts[i].tbase, 16ull * p.survivor_count,
+                                hipMemcpyDeviceToHost, stream));
+      toff += p.survivor_count;
+    }
   HIP_TRY(c, hipStreamSynchronize(stream));
   if (tri_ids) {
     toff = 0;
@@ -1577,7 +1633,8 @@ void frame_release(t3d_ctx* c, t3d_frame* f) {
   dev_free(c, f->tables);
   dev_free(c, f->scratch);
   dev_free(c, f->outputs);
-  f->tables_cap = f->scratch_cap = f->outputs_cap = 0;
+  dev_free(c, f->pack);
+  f->tables_cap = f->scratch_cap = f->outputs_cap = f->pack_cap = 0;
   if (f->pinned) (void)hipHostFree(f->pinned);
   if (f->pinned_stats) (void)hipHostFree(f->pinned_stats);
   f->pinned = nullptr;
@@ -1667,7 +1724,9 @@ t3d_status t3d_frame_download(t3d_ctx* c, const t3d_frame* f, t3d_vertex* cv, t3
   std::lock_guard<std::mutex> lk(c->mu);
   if (f->inflight) return fail(c, T3D_ERR_NOT_READY, "frame_download: render in flight (t3d_frame_wait first)");
   (void)hipSetDevice(c->device);
-  return frame_download_range(c, f, 0, (uint32_t)f->pairs.size(), c->stream, cv, dv, idx, nullptr);
+  t3d_frame* fm = const_cast<t3d_frame*>(f);   // the staging buffer only; the frame's results are untouched
+  return frame_download_range(c, f, 0, (uint32_t)f->pairs.size(), c->stream, cv, dv, idx, nullptr, &fm->pack,
+                              &fm->pack_cap);
 }
 
 t3d_status t3d_frame_download_triangles(t3d_ctx* c, const t3d_frame* f, uint32_t* ids) {

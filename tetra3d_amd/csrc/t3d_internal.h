@@ -171,6 +171,24 @@ struct FrameArgs {
   uint32_t npal_jobs, pad2;
 };
 
+// Packing of a range of render pairs' survivors for download: pair j of the
+// range copies its first nvert[j] output vertices (and indices) from vertex
+// src[j] of the frame outputs to vertex dst[j] of the packed arrays; blocks
+// [blk_prefix[j], blk_prefix[j+1]) cover it, 256 vertices each.
+struct PackArgs {
+  const uint32_t* blk_prefix;   // [npairs + 1]
+  const uint64_t* src;          // [npairs]
+  const uint64_t* dst;          // [npairs]
+  const uint32_t* nvert;        // [npairs]
+  uint32_t npairs, pad;
+  const float4* color;          // frame outputs, t3d_vertex as 3 float4
+  const float4* depth;
+  const uint32_t* idx;
+  float4* pcolor;               // packed
+  float4* pdepth;
+  uint32_t* pidx;
+};
+
 // Kernel launchers (t3d_kernels.hip).  blk0 offsets blockIdx so the plain
 // form can launch one part's blocks at a time.
 hipError_t launch_palettes(const FrameArgs& a, hipStream_t s);
@@ -184,6 +202,7 @@ hipError_t launch_scan_part(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hi
 hipError_t launch_scatter(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t waves, hipStream_t s);
 hipError_t launch_assemble(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t mode, uint32_t split,
                            hipStream_t s);
+hipError_t launch_pack(const PackArgs& a, uint32_t nblocks, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 hipError_t launch_fill(void* dst, uint64_t bytes, uint32_t word, hipStream_t s);
 hipError_t launch_fp_selftest(const float* a, const float* b, float* out, uint32_t n, hipStream_t s);

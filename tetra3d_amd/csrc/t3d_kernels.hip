@@ -1323,6 +1323,28 @@ __global__ __launch_bounds__(256) void k_fill(uint32_t* __restrict__ p, uint64_t
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
+// Survivors of a range of pairs packed back to back for the download (one
+// transfer per stream instead of three per pair).  A block finds its pair by
+// binary search of the block prefix: the pair p with blk_prefix[p] <= block <
+// blk_prefix[p + 1] (every listed pair has at least one vertex).
+__global__ __launch_bounds__(256) void k_pack(PackArgs a) {
+  const uint32_t b = blockIdx.x;
+  uint32_t lo = 0, hi = a.npairs;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a.blk_prefix[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  const uint64_t v = (uint64_t)(b - a.blk_prefix[lo]) * 256 + threadIdx.x;
+  if (v >= a.nvert[lo]) return;
+  const uint64_t s = a.src[lo] + v, d = a.dst[lo] + v;
+  for (int k = 0; k < 3; k++) {
+    a.pcolor[3 * d + k] = a.color[3 * s + k];
+    a.pdepth[3 * d + k] = a.depth[3 * s + k];
+  }
+  a.pidx[d] = a.idx[s];
+}
+
 // Plain float4 copy of a known size: the reference point for HBM byte counters.
 __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
@@ -1414,6 +1436,11 @@ hipError_t launch_assemble(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uin
 hipError_t launch_fill(void* dst, uint64_t bytes, uint32_t word, hipStream_t s) {
   if (bytes < 4) return hipSuccess;
   hipLaunchKernelGGL(k_fill, dim3(2048), dim3(256), 0, s, reinterpret_cast<uint32_t*>(dst), bytes / 4, word);
+  return hipGetLastError();
+}
+hipError_t launch_pack(const PackArgs& a, uint32_t nblocks, hipStream_t s) {
+  if (!nblocks) return hipSuccess;
+  hipLaunchKernelGGL(k_pack, dim3(nblocks), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s) {

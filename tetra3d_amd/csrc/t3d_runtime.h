@@ -65,6 +65,10 @@ struct t3d_frame {
   // device tables and scratch
   t3d::DevBuf tables, scratch, outputs;
   uint64_t tables_cap = 0, scratch_cap = 0, outputs_cap = 0;
+  // download staging of t3d_frame_download: the survivors packed back to back, so the copy to the host is one
+  // transfer per stream (grown on demand, kept with the frame)
+  t3d::DevBuf pack;
+  uint64_t pack_cap = 0;
   void* pinned = nullptr;          // host staging of the tables
   uint64_t pinned_cap = 0;
   uint32_t* pinned_stats = nullptr;
@@ -141,9 +145,12 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
 t3d_status frame_launch(t3d_ctx* c, t3d_frame* f, hipStream_t stream, hipStream_t copy_stream, int32_t exec_mode,
                         bool allow_graph, bool profile);
 void frame_collect(t3d_ctx* c, t3d_frame* f, bool profile);
-// Copy pairs [p0, p1) of a frame, concatenated, to host memory (any output may be null).
+// Copy pairs [p0, p1) of a frame, concatenated, to host memory (any output may be null).  With `pack`
+// (the caller holds ctx->mu) the survivors are first packed on the device into that buffer, so vertices and
+// indices cross to the host in one copy per stream; without it each pair is copied on its own.
 t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uint32_t p1, hipStream_t stream,
-                                t3d_vertex* cv, t3d_vertex* dv, uint32_t* idx, uint32_t* tri_ids);
+                                t3d_vertex* cv, t3d_vertex* dv, uint32_t* idx, uint32_t* tri_ids,
+                                t3d::DevBuf* pack = nullptr, uint64_t* pack_cap = nullptr);
 // Frame memory a job may need (scratch + outputs + tables), before frustum culling.
 uint64_t estimate_job_bytes(t3d_ctx* c, const t3d_render_desc* desc);
 
