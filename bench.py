@@ -525,8 +525,8 @@ def main():
         "latency_ms": {"p50": p50, "p99": p99, "mean": sync_ms,
                        "note": "synchronous frames: submit -> outputs complete in HBM",
                        "with_d2h_p50": d2h50, "with_d2h_p99": d2h99,
-                       "with_d2h_note": "submit -> outputs copied to pinned host memory (one copy per render "
-                                        "pair and stream, t3d_frame_download)"},
+                       "with_d2h_note": "submit -> outputs copied to pinned host memory (t3d_frame_download: "
+                                        "survivors packed on the device, one copy per stream)"},
         "pipelined": None if args.sync_only else {"ms_per_step": ms, "frames_in_flight": 2},
         "hbm": {"algorithmic_bytes_per_frame": B, "frame_pct_of_peak": 100.0 * B / (ms / 1e3) / (HBM_PEAK_GBS * 1e9),
                 "device_ms_per_frame": device_ms, "host_prologue_ms": host_ms,
