@@ -168,6 +168,16 @@ def feature_scene(seed):
         if rng.random() < 0.2 and lights:
             m.light_group = [l for l in lights if not isinstance(l, LightVolume) and rng.random() < 0.6]
         models.append(m)
+    # further options, drawn from a stream of their own so the draws above stay as they were
+    extra = np.random.default_rng(10_000 + seed)
+    for m in models:
+        m.auto_batch_static = bool(extra.random() < 0.05)           # skipped (camera.go:2067-2071)
+        for p in m.mesh.mesh_parts:
+            if p.material is not None and extra.random() < 0.15:
+                p.material.blend_source_over = False                # a transparent pair (camera.go:1420-1428)
+    for l in lights:
+        if isinstance(l, LightVolume) and extra.random() < 0.5:
+            l.offset = extra.uniform(-1.5, 1.5, 3).astype(f32)
     return W.Workload("features_%d" % seed, cam, Scene("features", world), lights, models)
 
 

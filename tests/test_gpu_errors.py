@@ -34,6 +34,11 @@ def test_invalid_render_inputs_are_refused(gpu_ctx, oracle):
         r.models[0].materials[0].visible = 1
         r.models[0].materials[0].billboard = 1
 
+    def custom_depth(r):
+        r.models[0].materials[0].present = 1
+        r.models[0].materials[0].visible = 1
+        r.models[0].materials[0].custom_depth_function = 1
+
     cases = [
         (lambda r: setattr(r.models[0], "mesh", 0x7FFFFFFF), A.T3D_ERR_NOT_FOUND),
         (lambda r: setattr(r.camera[0], "width", 0), A.T3D_ERR_INVALID_ARGUMENT),
@@ -43,6 +48,7 @@ def test_invalid_render_inputs_are_refused(gpu_ctx, oracle):
         (lambda r: setattr(r.lights[0], "type", 99), A.T3D_ERR_INVALID_ARGUMENT),
         (lambda r: setattr(r.models[0], "skinned", 1), A.T3D_ERR_INVALID_ARGUMENT),   # no skin data
         (billboard, A.T3D_ERR_UNSUPPORTED),
+        (custom_depth, A.T3D_ERR_UNSUPPORTED),
     ]
     for mutate, status in cases:
         _refused(gpu_ctx, wl, mutate, status)
