@@ -172,6 +172,7 @@ def feature_scene(seed):
     extra = np.random.default_rng(10_000 + seed)
     for m in models:
         m.auto_batch_static = bool(extra.random() < 0.05)           # skipped (camera.go:2067-2071)
+        m.visible = bool(extra.random() < 0.95)
         for p in m.mesh.mesh_parts:
             if p.material is not None and extra.random() < 0.15:
                 p.material.blend_source_over = False                # a transparent pair (camera.go:1420-1428)

@@ -8,7 +8,7 @@ from tests.feature_scenes import feature_scene
 
 def test_feature_scenes_render_and_repeat(oracle):
     sort_modes, transparency, lv_modes = set(), set(), set()
-    skinned = shape_keys = groups = ortho = batched_static = blend_off = offset = 0
+    skinned = shape_keys = groups = ortho = batched_static = blend_off = offset = hidden = 0
     for seed in range(40):
         wl = feature_scene(seed)
         a = oracle.render_workload(wl)
@@ -20,6 +20,7 @@ def test_feature_scenes_render_and_repeat(oracle):
         offset += sum(bool(np.any(getattr(l, "offset", np.zeros(3)) != 0)) for l in wl.lights)
         for m in wl.models:
             batched_static += m.auto_batch_static
+            hidden += not m.visible
             blend_off += sum(p.material is not None and not p.material.blend_source_over for p in m.mesh.mesh_parts)
             skinned += m.skinned
             shape_keys += m.shape_key_weights is not None
@@ -31,4 +32,4 @@ def test_feature_scenes_render_and_repeat(oracle):
                     lv_modes.add(p.material.light_volume_shading_mode)
     assert sort_modes == {A.SORT_BACK_TO_FRONT, A.SORT_FRONT_TO_BACK, A.SORT_NONE}
     assert len(transparency) == 4 and len(lv_modes) == 4
-    assert skinned and shape_keys and groups and ortho and batched_static and blend_off and offset
+    assert skinned and shape_keys and groups and ortho and batched_static and blend_off and offset and hidden
