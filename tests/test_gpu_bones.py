@@ -88,6 +88,10 @@ def animated_scene(seed):
         if m.bone_poses is not None:
             m.animation = W.random_animation(rng.child(i), max(1, len(m.bone_poses) - 3), keys=9 + i)
             m.animation_time = 0.3 + 0.2 * i
+            # a one-keyframe track (always that keyframe) and a zero quaternion (the pose's rotation stays)
+            m.animation.add_track(0, "scale", [0.5], [[1.25, 0.75, 1.0]])
+            if len(m.animation.channels) > 2:
+                m.animation.add_track(2, "rotation", [0.0, 1.0], [[0, 0, 0, 0], [0, 0, 0, 0]])
     return wl
 
 

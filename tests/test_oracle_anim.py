@@ -170,6 +170,13 @@ def test_keyframe_search_and_interpolation(oracle_lib):
     assert p[0].scale[:] == [1, 1, 1] and np.array_equal(np.array(A.mat4_to_list(p[0].rotation)), np.eye(4))
 
 
+def test_single_keyframe_track(oracle_lib):
+    an = Animation("single", 1)
+    an.add_track(0, "position", [0.5], [[1, 2, 3]])
+    for t in (-1.0, 0.5, 0.75, 9.0):           # before, on and after its one keyframe
+        assert oracle_lib.animate_poses(an, t, _poses(1))[0].position[:] == [1, 2, 3]
+
+
 def test_zero_quaternion_keeps_rotation(oracle_lib):
     an = Animation("zero", 1)
     an.add_track(0, "rotation", [0.0, 1.0], [(0, 0, 0, 0), (0, 0, 0, 0)])
