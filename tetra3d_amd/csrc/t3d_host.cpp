@@ -1551,7 +1551,7 @@ t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uin
       src.push_back(f->pairs[i].vertex_offset);
       dst.push_back(off);
       nv.push_back((uint32_t)n);
-      blk.push_back(blk.back() + (uint32_t)((n + 255) / 256));
+      blk.push_back(blk.back() + (uint32_t)((3 * n + 255) / 256));   // 3 float4 per vertex, 256 to a block
       off += n;
     }
   if (off) {

@@ -174,7 +174,7 @@ struct FrameArgs {
 // Packing of a range of render pairs' survivors for download: pair j of the
 // range copies its first nvert[j] output vertices (and indices) from vertex
 // src[j] of the frame outputs to vertex dst[j] of the packed arrays; blocks
-// [blk_prefix[j], blk_prefix[j+1]) cover it, 256 vertices each.
+// [blk_prefix[j], blk_prefix[j+1]) cover its 3 * nvert float4 words, 256 each.
 struct PackArgs {
   const uint32_t* blk_prefix;   // [npairs + 1]
   const uint64_t* src;          // [npairs]

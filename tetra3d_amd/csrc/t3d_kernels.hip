@@ -1326,7 +1326,8 @@ __global__ __launch_bounds__(256) void k_fill(uint32_t* __restrict__ p, uint64_t
 // Survivors of a range of pairs packed back to back for the download (one
 // transfer per stream instead of three per pair).  A block finds its pair by
 // binary search of the block prefix: the pair p with blk_prefix[p] <= block <
-// blk_prefix[p + 1] (every listed pair has at least one vertex).
+// blk_prefix[p + 1] (every listed pair has at least one vertex, and 3 * nvert
+// elements, 256 to a block).
 __global__ __launch_bounds__(256) void k_pack(PackArgs a) {
   const uint32_t b = blockIdx.x;
   uint32_t lo = 0, hi = a.npairs;
@@ -1335,14 +1336,15 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs a) {
     if (a.blk_prefix[mid] <= b) lo = mid;
     else hi = mid;
   }
-  const uint64_t v = (uint64_t)(b - a.blk_prefix[lo]) * 256 + threadIdx.x;
-  if (v >= a.nvert[lo]) return;
-  const uint64_t s = a.src[lo] + v, d = a.dst[lo] + v;
-  for (int k = 0; k < 3; k++) {
-    a.pcolor[3 * d + k] = a.color[3 * s + k];
-    a.pdepth[3 * d + k] = a.depth[3 * s + k];
-  }
-  a.pidx[d] = a.idx[s];
+  // element e of the pair: float4 e of its colour and depth runs (3 per vertex), and index e while e < nvert,
+  // so consecutive lanes move consecutive 16-byte words
+  const uint64_t e = (uint64_t)(b - a.blk_prefix[lo]) * 256 + threadIdx.x;
+  const uint64_t n = a.nvert[lo];
+  if (e >= 3 * n) return;
+  const uint64_t s = a.src[lo], d = a.dst[lo];
+  a.pcolor[3 * d + e] = a.color[3 * s + e];
+  a.pdepth[3 * d + e] = a.depth[3 * s + e];
+  if (e < n) a.pidx[d + e] = a.idx[s + e];
 }
 
 // Plain float4 copy of a known size: the reference point for HBM byte counters.
