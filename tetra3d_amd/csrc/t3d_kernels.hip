@@ -334,7 +334,7 @@ __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
 // transforms too up to kLdsPoseBones, in device memory past that.
 constexpr uint32_t kLdsPoseBones = 256;     // 16 KB of LDS
 constexpr uint32_t kLevelPoseBones = 4096;  // armatures walked level by level up to this size
-constexpr uint32_t kPoseThreads = 256;
+constexpr uint32_t kPoseThreads = 64;
 
 // Local transform of bone b: Scale·rotation·Translate of its pose, after the
 // armature's animation (if any) has set the channels it samples.
