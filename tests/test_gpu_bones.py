@@ -165,3 +165,27 @@ def test_posed_jobs_same_alone_and_in_mix(gpu_ctx):
             compare_frames(r, alone[k])
             srv.release(i)
     print("posed jobs in shared batches", shared, flush=True)
+
+
+def test_submitted_poses_are_copied(gpu_ctx):
+    """The server copies a job's poses, armature parent and playhead at
+    submit: changing them afterwards leaves the job's result as it was."""
+    from tetra3d_amd.serving import Server
+    wl = animated_scene(12)
+    before = gpu_ctx.render_workload(wl)
+    inp = gpu_ctx.inputs(wl.camera, wl.scene, wl.lights, wl.models)
+    with Server(gpu_ctx, slots=1, max_batch_jobs=2) as srv:
+        job = srv.submit_inputs(inp)
+        for m in wl.models:                       # scribble over what the descriptor pointed to
+            if m.bone_poses is not None:
+                for b in range(len(m.bone_poses)):
+                    m.bone_poses[b].position[:] = [100.0, 100.0, 100.0]
+        for i in range(inp.render.model_count):
+            md = inp.render.models[i]
+            if md.armature_parent:
+                md.armature_parent[0].m[3][0] = 500.0
+            md.animation_time = 7.0
+        srv.wait(job)
+        r = srv.download(job)
+        compare_frames(r, before)
+        srv.release(job)
