@@ -1521,6 +1521,23 @@ void frame_collect(t3d_ctx* c, t3d_frame* f, bool profile) {
   if (profile && c->profiling) c->prof["k_assemble"].bytes += 300ull * total;   // 3 x (48 B colour + 48 B depth + 4 B index)
 }
 
+// Packing buffer layout: block prefix, source and destination offsets and vertex counts of np pairs, then the
+// packed colour and depth vertices and indices of `verts` corners.
+static uint64_t pack_layout_bytes(uint64_t np, uint64_t verts) {
+  return carve_size<uint32_t>(np + 1) + 2 * carve_size<uint64_t>(np) + carve_size<uint32_t>(np) +
+         2 * carve_size<t3d_vertex>(verts) + carve_size<uint32_t>(verts);
+}
+
+uint64_t download_pack_bytes(const t3d_frame* f, uint32_t p0, uint32_t p1) {
+  uint64_t np = 0, verts = 0;
+  for (uint32_t i = p0; i < p1; i++)
+    if (f->pairs[i].survivor_count) {
+      np++;
+      verts += 3ull * f->pairs[i].survivor_count;
+    }
+  return verts ? pack_layout_bytes(np, verts) : 0;
+}
+
 t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uint32_t p1, hipStream_t stream,
                                 t3d_vertex* cv, t3d_vertex* dv, uint32_t* idx, uint32_t* tri_ids, DevBuf* pack,
                                 uint64_t* pack_cap) {
@@ -1531,20 +1548,8 @@ t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uin
   // vertices and indices: the pairs' survivors packed on the device (k_pack), then one copy per stream
   std::vector<uint32_t> blk(1, 0), nv;
   std::vector<uint64_t> src, dst;
-  if (!pack) {   // one copy per pair and stream
-    for (uint32_t i = p0; i < p1; i++) {
-      const auto& p = f->pairs[i];
-      const uint64_t n = 3ull * p.survivor_count;
-      if (!n) continue;
-      const t3d_vertex* oc = reinterpret_cast<const t3d_vertex*>(f->args.out_color) + p.vertex_offset;
-      const t3d_vertex* od = reinterpret_cast<const t3d_vertex*>(f->args.out_depth) + p.vertex_offset;
-      if (cv) HIP_TRY(c, hipMemcpyAsync(cv + off, oc, n * sizeof(t3d_vertex), hipMemcpyDeviceToHost, stream));
-      if (dv) HIP_TRY(c, hipMemcpyAsync(dv + off, od, n * sizeof(t3d_vertex), hipMemcpyDeviceToHost, stream));
-      if (idx) HIP_TRY(c, hipMemcpyAsync(idx + off, f->args.out_idx + p.vertex_offset, n * 4, hipMemcpyDeviceToHost, stream));
-      off += n;
-    }
-    off = 0;
-  } else if (cv || dv || idx)
+  if ((cv || dv || idx) && !pack) return fail(c, T3D_ERR_INVALID_ARGUMENT, "download: no packing buffer");
+  if (cv || dv || idx)
     for (uint32_t i = p0; i < p1; i++) {
       const uint64_t n = 3ull * f->pairs[i].survivor_count;
       if (!n) continue;
@@ -1556,9 +1561,7 @@ t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uin
     }
   if (off) {
     const uint32_t np = (uint32_t)nv.size();
-    const uint64_t bytes = carve_size<uint32_t>(np + 1) + 2 * carve_size<uint64_t>(np) + carve_size<uint32_t>(np) +
-                           2 * carve_size<t3d_vertex>(off) + carve_size<uint32_t>(off);
-    t3d_status st = ensure(c, *pack, *pack_cap, bytes);
+    t3d_status st = ensure(c, *pack, *pack_cap, pack_layout_bytes(np, off));
     if (st) return st;
     Carve pc{reinterpret_cast<uint8_t*>(pack->ptr)};
     PackArgs pa{};

@@ -132,10 +132,14 @@ namespace t3d {
 
 t3d_status fail(t3d_ctx* c, t3d_status st, const char* fmt, ...);
 t3d_status dev_alloc(t3d_ctx* c, DevBuf& b, uint64_t bytes);
+// Grow `b` to at least `bytes` (contents discarded); the caller holds ctx->mu.
+t3d_status ensure(t3d_ctx* c, DevBuf& b, uint64_t& cap, uint64_t bytes);
 void dev_free(t3d_ctx* c, DevBuf& b);
 void frame_release(t3d_ctx* c, t3d_frame* f);
-// Device bytes a frame holds between renders (tables, scratch, outputs).
-inline uint64_t frame_capacity(const t3d_frame* f) { return f->tables.bytes + f->scratch.bytes + f->outputs.bytes; }
+// Device bytes a frame holds between renders (tables, scratch, outputs, download packing).
+inline uint64_t frame_capacity(const t3d_frame* f) {
+  return f->tables.bytes + f->scratch.bytes + f->outputs.bytes + f->pack.bytes;
+}
 
 // One frame of one or more Camera.Render calls: build (host work, caller
 // holds ctx->mu), launch (enqueue on `stream`), collect (after completion).
@@ -145,9 +149,11 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
 t3d_status frame_launch(t3d_ctx* c, t3d_frame* f, hipStream_t stream, hipStream_t copy_stream, int32_t exec_mode,
                         bool allow_graph, bool profile);
 void frame_collect(t3d_ctx* c, t3d_frame* f, bool profile);
-// Copy pairs [p0, p1) of a frame, concatenated, to host memory (any output may be null).  With `pack`
-// (the caller holds ctx->mu) the survivors are first packed on the device into that buffer, so vertices and
-// indices cross to the host in one copy per stream; without it each pair is copied on its own.
+// Device bytes frame_download_range needs in `pack` for pairs [p0, p1) (0 when none survive).
+uint64_t download_pack_bytes(const t3d_frame* f, uint32_t p0, uint32_t p1);
+// Copy pairs [p0, p1) of a frame, concatenated, to host memory (any output may be null).  Vertices and
+// indices are first packed on the device into `pack` (grown here unless it already holds
+// download_pack_bytes, which needs ctx->mu), so they cross to the host in one copy per stream.
 t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uint32_t p1, hipStream_t stream,
                                 t3d_vertex* cv, t3d_vertex* dv, uint32_t* idx, uint32_t* tri_ids,
                                 t3d::DevBuf* pack = nullptr, uint64_t* pack_cap = nullptr);

@@ -617,7 +617,15 @@ t3d_status t3d_job_download(t3d_server* s, uint64_t job, t3d_pair_result* pairs,
   if (cv || dv || idx || tri_ids) {
     (void)hipSetDevice(s->ctx->device);
     std::lock_guard<std::mutex> dl(s->dl_mu);   // one download at a time on the download stream
-    st = frame_download_range(s->ctx, f, p0, p1, s->dl_stream, cv, dv, idx, tri_ids);
+    // the batch frame's packing buffer: grown under ctx->mu (memory accounting), used under dl_mu only
+    t3d_frame* fm = const_cast<t3d_frame*>(f);
+    const uint64_t need = (cv || dv || idx) ? download_pack_bytes(f, p0, p1) : 0;
+    if (need) {
+      std::lock_guard<std::mutex> cl(s->ctx->mu);
+      st = ensure(s->ctx, fm->pack, fm->pack_cap, need);
+    }
+    if (!st)
+      st = frame_download_range(s->ctx, f, p0, p1, s->dl_stream, cv, dv, idx, tri_ids, &fm->pack, &fm->pack_cap);
   }
   std::lock_guard<std::mutex> lk(s->mu);
   jp->pins--;
