@@ -14,7 +14,11 @@
  * which survives rounding to float32 only in rare boundary cases).
  *
  * Every function cites the reference file:line it restates.  Parity status of
- * each part is recorded next to its tests.
+ * each part is recorded next to its tests.  Parity unpinned: go_sin and
+ * go_atan2 (Go's math.Sin/Atan2 restated; checked against the C library and
+ * Go's special cases, not against Go's own outputs), the product order inside
+ * q_to_m4 (its two factors commute), and go_stable_sort under the two
+ * reference comparators that are not strict orders.
  */
 #include "t3d_oracle.h"
 
