@@ -15,7 +15,9 @@ from tetra3d_amd import _abi as A
 from tetra3d_amd.scene import RenderInputs
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libt3d_oracle.so")
+# T3D_ORACLE_LIB points the checker at another build of the same sources (a
+# copy with a planted mistake, to show that a test fails on it)
+LIB_PATH = os.environ.get("T3D_ORACLE_LIB") or os.path.join(HERE, "libt3d_oracle.so")
 
 
 class OracleResult(C.Structure):
