@@ -376,6 +376,30 @@ t3d_status t3d_profile_enable(t3d_ctx* ctx, int32_t on);
 t3d_status t3d_profile_read(t3d_ctx* ctx, t3d_kernel_time* per_kernel, uint32_t max_kernels,
                             uint32_t* out_count, int32_t reset);
 
+/* ---- rendering without allocation ----------------------------------- */
+
+/* Bytes of the static device scratch t3d_render_noalloc carves a frame's
+ * tables and scratch from (one array per GPU, held until the process ends). */
+#define T3D_NOALLOC_SCRATCH_BYTES (64ull << 20)
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc): the same
+ * Camera.Render as t3d_render (same kernels, same results), but the frame's
+ * tables and scratch come from a static per-GPU device array of
+ * T3D_NOALLOC_SCRATCH_BYTES and the outputs are the caller's device arrays,
+ * each holding vertex_capacity entries.  Calls no hipMalloc, hipHostMalloc or
+ * hipFree and builds no graph; the tables go up from pageable host memory
+ * and the call waits for the frame.  Pair p's results occupy vertices
+ * [vertex_offset, vertex_offset + 3*survivor_count) of the caller's arrays
+ * (t3d_frame_pairs), so the arrays must hold 3 x the triangles of every drawn
+ * render pair (T3D_ERR_INVALID_ARGUMENT otherwise, naming the count); a frame
+ * whose tables and scratch exceed the array is refused with T3D_ERR_BUDGET.
+ * The frame argument carries only host bookkeeping here.  Every call on one
+ * GPU shares its array, from any context: calls that overlap on one GPU race. */
+t3d_status t3d_render_noalloc(t3d_ctx* ctx, t3d_frame* frame, const t3d_render_desc* desc,
+                              t3d_vertex* color_vertices, t3d_vertex* depth_vertices, uint32_t* indices,
+                              uint64_t vertex_capacity);
+
 /* ---- tuning ---------------------------------------------------------- */
 
 /* Launch shape of the batched form: chosen by t3d_autotune or set by hand. */

@@ -509,6 +509,13 @@ t3d_status t3d_ctx_create(int32_t device, uint64_t hbm_budget_bytes, t3d_ctx** o
     delete c;
     return T3D_ERR_DEVICE;
   }
+  // this device's copy of the scratch array of t3d_render_noalloc, looked up with the device current
+  if (noalloc_scratch_address(device, &c->noalloc_scratch) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    (void)hipStreamDestroy(c->copy_stream);
+    delete c;
+    return T3D_ERR_DEVICE;
+  }
   *out = c;
   return T3D_OK;
 }
@@ -859,7 +866,8 @@ t3d_status run_chain(t3d_ctx* c, t3d_frame* f, uint32_t p0, uint32_t p1, hipStre
 // index).  Pure host work: nothing is sent to the device yet.
 namespace t3d {
 
-t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* descs, uint32_t ndesc) {
+t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* descs, uint32_t ndesc,
+                       const Placement* place) {
   auto wall0 = std::chrono::steady_clock::now();
   if (!c || !f || !descs || ndesc == 0) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: null argument");
   (void)hipSetDevice(c->device);
@@ -1279,14 +1287,6 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
                           carve_size<float>(pal_floats) + carve_size<float>(skw_floats) +
                           carve_size<t3d_bone_pose>(pose_bones) + carve_size<M4>(pose_roots) +
                           carve_size<PaletteJob>(npose);
-  t3d_status st = ensure(c, f->tables, f->tables_cap, tbytes);
-  if (st) return st;
-  if (tbytes > f->pinned_cap) {
-    if (f->pinned) (void)hipHostFree(f->pinned);
-    f->pinned = nullptr;
-    f->pinned_cap = align_up(tbytes + tbytes / 4, 1 << 16);
-    HIP_TRY(c, hipHostMalloc(&f->pinned, f->pinned_cap, hipHostMallocDefault));
-  }
   // scratch: vertex records, skinned positions, keys, survivor records, stats, histograms, block sums
   const uint32_t ntiles_total = f->tile_prefix.back();
   const uint64_t sbytes = carve_size<float4>(2 * sumV) + carve_size<float4>(sumV) + carve_size<uint32_t>(sumT) +
@@ -1294,21 +1294,58 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
                           carve_size<uint32_t>(sblocks) + 2 * carve_size<uint32_t>(ntiles_total) +
                           carve_size<uint32_t>(f->vchunk_prefix.back()) + carve_size<float4>(128) +
                           carve_size<PartLight>(n_plights) + carve_size<M4>(pose_bones);
-  st = ensure(c, f->scratch, f->scratch_cap, sbytes);
-  if (st) return st;
-  const uint64_t obytes = 2 * carve_size<t3d_vertex>(3 * sumT) + carve_size<uint32_t>(3 * sumT);
-  st = ensure(c, f->outputs, f->outputs_cap, obytes);
-  if (st) return st;
-  if (4ull * P_n > f->pinned_stats_cap) {
-    if (f->pinned_stats) (void)hipHostFree(f->pinned_stats);
-    f->pinned_stats = nullptr;
-    f->pinned_stats_cap = align_up(4ull * P_n + 1024, 1024);
-    HIP_TRY(c, hipHostMalloc((void**)&f->pinned_stats, f->pinned_stats_cap * 4, hipHostMallocDefault));
+  uint8_t *tables_dev, *tables_host, *scratch_dev;
+  t3d_vertex *out_color, *out_depth;
+  uint32_t* out_idx;
+  if (place) {
+    // no allocation: tables and scratch carved from the placement's scratch, outputs the caller's
+    if (tbytes + sbytes > place->scratch_bytes)
+      return fail(c, T3D_ERR_BUDGET, "render_noalloc: the frame needs %llu bytes of scratch, the static scratch holds %llu",
+                  (unsigned long long)(tbytes + sbytes), (unsigned long long)place->scratch_bytes);
+    if (3 * sumT > place->capacity)
+      return fail(c, T3D_ERR_INVALID_ARGUMENT, "render_noalloc: outputs hold %llu vertices, the frame needs %llu",
+                  (unsigned long long)place->capacity, (unsigned long long)(3 * sumT));
+    if (f->host_tables.size() < tbytes) f->host_tables.resize(tbytes);
+    if (f->host_stats.size() < 4ull * P_n) f->host_stats.resize(4ull * P_n);
+    tables_dev = place->scratch;
+    scratch_dev = place->scratch + tbytes;   // tbytes is a sum of 256-byte-aligned slices
+    tables_host = f->host_tables.data();
+    out_color = place->color;
+    out_depth = place->depth;
+    out_idx = place->idx;
+  } else {
+    t3d_status st = ensure(c, f->tables, f->tables_cap, tbytes);
+    if (st) return st;
+    if (tbytes > f->pinned_cap) {
+      if (f->pinned) (void)hipHostFree(f->pinned);
+      f->pinned = nullptr;
+      f->pinned_cap = align_up(tbytes + tbytes / 4, 1 << 16);
+      HIP_TRY(c, hipHostMalloc(&f->pinned, f->pinned_cap, hipHostMallocDefault));
+    }
+    st = ensure(c, f->scratch, f->scratch_cap, sbytes);
+    if (st) return st;
+    const uint64_t obytes = 2 * carve_size<t3d_vertex>(3 * sumT) + carve_size<uint32_t>(3 * sumT);
+    st = ensure(c, f->outputs, f->outputs_cap, obytes);
+    if (st) return st;
+    if (4ull * P_n > f->pinned_stats_cap) {
+      if (f->pinned_stats) (void)hipHostFree(f->pinned_stats);
+      f->pinned_stats = nullptr;
+      f->pinned_stats_cap = align_up(4ull * P_n + 1024, 1024);
+      HIP_TRY(c, hipHostMalloc((void**)&f->pinned_stats, f->pinned_stats_cap * 4, hipHostMallocDefault));
+    }
+    tables_dev = reinterpret_cast<uint8_t*>(f->tables.ptr);
+    scratch_dev = reinterpret_cast<uint8_t*>(f->scratch.ptr);
+    tables_host = reinterpret_cast<uint8_t*>(f->pinned);
+    Carve oc{reinterpret_cast<uint8_t*>(f->outputs.ptr)};
+    out_color = oc.take<t3d_vertex>(3 * sumT);
+    out_depth = oc.take<t3d_vertex>(3 * sumT);
+    out_idx = oc.take<uint32_t>(3 * sumT);
   }
+  f->tables_dev = tables_dev;
 
-  // fill the pinned staging block with the same carve layout as the device block
-  Carve dev{reinterpret_cast<uint8_t*>(f->tables.ptr)};
-  Carve host{reinterpret_cast<uint8_t*>(f->pinned)};
+  // fill the host staging block with the same carve layout as the device block
+  Carve dev{tables_dev};
+  Carve host{tables_host};
   FrameConst* d_fc = dev.take<FrameConst>(ndesc);
   FrameConst* h_fc = host.take<FrameConst>(ndesc);
   std::memcpy(h_fc, f->fcs.data(), sizeof(FrameConst) * ndesc);
@@ -1385,8 +1422,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   std::memcpy(h_ppre, f->pscan_prefix.data(), 4 * (P_n + 1));
   f->table_bytes = host.off;
 
-  Carve sc{reinterpret_cast<uint8_t*>(f->scratch.ptr)};
-  Carve oc{reinterpret_cast<uint8_t*>(f->outputs.ptr)};
+  Carve sc{scratch_dev};
   FrameArgs& a = f->args;
   a.parts = d_parts;
   a.nparts = P_n;
@@ -1421,9 +1457,9 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   a.npal_jobs = npose;
   // poses and armature transforms read, palettes written
   f->bytes_palettes = (sizeof(t3d_bone_pose) + 64) * pose_bones + 64 * pose_roots + sizeof(PaletteJob) * npose;
-  a.out_color = reinterpret_cast<float4*>(oc.take<t3d_vertex>(3 * sumT));
-  a.out_depth = reinterpret_cast<float4*>(oc.take<t3d_vertex>(3 * sumT));
-  a.out_idx = oc.take<uint32_t>(3 * sumT);
+  a.out_color = reinterpret_cast<float4*>(out_color);
+  a.out_depth = reinterpret_cast<float4*>(out_depth);
+  a.out_idx = out_idx;
   a.fc = d_fc;
   f->vertex_threads = vthreads;
   f->total_vchunks = f->vchunk_prefix.back();
@@ -1506,7 +1542,36 @@ t3d_status frame_launch(t3d_ctx* c, t3d_frame* f, hipStream_t stream, hipStream_
     if (st) return st;
   }
   HIP_TRY(c, hipMemcpyAsync(f->pinned_stats, a.stats, 16ull * P_n, hipMemcpyDeviceToHost, stream));
+  f->stats_host = f->pinned_stats;
   HIP_TRY(c, hipEventRecord(f->done, stream));
+  return T3D_OK;
+}
+
+t3d_status frame_run_placed(t3d_ctx* c, t3d_frame* f, hipStream_t stream, int32_t exec_mode) {
+  const uint32_t P_n = f->nparts;
+  const FrameArgs& a = f->args;
+  HIP_TRY(c, hipMemcpyAsync(f->tables_dev, f->host_tables.data(), f->table_bytes, hipMemcpyHostToDevice, stream));
+  if (c->poison && f->total_triangles) {
+    const uint64_t nv = 3 * f->total_triangles;
+    HIP_TRY(c, launch_fill(a.out_color, nv * sizeof(t3d_vertex), c->poison, stream));
+    HIP_TRY(c, launch_fill(a.out_depth, nv * sizeof(t3d_vertex), c->poison, stream));
+    HIP_TRY(c, launch_fill(a.out_idx, nv * 4, c->poison, stream));
+  }
+  if (P_n) {
+    if (exec_mode == T3D_EXEC_PLAIN) {
+      for (uint32_t i = 0; i < P_n; i++) {
+        t3d_status st = run_chain(c, f, i, i + 1, stream, false);
+        if (st) return st;
+      }
+    } else {
+      t3d_status st = run_chain(c, f, 0, P_n, stream, false);
+      if (st) return st;
+    }
+    HIP_TRY(c, hipMemcpyAsync(f->host_stats.data(), a.stats, 16ull * P_n, hipMemcpyDeviceToHost, stream));
+  }
+  HIP_TRY(c, hipStreamSynchronize(stream));
+  f->stats_host = f->host_stats.data();
+  frame_collect(c, f, false);
   return T3D_OK;
 }
 
@@ -1514,7 +1579,7 @@ t3d_status frame_launch(t3d_ctx* c, t3d_frame* f, hipStream_t stream, hipStream_
 void frame_collect(t3d_ctx* c, t3d_frame* f, bool profile) {
   uint64_t total = 0;
   for (uint32_t i = 0; i < f->nparts; i++) {
-    f->pairs[i].survivor_count = f->pinned_stats[4 * i + 2];
+    f->pairs[i].survivor_count = f->stats_host[4 * i + 2];
     total += f->pairs[i].survivor_count;
   }
   f->total_survivors = total;

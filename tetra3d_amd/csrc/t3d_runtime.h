@@ -73,6 +73,12 @@ struct t3d_frame {
   uint64_t pinned_cap = 0;
   uint32_t* pinned_stats = nullptr;
   uint64_t pinned_stats_cap = 0;
+  // a render placed in external memory (t3d_render_noalloc): pageable host staging of the tables and of the
+  // stats readback, and where the last render's stats were read back to
+  std::vector<uint8_t> host_tables;
+  std::vector<uint32_t> host_stats;
+  const uint32_t* stats_host = nullptr;
+  uint8_t* tables_dev = nullptr;   // device table block of the last build (own or placed)
   t3d::FrameArgs args{};
   // per-render bookkeeping
   std::vector<t3d_pair_result> pairs;
@@ -118,6 +124,8 @@ struct t3d_ctx {
   std::mutex mu;
   // nonzero: fill every frame's outputs with this word before its render
   uint32_t poison = 0;
+  // this device's copy of the static scratch array of t3d_render_noalloc (looked up at context creation)
+  void* noalloc_scratch = nullptr;
   // profiling (synchronous render path)
   bool profiling = false;
   struct Prof {
@@ -131,6 +139,9 @@ struct t3d_ctx {
 namespace t3d {
 
 t3d_status fail(t3d_ctx* c, t3d_status st, const char* fmt, ...);
+// Device address of `device`'s copy of the no-allocation scratch array (t3d_noalloc.hip); the device must be
+// current.  Looked up once per device.
+hipError_t noalloc_scratch_address(int device, void** out);
 t3d_status dev_alloc(t3d_ctx* c, DevBuf& b, uint64_t bytes);
 // Grow `b` to at least `bytes` (contents discarded); the caller holds ctx->mu.
 t3d_status ensure(t3d_ctx* c, DevBuf& b, uint64_t& cap, uint64_t bytes);
@@ -141,9 +152,28 @@ inline uint64_t frame_capacity(const t3d_frame* f) {
   return f->tables.bytes + f->scratch.bytes + f->outputs.bytes + f->pack.bytes;
 }
 
+// Device memory a frame uses without allocating (t3d_render_noalloc): its
+// tables and scratch are carved from `scratch`, its outputs are the caller's.
+struct Placement {
+  uint8_t* scratch = nullptr;
+  uint64_t scratch_bytes = 0;
+  t3d_vertex* color = nullptr;
+  t3d_vertex* depth = nullptr;
+  uint32_t* idx = nullptr;
+  uint64_t capacity = 0;         // vertices each output array holds
+};
+
 // One frame of one or more Camera.Render calls: build (host work, caller
 // holds ctx->mu), launch (enqueue on `stream`), collect (after completion).
-t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* descs, uint32_t ndesc);
+// With `place`, nothing is allocated: the frame's device memory is the
+// placement's, its table staging pageable host memory.
+t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* descs, uint32_t ndesc,
+                       const Placement* place = nullptr);
+// Runs a frame built with a placement on `stream` and waits for it: table
+// upload from pageable memory, the chain (per pair for the plain form, one
+// chain for the batched form; never a graph), stats readback.  Calls no
+// allocating HIP routine.
+t3d_status frame_run_placed(t3d_ctx* c, t3d_frame* f, hipStream_t stream, int32_t exec_mode);
 // The table upload goes on `copy_stream` (may equal `stream`); `stream` waits
 // for it, runs the chain and the survivor readback, then records f->done.
 t3d_status frame_launch(t3d_ctx* c, t3d_frame* f, hipStream_t stream, hipStream_t copy_stream, int32_t exec_mode,

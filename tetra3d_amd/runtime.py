@@ -62,6 +62,7 @@ def lib():
     L.t3d_frame_destroy.argtypes = [vp, vp]
     L.t3d_render.argtypes = [vp, vp, P(A.RenderDesc)]
     L.t3d_render_async.argtypes = [vp, vp, P(A.RenderDesc)]
+    L.t3d_render_noalloc.argtypes = [vp, vp, P(A.RenderDesc), vp, vp, vp, C.c_uint64]
     L.t3d_debug_poison_outputs.argtypes = [vp, C.c_uint32]
     L.t3d_frame_wait.argtypes = [vp, vp]
     L.t3d_frame_pairs.argtypes = [vp, P(C.c_uint32), P(P(A.PairResult))]
@@ -84,7 +85,7 @@ def lib():
                  "t3d_frame_device_outputs", "t3d_frame_download", "t3d_frame_download_triangles",
                  "t3d_profile_enable", "t3d_profile_read", "t3d_get_tuning", "t3d_set_tuning", "t3d_autotune",
                  "t3d_selftest_fp", "t3d_selftest_copy", "t3d_debug_model_spheres", "t3d_animation_upload",
-                 "t3d_animation_free"):
+                 "t3d_animation_free", "t3d_render_noalloc"):
         getattr(L, name).restype = C.c_int32
     _lib = L
     return L
@@ -241,6 +242,13 @@ class Context:
     def render_inputs(self, inp):
         """Run one Camera.Render on the device; returns nothing (outputs stay in HBM)."""
         self._check(self.L.t3d_render(self.ctx, self.frame, C.byref(inp.render)))
+
+    def render_noalloc(self, inp, color, depth, indices, capacity, frame=None):
+        """Camera.Render into caller-owned device arrays (device addresses as
+        ints, each `capacity` vertices long) with no device allocation
+        (t3d_render_noalloc); results are read with pairs(frame)."""
+        self._check(self.L.t3d_render_noalloc(self.ctx, frame or self.frame, C.byref(inp.render), color, depth,
+                                              indices, capacity))
 
     def poison_outputs(self, word=0x7FC0DEAD):
         """Fill frame outputs with `word` before every render (0: off); tests
