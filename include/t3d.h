@@ -306,6 +306,7 @@ t3d_status t3d_mesh_free(t3d_ctx* ctx, uint64_t handle);
 /* Upload LightVolume cell colours: [cell_count] RGBA float32, cell order
  * x + y*cx + z*cx*cy (getLightCell, light.go:1267-1272). */
 t3d_status t3d_volume_upload(t3d_ctx* ctx, const float* cells_rgba, uint32_t cell_count, uint64_t* out_handle);
+/* T3D_ERR_BUSY while a submitted job still references the volume. */
 t3d_status t3d_volume_free(t3d_ctx* ctx, uint64_t handle);
 
 /* An Animation's channels (animation.go:188-260), resident on the device;
@@ -314,6 +315,7 @@ t3d_status t3d_volume_free(t3d_ctx* ctx, uint64_t handle);
  * Times must be finite and non-decreasing within a track. */
 t3d_status t3d_animation_upload(t3d_ctx* ctx, const t3d_anim_channel* channels, uint32_t channel_count,
                                 uint64_t* out_handle);
+/* T3D_ERR_BUSY while a submitted job still references the animation. */
 t3d_status t3d_animation_free(t3d_ctx* ctx, uint64_t handle);
 
 /* ---- frames ---------------------------------------------------------- */

@@ -666,6 +666,7 @@ t3d_status t3d_volume_free(t3d_ctx* c, uint64_t handle) {
   std::lock_guard<std::mutex> lk(c->mu);
   auto it = c->volumes.find(handle);
   if (it == c->volumes.end()) return fail(c, T3D_ERR_NOT_FOUND, "volume_free: unknown handle");
+  if (it->second->refs > 0) return fail(c, T3D_ERR_BUSY, "volume_free: volume %llu is referenced by in-flight work", (unsigned long long)handle);
   (void)hipDeviceSynchronize();
   dev_free(c, it->second->buf);
   c->volumes.erase(it);
@@ -732,6 +733,8 @@ t3d_status t3d_animation_free(t3d_ctx* c, uint64_t handle) {
   std::lock_guard<std::mutex> lk(c->mu);
   auto it = c->animations.find(handle);
   if (it == c->animations.end()) return fail(c, T3D_ERR_NOT_FOUND, "animation_free: unknown handle");
+  if (it->second->refs > 0)
+    return fail(c, T3D_ERR_BUSY, "animation_free: animation %llu is referenced by in-flight work", (unsigned long long)handle);
   (void)hipDeviceSynchronize();
   dev_free(c, it->second->buf);
   c->animations.erase(it);
@@ -861,11 +864,88 @@ t3d_status run_chain(t3d_ctx* c, t3d_frame* f, uint32_t p0, uint32_t p1, hipStre
 // frame build / launch / finish
 // ---------------------------------------------------------------------------
 
+namespace t3d {
+
+// Every refusal of one Camera.Render descriptor that does not depend on the
+// frustum test, in model order: a descriptor it accepts renders.  The
+// serving form runs it at submit, so a job that would fail is refused alone
+// instead of failing the batch it would join.  Caller holds ctx->mu.
+t3d_status validate_desc(t3d_ctx* c, const t3d_render_desc* desc) {
+  if (!desc || !desc->camera) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: null camera");
+  const t3d_camera_desc* cd = desc->camera;
+  if (cd->width <= 0 || cd->height <= 0) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: camera size %dx%d", cd->width, cd->height);
+  if (cd->render_normals) return fail(c, T3D_ERR_UNSUPPORTED, "render: Camera.RenderNormals is outside the GPU path");
+  if (desc->light_count && !desc->lights) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: lights missing");
+  if (desc->model_count && !desc->models) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: models missing");
+  const t3d_world_desc* world = (desc->world && desc->world->present) ? desc->world : nullptr;
+  auto check_light = [&](const t3d_light_desc& l, const char* what, uint32_t i) -> t3d_status {
+    if (l.type < T3D_LIGHT_AMBIENT || l.type > T3D_LIGHT_VOLUME)
+      return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: %s %u has unknown type %d", what, i, l.type);
+    if (l.type == T3D_LIGHT_VOLUME && !c->volumes.count(l.volume))
+      return fail(c, T3D_ERR_NOT_FOUND, "render: %s %u references unknown volume", what, i);
+    return T3D_OK;
+  };
+  // scene lights a lit pair sees: the world ambient light and the visible lights, cut to MaxLightCount
+  uint32_t n_scene = (world && world->lighting_on && world->ambient_visible) ? 1u : 0u;
+  for (uint32_t i = 0; i < desc->light_count; i++) {
+    t3d_status st = check_light(desc->lights[i], "light", i);
+    if (st) return st;
+    if ((!world || world->lighting_on) && desc->lights[i].visible) n_scene++;
+  }
+  if (cd->max_light_count > 0) n_scene = std::min<uint32_t>(n_scene, (uint32_t)cd->max_light_count);
+  for (uint32_t mi = 0; mi < desc->model_count; mi++) {
+    const t3d_model_desc& md = desc->models[mi];
+    if (!md.visible || md.mesh == 0) continue;
+    auto it = c->meshes.find(md.mesh);
+    if (it == c->meshes.end())
+      return fail(c, T3D_ERR_NOT_FOUND, "render: model %u references unknown mesh %llu", mi, (unsigned long long)md.mesh);
+    const Mesh& mesh = *it->second;
+    if (md.has_vertex_transform_function || md.has_vertex_clip_function)
+      return fail(c, T3D_ERR_UNSUPPORTED, "render: model %u uses a Go vertex callback", mi);
+    if (md.skinned) {
+      if (!mesh.has_skin || (!md.bone_influences && !md.bone_poses))
+        return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: skinned model %u without skin data", mi);
+      if (md.bone_poses)
+        for (uint32_t b = 0; b < md.bone_count; b++)
+          if (md.bone_poses[b].parent < -1 || md.bone_poses[b].parent >= (int32_t)b)
+            return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u bone %u parent is not an earlier bone", mi, b);
+      if (md.animation) {
+        if (!md.bone_poses) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u animation without bone poses", mi);
+        if (!c->animations.count(md.animation))
+          return fail(c, T3D_ERR_NOT_FOUND, "render: model %u references unknown animation %llu", mi,
+                      (unsigned long long)md.animation);
+        if (std::isnan(md.animation_time)) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u animation time is NaN", mi);
+      }
+      if (mesh.max_bone_index >= md.bone_count) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u bone index out of palette", mi);
+    }
+    if (md.light_group_active && md.light_group_count && !md.light_group)
+      return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u light group missing", mi);
+    if (md.light_group_active)
+      for (uint32_t k = 0; k < md.light_group_count; k++) {
+        t3d_status st = check_light(md.light_group[k], "light group entry", k);
+        if (st) return st;
+      }
+    if (md.auto_batch_static) continue;   // never drawn (camera.go:2066)
+    for (uint32_t p = 0; p < mesh.parts.size(); p++) {
+      const t3d_material_desc* mat = part_material(md, p);
+      if (mat && !mat->visible) continue;
+      if (mat && (mat->billboard || mat->custom_depth_function))
+        return fail(c, T3D_ERR_UNSUPPORTED, "render: model %u part %u uses billboarding or a custom depth function", mi, p);
+      if (mat && (mat->triangle_sort_mode < T3D_SORT_BACK_TO_FRONT || mat->triangle_sort_mode > T3D_SORT_NONE))
+        return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: triangle sort mode %d", mat->triangle_sort_mode);
+      const bool lighting = world && world->lighting_on && !md.shadeless && !(mat && mat->shadeless);   // camera.go:1521-1528
+      const uint32_t lcount = md.light_group_active ? md.light_group_count : n_scene;
+      if (lighting && lcount > kMaxLights)
+        return fail(c, T3D_ERR_UNSUPPORTED, "render: %u lights on model %u (max %u)", lcount, mi, kMaxLights);
+    }
+  }
+  return T3D_OK;
+}
+
+
 // Builds the frame tables of one or more Camera.Render calls (a batch of
 // jobs shares one launch chain; every render pair carries its job's camera
 // index).  Pure host work: nothing is sent to the device yet.
-namespace t3d {
-
 t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* descs, uint32_t ndesc,
                        const Placement* place) {
   auto wall0 = std::chrono::steady_clock::now();
@@ -917,12 +997,9 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   for (uint32_t job = 0; job < ndesc; job++) {
     const auto job0 = std::chrono::steady_clock::now();
     const t3d_render_desc* desc = descs[job];
-    if (!desc || !desc->camera) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: null camera");
+    t3d_status vst = validate_desc(c, desc);
+    if (vst) return vst;
     const t3d_camera_desc* cd = desc->camera;
-    if (cd->width <= 0 || cd->height <= 0) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: camera size %dx%d", cd->width, cd->height);
-    if (cd->render_normals) return fail(c, T3D_ERR_UNSUPPORTED, "render: Camera.RenderNormals is outside the GPU path");
-    if (desc->light_count && !desc->lights) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: lights missing");
-    if (desc->model_count && !desc->models) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: models missing");
     const t3d_world_desc* world = (desc->world && desc->world->present) ? desc->world : nullptr;
 
     Cam cam;
@@ -941,10 +1018,6 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
     }
     for (uint32_t i = 0; i < desc->light_count; i++) {
       const t3d_light_desc& l = desc->lights[i];
-      if (l.type < T3D_LIGHT_AMBIENT || l.type > T3D_LIGHT_VOLUME)
-        return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: light %u has unknown type %d", i, l.type);
-      if (l.type == T3D_LIGHT_VOLUME && !c->volumes.count(l.volume))
-        return fail(c, T3D_ERR_NOT_FOUND, "render: light %u references unknown volume", i);
       if ((!world || world->lighting_on) && l.visible) scene_lights.push_back(SceneLight{&l, {}});
     }
 
@@ -1006,26 +1079,6 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
       if (!mp)
         return fail(c, T3D_ERR_NOT_FOUND, "render: model %u references unknown mesh %llu", mi, (unsigned long long)md.mesh);
       const Mesh& mesh = *mp;
-      if (md.has_vertex_transform_function || md.has_vertex_clip_function)
-        return fail(c, T3D_ERR_UNSUPPORTED, "render: model %u uses a Go vertex callback", mi);
-      if (md.skinned) {
-        if (!mesh.has_skin || (!md.bone_influences && !md.bone_poses))
-          return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: skinned model %u without skin data", mi);
-        if (md.bone_poses)
-          for (uint32_t b = 0; b < md.bone_count; b++)
-            if (md.bone_poses[b].parent < -1 || md.bone_poses[b].parent >= (int32_t)b)
-              return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u bone %u parent is not an earlier bone", mi, b);
-        if (md.animation) {
-          if (!md.bone_poses) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u animation without bone poses", mi);
-          if (!c->animations.count(md.animation))
-            return fail(c, T3D_ERR_NOT_FOUND, "render: model %u references unknown animation %llu", mi,
-                        (unsigned long long)md.animation);
-          if (std::isnan(md.animation_time)) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u animation time is NaN", mi);
-        }
-        if (mesh.max_bone_index >= md.bone_count) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u bone index out of palette", mi);
-      }
-      if (md.light_group_active && md.light_group_count && !md.light_group)
-        return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u light group missing", mi);
       if (md.frustum_culling && !in_frustum[mi]) continue;
       for (uint32_t p = 0; p < mesh.parts.size(); p++) {
         const t3d_material_desc* mat = part_material(md, p);
@@ -1120,8 +1173,6 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
       if (!md.visible || md.auto_batch_static) continue;               // camera.go:2067-2071
       const Mesh& mesh = *rp.mesh;
       const t3d_material_desc* mat = part_material(md, rp.part);
-      if (mat && (mat->billboard || mat->custom_depth_function))
-        return fail(c, T3D_ERR_UNSUPPORTED, "render: model %u part %u uses billboarding or a custom depth function", rp.model, rp.part);
       const t3d_part_range& pr = mesh.parts[rp.part];
       bool lighting = false;                                           // camera.go:1521-1528
       if (world) lighting = mat ? (world->lighting_on && !mat->shadeless && !md.shadeless) : (world->lighting_on && !md.shadeless);
@@ -1150,7 +1201,6 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
       if (mesh.uv) flags |= PF_HAS_UV;
       if (mesh.K) flags |= PF_SHAPE_KEYS;
       const uint32_t sort_mode = mat ? (uint32_t)mat->triangle_sort_mode : T3D_SORT_BACK_TO_FRONT;
-      if (sort_mode > T3D_SORT_NONE) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: triangle sort mode %u", sort_mode);
       flags |= sort_mode << PF_SORT_SHIFT;
       flags |= ((uint32_t)(mat ? mat->lighting_mode : 0) & 3u) << PF_LMODE_SHIFT;
       const uint32_t lv_mode = (uint32_t)(mat ? mat->light_volume_shading_mode : 0) & 3u;
@@ -1165,18 +1215,14 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
           auto g = group_base.find(rp.model);
           if (g == group_base.end()) {
             uint32_t gb = (uint32_t)lights.size();
-            for (uint32_t k = 0; k < md.light_group_count; k++) {
-              const t3d_light_desc& l = md.light_group[k];
-              if (l.type == T3D_LIGHT_VOLUME && !c->volumes.count(l.volume))
-                return fail(c, T3D_ERR_NOT_FOUND, "render: light group references unknown volume");
-              lights.push_back(make_light(c, l));
-            }
+            for (uint32_t k = 0; k < md.light_group_count; k++) lights.push_back(make_light(c, md.light_group[k]));
             g = group_base.emplace(rp.model, gb).first;
           }
           lbase = g->second;
           lcount = md.light_group_count;
         }
-        if (lcount > kMaxLights) return fail(c, T3D_ERR_UNSUPPORTED, "render: %u lights on one model (max %u)", lcount, kMaxLights);
+        if (lcount > kMaxLights)   // validate_desc refuses the descriptor first
+          return fail(c, T3D_ERR_UNSUPPORTED, "render: %u lights on one model (max %u)", lcount, kMaxLights);
         P.light_base = (uint32_t)n_plights;
         P.light_count = lcount;
         P.light0 = lbase;
@@ -1605,7 +1651,22 @@ uint64_t download_pack_bytes(const t3d_frame* f, uint32_t p0, uint32_t p1) {
 
 t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uint32_t p1, hipStream_t stream,
                                 t3d_vertex* cv, t3d_vertex* dv, uint32_t* idx, uint32_t* tri_ids, DevBuf* pack,
-                                uint64_t* pack_cap) {
+                                uint64_t* pack_cap, std::string* err_out) {
+  // Every exit after the first asynchronous copy waits for the stream, so no copy is still writing into `recs`
+  // or the caller's buffers when this returns; the message goes to *err_out when given, else to ctx->err.
+  bool enqueued = false;
+  auto bail = [&](t3d_status st, const std::string& msg) {
+    if (enqueued) (void)hipStreamSynchronize(stream);
+    if (err_out) *err_out = msg;
+    else c->err = msg;
+    return st;
+  };
+#define DL_TRY(expr)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) return bail(T3D_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    enqueued = true;                                                                       \
+  } while (0)
   uint64_t off = 0, toff = 0;
   uint64_t nrec = 0;
   for (uint32_t i = p0; i < p1; i++) nrec += f->pairs[i].survivor_count;
@@ -1613,7 +1674,7 @@ t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uin
   // vertices and indices: the pairs' survivors packed on the device (k_pack), then one copy per stream
   std::vector<uint32_t> blk(1, 0), nv;
   std::vector<uint64_t> src, dst;
-  if ((cv || dv || idx) && !pack) return fail(c, T3D_ERR_INVALID_ARGUMENT, "download: no packing buffer");
+  if ((cv || dv || idx) && !pack) return bail(T3D_ERR_INVALID_ARGUMENT, "download: no packing buffer");
   if (cv || dv || idx)
     for (uint32_t i = p0; i < p1; i++) {
       const uint64_t n = 3ull * f->pairs[i].survivor_count;
@@ -1626,18 +1687,22 @@ t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uin
     }
   if (off) {
     const uint32_t np = (uint32_t)nv.size();
-    t3d_status st = ensure(c, *pack, *pack_cap, pack_layout_bytes(np, off));
-    if (st) return st;
+    const uint64_t need = pack_layout_bytes(np, off);
+    if (need > *pack_cap) {
+      if (err_out) return bail(T3D_ERR_INVALID_ARGUMENT, "download: packing buffer too small");   // grown by the caller
+      t3d_status st = ensure(c, *pack, *pack_cap, need);
+      if (st) return st;
+    }
     Carve pc{reinterpret_cast<uint8_t*>(pack->ptr)};
     PackArgs pa{};
     uint32_t* d_blk = pc.take<uint32_t>(np + 1);
     uint64_t* d_src = pc.take<uint64_t>(np);
     uint64_t* d_dst = pc.take<uint64_t>(np);
     uint32_t* d_nv = pc.take<uint32_t>(np);
-    HIP_TRY(c, hipMemcpyAsync(d_blk, blk.data(), 4ull * (np + 1), hipMemcpyHostToDevice, stream));
-    HIP_TRY(c, hipMemcpyAsync(d_src, src.data(), 8ull * np, hipMemcpyHostToDevice, stream));
-    HIP_TRY(c, hipMemcpyAsync(d_dst, dst.data(), 8ull * np, hipMemcpyHostToDevice, stream));
-    HIP_TRY(c, hipMemcpyAsync(d_nv, nv.data(), 4ull * np, hipMemcpyHostToDevice, stream));
+    DL_TRY(hipMemcpyAsync(d_blk, blk.data(), 4ull * (np + 1), hipMemcpyHostToDevice, stream));
+    DL_TRY(hipMemcpyAsync(d_src, src.data(), 8ull * np, hipMemcpyHostToDevice, stream));
+    DL_TRY(hipMemcpyAsync(d_dst, dst.data(), 8ull * np, hipMemcpyHostToDevice, stream));
+    DL_TRY(hipMemcpyAsync(d_nv, nv.data(), 4ull * np, hipMemcpyHostToDevice, stream));
     pa.blk_prefix = d_blk;
     pa.src = d_src;
     pa.dst = d_dst;
@@ -1649,20 +1714,25 @@ t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uin
     pa.pcolor = reinterpret_cast<float4*>(pc.take<t3d_vertex>(off));
     pa.pdepth = reinterpret_cast<float4*>(pc.take<t3d_vertex>(off));
     pa.pidx = pc.take<uint32_t>(off);
-    HIP_TRY(c, launch_pack(pa, blk.back(), stream));
-    if (cv) HIP_TRY(c, hipMemcpyAsync(cv, pa.pcolor, off * sizeof(t3d_vertex), hipMemcpyDeviceToHost, stream));
-    if (dv) HIP_TRY(c, hipMemcpyAsync(dv, pa.pdepth, off * sizeof(t3d_vertex), hipMemcpyDeviceToHost, stream));
-    if (idx) HIP_TRY(c, hipMemcpyAsync(idx, pa.pidx, off * 4, hipMemcpyDeviceToHost, stream));
+    DL_TRY(launch_pack(pa, blk.back(), stream));
+    if (cv) DL_TRY(hipMemcpyAsync(cv, pa.pcolor, off * sizeof(t3d_vertex), hipMemcpyDeviceToHost, stream));
+    if (dv) DL_TRY(hipMemcpyAsync(dv, pa.pdepth, off * sizeof(t3d_vertex), hipMemcpyDeviceToHost, stream));
+    if (idx) DL_TRY(hipMemcpyAsync(idx, pa.pidx, off * 4, hipMemcpyDeviceToHost, stream));
   }
   if (tri_ids)
     for (uint32_t i = p0; i < p1; i++) {
       const auto& p = f->pairs[i];
       if (!p.survivor_count) continue;
-      HIP_TRY(c, hipMemcpyAsync(recs.data() + toff, f->args.tri_vo + f->parts[i].tbase, 16ull * p.survivor_count,
-                                hipMemcpyDeviceToHost, stream));
+      DL_TRY(hipMemcpyAsync(recs.data() + toff, f->args.tri_vo + f->parts[i].tbase, 16ull * p.survivor_count,
+                            hipMemcpyDeviceToHost, stream));
       toff += p.survivor_count;
     }
-  HIP_TRY(c, hipStreamSynchronize(stream));
+  {
+    hipError_t e = hipStreamSynchronize(stream);
+    enqueued = false;
+    if (e != hipSuccess) return bail(T3D_ERR_DEVICE, std::string("download: ") + hipGetErrorString(e));
+  }
+#undef DL_TRY
   if (tri_ids) {
     toff = 0;
     for (uint32_t i = p0; i < p1; i++) {

@@ -50,6 +50,7 @@ struct Volume {
   DevBuf buf;
   uint32_t n = 0;
   std::vector<float> host;         // for per-object lookups on the host
+  int refs = 0;                    // submitted jobs referencing the volume
 };
 
 struct Animation {
@@ -57,6 +58,7 @@ struct Animation {
   DevBuf buf;                       // channels (device pointers) followed by their times and values
   uint32_t nchannels = 0;
   const t3d_anim_channel* channels = nullptr;   // device address
+  int refs = 0;                     // submitted jobs referencing the animation
 };
 
 }  // namespace t3d
@@ -152,6 +154,10 @@ inline uint64_t frame_capacity(const t3d_frame* f) {
   return f->tables.bytes + f->scratch.bytes + f->outputs.bytes + f->pack.bytes;
 }
 
+// Every refusal of a Camera.Render descriptor that does not depend on the frustum test (caller holds ctx->mu);
+// frame_build runs it for each descriptor, the serving form at submit.
+t3d_status validate_desc(t3d_ctx* c, const t3d_render_desc* desc);
+
 // Device memory a frame uses without allocating (t3d_render_noalloc): its
 // tables and scratch are carved from `scratch`, its outputs are the caller's.
 struct Placement {
@@ -182,11 +188,14 @@ void frame_collect(t3d_ctx* c, t3d_frame* f, bool profile);
 // Device bytes frame_download_range needs in `pack` for pairs [p0, p1) (0 when none survive).
 uint64_t download_pack_bytes(const t3d_frame* f, uint32_t p0, uint32_t p1);
 // Copy pairs [p0, p1) of a frame, concatenated, to host memory (any output may be null).  Vertices and
-// indices are first packed on the device into `pack` (grown here unless it already holds
-// download_pack_bytes, which needs ctx->mu), so they cross to the host in one copy per stream.
+// indices are first packed on the device into `pack`, so they cross to the host in one copy per stream.
+// Without err_out, `pack` is grown here when too small (needs ctx->mu) and errors go to ctx->err; with it,
+// the caller has grown `pack` to download_pack_bytes already and the message goes to *err_out (no ctx state
+// is touched).  Returns only after every copy it enqueued has finished.
 t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uint32_t p1, hipStream_t stream,
                                 t3d_vertex* cv, t3d_vertex* dv, uint32_t* idx, uint32_t* tri_ids,
-                                t3d::DevBuf* pack = nullptr, uint64_t* pack_cap = nullptr);
+                                t3d::DevBuf* pack = nullptr, uint64_t* pack_cap = nullptr,
+                                std::string* err_out = nullptr);
 // Frame memory a job may need (scratch + outputs + tables), before frustum culling.
 uint64_t estimate_job_bytes(t3d_ctx* c, const t3d_render_desc* desc);
 

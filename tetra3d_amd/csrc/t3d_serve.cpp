@@ -55,6 +55,9 @@ struct Job {
   uint64_t id = 0;
   OwnedDesc d;
   std::vector<uint64_t> meshes;      // referenced mesh handles (refs held)
+  std::vector<uint64_t> volumes;     // referenced light volumes (refs held)
+  std::vector<uint64_t> animations;  // referenced animations (refs held)
+  bool solo = false;                 // runs in a batch of its own (its batch failed to build with others)
   uint64_t est_bytes = 0;
   uint64_t triangles = 0;
   int32_t state = T3D_JOB_QUEUED;
@@ -103,6 +106,12 @@ struct t3d_server {
   std::vector<t3d_frame*> free_frames;   // frames of finished, fully released batches
   hipStream_t dl_stream = nullptr;       // downloads
   std::mutex dl_mu;
+  // download packing buffer of t3d_job_download, shared by all jobs: grown under dl_mu and ctx->mu, its size
+  // readable under mu (counted with the frames the server holds)
+  t3d::DevBuf pack;
+  uint64_t pack_cap = 0;
+  std::atomic<uint64_t> pack_bytes{0};
+  uint64_t wakeups = 0;                  // slot completions signalled from the slot streams
   std::thread worker;
   bool stop = false;
   uint64_t next_id = 1, next_batch = 1;
@@ -187,7 +196,13 @@ t3d_status copy_desc(t3d_ctx* c, const t3d_render_desc* in, Job& j) {
     }
     j.meshes.push_back(m.mesh);
     for (const auto& pr : mesh.parts) j.triangles += pr.triangle_count;
+    if (m.skinned && m.bone_poses && m.animation) j.animations.push_back(m.animation);
+    if (m.light_group_active && m.light_group)
+      for (uint32_t k = 0; k < m.light_group_count; k++)
+        if (m.light_group[k].type == T3D_LIGHT_VOLUME) j.volumes.push_back(m.light_group[k].volume);
   }
+  for (const auto& l : o.lights)
+    if (l.type == T3D_LIGHT_VOLUME) j.volumes.push_back(l.volume);
   o.desc.camera = &o.cam;
   o.desc.world = in->world ? &o.world : nullptr;
   o.desc.light_count = in->light_count;
@@ -207,12 +222,29 @@ void drop_job(t3d_server* s, Job& j) {
       auto it = s->ctx->meshes.find(h);
       if (it != s->ctx->meshes.end()) it->second->refs--;
     }
+    for (uint64_t h : j.volumes) {
+      auto it = s->ctx->volumes.find(h);
+      if (it != s->ctx->volumes.end()) it->second->refs--;
+    }
+    for (uint64_t h : j.animations) {
+      auto it = s->ctx->animations.find(h);
+      if (it != s->ctx->animations.end()) it->second->refs--;
+    }
   }
   j.meshes.clear();
+  j.volumes.clear();
+  j.animations.clear();
+}
+
+// Called with ctx->mu held: takes the references drop_job gives back.
+void hold_job(t3d_ctx* c, Job& j) {
+  for (uint64_t h : j.meshes) c->meshes[h]->refs++;
+  for (uint64_t h : j.volumes) c->volumes[h]->refs++;
+  for (uint64_t h : j.animations) c->animations[h]->refs++;
 }
 
 uint64_t frames_held(const t3d_server* s) {
-  uint64_t b = 0;
+  uint64_t b = s->pack_bytes.load();
   for (const auto& kv : s->batches_alive) b += frame_capacity(kv.second->frame);
   for (const t3d_frame* f : s->free_frames) b += frame_capacity(f);
   return b;
@@ -267,6 +299,37 @@ void complete_slot(t3d_server* s, int si) {
   else
     retire_batch(s, b);
   s->cv_done.notify_all();
+}
+
+// Called with s->mu held: a batch of several jobs failed to build or
+// launch.  Its jobs go back to the front of the queue, each to run in a
+// batch of its own, so only a job that fails by itself fails.
+void requeue_solo(t3d_server* s, Batch* b) {
+  for (auto it = b->jobs.rbegin(); it != b->jobs.rend(); ++it) {
+    Job& j = *s->jobs[*it];
+    j.state = T3D_JOB_QUEUED;
+    j.solo = true;
+    j.batch = nullptr;
+    j.batch_id = 0;
+    j.batch_jobs = 0;
+    s->inflight_bytes -= j.est_bytes;
+    s->queue.push_front(j.id);
+  }
+  b->running = false;
+  b->unreleased = 0;
+  retire_batch(s, b);
+}
+
+// Slot stream callback after a batch's chain: wakes the worker, which then
+// finds the slot's completion event done.  Runs on a runtime thread; takes
+// the server lock and calls no HIP function.
+void slot_done(void* arg) {
+  t3d_server* s = static_cast<t3d_server*>(arg);
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->wakeups++;
+  }
+  s->cv_work.notify_one();
 }
 
 // Called with s->mu held: the batch never ran (or its launch failed).
@@ -329,11 +392,12 @@ void worker_main(t3d_server* s) {
       while (!s->queue.empty() && take.size() < s->cfg.max_batch_jobs) {
         Job& j = *s->jobs[s->queue.front()];
         if (s->cfg.hbm_budget_bytes && s->inflight_bytes + j.est_bytes > s->cfg.hbm_budget_bytes) break;
-        if (!take.empty() && need + j.est_bytes > batch_cap) break;
+        if (!take.empty() && (j.solo || need + j.est_bytes > batch_cap)) break;
         s->inflight_bytes += j.est_bytes;
         need += j.est_bytes;
         take.push_back(j.id);
         s->queue.pop_front();
+        if (j.solo) break;
       }
       if (!take.empty()) {
         Slot& sl = s->slots[free_slot];
@@ -391,23 +455,34 @@ void worker_main(t3d_server* s) {
           }
           st = frame_build(s->ctx, b->frame, descs.data(), (uint32_t)descs.size());
           if (st == T3D_OK) st = frame_launch(s->ctx, b->frame, sl.stream, sl.stream, T3D_EXEC_BATCHED, false, false);
-          if (st != T3D_OK) err = s->ctx->err;
+          if (st != T3D_OK) {
+            err = s->ctx->err;
+            (void)hipStreamSynchronize(sl.stream);   // nothing of a failed launch stays in flight
+            frame_release(s->ctx, b->frame);          // buffers sized for the failed batch go back at once
+          }
         }
         if (st == T3D_OK && hipEventRecord(sl.done, sl.stream) != hipSuccess) {
           st = T3D_ERR_DEVICE;
           err = "hipEventRecord failed";
         }
+        if (st == T3D_OK && hipLaunchHostFunc(sl.stream, slot_done, s) != hipSuccess) {
+          st = T3D_ERR_DEVICE;
+          err = "hipLaunchHostFunc failed";
+          (void)hipStreamSynchronize(sl.stream);
+        }
         lk.lock();
         sl.reserved = false;
         if (st != T3D_OK) {
           sl.batch = nullptr;
-          fail_batch(s, b, st, err);
+          if (take.size() > 1) requeue_solo(s, b);
+          else fail_batch(s, b, st, err);
         }
         continue;
       }
     }
     if (!progressed) {
-      if (running) s->cv_work.wait_for(lk, std::chrono::microseconds(20));
+      // woken by submit, release, stop, or a slot stream's completion callback; the timeout is only a safety net
+      if (running) s->cv_work.wait_for(lk, std::chrono::milliseconds(2));
       else s->cv_work.wait(lk);
     }
   }
@@ -486,6 +561,7 @@ t3d_status t3d_server_destroy(t3d_server* s) {
     }
     (void)hipStreamSynchronize(s->dl_stream);
     (void)hipStreamDestroy(s->dl_stream);
+    dev_free(s->ctx, s->pack);
     for (t3d_frame* f : frames) {
       frame_release(s->ctx, f);
       delete f;
@@ -504,11 +580,13 @@ t3d_status t3d_job_submit(t3d_server* s, const t3d_render_desc* desc, uint64_t* 
     std::lock_guard<std::mutex> cl(s->ctx->mu);
     t3d_status st = copy_desc(s->ctx, desc, *j);
     if (st) return st;
+    // every refusal a render of this descriptor alone would give, so a bad job never fails a batch
+    st = validate_desc(s->ctx, &j->d.desc);
+    if (st) return st;
     j->est_bytes = estimate_job_bytes(s->ctx, &j->d.desc);
     over_budget = s->cfg.hbm_budget_bytes && j->est_bytes > s->cfg.hbm_budget_bytes;
     if (over_budget) s->ctx->err = "submit: the job alone needs more frame memory than the server's HBM budget";
-    else
-      for (uint64_t h : j->meshes) s->ctx->meshes[h]->refs++;
+    else hold_job(s->ctx, *j);
   }
   // the server lock is taken only after the context lock is dropped (the
   // worker nests them server -> context)
@@ -617,15 +695,22 @@ t3d_status t3d_job_download(t3d_server* s, uint64_t job, t3d_pair_result* pairs,
   if (cv || dv || idx || tri_ids) {
     (void)hipSetDevice(s->ctx->device);
     std::lock_guard<std::mutex> dl(s->dl_mu);   // one download at a time on the download stream
-    // the batch frame's packing buffer: grown under ctx->mu (memory accounting), used under dl_mu only
-    t3d_frame* fm = const_cast<t3d_frame*>(f);
+    // the server's packing buffer: grown under ctx->mu (memory accounting), used under dl_mu only
     const uint64_t need = (cv || dv || idx) ? download_pack_bytes(f, p0, p1) : 0;
-    if (need) {
+    if (need > s->pack_cap) {
       std::lock_guard<std::mutex> cl(s->ctx->mu);
-      st = ensure(s->ctx, fm->pack, fm->pack_cap, need);
+      st = ensure(s->ctx, s->pack, s->pack_cap, need);
+      s->pack_bytes = s->pack.bytes;
     }
-    if (!st)
-      st = frame_download_range(s->ctx, f, p0, p1, s->dl_stream, cv, dv, idx, tri_ids, &fm->pack, &fm->pack_cap);
+    if (!st) {
+      // errors are reported into a local string (the worker may be writing ctx->err under ctx->mu)
+      std::string err;
+      st = frame_download_range(s->ctx, f, p0, p1, s->dl_stream, cv, dv, idx, tri_ids, &s->pack, &s->pack_cap, &err);
+      if (st) {
+        std::lock_guard<std::mutex> cl(s->ctx->mu);
+        s->ctx->err = err;
+      }
+    }
   }
   std::lock_guard<std::mutex> lk(s->mu);
   jp->pins--;

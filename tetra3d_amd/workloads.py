@@ -505,12 +505,14 @@ def _mixed_chunk(items):
     return [_mixed_one(a) for a in items]
 
 
-def mixed_models(rng, count, skinned_every=4, spread=(60.0, 60.0, 30.0, 250.0), vmin=24, vmax=65536, workers=1):
+def mixed_models(rng, count, skinned_every=4, spread=(60.0, 60.0, 30.0, 250.0), vmin=24, vmax=65536, workers=1,
+                 only=None):
     """`count` models with vertex counts log-uniform in [vmin, vmax]: grids
     (with random vertex colours) and, every fourth model, shared-size
     icospheres; one model in `skinned_every` is skinned.  Model i draws its
     mesh and skin from streams of its own, so `workers` processes build the
-    same models as one."""
+    same models as one, and `only` (a set of indices) builds just those,
+    returned as a dict {index: Model}."""
     sx, sy, z0, z1 = spread
     lv = rng.child(11).floats(count).astype(np.float64)
     targets = np.exp(np.log(vmin) + (np.log(vmax) - np.log(vmin)) * lv)
@@ -520,6 +522,8 @@ def mixed_models(rng, count, skinned_every=4, spread=(60.0, 60.0, 30.0, 250.0), 
     sphere_cache = {}
     items = []
     for i in range(count):
+        if only is not None and i not in only:
+            continue
         skinned = (i % skinned_every) == 1
         sphere = None
         if i % 4 == 3 and not skinned:        # unskinned icospheres share one mesh per level
@@ -528,6 +532,8 @@ def mixed_models(rng, count, skinned_every=4, spread=(60.0, 60.0, 30.0, 250.0), 
                 sphere_cache[level] = _mixed_mesh(None, targets[i], "sphere", i)
             sphere = sphere_cache[level]
         items.append((i, float(targets[i]), pos[i], quats[i], int(rng.seed), skinned, sphere))
+    if only is not None:
+        return {it[0]: m for it, m in zip(items, _mixed_chunk(items))}
     if workers <= 1 or count < 64:
         return _mixed_chunk(items)
     import multiprocessing as mp
@@ -655,13 +661,22 @@ class Job:
 
 
 def serving_trace(seed=DEFAULT_SEED, pool_size=10000, jobs=2000, rate_hz=2000.0, max_models=200, max_lights=16,
-                  workers=1):
+                  workers=1, models_for=None):
     """A resident pool of `pool_size` models (the mixed distribution) and a
     Poisson job trace at `rate_hz`: each job has a seeded camera (640x360 to
-    1920x1080), 1-`max_models` models and 1-`max_lights` lights."""
+    1920x1080), 1-`max_models` models and 1-`max_lights` lights.
+
+    models_for: a function of the job list returning the pool indices to
+    build; the models then come back as a dict {index: Model} holding just
+    those (each identical to the full pool's model of that index)."""
     rng = SplitMix64(seed)
-    models = mixed_models(rng.child(31), pool_size, workers=workers)
-    jr = rng.child(32)
+    out = _serving_jobs(rng.child(32), pool_size, jobs, rate_hz, max_models, max_lights)
+    only = None if models_for is None else set(int(i) for i in models_for(out))
+    models = mixed_models(rng.child(31), pool_size, workers=workers, only=only)
+    return models, out
+
+
+def _serving_jobs(jr, pool_size, jobs, rate_hz, max_models, max_lights):
     gaps = -np.log(np.maximum(jr.floats(jobs).astype(np.float64), 1e-12)) / rate_hz
     arrivals = np.cumsum(gaps)
     res = [(640, 360), (1280, 720), (1600, 900), (1920, 1080)]
@@ -679,7 +694,7 @@ def serving_trace(seed=DEFAULT_SEED, pool_size=10000, jobs=2000, rate_hz=2000.0,
             p = (jr.uniform(-60, 60, 1)[0], jr.uniform(-60, 60, 1)[0], jr.uniform(-250, -30, 1)[0])
             lights.append(_point("p%d_%d" % (j, k), p, 80))
         out.append(Job(float(arrivals[j]), cam, ids, lights))
-    return models, out
+    return out
 
 
 SCENES = {
