@@ -66,6 +66,15 @@ def measured_traffic(workload, kernel):
 
 
 def cpu_model():
+    """The host CPU's model name as lscpu prints it (else /proc/cpuinfo's)."""
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.strip().startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
@@ -76,11 +85,10 @@ def cpu_model():
     return platform.processor() or platform.machine()
 
 
-def cpu_baseline(wl, budget_s):
+def cpu_baseline(wl, warmup=3, frames=20):
     """The oracle (single-threaded restatement of the reference CPU path) on
-    whole frames of the same workload, pinned to one core: up to 3 warm-up
-    frames, then timed frames for about budget_s seconds (at least 3, and
-    at least 20 while they fit in that time).
+    whole frames of the same workload, pinned to one core: `warmup` untimed
+    frames, then `frames` timed ones.
     Returns the per-frame seconds of the timed frames and the warm-up count."""
     from oracle.oracle import Oracle
     o = Oracle()
@@ -99,15 +107,10 @@ def cpu_baseline(wl, budget_s):
     affinity = os.sched_getaffinity(0)
     os.sched_setaffinity(0, {min(affinity)})
     try:
-        first = o.render_seconds(wl.camera, wl.scene, wl.lights, wl.models)
-        warm = 1 + min(2, int(budget_s / 8 / max(first, 1e-9)))   # 3 when a frame is short enough
-        for _ in range(warm - 1):
+        for _ in range(warmup):
             o.render_seconds(wl.camera, wl.scene, wl.lights, wl.models)
-        times = []   # at least 20 frames when they are short, and about budget_s of them in all
-        while len(times) < 20 or sum(times) < budget_s:
-            times.append(o.render_seconds(wl.camera, wl.scene, wl.lights, wl.models))
-            if sum(times) >= budget_s and len(times) >= 3:
-                break
+        times = [o.render_seconds(wl.camera, wl.scene, wl.lights, wl.models) for _ in range(frames)]
+        warm = warmup
     finally:
         os.sched_setaffinity(0, affinity)
     for v, h in zip(wl.volumes, saved):
@@ -156,7 +159,24 @@ def job_inputs(ctx, models, jobs, scene):
     return out
 
 
+def serving_pool(args):
+    """The serving workload's pool and trace, generated with worker
+    processes: called before this process first touches the GPU."""
+    from tetra3d_amd import workloads as W
+    t = time.perf_counter()
+    models, jobs = W.serving_trace(pool_size=args.pool, jobs=args.jobs, rate_hz=args.rate,
+                                   workers=min(16, os.cpu_count() or 1))
+    log("serving trace generated in %.1fs" % (time.perf_counter() - t))
+    return models, jobs
+
+
 def serve_bench(args):
+    out = serve_measure(args, *serving_pool(args), args.extra_rates, args.plain_jobs)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def serve_measure(args, models, jobs, extra_rates, plain_jobs):
     """Serving loop: a resident pool of mixed models, a Poisson job trace
     replayed at its arrival times against the server (continuous batching on
     several streams, admission by HBM budget); per-job latency is submit ->
@@ -168,13 +188,11 @@ def serve_bench(args):
     from tetra3d_amd.serving import Server
 
     t = time.perf_counter()
-    models, jobs = W.serving_trace(pool_size=args.pool, jobs=args.jobs, rate_hz=args.rate,
-                                   workers=min(16, os.cpu_count() or 1))
     scene = W.Scene("serving", W._world(0.2))
     pool_v = sum(m.mesh.vertex_count for m in models)
     pool_t = sum(m.mesh.triangle_count for m in models)
-    log("serving pool: %d models, %d vertices, %d triangles; %d jobs at %.0f/s (generated in %.1fs)"
-        % (len(models), pool_v, pool_t, len(jobs), args.rate, time.perf_counter() - t))
+    log("serving pool: %d models, %d vertices, %d triangles; %d jobs at %.0f/s"
+        % (len(models), pool_v, pool_t, len(jobs), args.rate))
     ctx = Context(0)
     t = time.perf_counter()
     for m in models:
@@ -209,7 +227,7 @@ def serve_bench(args):
         del warm
         # latency under heavier load: the same trace replayed faster
         sweep = []
-        for rate in [float(r) for r in args.extra_rates.split(",") if r]:
+        for rate in [float(r) for r in extra_rates.split(",") if r]:
             srv.reset_stats()
             arr = arrivals * (args.rate / rate)
             t0 = time.perf_counter()
@@ -239,7 +257,7 @@ def serve_bench(args):
     log("saturated: %d jobs in %d batches, %.3g tris/s" % (sat.completed, sat.batches, capacity))
 
     # plain per-job path on a sample: one kernel chain per render pair, jobs one after another
-    sample = list(range(0, len(jobs), max(1, len(jobs) // args.plain_jobs)))[:args.plain_jobs]
+    sample = list(range(0, len(jobs), max(1, len(jobs) // plain_jobs)))[:plain_jobs]
     ctx.render(jobs[sample[0]].camera, scene, jobs[sample[0]].lights,
                [models[i] for i in jobs[sample[0]].model_ids], A.EXEC_PLAIN, download=False)
     lat, tt = [], 0
@@ -277,9 +295,8 @@ def serve_bench(args):
                   "speedup_capacity_vs_plain": capacity / plain_value,
                   "speedup_at_rates_vs_plain": {str(int(x["rate_hz"])): x["value"] / plain_value for x in sweep}},
     }
-    print(json.dumps(out), flush=True)
     ctx.close()
-    return 0
+    return out
 
 
 def hbm_probe(nbytes=1 << 30, reps=20):
@@ -321,8 +338,11 @@ def main():
     ap.add_argument("--sync-only", action="store_true",
                     help="time synchronous frames only (no two frames in flight): for profiler runs, where "
                          "overlapping frames would stretch each dispatch")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-warmup", type=int, default=3)
+    ap.add_argument("--cpu-frames", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve-summary", action="store_true",
+                    help="leave the serving summary (10,000-model pool, 2,000 jobs at 2,000/s) out of the frame line")
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain device copy of BYTES (counter calibration) and exit")
@@ -366,6 +386,10 @@ def main():
         return 0
 
     wname, wdesc = WORKLOADS[args.workload]
+    # the serving summary's pool is generated first, by worker processes, before the GPU is touched
+    serve_pool = None
+    if args.workload == "mixed" and not args.no_serve_summary:
+        serve_pool = serving_pool(args)
     t = time.perf_counter()
     wl = W.SCENES[args.workload]()
     log("workload %s: %d models, %d vertices, %d triangles (generated in %.1fs)"
@@ -496,9 +520,25 @@ def main():
                  "speedup_batched_vs_plain": value / plain_value}
         log("plain: %.3f ms/frame, %.3g tris/s" % (plain_ms, plain_value))
 
+    serving = None
+    if serve_pool is not None:
+        full = serve_measure(args, *serve_pool, "", min(args.plain_jobs, 10))
+        serve_pool = None
+        serving = {"workload": full["config"]["workload"], "value": full["value"], "unit": "tris/s",
+                   "p50_ms": full["latency_ms"]["p50"], "p99_ms": full["latency_ms"]["p99"],
+                   "capacity_tris_per_s": full["saturated"]["value"], "capacity_p99_ms": full["saturated"]["p99_ms"],
+                   "mean_jobs_per_batch": full["mean_jobs_per_batch"], "batches": full["batches"],
+                   "completed": full["completed"], "offered_tris_per_s": full["config"]["offered_tris_per_s"],
+                   "plain_per_job_tris_per_s": full["plain"]["value"],
+                   "speedup_served_vs_plain": full["plain"]["speedup_served_vs_plain"],
+                   "speedup_capacity_vs_plain": full["plain"]["speedup_capacity_vs_plain"],
+                   "note": "10,000-model pool resident in HBM; 2,000 Poisson jobs at 2,000/s replayed at their arrival "
+                           "times, 4 slots, up to 32 jobs per batch; latency submit -> outputs in HBM; capacity: the "
+                           "same jobs submitted at once"}
+
     cpu = None
     if not args.no_cpu:
-        times, warm = cpu_baseline(wl, args.cpu_seconds)
+        times, warm = cpu_baseline(wl, args.cpu_warmup, args.cpu_frames)
         frames, el = len(times), float(sum(times))
         cpu = {"value": tris * frames / el, "unit": "tris/s", "cores": 1, "kind": "port",
                "sample": "%d warm-up and %d timed whole frame(s) of %s on the oracle (C restatement of camera.go/"
@@ -543,6 +583,7 @@ def main():
                      "algorithmic_bytes_per_launch": dk["algorithmic_bytes_per_launch"], "avg_ms": dk["avg_ms"],
                      "longest_kernel": longest},
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

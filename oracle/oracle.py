@@ -30,6 +30,7 @@ class OracleResult(C.Structure):
         ("indices", C.POINTER(C.c_uint32)),
         ("triangle_ids", C.POINTER(C.c_uint32)),
         ("depth_keys", C.POINTER(C.c_float)),
+        ("normals", C.POINTER(A.Vertex)),
     ]
 
 
@@ -134,6 +135,7 @@ class Frame:
         t = n // 3
         self.triangle_ids = np.ctypeslib.as_array(res.triangle_ids, shape=(t,)).copy() if t else np.zeros(0, np.uint32)
         self.depth_keys = np.ctypeslib.as_array(res.depth_keys, shape=(t,)).copy() if t else np.zeros(0, np.float32)
+        self.normals = _vertices(res.normals, n) if res.normals else None   # Camera.RenderNormals only
         self.survivors = t
 
 

@@ -714,6 +714,7 @@ typedef struct {
   v3* alt_nrm;   /* globalMeshAlteredVertexNormals */
   v4* clip;      /* globalVertexTransforms */
   float* lights; /* mesh.vertexLights.colors (RGBA) */
+  v3* tnrm;      /* globalVertexTransformedNormals (Camera.RenderNormals) */
 } oscratch;
 
 /* getLightCell with convertToXYZIndices: light.go:717-724, 1267-1272. */
@@ -875,6 +876,8 @@ typedef struct {
   uint32_t* idx;
   uint32_t* tri_ids;
   float* keys;
+  t3d_vertex* normals;      /* normalVertexList, when want_normals */
+  int want_normals;
   uint64_t n, cap;          /* vertices */
 } obuf;
 
@@ -887,6 +890,7 @@ static void obuf_reserve(obuf* b, uint64_t extra) {
   b->idx = (uint32_t*)realloc(b->idx, nc * sizeof(uint32_t));
   b->tri_ids = (uint32_t*)realloc(b->tri_ids, (nc / 3 + 1) * sizeof(uint32_t));
   b->keys = (float*)realloc(b->keys, (nc / 3 + 1) * sizeof(float));
+  if (b->want_normals) b->normals = (t3d_vertex*)realloc(b->normals, nc * sizeof(t3d_vertex));
   b->cap = nc;
 }
 
@@ -1249,6 +1253,13 @@ static int render_pair(t3do_ctx* c, const ocam* cam, const t3d_world_desc* world
   int skinned = model->skinned;
   int snapping = cd->vertex_snapping > 0;
   int backface = mat && mat->backface_culling;
+  int render_normals = cd->render_normals;
+  m4 mvn = m4_identity();                                           /* mvJustRForNormals, model.go:705-708 */
+  if (render_normals) {
+    m4 mv = m4_mult(M, &cam->view);
+    v3 dp, ds;
+    m4_decompose(&mv, &dp, &ds, &mvn);
+  }
 
   uint32_t T = pr->triangle_count;
   uint32_t* bucket_tri = (uint32_t*)malloc(sizeof(uint32_t) * (T ? T : 1));
@@ -1278,6 +1289,9 @@ static int render_pair(t3do_ctx* c, const ocam* cam, const t3d_world_desc* world
         s->clip[v].y = round32(s->clip[v].y * vs) / vs;
         s->clip[v].z = round32(s->clip[v].z * vs) / vs;
       }
+      /* model.go:814-818: the rotation of Model.Transform()·ViewMatrix() applied to the (shape-keyed or
+       * skinned) vertex position, as the reference writes it (not to the normal) */
+      if (render_normals) s->tnrm[v] = m4_mult_vec(&mvn, vert);
     }
     v3 stc = {0, 0, 0};
     int outOfBounds = 1;
@@ -1438,6 +1452,17 @@ static int render_pair(t3do_ctx* c, const ocam* cam, const t3d_world_desc* world
       }
       dv->src_x = uvU;
       dv->src_y = uvV;
+      if (render_normals) {                                                  /* camera.go:1710-1722 */
+        t3d_vertex* nv = &ob->normals[o];
+        memset(nv, 0, sizeof *nv);
+        nv->dst_x = dx;
+        nv->dst_y = dy;
+        nv->src_x = uvU;
+        nv->src_y = uvV;
+        nv->color_r = s->tnrm[v].x * 0.5f + 0.5f;
+        nv->color_g = s->tnrm[v].y * 0.5f + 0.5f;
+        nv->color_b = s->tnrm[v].z * 0.5f + 0.5f;
+      }
       if (vcol) {
         cv->color_r = vcol[4 * (size_t)v + 0] * mp[0];
         cv->color_g = vcol[4 * (size_t)v + 1] * mp[1];
@@ -1497,7 +1522,6 @@ int t3do_render(t3do_ctx* c, const t3d_render_desc* desc, t3do_result* res) {
   if (!desc || !desc->camera) return set_err(c, "render: missing camera");
   const t3d_camera_desc* cd = desc->camera;
   if (cd->width <= 0 || cd->height <= 0) return set_err(c, "render: camera size");
-  if (cd->render_normals) return set_err(c, "RenderNormals unsupported"), T3D_ERR_UNSUPPORTED;
   const t3d_world_desc* world = (desc->world && desc->world->present) ? desc->world : NULL;
 
   ocam cam;
@@ -1598,8 +1622,10 @@ int t3do_render(t3do_ctx* c, const t3d_render_desc* desc, t3do_result* res) {
     s.alt_nrm = (v3*)calloc(maxV, sizeof(v3));
     s.clip = (v4*)calloc(maxV, sizeof(v4));
     s.lights = (float*)calloc((size_t)maxV * 4, sizeof(float));
+    s.tnrm = cd->render_normals ? (v3*)calloc(maxV, sizeof(v3)) : NULL;
     obuf ob;
     memset(&ob, 0, sizeof ob);
+    ob.want_normals = cd->render_normals;
     uint32_t pair_cap = 0;
     for (int pass = 0; pass < 2 && rc == T3D_OK; pass++) {                        /* camera.go:2057-2109 */
       int32_t* list = pass == 0 ? solids : transp;
@@ -1647,7 +1673,9 @@ int t3do_render(t3do_ctx* c, const t3d_render_desc* desc, t3do_result* res) {
     free(s.alt_nrm);
     free(s.clip);
     free(s.lights);
+    free(s.tnrm);
     res->vertex_count = ob.n;
+    res->normals = ob.normals;
     res->color = ob.color;
     res->depth = ob.depth;
     res->indices = ob.idx;
@@ -1672,6 +1700,7 @@ void t3do_result_free(t3do_result* r) {
   free(r->indices);
   free(r->triangle_ids);
   free(r->depth_keys);
+  free(r->normals);
   memset(r, 0, sizeof *r);
 }
 

@@ -32,6 +32,7 @@ typedef struct {
   uint32_t* indices;             /* indexList per flush (pair-local), concatenated */
   uint32_t* triangle_ids;        /* mesh triangle index of each drawn triangle, draw order */
   float* depth_keys;             /* bucket depth of each drawn triangle */
+  t3d_vertex* normals;           /* normalVertexList per flush when Camera.RenderNormals, else NULL */
 } t3do_result;
 
 t3do_ctx* t3do_create(void);

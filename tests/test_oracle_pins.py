@@ -518,3 +518,67 @@ def test_auto_batch_static_skipped(oracle):
     b.auto_batch_static = True
     fr = render(oracle, [a, b, c], cam=persp_camera(far=50))
     assert [p[0] for p in fr.pairs] == [0, 2]
+
+
+# ---------------------------------------------------------------------------
+# RenderNormals
+# ---------------------------------------------------------------------------
+
+ROT_Z90 = np.array([[0, 1, 0, 0], [-1, 0, 0, 0], [0, 0, 1, 0], [0, 0, 0, 1]], dtype=f32)   # (x, y, z) -> (-y, x, z)
+
+
+def _normals(oracle, mdl, cam):
+    cam.render_normals = True
+    fr = render(oracle, [mdl], cam=cam)
+    assert fr.normals is not None and fr.normals.shape == fr.color.shape
+    return fr
+
+
+def test_render_normals_closed_form(oracle):
+    """Camera.RenderNormals (model.go:705-708, 814-818; camera.go:1710-1722):
+    the normal vertex takes the colour vertex's Dst and Src, and as colour
+    the rotation of Model.Transform()*ViewMatrix() applied to the vertex
+    POSITION (as the reference writes it), times 0.5 plus 0.5; alpha and the
+    custom fields stay 0."""
+    cam = ortho_camera()
+    mesh = tri_mesh(ORTHO_TRI, nrm=(0, 0, 1), uvs=np.array(PC_UV, dtype=f32))
+    mesh.mesh_parts[0].material.texture_size = (64, 32)
+    fr = _normals(oracle, model(mesh), cam)
+    for k in range(3):
+        c, _ = corner(fr, 0, k)
+        n = fr.normals[3 * list(fr.triangle_ids).index(0) + k]
+        np.testing.assert_array_equal(n[:4], c[:4])
+        p = np.array(ORTHO_TRI[k], dtype=f32)
+        np.testing.assert_array_equal(n[4:7], p * f32(0.5) + f32(0.5))
+        assert np.all(n[7:] == 0)
+
+
+def test_render_normals_rotation_only(oracle):
+    """Only the rotation of model*view counts: the model's translation and
+    scale and the camera's position drop out (Decompose's rotation has unit
+    rows and no translation), a model rotation turns the position, and a
+    camera turned the same way cancels it."""
+    cam = ortho_camera()
+    mdl = model(tri_mesh([(1, 0.5, 0.25), (-0.5, 0.25, 0.125), (0.25, -1, 0.5)]))
+    mdl.set_local_rotation(ROT_Z90)
+    mdl.set_local_scale(2, 2, 2)
+    mdl.set_local_position(0.5, 0, -5)
+    fr = _normals(oracle, mdl, cam)
+    assert fr.pairs[0][3] == 1
+    for k, p in enumerate([(1, 0.5, 0.25), (-0.5, 0.25, 0.125), (0.25, -1, 0.5)]):
+        n = fr.normals[3 * list(fr.triangle_ids).index(0) + k]
+        turned = np.array([-p[1], p[0], p[2]], dtype=f32)
+        np.testing.assert_array_equal(n[4:7], turned * f32(0.5) + f32(0.5))
+    cam2 = ortho_camera()
+    cam2.set_local_rotation(ROT_Z90)
+    cam2.set_local_position(0.25, 0.5, 0)
+    fr = _normals(oracle, mdl, cam2)
+    assert fr.pairs[0][3] == 1
+    for k, p in enumerate([(1, 0.5, 0.25), (-0.5, 0.25, 0.125), (0.25, -1, 0.5)]):
+        n = fr.normals[3 * list(fr.triangle_ids).index(0) + k]
+        np.testing.assert_array_equal(n[4:7], np.array(p, dtype=f32) * f32(0.5) + f32(0.5))
+
+
+def test_render_normals_off_gives_no_list(oracle):
+    fr = render(oracle, [model(tri_mesh(ORTHO_TRI))])
+    assert fr.normals is None
