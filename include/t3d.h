@@ -68,6 +68,10 @@ enum {
   T3D_LV_PER_OBJECT = 2,
   T3D_LV_OFF = 3
 };
+/* BillboardUpDirection (material.go:29-32) */
+enum { T3D_BILLBOARD_UP_GLOBAL_Y = 0, T3D_BILLBOARD_UP_CAMERA_Y = 1 };
+/* BillboardedDepthMode (material.go:44-47) */
+enum { T3D_DEPTH_MODE_DEFAULT = 0, T3D_DEPTH_MODE_UNBILLBOARDED = 1 };
 /* FogBlendMode (world.go:8-11) */
 enum { T3D_FOG_ADD = 0, T3D_FOG_SUB = 1, T3D_FOG_OVERWRITE = 2, T3D_FOG_TRANSPARENT = 3 };
 /* Light node types handled on the path (node.go:25-43) */
@@ -136,9 +140,11 @@ typedef struct {
   int32_t render_order;              /* Material.RenderOrder */
   int32_t use_texture;               /* Material.Texture != nil && Material.UseTexture */
   float texture_size[2];             /* texture Bounds().Dx(), Dy() */
-  int32_t billboard;                 /* BillboardEnabled: unsupported on the GPU path */
+  int32_t billboard;                 /* BillboardEnabled (model.go:634-693) */
   int32_t custom_depth_function;     /* CustomDepthFunction != nil: unsupported */
-  int32_t depth_mode;                /* BillboardedDepthMode */
+  int32_t depth_mode;                /* BillboardedDepthMode: T3D_DEPTH_MODE_* (unbillboarded needs a static model) */
+  int32_t billboard_up;              /* BillboardUpDirection: T3D_BILLBOARD_UP_* */
+  int32_t billboard_lock[3];         /* BillboardLockX, BillboardLockY, BillboardLockZ */
 } t3d_material_desc;
 
 /* A light as seen by Camera.Render (light.go:12-30 and the concrete types). */
@@ -240,7 +246,7 @@ typedef struct {
   t3d_mat4 transform;                /* Node.Transform(): world position is row 3 */
   t3d_mat4 world_rotation;           /* Node.WorldRotation() at the last Clear() (camera.go:1085-1088) */
   int32_t render_depth;              /* Camera.RenderDepth */
-  int32_t render_normals;            /* Camera.RenderNormals: unsupported */
+  int32_t render_normals;            /* Camera.RenderNormals: also fill the normal vertex list */
   int32_t perspective_corrected_texture_mapping;
   float depth_margin;                /* Camera.DepthMargin */
   float vertex_snapping;             /* Camera.VertexSnapping */
@@ -358,6 +364,13 @@ t3d_status t3d_frame_device_outputs(const t3d_frame* frame, const t3d_vertex** c
  * the frame is in flight (t3d_frame_wait first). */
 t3d_status t3d_frame_download(t3d_ctx* ctx, const t3d_frame* frame, t3d_vertex* color_vertices,
                               t3d_vertex* depth_vertices, uint32_t* indices);
+/* Camera.RenderNormals: the normalVertexList each flush draws into the
+ * normal texture (camera.go:1710-1722, 1989-1992), laid out like the colour
+ * vertices (device pointer; T3D_ERR_NOT_FOUND when the last render's camera
+ * does not render normals), and its copy to host memory, concatenated in
+ * pair order like t3d_frame_download. */
+t3d_status t3d_frame_device_normals(const t3d_frame* frame, const t3d_vertex** normal_vertices);
+t3d_status t3d_frame_download_normals(t3d_ctx* ctx, const t3d_frame* frame, t3d_vertex* normal_vertices);
 /* Mesh triangle index of every drawn triangle, pair-concatenated in draw
  * order: the order sortingTriangleBucket.ForEach visits them
  * (sortingTriangles.go:93-125).  Holds sum(survivor_count) entries. */
@@ -414,7 +427,8 @@ typedef struct {
   uint32_t store_mode;               /* assembly stores: bit 0 = direct per-lane stores instead of
                                         LDS-staged rows, bit 1 = non-temporal (streaming) stores,
                                         bit 2 = stage 32 rows at a time instead of 64, bit 3 =
-                                        software-pipelined gathers (0-4, 6, 8, 10, 12, 14) */
+                                        software-pipelined gathers, bit 4 = four wavefronts per block
+                                        (0-4, 6, 8, 10, 12, 14, 16, 18, 20, 22) */
   uint32_t assemble_split;           /* assembly blocks per tile: a power of two, tile/split >= 64 */
   uint32_t xcd_swizzle;              /* 1: each XCD takes a contiguous range of a launch's blocks */
   uint32_t vertex_split;             /* 1: clip positions and colours in two arrays instead of one
@@ -484,6 +498,9 @@ t3d_status t3d_job_info_get(t3d_server* s, uint64_t job, t3d_job_info* out);
  * pairs holds pair_count entries; the arrays 3*survivors (tri_ids: survivors). */
 t3d_status t3d_job_download(t3d_server* s, uint64_t job, t3d_pair_result* pairs, t3d_vertex* color_vertices,
                             t3d_vertex* depth_vertices, uint32_t* indices, uint32_t* triangle_ids);
+/* The normal vertex list of a completed job whose camera renders normals
+ * (3*survivors entries; T3D_ERR_NOT_FOUND otherwise). */
+t3d_status t3d_job_download_normals(t3d_server* s, uint64_t job, t3d_vertex* normal_vertices);
 /* Frees a finished job's outputs (its batch's memory is reused once all of
  * the batch's jobs are released). */
 t3d_status t3d_job_release(t3d_server* s, uint64_t job);

@@ -95,6 +95,19 @@ static v4 v4_unit(v4 v) {                                                       
 }
 static float v4_mag(v4 v) { return sqrt32(v.x * v.x + v.y * v.y + v.z * v.z); }         /* vector.go:1880-1882 */
 static v3 v3_from(const float* f) { v3 r = {f[0], f[1], f[2]}; return r; }
+/* Vector3.Cross: vector.go:164-176, components in the reference's order. */
+static v3 v3_cross(v3 a, v3 o) {
+  v3 r;
+  r.z = a.x * o.y - o.x * a.y;
+  r.y = a.z * o.x - o.z * a.x;
+  r.x = a.y * o.z - o.y * a.z;
+  return r;
+}
+/* Vector3.Equals: vector.go:415-425, 1e-4 per component. */
+static int v3_equals(v3 a, v3 b) {
+  const float eps = 1e-4f;
+  return !(fabsf(a.x - b.x) > eps || fabsf(a.y - b.y) > eps || fabsf(a.z - b.z) > eps);
+}
 
 /* ------------------------------------------------------------------------ */
 /* matrices (matrix.go, matrix_none.go, matrix_amd64.s)                      */
@@ -255,6 +268,58 @@ static m4 projection_orthographic(float near, float far, float right, float left
   p.m[2][2] = -2 / (far - near);
   p.m[3][3] = 1;
   return p;
+}
+
+/* NewMatrix4LookAt: matrix.go:788-815. */
+static m4 m4_lookat(v3 from, v3 to, v3 up) {
+  if (v3_equals(from, to)) return m4_identity();
+  v3 z = v3_unit(v3_sub(to, from));
+  up = v3_unit(up);
+  v3 nup = {-up.x, -up.y, -up.z};
+  if (v3_equals(z, up) || v3_equals(z, nup)) {
+    const v3 right = {1, 0, 0}, backward = {0, 0, 1};   /* WorldRight, WorldBackward (vector.go:15, 27) */
+    up = !v3_equals(up, right) ? right : backward;
+  }
+  v3 x = v3_unit(v3_cross(up, z));
+  v3 y = v3_cross(z, x);
+  m4 r = m4_identity();
+  r.m[0][0] = x.x; r.m[0][1] = x.y; r.m[0][2] = x.z;
+  r.m[1][0] = y.x; r.m[1][1] = y.y; r.m[1][2] = y.z;
+  r.m[2][0] = z.x; r.m[2][1] = z.y; r.m[2][2] = z.z;
+  return r;
+}
+
+/* The billboarded base transform of ProcessVertices: model.go:634-693.  The
+ * look-at from the model's world position to the camera, the locks in X, Y, Z
+ * order (each sets its row to the axis and zeroes that component of the other
+ * two rows before renormalising them, W kept), then Decompose's rotation times
+ * Scale(scale) times the look-at, with the position put back in row 3. */
+static m4 billboard_base(const m4* M, const t3d_material_desc* mat, v3 cam_pos, v3 cam_up) {
+  v3 up = {0, 1, 0};                                                /* WorldUp, vector.go:21 */
+  if (mat->billboard_up == T3D_BILLBOARD_UP_CAMERA_Y) up = cam_up;
+  m4 la = m4_lookat(m4_row3(M), cam_pos, up);
+  for (int axis = 0; axis < 3; axis++) {
+    if (!mat->billboard_lock[axis]) continue;
+    v4 e = {axis == 0 ? 1.0f : 0.0f, axis == 1 ? 1.0f : 0.0f, axis == 2 ? 1.0f : 0.0f, 0.0f};
+    m4_set_row(&la, axis, e);
+    for (int r = 0; r < 3; r++) {
+      if (r == axis) continue;
+      v4 row = m4_row(&la, r);
+      if (axis == 0) row.x = 0;
+      else if (axis == 1) row.y = 0;
+      else row.z = 0;
+      m4_set_row(&la, r, v4_unit(row));
+    }
+  }
+  v3 p, s;
+  m4 rot;
+  m4_decompose(M, &p, &s, &rot);
+  m4 sc = m4_scale(s.x, s.y, s.z);
+  m4 base = m4_mult(&rot, &sc);
+  base = m4_mult(&base, &la);
+  v4 r3 = {p.x, p.y, p.z, 1};
+  m4_set_row(&base, 3, r3);
+  return base;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -715,6 +780,7 @@ typedef struct {
   v4* clip;      /* globalVertexTransforms */
   float* lights; /* mesh.vertexLights.colors (RGBA) */
   v3* tnrm;      /* globalVertexTransformedNormals (Camera.RenderNormals) */
+  float* udepth; /* globalVertexDepthUnbillboarded */
 } oscratch;
 
 /* getLightCell with convertToXYZIndices: light.go:717-724, 1267-1272. */
@@ -1191,6 +1257,14 @@ static void skin_vertex(const omesh* m, const t3d_model_desc* model, uint32_t v,
 static int validate_model(t3do_ctx* c, const t3d_model_desc* model, const omesh* m, const t3d_camera_desc* cam) {
   if (model->has_vertex_transform_function || model->has_vertex_clip_function)
     return set_err(c, "model uses a Go vertex callback") , T3D_ERR_UNSUPPORTED;
+  /* A skinned model never writes the unbillboarded depth the depth list would read (model.go:760-775 has no
+   * such branch), so the reference draws whatever an earlier model left there: refused. */
+  if (model->skinned && model->materials)
+    for (uint32_t p = 0; p < m->d.part_count; p++) {
+      const t3d_material_desc* mat = &model->materials[p];
+      if (mat->present && mat->visible && mat->depth_mode == T3D_DEPTH_MODE_UNBILLBOARDED)
+        return set_err(c, "skinned model with an unbillboarded depth mode"), T3D_ERR_UNSUPPORTED;
+    }
   if (model->skinned) {
     if (!m->bones || !m->weights || (!model->bone_influences && !model->bone_poses))
       return set_err(c, "skinned model without skin data");
@@ -1241,13 +1315,18 @@ static int render_pair(t3do_ctx* c, const ocam* cam, const t3d_world_desc* world
 
   /* ProcessVertices early out: model.go:604-606 */
   if ((model->color[3] == 0 && is_transparent(model, mat)) || !model->visible) return T3D_OK;
-  if (mat && (mat->billboard || mat->custom_depth_function))
-    return set_err(c, "billboarded material / custom depth function"), T3D_ERR_UNSUPPORTED;
+  if (mat && mat->custom_depth_function)
+    return set_err(c, "custom depth function"), T3D_ERR_UNSUPPORTED;
 
   const m4* M = &model->transform;
   v3 camPos = cam->pos;
   v3 icp = inverted_pos(M, camPos);                                 /* model.go:629-630 */
-  m4 mvp = m4_mult(M, &cam->vp);                                    /* model.go:695 */
+  m4 bbase = *M;
+  if (mat && mat->billboard) bbase = billboard_base(M, mat, camPos, cam->up);
+  m4 mvp = m4_mult(&bbase, &cam->vp);                               /* model.go:695 */
+  /* DepthModeUnbillboarded (model.go:697-703): the depth list takes Z of the unbillboarded MVP, unsnapped */
+  const int unbb = mat && mat->depth_mode == T3D_DEPTH_MODE_UNBILLBOARDED;
+  m4 umvp = m4_mult(M, &cam->vp);
   float minDepth = MAXF32, maxDepth = -MAXF32;
   float camNear = cd->near, camFar = cd->far;
   int skinned = model->skinned;
@@ -1282,6 +1361,7 @@ static int render_pair(t3do_ctx* c, const ocam* cam, const t3d_world_desc* world
         s->alt_pos[v] = vert;
         s->alt_nrm[v] = normal;
         s->clip[v] = m4_mult_vec_w(&mvp, vert);
+        if (unbb) s->udepth[v] = m4_mult_vec_w(&umvp, vert).z;           /* model.go:800-803 */
       }
       if (snapping) {                                                        /* model.go:807-812 */
         float vs = cd->vertex_snapping;
@@ -1481,7 +1561,7 @@ static int render_pair(t3do_ctx* c, const ocam* cam, const t3d_world_desc* world
         cv->color_b *= L[2];
       }
       if (cd->render_depth) {
-        float vd = target.z;
+        float vd = unbb ? s->udepth[v] : target.z;                           /* camera.go:1765-1770 */
         float depth = (vd + dmp) / camSpread;
         if (depth < 0) depth = 0;
         else if (depth > 1) depth = 1;
@@ -1623,6 +1703,7 @@ int t3do_render(t3do_ctx* c, const t3d_render_desc* desc, t3do_result* res) {
     s.clip = (v4*)calloc(maxV, sizeof(v4));
     s.lights = (float*)calloc((size_t)maxV * 4, sizeof(float));
     s.tnrm = cd->render_normals ? (v3*)calloc(maxV, sizeof(v3)) : NULL;
+    s.udepth = (float*)calloc(maxV, sizeof(float));
     obuf ob;
     memset(&ob, 0, sizeof ob);
     ob.want_normals = cd->render_normals;
@@ -1674,7 +1755,9 @@ int t3do_render(t3do_ctx* c, const t3d_render_desc* desc, t3do_result* res) {
     free(s.clip);
     free(s.lights);
     free(s.tnrm);
+    free(s.udepth);
     res->vertex_count = ob.n;
+    if (ob.want_normals && !ob.normals) ob.normals = (t3d_vertex*)calloc(1, sizeof(t3d_vertex));   /* present, empty */
     res->normals = ob.normals;
     res->color = ob.color;
     res->depth = ob.depth;

@@ -179,6 +179,15 @@ def feature_scene(seed):
     for l in lights:
         if isinstance(l, LightVolume) and extra.random() < 0.5:
             l.offset = extra.uniform(-1.5, 1.5, 3).astype(f32)
+    cam.render_normals = bool(extra.random() < 0.35)                # the normal vertex list (camera.go:1710-1722)
+    for m in models:                                                 # billboards (model.go:634-706)
+        for p in m.mesh.mesh_parts:
+            if p.material is not None and extra.random() < 0.25:
+                p.material.billboard = True
+                p.material.billboard_up = int(extra.integers(0, 2))
+                p.material.billboard_lock = tuple(bool(x) for x in extra.random(3) < 0.3)
+                if not m.skinned:
+                    p.material.depth_mode = int(extra.integers(0, 2))
     return W.Workload("features_%d" % seed, cam, Scene("features", world), lights, models)
 
 

@@ -22,7 +22,10 @@ def compare_frames(gpu, ora, exact=True):
     np.testing.assert_array_equal(gpu.triangle_ids, ora.triangle_ids)
     np.testing.assert_array_equal(gpu.indices, ora.indices)
     summary = {"pairs": len(gpu.pairs), "survivors": gpu.survivors, "max_rel": 0.0, "bit_mismatch": 0}
-    for name in ("color", "depth"):
+    gn, on = getattr(gpu, "normals", None), getattr(ora, "normals", None)
+    assert (gn is None) == (on is None), "normal list present on one side only"
+    names = ("color", "depth") if gn is None else ("color", "depth", "normals")
+    for name in names:
         g, o = getattr(gpu, name), getattr(ora, name)
         mism = int(np.count_nonzero(g.view(np.uint32) != o.view(np.uint32)))
         e = rel_err(g, o)

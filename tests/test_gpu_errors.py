@@ -29,11 +29,6 @@ def test_invalid_render_inputs_are_refused(gpu_ctx, oracle):
     gpu_ctx.prepare(wl)
     print("invalid render inputs", flush=True)
 
-    def billboard(r):
-        r.models[0].materials[0].present = 1
-        r.models[0].materials[0].visible = 1
-        r.models[0].materials[0].billboard = 1
-
     def custom_depth(r):
         r.models[0].materials[0].present = 1
         r.models[0].materials[0].visible = 1
@@ -42,12 +37,10 @@ def test_invalid_render_inputs_are_refused(gpu_ctx, oracle):
     cases = [
         (lambda r: setattr(r.models[0], "mesh", 0x7FFFFFFF), A.T3D_ERR_NOT_FOUND),
         (lambda r: setattr(r.camera[0], "width", 0), A.T3D_ERR_INVALID_ARGUMENT),
-        (lambda r: setattr(r.camera[0], "render_normals", 1), A.T3D_ERR_UNSUPPORTED),
         (lambda r: setattr(r.models[0], "has_vertex_transform_function", 1), A.T3D_ERR_UNSUPPORTED),
         (lambda r: setattr(r.models[0], "has_vertex_clip_function", 1), A.T3D_ERR_UNSUPPORTED),
         (lambda r: setattr(r.lights[0], "type", 99), A.T3D_ERR_INVALID_ARGUMENT),
         (lambda r: setattr(r.models[0], "skinned", 1), A.T3D_ERR_INVALID_ARGUMENT),   # no skin data
-        (billboard, A.T3D_ERR_UNSUPPORTED),
         (custom_depth, A.T3D_ERR_UNSUPPORTED),
     ]
     for mutate, status in cases:
@@ -56,6 +49,14 @@ def test_invalid_render_inputs_are_refused(gpu_ctx, oracle):
     sk = many_bones_scene(0)
     gpu_ctx.prepare(sk)
     _refused(gpu_ctx, sk, lambda r: setattr(r.models[3], "bone_count", 10), A.T3D_ERR_INVALID_ARGUMENT)
+
+    # a skinned model with the unbillboarded depth mode (the reference reads a depth it never wrote)
+    def unbb_skinned(r):
+        m = next(i for i in range(r.model_count) if r.models[i].skinned)
+        r.models[m].materials[0].present = 1
+        r.models[m].materials[0].visible = 1
+        r.models[m].materials[0].depth_mode = A.DEPTH_MODE_UNBILLBOARDED
+    _refused(gpu_ctx, sk, unbb_skinned, A.T3D_ERR_UNSUPPORTED)
     # more lights on one model than the vertex pass stages (64)
     many = [W._point("p%d" % i, (0.1 * i, 2, 2), 10) for i in range(70)]
     lights_wl = W.Workload("many_lights", wl.camera, wl.scene, many, wl.models)

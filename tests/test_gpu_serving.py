@@ -133,11 +133,11 @@ def test_invalid_job_refused_at_submit_mix_unaffected(gpu_ctx, oracle, pool):
     good = [gpu_ctx.inputs(j.camera, scene, j.lights, job_models(models, j)) for j in jobs[:6]]
     alone = [gpu_ctx.render(j.camera, scene, j.lights, job_models(models, j)) for j in jobs[:6]]
 
-    def billboard(inp):
+    def custom_depth(inp):
         mat = inp.render.models[0].materials[0]
         mat.present = 1
         mat.visible = 1
-        mat.billboard = 1
+        mat.custom_depth_function = 1
 
     def bad_light(inp):
         inp.render.lights[0].type = 99
@@ -145,7 +145,7 @@ def test_invalid_job_refused_at_submit_mix_unaffected(gpu_ctx, oracle, pool):
     def go_callback(inp):
         inp.render.models[0].has_vertex_transform_function = 1
 
-    cases = [(billboard, A.T3D_ERR_UNSUPPORTED), (bad_light, A.T3D_ERR_INVALID_ARGUMENT),
+    cases = [(custom_depth, A.T3D_ERR_UNSUPPORTED), (bad_light, A.T3D_ERR_INVALID_ARGUMENT),
              (go_callback, A.T3D_ERR_UNSUPPORTED)]
     with Server(gpu_ctx, slots=1, max_batch_jobs=16) as srv:
         ids = []

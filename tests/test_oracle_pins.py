@@ -10,8 +10,10 @@ texture mapping with and without perspective correction and Custom0
 (camera.go:1689-1711), shape keys (mesh.go:796-822), LightVolume's
 PerVertexWithNormal and PerObject modes (light.go:583-663), LightGroup
 substitution (camera.go:1576-1581), transparency routing and the alpha-0
-early out (model.go:604-606, 1274-1282), and AutoBatchStatic
-(camera.go:2066)."""
+early out (model.go:604-606, 1274-1282), AutoBatchStatic (camera.go:2066),
+RenderNormals (model.go:705-708, 814-818; camera.go:1710-1722), and
+billboarding with its locks, up modes and unbillboarded depth
+(model.go:634-703, 800-803; matrix.go:788-815; camera.go:1761-1762)."""
 import numpy as np
 import pytest
 
@@ -582,3 +584,165 @@ def test_render_normals_rotation_only(oracle):
 def test_render_normals_off_gives_no_list(oracle):
     fr = render(oracle, [model(tri_mesh(ORTHO_TRI))])
     assert fr.normals is None
+
+
+# ---------------------------------------------------------------------------
+# billboarding (model.go:634-703, matrix.go:788-815)
+# ---------------------------------------------------------------------------
+
+# a camera turned to look down -X: its rotation rows are x = (0, 0, -1),
+# y = (0, 1, 0), z = (1, 0, 0), which is also LookAt(from = (-5, 0, 0),
+# to = origin, up = +Y) worked by hand: z = (1, 0, 0), x = up x z = (0, 0, -1),
+# y = z x x = (0, 1, 0)
+ROT_LOOK_NX = np.array([[0, 0, -1, 0], [0, 1, 0, 0], [1, 0, 0, 0], [0, 0, 0, 1]], dtype=f32)
+FLAT_TRI = [(1, 1, 0), (-1, 1, 0), (0, -1, 0)]       # ORTHO_TRI in the model's own frame
+
+
+def _board(pos, tri=FLAT_TRI, billboard=True, up=A.BILLBOARD_UP_GLOBAL_Y, lock=(False, False, False),
+           depth_mode=A.DEPTH_MODE_DEFAULT):
+    mdl = model(tri_mesh(tri))
+    mdl.set_local_position(*pos)
+    mat = mdl.mesh.mesh_parts[0].material
+    mat.billboard = billboard
+    mat.billboard_up = up
+    mat.billboard_lock = lock
+    mat.depth_mode = depth_mode
+    return mdl
+
+
+def _screen(fr):
+    """Colour and depth rows of drawn triangle 0, corners in mesh order."""
+    assert fr.pairs[0][3] == 1
+    rows = [corner(fr, 0, k) for k in range(3)]
+    return np.array([c for c, _ in rows]), np.array([d for _, d in rows])
+
+
+def test_billboard_facing_camera_is_identity(oracle):
+    """A billboard straight down the camera's view axis: LookAt gives
+    z = (0, 0, 1), x = (1, 0, 0), y = (0, 1, 0), the identity, so it draws as
+    the plain model does."""
+    plain = _screen(render(oracle, [_board((0, 0, -5), billboard=False)]))
+    board = _screen(render(oracle, [_board((0, 0, -5))]))
+    np.testing.assert_array_equal(board[0].view(np.uint32), plain[0].view(np.uint32))
+    np.testing.assert_array_equal(board[1].view(np.uint32), plain[1].view(np.uint32))
+
+
+def test_billboard_turns_to_a_side_camera(oracle):
+    """A camera at the origin looking down -X sees a billboard at (-5, 0, 0)
+    face on: base = LookAt with row 3 = the position, so a model-frame
+    corner (a, b, 0) lands at world (-5, b, -a) and at view (a, b, -5), the
+    screen of the plain triangle seen down -Z.  Without billboarding the
+    same triangle is edge on to that camera."""
+    cam = ortho_camera()
+    ref = _screen(render(oracle, [_board((0, 0, -5), billboard=False)], cam=cam))
+    side = ortho_camera()
+    side.set_local_rotation(ROT_LOOK_NX)
+    board = _screen(render(oracle, [_board((-5, 0, 0))], cam=side))
+    np.testing.assert_array_equal(board[0][:, :2], ref[0][:, :2])
+    np.testing.assert_array_equal(board[1][:, 4], ref[1][:, 4])
+    edge = render(oracle, [_board((-5, 0, 0), billboard=False)], cam=side)
+    if edge.pairs and edge.pairs[0][3]:
+        xs = edge.color[:3, 0]
+        assert xs[0] == xs[1] == xs[2]           # all three corners on one screen column
+
+
+def test_billboard_scale_applies_after_rotation(oracle):
+    """base = R * Scale * LookAt (model.go:695-697): with the model turned
+    90 degrees about Z and scaled (2, 1, 1), a corner (x, y, z) goes to
+    (-2y, x, z), where the model's own transform (scale first) gives
+    (-y, 2x, z)."""
+    def drawn(billboard):
+        mdl = _board((0, 0, -5), tri=[(1, 0.5, 0), (-0.5, 0.25, 0), (0.25, -1, 0)], billboard=billboard)
+        mdl.set_local_rotation(ROT_Z90)
+        mdl.set_local_scale(2, 1, 1)
+        return _screen(render(oracle, [mdl]))[0]
+    board, plain = drawn(True), drawn(False)
+    for k, (x, y) in enumerate([(1, 0.5), (-0.5, 0.25), (0.25, -1)]):
+        # orthographic scale 4 over 640x320: DstX = X/4*640 + 320, DstY = -(Y/2)*320 + 160
+        assert (board[k, 0], board[k, 1]) == (f32(-2 * y) / 4 * 640 + 320, -f32(x) / 2 * 320 + 160)
+        assert (plain[k, 0], plain[k, 1]) == (f32(-y) / 4 * 640 + 320, -f32(2 * x) / 2 * 320 + 160)
+
+
+def test_billboard_straight_below_swaps_in_world_right(oracle):
+    """With the camera straight above the billboard, the look direction is
+    the up vector, and LookAt takes WorldRight as up (matrix.go:798-804):
+    z = (0, 1, 0), x = (1, 0, 0) x z = (0, 0, 1), y = z x x = (1, 0, 0).  A
+    corner (a, b, 0) lands at world (b, -5, a); a camera looking down -Y
+    (rows x = (1, 0, 0), y = (0, 0, -1), z = (0, 1, 0)) sees it at view
+    (b, -a, -5)."""
+    down = ortho_camera()
+    down.set_local_rotation(np.array([[1, 0, 0, 0], [0, 0, -1, 0], [0, 1, 0, 0], [0, 0, 0, 1]], dtype=f32))
+    board = _screen(render(oracle, [_board((0, -5, 0))], cam=down))[0]
+    for k, (a, b, _) in enumerate(FLAT_TRI):
+        assert (board[k, 0], board[k, 1]) == (f32(b) / 4 * 640 + 320, -f32(-a) / 2 * 320 + 160)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2], ids=["x", "y", "z"])
+def test_billboard_locks(oracle, axis):
+    """BillboardLockX/Y/Z (model.go:644-689) set the locked axis's row to that
+    axis and drop that component from the other two rows.  Seen from a
+    billboard 3 aside and 4 behind (direction (0.6, 0, 0.8)), LookAt's rows
+    are (0.8, 0, -0.6), (0, 1, 0), (0.6, 0, 0.8); from 3 below and 4 behind,
+    (1, 0, 0), (0, 0.8, -0.6), (0, 0.6, 0.8).  LockX collapses the first to
+    the identity and keeps the second's tilt (corner (a, b, 0) at
+    (a, 0.8b)); LockY the other way round (the first keeps (0.8a, b));
+    LockZ collapses both.  Collapsed, it draws as the plain model does."""
+    wide = ortho_camera(scale=16)
+    aside, below = (3, 0, -4), (0, -3, -4)
+    lock = tuple(i == axis for i in range(3))
+    for pos in (aside, below):
+        plain = _screen(render(oracle, [_board(pos, billboard=False)], cam=wide))[0]
+        locked = _screen(render(oracle, [_board(pos, lock=lock)], cam=wide))[0]
+        free = _screen(render(oracle, [_board(pos)], cam=wide))[0]
+        assert not np.array_equal(free[:, :2], plain[:, :2])
+        kept = (axis == 0 and pos == below) or (axis == 1 and pos == aside)
+        if not kept:
+            np.testing.assert_array_equal(locked[:, :2], plain[:, :2])
+            continue
+        # DstX = X*640 + 320 with X = x/16, DstY = -Y*320 + 160 with Y = y/8
+        cx, cy = f32(pos[0]) / 16 * 640 + 320, -f32(pos[1]) / 8 * 320 + 160
+        for k, (a, b, _) in enumerate(FLAT_TRI):
+            sa = f32(0.8) * f32(a) if axis == 1 else f32(a)
+            sb = f32(0.8) * f32(b) if axis == 0 else f32(b)
+            np.testing.assert_allclose(locked[k, 0], sa / 16 * 640 + cx, rtol=0, atol=1e-4)
+            np.testing.assert_allclose(locked[k, 1], -sb / 8 * 320 + cy, rtol=0, atol=1e-4)
+
+
+def test_billboard_camera_up_rolls_with_the_camera(oracle):
+    """BillboardYUpCameraY takes the camera's up vector (camera.go:1088,
+    model.go:638-641).  A camera rolled 90 degrees about Z has up (-1, 0, 0);
+    LookAt then gives x = (0, 1, 0), y = (-1, 0, 0), z = (0, 0, 1), the roll
+    itself, so the billboard draws as the plain model seen by an unrolled
+    camera.  With the world's up it stays put and the roll turns it on
+    screen."""
+    rolled = ortho_camera()
+    rolled.set_local_rotation(ROT_Z90)
+    ref = _screen(render(oracle, [_board((0, 0, -5), billboard=False)]))[0]
+    cam_up = _screen(render(oracle, [_board((0, 0, -5), up=A.BILLBOARD_UP_CAMERA_Y)], cam=rolled))[0]
+    world_up = _screen(render(oracle, [_board((0, 0, -5))], cam=rolled))[0]
+    np.testing.assert_array_equal(cam_up[:, :2], ref[:, :2])
+    assert not np.array_equal(world_up[:, :2], ref[:, :2])
+
+
+def test_unbillboarded_depth_is_the_model_transform_depth(oracle):
+    """DepthModeUnbillboarded (model.go:698-703, 800-803; camera.go:1761-1762)
+    writes the depth of Model.Transform()*VP: the side camera's billboard at
+    (-5, 0, 0) draws face on, but corner (a, b, 0) keeps the depth of the
+    edge-on triangle, view z = a - 5, Z = (5 - a)/4, depth Z/8; the default
+    mode writes the billboard's own Z = 5/4.  The screen positions do not
+    change.  Fog still reads the billboarded Z."""
+    side = ortho_camera()
+    side.set_local_rotation(ROT_LOOK_NX)
+    bb = _screen(render(oracle, [_board((-5, 0, 0))], cam=side))
+    ub = _screen(render(oracle, [_board((-5, 0, 0), depth_mode=A.DEPTH_MODE_UNBILLBOARDED)], cam=side))
+    np.testing.assert_array_equal(ub[0][:, :2], bb[0][:, :2])
+    for k, (a, _, _) in enumerate(FLAT_TRI):
+        assert bb[1][k, 4] == f32(1.25) / f32(8)
+        assert ub[1][k, 4] == f32(5 - a) / 4 / f32(8)
+    fogged = ortho_camera()
+    fogged.set_local_rotation(ROT_LOOK_NX)
+    fogged.render_depth = False
+    w = _fog_world(A.FOG_ADD)
+    fb = _screen(render(oracle, [_board((-5, 0, 0))], w=w, cam=fogged))[0]
+    fu = _screen(render(oracle, [_board((-5, 0, 0), depth_mode=A.DEPTH_MODE_UNBILLBOARDED)], w=w, cam=fogged))[0]
+    np.testing.assert_array_equal(fu.view(np.uint32), fb.view(np.uint32))

@@ -32,6 +32,9 @@ TRANSPARENCY_AUTO, TRANSPARENCY_OPAQUE, TRANSPARENCY_ALPHA_CLIP, TRANSPARENCY_TR
 LIGHTING_DEFAULT, LIGHTING_FIXED_NORMALS, LIGHTING_DOUBLE_SIDED = 0, 1, 2
 # LightVolumeShadingMode (material.go:50-55)
 LV_PER_VERTEX_WITHOUT_NORMAL, LV_PER_VERTEX_WITH_NORMAL, LV_PER_OBJECT, LV_OFF = 0, 1, 2, 3
+# BillboardUpDirection (material.go:29-32), BillboardedDepthMode (material.go:44-47)
+BILLBOARD_UP_GLOBAL_Y, BILLBOARD_UP_CAMERA_Y = 0, 1
+DEPTH_MODE_DEFAULT, DEPTH_MODE_UNBILLBOARDED = 0, 1
 # FogBlendMode (world.go:8-11)
 FOG_ADD, FOG_SUB, FOG_OVERWRITE, FOG_TRANSPARENT = 0, 1, 2, 3
 # light types
@@ -99,6 +102,8 @@ class MaterialDesc(C.Structure):
         ("billboard", C.c_int32),
         ("custom_depth_function", C.c_int32),
         ("depth_mode", C.c_int32),
+        ("billboard_up", C.c_int32),
+        ("billboard_lock", C.c_int32 * 3),
     ]
 
 

@@ -220,7 +220,7 @@ uint64_t carve_size(uint64_t count) {
 struct Cam {
   const t3d_camera_desc* d;
   V3 pos;
-  M4 vp;
+  M4 view, vp;
   V3 fwd, right, up;
   float sf_x, sf_y, sf_tang, aspect;
 };
@@ -261,8 +261,8 @@ void cam_setup(Cam& c, const t3d_camera_desc* d) {
   c.sf_y = 1.0f / (float)std::cos((double)angle);
   c.sf_x = 1.0f / (float)std::cos(std::atan((double)(c.sf_tang * c.aspect)));
   M4 t = translate(-c.pos.x, -c.pos.y, -c.pos.z);                    // ViewMatrix, camera.go:638-648
-  M4 view = mult(t, transposed(d->world_rotation));
-  c.vp = mult(view, projection(d));                                  // camera.go:1329
+  c.view = mult(t, transposed(d->world_rotation));
+  c.vp = mult(c.view, projection(d));                                // camera.go:1329
   c.fwd = forward(d->world_rotation);
   c.right = right(d->world_rotation);
   c.up = up(d->world_rotation);
@@ -855,6 +855,10 @@ t3d_status run_chain(t3d_ctx* c, t3d_frame* f, uint32_t p0, uint32_t p1, hipStre
     ProfScope ps(c, st, "k_assemble", tr.uv, prof);
     HIP_TRY(c, launch_assemble(a, tb0, tb1 - tb0, c->tuning.store_mode, c->tuning.assemble_split, st));
   }
+  if (f->any_normals) {
+    ProfScope ps(c, st, "k_normals", 0, prof);
+    HIP_TRY(c, launch_normals(a, tb0, tb1 - tb0, st));
+  }
   return T3D_OK;
 }
 
@@ -874,7 +878,6 @@ t3d_status validate_desc(t3d_ctx* c, const t3d_render_desc* desc) {
   if (!desc || !desc->camera) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: null camera");
   const t3d_camera_desc* cd = desc->camera;
   if (cd->width <= 0 || cd->height <= 0) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: camera size %dx%d", cd->width, cd->height);
-  if (cd->render_normals) return fail(c, T3D_ERR_UNSUPPORTED, "render: Camera.RenderNormals is outside the GPU path");
   if (desc->light_count && !desc->lights) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: lights missing");
   if (desc->model_count && !desc->models) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: models missing");
   const t3d_world_desc* world = (desc->world && desc->world->present) ? desc->world : nullptr;
@@ -929,8 +932,12 @@ t3d_status validate_desc(t3d_ctx* c, const t3d_render_desc* desc) {
     for (uint32_t p = 0; p < mesh.parts.size(); p++) {
       const t3d_material_desc* mat = part_material(md, p);
       if (mat && !mat->visible) continue;
-      if (mat && (mat->billboard || mat->custom_depth_function))
-        return fail(c, T3D_ERR_UNSUPPORTED, "render: model %u part %u uses billboarding or a custom depth function", mi, p);
+      if (mat && mat->custom_depth_function)
+        return fail(c, T3D_ERR_UNSUPPORTED, "render: model %u part %u uses a custom depth function", mi, p);
+      // a skinned model never writes the unbillboarded depth its depth list would read (model.go:760-775): the
+      // reference draws what an earlier model left there
+      if (mat && md.skinned && mat->depth_mode == T3D_DEPTH_MODE_UNBILLBOARDED)
+        return fail(c, T3D_ERR_UNSUPPORTED, "render: skinned model %u part %u with an unbillboarded depth mode", mi, p);
       if (mat && (mat->triangle_sort_mode < T3D_SORT_BACK_TO_FRONT || mat->triangle_sort_mode > T3D_SORT_NONE))
         return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: triangle sort mode %d", mat->triangle_sort_mode);
       const bool lighting = world && world->lighting_on && !md.shadeless && !(mat && mat->shadeless);   // camera.go:1521-1528
@@ -993,6 +1000,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   f->bytes_uv.clear();
 
   double pair_list_ms = 0, pair_sort_ms = 0;   // frustum and render pairs; their sorts
+  bool any_normals = false;                    // some camera of the frame renders the normal list
   std::vector<uint8_t> in_frustum;
   for (uint32_t job = 0; job < ndesc; job++) {
     const auto job0 = std::chrono::steady_clock::now();
@@ -1158,6 +1166,9 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
     fc.perspective = cd->perspective;
     fc.pc_tex = cd->perspective_corrected_texture_mapping;
     fc.render_depth = cd->render_depth;
+    fc.render_normals = cd->render_normals ? 1 : 0;
+    fc.view = cam.view;
+    any_normals = any_normals || cd->render_normals;
     fc.fog_active = (!cd->render_depth && world && world->fog_on) ? 1 : 0;
     fc.fog_mode = world ? world->fog_mode : 0;
     if (world) {
@@ -1183,6 +1194,13 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
       f->parts.emplace_back();   // zeroed in place: no copy of the ~450-byte record per pair
       PartGPU& P = f->parts.back();
       P.model = md.transform;
+      // the base transform k_uniforms multiplies by VP: the model's, or its billboarded form (model.go:634-693)
+      if (mat && mat->billboard && !md.skinned)
+        P.mvp = billboard_base(md.transform, cam.pos,
+                               mat->billboard_up == T3D_BILLBOARD_UP_CAMERA_Y ? cam.up : v3(0.0f, 1.0f, 0.0f),
+                               mat->billboard_lock[0] != 0, mat->billboard_lock[1] != 0, mat->billboard_lock[2] != 0);
+      else
+        P.mvp = md.transform;
       P.cam = job;
       P.max_span = mesh.max_span;
       float mp[4] = {md.color[0], md.color[1], md.color[2], md.color[3]};   // camera.go:1597-1601
@@ -1200,6 +1218,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
       if (mesh.col) flags |= PF_HAS_COLOR;
       if (mesh.uv) flags |= PF_HAS_UV;
       if (mesh.K) flags |= PF_SHAPE_KEYS;
+      if (mat && mat->depth_mode == T3D_DEPTH_MODE_UNBILLBOARDED) flags |= PF_UNBB;
       const uint32_t sort_mode = mat ? (uint32_t)mat->triangle_sort_mode : T3D_SORT_BACK_TO_FRONT;
       flags |= sort_mode << PF_SORT_SHIFT;
       flags |= ((uint32_t)(mat ? mat->lighting_mode : 0) & 3u) << PF_LMODE_SHIFT;
@@ -1339,11 +1358,15 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
                           2 * carve_size<uint4>(sumT) + carve_size<uint32_t>(4ull * P_n) + carve_size<uint32_t>(sumHist) +
                           carve_size<uint32_t>(sblocks) + 2 * carve_size<uint32_t>(ntiles_total) +
                           carve_size<uint32_t>(f->vchunk_prefix.back()) + carve_size<float4>(128) +
-                          carve_size<PartLight>(n_plights) + carve_size<M4>(pose_bones);
+                          carve_size<PartLight>(n_plights) + carve_size<M4>(pose_bones) +
+                          carve_size<float4>(any_normals ? sumV : 1) + carve_size<float>(sumV);
   uint8_t *tables_dev, *tables_host, *scratch_dev;
-  t3d_vertex *out_color, *out_depth;
+  t3d_vertex *out_color, *out_depth, *out_normal = nullptr;
   uint32_t* out_idx;
   if (place) {
+    if (any_normals)
+      return fail(c, T3D_ERR_UNSUPPORTED, "render_noalloc: Camera.RenderNormals needs a normal vertex array, which "
+                                          "this entry point does not take");
     // no allocation: tables and scratch carved from the placement's scratch, outputs the caller's
     if (tbytes + sbytes > place->scratch_bytes)
       return fail(c, T3D_ERR_BUDGET, "render_noalloc: the frame needs %llu bytes of scratch, the static scratch holds %llu",
@@ -1370,7 +1393,8 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
     }
     st = ensure(c, f->scratch, f->scratch_cap, sbytes);
     if (st) return st;
-    const uint64_t obytes = 2 * carve_size<t3d_vertex>(3 * sumT) + carve_size<uint32_t>(3 * sumT);
+    const uint64_t obytes = 2 * carve_size<t3d_vertex>(3 * sumT) + carve_size<uint32_t>(3 * sumT) +
+                            (any_normals ? carve_size<t3d_vertex>(3 * sumT) : 0);
     st = ensure(c, f->outputs, f->outputs_cap, obytes);
     if (st) return st;
     if (4ull * P_n > f->pinned_stats_cap) {
@@ -1386,7 +1410,9 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
     out_color = oc.take<t3d_vertex>(3 * sumT);
     out_depth = oc.take<t3d_vertex>(3 * sumT);
     out_idx = oc.take<uint32_t>(3 * sumT);
+    if (any_normals) out_normal = oc.take<t3d_vertex>(3 * sumT);
   }
+  f->any_normals = any_normals;
   f->tables_dev = tables_dev;
 
   // fill the host staging block with the same carve layout as the device block
@@ -1499,6 +1525,9 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   a.sink = sc.take<float4>(128);
   a.plights = sc.take<PartLight>(n_plights);
   a.pose_world = sc.take<M4>(pose_bones);
+  a.vnrm = sc.take<float4>(any_normals ? sumV : 1);
+  a.vud = sc.take<float>(sumV);
+  a.out_normal = reinterpret_cast<float4*>(out_normal);
   a.pal_jobs = d_pjobs;
   a.npal_jobs = npose;
   // poses and armature transforms read, palettes written
@@ -1634,24 +1663,25 @@ void frame_collect(t3d_ctx* c, t3d_frame* f, bool profile) {
 
 // Packing buffer layout: block prefix, source and destination offsets and vertex counts of np pairs, then the
 // packed colour and depth vertices and indices of `verts` corners.
-static uint64_t pack_layout_bytes(uint64_t np, uint64_t verts) {
+static uint64_t pack_layout_bytes(uint64_t np, uint64_t verts, bool normals) {
   return carve_size<uint32_t>(np + 1) + 2 * carve_size<uint64_t>(np) + carve_size<uint32_t>(np) +
-         2 * carve_size<t3d_vertex>(verts) + carve_size<uint32_t>(verts);
+         2 * carve_size<t3d_vertex>(verts) + carve_size<uint32_t>(verts) +
+         (normals ? carve_size<t3d_vertex>(verts) : 0);
 }
 
-uint64_t download_pack_bytes(const t3d_frame* f, uint32_t p0, uint32_t p1) {
+uint64_t download_pack_bytes(const t3d_frame* f, uint32_t p0, uint32_t p1, bool normals) {
   uint64_t np = 0, verts = 0;
   for (uint32_t i = p0; i < p1; i++)
     if (f->pairs[i].survivor_count) {
       np++;
       verts += 3ull * f->pairs[i].survivor_count;
     }
-  return verts ? pack_layout_bytes(np, verts) : 0;
+  return verts ? pack_layout_bytes(np, verts, normals) : 0;
 }
 
 t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uint32_t p1, hipStream_t stream,
-                                t3d_vertex* cv, t3d_vertex* dv, uint32_t* idx, uint32_t* tri_ids, DevBuf* pack,
-                                uint64_t* pack_cap, std::string* err_out) {
+                                t3d_vertex* cv, t3d_vertex* dv, uint32_t* idx, uint32_t* tri_ids, t3d_vertex* nv,
+                                DevBuf* pack, uint64_t* pack_cap, std::string* err_out) {
   // Every exit after the first asynchronous copy waits for the stream, so no copy is still writing into `recs`
   // or the caller's buffers when this returns; the message goes to *err_out when given, else to ctx->err.
   bool enqueued = false;
@@ -1672,22 +1702,23 @@ t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uin
   for (uint32_t i = p0; i < p1; i++) nrec += f->pairs[i].survivor_count;
   std::vector<uint4> recs(tri_ids ? std::max<uint64_t>(nrec, 1) : 0);
   // vertices and indices: the pairs' survivors packed on the device (k_pack), then one copy per stream
-  std::vector<uint32_t> blk(1, 0), nv;
+  std::vector<uint32_t> blk(1, 0), nvert;
   std::vector<uint64_t> src, dst;
-  if ((cv || dv || idx) && !pack) return bail(T3D_ERR_INVALID_ARGUMENT, "download: no packing buffer");
-  if (cv || dv || idx)
+  if (nv && !f->args.out_normal) return bail(T3D_ERR_NOT_FOUND, "download: the render has no normal list (Camera.RenderNormals off)");
+  if ((cv || dv || idx || nv) && !pack) return bail(T3D_ERR_INVALID_ARGUMENT, "download: no packing buffer");
+  if (cv || dv || idx || nv)
     for (uint32_t i = p0; i < p1; i++) {
       const uint64_t n = 3ull * f->pairs[i].survivor_count;
       if (!n) continue;
       src.push_back(f->pairs[i].vertex_offset);
       dst.push_back(off);
-      nv.push_back((uint32_t)n);
+      nvert.push_back((uint32_t)n);
       blk.push_back(blk.back() + (uint32_t)((3 * n + 255) / 256));   // 3 float4 per vertex, 256 to a block
       off += n;
     }
   if (off) {
-    const uint32_t np = (uint32_t)nv.size();
-    const uint64_t need = pack_layout_bytes(np, off);
+    const uint32_t np = (uint32_t)nvert.size();
+    const uint64_t need = pack_layout_bytes(np, off, nv != nullptr);
     if (need > *pack_cap) {
       if (err_out) return bail(T3D_ERR_INVALID_ARGUMENT, "download: packing buffer too small");   // grown by the caller
       t3d_status st = ensure(c, *pack, *pack_cap, need);
@@ -1702,7 +1733,7 @@ t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uin
     DL_TRY(hipMemcpyAsync(d_blk, blk.data(), 4ull * (np + 1), hipMemcpyHostToDevice, stream));
     DL_TRY(hipMemcpyAsync(d_src, src.data(), 8ull * np, hipMemcpyHostToDevice, stream));
     DL_TRY(hipMemcpyAsync(d_dst, dst.data(), 8ull * np, hipMemcpyHostToDevice, stream));
-    DL_TRY(hipMemcpyAsync(d_nv, nv.data(), 4ull * np, hipMemcpyHostToDevice, stream));
+    DL_TRY(hipMemcpyAsync(d_nv, nvert.data(), 4ull * np, hipMemcpyHostToDevice, stream));
     pa.blk_prefix = d_blk;
     pa.src = d_src;
     pa.dst = d_dst;
@@ -1711,13 +1742,20 @@ t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uin
     pa.color = f->args.out_color;
     pa.depth = f->args.out_depth;
     pa.idx = f->args.out_idx;
-    pa.pcolor = reinterpret_cast<float4*>(pc.take<t3d_vertex>(off));
-    pa.pdepth = reinterpret_cast<float4*>(pc.take<t3d_vertex>(off));
-    pa.pidx = pc.take<uint32_t>(off);
+    float4* pcol = reinterpret_cast<float4*>(pc.take<t3d_vertex>(off));
+    float4* pdep = reinterpret_cast<float4*>(pc.take<t3d_vertex>(off));
+    uint32_t* pidx = pc.take<uint32_t>(off);
+    float4* pnrm = nv ? reinterpret_cast<float4*>(pc.take<t3d_vertex>(off)) : nullptr;
+    pa.pcolor = cv ? pcol : nullptr;   // only the arrays asked for are packed
+    pa.pdepth = dv ? pdep : nullptr;
+    pa.pidx = idx ? pidx : nullptr;
+    pa.normal = f->args.out_normal;
+    pa.pnormal = pnrm;
     DL_TRY(launch_pack(pa, blk.back(), stream));
-    if (cv) DL_TRY(hipMemcpyAsync(cv, pa.pcolor, off * sizeof(t3d_vertex), hipMemcpyDeviceToHost, stream));
-    if (dv) DL_TRY(hipMemcpyAsync(dv, pa.pdepth, off * sizeof(t3d_vertex), hipMemcpyDeviceToHost, stream));
-    if (idx) DL_TRY(hipMemcpyAsync(idx, pa.pidx, off * 4, hipMemcpyDeviceToHost, stream));
+    if (cv) DL_TRY(hipMemcpyAsync(cv, pcol, off * sizeof(t3d_vertex), hipMemcpyDeviceToHost, stream));
+    if (dv) DL_TRY(hipMemcpyAsync(dv, pdep, off * sizeof(t3d_vertex), hipMemcpyDeviceToHost, stream));
+    if (idx) DL_TRY(hipMemcpyAsync(idx, pidx, off * 4, hipMemcpyDeviceToHost, stream));
+    if (nv) DL_TRY(hipMemcpyAsync(nv, pnrm, off * sizeof(t3d_vertex), hipMemcpyDeviceToHost, stream));
   }
   if (tri_ids)
     for (uint32_t i = p0; i < p1; i++) {
@@ -1745,7 +1783,9 @@ t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uin
 }
 
 uint64_t estimate_job_bytes(t3d_ctx* c, const t3d_render_desc* desc) {
-  // scratch: clip + colour + skinned position (48 B/V) and a block->pair entry per vertex block (at least 64
+  // scratch: clip + colour + skinned position + unbillboarded depth (52 B/V; a job whose camera renders normals
+  // adds 16 B/V here and
+  // 144 B/T of normal vertices) and a block->pair entry per vertex block (at least 64
   // vertices); key + two records (36 B/T); per tile a 512-bin histogram column, a scan block sum share, a survivor
   // count and a tile->pair entry (~2 KB); outputs: 3 x (48 + 48 + 4) = 300 B/T; tables: ~1 KB per model plus
   // palettes
@@ -1764,7 +1804,8 @@ uint64_t estimate_job_bytes(t3d_ctx* c, const t3d_render_desc* desc) {
     // palettes; posed bones add their poses (160 B) and an armature transform (64 B) in scratch
     extra += 1024 + (md.skinned ? (md.bone_poses ? 288ull : 64ull) * md.bone_count : 0);
   }
-  return 48 * V + 4 * (V / 64 + 1) + 336 * T + (4ull * kBins + 24) * tiles + extra;
+  const bool normals = desc->camera && desc->camera->render_normals;
+  return (normals ? 68 : 52) * V + 4 * (V / 64 + 1) + (normals ? 480 : 336) * T + (4ull * kBins + 24) * tiles + extra;
 }
 
 void frame_release(t3d_ctx* c, t3d_frame* f) {
@@ -1855,6 +1896,13 @@ t3d_status t3d_frame_device_outputs(const t3d_frame* f, const t3d_vertex** cv, c
   return T3D_OK;
 }
 
+t3d_status t3d_frame_device_normals(const t3d_frame* f, const t3d_vertex** nv) {
+  if (!f || !nv) return T3D_ERR_INVALID_ARGUMENT;
+  if (f->inflight) return T3D_ERR_NOT_READY;
+  *nv = reinterpret_cast<const t3d_vertex*>(f->args.out_normal);
+  return f->args.out_normal ? T3D_OK : T3D_ERR_NOT_FOUND;
+}
+
 double t3d_frame_last_ms(const t3d_frame* f) { return f ? f->last_ms : 0.0; }
 
 t3d_status t3d_frame_download(t3d_ctx* c, const t3d_frame* f, t3d_vertex* cv, t3d_vertex* dv, uint32_t* idx) {
@@ -1863,8 +1911,18 @@ t3d_status t3d_frame_download(t3d_ctx* c, const t3d_frame* f, t3d_vertex* cv, t3
   if (f->inflight) return fail(c, T3D_ERR_NOT_READY, "frame_download: render in flight (t3d_frame_wait first)");
   (void)hipSetDevice(c->device);
   t3d_frame* fm = const_cast<t3d_frame*>(f);   // the staging buffer only; the frame's results are untouched
-  return frame_download_range(c, f, 0, (uint32_t)f->pairs.size(), c->stream, cv, dv, idx, nullptr, &fm->pack,
-                              &fm->pack_cap);
+  return frame_download_range(c, f, 0, (uint32_t)f->pairs.size(), c->stream, cv, dv, idx, nullptr, nullptr,
+                              &fm->pack, &fm->pack_cap);
+}
+
+t3d_status t3d_frame_download_normals(t3d_ctx* c, const t3d_frame* f, t3d_vertex* nv) {
+  if (!c || !f || !nv) return T3D_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (f->inflight) return fail(c, T3D_ERR_NOT_READY, "frame_download: render in flight (t3d_frame_wait first)");
+  (void)hipSetDevice(c->device);
+  t3d_frame* fm = const_cast<t3d_frame*>(f);
+  return frame_download_range(c, f, 0, (uint32_t)f->pairs.size(), c->stream, nullptr, nullptr, nullptr, nullptr, nv,
+                              &fm->pack, &fm->pack_cap);
 }
 
 t3d_status t3d_frame_download_triangles(t3d_ctx* c, const t3d_frame* f, uint32_t* ids) {
@@ -1872,7 +1930,7 @@ t3d_status t3d_frame_download_triangles(t3d_ctx* c, const t3d_frame* f, uint32_t
   std::lock_guard<std::mutex> lk(c->mu);
   if (f->inflight) return fail(c, T3D_ERR_NOT_READY, "frame_download: render in flight (t3d_frame_wait first)");
   (void)hipSetDevice(c->device);
-  return frame_download_range(c, f, 0, (uint32_t)f->pairs.size(), c->stream, nullptr, nullptr, nullptr, ids);
+  return frame_download_range(c, f, 0, (uint32_t)f->pairs.size(), c->stream, nullptr, nullptr, nullptr, ids, nullptr);
 }
 
 t3d_status t3d_debug_model_spheres(const t3d_mat4* transforms, const float* dims, const uint8_t* skinned, uint32_t n,
@@ -1957,8 +2015,8 @@ t3d_status t3d_set_tuning(t3d_ctx* c, const t3d_tuning* t) {
   if (t->triangle_tile < 64 || t->triangle_tile > kMaxTile || t->triangle_tile % 64)
     return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: triangle_tile must be a multiple of 64 in [64, %u]", kMaxTile);
   if (!(t->store_mode <= 4 || t->store_mode == 6 || t->store_mode == 8 || t->store_mode == 10 || t->store_mode == 12 ||
-        t->store_mode == 14))
-    return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: store_mode is 0-4, 6, 8, 10, 12 or 14");
+        t->store_mode == 14 || t->store_mode == 16 || t->store_mode == 18 || t->store_mode == 20 || t->store_mode == 22))
+    return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: store_mode is 0-4, 6, 8, 10, 12, 14, 16, 18, 20 or 22");
   if (t->scatter_waves != 1 && t->scatter_waves != 4)
     return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: scatter_waves must be 1 or 4");
   if (t->assemble_split == 0 || (t->assemble_split & (t->assemble_split - 1)) ||
@@ -2008,7 +2066,8 @@ t3d_status t3d_autotune(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc, u
       {&t3d_tuning::vertices_per_thread, {1, 2, 4}},
       {&t3d_tuning::scatter_waves, {1, 4}},
       {&t3d_tuning::assemble_split, {1, 2, 4, 8, 16, 32}},
-      {&t3d_tuning::store_mode, {0, 2, 4, 6, 8, 10, 12, 14}},   // staged rows (64 or 32), streaming or not, pipelined or not
+      {&t3d_tuning::store_mode, {0, 2, 4, 6, 8, 10, 12, 14, 16, 18, 20, 22}},   // staged rows (64 or 32), streaming or
+                                                                               // not, pipelined or not, 1 or 4 waves
       {&t3d_tuning::xcd_swizzle, {0, 1}},
       {&t3d_tuning::vertex_split, {0, 1}},
       {&t3d_tuning::cull_block, {256, 512, 1024}},
@@ -2017,22 +2076,39 @@ t3d_status t3d_autotune(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc, u
   if (!valid(best)) best.assemble_split = 1;
   double best_ms = 0;
   t3d_status st = time_of(best, best_ms);
-  for (int sweep = 0; sweep < 2 && st == T3D_OK; sweep++)
+  auto consider = [&](const t3d_tuning& t) -> t3d_status {
+    double ms = 0;
+    t3d_status s2 = time_of(t, ms);
+    if (s2) return s2;
+    if (ms < best_ms * 0.99) {   // 1 % better, so timing noise does not pick between equals
+      best_ms = ms;
+      best = t;
+    }
+    return T3D_OK;
+  };
+  for (int sweep = 0; sweep < 2 && st == T3D_OK; sweep++) {
     for (const Axis& ax : axes) {
       for (uint32_t v : ax.values) {
         t3d_tuning t = best;
         t.*(ax.field) = v;
         if (t.*(ax.field) == best.*(ax.field) || !valid(t)) continue;
-        double ms = 0;
-        st = time_of(t, ms);
+        st = consider(t);
         if (st) break;
-        if (ms < best_ms * 0.99) {   // 1 % better, so timing noise does not pick between equals
-          best_ms = ms;
-          best = t;
-        }
       }
       if (st) break;
     }
+    // four-wave assembly blocks want larger slot ranges per block than one-wave blocks: store mode and
+    // assembly split together
+    for (uint32_t mode : {16u, 18u, 20u, 22u})
+      for (uint32_t split : {1u, 2u, 4u, 8u}) {
+        if (st) break;
+        t3d_tuning t = best;
+        t.store_mode = mode;
+        t.assemble_split = split;
+        if ((t.store_mode == best.store_mode && t.assemble_split == best.assemble_split) || !valid(t)) continue;
+        st = consider(t);
+      }
+  }
   if (st) {
     c->tuning = base;
     return st;

@@ -40,6 +40,7 @@ enum : uint32_t {
   PF_HAS_UV = 1u << 4,
   PF_SHAPE_KEYS = 1u << 5,
   PF_NEED_RD = 1u << 6,  // a directional light lights this static part: k_uniforms inverts its rotation
+  PF_UNBB = 1u << 7,     // DepthModeUnbillboarded: the depth list takes Z of Model.Transform()·VP, unsnapped
   PF_SORT_SHIFT = 8,     // 2 bits: T3D_SORT_*
   PF_LMODE_SHIFT = 10,   // 2 bits: T3D_LIGHTING_*
   PF_LVMODE_SHIFT = 12,  // 2 bits: T3D_LV_*
@@ -48,7 +49,8 @@ enum : uint32_t {
 // Per render pair: uniforms (camera.go:1507-1601, model.go:600-695) and the
 // pair's slices of the frame arrays.
 struct alignas(16) PartGPU {
-  M4 mvp;              // Model.Transform()·VP (model.go:695), computed by k_uniforms
+  M4 mvp;              // uploaded as the base transform (Model.Transform(), or its billboarded form, model.go:634-693);
+                       // k_uniforms turns it into base·VP (model.go:695)
   M4 rd;               // DirectionalLight working rotation: Transposed(Inverted(WorldRotation)) (light.go:326-330);
                        // uploaded as the model's WorldRotation, k_uniforms turns it into this
   M4 model;            // Model.Transform() (LightVolume lookups, light.go:610-628)
@@ -79,6 +81,8 @@ struct alignas(16) PartGPU {
   float max_span;             // MaxSpan of the mesh dimensions (mesh.go:41-43), for the point-light skip
   uint32_t light0, pad2;      // first FrameArgs.lights entry of the pair's light list
   uint64_t out_base;          // first output vertex
+  M4 nrot;                    // rotation of Model.Transform()·ViewMatrix() (model.go:705-708), by k_uniforms
+                              // when the pair's camera renders normals
 };
 
 // Frame-level light constants (light.go; beginRender results).
@@ -119,6 +123,9 @@ struct FrameConst {
   int32_t fog_active, fog_mode;
   float fog_color[4];
   float fog_range[2];
+  int32_t render_normals;   // Camera.RenderNormals: the pairs of this camera fill the normal vertex list
+  int32_t pad_fc[3];
+  M4 view;                  // ViewMatrix (camera.go:638-648), for the normal list's rotation
 };
 
 // One armature whose palette k_palettes computes from bone poses
@@ -165,6 +172,9 @@ struct FrameArgs {
   float4* out_color;            // t3d_vertex as 3 float4
   float4* out_depth;
   uint32_t* out_idx;
+  float4* vnrm;                 // [sumV] normal-list colour per vertex record (render_normals cameras only)
+  float* vud;                   // [sumV] unbillboarded depth Z per vertex record (PF_UNBB parts only)
+  float4* out_normal;           // normal vertices, laid out as out_color (null when no camera renders normals)
   const FrameConst* fc;         // [jobs] device copies, so graph kernel arguments stay fixed across frames
   const PaletteJob* pal_jobs;   // [npal_jobs] armatures posed on the device
   M4* pose_world;               // bone Transform() scratch of the palette jobs
@@ -184,9 +194,11 @@ struct PackArgs {
   const float4* color;          // frame outputs, t3d_vertex as 3 float4
   const float4* depth;
   const uint32_t* idx;
-  float4* pcolor;               // packed
+  float4* pcolor;               // packed (each null when not asked for)
   float4* pdepth;
   uint32_t* pidx;
+  const float4* normal;         // frame normal vertices and their packed copy (both null: none packed)
+  float4* pnormal;
 };
 
 // Kernel launchers (t3d_kernels.hip).  blk0 offsets blockIdx so the plain
@@ -203,6 +215,7 @@ hipError_t launch_scatter(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint
 hipError_t launch_assemble(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_t mode, uint32_t split,
                            hipStream_t s);
 hipError_t launch_pack(const PackArgs& a, uint32_t nblocks, hipStream_t s);
+hipError_t launch_normals(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 hipError_t launch_fill(void* dst, uint64_t bytes, uint32_t word, hipStream_t s);
 hipError_t launch_fp_selftest(const float* a, const float* b, float* out, uint32_t n, hipStream_t s);

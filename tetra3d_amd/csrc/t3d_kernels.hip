@@ -18,6 +18,7 @@
 //   k_assemble  per drawn triangle in slot order: ebiten colour/depth vertices
 //               and indices, staged in LDS and written as contiguous float4 rows
 //               with streaming stores (camera.go:1646-1820)
+//   k_normals   Camera.RenderNormals: the normal vertex list (camera.go:1710-1722)
 //
 // All arithmetic is float32 in the reference's order; the build uses
 // -ffp-contract=off and correctly rounded division and square root.
@@ -194,6 +195,8 @@ __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
     const float4* src = reinterpret_cast<const float4*>(P.palette);
     for (uint32_t i = threadIdx.x; i < 4u * P.bone_count; i += blockDim.x) s_bone_rows[(i / 4) * kBoneRows + i % 4] = src[i];
   }
+  __shared__ M4 s_umvp;                  // unaltered MVP of DepthModeUnbillboarded (model.go:697-703)
+  if ((flags & PF_UNBB) && threadIdx.x == 0) s_umvp = mult(P.model, a.fc[P.cam].vp);
   __syncthreads();
 
   const FrameConst fc = a.fc[P.cam];
@@ -263,6 +266,7 @@ __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
       vp = pos;
       vn = nrm;
       clip = mult_vec_w(P.mvp, pos);                                // model.go:795-798
+      if (flags & PF_UNBB) a.vud[P.vbase + i] = mult_vec_w(s_umvp, pos).z;   // model.go:800-803, not snapped
     }
     if (fc.snap > 0) {                                              // model.go:807-812
       clip.x = fdiv(roundf(clip.x * fc.snap), fc.snap);
@@ -310,6 +314,10 @@ __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
       }
     }
     const uint32_t o = P.vbase + i;
+    if (fc.render_normals) {   // model.go:814-818: the rotation applied to the position; colour camera.go:1718-1720
+      const V3 tn = mult_vec(P.nrot, vp);
+      a.vnrm[o] = make_float4(tn.x * 0.5f + 0.5f, tn.y * 0.5f + 0.5f, tn.z * 0.5f + 0.5f, 0.0f);
+    }
     a.vclip[(size_t)o * a.vstride] = make_float4(clip.x, clip.y, clip.z, clip.w);
     a.vcol[(size_t)o * a.vstride] = make_float4(cr, cg, cb, ca);
     if (skinned) a.vpos[o] = make_float4(vp.x, vp.y, vp.z, 0.0f);
@@ -475,7 +483,13 @@ __global__ __launch_bounds__(64) void k_uniforms(FrameArgs a, uint32_t part0, ui
     st[1] = 0u;                        // max over non-negative keys
     st[2] = 0u;                        // survivors
     st[3] = 0u;
-    P.mvp = mult(M, fc.vp);
+    P.mvp = mult(P.mvp, fc.vp);          // the uploaded base (model.go:634-695)
+    if (fc.render_normals) {             // mvJustRForNormals (model.go:705-708)
+      V3 np, ns;
+      M4 nr;
+      decompose(mult(M, fc.view), np, ns, nr);
+      P.nrot = nr;
+    }
     const V3 icp = add(mult_vec(dr, v3(fc.campos[0], fc.campos[1], fc.campos[2])), hmul(dp, is));
     P.icp[0] = icp.x;
     P.icp[1] = icp.y;
@@ -1046,8 +1060,8 @@ __device__ __forceinline__ void store1(uint32_t* p, uint32_t v) {
 // lane stores its own 144-byte runs); kNT: non-temporal (streaming) stores.
 // `split` blocks share one tile's slot range (a power of two dividing the
 // tile), so the grid follows the survivors more finely than the sort's tiles.
-template <int kStage, bool kNT>
-__global__ __launch_bounds__(64) void k_assemble(FrameArgs a, uint32_t blk0, uint32_t split) {
+template <int kStage, bool kNT, uint32_t kWaves>
+__global__ __launch_bounds__(64 * kWaves) void k_assemble(FrameArgs a, uint32_t blk0, uint32_t split) {
   const uint32_t li = block_index(a.xcd_swizzle);
   const uint32_t b = blk0 + li / split;
   const uint32_t p = a.tile_part[b];
@@ -1055,15 +1069,17 @@ __global__ __launch_bounds__(64) void k_assemble(FrameArgs a, uint32_t blk0, uin
   const uint32_t S = a.stats[4 * p + 2];
   const uint32_t sub = a.tile / split;
   const uint32_t k0 = (b - a.tile_prefix[p]) * a.tile + (li % split) * sub;
-  if (k0 >= S) return;
+  if (k0 >= S) return;                 // block-uniform: every wave of the block leaves together
   const uint32_t k1 = min(k0 + sub, S);
   const FrameConst fc = a.fc[P.cam];
-  const uint32_t lane = threadIdx.x;
+  const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
   // Everything uniform is read once into registers here: the output stores
   // below may alias these tables as far as the compiler knows, and would
   // otherwise force a reload (a full memory round trip) per vertex.
   const uint32_t vbase = P.vbase, vmin = P.vmin, tbase = P.tbase;
   const bool has_uv = (P.flags & PF_HAS_UV) != 0;
+  const bool unbb = (P.flags & PF_UNBB) != 0;
+  const float* __restrict__ vud = a.vud;
   const float src_w = P.src_w, src_h = P.src_h;
   const float* __restrict__ uvp = P.uv;
   const uint64_t out_base = P.out_base;
@@ -1079,18 +1095,23 @@ __global__ __launch_bounds__(64) void k_assemble(FrameArgs a, uint32_t blk0, uin
   uint32_t* __restrict__ out_i = a.out_idx + out_base;
   const float da = rdepth ? 1.0f : 0.0f;
 
-  // kStage 1: all 64 triangles staged at once; 2: 32 at a time, halving the
-  // LDS per wavefront so twice as many wavefronts fit on a CU
+  // kStage 1: all 64 triangles of a wave staged at once; 2: 32 at a time,
+  // halving the LDS per wavefront so twice as many wavefronts fit on a CU.
+  // With kWaves > 1 the block's waves take consecutive 64-triangle chunks of
+  // its slot range, each staging in its own LDS rows.
   constexpr uint32_t kRows = kStage == 2 ? 32 : 64;
-  __shared__ float4 stage[kStage ? kRows * 9 : 1];
+  __shared__ float4 stage_all[kStage ? kWaves * kRows * 9 : 1];
+  float4* stage = stage_all + (kStage ? wave * kRows * 9 : 0);
+  constexpr uint32_t kStep = 64 * kWaves;
   uint4 vo_next = make_uint4(0, 0, 0, 0);   // slot records one chunk ahead
-  if (k0 + lane < k1) vo_next = tvo[k0 + lane];
-  for (uint32_t wk0 = k0; wk0 < k1; wk0 += 64) {
+  if (k0 + wave * 64 + lane < k1) vo_next = tvo[k0 + wave * 64 + lane];
+  for (uint32_t base = k0; base < k1; base += kStep) {   // block-uniform trip count
+    const uint32_t wk0 = base + wave * 64;
     const uint32_t k = wk0 + lane;
     const bool active = k < k1;
-    const uint32_t wcount = min(64u, k1 - wk0);
+    const uint32_t wcount = wk0 < k1 ? min(64u, k1 - wk0) : 0u;
     const uint4 vo = vo_next;
-    if (k + 64 < k1) vo_next = tvo[k + 64];
+    if (k + kStep < k1) vo_next = tvo[k + kStep];
     float4 cq[9];
     float dd[3];
     if (active) {
@@ -1120,8 +1141,9 @@ __global__ __launch_bounds__(64) void k_assemble(FrameArgs a, uint32_t blk0, uin
           uvV = (1.0f - V) * src_h;
         }
         float depth = 0.0f;
-        if (rdepth) {                                             // camera.go:1772-1784
-          depth = fdiv(c[vi].z + dmp, spread);
+        if (rdepth) {                                             // camera.go:1765-1784
+          const float zd = unbb ? vud[ov[vi]] : c[vi].z;          // DepthModeUnbillboarded: the unaltered Z
+          depth = fdiv(zd + dmp, spread);
           if (depth < 0) depth = 0;
           else if (depth > 1) depth = 1;
         }
@@ -1198,6 +1220,8 @@ __global__ __launch_bounds__(64) void k_assemble_pipe(FrameArgs a, uint32_t blk0
   const uint32_t lane = threadIdx.x;
   const uint32_t vbase = P.vbase, vmin = P.vmin, tbase = P.tbase;
   const bool has_uv = (P.flags & PF_HAS_UV) != 0;
+  const bool unbb = (P.flags & PF_UNBB) != 0;
+  const float* __restrict__ vud = a.vud;
   const float src_w = P.src_w, src_h = P.src_h;
   const float2* __restrict__ uvp = reinterpret_cast<const float2*>(P.uv);
   const uint64_t out_base = P.out_base;
@@ -1219,6 +1243,7 @@ __global__ __launch_bounds__(64) void k_assemble_pipe(FrameArgs a, uint32_t blk0
   struct Gathered {
     float4 c[3], col[3];
     float2 uv[3];
+    float zd[3];                 // Z the depth list takes: clip Z, or the unbillboarded Z
   };
   auto gather = [&](Gathered& g, const uint4 vo) {
     const uint32_t ov[3] = {vo.x, vo.y, vo.z};
@@ -1226,6 +1251,7 @@ __global__ __launch_bounds__(64) void k_assemble_pipe(FrameArgs a, uint32_t blk0
       g.c[vi] = vclip[(size_t)ov[vi] * vstride];
       g.col[vi] = vcol[(size_t)ov[vi] * vstride];
       g.uv[vi] = has_uv ? uvp[ov[vi] - vbase + vmin] : make_float2(0.0f, 0.0f);
+      g.zd[vi] = unbb ? vud[ov[vi]] : g.c[vi].z;
     }
   };
   auto compute = [&](const Gathered& g, float4 (&cq)[9], float (&dd)[3]) {
@@ -1246,8 +1272,8 @@ __global__ __launch_bounds__(64) void k_assemble_pipe(FrameArgs a, uint32_t blk0
         uvV = (1.0f - V) * src_h;
       }
       float depth = 0.0f;
-      if (rdepth) {                                               // camera.go:1772-1784
-        depth = fdiv(g.c[vi].z + dmp, spread);
+      if (rdepth) {                                               // camera.go:1765-1784
+        depth = fdiv(g.zd[vi] + dmp, spread);
         if (depth < 0) depth = 0;
         else if (depth > 1) depth = 1;
       }
@@ -1342,9 +1368,39 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs a) {
   const uint64_t n = a.nvert[lo];
   if (e >= 3 * n) return;
   const uint64_t s = a.src[lo], d = a.dst[lo];
-  a.pcolor[3 * d + e] = a.color[3 * s + e];
-  a.pdepth[3 * d + e] = a.depth[3 * s + e];
-  if (e < n) a.pidx[d + e] = a.idx[s + e];
+  if (a.pcolor) a.pcolor[3 * d + e] = a.color[3 * s + e];
+  if (a.pdepth) a.pdepth[3 * d + e] = a.depth[3 * s + e];
+  if (a.pnormal) a.pnormal[3 * d + e] = a.normal[3 * s + e];
+  if (a.pidx && e < n) a.pidx[d + e] = a.idx[s + e];
+}
+
+// Camera.RenderNormals: the normal vertex list (camera.go:1710-1722), after
+// the assembly.  Per drawn triangle in slot order, each corner takes the
+// colour vertex's DstX, DstY, SrcX, SrcY (its first float4, just written)
+// and as colour the vertex's rotated position times 0.5 plus 0.5
+// (k_vertex's vnrm); alpha and the custom fields stay 0.  One 256-thread
+// block per sort tile, one thread per (triangle, corner).
+__global__ __launch_bounds__(256) void k_normals(FrameArgs a, uint32_t blk0) {
+  const uint32_t b = blk0 + block_index(a.xcd_swizzle);
+  const uint32_t p = a.tile_part[b];
+  const PartGPU P = a.parts[p];
+  if (!a.fc[P.cam].render_normals) return;
+  const uint32_t S = a.stats[4 * p + 2];
+  const uint32_t k0 = (b - a.tile_prefix[p]) * a.tile;
+  if (k0 >= S) return;
+  const uint32_t k1 = min(k0 + a.tile, S);
+  const uint4* __restrict__ tvo = a.tri_vo + P.tbase;
+  const float4* __restrict__ oc = a.out_color + P.out_base * 3;
+  float4* __restrict__ on = a.out_normal + P.out_base * 3;
+  const float4* __restrict__ vn = a.vnrm;
+  for (uint32_t c = 3 * k0 + threadIdx.x; c < 3 * k1; c += blockDim.x) {   // corner c of the pair's list
+    const uint4 vo = tvo[c / 3];
+    const uint32_t vi = c % 3;
+    const uint32_t o = vi == 0 ? vo.x : (vi == 1 ? vo.y : vo.z);
+    on[3 * (size_t)c + 0] = oc[3 * (size_t)c + 0];
+    on[3 * (size_t)c + 1] = vn[o];
+    on[3 * (size_t)c + 2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
 }
 
 // Plain float4 copy of a known size: the reference point for HBM byte counters.
@@ -1425,12 +1481,17 @@ hipError_t launch_assemble(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uin
     case 10: hipLaunchKernelGGL((k_assemble_pipe<64, true>), g, dim3(64), 0, s, a, blk0, split); break;
     case 12: hipLaunchKernelGGL((k_assemble_pipe<32, false>), g, dim3(64), 0, s, a, blk0, split); break;
     case 14: hipLaunchKernelGGL((k_assemble_pipe<32, true>), g, dim3(64), 0, s, a, blk0, split); break;
-    case 0: hipLaunchKernelGGL((k_assemble<1, false>), g, dim3(64), 0, s, a, blk0, split); break;
-    case 1: hipLaunchKernelGGL((k_assemble<0, false>), g, dim3(64), 0, s, a, blk0, split); break;
-    case 2: hipLaunchKernelGGL((k_assemble<1, true>), g, dim3(64), 0, s, a, blk0, split); break;
-    case 3: hipLaunchKernelGGL((k_assemble<0, true>), g, dim3(64), 0, s, a, blk0, split); break;
-    case 4: hipLaunchKernelGGL((k_assemble<2, false>), g, dim3(64), 0, s, a, blk0, split); break;
-    case 6: hipLaunchKernelGGL((k_assemble<2, true>), g, dim3(64), 0, s, a, blk0, split); break;
+    case 0: hipLaunchKernelGGL((k_assemble<1, false, 1>), g, dim3(64), 0, s, a, blk0, split); break;
+    case 1: hipLaunchKernelGGL((k_assemble<0, false, 1>), g, dim3(64), 0, s, a, blk0, split); break;
+    case 2: hipLaunchKernelGGL((k_assemble<1, true, 1>), g, dim3(64), 0, s, a, blk0, split); break;
+    case 3: hipLaunchKernelGGL((k_assemble<0, true, 1>), g, dim3(64), 0, s, a, blk0, split); break;
+    case 4: hipLaunchKernelGGL((k_assemble<2, false, 1>), g, dim3(64), 0, s, a, blk0, split); break;
+    case 6: hipLaunchKernelGGL((k_assemble<2, true, 1>), g, dim3(64), 0, s, a, blk0, split); break;
+    // bit 4: four wavefronts per block, each staging its own rows
+    case 16: hipLaunchKernelGGL((k_assemble<1, false, 4>), g, dim3(256), 0, s, a, blk0, split); break;
+    case 18: hipLaunchKernelGGL((k_assemble<1, true, 4>), g, dim3(256), 0, s, a, blk0, split); break;
+    case 20: hipLaunchKernelGGL((k_assemble<2, false, 4>), g, dim3(256), 0, s, a, blk0, split); break;
+    case 22: hipLaunchKernelGGL((k_assemble<2, true, 4>), g, dim3(256), 0, s, a, blk0, split); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -1443,6 +1504,11 @@ hipError_t launch_fill(void* dst, uint64_t bytes, uint32_t word, hipStream_t s) 
 hipError_t launch_pack(const PackArgs& a, uint32_t nblocks, hipStream_t s) {
   if (!nblocks) return hipSuccess;
   hipLaunchKernelGGL(k_pack, dim3(nblocks), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_normals(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s) {
+  if (!nblk) return hipSuccess;
+  hipLaunchKernelGGL(k_normals, dim3(nblk), dim3(256), 0, s, a, blk0);
   return hipGetLastError();
 }
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s) {

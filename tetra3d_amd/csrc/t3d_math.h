@@ -190,6 +190,65 @@ T3D_HD V3 forward(const M4& a) { return unit(v3(a.m[2][0], a.m[2][1], a.m[2][2])
 T3D_HD V3 right(const M4& a) { return unit(v3(a.m[0][0], a.m[0][1], a.m[0][2])); }
 T3D_HD V3 up(const M4& a) { return unit(v3(a.m[1][0], a.m[1][1], a.m[1][2])); }
 
+// Vector3.Cross (vector.go:164-176), components in the reference's order.
+T3D_HD V3 cross(V3 a, V3 o) {
+  V3 r;
+  r.z = a.x * o.y - o.x * a.y;
+  r.y = a.z * o.x - o.z * a.x;
+  r.x = a.y * o.z - o.y * a.z;
+  return r;
+}
+// Vector3.Equals (vector.go:415-425): within 1e-4 per component.
+T3D_HD bool equals3(V3 a, V3 b) {
+  const float eps = 1e-4f;
+  return !(fabsf(a.x - b.x) > eps || fabsf(a.y - b.y) > eps || fabsf(a.z - b.z) > eps);
+}
+// NewMatrix4LookAt (matrix.go:788-815).
+T3D_HD M4 look_at(V3 from, V3 to, V3 up) {
+  if (equals3(from, to)) return identity();
+  const V3 z = unit(sub(to, from));
+  up = unit(up);
+  if (equals3(z, up) || equals3(z, v3(-up.x, -up.y, -up.z)))
+    up = !equals3(up, v3(1, 0, 0)) ? v3(1, 0, 0) : v3(0, 0, 1);   // WorldRight, else WorldBackward (vector.go:15, 27)
+  const V3 x = unit(cross(up, z));
+  const V3 y = cross(z, x);
+  M4 r = identity();
+  r.m[0][0] = x.x; r.m[0][1] = x.y; r.m[0][2] = x.z;
+  r.m[1][0] = y.x; r.m[1][1] = y.y; r.m[1][2] = y.z;
+  r.m[2][0] = z.x; r.m[2][1] = z.y; r.m[2][2] = z.z;
+  return r;
+}
+// The billboarded base transform of ProcessVertices (model.go:634-693): the
+// look-at from the model's world position to the camera (up: WorldUp, or the
+// camera's up for BillboardYUpCameraY), the locks in X, Y, Z order (each sets
+// its row to the axis and zeroes that component of the other two rows before
+// renormalising them, W kept), then Decompose's rotation times Scale(scale)
+// times the look-at, with the position put back in row 3.
+T3D_HD M4 billboard_base(const M4& model, V3 cam_pos, V3 up, bool lock_x, bool lock_y, bool lock_z) {
+  M4 la = look_at(row3(model), cam_pos, up);
+  const bool lock[3] = {lock_x, lock_y, lock_z};
+  for (int axis = 0; axis < 3; axis++) {
+    if (!lock[axis]) continue;
+    V4 e = {axis == 0 ? 1.0f : 0.0f, axis == 1 ? 1.0f : 0.0f, axis == 2 ? 1.0f : 0.0f, 0.0f};
+    set_row(la, axis, e);
+    for (int r = 0; r < 3; r++) {
+      if (r == axis) continue;
+      V4 w = row(la, r);
+      if (axis == 0) w.x = 0;
+      else if (axis == 1) w.y = 0;
+      else w.z = 0;
+      set_row(la, r, unit4(w));
+    }
+  }
+  V3 p, s;
+  M4 rot;
+  decompose(model, p, s, rot);
+  M4 base = mult(mult(rot, scale4(s.x, s.y, s.z)), la);
+  V4 r3 = {p.x, p.y, p.z, 1.0f};
+  set_row(base, 3, r3);
+  return base;
+}
+
 // invertedCamPos (model.go:629-630) and PointLight.workingPosition
 // (light.go:175-189): r·x + p ⊙ (1/s) with (p, s, r) = Decompose(Inverted(M)).
 T3D_HD V3 inverted_position(const M4& model, V3 world) {

@@ -81,6 +81,7 @@ struct t3d_frame {
   std::vector<uint32_t> host_stats;
   const uint32_t* stats_host = nullptr;
   uint8_t* tables_dev = nullptr;   // device table block of the last build (own or placed)
+  bool any_normals = false;        // the last build fills the normal vertex list (Camera.RenderNormals)
   t3d::FrameArgs args{};
   // per-render bookkeeping
   std::vector<t3d_pair_result> pairs;
@@ -186,14 +187,14 @@ t3d_status frame_launch(t3d_ctx* c, t3d_frame* f, hipStream_t stream, hipStream_
                         bool allow_graph, bool profile);
 void frame_collect(t3d_ctx* c, t3d_frame* f, bool profile);
 // Device bytes frame_download_range needs in `pack` for pairs [p0, p1) (0 when none survive).
-uint64_t download_pack_bytes(const t3d_frame* f, uint32_t p0, uint32_t p1);
+uint64_t download_pack_bytes(const t3d_frame* f, uint32_t p0, uint32_t p1, bool normals = false);
 // Copy pairs [p0, p1) of a frame, concatenated, to host memory (any output may be null).  Vertices and
 // indices are first packed on the device into `pack`, so they cross to the host in one copy per stream.
 // Without err_out, `pack` is grown here when too small (needs ctx->mu) and errors go to ctx->err; with it,
 // the caller has grown `pack` to download_pack_bytes already and the message goes to *err_out (no ctx state
 // is touched).  Returns only after every copy it enqueued has finished.
 t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uint32_t p1, hipStream_t stream,
-                                t3d_vertex* cv, t3d_vertex* dv, uint32_t* idx, uint32_t* tri_ids,
+                                t3d_vertex* cv, t3d_vertex* dv, uint32_t* idx, uint32_t* tri_ids, t3d_vertex* nv,
                                 t3d::DevBuf* pack = nullptr, uint64_t* pack_cap = nullptr,
                                 std::string* err_out = nullptr);
 // Frame memory a job may need (scratch + outputs + tables), before frustum culling.

@@ -664,8 +664,13 @@ t3d_status t3d_job_info_get(t3d_server* s, uint64_t job, t3d_job_info* out) {
   return T3D_OK;
 }
 
-t3d_status t3d_job_download(t3d_server* s, uint64_t job, t3d_pair_result* pairs, t3d_vertex* cv, t3d_vertex* dv,
-                            uint32_t* idx, uint32_t* tri_ids) {
+}  // extern "C"
+
+namespace {
+
+// t3d_job_download and t3d_job_download_normals: the job's pair records and whichever arrays are asked for.
+t3d_status job_download(t3d_server* s, uint64_t job, t3d_pair_result* pairs, t3d_vertex* cv, t3d_vertex* dv,
+                        uint32_t* idx, uint32_t* tri_ids, t3d_vertex* nv) {
   if (!s) return T3D_ERR_INVALID_ARGUMENT;
   const t3d_frame* f = nullptr;
   uint32_t p0 = 0, p1 = 0;
@@ -677,6 +682,8 @@ t3d_status t3d_job_download(t3d_server* s, uint64_t job, t3d_pair_result* pairs,
     Job& j = *it->second;
     if (j.state != T3D_JOB_DONE) return job_status(j) == T3D_OK ? T3D_ERR_NOT_READY : job_status(j);
     if (j.released || !j.batch) return sfail(s, T3D_ERR_NOT_FOUND, "job_download: the job's outputs were released");
+    if (nv && !j.d.cam.render_normals)
+      return sfail(s, T3D_ERR_NOT_FOUND, "job_download: the job's camera does not render normals");
     f = j.batch->frame;   // stays with the batch while this job is unreleased (pins block release)
     p0 = f->job_pair_begin[j.index_in_batch];
     p1 = f->job_pair_begin[j.index_in_batch + 1];
@@ -692,11 +699,11 @@ t3d_status t3d_job_download(t3d_server* s, uint64_t job, t3d_pair_result* pairs,
     jp = &j;
   }
   t3d_status st = T3D_OK;
-  if (cv || dv || idx || tri_ids) {
+  if (cv || dv || idx || tri_ids || nv) {
     (void)hipSetDevice(s->ctx->device);
     std::lock_guard<std::mutex> dl(s->dl_mu);   // one download at a time on the download stream
     // the server's packing buffer: grown under ctx->mu (memory accounting), used under dl_mu only
-    const uint64_t need = (cv || dv || idx) ? download_pack_bytes(f, p0, p1) : 0;
+    const uint64_t need = (cv || dv || idx || nv) ? download_pack_bytes(f, p0, p1, nv != nullptr) : 0;
     if (need > s->pack_cap) {
       std::lock_guard<std::mutex> cl(s->ctx->mu);
       st = ensure(s->ctx, s->pack, s->pack_cap, need);
@@ -705,7 +712,8 @@ t3d_status t3d_job_download(t3d_server* s, uint64_t job, t3d_pair_result* pairs,
     if (!st) {
       // errors are reported into a local string (the worker may be writing ctx->err under ctx->mu)
       std::string err;
-      st = frame_download_range(s->ctx, f, p0, p1, s->dl_stream, cv, dv, idx, tri_ids, &s->pack, &s->pack_cap, &err);
+      st = frame_download_range(s->ctx, f, p0, p1, s->dl_stream, cv, dv, idx, tri_ids, nv, &s->pack, &s->pack_cap,
+                                &err);
       if (st) {
         std::lock_guard<std::mutex> cl(s->ctx->mu);
         s->ctx->err = err;
@@ -715,6 +723,20 @@ t3d_status t3d_job_download(t3d_server* s, uint64_t job, t3d_pair_result* pairs,
   std::lock_guard<std::mutex> lk(s->mu);
   jp->pins--;
   return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+t3d_status t3d_job_download(t3d_server* s, uint64_t job, t3d_pair_result* pairs, t3d_vertex* cv, t3d_vertex* dv,
+                            uint32_t* idx, uint32_t* tri_ids) {
+  return job_download(s, job, pairs, cv, dv, idx, tri_ids, nullptr);
+}
+
+t3d_status t3d_job_download_normals(t3d_server* s, uint64_t job, t3d_vertex* nv) {
+  if (!nv) return T3D_ERR_INVALID_ARGUMENT;
+  return job_download(s, job, nullptr, nullptr, nullptr, nullptr, nullptr, nv);
 }
 
 t3d_status t3d_job_release(t3d_server* s, uint64_t job) {

@@ -69,6 +69,8 @@ def lib():
     L.t3d_frame_device_outputs.argtypes = [vp, P(vp), P(vp), P(vp)]
     L.t3d_frame_download.argtypes = [vp, vp, P(A.Vertex), P(A.Vertex), P(C.c_uint32)]
     L.t3d_frame_download_triangles.argtypes = [vp, vp, P(C.c_uint32)]
+    L.t3d_frame_device_normals.argtypes = [vp, P(vp)]
+    L.t3d_frame_download_normals.argtypes = [vp, vp, P(A.Vertex)]
     L.t3d_frame_last_ms.argtypes = [vp]
     L.t3d_frame_last_ms.restype = C.c_double
     L.t3d_profile_enable.argtypes = [vp, C.c_int32]
@@ -85,7 +87,8 @@ def lib():
                  "t3d_frame_device_outputs", "t3d_frame_download", "t3d_frame_download_triangles",
                  "t3d_profile_enable", "t3d_profile_read", "t3d_get_tuning", "t3d_set_tuning", "t3d_autotune",
                  "t3d_selftest_fp", "t3d_selftest_copy", "t3d_debug_model_spheres", "t3d_animation_upload",
-                 "t3d_animation_free", "t3d_render_noalloc"):
+                 "t3d_animation_free", "t3d_render_noalloc", "t3d_frame_device_normals",
+                 "t3d_frame_download_normals"):
         getattr(L, name).restype = C.c_int32
     _lib = L
     return L
@@ -101,13 +104,14 @@ def struct_sizes():
 class Frame:
     """Host copy of one render's outputs, pair-concatenated in draw order."""
 
-    def __init__(self, pairs, color, depth, indices, triangle_ids, ms):
+    def __init__(self, pairs, color, depth, indices, triangle_ids, ms, normals=None):
         self.pairs = pairs            # (model, part, triangles, survivors, device vertex offset)
         self.color = color            # (3*S, 12) float32 ebiten vertices
         self.depth = depth
         self.indices = indices        # (3*S,) uint32, pair-local
         self.triangle_ids = triangle_ids
         self.ms = ms
+        self.normals = normals        # (3*S, 12) normal vertices when Camera.RenderNormals, else None
 
     @property
     def survivors(self):
@@ -300,7 +304,12 @@ class Context:
             tri = np.zeros(S, dtype=np.uint32)
             self._check(self.L.t3d_frame_download_triangles(self.ctx, frame,
                                                             tri.ctypes.data_as(C.POINTER(C.c_uint32))))
-        return Frame(pairs, color, depth, idx, tri, self.last_ms())
+        normals = None
+        dev = C.c_void_p()
+        if self.L.t3d_frame_device_normals(frame, C.byref(dev)) == 0:   # the render filled a normal list
+            normals = np.zeros((3 * S, 12), dtype=np.float32)
+            self._check(self.L.t3d_frame_download_normals(self.ctx, frame, normals.ctypes.data_as(VP)))
+        return Frame(pairs, color, depth, idx, tri, self.last_ms(), normals)
 
     def render(self, camera, scene, lights, models, exec_mode=A.EXEC_BATCHED, download=True):
         """Camera.Render(scene, lights, models) (camera.go:1269) on the GPU."""

@@ -146,8 +146,10 @@ class Material:
         self.visible = True
         self.texture_size = None  # (w, h) when a texture is set and UseTexture is on
         self.billboard = False
+        self.billboard_up = A.BILLBOARD_UP_GLOBAL_Y
+        self.billboard_lock = (False, False, False)
         self.custom_depth_function = False
-        self.depth_mode = 0
+        self.depth_mode = A.DEPTH_MODE_DEFAULT
 
     def desc(self):
         d = A.MaterialDesc()
@@ -169,6 +171,8 @@ class Material:
         d.billboard = int(self.billboard)
         d.custom_depth_function = int(self.custom_depth_function)
         d.depth_mode = self.depth_mode
+        d.billboard_up = self.billboard_up
+        d.billboard_lock = (C.c_int32 * 3)(*[int(bool(x)) for x in self.billboard_lock])
         return d
 
 

@@ -76,19 +76,21 @@ def _bind():
     L.t3d_job_download.argtypes = [vp, C.c_uint64, P(A.PairResult), P(A.Vertex), P(A.Vertex), P(C.c_uint32),
                                    P(C.c_uint32)]
     L.t3d_job_release.argtypes = [vp, C.c_uint64]
+    L.t3d_job_download_normals.argtypes = [vp, C.c_uint64, P(A.Vertex)]
     L.t3d_job_cancel.argtypes = [vp, C.c_uint64]
     L.t3d_server_drain.argtypes = [vp, C.c_uint32]
     L.t3d_server_stats_get.argtypes = [vp, P(ServerStats)]
     L.t3d_server_stats_reset.argtypes = [vp]
     for n in ("t3d_server_create", "t3d_server_destroy", "t3d_job_submit", "t3d_job_poll", "t3d_job_wait",
               "t3d_job_info_get", "t3d_job_download", "t3d_job_release", "t3d_job_cancel", "t3d_server_drain",
-              "t3d_server_stats_get", "t3d_server_stats_reset"):
+              "t3d_server_stats_get", "t3d_server_stats_reset", "t3d_job_download_normals"):
         getattr(L, n).restype = C.c_int32
     _bound = True
 
 
 class JobResult:
-    def __init__(self, pairs, color, depth, indices, triangle_ids, info):
+    def __init__(self, pairs, color, depth, indices, triangle_ids, info, normals=None):
+        self.normals = normals
         self.pairs = pairs
         self.color = color
         self.depth = depth
@@ -149,7 +151,7 @@ class Server:
         self._check(self.L.t3d_job_info_get(self.srv, job, C.byref(i)))
         return i
 
-    def download(self, job, triangles=True):
+    def download(self, job, triangles=True, normals=False):
         i = self.info(job)
         pairs = (A.PairResult * max(1, i.pair_count))()
         S = int(i.survivors)
@@ -163,7 +165,11 @@ class Server:
                                             idx.ctypes.data_as(U32), tri.ctypes.data_as(U32) if triangles else None))
         pl = [(p.model_index, p.part_index, p.triangle_count, p.survivor_count, p.vertex_offset)
               for p in pairs[:i.pair_count]]
-        return JobResult(pl, color, depth, idx, tri[:S] if triangles else None, i)
+        nrm = None
+        if normals:
+            nrm = np.zeros((3 * S, 12), dtype=np.float32)
+            self._check(self.L.t3d_job_download_normals(self.srv, job, nrm.ctypes.data_as(VP)))
+        return JobResult(pl, color, depth, idx, tri[:S] if triangles else None, i, nrm)
 
     def release(self, job):
         self._check(self.L.t3d_job_release(self.srv, job))
