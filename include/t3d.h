@@ -234,6 +234,23 @@ typedef struct {
   uint64_t animation;
   float animation_time;
   uint32_t pad_animation;
+  /* Dynamic batching (Model.DynamicBatchAdd, model.go:217-247).  With
+   * dynamic_batched set, this model is in the dynamic batch of
+   * models[batch_owner] under that model's MeshPart batch_part
+   * (Model.DynamicBatchOwner != nil): it is no render pair of its own and
+   * draws only inside its owner's flush.  batch_order is its position in the
+   * owner's DynamicBatchModels[part] slice (the order DynamicBatchAdd appended
+   * it).  A model named as owner is a dynamic batcher: its own mesh is not
+   * drawn, and each of its parts with batched models is one render pair
+   * (camera.go:1402-1453) whose flush holds every part of its visible batched
+   * models, farthest from the camera first (camera.go:2075-2099).  Parts of
+   * one owner go in ascending part order (the reference walks a Go map).  A
+   * batched model's own auto_batch_static and hidden materials are not
+   * consulted, as in the reference. */
+  int32_t dynamic_batched;
+  uint32_t batch_owner;
+  uint32_t batch_part;
+  uint32_t batch_order;
 } t3d_model_desc;
 
 /* Camera state (camera.go:256-321, 324-547). */
@@ -279,7 +296,11 @@ typedef struct {
   int32_t exec_mode;                 /* T3D_EXEC_* */
 } t3d_render_desc;
 
-/* One render pair = one ProcessVertices + flush (camera.go:2057-2109). */
+/* One render pair = one ProcessVertices + flush (camera.go:2057-2109).  For a
+ * dynamic batch the pair is the owner's (model_index = owner, part_index =
+ * its batched part), and the flush's vertex lists hold the batched models'
+ * parts one after another, indices continuing across them as the
+ * reference's vertexListIndex does; triangle_count then sums their parts. */
 typedef struct {
   uint32_t model_index;              /* index into t3d_render_desc.models */
   uint32_t part_index;               /* MeshPart index within the mesh */

@@ -934,6 +934,7 @@ typedef struct {
   uint32_t model, part;
   float depth;
   int32_t order;
+  int batched;     /* a dynamic batcher's part (Model.DynamicBatcher(), model.go:272-274) */
 } opair;
 
 typedef struct {
@@ -1289,7 +1290,7 @@ static int validate_model(t3do_ctx* c, const t3d_model_desc* model, const omesh*
 static int render_pair(t3do_ctx* c, const ocam* cam, const t3d_world_desc* world, const t3d_model_desc* model,
                        uint32_t model_index, uint32_t part_index, const omesh* m, olight* scene_lights, int nscene,
                        olight* group_lights, const t3d_light_desc* group_descs, int ngroup, oscratch* s, obuf* ob,
-                       t3do_result* res, uint32_t* pair_cap) {
+                       t3do_result* res, uint32_t* pair_cap, int64_t flush) {
   const t3d_camera_desc* cd = cam->d;
   const t3d_material_desc* mat = part_mat(model, part_index);
   const t3d_part_range* pr = &m->parts[part_index];
@@ -1301,17 +1302,25 @@ static int render_pair(t3do_ctx* c, const ocam* cam, const t3d_world_desc* world
   }
   int sort_mode = mat ? mat->triangle_sort_mode : T3D_SORT_BACK_TO_FRONT;
 
-  /* pair record (emitted even when empty, survivor_count 0) */
-  if (res->pair_count == *pair_cap) {
-    *pair_cap = *pair_cap ? 2 * *pair_cap : 64;
-    res->pairs = (t3d_pair_result*)realloc(res->pairs, sizeof(t3d_pair_result) * *pair_cap);
+  /* pair record (emitted even when empty, survivor_count 0); a dynamic batch's parts all add to the flush record
+   * of their batcher (flush >= 0), their vertices following on in its vertex list (camera.go:1552, 2084-2099) */
+  t3d_pair_result* out_pair;
+  if (flush >= 0) {
+    out_pair = &res->pairs[flush];
+    out_pair->triangle_count += pr->triangle_count;
+  } else {
+    if (res->pair_count == *pair_cap) {
+      *pair_cap = *pair_cap ? 2 * *pair_cap : 64;
+      res->pairs = (t3d_pair_result*)realloc(res->pairs, sizeof(t3d_pair_result) * *pair_cap);
+    }
+    out_pair = &res->pairs[res->pair_count++];
+    out_pair->model_index = model_index;
+    out_pair->part_index = part_index;
+    out_pair->triangle_count = pr->triangle_count;
+    out_pair->survivor_count = 0;
+    out_pair->vertex_offset = ob->n;
   }
-  t3d_pair_result* out_pair = &res->pairs[res->pair_count++];
-  out_pair->model_index = model_index;
-  out_pair->part_index = part_index;
-  out_pair->triangle_count = pr->triangle_count;
-  out_pair->survivor_count = 0;
-  out_pair->vertex_offset = ob->n;
+  const uint32_t vli0 = 3 * out_pair->survivor_count;              /* vertexListIndex when this render starts */
 
   /* ProcessVertices early out: model.go:604-606 */
   if ((model->color[3] == 0 && is_transparent(model, mat)) || !model->visible) return T3D_OK;
@@ -1585,16 +1594,133 @@ static int render_pair(t3do_ctx* c, const ocam* cam, const t3d_world_desc* world
           cv->color_b *= world->fog_color[2] * depth;
         }
       }
-      ob->idx[o] = (uint32_t)(3 * k + vi);                                   /* indexList, camera.go:1816 */
+      ob->idx[o] = (uint32_t)(vli0 + 3 * k + vi);                            /* indexList, camera.go:1816 */
     }
   }
   ob->n += 3 * (uint64_t)nb;
-  out_pair->survivor_count = nb;
+  out_pair->survivor_count += nb;
   free(order);
   free(bins);
   free(bucket_tri);
   free(bucket_depth);
   return T3D_OK;
+}
+
+/* Dynamic batching (model.go:217-274).  The batched models of (owner, part): the models naming that owner and part,
+ * in their DynamicBatchModels slice order (batch_order). */
+typedef struct { const t3d_render_desc* desc; const float* dist; } batch_env;
+static int less_batch_order(void* env, int32_t a, int32_t b) {
+  const t3d_model_desc* m = ((batch_env*)env)->desc->models;
+  return m[a].batch_order < m[b].batch_order;
+}
+static int less_farther(void* env, int32_t a, int32_t b) {
+  const float* d = ((batch_env*)env)->dist;
+  return d[a] > d[b];
+}
+static int batch_models(const t3d_render_desc* desc, uint32_t owner, uint32_t part, int32_t* out) {
+  int n = 0;
+  for (uint32_t k = 0; k < desc->model_count; k++) {
+    const t3d_model_desc* b = &desc->models[k];
+    if (b->dynamic_batched && b->batch_owner == owner && b->batch_part == part) out[n++] = (int32_t)k;
+  }
+  batch_env env = {desc, NULL};
+  go_stable_sort(out, n, less_batch_order, &env);
+  return n;
+}
+
+/* ProcessVertices + the render closure of one (model, part), with the model's light group and posed palette
+ * (camera.go:1507-1827); flush >= 0 adds it to that flush record (a dynamic batch). */
+static int render_model_part(t3do_ctx* c, const t3d_render_desc* desc, const ocam* cam, const t3d_world_desc* world,
+                             uint32_t mi, uint32_t part, olight* lights, int nl, oscratch* s, obuf* ob, t3do_result* res,
+                             uint32_t* pair_cap, int64_t flush) {
+  const t3d_model_desc* model = &desc->models[mi];
+  omesh* m = find_mesh(c, model->mesh);
+  olight* glights = NULL;
+  int ng = 0;
+  if (model->light_group_active) {
+    ng = (int)model->light_group_count;
+    glights = (olight*)calloc(ng + 1, sizeof(olight));
+    for (int g = 0; g < ng; g++) light_from_desc(c, &glights[g], &model->light_group[g]);
+  }
+  /* bones given as poses: the palette from Node.Transform down the armature */
+  t3d_model_desc posed;
+  m4* pal = NULL;
+  if (model->skinned && model->bone_poses) {
+    const uint32_t nb = model->bone_count;
+    pal = (m4*)malloc(sizeof(m4) * 2 * (nb ? nb : 1));
+    const t3d_bone_pose* poses = model->bone_poses;
+    t3d_bone_pose* animated = NULL;
+    if (model->animation) {
+      const oanim* an = find_anim(c, model->animation);
+      animated = (t3d_bone_pose*)malloc(sizeof(t3d_bone_pose) * (nb ? nb : 1));
+      memcpy(animated, poses, sizeof(t3d_bone_pose) * nb);
+      animate_poses(an->ch, an->n, model->animation_time, animated, nb);
+      poses = animated;
+    }
+    bone_palettes(poses, nb, model->armature_parent, pal + nb, pal);
+    free(animated);
+    posed = *model;
+    posed.bone_influences = pal;
+    model = &posed;
+  }
+  int rc = render_pair(c, cam, world, model, mi, part, m, lights, nl, glights, model->light_group, ng, s, ob, res,
+                       pair_cap, flush);
+  free(pal);
+  free(glights);
+  return rc;
+}
+
+/* A dynamic batcher's pair in the pass loop (camera.go:2075-2099): its batched models sorted farther first with
+ * sort.Slice (insertion sort for up to 12, as Go does; longer slices use a stable sort here, which orders equal
+ * distances as pdqsort need not), each visible one inside the frustum rendering every part of its mesh into the
+ * one flush of the batcher's pair. */
+static int render_batch(t3do_ctx* c, const t3d_render_desc* desc, const ocam* cam, const t3d_world_desc* world,
+                        const opair* rp, olight* lights, int nl, oscratch* s, obuf* ob, t3do_result* res,
+                        uint32_t* pair_cap) {
+  if (res->pair_count == *pair_cap) {
+    *pair_cap = *pair_cap ? 2 * *pair_cap : 64;
+    res->pairs = (t3d_pair_result*)realloc(res->pairs, sizeof(t3d_pair_result) * *pair_cap);
+  }
+  const int64_t flush = res->pair_count++;
+  t3d_pair_result* fp = &res->pairs[flush];
+  memset(fp, 0, sizeof *fp);
+  fp->model_index = rp->model;
+  fp->part_index = rp->part;
+  fp->vertex_offset = ob->n;
+  int32_t* kids = (int32_t*)malloc(sizeof(int32_t) * (desc->model_count + 1));
+  float* dist = (float*)malloc(sizeof(float) * (desc->model_count + 1));
+  int n = batch_models(desc, rp->model, rp->part, kids);
+  for (int i = 0; i < n; i++) dist[kids[i]] = v3_dist_sq(cam->pos, m4_row3(&desc->models[kids[i]].transform));
+  if (n <= 12) {
+    for (int i = 1; i < n; i++)
+      for (int j = i; j > 0 && dist[kids[j]] > dist[kids[j - 1]]; j--) {
+        int32_t t = kids[j];
+        kids[j] = kids[j - 1];
+        kids[j - 1] = t;
+      }
+  } else {
+    batch_env env = {desc, dist};
+    go_stable_sort(kids, n, less_farther, &env);
+  }
+  int rc = T3D_OK;
+  for (int i = 0; i < n && rc == T3D_OK; i++) {
+    const t3d_model_desc* km = &desc->models[kids[i]];
+    if (!km->visible) continue;
+    omesh* m = find_mesh(c, km->mesh);
+    if (!m) { rc = set_err(c, "render: unknown mesh handle"); break; }
+    if (km->frustum_culling) {
+      v3 sp;
+      float sr;
+      model_sphere(&km->transform, km->skinned, m->d.dimensions_min, m->d.dimensions_max, &sp, &sr);
+      if (!sphere_in_frustum(cam, sp, sr)) continue;
+    }
+    rc = validate_model(c, km, m, cam->d);
+    for (uint32_t q = 0; q < m->d.part_count && rc == T3D_OK; q++)
+      rc = render_model_part(c, desc, cam, world, (uint32_t)kids[i], q, lights, nl, s, ob, res, pair_cap, flush);
+  }
+  free(kids);
+  free(dist);
+  return rc;
 }
 
 int t3do_render(t3do_ctx* c, const t3d_render_desc* desc, t3do_result* res) {
@@ -1638,13 +1764,61 @@ int t3do_render(t3do_ctx* c, const t3d_render_desc* desc, t3do_result* res) {
   int32_t* transp = NULL;
   int ns = 0, nt = 0;
   int rc = T3D_OK;
+  int32_t* kids = (int32_t*)malloc(sizeof(int32_t) * (desc->model_count + 1));
   for (uint32_t mi = 0; mi < desc->model_count; mi++) {
     const t3d_model_desc* model = &desc->models[mi];
+    if (model->dynamic_batched) continue;                           /* DynamicBatchOwner != nil, camera.go:1346 */
     if (!model->visible || model->mesh == 0) continue;             /* camera.go:1346-1349, 1371 */
     omesh* m = find_mesh(c, model->mesh);
-    if (!m) { rc = set_err(c, "render: unknown mesh handle"); goto done; }
+    if (!m) { rc = set_err(c, "render: unknown mesh handle"); free(kids); goto done; }
     rc = validate_model(c, model, m, cd);
-    if (rc) goto done;
+    if (rc) { free(kids); goto done; }
+    int batcher = 0;                                                /* Model.DynamicBatcher(), model.go:272-274 */
+    for (uint32_t k = 0; k < desc->model_count && !batcher; k++)
+      batcher = desc->models[k].dynamic_batched && desc->models[k].batch_owner == mi;
+    if (batcher) {                                                  /* camera.go:1398-1453 */
+      /* one `transparent` for the whole batcher: once set it stays set for its later parts, as written */
+      int transparent = 0;
+      for (uint32_t p = 0; p < m->d.part_count; p++) {              /* ascending part (the reference walks a map) */
+        int n = batch_models(desc, mi, p, kids);
+        if (!n) continue;
+        const t3d_material_desc* mat = part_mat(model, p);
+        if (mat && !mat->visible) continue;
+        for (int i = 0; i < n; i++) {
+          const t3d_model_desc* ch = &desc->models[kids[i]];
+          if (!ch->visible) continue;
+          if (!transparent) {
+            const omesh* cm = find_mesh(c, ch->mesh);
+            for (uint32_t q = 0; cm && q < cm->d.part_count; q++) {
+              const t3d_material_desc* cmat = part_mat(ch, q);
+              if ((!cmat || cmat->visible) && is_transparent(ch, cmat)) {
+                transparent = 1;
+                break;
+              }
+            }
+          }
+        }
+        if (np == capp) {
+          capp *= 2;
+          pairs = (opair*)realloc(pairs, sizeof(opair) * capp);
+        }
+        opair* rp = &pairs[np];
+        rp->model = mi;
+        rp->part = p;
+        rp->depth = (transparent || !cd->render_depth) ? v3_dist_sq(cam.pos, m4_row3(&model->transform)) : 0;
+        rp->order = mat ? mat->render_order : 0;
+        rp->batched = 1;
+        if (transparent) {
+          transp = (int32_t*)realloc(transp, sizeof(int32_t) * (nt + 1));
+          transp[nt++] = np;
+        } else {
+          solids = (int32_t*)realloc(solids, sizeof(int32_t) * (ns + 1));
+          solids[ns++] = np;
+        }
+        np++;
+      }
+      continue;
+    }
     if (model->frustum_culling) {
       v3 sp;
       float sr;
@@ -1663,6 +1837,7 @@ int t3do_render(t3do_ctx* c, const t3d_render_desc* desc, t3do_result* res) {
       rp->part = p;
       rp->depth = v3_dist_sq(cam.pos, m4_row3(&model->transform));
       rp->order = mat ? mat->render_order : 0;
+      rp->batched = 0;
       if (is_transparent(model, mat)) {
         transp = (int32_t*)realloc(transp, sizeof(int32_t) * (nt + 1));
         transp[nt++] = np;
@@ -1673,6 +1848,7 @@ int t3do_render(t3do_ctx* c, const t3d_render_desc* desc, t3do_result* res) {
       np++;
     }
   }
+  free(kids);
   pair_env penv = {pairs};
   if (!cd->render_depth) go_stable_sort(solids, ns, less_solid_nodepth, &penv);  /* camera.go:1458-1467 */
 
@@ -1715,39 +1891,10 @@ int t3do_render(t3do_ctx* c, const t3d_render_desc* desc, t3do_result* res) {
         const opair* rp = &pairs[list[i]];
         const t3d_model_desc* model = &desc->models[rp->model];
         if (!model->visible || model->auto_batch_static) continue;
-        omesh* m = find_mesh(c, model->mesh);
-        olight* glights = NULL;
-        int ng = 0;
-        if (model->light_group_active) {
-          ng = (int)model->light_group_count;
-          glights = (olight*)calloc(ng + 1, sizeof(olight));
-          for (int g = 0; g < ng; g++) light_from_desc(c, &glights[g], &model->light_group[g]);
-        }
-        /* bones given as poses: the palette from Node.Transform down the armature */
-        t3d_model_desc posed;
-        m4* pal = NULL;
-        if (model->skinned && model->bone_poses) {
-          const uint32_t nb = model->bone_count;
-          pal = (m4*)malloc(sizeof(m4) * 2 * (nb ? nb : 1));
-          const t3d_bone_pose* poses = model->bone_poses;
-          t3d_bone_pose* animated = NULL;
-          if (model->animation) {
-            const oanim* an = find_anim(c, model->animation);
-            animated = (t3d_bone_pose*)malloc(sizeof(t3d_bone_pose) * (nb ? nb : 1));
-            memcpy(animated, poses, sizeof(t3d_bone_pose) * nb);
-            animate_poses(an->ch, an->n, model->animation_time, animated, nb);
-            poses = animated;
-          }
-          bone_palettes(poses, nb, model->armature_parent, pal + nb, pal);
-          free(animated);
-          posed = *model;
-          posed.bone_influences = pal;
-          model = &posed;
-        }
-        rc = render_pair(c, &cam, world, model, rp->model, rp->part, m, lights, nl, glights, model->light_group, ng,
-                         &s, &ob, res, &pair_cap);
-        free(pal);
-        free(glights);
+        if (rp->batched)
+          rc = render_batch(c, desc, &cam, world, rp, lights, nl, &s, &ob, res, &pair_cap);
+        else
+          rc = render_model_part(c, desc, &cam, world, rp->model, rp->part, lights, nl, &s, &ob, res, &pair_cap, -1);
       }
     }
     free(s.alt_pos);

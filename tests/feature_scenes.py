@@ -188,6 +188,11 @@ def feature_scene(seed):
                 p.material.billboard_lock = tuple(bool(x) for x in extra.random(3) < 0.3)
                 if not m.skinned:
                     p.material.depth_mode = int(extra.integers(0, 2))
+    if extra.random() < 0.5 and len(models) > 3:                     # dynamic batches (model.go:217-247)
+        owner = models[int(extra.integers(0, len(models)))]
+        others = [m for m in models if m is not owner]
+        for i in extra.choice(len(others), size=int(extra.integers(1, min(len(others), 6) + 1)), replace=False):
+            owner.dynamic_batch_add(int(extra.integers(0, len(owner.mesh.mesh_parts))), others[int(i)])
     return W.Workload("features_%d" % seed, cam, Scene("features", world), lights, models)
 
 

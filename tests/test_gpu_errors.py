@@ -57,6 +57,16 @@ def test_invalid_render_inputs_are_refused(gpu_ctx, oracle):
         r.models[m].materials[0].visible = 1
         r.models[m].materials[0].depth_mode = A.DEPTH_MODE_UNBILLBOARDED
     _refused(gpu_ctx, sk, unbb_skinned, A.T3D_ERR_UNSUPPORTED)
+
+    # dynamic batches naming no model, the batched model itself, or a part the owner does not have
+    def batched(owner, part):
+        def f(r):
+            r.models[1].dynamic_batched = 1
+            r.models[1].batch_owner = owner
+            r.models[1].batch_part = part
+        return f
+    for owner, part in (len(sk.models) + 5, 0), (1, 0), (0, 99):
+        _refused(gpu_ctx, sk, batched(owner, part), A.T3D_ERR_INVALID_ARGUMENT)
     # more lights on one model than the vertex pass stages (64)
     many = [W._point("p%d" % i, (0.1 * i, 2, 2), 10) for i in range(70)]
     lights_wl = W.Workload("many_lights", wl.camera, wl.scene, many, wl.models)

@@ -9,7 +9,7 @@ from tests.feature_scenes import feature_scene
 def test_feature_scenes_render_and_repeat(oracle):
     sort_modes, transparency, lv_modes = set(), set(), set()
     skinned = shape_keys = groups = ortho = batched_static = blend_off = offset = hidden = normals = 0
-    billboards = unbillboarded = 0
+    billboards = unbillboarded = batchers = 0
     for seed in range(40):
         wl = feature_scene(seed)
         a = oracle.render_workload(wl)
@@ -24,6 +24,7 @@ def test_feature_scenes_render_and_repeat(oracle):
             np.testing.assert_array_equal(a.normals.view(np.uint32), b.normals.view(np.uint32))
         offset += sum(bool(np.any(getattr(l, "offset", np.zeros(3)) != 0)) for l in wl.lights)
         for m in wl.models:
+            batchers += bool(m.dynamic_batch_models)
             batched_static += m.auto_batch_static
             hidden += not m.visible
             blend_off += sum(p.material is not None and not p.material.blend_source_over for p in m.mesh.mesh_parts)
@@ -40,4 +41,4 @@ def test_feature_scenes_render_and_repeat(oracle):
     assert sort_modes == {A.SORT_BACK_TO_FRONT, A.SORT_FRONT_TO_BACK, A.SORT_NONE}
     assert len(transparency) == 4 and len(lv_modes) == 4
     assert skinned and shape_keys and groups and ortho and batched_static and blend_off and offset and hidden and normals
-    assert billboards and unbillboarded
+    assert billboards and unbillboarded and batchers

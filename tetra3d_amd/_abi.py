@@ -178,6 +178,10 @@ class ModelDesc(C.Structure):
         ("animation", C.c_uint64),
         ("animation_time", C.c_float),
         ("pad_animation", C.c_uint32),
+        ("dynamic_batched", C.c_int32),
+        ("batch_owner", C.c_uint32),
+        ("batch_part", C.c_uint32),
+        ("batch_order", C.c_uint32),
     ]
 
 

@@ -436,7 +436,31 @@ struct Pair {
   float depth;
   int32_t order;
   const Mesh* mesh;
+  int32_t batch = -1;   // a dynamic batcher's part: index of its batch range
 };
+
+// The batched models of one dynamic batcher's part: batch_idx[begin, end)
+// (DynamicBatchModels[part], model.go:236-241).
+struct BatchRange {
+  uint32_t owner, part, begin, end;
+};
+struct BatchKid {
+  float dist;      // camera.DistanceSquaredTo(model) (node.go:1720-1722)
+  uint32_t model;
+};
+
+// sort.Slice(modelSlice, farther first) (camera.go:2078-2080).  Go sorts
+// slices of up to 12 elements by plain insertion sort, reproduced here step
+// for step; longer slices go through pdqsort, whose order among equal
+// distances this stable sort does not reproduce (any strict order it does).
+void sort_batch_far_first(std::vector<BatchKid>& k) {
+  if (k.size() <= 12) {
+    for (size_t i = 1; i < k.size(); i++)
+      for (size_t j = i; j > 0 && k[j].dist > k[j - 1].dist; j--) std::swap(k[j], k[j - 1]);
+    return;
+  }
+  std::stable_sort(k.begin(), k.end(), [](const BatchKid& a, const BatchKid& b) { return a.dist > b.dist; });
+}
 
 // Sorts render-pair indices by (order ascending, depth descending), ties in
 // their current position order.  Where that comparator is a strict weak
@@ -838,6 +862,10 @@ t3d_status run_chain(t3d_ctx* c, t3d_frame* f, uint32_t p0, uint32_t p1, hipStre
     ProfScope ps(c, st, "k_cull", tr.cull, prof);
     HIP_TRY(c, launch_cull(a, tb0, tb1 - tb0, c->tuning.cull_block, st));
   }
+  if (a.ngroups) {   // dynamic batches: each later render pair of a flush starts where the one before it ended
+    ProfScope ps(c, st, "k_groups", 0, prof);
+    HIP_TRY(c, launch_groups(a, p0, p1, st));
+  }
   if (sb1 > sb0 || ps1 > ps0) {
     {
       ProfScope ps(c, st, "k_hist", 0, prof);
@@ -898,6 +926,21 @@ t3d_status validate_desc(t3d_ctx* c, const t3d_render_desc* desc) {
   if (cd->max_light_count > 0) n_scene = std::min<uint32_t>(n_scene, (uint32_t)cd->max_light_count);
   for (uint32_t mi = 0; mi < desc->model_count; mi++) {
     const t3d_model_desc& md = desc->models[mi];
+    if (md.dynamic_batched) {   // DynamicBatchAdd (model.go:217-247): a part of another model of this call
+      if (md.batch_owner >= desc->model_count || md.batch_owner == mi)
+        return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u is batched under model %u of %u", mi, md.batch_owner,
+                    desc->model_count);
+      const t3d_model_desc& ow = desc->models[md.batch_owner];
+      auto oit = ow.mesh ? c->meshes.find(ow.mesh) : c->meshes.end();
+      if (oit == c->meshes.end())
+        return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u is batched under model %u, which has no known mesh", mi,
+                    md.batch_owner);
+      if (md.batch_part >= oit->second->parts.size())
+        return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: model %u is batched under part %u of model %u (%zu parts)", mi,
+                    md.batch_part, md.batch_owner, oit->second->parts.size());
+      if (md.visible && md.mesh == 0)   // the reference dereferences its mesh
+        return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: batched model %u has no mesh", mi);
+    }
     if (!md.visible || md.mesh == 0) continue;
     auto it = c->meshes.find(md.mesh);
     if (it == c->meshes.end())
@@ -928,10 +971,11 @@ t3d_status validate_desc(t3d_ctx* c, const t3d_render_desc* desc) {
         t3d_status st = check_light(md.light_group[k], "light group entry", k);
         if (st) return st;
       }
-    if (md.auto_batch_static) continue;   // never drawn (camera.go:2066)
+    // never drawn (camera.go:2066), unless batched: the batch draws every part of its models
+    if (md.auto_batch_static && !md.dynamic_batched) continue;
     for (uint32_t p = 0; p < mesh.parts.size(); p++) {
       const t3d_material_desc* mat = part_material(md, p);
-      if (mat && !mat->visible) continue;
+      if (mat && !mat->visible && !md.dynamic_batched) continue;
       if (mat && mat->custom_depth_function)
         return fail(c, T3D_ERR_UNSUPPORTED, "render: model %u part %u uses a custom depth function", mi, p);
       // a skinned model never writes the unbillboarded depth its depth list would read (model.go:760-775): the
@@ -967,6 +1011,17 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   f->parts.clear();
   f->part_first_tri.clear();
   f->job_pair_begin.assign(1, 0);
+  f->flush.clear();
+  f->flush_begin.clear();
+  f->job_flush_begin.assign(1, 0);
+  f->groups.clear();
+  f->grouped = false;   // some model of the frame is dynamically batched: flush records differ from pairs
+  for (uint32_t job = 0; job < ndesc && !f->grouped; job++)
+    for (uint32_t mi = 0; descs[job] && mi < descs[job]->model_count && !f->grouped; mi++)
+      f->grouped = descs[job]->models && descs[job]->models[mi].dynamic_batched != 0;
+  std::vector<uint32_t> batch_idx;
+  std::vector<BatchRange> batches;
+  std::vector<BatchKid> kids;
   f->fcs.clear();
   std::vector<LightGPU> lights;
   uint64_t n_plights = 0;   // (pair, light) entries, filled on the device
@@ -1075,18 +1130,71 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
       if (nq) flush();
     }
 
+    // dynamic batches of this job: batched models by (owner, part, slice position)
+    batch_idx.clear();
+    batches.clear();
+    if (f->grouped) {
+      for (uint32_t mi = 0; mi < desc->model_count; mi++)
+        if (desc->models[mi].dynamic_batched) batch_idx.push_back(mi);
+      std::stable_sort(batch_idx.begin(), batch_idx.end(), [&](uint32_t x, uint32_t y) {
+        const t3d_model_desc& a = desc->models[x];
+        const t3d_model_desc& b = desc->models[y];
+        if (a.batch_owner != b.batch_owner) return a.batch_owner < b.batch_owner;
+        if (a.batch_part != b.batch_part) return a.batch_part < b.batch_part;
+        return a.batch_order < b.batch_order;
+      });
+      for (uint32_t k = 0; k < batch_idx.size(); k++) {
+        const t3d_model_desc& b = desc->models[batch_idx[k]];
+        if (batches.empty() || batches.back().owner != b.batch_owner || batches.back().part != b.batch_part)
+          batches.push_back(BatchRange{b.batch_owner, b.batch_part, k, k});
+        batches.back().end = k + 1;
+      }
+    }
+
     // render pairs (camera.go:1343-1457)
     std::vector<Pair> pairs;
     std::vector<int32_t> solids, transparents;
     pairs.reserve(desc->model_count);
     solids.reserve(desc->model_count);
+    size_t next_batch = 0;   // batches are ordered by owner, and models are visited in owner order
     for (uint32_t mi = 0; mi < desc->model_count; mi++) {
       const t3d_model_desc& md = desc->models[mi];
+      if (f->grouped && md.dynamic_batched) continue;                  // DynamicBatchOwner != nil
+      while (next_batch < batches.size() && batches[next_batch].owner < mi) next_batch++;
       if (!md.visible || md.mesh == 0) continue;
       const Mesh* mp = find_mesh(md.mesh);
       if (!mp)
         return fail(c, T3D_ERR_NOT_FOUND, "render: model %u references unknown mesh %llu", mi, (unsigned long long)md.mesh);
       const Mesh& mesh = *mp;
+      if (next_batch < batches.size() && batches[next_batch].owner == mi) {   // a dynamic batcher (camera.go:1398-1453)
+        // `transparent` is set once for the batcher and stays set for its later parts, as in the reference
+        bool transparent = false;
+        for (size_t bi = next_batch; bi < batches.size() && batches[bi].owner == mi; bi++) {
+          const BatchRange& br = batches[bi];
+          const t3d_material_desc* pm = part_material(md, br.part);
+          if (pm && !pm->visible) continue;                            // MeshPart.isVisible()
+          for (uint32_t k = br.begin; k < br.end && !transparent; k++) {
+            const t3d_model_desc& ch = desc->models[batch_idx[k]];
+            if (!ch.visible) continue;
+            const Mesh* cmesh = find_mesh(ch.mesh);
+            if (!cmesh)
+              return fail(c, T3D_ERR_NOT_FOUND, "render: model %u references unknown mesh %llu", batch_idx[k],
+                          (unsigned long long)ch.mesh);
+            for (uint32_t q = 0; q < cmesh->parts.size(); q++) {
+              const t3d_material_desc* cmat = part_material(ch, q);
+              if ((!cmat || cmat->visible) && is_transparent(ch, cmat)) {
+                transparent = true;
+                break;
+              }
+            }
+          }
+          Pair rp{mi, br.part, (transparent || !cd->render_depth) ? dist_sq(cam.pos, row3(md.transform)) : 0.0f,
+                  pm ? pm->render_order : 0, &mesh, (int32_t)bi};
+          (transparent ? transparents : solids).push_back((int32_t)pairs.size());
+          pairs.push_back(rp);
+        }
+        continue;
+      }
       if (md.frustum_culling && !in_frustum[mi]) continue;
       for (uint32_t p = 0; p < mesh.parts.size(); p++) {
         const t3d_material_desc* mat = part_material(md, p);
@@ -1178,160 +1286,210 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
     }
     f->fcs.push_back(fc);
 
+    // one render pair's uniforms and slices: Model.ProcessVertices + the render closure (camera.go:1507-1827)
+    auto emit_pair = [&](uint32_t pmi, uint32_t ppart, const Mesh& mesh) -> t3d_status {
+      const t3d_model_desc& md = desc->models[pmi];
+        const t3d_material_desc* mat = part_material(md, ppart);
+        const t3d_part_range& pr = mesh.parts[ppart];
+        bool lighting = false;                                           // camera.go:1521-1528
+        if (world) lighting = mat ? (world->lighting_on && !mat->shadeless && !md.shadeless) : (world->lighting_on && !md.shadeless);
+        const bool early_out = (md.color[3] == 0 && is_transparent(md, mat)) || !md.visible;   // model.go:604-606
+
+        // MVP, invertedCamPos, the directional working rotation and the
+        // per-light values are computed on the device (k_uniforms)
+        f->parts.emplace_back();   // zeroed in place: no copy of the ~450-byte record per pair
+        PartGPU& P = f->parts.back();
+        P.model = md.transform;
+        // the base transform k_uniforms multiplies by VP: the model's, or its billboarded form (model.go:634-693)
+        if (mat && mat->billboard && !md.skinned)
+          P.mvp = billboard_base(md.transform, cam.pos,
+                                 mat->billboard_up == T3D_BILLBOARD_UP_CAMERA_Y ? cam.up : v3(0.0f, 1.0f, 0.0f),
+                                 mat->billboard_lock[0] != 0, mat->billboard_lock[1] != 0, mat->billboard_lock[2] != 0);
+        else
+          P.mvp = md.transform;
+        P.cam = job;
+        P.max_span = mesh.max_span;
+        float mp[4] = {md.color[0], md.color[1], md.color[2], md.color[3]};   // camera.go:1597-1601
+        if (mat)
+          for (int i = 0; i < 4; i++) mp[i] *= mat->color[i];
+        for (int i = 0; i < 4; i++) P.mp[i] = mp[i];
+        if (mat && mat->use_texture) {
+          P.src_w = mat->texture_size[0];
+          P.src_h = mat->texture_size[1];
+        }
+        uint32_t flags = 0;
+        if (md.skinned) flags |= PF_SKINNED;
+        if (mat && mat->backface_culling) flags |= PF_BACKFACE;
+        if (lighting) flags |= PF_LIGHTING;
+        if (mesh.col) flags |= PF_HAS_COLOR;
+        if (mesh.uv) flags |= PF_HAS_UV;
+        if (mesh.K) flags |= PF_SHAPE_KEYS;
+        if (mat && mat->depth_mode == T3D_DEPTH_MODE_UNBILLBOARDED) flags |= PF_UNBB;
+        const uint32_t sort_mode = mat ? (uint32_t)mat->triangle_sort_mode : T3D_SORT_BACK_TO_FRONT;
+        flags |= sort_mode << PF_SORT_SHIFT;
+        flags |= ((uint32_t)(mat ? mat->lighting_mode : 0) & 3u) << PF_LMODE_SHIFT;
+        const uint32_t lv_mode = (uint32_t)(mat ? mat->light_volume_shading_mode : 0) & 3u;
+        flags |= lv_mode << PF_LVMODE_SHIFT;
+        P.flags = flags;
+
+        // lights of this part: beginModel (light.go:175-189, 326-330) and the
+        // per-model point-light skip (camera.go:1592, 1625-1629)
+        if (lighting) {
+          uint32_t lbase = lbase_scene, lcount = n_scene;
+          if (md.light_group_active) {
+            auto g = group_base.find(pmi);
+            if (g == group_base.end()) {
+              uint32_t gb = (uint32_t)lights.size();
+              for (uint32_t k = 0; k < md.light_group_count; k++) lights.push_back(make_light(c, md.light_group[k]));
+              g = group_base.emplace(pmi, gb).first;
+            }
+            lbase = g->second;
+            lcount = md.light_group_count;
+          }
+          if (lcount > kMaxLights)   // validate_desc refuses the descriptor first
+            return fail(c, T3D_ERR_UNSUPPORTED, "render: %u lights on one model (max %u)", lcount, kMaxLights);
+          P.light_base = (uint32_t)n_plights;
+          P.light_count = lcount;
+          P.light0 = lbase;
+          n_plights += lcount;
+          bool need_rd = false;
+          for (uint32_t k = 0; k < lcount && !need_rd; k++)
+            need_rd = lights[lbase + k].type == T3D_LIGHT_DIRECTIONAL;
+          if (need_rd && !md.skinned) {       // k_uniforms inverts it (light.go:326-330)
+            P.rd = md.world_rotation;
+            P.flags |= PF_NEED_RD;
+          }
+        }
+
+        // mesh arrays and frame slices
+        P.pos = mesh.pos;
+        P.nrm = mesh.nrm;
+        P.col = mesh.col;
+        P.uv = mesh.uv;
+        P.bones = mesh.bones;
+        P.weights = mesh.weights;
+        P.idx = mesh.idx ? mesh.idx + 3 * (size_t)pr.triangle_start : nullptr;
+        P.idx16 = mesh.idx16 ? mesh.idx16 + 3 * (size_t)pr.triangle_start : nullptr;
+        P.sk_pos = mesh.sk_pos;
+        P.sk_nrm = mesh.sk_nrm;
+        P.shape_keys = mesh.K;
+        P.mesh_vcount = mesh.V;
+        P.bone_count = md.skinned ? md.bone_count : 0;
+        P.vmin = mesh.part_vmin[ppart];
+        P.vcount = early_out ? 0 : mesh.part_vcount[ppart];
+        P.tcount = early_out ? 0 : pr.triangle_count;
+        P.vbase = (uint32_t)sumV;
+        P.tbase = (uint32_t)sumT;
+        P.out_base = 3 * sumT;
+        const uint32_t nvchunks = (P.vcount + vchunk - 1) / vchunk;
+        const uint32_t ntiles = (P.tcount + tile - 1) / tile;
+        P.ntiles = ntiles;
+        uint32_t pscan = 0;
+        if (ntiles > 1) {
+          P.hbase = (uint32_t)sumHist;
+          P.sblock0 = sblocks;
+          if (kBins * ntiles <= kScanPartMax) {
+            P.nsblocks = 0;          // scanned whole by one block of k_scan_part
+            pscan = 1;
+          } else {
+            P.nsblocks = (kBins * ntiles + kScanBlock - 1) / kScanBlock;
+          }
+          sumHist += (uint64_t)kBins * ntiles;
+          sblocks += P.nsblocks;
+        }
+        sumV += P.vcount;
+        sumT += P.tcount;
+        if (sumV > 0xFFFFFFFFull || sumT > 0x3FFFFFFFull)
+          return fail(c, T3D_ERR_UNSUPPORTED, "render: frame exceeds 2^32 vertices or 2^30 triangles");
+        f->vchunk_prefix.push_back(f->vchunk_prefix.back() + nvchunks);
+        f->tile_prefix.push_back(f->tile_prefix.back() + ntiles);
+        f->sblock_prefix.push_back(sblocks);
+        f->pscan_prefix.push_back(f->pscan_prefix.back() + pscan);
+        int32_t pj = -1;
+        if (md.skinned && md.bone_poses) {
+          auto it = model_pose_job.find(pmi);
+          if (it == model_pose_job.end()) {
+            it = model_pose_job.emplace(pmi, (uint32_t)pose_jobs.size()).first;
+            const Animation* an = md.animation ? c->animations.at(md.animation).get() : nullptr;
+            pose_jobs.push_back({md.bone_poses, md.armature_parent, md.bone_count, an, md.animation_time});
+          }
+          pj = (int32_t)it->second;
+        }
+        part_pose_job.push_back(pj);
+        palettes_host.push_back(md.skinned && pj < 0 ? &md.bone_influences[0].m[0][0] : nullptr);
+        palette_count.push_back(md.skinned && pj < 0 ? md.bone_count : 0);
+        skw_host.push_back(mesh.K ? md.shape_key_weights : nullptr);
+        skw_count.push_back(mesh.K);
+        // algorithmic bytes: mesh streams read once, triangle indices once, outputs
+        const uint64_t vb = 24 + (mesh.col ? 16 : 0) + (md.skinned ? 24 : 0);
+        f->bytes_vertex.push_back((uint64_t)P.vcount * vb + (md.skinned ? 64ull * md.bone_count : 0) + sizeof(PartGPU));
+        f->bytes_cull.push_back((mesh.idx16 ? 6ull : 12ull) * P.tcount);
+        f->bytes_uv.push_back(mesh.uv ? 8ull * P.vcount : 0);   // UVs are read by the assembly
+        f->part_first_tri.push_back(pr.triangle_start);
+        t3d_pair_result r{};
+        r.model_index = pmi;
+        r.part_index = ppart;
+        r.triangle_count = pr.triangle_count;
+        r.vertex_offset = P.out_base;
+        f->pairs.push_back(r);
+      return T3D_OK;
+    };
+
     for (int32_t di : draw) {
       const Pair& rp = pairs[di];
       const t3d_model_desc& md = desc->models[rp.model];
       if (!md.visible || md.auto_batch_static) continue;               // camera.go:2067-2071
-      const Mesh& mesh = *rp.mesh;
-      const t3d_material_desc* mat = part_material(md, rp.part);
-      const t3d_part_range& pr = mesh.parts[rp.part];
-      bool lighting = false;                                           // camera.go:1521-1528
-      if (world) lighting = mat ? (world->lighting_on && !mat->shadeless && !md.shadeless) : (world->lighting_on && !md.shadeless);
-      const bool early_out = (md.color[3] == 0 && is_transparent(md, mat)) || !md.visible;   // model.go:604-606
-
-      // MVP, invertedCamPos, the directional working rotation and the
-      // per-light values are computed on the device (k_uniforms)
-      f->parts.emplace_back();   // zeroed in place: no copy of the ~450-byte record per pair
-      PartGPU& P = f->parts.back();
-      P.model = md.transform;
-      // the base transform k_uniforms multiplies by VP: the model's, or its billboarded form (model.go:634-693)
-      if (mat && mat->billboard && !md.skinned)
-        P.mvp = billboard_base(md.transform, cam.pos,
-                               mat->billboard_up == T3D_BILLBOARD_UP_CAMERA_Y ? cam.up : v3(0.0f, 1.0f, 0.0f),
-                               mat->billboard_lock[0] != 0, mat->billboard_lock[1] != 0, mat->billboard_lock[2] != 0);
-      else
-        P.mvp = md.transform;
-      P.cam = job;
-      P.max_span = mesh.max_span;
-      float mp[4] = {md.color[0], md.color[1], md.color[2], md.color[3]};   // camera.go:1597-1601
-      if (mat)
-        for (int i = 0; i < 4; i++) mp[i] *= mat->color[i];
-      for (int i = 0; i < 4; i++) P.mp[i] = mp[i];
-      if (mat && mat->use_texture) {
-        P.src_w = mat->texture_size[0];
-        P.src_h = mat->texture_size[1];
-      }
-      uint32_t flags = 0;
-      if (md.skinned) flags |= PF_SKINNED;
-      if (mat && mat->backface_culling) flags |= PF_BACKFACE;
-      if (lighting) flags |= PF_LIGHTING;
-      if (mesh.col) flags |= PF_HAS_COLOR;
-      if (mesh.uv) flags |= PF_HAS_UV;
-      if (mesh.K) flags |= PF_SHAPE_KEYS;
-      if (mat && mat->depth_mode == T3D_DEPTH_MODE_UNBILLBOARDED) flags |= PF_UNBB;
-      const uint32_t sort_mode = mat ? (uint32_t)mat->triangle_sort_mode : T3D_SORT_BACK_TO_FRONT;
-      flags |= sort_mode << PF_SORT_SHIFT;
-      flags |= ((uint32_t)(mat ? mat->lighting_mode : 0) & 3u) << PF_LMODE_SHIFT;
-      const uint32_t lv_mode = (uint32_t)(mat ? mat->light_volume_shading_mode : 0) & 3u;
-      flags |= lv_mode << PF_LVMODE_SHIFT;
-      P.flags = flags;
-
-      // lights of this part: beginModel (light.go:175-189, 326-330) and the
-      // per-model point-light skip (camera.go:1592, 1625-1629)
-      if (lighting) {
-        uint32_t lbase = lbase_scene, lcount = n_scene;
-        if (md.light_group_active) {
-          auto g = group_base.find(rp.model);
-          if (g == group_base.end()) {
-            uint32_t gb = (uint32_t)lights.size();
-            for (uint32_t k = 0; k < md.light_group_count; k++) lights.push_back(make_light(c, md.light_group[k]));
-            g = group_base.emplace(rp.model, gb).first;
-          }
-          lbase = g->second;
-          lcount = md.light_group_count;
+      if (rp.batch < 0) {
+        const uint32_t first = (uint32_t)f->pairs.size();
+        t3d_status est = emit_pair(rp.model, rp.part, *rp.mesh);
+        if (est) return est;
+        if (f->grouped) {   // a flush of its own render pair
+          f->flush_begin.push_back(first);
+          f->flush.push_back(f->pairs.back());
         }
-        if (lcount > kMaxLights)   // validate_desc refuses the descriptor first
-          return fail(c, T3D_ERR_UNSUPPORTED, "render: %u lights on one model (max %u)", lcount, kMaxLights);
-        P.light_base = (uint32_t)n_plights;
-        P.light_count = lcount;
-        P.light0 = lbase;
-        n_plights += lcount;
-        bool need_rd = false;
-        for (uint32_t k = 0; k < lcount && !need_rd; k++)
-          need_rd = lights[lbase + k].type == T3D_LIGHT_DIRECTIONAL;
-        if (need_rd && !md.skinned) {       // k_uniforms inverts it (light.go:326-330)
-          P.rd = md.world_rotation;
-          P.flags |= PF_NEED_RD;
+        continue;
+      }
+      // a dynamic batch (camera.go:2075-2099): every part of each visible batched model in the frustum, farthest
+      // model first, into one flush
+      const BatchRange& br = batches[rp.batch];
+      kids.clear();
+      for (uint32_t k = br.begin; k < br.end; k++) {
+        const uint32_t ki = batch_idx[k];
+        kids.push_back({dist_sq(cam.pos, row3(desc->models[ki].transform)), ki});
+      }
+      sort_batch_far_first(kids);
+      t3d_pair_result g{};
+      g.model_index = rp.model;
+      g.part_index = rp.part;
+      g.vertex_offset = 3 * sumT;
+      const uint32_t first = (uint32_t)f->pairs.size();
+      for (const BatchKid& kd : kids) {
+        const t3d_model_desc& km = desc->models[kd.model];
+        if (!km.visible) continue;
+        if (km.frustum_culling && !in_frustum[kd.model]) continue;
+        const Mesh* kmesh = find_mesh(km.mesh);
+        if (!kmesh)
+          return fail(c, T3D_ERR_NOT_FOUND, "render: model %u references unknown mesh %llu", kd.model,
+                      (unsigned long long)km.mesh);
+        for (uint32_t q = 0; q < kmesh->parts.size(); q++) {
+          t3d_status est = emit_pair(kd.model, q, *kmesh);
+          if (est) return est;
+          g.triangle_count += f->pairs.back().triangle_count;
         }
       }
-
-      // mesh arrays and frame slices
-      P.pos = mesh.pos;
-      P.nrm = mesh.nrm;
-      P.col = mesh.col;
-      P.uv = mesh.uv;
-      P.bones = mesh.bones;
-      P.weights = mesh.weights;
-      P.idx = mesh.idx ? mesh.idx + 3 * (size_t)pr.triangle_start : nullptr;
-      P.idx16 = mesh.idx16 ? mesh.idx16 + 3 * (size_t)pr.triangle_start : nullptr;
-      P.sk_pos = mesh.sk_pos;
-      P.sk_nrm = mesh.sk_nrm;
-      P.shape_keys = mesh.K;
-      P.mesh_vcount = mesh.V;
-      P.bone_count = md.skinned ? md.bone_count : 0;
-      P.vmin = mesh.part_vmin[rp.part];
-      P.vcount = early_out ? 0 : mesh.part_vcount[rp.part];
-      P.tcount = early_out ? 0 : pr.triangle_count;
-      P.vbase = (uint32_t)sumV;
-      P.tbase = (uint32_t)sumT;
-      P.out_base = 3 * sumT;
-      const uint32_t nvchunks = (P.vcount + vchunk - 1) / vchunk;
-      const uint32_t ntiles = (P.tcount + tile - 1) / tile;
-      P.ntiles = ntiles;
-      uint32_t pscan = 0;
-      if (ntiles > 1) {
-        P.hbase = (uint32_t)sumHist;
-        P.sblock0 = sblocks;
-        if (kBins * ntiles <= kScanPartMax) {
-          P.nsblocks = 0;          // scanned whole by one block of k_scan_part
-          pscan = 1;
-        } else {
-          P.nsblocks = (kBins * ntiles + kScanBlock - 1) / kScanBlock;
-        }
-        sumHist += (uint64_t)kBins * ntiles;
-        sblocks += P.nsblocks;
-      }
-      sumV += P.vcount;
-      sumT += P.tcount;
-      if (sumV > 0xFFFFFFFFull || sumT > 0x3FFFFFFFull)
-        return fail(c, T3D_ERR_UNSUPPORTED, "render: frame exceeds 2^32 vertices or 2^30 triangles");
-      f->vchunk_prefix.push_back(f->vchunk_prefix.back() + nvchunks);
-      f->tile_prefix.push_back(f->tile_prefix.back() + ntiles);
-      f->sblock_prefix.push_back(sblocks);
-      f->pscan_prefix.push_back(f->pscan_prefix.back() + pscan);
-      int32_t pj = -1;
-      if (md.skinned && md.bone_poses) {
-        auto it = model_pose_job.find(rp.model);
-        if (it == model_pose_job.end()) {
-          it = model_pose_job.emplace(rp.model, (uint32_t)pose_jobs.size()).first;
-          const Animation* an = md.animation ? c->animations.at(md.animation).get() : nullptr;
-          pose_jobs.push_back({md.bone_poses, md.armature_parent, md.bone_count, an, md.animation_time});
-        }
-        pj = (int32_t)it->second;
-      }
-      part_pose_job.push_back(pj);
-      palettes_host.push_back(md.skinned && pj < 0 ? &md.bone_influences[0].m[0][0] : nullptr);
-      palette_count.push_back(md.skinned && pj < 0 ? md.bone_count : 0);
-      skw_host.push_back(mesh.K ? md.shape_key_weights : nullptr);
-      skw_count.push_back(mesh.K);
-      // algorithmic bytes: mesh streams read once, triangle indices once, outputs
-      const uint64_t vb = 24 + (mesh.col ? 16 : 0) + (md.skinned ? 24 : 0);
-      f->bytes_vertex.push_back((uint64_t)P.vcount * vb + (md.skinned ? 64ull * md.bone_count : 0) + sizeof(PartGPU));
-      f->bytes_cull.push_back((mesh.idx16 ? 6ull : 12ull) * P.tcount);
-      f->bytes_uv.push_back(mesh.uv ? 8ull * P.vcount : 0);   // UVs are read by the assembly
-      f->part_first_tri.push_back(pr.triangle_start);
-      t3d_pair_result r{};
-      r.model_index = rp.model;
-      r.part_index = rp.part;
-      r.triangle_count = pr.triangle_count;
-      r.vertex_offset = P.out_base;
-      f->pairs.push_back(r);
+      const uint32_t n = (uint32_t)f->pairs.size() - first;
+      f->flush_begin.push_back(first);
+      f->flush.push_back(g);
+      if (n > 1) f->groups.push_back(make_uint2(first, n));
     }
     f->job_pair_begin.push_back((uint32_t)f->pairs.size());
+    f->job_flush_begin.push_back((uint32_t)f->flush.size());
   }
   const auto wall1 = std::chrono::steady_clock::now();   // render pairs and uniforms done
   const uint32_t P_n = (uint32_t)f->parts.size();
   f->nparts = P_n;
+  if (f->grouped) f->flush_begin.push_back(P_n);
+  const uint32_t ngroups = (uint32_t)f->groups.size();
 
   // palettes and shape-key weights live in the table block
   uint64_t pal_floats = 0, skw_floats = 0, pose_bones = 0, pose_roots = 0;
@@ -1351,7 +1509,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
                           4 * carve_size<uint32_t>(P_n + 1) +
                           carve_size<float>(pal_floats) + carve_size<float>(skw_floats) +
                           carve_size<t3d_bone_pose>(pose_bones) + carve_size<M4>(pose_roots) +
-                          carve_size<PaletteJob>(npose);
+                          carve_size<PaletteJob>(npose) + carve_size<uint2>(ngroups);
   // scratch: vertex records, skinned positions, keys, survivor records, stats, histograms, block sums
   const uint32_t ntiles_total = f->tile_prefix.back();
   const uint64_t sbytes = carve_size<float4>(2 * sumV) + carve_size<float4>(sumV) + carve_size<uint32_t>(sumT) +
@@ -1443,6 +1601,9 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   M4* h_roots = host.take<M4>(pose_roots);
   PaletteJob* d_pjobs = dev.take<PaletteJob>(npose);
   PaletteJob* h_pjobs = host.take<PaletteJob>(npose);
+  uint2* d_groups = dev.take<uint2>(ngroups);
+  uint2* h_groups = host.take<uint2>(ngroups);
+  if (ngroups) std::memcpy(h_groups, f->groups.data(), sizeof(uint2) * ngroups);
   uint64_t pal_off = 0, skw_off = 0;
   for (uint32_t i = 0; i < P_n; i++) {
     PartGPU& P = f->parts[i];
@@ -1529,6 +1690,8 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   a.vud = sc.take<float>(sumV);
   a.out_normal = reinterpret_cast<float4*>(out_normal);
   a.pal_jobs = d_pjobs;
+  a.groups = d_groups;
+  a.ngroups = ngroups;
   a.npal_jobs = npose;
   // poses and armature transforms read, palettes written
   f->bytes_palettes = (sizeof(t3d_bone_pose) + 64) * pose_bones + 64 * pose_roots + sizeof(PaletteJob) * npose;
@@ -1657,6 +1820,17 @@ void frame_collect(t3d_ctx* c, t3d_frame* f, bool profile) {
     f->pairs[i].survivor_count = f->stats_host[4 * i + 2];
     total += f->pairs[i].survivor_count;
   }
+  if (f->grouped)   // a flush's render pairs follow one another in its vertex lists, as k_groups placed them
+    for (size_t g = 0; g < f->flush.size(); g++) {
+      uint64_t off = f->flush[g].vertex_offset;
+      uint32_t n = 0;
+      for (uint32_t i = f->flush_begin[g]; i < f->flush_begin[g + 1]; i++) {
+        f->pairs[i].vertex_offset = off;
+        off += 3ull * f->pairs[i].survivor_count;
+        n += f->pairs[i].survivor_count;
+      }
+      f->flush[g].survivor_count = n;
+    }
   f->total_survivors = total;
   if (profile && c->profiling) c->prof["k_assemble"].bytes += 300ull * total;   // 3 x (48 B colour + 48 B depth + 4 B index)
 }
@@ -1847,7 +2021,13 @@ t3d_status render_submit(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc) 
   t3d_status st = frame_finish(c, f);
   if (st) return st;
   st = frame_build(c, f, &desc, 1);
-  if (st) return st;
+  if (st) {   // nothing of a refused frame is published
+    f->nparts = 0;
+    f->pairs.clear();
+    f->flush.clear();
+    f->grouped = false;
+    return st;
+  }
   st = frame_launch(c, f, f->stream ? f->stream : c->stream, c->copy_stream, desc->exec_mode, true, true);
   if (st) return st;
   f->submitted = wall0;
@@ -1882,8 +2062,8 @@ extern "C" {
 t3d_status t3d_frame_pairs(const t3d_frame* f, uint32_t* out_count, const t3d_pair_result** out_pairs) {
   if (!f || !out_count || !out_pairs) return T3D_ERR_INVALID_ARGUMENT;
   if (f->inflight) return T3D_ERR_NOT_READY;
-  *out_count = (uint32_t)f->pairs.size();
-  *out_pairs = f->pairs.data();
+  *out_count = (uint32_t)f->out_pairs().size();
+  *out_pairs = f->out_pairs().data();
   return T3D_OK;
 }
 

@@ -79,7 +79,9 @@ struct alignas(16) PartGPU {
   uint32_t nsblocks, bone_count;
   uint32_t cam;               // camera (job) of the pair in FrameArgs.fc
   float max_span;             // MaxSpan of the mesh dimensions (mesh.go:41-43), for the point-light skip
-  uint32_t light0, pad2;      // first FrameArgs.lights entry of the pair's light list
+  uint32_t light0;            // first FrameArgs.lights entry of the pair's light list
+  uint32_t idx_base;          // first index value: 0, or for a later part of a dynamic batch's flush the corners
+                              // drawn before it in that flush (k_groups, which also moves out_base past them)
   uint64_t out_base;          // first output vertex
   M4 nrot;                    // rotation of Model.Transform()·ViewMatrix() (model.go:705-708), by k_uniforms
                               // when the pair's camera renders normals
@@ -178,7 +180,9 @@ struct FrameArgs {
   const FrameConst* fc;         // [jobs] device copies, so graph kernel arguments stay fixed across frames
   const PaletteJob* pal_jobs;   // [npal_jobs] armatures posed on the device
   M4* pose_world;               // bone Transform() scratch of the palette jobs
-  uint32_t npal_jobs, pad2;
+  uint32_t npal_jobs;
+  uint32_t ngroups;             // dynamic-batch flushes of more than one render pair
+  const uint2* groups;          // [ngroups] {first render pair, render pairs} of each, pairs consecutive
 };
 
 // Packing of a range of render pairs' survivors for download: pair j of the
@@ -216,6 +220,7 @@ hipError_t launch_assemble(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uin
                            hipStream_t s);
 hipError_t launch_pack(const PackArgs& a, uint32_t nblocks, hipStream_t s);
 hipError_t launch_normals(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s);
+hipError_t launch_groups(const FrameArgs& a, uint32_t part0, uint32_t part1, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 hipError_t launch_fill(void* dst, uint64_t bytes, uint32_t word, hipStream_t s);
 hipError_t launch_fp_selftest(const float* a, const float* b, float* out, uint32_t n, hipStream_t s);

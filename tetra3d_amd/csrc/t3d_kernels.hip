@@ -11,6 +11,8 @@
 //   k_cull      per triangle: w clamp, z-range / screen / backface rejects,
 //               depth key, per-part survivor count and min/max depth
 //               (model.go:828-969)
+//   k_groups    dynamic batches: each later part of one flush placed after
+//               the survivors of the parts before it (camera.go:2075-2099)
 //   k_hist      per tile of a multi-tile part: 512-bin histogram
 //   k_scan*     exclusive scan of each part's [bin][tile] histogram matrix
 //   k_scatter   stable counting sort: output slot of every survivor
@@ -1083,6 +1085,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_assemble(FrameArgs a, uint32_t 
   const float src_w = P.src_w, src_h = P.src_h;
   const float* __restrict__ uvp = P.uv;
   const uint64_t out_base = P.out_base;
+  const uint32_t idx_base = P.idx_base;
   const bool persp = fc.perspective != 0, pc_tex = fc.pc_tex != 0, rdepth = fc.render_depth != 0;
   const float cam_w = fc.cam_w, half_w = fc.half_w, cam_h = fc.cam_h, half_h = fc.half_h;
   const float dmp = fc.dmp, spread = fc.spread;
@@ -1190,11 +1193,11 @@ __global__ __launch_bounds__(64 * kWaves) void k_assemble(FrameArgs a, uint32_t 
         store4<kNT>(&od[3 * vi + 2], cq[3 * vi + 2]);
       }
     }
-    // indices: pair-local iota, as the reference's indexList (camera.go:1816)
+    // indices: flush-local iota, as the reference's indexList (camera.go:1816)
 #pragma unroll
     for (uint32_t q = 0; q < 3; q++) {
       const uint32_t f = q * 64 + lane;
-      if (f < wcount * 3) store1<kNT>(&out_i[(size_t)wk0 * 3 + f], wk0 * 3 + f);
+      if (f < wcount * 3) store1<kNT>(&out_i[(size_t)wk0 * 3 + f], idx_base + wk0 * 3 + f);
     }
   }
 }
@@ -1225,6 +1228,7 @@ __global__ __launch_bounds__(64) void k_assemble_pipe(FrameArgs a, uint32_t blk0
   const float src_w = P.src_w, src_h = P.src_h;
   const float2* __restrict__ uvp = reinterpret_cast<const float2*>(P.uv);
   const uint64_t out_base = P.out_base;
+  const uint32_t idx_base = P.idx_base;
   const bool persp = fc.perspective != 0, pc_tex = fc.pc_tex != 0, rdepth = fc.render_depth != 0;
   const float cam_w = fc.cam_w, half_w = fc.half_w, cam_h = fc.cam_h, half_h = fc.half_h;
   const float dmp = fc.dmp, spread = fc.spread;
@@ -1312,7 +1316,7 @@ __global__ __launch_bounds__(64) void k_assemble_pipe(FrameArgs a, uint32_t blk0
 #pragma unroll
     for (uint32_t q = 0; q < 3; q++) {                            // indices (camera.go:1816)
       const uint32_t f = q * 64 + lane;
-      store1<kNT>(f < wcount * 3 ? &out_i[(size_t)wk0 * 3 + f] : sink_i, wk0 * 3 + f);
+      store1<kNT>(f < wcount * 3 ? &out_i[(size_t)wk0 * 3 + f] : sink_i, idx_base + wk0 * 3 + f);
     }
   };
 
@@ -1372,6 +1376,46 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs a) {
   if (a.pdepth) a.pdepth[3 * d + e] = a.depth[3 * s + e];
   if (a.pnormal) a.pnormal[3 * d + e] = a.normal[3 * s + e];
   if (a.pidx && e < n) a.pidx[d + e] = a.idx[s + e];
+}
+
+// Dynamic batches (camera.go:2075-2099): the render pairs of one flush write
+// into one vertex list, each after the corners of the ones before it, and
+// their indices count on from there (the reference's vertexListIndex).  After
+// k_cull, one block per batch scans its pairs' survivor counts and moves each
+// pair's out_base and idx_base accordingly.  Only pairs in [part0, part1) are
+// written (the plain form runs one pair's chain at a time; the pairs before
+// it in its flush are final by then).  The first pair of a flush keeps the
+// flush's own out_base, which every block reads before any write.
+__global__ __launch_bounds__(256) void k_groups(FrameArgs a, uint32_t part0, uint32_t part1) {
+  const uint2 g = a.groups[blockIdx.x];
+  const uint32_t first = g.x, end = g.x + g.y;
+  if (end <= part0 || first >= part1) return;                       // block-uniform
+  PartGPU* parts = const_cast<PartGPU*>(a.parts);
+  const uint64_t base = parts[first].out_base;
+  __shared__ uint32_t s_scan[256];
+  __shared__ uint32_t s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (uint32_t c0 = first; c0 < end; c0 += 256) {                  // block-uniform trip count
+    const uint32_t i = c0 + threadIdx.x;
+    const uint32_t v = i < end ? a.stats[4 * i + 2] : 0u;
+    s_scan[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t d = 1; d < 256; d <<= 1) {                        // inclusive scan
+      const uint32_t t = threadIdx.x >= d ? s_scan[threadIdx.x - d] : 0u;
+      __syncthreads();
+      s_scan[threadIdx.x] += t;
+      __syncthreads();
+    }
+    const uint32_t before = s_carry + s_scan[threadIdx.x] - v;      // survivors of the flush's earlier pairs
+    if (i < end && i >= part0 && i < part1) {
+      parts[i].out_base = base + 3ull * before;
+      parts[i].idx_base = 3u * before;
+    }
+    __syncthreads();
+    if (threadIdx.x == 255) s_carry += s_scan[255];
+    __syncthreads();
+  }
 }
 
 // Camera.RenderNormals: the normal vertex list (camera.go:1710-1722), after
@@ -1509,6 +1553,11 @@ hipError_t launch_pack(const PackArgs& a, uint32_t nblocks, hipStream_t s) {
 hipError_t launch_normals(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s) {
   if (!nblk) return hipSuccess;
   hipLaunchKernelGGL(k_normals, dim3(nblk), dim3(256), 0, s, a, blk0);
+  return hipGetLastError();
+}
+hipError_t launch_groups(const FrameArgs& a, uint32_t part0, uint32_t part1, hipStream_t s) {
+  if (!a.ngroups) return hipSuccess;
+  hipLaunchKernelGGL(k_groups, dim3(a.ngroups), dim3(256), 0, s, a, part0, part1);
   return hipGetLastError();
 }
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s) {

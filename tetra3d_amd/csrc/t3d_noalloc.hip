@@ -78,6 +78,8 @@ extern "C" t3d_status t3d_render_noalloc(t3d_ctx* c, t3d_frame* f, const t3d_ren
   if (st) {
     f->nparts = 0;   // nothing of a refused frame is published
     f->pairs.clear();
+    f->flush.clear();
+    f->grouped = false;
     return st;
   }
   st = frame_run_placed(c, f, c->stream, desc->exec_mode);

@@ -91,6 +91,16 @@ struct t3d_frame {
   uint64_t bytes_palettes = 0;                               // of k_palettes, all armatures
   std::vector<t3d::FrameConst> fcs;      // one camera per job of the frame
   std::vector<uint32_t> job_pair_begin;  // [jobs + 1] first render pair of each job
+  // Flushes (the render pairs the caller sees): with no dynamic batch in the frame, one per render pair and
+  // `pairs` itself; with one, `flush` holds a record per flush, flush_begin [flushes + 1] its first render pair,
+  // job_flush_begin [jobs + 1] each job's first flush, and groups the batches of more than one render pair.
+  bool grouped = false;
+  std::vector<t3d_pair_result> flush;
+  std::vector<uint32_t> flush_begin;
+  std::vector<uint32_t> job_flush_begin;
+  std::vector<uint2> groups;
+  const std::vector<t3d_pair_result>& out_pairs() const { return grouped ? flush : pairs; }
+  uint32_t job_out_begin(uint32_t k) const { return grouped ? job_flush_begin[k] : job_pair_begin[k]; }
   uint64_t table_bytes = 0;
   uint64_t total_triangles = 0;
   uint32_t nparts = 0;

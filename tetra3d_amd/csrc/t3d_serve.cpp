@@ -283,10 +283,11 @@ void complete_slot(t3d_server* s, int si) {
   const double now = s->now_ms();
   for (uint32_t k = 0; k < b->jobs.size(); k++) {
     Job& j = *s->jobs[b->jobs[k]];
-    const uint32_t p0 = b->frame->job_pair_begin[k], p1 = b->frame->job_pair_begin[k + 1];
+    const auto& op = b->frame->out_pairs();
+    const uint32_t p0 = b->frame->job_out_begin(k), p1 = b->frame->job_out_begin(k + 1);
     j.pairs = p1 - p0;
     j.survivors = 0;
-    for (uint32_t p = p0; p < p1; p++) j.survivors += b->frame->pairs[p].survivor_count;
+    for (uint32_t p = p0; p < p1; p++) j.survivors += op[p].survivor_count;
     j.state = T3D_JOB_DONE;
     j.done_ms = now;
     s->completed++;
@@ -687,12 +688,14 @@ t3d_status job_download(t3d_server* s, uint64_t job, t3d_pair_result* pairs, t3d
     f = j.batch->frame;   // stays with the batch while this job is unreleased (pins block release)
     p0 = f->job_pair_begin[j.index_in_batch];
     p1 = f->job_pair_begin[j.index_in_batch + 1];
-    if (pairs) {
+    if (pairs) {   // the job's flush records, offsets into its own concatenated arrays
+      const auto& op = f->out_pairs();
+      const uint32_t g0 = f->job_out_begin(j.index_in_batch), g1 = f->job_out_begin(j.index_in_batch + 1);
       uint64_t off = 0;
-      for (uint32_t p = p0; p < p1; p++) {
-        pairs[p - p0] = f->pairs[p];
-        pairs[p - p0].vertex_offset = off;
-        off += 3ull * pairs[p - p0].survivor_count;
+      for (uint32_t g = g0; g < g1; g++) {
+        pairs[g - g0] = op[g];
+        pairs[g - g0].vertex_offset = off;
+        off += 3ull * pairs[g - g0].survivor_count;
       }
     }
     j.pins++;

@@ -11,7 +11,7 @@ import re
 import numpy as np
 
 from . import _abi as A
-from .scene import RenderInputs
+from .scene import RenderInputs, with_batched
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libt3d.so")
@@ -313,7 +313,7 @@ class Context:
 
     def render(self, camera, scene, lights, models, exec_mode=A.EXEC_BATCHED, download=True):
         """Camera.Render(scene, lights, models) (camera.go:1269) on the GPU."""
-        for m in models:
+        for m in with_batched(models):
             if m.mesh is not None:
                 self.upload_mesh(m.mesh)
         for l in lights:

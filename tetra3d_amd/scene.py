@@ -267,6 +267,35 @@ class Model(Node):
         self.animation_time = 0.0    # its playhead (AnimationPlayer.Playhead() when Update samples)
         self.shape_key_weights = None
         self.light_group = None      # list of lights when a LightGroup is active
+        self.dynamic_batch_models = {}   # part index -> [Model], DynamicBatchModels (model.go:29)
+        self.dynamic_batch_owner = None  # DynamicBatchOwner (model.go:30)
+
+    def dynamic_batch_add(self, part_index, *models):
+        """DynamicBatchAdd (model.go:217-247): the models draw inside this
+        model's flush for its MeshPart `part_index`, and this model's own mesh
+        is no longer drawn.  A model batched elsewhere moves here.  (The
+        reference's MaxTriangleCount cap follows from 16-bit index lists; the
+        GPU path writes 32-bit indices and has no such cap.)"""
+        for other in models:
+            if other is self or any(other is m for ms in self.dynamic_batch_models.values() for m in ms):
+                continue
+            if other.dynamic_batch_owner is not None:
+                other.dynamic_batch_owner.dynamic_batch_remove(other)
+            self.dynamic_batch_models.setdefault(part_index, []).append(other)
+            other.dynamic_batch_owner = self
+
+    def dynamic_batch_remove(self, *models):
+        """DynamicBatchRemove (model.go:249-270)."""
+        for m in models:
+            for part in list(self.dynamic_batch_models):
+                lst = self.dynamic_batch_models[part]
+                for i, existing in enumerate(lst):
+                    if existing is m:
+                        del lst[i]
+                        m.dynamic_batch_owner = None
+                        break
+        for part in [p for p, lst in self.dynamic_batch_models.items() if not lst]:
+            del self.dynamic_batch_models[part]
 
     def desc(self, mesh_handle, keep):
         d = A.ModelDesc()
@@ -513,6 +542,23 @@ class Camera(Node):
         return d
 
 
+def with_batched(models):
+    """The models of a render call followed by every dynamically batched
+    model of a batcher among them that is not listed itself: those draw
+    through their owner's flush, so the call describes them too."""
+    models = list(models)
+    seen = {id(m) for m in models}
+    k = 0
+    while k < len(models):
+        for lst in models[k].dynamic_batch_models.values():
+            for b in lst:
+                if id(b) not in seen:
+                    seen.add(id(b))
+                    models.append(b)
+        k += 1
+    return models
+
+
 class RenderInputs:
     """The descriptors of one Camera.Render(scene, lights, models) call.
 
@@ -528,10 +574,24 @@ class RenderInputs:
         self.lights = (A.LightDesc * max(1, len(lights)))()
         for i, l in enumerate(lights):
             self.lights[i] = l.desc()
+        models = with_batched(models)
+        index = {id(m): i for i, m in enumerate(models)}
+        self.model_list = models
         self.models = (A.ModelDesc * max(1, len(models)))()
         for i, m in enumerate(models):
             h = mesh_handle(m.mesh) if m.mesh is not None else 0
             self.models[i] = m.desc(h, self.keep)
+            if m.dynamic_batch_owner is not None and id(m.dynamic_batch_owner) not in index:
+                # batched under a model this call does not render: never drawn (camera.go:1346)
+                self.models[i].visible = 0
+        for i, m in enumerate(models):
+            for part, lst in m.dynamic_batch_models.items():
+                for order, b in enumerate(lst):
+                    d = self.models[index[id(b)]]
+                    d.dynamic_batched = 1
+                    d.batch_owner = i
+                    d.batch_part = part
+                    d.batch_order = order
         self.render = A.RenderDesc()
         self.render.camera = C.pointer(self.cam)
         self.render.world = C.pointer(self.world)

@@ -20,7 +20,7 @@ import numpy as np
 from . import _abi as A
 from .scene import (AmbientLight, Camera, DirectionalLight, LightVolume, Material, Mesh, MeshPart, Model,
                     PointLight, Scene, World, identity, mat_mult, mat_scale, mat_translate, quat_to_matrix,
-                    rotation_from_forward)
+                    rotation_from_forward, with_batched)
 
 f32 = np.float32
 DEFAULT_SEED = 0x7E7A3D
@@ -328,7 +328,7 @@ class Workload:
         self.lights = lights
         self.models = models
         seen, meshes = set(), []
-        for m in models:
+        for m in with_batched(models):   # dynamically batched models draw through their owners
             if m.mesh is not None and id(m.mesh) not in seen:
                 seen.add(id(m.mesh))
                 meshes.append(m.mesh)
@@ -340,7 +340,7 @@ class Workload:
         """The distinct animations the models play (read when rendered, so a
         model may be given one after the workload is made)."""
         anims = {}
-        for m in self.models:
+        for m in with_batched(self.models):
             an = getattr(m, "animation", None)
             if an is not None:
                 anims[id(an)] = an
