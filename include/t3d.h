@@ -74,6 +74,9 @@ enum { T3D_BILLBOARD_UP_GLOBAL_Y = 0, T3D_BILLBOARD_UP_CAMERA_Y = 1 };
 enum { T3D_DEPTH_MODE_DEFAULT = 0, T3D_DEPTH_MODE_UNBILLBOARDED = 1 };
 /* FogBlendMode (world.go:8-11) */
 enum { T3D_FOG_ADD = 0, T3D_FOG_SUB = 1, T3D_FOG_OVERWRITE = 2, T3D_FOG_TRANSPARENT = 3 };
+/* World.FogCurve (world.go:13-20) and Material.TextureFilterMode (material.go) */
+enum { T3D_FOG_CURVE_LINEAR = 0, T3D_FOG_CURVE_OUT_CIRC = 1, T3D_FOG_CURVE_IN_CIRC = 2 };
+enum { T3D_FILTER_NEAREST = 0, T3D_FILTER_LINEAR = 1 };
 /* Light node types handled on the path (node.go:25-43) */
 enum { T3D_LIGHT_AMBIENT = 0, T3D_LIGHT_POINT = 1, T3D_LIGHT_DIRECTIONAL = 2, T3D_LIGHT_VOLUME = 3 };
 /* Execution forms of t3d_render */
@@ -145,6 +148,10 @@ typedef struct {
   int32_t depth_mode;                /* BillboardedDepthMode: T3D_DEPTH_MODE_* (unbillboarded needs a static model) */
   int32_t billboard_up;              /* BillboardUpDirection: T3D_BILLBOARD_UP_* */
   int32_t billboard_lock[3];         /* BillboardLockX, BillboardLockY, BillboardLockZ */
+  /* read only by the raster sink (t3d_frame_rasterize) */
+  uint64_t texture;                  /* with use_texture: handle from t3d_texture_upload (0: the default white image) */
+  int32_t texture_filter;            /* Material.TextureFilterMode: T3D_FILTER_* */
+  int32_t pad_texture;
 } t3d_material_desc;
 
 /* A light as seen by Camera.Render (light.go:12-30 and the concrete types). */
@@ -283,6 +290,10 @@ typedef struct {
   float ambient_color[3];            /* World.AmbientLight */
   float ambient_energy;
   int32_t ambient_visible;
+  /* read only by the raster sink */
+  float clear_color[4];              /* World.ClearColor, what Camera.Clear fills the colour texture with (camera.go:1002-1013) */
+  int32_t fog_curve;                 /* World.FogCurve: T3D_FOG_CURVE_* */
+  float dithered_fog_size;           /* World.DitheredFogSize (0: no dithering) */
 } t3d_world_desc;
 
 /* One Camera.Render(scene, lights, models) call. */
@@ -392,6 +403,33 @@ t3d_status t3d_frame_download(t3d_ctx* ctx, const t3d_frame* frame, t3d_vertex* 
  * pair order like t3d_frame_download. */
 t3d_status t3d_frame_device_normals(const t3d_frame* frame, const t3d_vertex** normal_vertices);
 t3d_status t3d_frame_download_normals(t3d_ctx* ctx, const t3d_frame* frame, t3d_vertex* normal_vertices);
+/* ---- raster sink ------------------------------------------------------
+ * The flushes of a rendered frame drawn into the camera's result textures on
+ * the GPU, as flush hands them to ebiten (camera.go:1829-2048) with the
+ * depth, clip-alpha and base3d shaders (camera.go:350-531,
+ * shaders/base3d.kage:61-153): Camera.Clear with the world's clear colour,
+ * then per flush the depth pass into a cleared intermediate tested against
+ * the result depth, the intermediate drawn over the result depth unless the
+ * pair is transparent, and the colour pass (texture, vertex colour, fog,
+ * dithering); without RenderDepth one textured draw per flush.  Images are
+ * W x H RGBA8 with premultiplied alpha, rows top first.  The fixed-function
+ * rules ebiten's GPU applies (pixel centres, top-left ties, linear
+ * interpolation, 8-bit rounding) are fixed by this library.  Frames of one
+ * Camera.Render only; flushes blending other than SourceOver are refused.
+ * (The reference's normal texture is not drawn: its normal vertices never
+ * get an alpha, camera.go:1710-1722, so that draw leaves it cleared.) */
+/* An ebiten.Image for Material.Texture: RGBA8, premultiplied, row-major.
+ * A texture must stay allocated while frames that name it are rasterized. */
+t3d_status t3d_texture_upload(t3d_ctx* ctx, uint32_t width, uint32_t height, const uint8_t* rgba8, uint64_t* out_handle);
+t3d_status t3d_texture_free(t3d_ctx* ctx, uint64_t handle);
+/* Draw the last render of `frame` (finished: after t3d_render, or after
+ * t3d_frame_wait) into its images; returns once they are complete in HBM. */
+t3d_status t3d_frame_rasterize(t3d_ctx* ctx, t3d_frame* frame);
+/* The images of the last t3d_frame_rasterize: device pointers (depth NULL
+ * without RenderDepth), valid until the frame's next rasterize; and their
+ * copy to host memory (W * H * 4 bytes each; depth may be NULL). */
+t3d_status t3d_frame_device_images(const t3d_frame* frame, const uint8_t** color, const uint8_t** depth);
+t3d_status t3d_frame_download_images(t3d_ctx* ctx, const t3d_frame* frame, uint8_t* color, uint8_t* depth);
 /* Mesh triangle index of every drawn triangle, pair-concatenated in draw
  * order: the order sortingTriangleBucket.ForEach visits them
  * (sortingTriangles.go:93-125).  Holds sum(survivor_count) entries. */

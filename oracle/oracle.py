@@ -12,7 +12,7 @@ import subprocess
 import numpy as np
 
 from tetra3d_amd import _abi as A
-from tetra3d_amd.scene import RenderInputs
+from tetra3d_amd.scene import RenderInputs, with_batched
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 # T3D_ORACLE_LIB points the checker at another build of the same sources (a
@@ -57,6 +57,9 @@ def lib():
         L.t3do_result_free.argtypes = [C.POINTER(OracleResult)]
         L.t3do_last_error.restype = C.c_char_p
         L.t3do_last_error.argtypes = [C.c_void_p]
+        U8 = C.POINTER(C.c_uint8)
+        L.t3do_texture_register.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, U8]
+        L.t3do_rasterize.argtypes = [C.c_void_p, C.POINTER(A.RenderDesc), C.POINTER(OracleResult), U8, U8]
         for n in ("t3do_mat4_mult",):
             getattr(L, n).argtypes = [M4, M4, M4]
         L.t3do_bone_palettes.argtypes = [C.POINTER(A.BonePose), C.c_uint32, M4, M4]
@@ -177,6 +180,47 @@ class Oracle:
 
     def handle_of(self, mesh):
         return self._handles[id(mesh)]
+
+    def register_texture(self, tex):
+        """Register a Texture under its handle (one of the oracle's own when
+        it has none yet)."""
+        if id(tex) in self._handles:
+            return
+        if not tex.handle:
+            self._own_handle += 1
+            tex.handle = self._own_handle
+        h, w = tex.rgba.shape[:2]
+        rc = self.L.t3do_texture_register(self.ctx, tex.handle, w, h, tex.rgba.ctypes.data_as(C.POINTER(C.c_uint8)))
+        if rc != 0:
+            raise RuntimeError("oracle texture register: %s" % self.L.t3do_last_error(self.ctx).decode())
+        self._handles[id(tex)] = tex.handle
+        self._refs.append(tex)
+
+    def rasterize(self, camera, scene, lights, models):
+        """Render, then draw the flushes into the camera's result textures:
+        (frame, colour (H, W, 4) uint8, depth (H, W, 4) uint8 or None)."""
+        for m in with_batched(models):
+            for p in (m.mesh.mesh_parts if m.mesh is not None else []):
+                if p.material is not None and p.material.texture is not None:
+                    self.register_texture(p.material.texture)
+        inp = RenderInputs(camera, scene, lights, models, self.handle_of)
+        res = OracleResult()
+        rc = self.L.t3do_render(self.ctx, C.byref(inp.render), C.byref(res))
+        if rc != 0:
+            raise RuntimeError("oracle render (%d): %s" % (rc, self.L.t3do_last_error(self.ctx).decode()))
+        try:
+            frame = Frame(res)
+            W, H = camera.width, camera.height
+            color = np.zeros((H, W, 4), dtype=np.uint8)
+            depth = np.zeros((H, W, 4), dtype=np.uint8) if camera.render_depth else None
+            U8 = C.POINTER(C.c_uint8)
+            rc = self.L.t3do_rasterize(self.ctx, C.byref(inp.render), C.byref(res), color.ctypes.data_as(U8),
+                                       depth.ctypes.data_as(U8) if depth is not None else None)
+            if rc != 0:
+                raise RuntimeError("oracle rasterize (%d): %s" % (rc, self.L.t3do_last_error(self.ctx).decode()))
+            return frame, color, depth
+        finally:
+            self.L.t3do_result_free(C.byref(res))
 
     def render(self, camera, scene, lights, models):
         inp = RenderInputs(camera, scene, lights, models, self.handle_of)

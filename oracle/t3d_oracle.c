@@ -452,6 +452,12 @@ typedef struct {
   uint32_t n;
 } oanim;
 
+typedef struct {
+  uint64_t handle;
+  uint32_t w, h;
+  uint8_t* px;            /* RGBA8, premultiplied, row-major */
+} otexture;
+
 struct t3do_ctx {
   omesh* meshes;
   int nmeshes, cap_meshes;
@@ -459,6 +465,8 @@ struct t3do_ctx {
   int nvols, cap_vols;
   oanim* anims;
   int nanims, cap_anims;
+  otexture* texs;
+  int ntexs, cap_texs;
   char err[512];
 };
 
@@ -485,9 +493,11 @@ void t3do_destroy(t3do_ctx* c) {
     free(c->anims[i].ch);
     free(c->anims[i].data);
   }
+  for (int i = 0; i < c->ntexs; i++) free(c->texs[i].px);
   free(c->meshes);
   free(c->vols);
   free(c->anims);
+  free(c->texs);
   free(c);
 }
 
@@ -1733,6 +1743,17 @@ int t3do_render(t3do_ctx* c, const t3d_render_desc* desc, t3do_result* res) {
   ocam cam;
   cam_setup(&cam, cd);
 
+  /* a LightVolume must name registered cells (as the product refuses an unknown volume handle) */
+  for (uint32_t i = 0; i < desc->light_count; i++)
+    if (desc->lights[i].type == T3D_LIGHT_VOLUME && !find_vol(c, desc->lights[i].volume))
+      return set_err(c, "render: light %u names an unregistered volume", i), T3D_ERR_NOT_FOUND;
+  for (uint32_t mi = 0; mi < desc->model_count; mi++) {
+    const t3d_model_desc* md = &desc->models[mi];
+    for (uint32_t k = 0; md->light_group_active && md->light_group && k < md->light_group_count; k++)
+      if (md->light_group[k].type == T3D_LIGHT_VOLUME && !find_vol(c, md->light_group[k].volume))
+        return set_err(c, "render: model %u light group names an unregistered volume", mi), T3D_ERR_NOT_FOUND;
+  }
+
   /* scene light list: camera.go:1275-1303 */
   int nl = 0;
   olight* lights = (olight*)calloc(desc->light_count + 1, sizeof(olight));
@@ -2023,4 +2044,370 @@ static int less_table(void* env, int32_t a, int32_t b) {
 void t3do_go_stable_sort_table(int32_t* items, uint32_t n, const uint8_t* less_table_) {
   table_env e = {less_table_, n};
   go_stable_sort(items, (int)n, less_table, &e);
+}
+
+/* ------------------------------------------------------------------------ */
+/* raster sink: flush and the Kage shaders (camera.go:350-531, 1829-2048;    */
+/* shaders/base3d.kage:61-153)                                               */
+/* ------------------------------------------------------------------------ */
+/*
+ * The reference hands each flush's vertex lists to ebiten, which draws them
+ * with the GPU's fixed-function rasteriser; that stage is not in the
+ * reference, so its rules are fixed here (and by the HIP raster kernels, to
+ * the bit):
+ *  - images are RGBA8 with premultiplied alpha; a float result x is stored as
+ *    floor(clamp(x, 0, 1) * 255 + 0.5);
+ *  - pixel (x, y) is sampled at its centre (x + 0.5, y + 0.5); a triangle
+ *    covers it when the three edge functions, computed in double precision
+ *    from each edge's endpoints taken in a fixed order (smaller (y, x) first)
+ *    so that a shared edge gives exactly opposite values, all have the sign of
+ *    the triangle's area, a zero counting only on a top or left edge;
+ *  - attributes are interpolated linearly in screen space (ebiten draws 2D
+ *    triangles), with barycentric weights edge / area in double precision;
+ *  - triangles of one draw are processed in index order; blending is
+ *    SourceOver, out = src + dst * (1 - src.a), in float32 per channel.
+ * The shader arithmetic follows the Kage sources in float32: pow(x, 2) is
+ * x * x, mod(x, y) = x - y * floor(x / y), mix(a, b, t) = a * (1 - t) + b * t.
+ */
+
+int t3do_texture_register(t3do_ctx* c, uint64_t handle, uint32_t w, uint32_t h, const uint8_t* rgba8) {
+  if (!w || !h || !rgba8) return set_err(c, "texture: empty");
+  if (c->ntexs == c->cap_texs) {
+    c->cap_texs = c->cap_texs ? 2 * c->cap_texs : 8;
+    c->texs = (otexture*)realloc(c->texs, sizeof(otexture) * c->cap_texs);
+  }
+  otexture* t = &c->texs[c->ntexs++];
+  t->handle = handle;
+  t->w = w;
+  t->h = h;
+  t->px = (uint8_t*)dup_mem(rgba8, 4ull * w * h);
+  return T3D_OK;
+}
+
+static const otexture* find_texture(t3do_ctx* c, uint64_t h) {
+  for (int i = 0; i < c->ntexs; i++)
+    if (c->texs[i].handle == h) return &c->texs[i];
+  return NULL;
+}
+
+static const uint8_t k_white[4] = {255, 255, 255, 255};
+static const otexture k_default_img = {0, 1, 1, (uint8_t*)k_white};   /* defaultImg: 1x1 white (tetra3d.go:12, 38) */
+
+static uint8_t q8(float x) {
+  if (!(x > 0.0f)) x = 0.0f;
+  if (x > 1.0f) x = 1.0f;
+  return (uint8_t)floorf(x * 255.0f + 0.5f);
+}
+static float fractf_(float x) { return x - floorf(x); }
+static float modf_(float x, float y) { return x - y * floorf(x / y); }
+static float mixf_(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
+/* imageSrcNUnsafeAt(p) for an unatlased image: the texel containing p */
+static void texel_at(const otexture* t, float x, float y, float out[4]) {
+  float fx = floorf(x), fy = floorf(y);
+  int ix = fx != fx ? 0 : (fx < 0 ? 0 : (fx >= (float)t->w ? (int)t->w - 1 : (int)fx));
+  int iy = fy != fy ? 0 : (fy < 0 ? 0 : (fy >= (float)t->h ? (int)t->h - 1 : (int)fy));
+  const uint8_t* p = t->px + 4 * ((size_t)iy * t->w + ix);
+  for (int k = 0; k < 4; k++) out[k] = (float)p[k] / 255.0f;
+}
+
+/* bilinearFilter (base3d.kage:33-50; the clip shader's, camera.go:420-437, without the atlas origin) */
+static void bilinear(const otexture* t, float x, float y, float out[4]) {
+  const float w = (float)t->w, h = (float)t->h;
+  const float p0x = modf_(x - 1.0f / 2.0f, w), p0y = modf_(y - 1.0f / 2.0f, h);
+  const float p1x = modf_(x + 1.0f / 2.0f, w), p1y = modf_(y + 1.0f / 2.0f, h);
+  float c0[4], c1[4], c2[4], c3[4];
+  texel_at(t, p0x, p0y, c0);
+  texel_at(t, p1x, p0y, c1);
+  texel_at(t, p0x, p1y, c2);
+  texel_at(t, p1x, p1y, c3);
+  const float rx = fractf_(p1x), ry = fractf_(p1y);
+  for (int k = 0; k < 4; k++) out[k] = mixf_(mixf_(c0[k], c1[k], rx), mixf_(c2[k], c3[k], rx), ry);
+}
+
+/* encodeDepth / decodeDepth (camera.go:354-364) */
+static void encode_depth(float d, float out[4]) {
+  out[0] = floorf(d * 255.0f) / 255.0f;
+  out[1] = floorf(fractf_(d * 255.0f) * 255.0f) / 255.0f;
+  out[2] = fractf_(d * 255.0f * 255.0f);
+  out[3] = 1.0f;
+}
+static float decode_depth(const uint8_t px[4]) {
+  return (float)px[0] / 255.0f + (((float)px[1] / 255.0f) / 255.0f) + (((float)px[2] / 255.0f) / 65025.0f);
+}
+
+static void source_over(uint8_t dst[4], const float src[4]) {
+  float o[4];
+  for (int k = 0; k < 4; k++) o[k] = src[k] + ((float)dst[k] / 255.0f) * (1.0f - src[3]);
+  for (int k = 0; k < 4; k++) dst[k] = q8(o[k]);
+}
+
+/* The edge function of a -> b at c, evaluated from the endpoint with the
+ * smaller (y, x) so that the two triangles sharing an edge get exactly
+ * opposite values. */
+static double edge_fn(float ax, float ay, float bx, float by, double cx, double cy) {
+  const int swap = (ay > by) || (ay == by && ax > bx);
+  const double px = swap ? bx : ax, py = swap ? by : ay, qx = swap ? ax : bx, qy = swap ? ay : by;
+  const double e = (qx - px) * (cy - py) - (qy - py) * (cx - px);
+  return swap ? -e : e;
+}
+/* a zero edge value counts on a top edge (horizontal, interior below) or a left edge (interior to the right, y down) */
+static int top_left(float ax, float ay, float bx, float by, double s) {
+  const double dx = s * ((double)bx - (double)ax), dy = s * ((double)by - (double)ay);
+  return (dy == 0.0 && dx > 0.0) || dy < 0.0;
+}
+
+typedef struct {
+  const t3d_vertex* v;    /* the triangle's three vertices in one list */
+  double area, s;
+  int x0, x1, y0, y1;     /* pixel range of its bounding box on the image */
+} otri;
+
+static int tri_setup(otri* t, const t3d_vertex* v, int W, int H) {
+  t->v = v;
+  t->area = edge_fn(v[0].dst_x, v[0].dst_y, v[1].dst_x, v[1].dst_y, (double)v[2].dst_x, (double)v[2].dst_y);
+  if (!(t->area != 0.0) || t->area != t->area) return 0;
+  t->s = t->area > 0 ? 1.0 : -1.0;
+  double mnx = v[0].dst_x, mxx = v[0].dst_x, mny = v[0].dst_y, mxy = v[0].dst_y;
+  for (int k = 1; k < 3; k++) {
+    if (v[k].dst_x < mnx) mnx = v[k].dst_x;
+    if (v[k].dst_x > mxx) mxx = v[k].dst_x;
+    if (v[k].dst_y < mny) mny = v[k].dst_y;
+    if (v[k].dst_y > mxy) mxy = v[k].dst_y;
+  }
+  /* pixels whose centre x + 0.5 lies in [mnx, mxx] */
+  double fx0 = ceil(mnx - 0.5), fx1 = floor(mxx - 0.5), fy0 = ceil(mny - 0.5), fy1 = floor(mxy - 0.5);
+  if (!(fx0 <= fx1) || !(fy0 <= fy1) || fx1 < 0 || fy1 < 0 || fx0 > W - 1 || fy0 > H - 1) return 0;
+  t->x0 = fx0 < 0 ? 0 : (int)fx0;
+  t->y0 = fy0 < 0 ? 0 : (int)fy0;
+  t->x1 = fx1 > W - 1 ? W - 1 : (int)fx1;
+  t->y1 = fy1 > H - 1 ? H - 1 : (int)fy1;
+  return 1;
+}
+
+/* coverage of pixel (x, y) and its barycentric weights */
+static int tri_cover(const otri* t, int x, int y, double l[3]) {
+  const t3d_vertex* v = t->v;
+  const double cx = x + 0.5, cy = y + 0.5;
+  const double e12 = edge_fn(v[1].dst_x, v[1].dst_y, v[2].dst_x, v[2].dst_y, cx, cy);
+  const double e20 = edge_fn(v[2].dst_x, v[2].dst_y, v[0].dst_x, v[0].dst_y, cx, cy);
+  const double e01 = edge_fn(v[0].dst_x, v[0].dst_y, v[1].dst_x, v[1].dst_y, cx, cy);
+  const double s = t->s;
+  if (!(s * e12 > 0 || (e12 == 0 && top_left(v[1].dst_x, v[1].dst_y, v[2].dst_x, v[2].dst_y, s)))) return 0;
+  if (!(s * e20 > 0 || (e20 == 0 && top_left(v[2].dst_x, v[2].dst_y, v[0].dst_x, v[0].dst_y, s)))) return 0;
+  if (!(s * e01 > 0 || (e01 == 0 && top_left(v[0].dst_x, v[0].dst_y, v[1].dst_x, v[1].dst_y, s)))) return 0;
+  l[0] = e12 / t->area;
+  l[1] = e20 / t->area;
+  l[2] = e01 / t->area;
+  return 1;
+}
+static float interp(const double l[3], float a0, float a1, float a2) {
+  return (float)(l[0] * (double)a0 + l[1] * (double)a1 + l[2] * (double)a2);
+}
+
+/* World uniforms of the colour pass (camera.go:1956-1977, world.go:96-110) */
+typedef struct {
+  float fog[4];           /* FogColor RGB, FogMode (-1: FogOn false) */
+  float fog_range[2];
+  float dither;
+  int curve;
+} ofog;
+
+/* sample an image at texel position (x, y): nearest (the texel containing it) or base3d's bilinear */
+static void sample(const otexture* img, int linear, float x, float y, float out[4]) {
+  if (linear) bilinear(img, x, y, out);
+  else texel_at(img, x, y, out);
+}
+
+static const float k_bayer[16] = {   /* camera.go:1257-1262 */
+    1.0f / 17.0f,  9.0f / 17.0f, 3.0f / 17.0f,  11.0f / 17.0f, 13.0f / 17.0f, 5.0f / 17.0f, 15.0f / 17.0f, 7.0f / 17.0f,
+    4.0f / 17.0f, 12.0f / 17.0f, 2.0f / 17.0f, 10.0f / 17.0f, 16.0f / 17.0f, 8.0f / 17.0f, 14.0f / 17.0f, 6.0f / 17.0f};
+
+static float smoothstepf_(float e0, float e1, float x) {
+  float t = (x - e0) / (e1 - e0);
+  t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+  return t * t * (3.0f - 2.0f * t);
+}
+
+/* base3d.kage Fragment (61-153) at a pixel whose depth intermediate has alpha > 0 */
+static void base3d(const otexture* img, int linear, const ofog* fg, int fogless, int pc, const uint8_t inter[4],
+                   const float vc[4], float sx, float sy, float c0, float px, float py, float out[4]) {
+  float tx = sx, ty = sy;
+  if (pc > 0) {                                                   /* tx *= 1.0 / custom.x */
+    const float r = 1.0f / c0;
+    tx *= r;
+    ty *= r;
+  }
+  tx = modf_(tx, (float)img->w);                                  /* mod(tx, srcSize) */
+  ty = modf_(ty, (float)img->h);
+  float tex[4];
+  sample(img, linear, tx, ty, tex);
+  for (int k = 0; k < 4; k++) out[k] = tex[k] * vc[k];
+  for (int k = 0; k < 3; k++) out[k] *= vc[3];                    /* colorTex.rgb *= color.a */
+  if (fogless) return;
+  const float dd = decode_depth(inter);
+  float v = dd;
+  if (fg->curve == T3D_FOG_CURVE_OUT_CIRC) v = sqrtf(1.0f - (dd - 1.0f) * (dd - 1.0f));
+  else if (fg->curve == T3D_FOG_CURVE_IN_CIRC) v = 1.0f - sqrtf(1.0f - dd * dd);
+  else if (fg->curve != T3D_FOG_CURVE_LINEAR) v = -INFINITY;     /* no curve matches: d stays 0 */
+  float d = v == -INFINITY ? 0.0f : smoothstepf_(fg->fog_range[0], fg->fog_range[1], v);
+  if (fg->dither > 0) {
+    const int yc = ((int)(py / fg->dither)) % 4, xc = ((int)(px / fg->dither)) % 4;
+    d = (d - k_bayer[yc * 4 + xc]) < 0.0f ? 0.0f : 1.0f;          /* step(0, d - Bayer) */
+  }
+  const float a = out[3];
+  if (fg->fog[3] == 0.0f) {
+    for (int k = 0; k < 3; k++) out[k] += fg->fog[k] * d * a;
+  } else if (fg->fog[3] == 1.0f) {
+    for (int k = 0; k < 3; k++) out[k] -= fg->fog[k] * d * a;
+  } else if (fg->fog[3] == 2.0f) {
+    for (int k = 0; k < 3; k++) out[k] = mixf_(out[k], fg->fog[k], d) * a;
+  }
+}
+
+/* Draws the flushes of a render into the camera's result textures: Camera.Clear (camera.go:1002-1040), then per
+ * flush in order (camera.go:1829-2048) the depth pass into a cleared intermediate against the result depth (the
+ * depth shader or, for AlphaClip, the clip-alpha shader), the intermediate drawn over the result depth unless the
+ * pair is transparent, and the colour pass through base3d with the intermediate; without RenderDepth one plain
+ * textured draw per flush.  color is W*H*4 bytes; depth (RenderDepth only) likewise, or NULL. */
+int t3do_rasterize(t3do_ctx* c, const t3d_render_desc* desc, const t3do_result* res, uint8_t* color, uint8_t* depth) {
+  const t3d_camera_desc* cd = desc->camera;
+  const int W = cd->width, H = cd->height;
+  const t3d_world_desc* world = (desc->world && desc->world->present) ? desc->world : NULL;
+  const int rdepth = cd->render_depth != 0;
+  if (rdepth && !depth) return set_err(c, "rasterize: a depth image is needed with RenderDepth");
+  /* Camera.Clear: the world's clear colour (transparent black without a world), premultiplied */
+  float clr[4] = {0, 0, 0, 0};
+  if (world)
+    for (int k = 0; k < 4; k++) clr[k] = world->clear_color[k];
+  const uint8_t cpx[4] = {q8(clr[0] * clr[3]), q8(clr[1] * clr[3]), q8(clr[2] * clr[3]), q8(clr[3])};
+  for (size_t i = 0; i < (size_t)W * H; i++) memcpy(color + 4 * i, cpx, 4);
+  if (depth) memset(depth, 0, 4ull * W * H);
+  ofog fg;
+  memset(&fg, 0, sizeof fg);
+  fg.fog_range[1] = 1.0f;
+  if (world) {
+    for (int k = 0; k < 3; k++) fg.fog[k] = world->fog_color[k];
+    fg.fog[3] = world->fog_on ? (float)world->fog_mode : -1.0f;
+    fg.fog_range[0] = world->fog_range[0];
+    fg.fog_range[1] = world->fog_range[1];
+    fg.dither = world->dithered_fog_size;
+    fg.curve = world->fog_curve;
+  }
+  uint8_t* inter = rdepth ? (uint8_t*)calloc((size_t)W * H, 4) : NULL;
+  int rc = T3D_OK;
+  for (uint32_t f = 0; f < res->pair_count && rc == T3D_OK; f++) {
+    const t3d_pair_result* pr = &res->pairs[f];
+    if (!pr->survivor_count) continue;                              /* flush: vertexListIndex == 0 */
+    const t3d_model_desc* model = &desc->models[pr->model_index];
+    const t3d_material_desc* mat = part_mat(model, pr->part_index);
+    if (mat && !mat->blend_source_over) { rc = set_err(c, "rasterize: only BlendSourceOver is drawn"); break; }
+    const otexture* img = &k_default_img;
+    if (mat && mat->use_texture && mat->texture) {
+      img = find_texture(c, mat->texture);
+      if (!img) { rc = set_err(c, "rasterize: unknown texture"); break; }
+    }
+    const int linear = mat && mat->texture_filter == T3D_FILTER_LINEAR;
+    const int fogless = mat && mat->fogless;
+    const int clip = mat && mat->transparency_mode == T3D_TRANSPARENCY_ALPHA_CLIP;
+    const int pc = cd->perspective_corrected_texture_mapping ? 1 : 0;
+    const t3d_vertex* cv = res->color + pr->vertex_offset;
+    const t3d_vertex* dv = res->depth + pr->vertex_offset;
+    const uint32_t nt = pr->survivor_count;
+    if (rdepth) {
+      /* depth pass: last passing fragment of the draw wins (each tests the result depth only) */
+      int bx0 = W, by0 = H, bx1 = -1, by1 = -1;
+      for (uint32_t k = 0; k < nt; k++) {
+        otri t;
+        if (!tri_setup(&t, dv + 3 * k, W, H)) continue;
+        if (t.x0 < bx0) bx0 = t.x0;
+        if (t.y0 < by0) by0 = t.y0;
+        if (t.x1 > bx1) bx1 = t.x1;
+        if (t.y1 > by1) by1 = t.y1;
+        const t3d_vertex* v = t.v;
+        for (int y = t.y0; y <= t.y1; y++)
+          for (int x = t.x0; x <= t.x1; x++) {
+            double l[3];
+            if (!tri_cover(&t, x, y, l)) continue;
+            const float d = interp(l, v[0].color_r, v[1].color_r, v[2].color_r);
+            const uint8_t* ex = depth + 4 * ((size_t)y * W + x);
+            if (!(ex[3] == 0 || decode_depth(ex) > d)) continue;
+            float enc[4];
+            encode_depth(d, enc);
+            uint8_t* ip = inter + 4 * ((size_t)y * W + x);
+            if (clip) {                                             /* camera.go:441-478 */
+              float tx = interp(l, v[0].src_x, v[1].src_x, v[2].src_x), ty = interp(l, v[0].src_y, v[1].src_y, v[2].src_y);
+              if (pc) {
+                const float r = 1.0f / interp(l, v[0].custom0, v[1].custom0, v[2].custom0);
+                tx *= r;
+                ty *= r;
+              }
+              tx = fractf_(tx / (float)img->w) * (float)img->w;
+              ty = fractf_(ty / (float)img->h) * (float)img->h;
+              float tex[4];
+              sample(img, linear, tx, ty, tex);
+              if (tex[3] <= 0.8f) continue;
+              enc[3] = tex[3];
+              source_over(ip, enc);
+            } else {
+              source_over(ip, enc);
+            }
+          }
+      }
+      if (!is_transparent(model, mat))                              /* resultDepthTexture.DrawImage(intermediate) */
+        for (int y = by0; y <= by1; y++)
+          for (int x = bx0; x <= bx1; x++) {
+            const uint8_t* ip = inter + 4 * ((size_t)y * W + x);
+            const float src[4] = {(float)ip[0] / 255.0f, (float)ip[1] / 255.0f, (float)ip[2] / 255.0f, (float)ip[3] / 255.0f};
+            source_over(depth + 4 * ((size_t)y * W + x), src);
+          }
+      /* colour pass through base3d.kage */
+      for (uint32_t k = 0; k < nt; k++) {
+        otri t;
+        if (!tri_setup(&t, cv + 3 * k, W, H)) continue;
+        const t3d_vertex* v = t.v;
+        for (int y = t.y0; y <= t.y1; y++)
+          for (int x = t.x0; x <= t.x1; x++) {
+            double l[3];
+            if (!tri_cover(&t, x, y, l)) continue;
+            const uint8_t* ip = inter + 4 * ((size_t)y * W + x);
+            if (ip[3] == 0) continue;                               /* discard() */
+            const float vc[4] = {interp(l, v[0].color_r, v[1].color_r, v[2].color_r),
+                                 interp(l, v[0].color_g, v[1].color_g, v[2].color_g),
+                                 interp(l, v[0].color_b, v[1].color_b, v[2].color_b),
+                                 interp(l, v[0].color_a, v[1].color_a, v[2].color_a)};
+            float o[4];
+            base3d(img, linear, &fg, fogless, pc, ip, vc, interp(l, v[0].src_x, v[1].src_x, v[2].src_x),
+                   interp(l, v[0].src_y, v[1].src_y, v[2].src_y), interp(l, v[0].custom0, v[1].custom0, v[2].custom0),
+                   (float)x + 0.5f, (float)y + 0.5f, o);
+            source_over(color + 4 * ((size_t)y * W + x), o);
+          }
+      }
+      for (int y = by0; y <= by1; y++) memset(inter + 4 * ((size_t)y * W + bx0), 0, bx1 >= bx0 ? 4ull * (bx1 - bx0 + 1) : 0);
+    } else {
+      /* DrawTriangles(colorVertexList, img): texel times the vertex colour made premultiplied */
+      for (uint32_t k = 0; k < nt; k++) {
+        otri t;
+        if (!tri_setup(&t, cv + 3 * k, W, H)) continue;
+        const t3d_vertex* v = t.v;
+        for (int y = t.y0; y <= t.y1; y++)
+          for (int x = t.x0; x <= t.x1; x++) {
+            double l[3];
+            if (!tri_cover(&t, x, y, l)) continue;
+            const float a = interp(l, v[0].color_a, v[1].color_a, v[2].color_a);
+            const float vc[4] = {interp(l, v[0].color_r, v[1].color_r, v[2].color_r) * a,
+                                 interp(l, v[0].color_g, v[1].color_g, v[2].color_g) * a,
+                                 interp(l, v[0].color_b, v[1].color_b, v[2].color_b) * a, a};
+            float tx = modf_(interp(l, v[0].src_x, v[1].src_x, v[2].src_x), (float)img->w);
+            float ty = modf_(interp(l, v[0].src_y, v[1].src_y, v[2].src_y), (float)img->h);
+            float tex[4], o[4];
+            sample(img, linear, tx, ty, tex);
+            for (int q = 0; q < 4; q++) o[q] = tex[q] * vc[q];
+            source_over(color + 4 * ((size_t)y * W + x), o);
+          }
+      }
+    }
+  }
+  free(inter);
+  return rc;
 }

@@ -45,6 +45,10 @@ int t3do_animation_register(t3do_ctx* ctx, uint64_t handle, const t3d_anim_chann
 int t3do_render(t3do_ctx* ctx, const t3d_render_desc* desc, t3do_result* out);
 void t3do_result_free(t3do_result* out);
 const char* t3do_last_error(const t3do_ctx* ctx);
+/* Raster sink restated: a texture under a caller-chosen handle (RGBA8, premultiplied), and the flushes of a
+ * render drawn into W*H*4-byte colour and (RenderDepth) depth images. */
+int t3do_texture_register(t3do_ctx* ctx, uint64_t handle, uint32_t width, uint32_t height, const uint8_t* rgba8);
+int t3do_rasterize(t3do_ctx* ctx, const t3d_render_desc* desc, const t3do_result* res, uint8_t* color, uint8_t* depth);
 
 /* ---- unit-level entry points (used by the tests that pin the oracle) ---- */
 void t3do_mat4_mult(const t3d_mat4* a, const t3d_mat4* b, t3d_mat4* out);

@@ -1,0 +1,216 @@
+"""Pins the oracle's raster sink (flush + the Kage shaders, camera.go:350-531,
+1829-2048; shaders/base3d.kage) on scenes whose images follow in closed form:
+coverage (exact areas, shared edges covered once), the depth encoding, the
+depth test between flushes and its absence inside one, transparent pairs
+leaving the depth texture alone, fog and its dithering, texture sampling,
+AlphaClip, the plain draw without RenderDepth and the clear colour.  CPU only.
+
+The fixed-function rules (pixel centres, top-left ties, linear interpolation,
+8-bit rounding) are the oracle's own, stated in its source; the HIP raster
+kernels follow them to the bit."""
+import numpy as np
+import pytest
+
+from tetra3d_amd import _abi as A
+from tetra3d_amd.scene import Material, MeshPart, Scene, Texture
+from tests.test_oracle_pins import f32, model, ortho_camera, tri_mesh, world
+
+W, H = 640, 320   # the orthographic test camera: DstX = 160 x + 320, DstY = -160 y + 160
+
+
+def q8(x):
+    return int(np.floor(np.float32(min(max(float(x), 0.0), 1.0)) * np.float32(255) + np.float32(0.5)))
+
+
+def _raster(oracle, models, w=None, cam=None):
+    for m in {id(m.mesh): m.mesh for m in models}.values():
+        if id(m) not in oracle._handles:
+            oracle.register_mesh(m, 1000 + len(oracle._refs))
+    return oracle.rasterize(cam or ortho_camera(), Scene("s", w if w is not None else _world()), [], models)
+
+
+def _world(**kw):
+    w = world()
+    w.lighting_on = False
+    w.fog_on = False
+    w.clear_color = np.array([0, 0, 0, 0], dtype=f32)
+    for k, v in kw.items():
+        setattr(w, k, v)
+    return w
+
+
+def _flat(tris, z=-5.0, color=(1, 0, 0, 1), sort=A.SORT_NONE, texture=None, uvs=None):
+    """A shadeless model of screen-space triangles given in world x, y on the
+    orthographic camera (x in [-2, 2], y in [-1, 1]) at depth z."""
+    pos = np.array([(x, y, 0) for t in tris for (x, y) in t], dtype=f32)
+    m = model(tri_mesh(pos, uvs=uvs))
+    m.set_local_position(0, 0, z)
+    m.shadeless = True
+    mat = m.mesh.mesh_parts[0].material
+    mat.color = np.array(color, dtype=f32)
+    mat.triangle_sort_mode = sort
+    if texture is not None:
+        mat.texture = texture
+        mat.texture_size = texture.size
+    return m
+
+
+def _px(x, y):
+    """World (x, y) of screen point (x, y) on the orthographic test camera
+    (integers are pixel corners)."""
+    return ((x - 320) / 160.0, (160 - y) / 160.0)
+
+
+def _square(x0, y0, x1, y1):
+    a, b, c, d = _px(x0, y0), _px(x1, y0), _px(x1, y1), _px(x0, y1)
+    return [(a, b, c), (a, c, d)]
+
+
+def test_coverage_is_exact_on_pixel_corners(oracle):
+    """A 64 x 48 pixel rectangle from two triangles covers exactly 3,072
+    pixels, the diagonal once: with colour alpha 0.5 every covered pixel holds
+    one SourceOver of it, none two."""
+    m = _flat(_square(100, 40, 164, 88), color=(1, 1, 1, 0.5))
+    fr, col, dep = _raster(oracle, [m])
+    hit = col[..., 3] > 0
+    assert hit.sum() == 64 * 48
+    ys, xs = np.nonzero(hit)
+    assert (xs.min(), xs.max(), ys.min(), ys.max()) == (100, 163, 40, 87)
+    # colorTex.rgb *= color.a: 0.5 once; a second SourceOver would give 0.5 + 0.5 * 0.5
+    assert set(np.unique(col[hit][:, 0])) == {q8(0.5)} and set(np.unique(col[hit][:, 3])) == {q8(0.5)}
+
+
+def test_fan_covers_each_pixel_once(oracle):
+    """Eight triangles around a point off the pixel grid, corners off the grid
+    too: no pixel is blended twice and none inside the octagon is missed (the
+    top-left rule on shared edges)."""
+    cx, cy = _px(201.3, 97.7)
+    ring = [_px(201.3 + 61.1 * np.cos(k * np.pi / 4 + 0.3), 97.7 + 47.9 * np.sin(k * np.pi / 4 + 0.3)) for k in range(8)]
+    tris = [((cx, cy), ring[k], ring[(k + 1) % 8]) for k in range(8)]
+    m = _flat(tris, color=(1, 1, 1, 0.5))
+    fr, col, dep = _raster(oracle, [m])
+    hit = col[..., 3] > 0
+    assert set(np.unique(col[hit][:, 3])) == {q8(0.5)}
+    # every pixel whose centre is well inside the octagon (radius 47.9 minus a margin) is covered
+    yy, xx = np.mgrid[0:H, 0:W]
+    inside = (xx + 0.5 - 201.3) ** 2 / 61.1 ** 2 + (yy + 0.5 - 97.7) ** 2 / 47.9 ** 2 < 0.85
+    assert hit[inside].all()
+
+
+def test_depth_encoding_closed_form(oracle):
+    """encodeDepth (camera.go:354-359) of depth 0.15625 (Z = 1.25 over a
+    spread of 8): r = floor(39.84375) = 39, g = floor(0.84375 * 255) = 215,
+    b = fract(10160.15625) = 0.15625, stored as 40; alpha 255."""
+    m = _flat(_square(100, 40, 164, 88))
+    fr, col, dep = _raster(oracle, [m])
+    assert tuple(dep[60, 120]) == (39, 215, 40, 255)
+    assert tuple(dep[10, 10]) == (0, 0, 0, 0)
+    assert tuple(col[60, 120]) == (255, 0, 0, 255)
+
+
+def test_depth_test_between_flushes_not_within(oracle):
+    """Between flushes the depth texture decides (camera.go:367-376): a
+    farther model drawn after a nearer one is hidden, drawn before it is
+    overdrawn.  Inside one flush the fragments test the result depth only, so
+    the last triangle in draw order wins even when farther (sort mode None
+    keeps mesh order)."""
+    near = _flat(_square(100, 40, 164, 88), z=-3, color=(1, 0, 0, 1))
+    far = _flat(_square(120, 50, 184, 98), z=-6, color=(0, 1, 0, 1))
+    _, col, dep = _raster(oracle, [near, far])
+    assert tuple(col[60, 130]) == (255, 0, 0, 255)            # overlap: the nearer stays
+    assert tuple(col[95, 180]) == (0, 255, 0, 255)
+    _, col2, _ = _raster(oracle, [far, near])
+    assert tuple(col2[60, 130]) == (255, 0, 0, 255)           # drawn later and nearer: overdraws
+    # one flush, one triangle drawn twice: at world z -4 (depth 1/8) then -6 (depth 3/16), sort mode None
+    tri = _square(100, 40, 164, 88)[0]
+    both = _flat([tri, tri], z=-4, color=(0, 0, 1, 1))
+    both.mesh.positions[3:, 2] = -2.0
+    _, _, dep = _raster(oracle, [both])
+    assert tuple(dep[45, 150]) == (47, 207, 48, 255)           # encodeDepth(0.1875): the later, farther one
+    swapped = _flat([tri, tri], z=-4, color=(0, 0, 1, 1))
+    swapped.mesh.positions[:3, 2] = -2.0
+    _, _, dep = _raster(oracle, [swapped])
+    assert tuple(dep[45, 150]) == (31, 223, 32, 255)           # encodeDepth(0.125)
+
+
+def test_transparent_flush_leaves_depth_alone(oracle):
+    """A transparent pair draws its depth into the intermediate only
+    (camera.go:1915-1917): the result depth keeps the opaque model's value,
+    and the colour is one SourceOver of the transparent colour over it."""
+    solid = _flat(_square(100, 40, 164, 88), z=-6, color=(0, 1, 0, 1))
+    glass = _flat(_square(100, 40, 164, 88), z=-3, color=(1, 0, 0, 0.5))
+    _, col, dep = _raster(oracle, [glass, solid])
+    _, _, dep_solid = _raster(oracle, [solid])
+    assert np.array_equal(dep, dep_solid)
+    r = np.float32(1) * np.float32(0.5)         # colorTex.rgb *= color.a, premultiplied
+    g = np.float32(0) + np.float32(1) * (np.float32(1) - np.float32(0.5))
+    assert tuple(col[60, 120]) == (q8(r), q8(g), 0, q8(np.float32(0.5) + np.float32(1) * np.float32(0.5)))
+
+
+def test_fog_linear_and_dithered(oracle):
+    """base3d's fog (base3d.kage:96-145): d = smoothstep(range, decodeDepth)
+    added as Fog.rgb * d * alpha; with DitheredFogSize a Bayer cell switches it
+    fully on or off."""
+    m = _flat(_square(96, 40, 160, 88), color=(0.5, 0.5, 0.5, 1))
+    w = _world(fog_on=True, fog_mode=A.FOG_ADD, fog_color=np.array([0.25, 0.5, 1, 1], dtype=f32),
+               fog_range=np.array([0.0, 0.5], dtype=f32))
+    _, col, dep = _raster(oracle, [m], w=w)
+    dd = np.float32(39) / np.float32(255) + (np.float32(215) / np.float32(255)) / np.float32(255) + \
+        (np.float32(40) / np.float32(255)) / np.float32(65025)
+    t = np.float32(min(max((dd - np.float32(0)) / (np.float32(0.5) - np.float32(0)), 0), 1))
+    d = t * t * (np.float32(3) - np.float32(2) * t)
+    for k, fc in enumerate((0.25, 0.5, 1.0)):
+        assert col[60, 120, k] == q8(np.float32(0.5) + np.float32(fc) * d * np.float32(1))
+    w.dithered_fog_size = 1.0
+    _, col, _ = _raster(oracle, [m], w=w)
+    bayer = np.array([1, 9, 3, 11, 13, 5, 15, 7, 4, 12, 2, 10, 16, 8, 14, 6], dtype=np.float32) / np.float32(17)
+    for (y, x) in ((40, 96), (41, 97), (43, 99), (50, 101)):
+        on = (d - bayer[(y % 4) * 4 + (x % 4)]) >= 0
+        assert col[y, x, 0] == q8(np.float32(0.5) + np.float32(0.25) * np.float32(1.0 if on else 0.0))
+
+
+def _checker():
+    """2 x 2 texture: red, green / blue, half-transparent white (premultiplied)."""
+    return Texture(np.array([[[255, 0, 0, 255], [0, 255, 0, 255]], [[0, 0, 255, 255], [100, 100, 100, 100]]],
+                            dtype=np.uint8))
+
+
+def test_texture_nearest_and_alpha_clip(oracle):
+    """A quad with UVs 0..1 shows the four texels in its quadrants (nearest:
+    the texel containing SrcX, SrcY; V runs up, SrcY = (1 - V) * h).  With
+    AlphaClip the texel of alpha 100/255 <= 0.8 is cut from the depth pass,
+    so the colour pass discards it too and the depth stays clear there."""
+    tex = _checker()
+    sq = _square(100, 40, 164, 104)
+    uvs = np.array([(0, 1), (1, 1), (1, 0), (0, 1), (1, 0), (0, 0)], dtype=f32)
+    m = _flat(sq, texture=tex, uvs=uvs, color=(1, 1, 1, 1))
+    _, col, dep = _raster(oracle, [m])
+    assert tuple(col[50, 110]) == (255, 0, 0, 255)
+    assert tuple(col[50, 150]) == (0, 255, 0, 255)
+    assert tuple(col[90, 110]) == (0, 0, 255, 255)
+    assert tuple(col[90, 150]) == (100, 100, 100, 100)
+    m.mesh.mesh_parts[0].material.transparency_mode = A.TRANSPARENCY_ALPHA_CLIP
+    _, col, dep = _raster(oracle, [m])
+    assert tuple(col[90, 150]) == (0, 0, 0, 0) and dep[90, 150, 3] == 0
+    assert tuple(col[90, 110]) == (0, 0, 255, 255) and dep[90, 110, 3] == 255
+
+
+def test_plain_draw_without_render_depth(oracle):
+    """Without RenderDepth a flush is one DrawTriangles: texel times the
+    vertex colour made premultiplied, no depth texture."""
+    cam = ortho_camera()
+    cam.render_depth = False
+    m = _flat(_square(100, 40, 164, 88), color=(1, 0.5, 0.25, 0.5))
+    _, col, dep = _raster(oracle, [m], cam=cam)
+    assert dep is None
+    a = np.float32(0.5)
+    assert tuple(col[60, 120]) == (q8(np.float32(1) * a), q8(np.float32(0.5) * a), q8(np.float32(0.25) * a), q8(a))
+
+
+def test_clear_colour_premultiplied(oracle):
+    """Camera.Clear fills with World.ClearColor (camera.go:1002-1013), stored
+    premultiplied."""
+    w = _world()
+    w.clear_color = np.array([0.5, 1.0, 0.25, 0.5], dtype=f32)
+    _, col, _ = _raster(oracle, [], w=w)
+    assert tuple(col[0, 0]) == (q8(np.float32(0.25)), q8(np.float32(0.5)), q8(np.float32(0.125)), q8(0.5))

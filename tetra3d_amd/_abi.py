@@ -37,6 +37,8 @@ BILLBOARD_UP_GLOBAL_Y, BILLBOARD_UP_CAMERA_Y = 0, 1
 DEPTH_MODE_DEFAULT, DEPTH_MODE_UNBILLBOARDED = 0, 1
 # FogBlendMode (world.go:8-11)
 FOG_ADD, FOG_SUB, FOG_OVERWRITE, FOG_TRANSPARENT = 0, 1, 2, 3
+FOG_CURVE_LINEAR, FOG_CURVE_OUT_CIRC, FOG_CURVE_IN_CIRC = 0, 1, 2
+FILTER_NEAREST, FILTER_LINEAR = 0, 1
 # light types
 LIGHT_AMBIENT, LIGHT_POINT, LIGHT_DIRECTIONAL, LIGHT_VOLUME = 0, 1, 2, 3
 # execution forms
@@ -104,6 +106,9 @@ class MaterialDesc(C.Structure):
         ("depth_mode", C.c_int32),
         ("billboard_up", C.c_int32),
         ("billboard_lock", C.c_int32 * 3),
+        ("texture", C.c_uint64),
+        ("texture_filter", C.c_int32),
+        ("pad_texture", C.c_int32),
     ]
 
 
@@ -218,6 +223,9 @@ class WorldDesc(C.Structure):
         ("ambient_color", F3),
         ("ambient_energy", C.c_float),
         ("ambient_visible", C.c_int32),
+        ("clear_color", F4),
+        ("fog_curve", C.c_int32),
+        ("dithered_fog_size", C.c_float),
     ]
 
 

@@ -1015,6 +1015,8 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   f->flush_begin.clear();
   f->job_flush_begin.assign(1, 0);
   f->groups.clear();
+  f->rflush.clear();
+  f->rflush_tex.clear();
   f->grouped = false;   // some model of the frame is dynamically batched: flush records differ from pairs
   for (uint32_t job = 0; job < ndesc && !f->grouped; job++)
     for (uint32_t mi = 0; descs[job] && mi < descs[job]->model_count && !f->grouped; mi++)
@@ -1435,10 +1437,55 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
       return T3D_OK;
     };
 
+    // what the raster sink's draw of a flush reads (camera.go:1838-1878, 1914-1917)
+    auto raster_of = [&](const t3d_model_desc& md, const t3d_material_desc* mat) {
+      RasterFlush r{};
+      r.tex = nullptr;   // the default image
+      r.tw = r.th = 1;
+      if (mat && mat->use_texture && mat->texture) {
+        auto it = c->textures.find(mat->texture);
+        if (it == c->textures.end()) r.flags |= RF_NO_TEXTURE;
+        else {
+          r.tex = reinterpret_cast<const uint8_t*>(it->second->buf.ptr);
+          r.tw = it->second->w;
+          r.th = it->second->h;
+        }
+      }
+      if (mat && mat->texture_filter == T3D_FILTER_LINEAR) r.flags |= RF_LINEAR;
+      if (mat && mat->transparency_mode == T3D_TRANSPARENCY_ALPHA_CLIP) r.flags |= RF_CLIP;
+      if (mat && mat->fogless) r.flags |= RF_FOGLESS;
+      if (!is_transparent(md, mat)) r.flags |= RF_COMPOSITE;
+      if (mat && !mat->blend_source_over) r.flags |= RF_BAD_BLEND;
+      return r;
+    };
+    if (job == 0) {
+      RasterFrame& rf = f->rframe;
+      rf = RasterFrame{};
+      rf.fog_range[1] = 1.0f;
+      if (world) {
+        for (int k = 0; k < 3; k++) rf.fog[k] = world->fog_color[k];
+        rf.fog[3] = world->fog_on ? (float)world->fog_mode : -1.0f;
+        rf.fog_range[0] = world->fog_range[0];
+        rf.fog_range[1] = world->fog_range[1];
+        rf.dither = world->dithered_fog_size;
+        rf.curve = world->fog_curve;
+        for (int k = 0; k < 4; k++) rf.clear[k] = world->clear_color[k];
+      }
+      rf.width = cd->width;
+      rf.height = cd->height;
+      rf.render_depth = cd->render_depth ? 1 : 0;
+      rf.pc_tex = cd->perspective_corrected_texture_mapping ? 1 : 0;
+    }
+
     for (int32_t di : draw) {
       const Pair& rp = pairs[di];
       const t3d_model_desc& md = desc->models[rp.model];
       if (!md.visible || md.auto_batch_static) continue;               // camera.go:2067-2071
+      {
+        const t3d_material_desc* rm = part_material(md, rp.part);
+        f->rflush.push_back(raster_of(md, rm));
+        f->rflush_tex.push_back(rm && rm->use_texture ? rm->texture : 0);
+      }
       if (rp.batch < 0) {
         const uint32_t first = (uint32_t)f->pairs.size();
         t3d_status est = emit_pair(rp.model, rp.part, *rp.mesh);

@@ -205,6 +205,39 @@ struct PackArgs {
   float4* pnormal;
 };
 
+// Raster sink (t3d_raster.hip): the state a flush's draw reads (camera.go:1829-2048).
+enum : uint32_t {
+  RF_LINEAR = 1u << 0,      // TextureFilterMode linear (base3d.kage bilinearFilter)
+  RF_CLIP = 1u << 1,        // TransparencyModeAlphaClip: the clip-alpha depth shader (camera.go:394-478)
+  RF_FOGLESS = 1u << 2,     // Material.Fogless
+  RF_COMPOSITE = 1u << 3,   // not transparent: its depth intermediate is drawn over the result depth
+  RF_BAD_BLEND = 1u << 4,   // a blend other than SourceOver: the raster sink refuses the frame
+  RF_NO_TEXTURE = 1u << 5,  // the material names a texture handle the context does not hold
+};
+struct RasterFlush {
+  const uint8_t* tex;       // RGBA8 premultiplied, or the default 1x1 white image (tetra3d.go:12, 38)
+  uint32_t tw, th;
+  uint32_t flags;
+  uint32_t pad;
+};
+// Camera and world uniforms of the draws (camera.go:1002-1013, 1956-1977; world.go:96-110).
+struct RasterFrame {
+  float fog[4];             // FogColor RGB, FogMode (-1 when the world's fog is off)
+  float fog_range[2];
+  float dither;             // DitheredFogSize
+  int32_t curve;            // FogCurve
+  float clear[4];           // ClearColor, straight alpha (transparent black without a world)
+  int32_t width, height;
+  int32_t render_depth, pc_tex;
+};
+// Per render pair, for the raster kernels: its first drawn triangle among all
+// the frame's, where its corners are, and its flush.
+struct RasterPair {
+  uint32_t tri0;            // prefix of survivors over the render pairs before it
+  uint32_t flush;
+  uint64_t vertex;          // first output vertex of its survivors
+};
+
 // Kernel launchers (t3d_kernels.hip).  blk0 offsets blockIdx so the plain
 // form can launch one part's blocks at a time.
 hipError_t launch_palettes(const FrameArgs& a, hipStream_t s);

@@ -61,6 +61,12 @@ struct Animation {
   int refs = 0;                     // submitted jobs referencing the animation
 };
 
+struct Texture {
+  uint64_t handle = 0;
+  DevBuf buf;                       // RGBA8, premultiplied, row-major
+  uint32_t w = 0, h = 0;
+};
+
 }  // namespace t3d
 
 struct t3d_frame {
@@ -100,6 +106,16 @@ struct t3d_frame {
   std::vector<uint32_t> job_flush_begin;
   std::vector<uint2> groups;
   const std::vector<t3d_pair_result>& out_pairs() const { return grouped ? flush : pairs; }
+  // raster sink: per flush the state its draw reads (material texture, filter, transparency), per frame the
+  // camera and world uniforms, and the images and working buffers of t3d_frame_rasterize
+  std::vector<t3d::RasterFlush> rflush;
+  t3d::RasterFrame rframe{};
+  std::vector<uint64_t> rflush_tex;   // texture handle of each flush (0: the default image), resolved again at draw time
+  t3d::DevBuf raster_buf, raster_buf2, images;
+  uint64_t raster_cap = 0, raster_cap2 = 0, images_cap = 0;
+  uint8_t* image_color = nullptr;   // device images of the last rasterize (null before)
+  uint8_t* image_depth = nullptr;
+  int32_t image_w = 0, image_h = 0;
   uint32_t job_out_begin(uint32_t k) const { return grouped ? job_flush_begin[k] : job_pair_begin[k]; }
   uint64_t table_bytes = 0;
   uint64_t total_triangles = 0;
@@ -130,6 +146,7 @@ struct t3d_ctx {
   std::map<uint64_t, std::unique_ptr<t3d::Mesh>> meshes;
   std::map<uint64_t, std::unique_ptr<t3d::Volume>> volumes;
   std::map<uint64_t, std::unique_ptr<t3d::Animation>> animations;
+  std::map<uint64_t, std::unique_ptr<t3d::Texture>> textures;
   std::string err;
   t3d_tuning tuning{256, 1, 2048, 4, 1, 2, 8, 1, 0, 256};
   // Guards resources, the budget and err between callers and the serving
@@ -162,7 +179,8 @@ void dev_free(t3d_ctx* c, DevBuf& b);
 void frame_release(t3d_ctx* c, t3d_frame* f);
 // Device bytes a frame holds between renders (tables, scratch, outputs, download packing).
 inline uint64_t frame_capacity(const t3d_frame* f) {
-  return f->tables.bytes + f->scratch.bytes + f->outputs.bytes + f->pack.bytes;
+  return f->tables.bytes + f->scratch.bytes + f->outputs.bytes + f->pack.bytes + f->raster_buf.bytes +
+         f->raster_buf2.bytes + f->images.bytes;
 }
 
 // Every refusal of a Camera.Render descriptor that does not depend on the frustum test (caller holds ctx->mu);

@@ -71,6 +71,11 @@ def lib():
     L.t3d_frame_download_triangles.argtypes = [vp, vp, P(C.c_uint32)]
     L.t3d_frame_device_normals.argtypes = [vp, P(vp)]
     L.t3d_frame_download_normals.argtypes = [vp, vp, P(A.Vertex)]
+    L.t3d_texture_upload.argtypes = [vp, C.c_uint32, C.c_uint32, P(C.c_uint8), P(C.c_uint64)]
+    L.t3d_texture_free.argtypes = [vp, C.c_uint64]
+    L.t3d_frame_rasterize.argtypes = [vp, vp]
+    L.t3d_frame_device_images.argtypes = [vp, P(vp), P(vp)]
+    L.t3d_frame_download_images.argtypes = [vp, vp, P(C.c_uint8), P(C.c_uint8)]
     L.t3d_frame_last_ms.argtypes = [vp]
     L.t3d_frame_last_ms.restype = C.c_double
     L.t3d_profile_enable.argtypes = [vp, C.c_int32]
@@ -88,7 +93,8 @@ def lib():
                  "t3d_profile_enable", "t3d_profile_read", "t3d_get_tuning", "t3d_set_tuning", "t3d_autotune",
                  "t3d_selftest_fp", "t3d_selftest_copy", "t3d_debug_model_spheres", "t3d_animation_upload",
                  "t3d_animation_free", "t3d_render_noalloc", "t3d_frame_device_normals",
-                 "t3d_frame_download_normals"):
+                 "t3d_frame_download_normals", "t3d_texture_upload", "t3d_texture_free", "t3d_frame_rasterize",
+                 "t3d_frame_device_images", "t3d_frame_download_images"):
         getattr(L, name).restype = C.c_int32
     _lib = L
     return L
@@ -132,6 +138,8 @@ class Context:
         self._extra_frames = []
         self._mesh_handles = {}   # id(mesh) -> handle
         self._mesh_refs = {}      # id(mesh) -> mesh: keeps ids from being reused while uploaded
+        self._tex_refs = {}       # id(texture) -> texture, uploaded
+        self._image_size = {}     # frame -> (width, height, render depth) of its last render
 
     def _check(self, rc):
         if rc != 0:
@@ -210,6 +218,7 @@ class Context:
     def prepare(self, workload):
         for m in workload.meshes:
             self.upload_mesh(m)
+        self.upload_textures(workload.models)
         for v in workload.volumes:
             if not v.handle:
                 self.upload_volume(v)
@@ -246,6 +255,42 @@ class Context:
     def render_inputs(self, inp):
         """Run one Camera.Render on the device; returns nothing (outputs stay in HBM)."""
         self._check(self.L.t3d_render(self.ctx, self.frame, C.byref(inp.render)))
+        self._image_size[id(self.frame)] = (inp.cam.width, inp.cam.height, bool(inp.cam.render_depth))
+
+    # -- raster sink --------------------------------------------------------
+    def upload_texture(self, tex):
+        """Upload a Texture (t3d_texture_upload); its handle goes into material descriptors."""
+        if id(tex) in self._tex_refs:
+            return tex.handle
+        h, w = tex.rgba.shape[:2]
+        out = C.c_uint64()
+        self._check(self.L.t3d_texture_upload(self.ctx, w, h, tex.rgba.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                              C.byref(out)))
+        tex.handle = out.value
+        self._tex_refs[id(tex)] = tex
+        return tex.handle
+
+    def upload_textures(self, models):
+        for m in with_batched(models):
+            for p in (m.mesh.mesh_parts if m.mesh is not None else []):
+                if p.material is not None and p.material.texture is not None:
+                    self.upload_texture(p.material.texture)
+
+    def rasterize(self, frame=None):
+        """Draw the frame's last render into its colour and depth images on the device."""
+        self._check(self.L.t3d_frame_rasterize(self.ctx, frame or self.frame))
+
+    def download_images(self, frame=None):
+        """The rasterized images: colour (H, W, 4) uint8 premultiplied RGBA, and depth
+        likewise or None without RenderDepth."""
+        frame = frame or self.frame
+        w, h, rdepth = self._image_size[id(frame)]
+        color = np.zeros((h, w, 4), dtype=np.uint8)
+        depth = np.zeros((h, w, 4), dtype=np.uint8) if rdepth else None
+        U8 = C.POINTER(C.c_uint8)
+        self._check(self.L.t3d_frame_download_images(self.ctx, frame, color.ctypes.data_as(U8),
+                                                     depth.ctypes.data_as(U8) if depth is not None else None))
+        return color, depth
 
     def render_noalloc(self, inp, color, depth, indices, capacity, frame=None):
         """Camera.Render into caller-owned device arrays (device addresses as
@@ -316,6 +361,7 @@ class Context:
         for m in with_batched(models):
             if m.mesh is not None:
                 self.upload_mesh(m.mesh)
+        self.upload_textures(models)
         for l in lights:
             if getattr(l, "TYPE", None) == A.LIGHT_VOLUME and not l.handle:
                 self.upload_volume(l)

@@ -145,6 +145,8 @@ class Material:
         self.render_order = 0
         self.visible = True
         self.texture_size = None  # (w, h) when a texture is set and UseTexture is on
+        self.texture = None       # a Texture, read by the raster sink (its handle goes in the descriptor)
+        self.texture_filter = A.FILTER_NEAREST
         self.billboard = False
         self.billboard_up = A.BILLBOARD_UP_GLOBAL_Y
         self.billboard_lock = (False, False, False)
@@ -168,12 +170,30 @@ class Material:
         if self.texture_size is not None:
             d.use_texture = 1
             d.texture_size = A.F2(float(self.texture_size[0]), float(self.texture_size[1]))
+        if self.texture is not None:
+            d.texture = self.texture.handle
+        d.texture_filter = int(self.texture_filter)
         d.billboard = int(self.billboard)
         d.custom_depth_function = int(self.custom_depth_function)
         d.depth_mode = self.depth_mode
         d.billboard_up = self.billboard_up
         d.billboard_lock = (C.c_int32 * 3)(*[int(bool(x)) for x in self.billboard_lock])
         return d
+
+
+class Texture:
+    """An ebiten.Image used as a material texture: (h, w, 4) uint8 RGBA with
+    premultiplied alpha, as ebiten stores images.  `handle` is set when it is
+    uploaded (t3d_texture_upload) and names it in material descriptors."""
+
+    def __init__(self, rgba):
+        self.rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+        assert self.rgba.ndim == 3 and self.rgba.shape[2] == 4
+        self.handle = 0
+
+    @property
+    def size(self):
+        return (self.rgba.shape[1], self.rgba.shape[0])
 
 
 class MeshPart:
@@ -477,6 +497,9 @@ class World:
         self.fog_color = np.zeros(4, dtype=f32)
         self.fog_range = np.array([0, 1], dtype=f32)
         self.ambient_light = AmbientLight("ambient light", 1, 1, 1, 0)
+        self.clear_color = np.array([0.08, 0.09, 0.1, 1], dtype=f32)   # NewWorld's ClearColor (world.go:56)
+        self.fog_curve = A.FOG_CURVE_LINEAR
+        self.dithered_fog_size = 0.0
 
     def desc(self):
         d = A.WorldDesc()
@@ -490,6 +513,9 @@ class World:
         d.ambient_color = A.F3(*[float(x) for x in a.color])
         d.ambient_energy = float(a.energy)
         d.ambient_visible = int(a.visible)
+        d.clear_color = A.F4(*[float(x) for x in self.clear_color])
+        d.fog_curve = int(self.fog_curve)
+        d.dithered_fog_size = float(self.dithered_fog_size)
         return d
 
 
