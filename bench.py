@@ -456,6 +456,30 @@ def main():
     d2h50, d2h99 = float(np.percentile(per_d2h, 50)), float(np.percentile(per_d2h, 99))
     log("batched, synchronous + D2H to pinned host: p50 %.3f p99 %.3f ms" % (d2h50, d2h99))
 
+    # the raster sink: the frame drawn into the camera's textures on the device (t3d_frame_rasterize) and the
+    # colour image copied to pinned host memory, what a host that shows the frame waits for
+    img_bytes = 4 * wl.camera.width * wl.camera.height
+    pimg = C.c_void_p()
+    if hip.hipHostMalloc(C.byref(pimg), C.c_size_t(img_bytes), C.c_uint(0)) != 0:
+        raise RuntimeError("hipHostMalloc of %d bytes failed" % img_bytes)
+    per_img, per_rast = [], []
+    try:
+        for _ in range(args.steps):
+            a = time.perf_counter()
+            ctx.render_inputs(inp)
+            b = time.perf_counter()
+            ctx.rasterize()
+            c = time.perf_counter()
+            ctx._check(ctx.L.t3d_frame_download_images(ctx.ctx, ctx.frame, C.cast(pimg, C.POINTER(C.c_uint8)), None))
+            per_img.append((time.perf_counter() - a) * 1e3)
+            per_rast.append((c - b) * 1e3)
+    finally:
+        hip.hipHostFree(pimg)
+    img50, img99 = float(np.percentile(per_img, 50)), float(np.percentile(per_img, 99))
+    rast50 = float(np.percentile(per_rast, 50))
+    log("batched, synchronous + raster sink + colour image to pinned host: p50 %.3f p99 %.3f ms (rasterize %.3f ms)"
+        % (img50, img99, rast50))
+
     # throughput (the timed region): K frames submitted asynchronously into
     # two alternating frames, so frame k's host prologue overlaps frame k-1's
     # kernels; every frame's outputs are complete in HBM before the clock stops
@@ -569,7 +593,11 @@ def main():
                        "note": "synchronous frames: submit -> outputs complete in HBM",
                        "with_d2h_p50": d2h50, "with_d2h_p99": d2h99,
                        "with_d2h_note": "submit -> outputs copied to pinned host memory (t3d_frame_download: "
-                                        "survivors packed on the device, one copy per stream)"},
+                                        "survivors packed on the device, one copy per stream)",
+                       "raster_image_p50": img50, "raster_image_p99": img99, "rasterize_p50": rast50,
+                       "raster_image_note": "submit -> flushes drawn into the camera's colour and depth textures on "
+                                            "the device (t3d_frame_rasterize) -> colour image (%d bytes) copied to "
+                                            "pinned host memory" % img_bytes},
         "pipelined": None if args.sync_only else {"ms_per_step": ms, "frames_in_flight": 2},
         "hbm": {"algorithmic_bytes_per_frame": B, "frame_pct_of_peak": 100.0 * B / (ms / 1e3) / (HBM_PEAK_GBS * 1e9),
                 "device_ms_per_frame": device_ms, "host_prologue_ms": host_ms,

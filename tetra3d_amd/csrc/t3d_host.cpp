@@ -2034,7 +2034,13 @@ void frame_release(t3d_ctx* c, t3d_frame* f) {
   dev_free(c, f->scratch);
   dev_free(c, f->outputs);
   dev_free(c, f->pack);
+  dev_free(c, f->raster_buf);
+  dev_free(c, f->raster_buf2);
+  dev_free(c, f->raster_buf3);
+  dev_free(c, f->images);
   f->tables_cap = f->scratch_cap = f->outputs_cap = f->pack_cap = 0;
+  f->raster_cap = f->raster_cap2 = f->raster_cap3 = f->images_cap = 0;
+  f->image_color = f->image_depth = nullptr;
   if (f->pinned) (void)hipHostFree(f->pinned);
   if (f->pinned_stats) (void)hipHostFree(f->pinned_stats);
   f->pinned = nullptr;

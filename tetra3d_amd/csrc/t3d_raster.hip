@@ -18,13 +18,21 @@
 //  - triangles of one draw go in index order; blending is SourceOver,
 //    src + dst * (1 - src.a), float32 per channel.
 //
-// Work split: every drawn triangle is binned by its pixel bounding box into
-// 16 x 16 tiles (count, exclusive scan, emit, stable radix sort by tile), so a
-// tile's list is in draw order.  One 256-thread block per tile then walks its
-// list, one thread per pixel, the pixel's colour, depth and depth-intermediate
-// values held in registers from the clear to the final store: per flush
-// segment a depth pass, the intermediate composited over the depth (opaque
-// pairs), then the colour pass through base3d.
+// Work split (every step a parallel kernel): a setup pass writes one
+// 160-byte record per drawn triangle (corners, the attributes the draws read,
+// area, pixel box, flush) and the area of its pixel box; a scan of the areas
+// numbers every box pixel ("slot"), and a table gives each 256 slots the
+// triangles of their first and last; one thread per slot tests coverage, a
+// wavefront's answers one 64-bit mask; a scan of the mask counts gives each
+// covered slot its fragment, written as (pixel, triangle and place in its
+// box); a stable radix sort by pixel leaves each pixel's fragments contiguous
+// and in draw order, where they are shaded (interpolation, texture, clip
+// alpha, base3d's colour before fog) into 32-byte fragments; one thread per
+// pixel replays them flush by flush: the depth pass into the intermediate,
+// the composite for opaque pairs, the colour pass with fog.
+// Every fragment of a frame (about 21 million on mixed-512 at 1920 x 1080,
+// ten per pixel) is shaded with full SIMD lanes, and no pixel waits on
+// another's triangles.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -37,8 +45,9 @@
 namespace t3d {
 namespace {
 
-constexpr int kTileEdge = 16;     // pixels per tile edge: one 256-thread block per tile
-constexpr uint32_t kStage = 64;   // triangles staged in LDS per round
+struct STri;
+
+struct Frag;
 
 struct RArgs {
   const RasterPair* pairs;        // render pairs with survivors, in draw order
@@ -48,19 +57,39 @@ struct RArgs {
   const float4* color;            // frame outputs, t3d_vertex as 3 float4
   const float4* depth;
   RasterFrame rf;
-  uint32_t tiles_x, tiles_y;
-  uint64_t* counts;               // [ntri + 1] tiles each triangle's box touches
+  STri* recs;                     // [ntri] the drawn triangles' records
+  uint64_t* counts;               // [ntri + 1] pixels of each triangle's box (its fragment slots)
   const uint64_t* offsets;        // [ntri + 1] exclusive scan of counts
-  uint32_t* keys;                 // [total] tile of each (triangle, tile) entry
-  uint32_t* vals;                 // [total] its triangle
-  const uint32_t* skeys;          // the entries sorted by tile, stable
-  const uint32_t* svals;
-  uint32_t* tile_begin;           // [tiles] range of each tile in the sorted entries
-  uint32_t* tile_end;
-  uint32_t total;
+  uint64_t nslots;
+  uint2* btri;                    // [nslot blocks] triangles of each 256 slots' first and last slot
+  uint64_t* masks;                // [nwaves] per 64 slots: which cover their pixel
+  uint32_t* wcount;               // [nwaves + 1] covered slots of each 64 (and a closing 0)
+  const uint32_t* wofs;           // [nwaves + 1] exclusive scan of wcount: each 64's first fragment
+  uint32_t nwaves;
+  uint32_t nfrags;
+  uint32_t* keys;                 // [nfrags] pixel of each fragment, draw order
+  uint64_t* vals;                 // [nfrags] its triangle (high word) and place in the triangle's box
+  const uint32_t* skeys;          // both sorted by pixel (stable: draw order within a pixel)
+  const uint64_t* svals;
+  Frag* sfrags;                   // [nfrags] the fragments shaded in that order
+  uint32_t* pix_begin;            // [W * H] range of each pixel in the sorted fragments
+  uint32_t* pix_end;
+  uint32_t* seg_start;            // [nfrags + 1] 1 where a (pixel, flush) run of the sorted fragments starts
+  const uint32_t* seg_id;         // [nfrags + 1] exclusive scan of seg_start: the run of each fragment (+1 at a start)
+  uint32_t* seg_end;              // [runs] one past the last fragment of each run
   uint32_t* img_color;            // RGBA8 packed, r in the low byte
   uint32_t* img_depth;
 };
+
+// One covered fragment, shaded: what the per-pixel resolve needs.
+struct alignas(16) Frag {
+  uint32_t flush;
+  float d;          // the depth list's interpolated ColorR
+  float clip_a;     // AlphaClip: the texel alpha (< 0: discarded by the clip shader); otherwise 1
+  uint32_t pad;
+  float col[4];     // base3d's colorTex before fog (RenderDepth), or the plain draw's source colour
+};
+
 
 // ---- the arithmetic of the draw (float32 shaders, double coverage) ----------
 
@@ -137,24 +166,28 @@ __device__ __forceinline__ bool top_left(float ax, float ay, float bx, float by,
   return (dy == 0.0 && dx > 0.0) || dy < 0.0;
 }
 
-// A triangle as staged: corner positions and the attributes the draws read.
-struct STri {
+// A drawn triangle's record (written once by k_rsetup, 160 bytes): corner
+// positions and the attributes the draws read.
+struct alignas(16) STri {
   float x[3], y[3];          // DstX, DstY
   float sx[3], sy[3], c0[3]; // SrcX, SrcY, Custom0 (the same on both lists)
   float r[3], g[3], b[3], a[3];
   float dz[3];               // the depth list's ColorR
-  double area, s;
+  double area;               // orientation: its sign
   int bx0, bx1, by0, by1;    // pixel box on the image; bx0 > bx1: nothing to draw
   uint32_t flush;
+  uint32_t pad[3];
 };
+static_assert(sizeof(STri) == 160, "raster record layout");
 
 // Area, orientation and the box of pixels whose centres the corners span.
 __device__ __forceinline__ bool setup(STri& t, int W, int H) {
   t.area = edge_fn(t.x[0], t.y[0], t.x[1], t.y[1], (double)t.x[2], (double)t.y[2]);
   t.bx0 = 1;
   t.bx1 = 0;
+  t.by0 = 1;
+  t.by1 = 0;
   if (!(t.area != 0.0) || t.area != t.area) return false;
-  t.s = t.area > 0 ? 1.0 : -1.0;
   double mnx = t.x[0], mxx = t.x[0], mny = t.y[0], mxy = t.y[0];
   for (int k = 1; k < 3; k++) {
     mnx = t.x[k] < mnx ? t.x[k] : mnx;
@@ -171,9 +204,21 @@ __device__ __forceinline__ bool setup(STri& t, int W, int H) {
   return true;
 }
 
+// coverage alone (no barycentric weights): the same tests as cover()
+__device__ __forceinline__ bool cover_test(const STri& t, int x, int y) {
+  if (x < t.bx0 || x > t.bx1 || y < t.by0 || y > t.by1) return false;
+  const double cx = x + 0.5, cy = y + 0.5, s = t.area > 0 ? 1.0 : -1.0;
+  const double e12 = edge_fn(t.x[1], t.y[1], t.x[2], t.y[2], cx, cy);
+  if (!(s * e12 > 0 || (e12 == 0 && top_left(t.x[1], t.y[1], t.x[2], t.y[2], s)))) return false;
+  const double e20 = edge_fn(t.x[2], t.y[2], t.x[0], t.y[0], cx, cy);
+  if (!(s * e20 > 0 || (e20 == 0 && top_left(t.x[2], t.y[2], t.x[0], t.y[0], s)))) return false;
+  const double e01 = edge_fn(t.x[0], t.y[0], t.x[1], t.y[1], cx, cy);
+  return s * e01 > 0 || (e01 == 0 && top_left(t.x[0], t.y[0], t.x[1], t.y[1], s));
+}
+
 __device__ __forceinline__ bool cover(const STri& t, int x, int y, double l[3]) {
   if (x < t.bx0 || x > t.bx1 || y < t.by0 || y > t.by1) return false;
-  const double cx = x + 0.5, cy = y + 0.5, s = t.s;
+  const double cx = x + 0.5, cy = y + 0.5, s = t.area > 0 ? 1.0 : -1.0;
   const double e12 = edge_fn(t.x[1], t.y[1], t.x[2], t.y[2], cx, cy);
   const double e20 = edge_fn(t.x[2], t.y[2], t.x[0], t.y[0], cx, cy);
   const double e01 = edge_fn(t.x[0], t.y[0], t.x[1], t.y[1], cx, cy);
@@ -221,49 +266,13 @@ __device__ void load_tri(const RArgs& a, uint32_t t, STri& s) {
   s.flush = rp.flush;
 }
 
-// ---- binning --------------------------------------------------------------------
+// ---- shading ---------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_rcount(RArgs a) {
-  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
-  if (t > a.ntri) return;
-  uint64_t n = 0;
-  if (t < a.ntri) {
-    STri s;
-    load_tri(a, t, s);
-    if (setup(s, a.rf.width, a.rf.height))
-      n = (uint64_t)(s.bx1 / kTileEdge - s.bx0 / kTileEdge + 1) * (uint64_t)(s.by1 / kTileEdge - s.by0 / kTileEdge + 1);
-  }
-  a.counts[t] = n;   // counts[ntri] = 0 closes the scan
-}
-
-__global__ __launch_bounds__(256) void k_remit(RArgs a) {
-  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= a.ntri || a.counts[t] == 0) return;
-  STri s;
-  load_tri(a, t, s);
-  if (!setup(s, a.rf.width, a.rf.height)) return;
-  uint64_t o = a.offsets[t];
-  for (int ty = s.by0 / kTileEdge; ty <= s.by1 / kTileEdge; ty++)
-    for (int tx = s.bx0 / kTileEdge; tx <= s.bx1 / kTileEdge; tx++) {
-      a.keys[o] = (uint32_t)ty * a.tiles_x + (uint32_t)tx;
-      a.vals[o] = t;
-      o++;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_rranges(RArgs a) {
-  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= a.total) return;
-  const uint32_t key = a.skeys[k];
-  if (k == 0 || a.skeys[k - 1] != key) a.tile_begin[key] = k;
-  if (k + 1 == a.total || a.skeys[k + 1] != key) a.tile_end[key] = k + 1;
-}
-
-// ---- drawing ---------------------------------------------------------------------
-
-// base3d.kage Fragment (61-153) at a pixel whose depth intermediate has alpha > 0
-__device__ void base3d(const RasterFlush& fl, const RasterFrame& rf, bool fogless, uint32_t inter, const float vc[4],
-                       float sx, float sy, float c0, float px, float py, float out[4]) {
+// base3d.kage Fragment (61-153), split in two: the texture and vertex colour
+// (colorTex after colorTex.rgb *= color.a), computed with the fragment, and the
+// fog, which reads the flush's final depth intermediate at the pixel
+__device__ void base3d_color(const RasterFlush& fl, const RasterFrame& rf, const float vc[4], float sx, float sy,
+                             float c0, float out[4]) {
   float tx = sx, ty = sy;
   if (rf.pc_tex > 0) {                                              // tx *= 1.0 / custom.x
     const float r = 1.0f / c0;
@@ -276,7 +285,9 @@ __device__ void base3d(const RasterFlush& fl, const RasterFrame& rf, bool fogles
   sample(fl, tx, ty, tex);
   for (int k = 0; k < 4; k++) out[k] = tex[k] * vc[k];
   for (int k = 0; k < 3; k++) out[k] *= vc[3];                      // colorTex.rgb *= color.a
-  if (fogless) return;
+}
+
+__device__ void base3d_fog(const RasterFrame& rf, uint32_t inter, float px, float py, float out[4]) {
   const float dd = decode_depth(inter);
   float v = dd;
   bool curve_ok = true;
@@ -306,127 +317,256 @@ __device__ void base3d(const RasterFlush& fl, const RasterFrame& rf, bool fogles
   }
 }
 
-// Stages the sorted entries [p, p + n) of this tile; returns, through s_lead,
-// how many of them from the first belong to flush `f` (f == ~0u: the first's).
-__device__ void stage(const RArgs& a, uint32_t p, uint32_t n, STri* s_tri, uint32_t* s_lead) {
-  const int W = a.rf.width, H = a.rf.height;
-  if (threadIdx.x < n) {
-    STri& s = s_tri[threadIdx.x];
-    load_tri(a, a.svals[p + threadIdx.x], s);
-    setup(s, W, H);
+// ---- fragments -------------------------------------------------------------------
+
+// per drawn triangle: its record and the number of pixels of its box (its fragment slots)
+__global__ __launch_bounds__(256) void k_rsetup(RArgs a) {
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  if (t > a.ntri) return;
+  uint64_t n = 0;
+  if (t < a.ntri) {
+    STri s;
+    load_tri(a, t, s);
+    if (setup(s, a.rf.width, a.rf.height)) n = (uint64_t)(s.bx1 - s.bx0 + 1) * (uint64_t)(s.by1 - s.by0 + 1);
+    s.pad[0] = s.pad[1] = s.pad[2] = 0;
+    const float4* src = reinterpret_cast<const float4*>(&s);
+    float4* dst = reinterpret_cast<float4*>(a.recs + t);
+    for (int k = 0; k < 10; k++) dst[k] = src[k];
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t m = 0;
-    while (m < n && s_tri[m].flush == s_tri[0].flush) m++;
-    *s_lead = m;
-  }
-  __syncthreads();
+  a.counts[t] = n;   // counts[ntri] = 0 closes the scan
 }
 
-__global__ __launch_bounds__(256) void k_raster(RArgs a) {
-  __shared__ STri s_tri[kStage];
-  __shared__ uint32_t s_lead;
-  const uint32_t tile = blockIdx.x;
-  const int W = a.rf.width, H = a.rf.height;
-  const int x = (int)(tile % a.tiles_x) * kTileEdge + (int)(threadIdx.x % kTileEdge);
-  const int y = (int)(tile / a.tiles_x) * kTileEdge + (int)(threadIdx.x / kTileEdge);
-  const bool valid = x < W && y < H;
-  const size_t pix = valid ? (size_t)y * W + x : 0;
-  uint32_t col = valid ? a.img_color[pix] : 0u;
-  uint32_t dep = (valid && a.rf.render_depth) ? a.img_depth[pix] : 0u;
-  const float fx = (float)x + 0.5f, fy = (float)y + 0.5f;   // dstPos
-  const uint32_t b = a.tile_begin[tile], e = a.tile_end[tile];
-  uint32_t pos = b;
-  while (pos < e) {                                         // block-uniform: one flush segment per round
-    uint32_t n = min(kStage, e - pos);
-    stage(a, pos, n, s_tri, &s_lead);
-    const uint32_t f = s_tri[0].flush;
-    const RasterFlush fl = a.flushes[f];
-    if (!a.rf.render_depth) {
-      // DrawTriangles(colorVertexList, img): texel times the premultiplied vertex colour
-      const uint32_t m = s_lead;
-      for (uint32_t i = 0; i < m && valid; i++) {
-        const STri& t = s_tri[i];
-        double l[3];
-        if (!cover(t, x, y, l)) continue;
-        const float al = interp(l, t.a);
-        const float vc[4] = {interp(l, t.r) * al, interp(l, t.g) * al, interp(l, t.b) * al, al};
-        const float tx = modf_(interp(l, t.sx), (float)fl.tw), ty = modf_(interp(l, t.sy), (float)fl.th);
-        float tex[4], o[4];
-        sample(fl, tx, ty, tex);
-        for (int q = 0; q < 4; q++) o[q] = tex[q] * vc[q];
-        col = source_over(col, o);
+// The triangle owning fragment slot `slot` among [lo, hi] (offsets: exclusive
+// scan of the box areas; triangles without pixels own no slot).
+__device__ __forceinline__ uint32_t tri_of_slot(const RArgs& a, uint64_t slot, uint32_t lo, uint32_t hi) {
+  if (hi < lo) hi = lo;
+  hi += 1;   // largest t in [lo, hi) with offsets[t] <= slot
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (a.offsets[mid] <= slot) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// a slot's pixel: boxes hold fewer than 2^31 pixels, so 32-bit division
+__device__ __forceinline__ void slot_pixel(const STri& t, uint64_t local, int& x, int& y) {
+  const uint32_t bw = (uint32_t)(t.bx1 - t.bx0 + 1), l = (uint32_t)local;
+  const uint32_t row = l / bw;
+  x = t.bx0 + (int)(l - row * bw);
+  y = t.by0 + (int)row;
+}
+
+// one thread per 256 slots: the triangles of the first and last, so that the
+// slot kernels' blocks start from them (thousands of independent searches
+// instead of a search at the head of every block)
+__global__ __launch_bounds__(256) void k_rblocks(RArgs a, uint32_t nblocks) {
+  const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= nblocks) return;
+  const uint64_t last = b * 256ull + 255 < a.nslots ? b * 256ull + 255 : a.nslots - 1;
+  const uint64_t first = b * 256ull < last ? b * 256ull : last;   // both real slots, first <= last
+  const uint32_t t0 = tri_of_slot(a, first, 0, a.ntri - 1);
+  a.btri[b] = make_uint2(t0, tri_of_slot(a, last, t0, a.ntri - 1));
+}
+
+// A block's 256 slots: the triangles of its first and last slot bound the
+// search of every thread's triangle, which runs over those triangles' offsets
+// staged in LDS (or in memory when triangles without pixels stretch the
+// range past 256).
+__device__ __forceinline__ uint32_t block_tri(const RArgs& a, uint64_t slot) {
+  __shared__ uint64_t s_off[256];
+  const uint2 bt = a.btri[blockIdx.x];
+  const uint32_t t0 = bt.x, t1 = bt.y;
+  const uint64_t last = blockIdx.x * 256ull + 255 < a.nslots ? blockIdx.x * 256ull + 255 : a.nslots - 1;
+  const uint64_t q = slot < a.nslots ? slot : last;
+  if (t1 - t0 >= 256) return tri_of_slot(a, q, t0, t1);      // block-uniform
+  s_off[threadIdx.x] = t0 + threadIdx.x <= t1 ? a.offsets[t0 + threadIdx.x] : ~0ull;
+  __syncthreads();
+  uint32_t lo = 0, hi = t1 - t0 + 1;                         // largest j in [lo, hi) with s_off[j] <= q
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (s_off[mid] <= q) lo = mid;
+    else hi = mid;
+  }
+  return t0 + lo;
+}
+
+// one thread per slot: is its pixel covered? Each wavefront's 64 answers
+// are one mask word and its count.
+__global__ __launch_bounds__(256) void k_rcover(RArgs a) {
+  const uint64_t slot = blockIdx.x * 256ull + threadIdx.x;
+  const uint32_t t = block_tri(a, slot);
+  bool covered = false;
+  if (slot < a.nslots) {
+    const STri& r = a.recs[t];
+    int x, y;
+    slot_pixel(r, slot - a.offsets[t], x, y);
+    covered = cover_test(r, x, y);
+  }
+  const uint64_t m = __ballot(covered);
+  const uint32_t w = (uint32_t)(slot >> 6);   // < nwaves: the grid is nwaves / 4 blocks
+  if ((threadIdx.x & 63) == 0) {
+    a.masks[w] = m;
+    a.wcount[w] = (uint32_t)__popcll(m);
+  }
+  if (slot == 0) a.wcount[a.nwaves] = 0;   // closes the scan
+}
+
+// each fragment, at its place from the scan: its pixel (the sort key) and its
+// triangle and place in the triangle's box (the value)
+__global__ __launch_bounds__(256) void k_rslots(RArgs a) {
+  const uint64_t slot = blockIdx.x * 256ull + threadIdx.x;
+  const uint32_t t = block_tri(a, slot);
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t m = a.masks[slot >> 6];
+  if (slot >= a.nslots || !((m >> lane) & 1)) return;
+  const uint32_t i = a.wofs[slot >> 6] + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+  const uint64_t local = slot - a.offsets[t];
+  int x, y;
+  slot_pixel(a.recs[t], local, x, y);
+  a.keys[i] = (uint32_t)y * (uint32_t)a.rf.width + (uint32_t)x;
+  a.vals[i] = ((uint64_t)t << 32) | local;
+}
+
+// each fragment shaded, in pixel order: what the per-pixel resolve needs
+__global__ __launch_bounds__(256) void k_rfrag(RArgs a) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.nfrags) return;
+  const uint64_t v = a.svals[i];
+  const STri& r = a.recs[(uint32_t)(v >> 32)];
+  int x, y;
+  slot_pixel(r, v & 0xFFFFFFFFull, x, y);
+  double l[3];
+  cover(r, x, y, l);
+  const RasterFlush fl = a.flushes[r.flush];
+  Frag fr;
+  fr.flush = r.flush;
+  fr.pad = 0;
+  fr.d = interp(l, r.dz);
+  fr.clip_a = 1.0f;
+  if (a.rf.render_depth) {
+    if (fl.flags & RF_CLIP) {                               // the clip-alpha depth shader (camera.go:441-478)
+      float tx = interp(l, r.sx), ty = interp(l, r.sy);
+      if (a.rf.pc_tex) {
+        const float rc = 1.0f / interp(l, r.c0);
+        tx *= rc;
+        ty *= rc;
       }
-      __syncthreads();
-      pos += m;
-      continue;
+      tx = fractf_(tx / (float)fl.tw) * (float)fl.tw;
+      ty = fractf_(ty / (float)fl.th) * (float)fl.th;
+      float tex[4];
+      sample(fl, tx, ty, tex);
+      fr.clip_a = tex[3] <= 0.8f ? -1.0f : tex[3];
     }
-    // depth pass over the segment, chunk by chunk, into this pixel's intermediate
+    const float vc[4] = {interp(l, r.r), interp(l, r.g), interp(l, r.b), interp(l, r.a)};
+    base3d_color(fl, a.rf, vc, interp(l, r.sx), interp(l, r.sy), interp(l, r.c0), fr.col);
+  } else {                                                  // DrawTriangles: texel times the premultiplied colour
+    const float al = interp(l, r.a);
+    const float vc[4] = {interp(l, r.r) * al, interp(l, r.g) * al, interp(l, r.b) * al, al};
+    const float tx = modf_(interp(l, r.sx), (float)fl.tw), ty = modf_(interp(l, r.sy), (float)fl.th);
+    float tex[4];
+    sample(fl, tx, ty, tex);
+    for (int q = 0; q < 4; q++) fr.col[q] = tex[q] * vc[q];
+  }
+  a.sfrags[i] = fr;
+}
+
+// runs of one pixel's fragments from one flush: where each starts ...
+__global__ __launch_bounds__(256) void k_rseg_start(RArgs a) {
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k > a.nfrags) return;
+  a.seg_start[k] = (k < a.nfrags && (k == 0 || a.skeys[k] != a.skeys[k - 1] ||
+                                     a.sfrags[k].flush != a.sfrags[k - 1].flush)) ? 1u : 0u;
+}
+// ... and where it ends (seg_id[k] counts the starts before k: the run of fragment k is seg_id[k + 1] - 1)
+__global__ __launch_bounds__(256) void k_rseg_end(RArgs a) {
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= a.nfrags) return;
+  if (k + 1 == a.nfrags || a.seg_start[k + 1]) a.seg_end[a.seg_id[k + 1] - 1] = k + 1;
+}
+
+__global__ __launch_bounds__(256) void k_rranges(RArgs a) {
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= a.nfrags) return;
+  const uint32_t key = a.skeys[k];
+  if (k == 0 || a.skeys[k - 1] != key) a.pix_begin[key] = k;
+  if (k + 1 == a.nfrags || a.skeys[k + 1] != key) a.pix_end[key] = k + 1;
+}
+
+// One thread per pixel: its fragments in draw order, flush by flush as flush
+// draws them: the depth pass into the intermediate (tested against the
+// result depth), the composite for opaque pairs, then the colour pass (fog
+// from the intermediate) blended over the colour.  A blend of alpha 1
+// replaces what is under it (src + dst * 0), so each pass starts at the last
+// fragment that does so: the last passing depth fragment (the plain depth
+// shader writes alpha 1) and the last colour fragment of alpha 1 — the same
+// bits as blending every fragment in order.
+__global__ __launch_bounds__(256) void k_resolve(RArgs a) {
+  const uint32_t pix = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t W = (uint32_t)a.rf.width, H = (uint32_t)a.rf.height;
+  if (pix >= W * H) return;
+  const float px = (float)(pix % W) + 0.5f, py = (float)(pix / W) + 0.5f;   // dstPos
+  const Frag* fs = a.sfrags;
+  uint32_t col = a.img_color[pix];
+  const uint32_t b = a.pix_begin[pix], e = a.pix_end[pix];
+  if (!a.rf.render_depth) {
+    uint32_t k0 = b;
+    for (uint32_t k = e; k > b; k--)
+      if (fs[k - 1].col[3] == 1.0f) {
+        k0 = k - 1;
+        break;
+      }
+    for (uint32_t k = k0; k < e; k++) col = source_over(col, fs[k].col);
+    a.img_color[pix] = col;
+    return;
+  }
+  uint32_t dep = a.img_depth[pix];
+  for (uint32_t k = b, run = b < e ? a.seg_id[b + 1] - 1 : 0; k < e; run++) {
+    const uint32_t f = fs[k].flush;
+    const uint32_t k1 = a.seg_end[run];
+    const RasterFlush fl = a.flushes[f];
+    const bool empty_dep = ((dep >> 24) & 255u) == 0;
+    const float dd = empty_dep ? 0.0f : decode_depth(dep);
     uint32_t inter = 0u;
-    const uint32_t seg0 = pos;
-    uint32_t p = pos, m = s_lead;
-    for (;;) {
-      for (uint32_t i = 0; i < m && valid; i++) {
-        const STri& t = s_tri[i];
-        double l[3];
-        if (!cover(t, x, y, l)) continue;
-        const float d = interp(l, t.dz);
-        if (!(((dep >> 24) & 255u) == 0 || decode_depth(dep) > d)) continue;
+    if (fl.flags & RF_CLIP) {                               // texel alphas below 1: every passing fragment blends
+      for (uint32_t q = k; q < k1; q++) {
+        if (fs[q].clip_a < 0.0f) continue;                  // discard(): texel alpha <= 0.8
+        if (!(empty_dep || dd > fs[q].d)) continue;
         float enc[4];
-        encode_depth(d, enc);
-        if (fl.flags & RF_CLIP) {                           // the clip-alpha depth shader (camera.go:441-478)
-          float tx = interp(l, t.sx), ty = interp(l, t.sy);
-          if (a.rf.pc_tex) {
-            const float r = 1.0f / interp(l, t.c0);
-            tx *= r;
-            ty *= r;
-          }
-          tx = fractf_(tx / (float)fl.tw) * (float)fl.tw;
-          ty = fractf_(ty / (float)fl.th) * (float)fl.th;
-          float tex[4];
-          sample(fl, tx, ty, tex);
-          if (tex[3] <= 0.8f) continue;
-          enc[3] = tex[3];
-        }
+        encode_depth(fs[q].d, enc);
+        enc[3] = fs[q].clip_a;
         inter = source_over(inter, enc);
       }
-      __syncthreads();
-      p += m;
-      if (m < n || p >= e) break;                           // the segment ends in this chunk
-      n = min(kStage, e - p);
-      stage(a, p, n, s_tri, &s_lead);
-      m = s_tri[0].flush == f ? s_lead : 0u;
-      if (m == 0) break;
+    } else {
+      for (uint32_t q = k1; q > k; q--)
+        if (empty_dep || dd > fs[q - 1].d) {
+          float enc[4];
+          encode_depth(fs[q - 1].d, enc);
+          inter = source_over(0u, enc);
+          break;
+        }
     }
-    const uint32_t seg1 = p;
     if (fl.flags & RF_COMPOSITE) {                          // resultDepthTexture.DrawImage(depthIntermediate)
       const float src[4] = {chan(inter, 0), chan(inter, 1), chan(inter, 2), chan(inter, 3)};
       dep = source_over(dep, src);
     }
-    // colour pass through base3d over the same segment
-    const bool fogless = (fl.flags & RF_FOGLESS) != 0;
-    for (p = seg0; p < seg1;) {
-      n = min(kStage, seg1 - p);
-      stage(a, p, n, s_tri, &s_lead);
-      for (uint32_t i = 0; i < n && valid; i++) {
-        const STri& t = s_tri[i];
-        double l[3];
-        if (((inter >> 24) & 255u) == 0 || !cover(t, x, y, l)) continue;   // discard()
-        const float vc[4] = {interp(l, t.r), interp(l, t.g), interp(l, t.b), interp(l, t.a)};
-        float o[4];
-        base3d(fl, a.rf, fogless, inter, vc, interp(l, t.sx), interp(l, t.sy), interp(l, t.c0), fx, fy, o);
+    if (((inter >> 24) & 255u) != 0) {                     // colour pass; discard() where the intermediate is empty
+      uint32_t j0 = k;
+      for (uint32_t q = k1; q > k; q--)
+        if (fs[q - 1].col[3] == 1.0f) {
+          j0 = q - 1;
+          break;
+        }
+      for (uint32_t q = j0; q < k1; q++) {
+        float o[4] = {fs[q].col[0], fs[q].col[1], fs[q].col[2], fs[q].col[3]};
+        if (!(fl.flags & RF_FOGLESS)) base3d_fog(a.rf, inter, px, py, o);
         col = source_over(col, o);
       }
-      __syncthreads();
-      p += n;
     }
-    pos = seg1;
+    k = k1;
   }
-  if (valid) {
-    a.img_color[pix] = col;
-    if (a.rf.render_depth) a.img_depth[pix] = dep;
-  }
+  a.img_color[pix] = col;
+  a.img_depth[pix] = dep;
 }
 
 uint32_t host_q8(float x) {   // the device's q8, on the host for the clear words
@@ -493,8 +633,6 @@ t3d_status t3d_frame_rasterize(t3d_ctx* c, t3d_frame* f) {
       return fail(c, T3D_ERR_NOT_FOUND, "rasterize: flush %zu names an unknown texture", g);
   }
   const uint32_t W = (uint32_t)rf.width, H = (uint32_t)rf.height;
-  const uint32_t tiles_x = (W + kTileEdge - 1) / kTileEdge, tiles_y = (H + kTileEdge - 1) / kTileEdge;
-  const uint32_t ntiles = tiles_x * tiles_y;
   // textures are resolved again: one freed since the render is refused rather than read
   std::vector<RasterFlush> rfl = f->rflush;
   for (size_t g = 0; g < rfl.size(); g++) {
@@ -540,14 +678,13 @@ t3d_status t3d_frame_rasterize(t3d_ctx* c, t3d_frame* f) {
     R_TRY(hipStreamSynchronize(st));
     return T3D_OK;
   }
-  // working memory, phase 1: tables, counts, offsets, scan storage
+  // phase 1: tables, triangle records, box areas and their scan
   size_t scan_tmp = 0;
   R_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_tmp, (const uint64_t*)nullptr, (uint64_t*)nullptr, ntri + 1, st));
   auto al = [](uint64_t x) { return (x + 255) / 256 * 256; };
   const uint64_t b_pairs = al(sizeof(RasterPair) * rp.size()), b_fl = al(sizeof(RasterFlush) * rfl.size());
-  const uint64_t b_cnt = al(8ull * (ntri + 1)), b_tiles = al(4ull * ntiles);
-  const uint64_t phase1 = b_pairs + b_fl + 2 * b_cnt + 2 * b_tiles + al(scan_tmp);
-  s = ensure(c, f->raster_buf, f->raster_cap, phase1);
+  const uint64_t b_cnt = al(8ull * (ntri + 1)), b_recs = al(sizeof(STri) * (uint64_t)ntri);
+  s = ensure(c, f->raster_buf, f->raster_cap, b_pairs + b_fl + 2 * b_cnt + b_recs + al(scan_tmp));
   if (s) return s;
   uint8_t* w = reinterpret_cast<uint8_t*>(f->raster_buf.ptr);
   RArgs a{};
@@ -558,54 +695,102 @@ t3d_status t3d_frame_rasterize(t3d_ctx* c, t3d_frame* f) {
   a.color = f->args.out_color;
   a.depth = f->args.out_depth;
   a.rf = rf;
-  a.tiles_x = tiles_x;
-  a.tiles_y = tiles_y;
   a.counts = reinterpret_cast<uint64_t*>(w + b_pairs + b_fl);
   uint64_t* offsets = reinterpret_cast<uint64_t*>(w + b_pairs + b_fl + b_cnt);
   a.offsets = offsets;
-  a.tile_begin = reinterpret_cast<uint32_t*>(w + b_pairs + b_fl + 2 * b_cnt);
-  a.tile_end = reinterpret_cast<uint32_t*>(w + b_pairs + b_fl + 2 * b_cnt + b_tiles);
-  void* scan_mem = w + b_pairs + b_fl + 2 * b_cnt + 2 * b_tiles;
+  a.recs = reinterpret_cast<STri*>(w + b_pairs + b_fl + 2 * b_cnt);
+  void* scan_mem = w + b_pairs + b_fl + 2 * b_cnt + b_recs;
   a.img_color = reinterpret_cast<uint32_t*>(f->image_color);
   a.img_depth = reinterpret_cast<uint32_t*>(f->image_depth);
   R_TRY(hipMemcpyAsync((void*)a.pairs, rp.data(), sizeof(RasterPair) * rp.size(), hipMemcpyHostToDevice, st));
   R_TRY(hipMemcpyAsync((void*)a.flushes, rfl.data(), sizeof(RasterFlush) * rfl.size(), hipMemcpyHostToDevice, st));
-  R_TRY(launch_fill(a.tile_begin, 2 * b_tiles, 0u, st));   // tile_begin and tile_end: empty tiles
   const uint32_t tblocks = (ntri + 1 + 255) / 256;
-  hipLaunchKernelGGL(k_rcount, dim3(tblocks), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_rsetup, dim3(tblocks), dim3(256), 0, st, a);
   R_TRY(hipGetLastError());
   R_TRY(hipcub::DeviceScan::ExclusiveSum(scan_mem, scan_tmp, a.counts, offsets, ntri + 1, st));
-  uint64_t total64 = 0;
-  R_TRY(hipMemcpyAsync(&total64, offsets + ntri, 8, hipMemcpyDeviceToHost, st));
+  uint64_t nslots = 0;
+  R_TRY(hipMemcpyAsync(&nslots, offsets + ntri, 8, hipMemcpyDeviceToHost, st));
   R_TRY(hipStreamSynchronize(st));
-  if (total64 >= 0x80000000ull) return fail(c, T3D_ERR_UNSUPPORTED, "rasterize: %llu tile entries", (unsigned long long)total64);
-  const uint32_t total = (uint32_t)total64;
-  a.total = total;
-  if (total) {
-    // phase 2: entries and their sort
-    uint32_t bits = 1;
-    while ((1u << bits) < ntiles) bits++;
-    size_t sort_tmp = 0;
-    R_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_tmp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                             (const uint32_t*)nullptr, (uint32_t*)nullptr, total, 0, (int)bits, st));
-    const uint64_t b_e = al(4ull * total);
-    s = ensure(c, f->raster_buf2, f->raster_cap2, 4 * b_e + al(sort_tmp));
+  if (nslots >= 0x7FFFFFFFull)
+    return fail(c, T3D_ERR_UNSUPPORTED, "rasterize: %llu box pixels (at most 2^31 - 1)", (unsigned long long)nslots);
+  a.nslots = nslots;
+  if (nslots) {
+    // phase 2: coverage masks, 64 slots a word, and the scan of their counts: each covered slot's fragment
+    const uint32_t sblocks = (uint32_t)((nslots + 255) / 256);
+    a.nwaves = 4 * sblocks;
+    size_t fscan_tmp = 0;
+    R_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, fscan_tmp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                           (int)(a.nwaves + 1), st));
+    const uint64_t b_masks = al(8ull * a.nwaves), b_wc = al(4ull * (a.nwaves + 1));
+    s = ensure(c, f->raster_buf2, f->raster_cap2, b_masks + 2 * b_wc + al(fscan_tmp) + al(8ull * sblocks));
     if (s) return s;
     uint8_t* w2 = reinterpret_cast<uint8_t*>(f->raster_buf2.ptr);
-    a.keys = reinterpret_cast<uint32_t*>(w2);
-    a.vals = reinterpret_cast<uint32_t*>(w2 + b_e);
-    uint32_t* sk = reinterpret_cast<uint32_t*>(w2 + 2 * b_e);
-    uint32_t* sv = reinterpret_cast<uint32_t*>(w2 + 3 * b_e);
-    a.skeys = sk;
-    a.svals = sv;
-    hipLaunchKernelGGL(k_remit, dim3(tblocks), dim3(256), 0, st, a);
+    a.masks = reinterpret_cast<uint64_t*>(w2);
+    a.wcount = reinterpret_cast<uint32_t*>(w2 + b_masks);
+    uint32_t* wofs = reinterpret_cast<uint32_t*>(w2 + b_masks + b_wc);
+    a.wofs = wofs;
+    a.btri = reinterpret_cast<uint2*>(w2 + b_masks + 2 * b_wc + al(fscan_tmp));
+    hipLaunchKernelGGL(k_rblocks, dim3((sblocks + 255) / 256), dim3(256), 0, st, a, sblocks);
     R_TRY(hipGetLastError());
-    R_TRY(hipcub::DeviceRadixSort::SortPairs(w2 + 4 * b_e, sort_tmp, a.keys, sk, a.vals, sv, total, 0, (int)bits, st));
-    hipLaunchKernelGGL(k_rranges, dim3((total + 255) / 256), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_rcover, dim3(sblocks), dim3(256), 0, st, a);
     R_TRY(hipGetLastError());
+    R_TRY(hipcub::DeviceScan::ExclusiveSum(w2 + b_masks + 2 * b_wc, fscan_tmp, a.wcount, wofs, (int)(a.nwaves + 1),
+                                           st));
+    uint32_t nfrags = 0;
+    R_TRY(hipMemcpyAsync(&nfrags, wofs + a.nwaves, 4, hipMemcpyDeviceToHost, st));
+    R_TRY(hipStreamSynchronize(st));
+    a.nfrags = nfrags;
+    if (nfrags > 0) {
+      // phase 3: shading, the sort by pixel, the fragments in that order, each pixel's range, the resolve
+      uint32_t bits = 1;
+      while ((1ull << bits) < (uint64_t)W * H) bits++;
+      size_t sort_tmp = 0;
+      R_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_tmp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                               (const uint64_t*)nullptr, (uint64_t*)nullptr, (int)nfrags, 0, (int)bits,
+                                               st));
+      const uint64_t b_fr = al(sizeof(Frag) * (uint64_t)nfrags), b_k = al(4ull * nfrags), b_pix = al(4ull * W * H);
+      size_t seg_tmp = 0;
+      R_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, seg_tmp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                             (int)nfrags + 1, st));
+      const uint64_t b_k1 = al(4ull * (nfrags + 1));
+      const uint64_t tail = std::max<uint64_t>(al(sort_tmp), al(seg_tmp));   // the two passes share it
+      s = ensure(c, f->raster_buf3, f->raster_cap3, b_fr + 6 * b_k + 2 * b_pix + 3 * b_k1 + tail);
+      if (s) return s;
+      uint8_t* w3 = reinterpret_cast<uint8_t*>(f->raster_buf3.ptr);
+      uint8_t* q = w3;
+      auto take = [&](uint64_t n) { uint8_t* r = q; q += n; return r; };
+      a.sfrags = reinterpret_cast<Frag*>(take(b_fr));
+      a.keys = reinterpret_cast<uint32_t*>(take(b_k));
+      uint32_t* sk = reinterpret_cast<uint32_t*>(take(b_k));
+      a.vals = reinterpret_cast<uint64_t*>(take(2 * b_k));
+      uint64_t* sv = reinterpret_cast<uint64_t*>(take(2 * b_k));
+      a.skeys = sk;
+      a.svals = sv;
+      a.pix_begin = reinterpret_cast<uint32_t*>(take(b_pix));
+      a.pix_end = reinterpret_cast<uint32_t*>(take(b_pix));
+      a.seg_start = reinterpret_cast<uint32_t*>(take(b_k1));
+      uint32_t* seg_id = reinterpret_cast<uint32_t*>(take(b_k1));
+      a.seg_id = seg_id;
+      a.seg_end = reinterpret_cast<uint32_t*>(take(b_k1));
+      uint8_t* tmp3 = take(tail);
+      const uint32_t fblocks = (nfrags + 255) / 256;
+      hipLaunchKernelGGL(k_rslots, dim3(sblocks), dim3(256), 0, st, a);
+      R_TRY(hipGetLastError());
+      R_TRY(hipcub::DeviceRadixSort::SortPairs(tmp3, sort_tmp, a.keys, sk, a.vals, sv, (int)nfrags, 0, (int)bits, st));
+      hipLaunchKernelGGL(k_rfrag, dim3(fblocks), dim3(256), 0, st, a);
+      R_TRY(hipGetLastError());
+      hipLaunchKernelGGL(k_rseg_start, dim3((nfrags + 1 + 255) / 256), dim3(256), 0, st, a);
+      R_TRY(hipGetLastError());
+      R_TRY(hipcub::DeviceScan::ExclusiveSum(tmp3, seg_tmp, a.seg_start, seg_id, (int)nfrags + 1, st));
+      hipLaunchKernelGGL(k_rseg_end, dim3(fblocks), dim3(256), 0, st, a);
+      R_TRY(hipGetLastError());
+      R_TRY(launch_fill(a.pix_begin, 2 * b_pix, 0u, st));   // pix_begin and pix_end: no fragments
+      hipLaunchKernelGGL(k_rranges, dim3(fblocks), dim3(256), 0, st, a);
+      R_TRY(hipGetLastError());
+      hipLaunchKernelGGL(k_resolve, dim3((W * H + 255) / 256), dim3(256), 0, st, a);
+      R_TRY(hipGetLastError());
+    }
   }
-  hipLaunchKernelGGL(k_raster, dim3(ntiles), dim3(256), 0, st, a);
-  R_TRY(hipGetLastError());
   R_TRY(hipStreamSynchronize(st));
   return T3D_OK;
 }

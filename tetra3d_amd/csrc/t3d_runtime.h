@@ -111,8 +111,8 @@ struct t3d_frame {
   std::vector<t3d::RasterFlush> rflush;
   t3d::RasterFrame rframe{};
   std::vector<uint64_t> rflush_tex;   // texture handle of each flush (0: the default image), resolved again at draw time
-  t3d::DevBuf raster_buf, raster_buf2, images;
-  uint64_t raster_cap = 0, raster_cap2 = 0, images_cap = 0;
+  t3d::DevBuf raster_buf, raster_buf2, raster_buf3, images;
+  uint64_t raster_cap = 0, raster_cap2 = 0, raster_cap3 = 0, images_cap = 0;
   uint8_t* image_color = nullptr;   // device images of the last rasterize (null before)
   uint8_t* image_depth = nullptr;
   int32_t image_w = 0, image_h = 0;
@@ -180,7 +180,7 @@ void frame_release(t3d_ctx* c, t3d_frame* f);
 // Device bytes a frame holds between renders (tables, scratch, outputs, download packing).
 inline uint64_t frame_capacity(const t3d_frame* f) {
   return f->tables.bytes + f->scratch.bytes + f->outputs.bytes + f->pack.bytes + f->raster_buf.bytes +
-         f->raster_buf2.bytes + f->images.bytes;
+         f->raster_buf2.bytes + f->raster_buf3.bytes + f->images.bytes;
 }
 
 // Every refusal of a Camera.Render descriptor that does not depend on the frustum test (caller holds ctx->mu);
