@@ -1,9 +1,11 @@
 """Pins the oracle's raster sink (flush + the Kage shaders, camera.go:350-531,
 1829-2048; shaders/base3d.kage) on scenes whose images follow in closed form:
-coverage (exact areas, shared edges covered once), the depth encoding, the
-depth test between flushes and its absence inside one, transparent pairs
-leaving the depth texture alone, fog and its dithering, texture sampling,
-AlphaClip, the plain draw without RenderDepth and the clear colour.  CPU only.
+coverage (exact areas, shared edges covered once, the top-left rule on edges
+through pixel centres), the depth encoding and decoding channel by channel,
+the depth test between flushes and its absence inside one, transparent pairs
+leaving the depth texture alone, fog (weighted by alpha) and the Bayer
+matrix's orientation, texture sampling, AlphaClip's threshold and alpha, the
+plain draw without RenderDepth and the clear colour.  CPU only.
 
 The fixed-function rules (pixel centres, top-left ties, linear interpolation,
 8-bit rounding) are the oracle's own, stated in its source; the HIP raster
@@ -97,6 +99,20 @@ def test_fan_covers_each_pixel_once(oracle):
     assert hit[inside].all()
 
 
+def test_top_left_rule_on_pixel_centres(oracle):
+    """A 10 x 10 square whose corners sit on pixel centres (102.5, 42.5) to
+    (112.5, 52.5): the centres on its top and left edges are covered, those on
+    its bottom and right edges are not (columns 102-111, rows 42-51), and the
+    diagonal through ten centres gives each to one triangle only."""
+    m = _flat(_square(102.5, 42.5, 112.5, 52.5), color=(1, 1, 1, 0.5))
+    fr, col, _ = _raster(oracle, [m])
+    assert fr.color[0, 0] == np.float32(102.5) and fr.color[0, 1] == np.float32(42.5)   # corners exactly on centres
+    hit = col[..., 3] > 0
+    ys, xs = np.nonzero(hit)
+    assert hit.sum() == 100 and (xs.min(), xs.max(), ys.min(), ys.max()) == (102, 111, 42, 51)
+    assert set(np.unique(col[hit][:, 3])) == {q8(0.5)}
+
+
 def test_depth_encoding_closed_form(oracle):
     """encodeDepth (camera.go:354-359) of depth 0.15625 (Z = 1.25 over a
     spread of 8): r = floor(39.84375) = 39, g = floor(0.84375 * 255) = 215,
@@ -106,6 +122,22 @@ def test_depth_encoding_closed_form(oracle):
     assert tuple(dep[60, 120]) == (39, 215, 40, 255)
     assert tuple(dep[10, 10]) == (0, 0, 0, 0)
     assert tuple(col[60, 120]) == (255, 0, 0, 255)
+    # depth 0.09619140625 (Z = 3.078125): d * 255 = 24.52880859375, 0.52880859375 * 255 = 134.84619140625
+    # (floored to 134, not rounded to 135), fract(d * 65025) = 0.84619140625 stored as 216
+    m = _flat(_square(100, 40, 164, 88), z=-3.078125)
+    _, _, dep = _raster(oracle, [m])
+    assert tuple(dep[60, 120]) == (24, 134, 216, 255)
+
+
+def test_depth_decode_resolves_the_blue_channel(oracle):
+    """decodeDepth (camera.go:361-364) weighs blue by 1/65025: the stored
+    (24, 134, 216) decodes to within 2e-8 of 0.09619140625, so a model at depth
+    0.09765625 drawn after it stays hidden (blue weighed by 1/255 would
+    decode to 0.0995 and let it through)."""
+    near = _flat(_square(100, 40, 164, 88), z=-3.078125, color=(1, 0, 0, 1))
+    far = _flat(_square(100, 40, 164, 88), z=-3.125, color=(0, 1, 0, 1))
+    _, col, dep = _raster(oracle, [near, far])
+    assert tuple(col[60, 120]) == (255, 0, 0, 255) and tuple(dep[60, 120]) == (24, 134, 216, 255)
 
 
 def test_depth_test_between_flushes_not_within(oracle):
@@ -167,6 +199,37 @@ def test_fog_linear_and_dithered(oracle):
     for (y, x) in ((40, 96), (41, 97), (43, 99), (50, 101)):
         on = (d - bayer[(y % 4) * 4 + (x % 4)]) >= 0
         assert col[y, x, 0] == q8(np.float32(0.5) + np.float32(0.25) * np.float32(1.0 if on else 0.0))
+    # the Bayer matrix's orientation: at Z = 7.375 the stored depth is (58, 196, 59), d falls between
+    # 7/17 and 8/17, so fog is on exactly where the matrix holds 1..7: (row, column) mod 4 in
+    # (0, 0), (0, 2), (1, 1), (1, 3), (2, 0), (2, 2), (3, 3) -- not (3, 1), the transposed cell of 7
+    m2 = _flat(_square(96, 40, 160, 88), z=-7.375, color=(0.5, 0.5, 0.5, 1))
+    _, col, dep = _raster(oracle, [m2], w=w)
+    assert tuple(dep[60, 120]) == (58, 196, 59, 255)
+    dd = np.float32(58) / np.float32(255) + (np.float32(196) / np.float32(255)) / np.float32(255) + \
+        (np.float32(59) / np.float32(255)) / np.float32(65025)
+    t = np.float32(min(max(dd / np.float32(0.5), 0), 1))
+    assert 7 / 17 < t * t * (np.float32(3) - np.float32(2) * t) < 8 / 17
+    on = col[40:48, 96:104, 0] > q8(0.5)
+    cells = {(0, 0), (0, 2), (1, 1), (1, 3), (2, 0), (2, 2), (3, 3)}
+    expect = np.array([[(y % 4, x % 4) in cells for x in range(8)] for y in range(8)])
+    assert np.array_equal(on, expect)
+
+
+def test_fog_weighted_by_alpha(oracle):
+    """The fog is added as Fog.rgb * d * colorTex.a (base3d.kage:127-135): a
+    half-transparent grey gets 0.25 + fog * d * 0.5 per channel."""
+    w = _world(fog_on=True, fog_mode=A.FOG_ADD, fog_color=np.array([0.25, 0.5, 1, 1], dtype=f32),
+               fog_range=np.array([0.0, 0.5], dtype=f32))
+    m = _flat(_square(96, 40, 160, 88), color=(0.5, 0.5, 0.5, 0.5))
+    _, col, _ = _raster(oracle, [m], w=w)
+    dd = np.float32(39) / np.float32(255) + (np.float32(215) / np.float32(255)) / np.float32(255) + \
+        (np.float32(40) / np.float32(255)) / np.float32(65025)
+    t = np.float32(min(max(dd / np.float32(0.5), 0), 1))
+    d = t * t * (np.float32(3) - np.float32(2) * t)
+    a = np.float32(0.5)
+    for k, fc in enumerate((0.25, 0.5, 1.0)):
+        assert col[60, 120, k] == q8(np.float32(0.5) * a + np.float32(fc) * d * a)
+    assert col[60, 120, 3] == q8(a)
 
 
 def _checker():
@@ -189,10 +252,28 @@ def test_texture_nearest_and_alpha_clip(oracle):
     assert tuple(col[50, 150]) == (0, 255, 0, 255)
     assert tuple(col[90, 110]) == (0, 0, 255, 255)
     assert tuple(col[90, 150]) == (100, 100, 100, 100)
+    # SrcX = 0.797 at column 125 (u = 25.5 / 64): the texel containing it is the first, not the nearest centre
+    assert tuple(col[50, 125]) == (255, 0, 0, 255)
     m.mesh.mesh_parts[0].material.transparency_mode = A.TRANSPARENCY_ALPHA_CLIP
     _, col, dep = _raster(oracle, [m])
     assert tuple(col[90, 150]) == (0, 0, 0, 0) and dep[90, 150, 3] == 0
     assert tuple(col[90, 110]) == (0, 0, 255, 255) and dep[90, 110, 3] == 255
+
+
+def test_alpha_clip_threshold_and_alpha(oracle):
+    """The clip-alpha shader (camera.go:441-478) discards texels of alpha
+    <= 0.8 -- 180/255 = 0.706 included -- and writes the depth of the others
+    with the texel's alpha: 230/255 leaves depth alpha 230."""
+    tex = Texture(np.array([[[255, 0, 0, 255], [0, 200, 0, 230]], [[0, 0, 180, 180], [100, 100, 100, 100]]],
+                           dtype=np.uint8))
+    uvs = np.array([(0, 1), (1, 1), (1, 0), (0, 1), (1, 0), (0, 0)], dtype=f32)
+    m = _flat(_square(100, 40, 164, 104), texture=tex, uvs=uvs, color=(1, 1, 1, 1))
+    m.mesh.mesh_parts[0].material.transparency_mode = A.TRANSPARENCY_ALPHA_CLIP
+    _, col, dep = _raster(oracle, [m])
+    assert tuple(col[90, 110]) == (0, 0, 0, 0) and dep[90, 110, 3] == 0
+    assert tuple(dep[50, 150]) == (39, 215, 40, 230)
+    assert tuple(col[50, 150]) == (0, 200, 0, 230)
+    assert tuple(dep[50, 110]) == (39, 215, 40, 255)
 
 
 def test_plain_draw_without_render_depth(oracle):
