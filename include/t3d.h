@@ -493,6 +493,10 @@ typedef struct {
   uint32_t vertex_split;             /* 1: clip positions and colours in two arrays instead of one
                                         interleaved 32-byte record per vertex */
   uint32_t cull_block;               /* threads per tile block in the cull pass: 256, 512 or 1024 */
+  uint32_t model_stage;              /* 1: the batched form runs the model-level stage (frustum test, render
+                                        pairs and their order, per-pair slices) on the device when the
+                                        frame allows it: no dynamic batch, RenderDepth on, every model at a
+                                        finite distance from its camera; 0: always on the host */
 } t3d_tuning;
 t3d_status t3d_get_tuning(const t3d_ctx* ctx, t3d_tuning* out);
 t3d_status t3d_set_tuning(t3d_ctx* ctx, const t3d_tuning* tuning);

@@ -272,6 +272,7 @@ class Tuning(C.Structure):
         ("xcd_swizzle", C.c_uint32),
         ("vertex_split", C.c_uint32),
         ("cull_block", C.c_uint32),
+        ("model_stage", C.c_uint32),
     ]
 
 

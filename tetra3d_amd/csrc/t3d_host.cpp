@@ -489,6 +489,101 @@ bool sort_pairs_keyed(std::vector<int32_t>& d, const std::vector<Pair>& pairs) {
   return true;
 }
 
+
+// The scene light list of one Camera.Render (camera.go:1275-1303): the
+// world's ambient light first, then the visible lights in order; cut to
+// Camera.MaxLightCount after Go's stable sort by type and distance to the
+// focus point (camera.go:1474-1497).
+std::vector<SceneLight> scene_light_list(const t3d_render_desc* desc, const t3d_world_desc* world, const Cam& cam) {
+  std::vector<SceneLight> scene_lights;
+  if (world && world->lighting_on && world->ambient_visible) {
+    SceneLight sl{};
+    sl.desc = nullptr;
+    sl.ambient.type = T3D_LIGHT_AMBIENT;
+    sl.ambient.visible = 1;
+    for (int i = 0; i < 3; i++) sl.ambient.color[i] = world->ambient_color[i];
+    sl.ambient.energy = world->ambient_energy;
+    scene_lights.push_back(sl);
+  }
+  for (uint32_t i = 0; i < desc->light_count; i++) {
+    const t3d_light_desc& l = desc->lights[i];
+    if ((!world || world->lighting_on) && l.visible) scene_lights.push_back(SceneLight{&l, {}});
+  }
+  const t3d_camera_desc* cd = desc->camera;
+  if (cd->max_light_count > 0) {
+    V3 fp = cd->has_max_light_count_focus
+                ? v3(cd->max_light_count_focus[0], cd->max_light_count_focus[1], cd->max_light_count_focus[2])
+                : cam.pos;
+    std::vector<int32_t> ord(scene_lights.size());
+    for (size_t i = 0; i < ord.size(); i++) ord[i] = (int32_t)i;
+    go_slice_stable(ord, [&](int32_t x, int32_t y) {
+      const t3d_light_desc& la = scene_lights[x].get();
+      const t3d_light_desc& lb = scene_lights[y].get();
+      if (la.type == T3D_LIGHT_AMBIENT) return true;
+      else if (lb.type == T3D_LIGHT_AMBIENT) return false;
+      else if (la.type == T3D_LIGHT_DIRECTIONAL) return true;
+      else if (lb.type == T3D_LIGHT_DIRECTIONAL) return false;
+      return dist_sq(fp, row3(la.transform)) < dist_sq(fp, row3(lb.transform));
+    });
+    std::vector<SceneLight> sorted;
+    for (int32_t i : ord) sorted.push_back(scene_lights[i]);
+    if ((int)sorted.size() > cd->max_light_count) sorted.resize(cd->max_light_count);
+    scene_lights = sorted;
+  }
+  return scene_lights;
+}
+
+// Camera constants of one Camera.Render (camera.go:1329-1341, 1591-1606).
+FrameConst frame_const(const t3d_camera_desc* cd, const t3d_world_desc* world, const Cam& cam) {
+  FrameConst fc{};
+  fc.vp = cam.vp;
+  fc.campos[0] = cam.pos.x;
+  fc.campos[1] = cam.pos.y;
+  fc.campos[2] = cam.pos.z;
+  fc.near_ = cd->near;
+  fc.far_ = cd->far;
+  fc.cam_w = (float)cd->width;
+  fc.cam_h = (float)cd->height;
+  fc.half_w = (float)cd->width / 2;
+  fc.half_h = (float)cd->height / 2;
+  fc.dmp = (cd->far - cd->near) * cd->depth_margin;
+  fc.spread = cd->far - cd->near + (fc.dmp * 2);
+  fc.snap = cd->vertex_snapping;
+  fc.perspective = cd->perspective;
+  fc.pc_tex = cd->perspective_corrected_texture_mapping;
+  fc.render_depth = cd->render_depth;
+  fc.render_normals = cd->render_normals ? 1 : 0;
+  fc.view = cam.view;
+  fc.fog_active = (!cd->render_depth && world && world->fog_on) ? 1 : 0;
+  fc.fog_mode = world ? world->fog_mode : 0;
+  if (world) {
+    for (int i = 0; i < 4; i++) fc.fog_color[i] = world->fog_color[i];
+    fc.fog_range[0] = world->fog_range[0];
+    fc.fog_range[1] = world->fog_range[1];
+  }
+  return fc;
+}
+
+// The camera and world uniforms of the raster sink's draws (camera.go:1002-1013, 1956-1977; world.go:96-110).
+RasterFrame raster_frame(const t3d_camera_desc* cd, const t3d_world_desc* world) {
+  RasterFrame rf{};
+  rf.fog_range[1] = 1.0f;
+  if (world) {
+    for (int k = 0; k < 3; k++) rf.fog[k] = world->fog_color[k];
+    rf.fog[3] = world->fog_on ? (float)world->fog_mode : -1.0f;
+    rf.fog_range[0] = world->fog_range[0];
+    rf.fog_range[1] = world->fog_range[1];
+    rf.dither = world->dithered_fog_size;
+    rf.curve = world->fog_curve;
+    for (int k = 0; k < 4; k++) rf.clear[k] = world->clear_color[k];
+  }
+  rf.width = cd->width;
+  rf.height = cd->height;
+  rf.render_depth = cd->render_depth ? 1 : 0;
+  rf.pc_tex = cd->perspective_corrected_texture_mapping ? 1 : 0;
+  return rf;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -610,7 +705,8 @@ t3d_status t3d_mesh_upload(t3d_ctx* c, const t3d_mesh_desc* d, uint64_t* out_han
   uint64_t bytes = carve_size<float>(3 * V) * 2 + (d->colors ? carve_size<float>(4 * V) : 0) +
                    (d->uvs ? carve_size<float>(2 * V) : 0) + (d->bone_indices ? carve_size<uint16_t>(4 * V) + carve_size<float>(4 * V) : 0) +
                    (K ? carve_size<float>(3 * K * V) * 2 : 0) +
-                   (V <= 65536 ? carve_size<uint16_t>(3 * T) : carve_size<uint32_t>(3 * T));
+                   (V <= 65536 ? carve_size<uint16_t>(3 * T) : carve_size<uint32_t>(3 * T)) +
+                   carve_size<MeshGPU>(1) + carve_size<MeshPartGPU>(std::max<size_t>(1, m->parts.size()));
   (void)hipSetDevice(c->device);
   t3d_status st = dev_alloc(c, m->buf, bytes);
   if (st) return st;
@@ -644,6 +740,33 @@ t3d_status t3d_mesh_upload(t3d_ctx* c, const t3d_mesh_desc* d, uint64_t* out_han
   } else {
     m->idx = put(d->indices, 3 * T, (uint32_t*)nullptr);
   }
+  // the mesh's record for the device model stage: its arrays and its parts' ranges
+  MeshGPU g{};
+  g.pos = m->pos;
+  g.nrm = m->nrm;
+  g.col = m->col;
+  g.uv = m->uv;
+  g.bones = m->bones;
+  g.weights = m->weights;
+  g.idx = m->idx;
+  g.idx16 = m->idx16;
+  g.sk_pos = m->sk_pos;
+  g.sk_nrm = m->sk_nrm;
+  g.V = m->V;
+  g.K = m->K;
+  g.nparts = (uint32_t)m->parts.size();
+  for (int i = 0; i < 3; i++) {
+    g.dmin[i] = m->dmin[i];
+    g.dmax[i] = m->dmax[i];
+  }
+  g.max_span = m->max_span;
+  std::vector<MeshPartGPU> gparts(m->parts.size());
+  for (size_t p = 0; p < m->parts.size(); p++)
+    gparts[p] = MeshPartGPU{m->parts[p].triangle_start, m->parts[p].triangle_count, m->part_vmin[p], m->part_vcount[p]};
+  MeshGPU* g_dev = cv.take<MeshGPU>(1);
+  g.parts = put(gparts.data(), gparts.size(), (MeshPartGPU*)nullptr);
+  copies.push_back({g_dev, {&g, sizeof(MeshGPU)}});
+  m->gpu = g_dev;
   for (auto& cp : copies) {
     if (!cp.second.second) continue;
     hipError_t e = hipMemcpy(cp.first, cp.second.first, cp.second.second, hipMemcpyHostToDevice);
@@ -836,16 +959,28 @@ t3d_status run_chain(t3d_ctx* c, t3d_frame* f, uint32_t p0, uint32_t p1, hipStre
   struct {
     uint64_t vertex = 0, cull = 0, uv = 0;
   } tr;
-  if (prof && c->profiling)
+  if (prof && c->profiling && !f->device_stage)
     for (uint32_t i = p0; i < p1; i++) {
       tr.vertex += f->bytes_vertex[i];
       tr.cull += f->bytes_cull[i];
       tr.uv += f->bytes_uv[i];
     }
-  const uint32_t vb0 = f->vchunk_prefix[p0], vb1 = f->vchunk_prefix[p1];
-  const uint32_t tb0 = f->tile_prefix[p0], tb1 = f->tile_prefix[p1];
-  const uint32_t sb0 = f->sblock_prefix[p0], sb1 = f->sblock_prefix[p1];
-  const uint32_t ps0 = f->pscan_prefix[p0], ps1 = f->pscan_prefix[p1];
+  uint32_t vb0 = 0, vb1 = f->total_vchunks, tb0 = 0, tb1 = f->total_tiles;
+  uint32_t sb0 = 0, sb1 = f->total_sblocks, ps0 = 0, ps1 = f->total_pscan;
+  if (f->device_stage) {
+    // the pairs are found on the device; every launch is sized for all candidates being drawn
+    ProfScope ps(c, st, "k_models", 0, prof);
+    HIP_TRY(c, launch_model_stage(a, st));
+  } else {
+    vb0 = f->vchunk_prefix[p0];
+    vb1 = f->vchunk_prefix[p1];
+    tb0 = f->tile_prefix[p0];
+    tb1 = f->tile_prefix[p1];
+    sb0 = f->sblock_prefix[p0];
+    sb1 = f->sblock_prefix[p1];
+    ps0 = f->pscan_prefix[p0];
+    ps1 = f->pscan_prefix[p1];
+  }
   if (p0 == 0 && a.npal_jobs) {   // every armature of the frame, before the first pair's vertices
     ProfScope ps(c, st, "k_palettes", f->bytes_palettes, prof);
     HIP_TRY(c, launch_palettes(a, st));
@@ -994,13 +1129,488 @@ t3d_status validate_desc(t3d_ctx* c, const t3d_render_desc* desc) {
 }
 
 
+namespace {
+
+// Whether the model-level stage of these descriptors can run on the device:
+// the tuning asks for it, no model is dynamically batched, every camera has
+// RenderDepth on (its solids then keep model order), and no drawable model is
+// at a NaN distance from its camera (the transparents' comparator is then a
+// strict weak order, so a stable sort by it is Go's SliceStable's order).
+bool stage_applies(const t3d_ctx* c, const t3d_render_desc* const* descs, uint32_t ndesc) {
+  if (!c->tuning.model_stage) return false;
+  for (uint32_t j = 0; j < ndesc; j++) {
+    const t3d_render_desc* d = descs[j];
+    if (!d || !d->camera || !d->camera->render_depth) return false;
+    if (d->model_count && !d->models) return false;
+    const V3 cp = row3(d->camera->transform);
+    for (uint32_t mi = 0; mi < d->model_count; mi++) {
+      const t3d_model_desc& md = d->models[mi];
+      if (md.dynamic_batched) return false;
+      if (md.visible && md.mesh && std::isnan(dist_sq(cp, row3(md.transform)))) return false;
+    }
+  }
+  return true;
+}
+
+MaterialGPU material_gpu(const t3d_material_desc& m) {
+  MaterialGPU g{};
+  for (int i = 0; i < 4; i++) g.color[i] = m.color[i];
+  uint32_t b = 0;
+  if (m.present) b |= MB_PRESENT;
+  if (m.visible) b |= MB_VISIBLE;
+  if (m.backface_culling) b |= MB_BACKFACE;
+  if (m.shadeless) b |= MB_SHADELESS;
+  if (m.blend_source_over) b |= MB_BLEND_SOURCE_OVER;
+  if (m.use_texture) b |= MB_TEXTURE;
+  if (m.billboard) b |= MB_BILLBOARD;
+  if (m.billboard_lock[0]) b |= MB_LOCK_X;
+  if (m.billboard_lock[1]) b |= MB_LOCK_Y;
+  if (m.billboard_lock[2]) b |= MB_LOCK_Z;
+  if (m.billboard_up == T3D_BILLBOARD_UP_CAMERA_Y) b |= MB_UP_CAMERA_Y;
+  if (m.depth_mode == T3D_DEPTH_MODE_UNBILLBOARDED) b |= MB_UNBILLBOARDED;
+  if (m.fogless) b |= MB_FOGLESS;
+  if (m.texture_filter == T3D_FILTER_LINEAR) b |= MB_LINEAR;
+  g.bits = b;
+  g.transparency_mode = m.transparency_mode;
+  g.render_order = m.render_order;
+  g.modes = ((uint32_t)m.triangle_sort_mode & 3u) | (((uint32_t)m.lighting_mode & 3u) << 2) |
+            (((uint32_t)m.light_volume_shading_mode & 3u) << 4);
+  g.texture_size[0] = m.texture_size[0];
+  g.texture_size[1] = m.texture_size[1];
+  g.texture = m.texture;
+  return g;
+}
+
+// What every part of `m` would need if all were drawn, for launch shape (vchunk, tile).
+void mesh_worst_case(Mesh& m, uint32_t vchunk, uint32_t tile) {
+  if (m.wc_vchunk == vchunk && m.wc_tile == tile) return;
+  m.wc_vchunk = vchunk;
+  m.wc_tile = tile;
+  m.wc_vchunks = m.wc_tiles = m.wc_sblocks = m.wc_pscan = 0;
+  m.wc_hist = m.wc_V = m.wc_T = 0;
+  for (size_t p = 0; p < m.parts.size(); p++) {
+    const uint32_t v = m.part_vcount[p], t = m.parts[p].triangle_count;
+    const uint32_t nt = (t + tile - 1) / tile;
+    m.wc_V += v;
+    m.wc_T += t;
+    m.wc_vchunks += (v + vchunk - 1) / vchunk;
+    m.wc_tiles += nt;
+    if (nt > 1) {
+      m.wc_hist += (uint64_t)kBins * nt;
+      if (kBins * nt <= kScanPartMax) m.wc_pscan += 1;
+      else m.wc_sblocks += (kBins * nt + kScanBlock - 1) / kScanBlock;
+    }
+  }
+}
+
+}  // namespace
+
+// frame_build for frames whose model-level stage runs on the device
+// (stage_applies): the host validates each descriptor and writes per job the
+// camera and light records, per drawable model one ModelGPU (and its
+// materials, palette, shape-key weights), and sizes the frame for every
+// candidate pair being drawn.  Returns T3D_ERR_UNSUPPORTED without touching
+// the frame when that worst case would not fit the chain's 32-bit counts
+// (frame_build then builds it on the host).
+t3d_status frame_build_stage(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* descs, uint32_t ndesc) {
+  const auto wall0 = std::chrono::steady_clock::now();
+  (void)hipSetDevice(c->device);
+  const t3d_tuning& tu = c->tuning;
+  const uint32_t vthreads = tu.vertex_block ? tu.vertex_block : 256;
+  const uint32_t vchunk = vthreads * std::max<uint32_t>(1, tu.vertices_per_thread);
+  const uint32_t tile = std::min<uint32_t>(kMaxTile, std::max<uint32_t>(64, tu.triangle_tile));
+
+  // pass 1: validation and sizes
+  uint64_t nm = 0, nmat = 0, ncand = 0, nlights = 0, pal_floats = 0, skw_floats = 0;
+  uint64_t pose_bones = 0, pose_roots = 0, npose = 0;
+  uint64_t wV = 0, wT = 0, wvch = 0, wtiles = 0, wsb = 0, wps = 0, whist = 0, wpl = 0;
+  bool any_normals = false;
+  uint64_t last_id = 0;
+  Mesh* last = nullptr;
+  auto find_mesh = [&](uint64_t id) -> Mesh* {
+    if (id != last_id || !last) {
+      auto it = c->meshes.find(id);
+      last = it == c->meshes.end() ? nullptr : it->second.get();
+      last_id = id;
+    }
+    return last;
+  };
+  for (uint32_t job = 0; job < ndesc; job++) {
+    const t3d_render_desc* desc = descs[job];
+    t3d_status vst = validate_desc(c, desc);
+    if (vst) return vst;
+    const t3d_world_desc* world = (desc->world && desc->world->present) ? desc->world : nullptr;
+    uint32_t n_scene = (world && world->lighting_on && world->ambient_visible) ? 1u : 0u;
+    for (uint32_t i = 0; i < desc->light_count; i++)
+      if ((!world || world->lighting_on) && desc->lights[i].visible) n_scene++;
+    if (desc->camera->max_light_count > 0) n_scene = std::min<uint32_t>(n_scene, (uint32_t)desc->camera->max_light_count);
+    nlights += n_scene;
+    any_normals = any_normals || desc->camera->render_normals;
+    for (uint32_t mi = 0; mi < desc->model_count; mi++) {
+      const t3d_model_desc& md = desc->models[mi];
+      if (!md.visible || md.mesh == 0 || md.auto_batch_static) continue;
+      Mesh* m = find_mesh(md.mesh);   // validate_desc found it
+      mesh_worst_case(*m, vchunk, tile);
+      const uint32_t np = (uint32_t)m->parts.size();
+      nm++;
+      ncand += np;
+      if (md.materials) nmat += np;
+      const uint32_t lc = md.light_group_active ? md.light_group_count : n_scene;
+      if (md.light_group_active) nlights += md.light_group_count;
+      wpl += (uint64_t)np * lc;
+      if (md.skinned) {
+        pal_floats += 16ull * md.bone_count;
+        if (md.bone_poses) {
+          pose_bones += md.bone_count;
+          pose_roots += md.armature_parent ? 1 : 0;
+          npose++;
+        }
+      }
+      skw_floats += m->K;
+      wV += m->wc_V;
+      wT += m->wc_T;
+      wvch += m->wc_vchunks;
+      wtiles += m->wc_tiles;
+      wsb += m->wc_sblocks;
+      wps += m->wc_pscan;
+      whist += m->wc_hist;
+    }
+  }
+  if (wV > 0xFFFFFFFFull || wT > 0x3FFFFFFFull || wvch > 0x7FFFFFFFull || wtiles > 0x7FFFFFFFull ||
+      whist > 0xFFFFFFFFull || wpl > 0xFFFFFFFFull || ncand > 0x3FFFFFFFull)
+    return T3D_ERR_UNSUPPORTED;   // the host build sizes the frame by what is drawn
+  const uint32_t maxP = (uint32_t)ncand;
+
+  f->pairs.clear();
+  f->parts.clear();
+  f->part_first_tri.clear();
+  f->pair_tbase.clear();
+  f->job_pair_begin.assign(1, 0);
+  f->flush.clear();
+  f->flush_begin.clear();
+  f->job_flush_begin.assign(1, 0);
+  f->groups.clear();
+  f->rflush.clear();
+  f->rflush_tex.clear();
+  f->grouped = false;
+  f->fcs.clear();
+  f->bytes_vertex.clear();
+  f->bytes_cull.clear();
+  f->bytes_uv.clear();
+  f->vchunk_prefix.assign(1, 0);
+  f->tile_prefix.assign(1, 0);
+  f->sblock_prefix.assign(1, 0);
+  f->pscan_prefix.assign(1, 0);
+
+  // device tables (uploaded): cameras, jobs, models, materials, lights, palettes, shape-key weights, poses
+  const uint64_t tbytes = carve_size<FrameConst>(ndesc) + carve_size<JobGPU>(ndesc) + carve_size<ModelGPU>(nm) +
+                          carve_size<MaterialGPU>(nmat) + carve_size<LightGPU>(nlights) + carve_size<float>(pal_floats) +
+                          carve_size<float>(skw_floats) + carve_size<t3d_bone_pose>(pose_bones) +
+                          carve_size<M4>(pose_roots) + carve_size<PaletteJob>(npose);
+  // read back after the frame: counts, each job's first pair, PairOut and stats per pair
+  const uint64_t rb_jobs = carve_size<uint32_t>(FC_WORDS);
+  const uint64_t rb_pairs = rb_jobs + carve_size<uint32_t>(ndesc + 1);
+  const uint64_t rb_stats = rb_pairs + carve_size<PairOut>(maxP);
+  const uint64_t rb_bytes = rb_stats + carve_size<uint32_t>(4ull * maxP);
+  // scratch: the stage's arrays, the pair records and prefix arrays, then the chain's scratch at worst-case sizes
+  const uint64_t sbytes = rb_bytes + carve_size<PartGPU>(maxP) + 4 * carve_size<uint32_t>(maxP + 1ull) +
+                          4 * carve_size<uint32_t>(ncand) + carve_size<float>(ncand) + carve_size<int32_t>(ncand) +
+                          carve_size<uint4>(2 * ncand) + 2 * carve_size<uint4>(ncand) + 2 * carve_size<uint32_t>(ndesc) +
+                          carve_size<uint32_t>(8ull * maxP) +
+                          carve_size<float4>(2 * wV) + carve_size<float4>(wV) + carve_size<uint32_t>(wT) +
+                          2 * carve_size<uint4>(wT) + carve_size<uint32_t>(whist) + carve_size<uint32_t>(wsb) +
+                          2 * carve_size<uint32_t>(wtiles) + carve_size<uint32_t>(wvch) + carve_size<float4>(128) +
+                          carve_size<PartLight>(wpl) + carve_size<M4>(pose_bones) +
+                          carve_size<float4>(any_normals ? wV : 1) + carve_size<float>(wV);
+  t3d_status st = ensure(c, f->tables, f->tables_cap, tbytes);
+  if (st) return st;
+  if (tbytes > f->pinned_cap) {
+    if (f->pinned) (void)hipHostFree(f->pinned);
+    f->pinned = nullptr;
+    f->pinned_cap = align_up(tbytes + tbytes / 4, 1 << 16);
+    HIP_TRY(c, hipHostMalloc(&f->pinned, f->pinned_cap, hipHostMallocDefault));
+  }
+  st = ensure(c, f->scratch, f->scratch_cap, sbytes);
+  if (st) return st;
+  const uint64_t obytes = 2 * carve_size<t3d_vertex>(3 * wT) + carve_size<uint32_t>(3 * wT) +
+                          (any_normals ? carve_size<t3d_vertex>(3 * wT) : 0);
+  st = ensure(c, f->outputs, f->outputs_cap, obytes);
+  if (st) return st;
+  if (rb_bytes > f->pinned_rb_cap) {
+    if (f->pinned_rb) (void)hipHostFree(f->pinned_rb);
+    f->pinned_rb = nullptr;
+    f->pinned_rb_cap = align_up(rb_bytes + rb_bytes / 4, 1 << 16);
+    HIP_TRY(c, hipHostMalloc((void**)&f->pinned_rb, f->pinned_rb_cap, hipHostMallocDefault));
+  }
+  uint8_t* tables_dev = reinterpret_cast<uint8_t*>(f->tables.ptr);
+  Carve dev{tables_dev};
+  Carve host{reinterpret_cast<uint8_t*>(f->pinned)};
+  FrameConst* h_fc = host.take<FrameConst>(ndesc);
+  const FrameConst* d_fc = dev.take<FrameConst>(ndesc);
+  JobGPU* h_jobs = host.take<JobGPU>(ndesc);
+  const JobGPU* d_jobs = dev.take<JobGPU>(ndesc);
+  ModelGPU* h_models = host.take<ModelGPU>(nm);
+  const ModelGPU* d_models = dev.take<ModelGPU>(nm);
+  MaterialGPU* h_mats = host.take<MaterialGPU>(nmat);
+  const MaterialGPU* d_mats = dev.take<MaterialGPU>(nmat);
+  LightGPU* h_lights = host.take<LightGPU>(nlights);
+  const LightGPU* d_lights = dev.take<LightGPU>(nlights);
+  float* h_pal = host.take<float>(pal_floats);
+  float* d_pal = dev.take<float>(pal_floats);
+  float* h_skw = host.take<float>(skw_floats);
+  float* d_skw = dev.take<float>(skw_floats);
+  t3d_bone_pose* h_poses = host.take<t3d_bone_pose>(pose_bones);
+  t3d_bone_pose* d_poses = dev.take<t3d_bone_pose>(pose_bones);
+  M4* h_roots = host.take<M4>(pose_roots);
+  M4* d_roots = dev.take<M4>(pose_roots);
+  PaletteJob* h_pjobs = host.take<PaletteJob>(npose);
+  PaletteJob* d_pjobs = dev.take<PaletteJob>(npose);
+
+  // pass 2: per job its camera, frustum and lights, per drawable model its records
+  uint32_t im = 0, imat = 0, il = 0, ipose = 0;
+  uint64_t pal_off = 0, skw_off = 0, bone_off = 0, root_off = 0, cand = 0;
+  for (uint32_t job = 0; job < ndesc; job++) {
+    const t3d_render_desc* desc = descs[job];
+    const t3d_camera_desc* cd = desc->camera;
+    const t3d_world_desc* world = (desc->world && desc->world->present) ? desc->world : nullptr;
+    Cam cam;
+    cam_setup(cam, cd);
+    h_fc[job] = frame_const(cd, world, cam);
+    f->fcs.push_back(h_fc[job]);
+    if (job == 0) f->rframe = raster_frame(cd, world);
+    JobGPU& J = h_jobs[job];
+    J = JobGPU{};
+    const V3 vs[4] = {cam.pos, cam.fwd, cam.up, cam.right};
+    float* dst[4] = {J.pos, J.fwd, J.up, J.right};
+    for (int k = 0; k < 4; k++) {
+      dst[k][0] = vs[k].x;
+      dst[k][1] = vs[k].y;
+      dst[k][2] = vs[k].z;
+    }
+    J.near_ = cd->near;
+    J.far_ = cd->far;
+    J.sf_tang = cam.sf_tang;
+    J.sf_x = cam.sf_x;
+    J.sf_y = cam.sf_y;
+    J.aspect = cam.aspect;
+    J.ortho_scale = cd->ortho_scale;
+    J.perspective = cd->perspective;
+    J.lighting_on = world && world->lighting_on;
+    J.cand0 = (uint32_t)cand;
+    // the scene lights (camera.go:1275-1303, 1474-1497)
+    const uint32_t lbase_scene = il;
+    bool scene_dir = false;
+    for (const SceneLight& sl : scene_light_list(desc, world, cam)) {
+      h_lights[il++] = make_light(c, sl.get());
+      scene_dir = scene_dir || sl.get().type == T3D_LIGHT_DIRECTIONAL;
+    }
+    const uint32_t n_scene = il - lbase_scene;
+    for (uint32_t mi = 0; mi < desc->model_count; mi++) {
+      const t3d_model_desc& md = desc->models[mi];
+      if (!md.visible || md.mesh == 0 || md.auto_batch_static) continue;
+      const Mesh* m = find_mesh(md.mesh);
+      const uint32_t np = (uint32_t)m->parts.size();
+      ModelGPU& M = h_models[im];
+      M.transform = md.transform;
+      for (int i = 0; i < 4; i++) M.color[i] = md.color[i];
+      M.mesh = m->gpu;
+      uint32_t flags = 0;
+      if (md.frustum_culling) flags |= MF_FRUSTUM;
+      if (md.skinned) flags |= MF_SKINNED;
+      if (md.shadeless) flags |= MF_SHADELESS;
+      if (md.materials) {
+        flags |= MF_MATERIALS;
+        M.mat = imat;
+        for (uint32_t p = 0; p < np; p++) h_mats[imat++] = material_gpu(md.materials[p]);
+      } else {
+        M.mat = 0;
+      }
+      bool dir = scene_dir;
+      if (md.light_group_active) {   // its LightGroup replaces the scene lights (camera.go:1576-1581)
+        M.light0 = il;
+        M.light_count = md.light_group_count;
+        dir = false;
+        for (uint32_t k = 0; k < md.light_group_count; k++) {
+          h_lights[il++] = make_light(c, md.light_group[k]);
+          dir = dir || md.light_group[k].type == T3D_LIGHT_DIRECTIONAL;
+        }
+      } else {
+        M.light0 = lbase_scene;
+        M.light_count = n_scene;
+      }
+      if (dir) {
+        flags |= MF_DIRECTIONAL;
+        if (!md.skinned) M.world_rotation = md.world_rotation;
+      }
+      M.flags = flags;
+      M.cam = job;
+      M.cand0 = (uint32_t)cand;
+      M.bone_count = md.skinned ? md.bone_count : 0;
+      M.palette = (uint32_t)pal_off;
+      if (md.skinned) {
+        if (md.bone_poses) {   // an armature posed on the device (k_palettes)
+          std::memcpy(h_poses + bone_off, md.bone_poses, sizeof(t3d_bone_pose) * md.bone_count);
+          PaletteJob& H = h_pjobs[ipose++];
+          const Animation* an = md.animation ? c->animations.at(md.animation).get() : nullptr;
+          H.poses = d_poses + bone_off;
+          H.palette = d_pal + pal_off;
+          H.world_off = bone_off;
+          H.n = md.bone_count;
+          H.time = md.animation_time;
+          H.channels = an ? an->channels : nullptr;
+          H.nchannels = an ? an->nchannels : 0;
+          H.pad = 0;
+          H.root = nullptr;
+          if (md.armature_parent) {
+            h_roots[root_off] = *md.armature_parent;
+            H.root = d_roots + root_off++;
+          }
+          bone_off += md.bone_count;
+        } else {
+          std::memcpy(h_pal + pal_off, &md.bone_influences[0].m[0][0], 64ull * md.bone_count);
+        }
+        pal_off += 16ull * md.bone_count;
+      }
+      M.skw = (uint32_t)skw_off;
+      for (uint32_t k = 0; k < m->K; k++) h_skw[skw_off + k] = md.shape_key_weights ? md.shape_key_weights[k] : 0.0f;
+      skw_off += m->K;
+      M.index = mi;
+      im++;
+      cand += np;
+    }
+    J.cand1 = (uint32_t)cand;
+  }
+  f->table_bytes = host.off;
+  f->stage_models = h_models;
+  f->stage_mats = h_mats;
+
+  // frame arguments: tables, the stage's scratch, the chain's scratch at worst-case sizes
+  Carve sc{reinterpret_cast<uint8_t*>(f->scratch.ptr)};
+  uint8_t* rb = sc.take<uint8_t>(rb_bytes);
+  f->rb_dev = rb;
+  f->rb_bytes = rb_bytes;
+  f->rb_jobs = rb_jobs;
+  f->rb_pairs = rb_pairs;
+  f->rb_stats = rb_stats;
+  FrameArgs& a = f->args;
+  a = FrameArgs{};
+  a.counts = reinterpret_cast<uint32_t*>(rb);
+  a.stats = reinterpret_cast<uint32_t*>(rb + rb_stats);
+  a.parts = sc.take<PartGPU>(maxP);
+  a.nparts = maxP;
+  a.vchunk = vchunk;
+  a.tile = tile;
+  a.xcd_swizzle = tu.xcd_swizzle ? 1u : 0u;
+  a.lights = d_lights;
+  ModelStage& ms = a.ms;
+  ms.vchunk_prefix = sc.take<uint32_t>(maxP + 1ull);
+  ms.tile_prefix = sc.take<uint32_t>(maxP + 1ull);
+  ms.sblock_prefix = sc.take<uint32_t>(maxP + 1ull);
+  ms.pscan_prefix = sc.take<uint32_t>(maxP + 1ull);
+  a.vchunk_prefix = ms.vchunk_prefix;
+  a.tile_prefix = ms.tile_prefix;
+  a.sblock_prefix = ms.sblock_prefix;
+  a.pscan_prefix = ms.pscan_prefix;
+  ms.models = d_models;
+  ms.mats = d_mats;
+  ms.jobs = d_jobs;
+  ms.nmodels = (uint32_t)nm;
+  ms.njobs = ndesc;
+  ms.ncand = (uint32_t)ncand;
+  ms.nlights = (uint32_t)nlights;
+  ms.cand = sc.take<uint32_t>(ncand);
+  ms.cand_model = sc.take<uint32_t>(ncand);
+  ms.tlist = sc.take<uint32_t>(ncand);
+  ms.pair_cand = sc.take<uint32_t>(ncand);
+  ms.cand_dist = sc.take<float>(ncand);
+  ms.cand_order = sc.take<int32_t>(ncand);
+  ms.cand_cnt = sc.take<uint4>(2 * ncand);
+  ms.cand_bytes = sc.take<uint4>(ncand);
+  ms.cand_info = sc.take<uint4>(ncand);
+  ms.job_s0 = sc.take<uint32_t>(ndesc);
+  ms.job_t0 = sc.take<uint32_t>(ndesc);
+  ms.job_pairs = reinterpret_cast<uint32_t*>(rb + rb_jobs);
+  ms.pair_out = reinterpret_cast<PairOut*>(rb + rb_pairs);
+  ms.pc = sc.take<uint32_t>(8ull * maxP);
+  ms.pal_base = d_pal;
+  ms.skw_base = d_skw;
+  ms.vchunk = vchunk;
+  ms.tile = tile;
+  ms.max_pairs = maxP;
+  {
+    float4* v = sc.take<float4>(2 * wV);
+    a.vstride = tu.vertex_split ? 1u : 2u;
+    a.vclip = v;
+    a.vcol = tu.vertex_split ? v + wV : v + 1;
+  }
+  a.vpos = sc.take<float4>(wV);
+  a.key = sc.take<float>(wT);
+  a.tri_rec = sc.take<uint4>(wT);
+  a.tri_vo = sc.take<uint4>(wT);
+  a.hist = sc.take<uint32_t>(whist);
+  a.bsum = sc.take<uint32_t>(wsb);
+  a.tile_count = sc.take<uint32_t>(wtiles);
+  a.tile_part = sc.take<uint32_t>(wtiles);
+  a.vchunk_part = sc.take<uint32_t>(wvch);
+  a.sink = sc.take<float4>(128);
+  a.plights = sc.take<PartLight>(wpl);
+  a.pose_world = sc.take<M4>(pose_bones);
+  a.vnrm = sc.take<float4>(any_normals ? wV : 1);
+  a.vud = sc.take<float>(wV);
+  Carve oc{reinterpret_cast<uint8_t*>(f->outputs.ptr)};
+  a.out_color = reinterpret_cast<float4*>(oc.take<t3d_vertex>(3 * wT));
+  a.out_depth = reinterpret_cast<float4*>(oc.take<t3d_vertex>(3 * wT));
+  a.out_idx = oc.take<uint32_t>(3 * wT);
+  a.out_normal = any_normals ? reinterpret_cast<float4*>(oc.take<t3d_vertex>(3 * wT)) : nullptr;
+  a.pal_jobs = d_pjobs;
+  a.npal_jobs = (uint32_t)npose;
+  a.groups = nullptr;
+  a.ngroups = 0;
+  a.fc = d_fc;
+  f->bytes_palettes = (sizeof(t3d_bone_pose) + 64) * pose_bones + 64 * pose_roots + sizeof(PaletteJob) * npose;
+  f->any_normals = any_normals;
+  f->tables_dev = tables_dev;
+  f->device_stage = true;
+  f->max_pairs = maxP;
+  f->stage_jobs = ndesc;
+  f->nparts = maxP;
+  f->vertex_threads = vthreads;
+  f->total_vchunks = (uint32_t)wvch;
+  f->total_tiles = (uint32_t)wtiles;
+  f->total_sblocks = (uint32_t)wsb;
+  f->total_pscan = (uint32_t)wps;
+  f->total_triangles = wT;
+  if (c->profiling) {
+    const double ms_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    auto& hp = c->prof["host:prologue"];
+    hp.ms += ms_;
+    hp.launches++;
+    auto& hu = c->prof["host:pairs+uniforms"];
+    hu.ms += ms_;
+    hu.launches++;
+    auto& hl = c->prof["host:pair-list"];
+    hl.launches++;
+    auto& hs = c->prof["host:pair-sort"];
+    hs.launches++;
+  }
+  return T3D_OK;
+}
+
 // Builds the frame tables of one or more Camera.Render calls (a batch of
 // jobs shares one launch chain; every render pair carries its job's camera
 // index).  Pure host work: nothing is sent to the device yet.
 t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* descs, uint32_t ndesc,
-                       const Placement* place) {
+                       const Placement* place, bool device_stage) {
   auto wall0 = std::chrono::steady_clock::now();
   if (!c || !f || !descs || ndesc == 0) return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: null argument");
+  if (device_stage && !place && stage_applies(c, descs, ndesc)) {
+    t3d_status sst = frame_build_stage(c, f, descs, ndesc);
+    if (sst != T3D_ERR_UNSUPPORTED) return sst;   // else too large for the worst case: sized by what is drawn
+  }
+  f->device_stage = false;
+  f->max_pairs = 0;
+  f->stage_models = nullptr;
+  f->stage_mats = nullptr;
   (void)hipSetDevice(c->device);
   const t3d_tuning& tu = c->tuning;
   const uint32_t vthreads = tu.vertex_block ? tu.vertex_block : 256;
@@ -1010,6 +1620,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
   f->pairs.clear();
   f->parts.clear();
   f->part_first_tri.clear();
+  f->pair_tbase.clear();
   f->job_pair_begin.assign(1, 0);
   f->flush.clear();
   f->flush_begin.clear();
@@ -1070,21 +1681,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
     Cam cam;
     cam_setup(cam, cd);
 
-    // scene light list (camera.go:1275-1303)
-    std::vector<SceneLight> scene_lights;
-    if (world && world->lighting_on && world->ambient_visible) {
-      SceneLight sl{};
-      sl.desc = nullptr;
-      sl.ambient.type = T3D_LIGHT_AMBIENT;
-      sl.ambient.visible = 1;
-      for (int i = 0; i < 3; i++) sl.ambient.color[i] = world->ambient_color[i];
-      sl.ambient.energy = world->ambient_energy;
-      scene_lights.push_back(sl);
-    }
-    for (uint32_t i = 0; i < desc->light_count; i++) {
-      const t3d_light_desc& l = desc->lights[i];
-      if ((!world || world->lighting_on) && l.visible) scene_lights.push_back(SceneLight{&l, {}});
-    }
+    std::vector<SceneLight> scene_lights = scene_light_list(desc, world, cam);
 
     // consecutive models usually share a mesh: one map lookup per change
     uint64_t last_mesh_id = 0;
@@ -1218,26 +1815,6 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
           return pairs[x].depth > pairs[y].depth;
         });
     }
-    if (cd->max_light_count > 0) {                                     // camera.go:1474-1497
-      V3 fp = cd->has_max_light_count_focus
-                  ? v3(cd->max_light_count_focus[0], cd->max_light_count_focus[1], cd->max_light_count_focus[2])
-                  : cam.pos;
-      std::vector<int32_t> ord(scene_lights.size());
-      for (size_t i = 0; i < ord.size(); i++) ord[i] = (int32_t)i;
-      go_slice_stable(ord, [&](int32_t x, int32_t y) {
-        const t3d_light_desc& la = scene_lights[x].get();
-        const t3d_light_desc& lb = scene_lights[y].get();
-        if (la.type == T3D_LIGHT_AMBIENT) return true;
-        else if (lb.type == T3D_LIGHT_AMBIENT) return false;
-        else if (la.type == T3D_LIGHT_DIRECTIONAL) return true;
-        else if (lb.type == T3D_LIGHT_DIRECTIONAL) return false;
-        return dist_sq(fp, row3(la.transform)) < dist_sq(fp, row3(lb.transform));
-      });
-      std::vector<SceneLight> sorted;
-      for (int32_t i : ord) sorted.push_back(scene_lights[i]);
-      if ((int)sorted.size() > cd->max_light_count) sorted.resize(cd->max_light_count);
-      scene_lights = sorted;
-    }
     if (!sort_pairs_keyed(transparents, pairs))                         // camera.go:2050-2055
       go_slice_stable(transparents, [&](int32_t x, int32_t y) {
         if (pairs[x].order == pairs[y].order) return pairs[x].depth > pairs[y].depth;
@@ -1258,35 +1835,8 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
     std::map<uint32_t, uint32_t> group_base;  // model -> first frame light of its group
     std::map<uint32_t, uint32_t> model_pose_job;  // posed model -> its palette job
 
-    // camera constants (camera.go:1329-1341, 1591-1606)
-    FrameConst fc{};
-    fc.vp = cam.vp;
-    fc.campos[0] = cam.pos.x;
-    fc.campos[1] = cam.pos.y;
-    fc.campos[2] = cam.pos.z;
-    fc.near_ = cd->near;
-    fc.far_ = cd->far;
-    fc.cam_w = (float)cd->width;
-    fc.cam_h = (float)cd->height;
-    fc.half_w = (float)cd->width / 2;
-    fc.half_h = (float)cd->height / 2;
-    fc.dmp = (cd->far - cd->near) * cd->depth_margin;
-    fc.spread = cd->far - cd->near + (fc.dmp * 2);
-    fc.snap = cd->vertex_snapping;
-    fc.perspective = cd->perspective;
-    fc.pc_tex = cd->perspective_corrected_texture_mapping;
-    fc.render_depth = cd->render_depth;
-    fc.render_normals = cd->render_normals ? 1 : 0;
-    fc.view = cam.view;
     any_normals = any_normals || cd->render_normals;
-    fc.fog_active = (!cd->render_depth && world && world->fog_on) ? 1 : 0;
-    fc.fog_mode = world ? world->fog_mode : 0;
-    if (world) {
-      for (int i = 0; i < 4; i++) fc.fog_color[i] = world->fog_color[i];
-      fc.fog_range[0] = world->fog_range[0];
-      fc.fog_range[1] = world->fog_range[1];
-    }
-    f->fcs.push_back(fc);
+    f->fcs.push_back(frame_const(cd, world, cam));
 
     // one render pair's uniforms and slices: Model.ProcessVertices + the render closure (camera.go:1507-1827)
     auto emit_pair = [&](uint32_t pmi, uint32_t ppart, const Mesh& mesh) -> t3d_status {
@@ -1428,6 +1978,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
         f->bytes_cull.push_back((mesh.idx16 ? 6ull : 12ull) * P.tcount);
         f->bytes_uv.push_back(mesh.uv ? 8ull * P.vcount : 0);   // UVs are read by the assembly
         f->part_first_tri.push_back(pr.triangle_start);
+        f->pair_tbase.push_back(P.tbase);
         t3d_pair_result r{};
         r.model_index = pmi;
         r.part_index = ppart;
@@ -1458,24 +2009,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
       if (mat && !mat->blend_source_over) r.flags |= RF_BAD_BLEND;
       return r;
     };
-    if (job == 0) {
-      RasterFrame& rf = f->rframe;
-      rf = RasterFrame{};
-      rf.fog_range[1] = 1.0f;
-      if (world) {
-        for (int k = 0; k < 3; k++) rf.fog[k] = world->fog_color[k];
-        rf.fog[3] = world->fog_on ? (float)world->fog_mode : -1.0f;
-        rf.fog_range[0] = world->fog_range[0];
-        rf.fog_range[1] = world->fog_range[1];
-        rf.dither = world->dithered_fog_size;
-        rf.curve = world->fog_curve;
-        for (int k = 0; k < 4; k++) rf.clear[k] = world->clear_color[k];
-      }
-      rf.width = cd->width;
-      rf.height = cd->height;
-      rf.render_depth = cd->render_depth ? 1 : 0;
-      rf.pc_tex = cd->perspective_corrected_texture_mapping ? 1 : 0;
-    }
+    if (job == 0) f->rframe = raster_frame(cd, world);
 
     for (int32_t di : draw) {
       const Pair& rp = pairs[di];
@@ -1704,6 +2238,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
 
   Carve sc{scratch_dev};
   FrameArgs& a = f->args;
+  a = FrameArgs{};   // nothing of an earlier device-built frame (its counts, the stage's arrays) carries over
   a.parts = d_parts;
   a.nparts = P_n;
   a.vchunk = vchunk;
@@ -1789,7 +2324,7 @@ t3d_status frame_launch(t3d_ctx* c, t3d_frame* f, hipStream_t stream, hipStream_
     return T3D_OK;
   }
   t3d_status st = T3D_OK;
-  if (exec_mode == T3D_EXEC_PLAIN) {
+  if (exec_mode == T3D_EXEC_PLAIN && !f->device_stage) {
     for (uint32_t i = 0; i < P_n; i++) {
       st = run_chain(c, f, i, i + 1, stream, profile);
       if (st) return st;
@@ -1826,8 +2361,13 @@ t3d_status frame_launch(t3d_ctx* c, t3d_frame* f, hipStream_t stream, hipStream_
     st = run_chain(c, f, 0, P_n, stream, profile);
     if (st) return st;
   }
-  HIP_TRY(c, hipMemcpyAsync(f->pinned_stats, a.stats, 16ull * P_n, hipMemcpyDeviceToHost, stream));
-  f->stats_host = f->pinned_stats;
+  if (f->device_stage) {   // counts, the jobs' first pairs, the pairs and their stats in one copy
+    HIP_TRY(c, hipMemcpyAsync(f->pinned_rb, f->rb_dev, f->rb_bytes, hipMemcpyDeviceToHost, stream));
+    f->stats_host = reinterpret_cast<const uint32_t*>(f->pinned_rb + f->rb_stats);
+  } else {
+    HIP_TRY(c, hipMemcpyAsync(f->pinned_stats, a.stats, 16ull * P_n, hipMemcpyDeviceToHost, stream));
+    f->stats_host = f->pinned_stats;
+  }
   HIP_TRY(c, hipEventRecord(f->done, stream));
   return T3D_OK;
 }
@@ -1861,7 +2401,81 @@ t3d_status frame_run_placed(t3d_ctx* c, t3d_frame* f, hipStream_t stream, int32_
 }
 
 // After the launch's work is complete: survivor counts into the pair results.
+namespace {
+
+// The render pairs of a frame whose model-level stage ran on the device, from the readback: what the host
+// build records while it builds (pairs, first triangles, triangle-scratch slots, jobs' first pairs, the raster
+// sink's flush records), and the profile's algorithmic bytes.
+void collect_stage(t3d_ctx* c, t3d_frame* f, bool profile) {
+  const uint32_t* cnt = reinterpret_cast<const uint32_t*>(f->pinned_rb);
+  const uint32_t P = f->max_pairs ? cnt[FC_PAIRS] : 0u;
+  const uint32_t* jp = reinterpret_cast<const uint32_t*>(f->pinned_rb + f->rb_jobs);
+  const PairOut* po = reinterpret_cast<const PairOut*>(f->pinned_rb + f->rb_pairs);
+  f->nparts = P;
+  f->pairs.resize(P);
+  f->part_first_tri.resize(P);
+  f->pair_tbase.resize(P);
+  f->rflush.resize(P);
+  f->rflush_tex.resize(P);
+  for (uint32_t i = 0; i < P; i++) {
+    t3d_pair_result& r = f->pairs[i];
+    r.model_index = po[i].model;
+    r.part_index = po[i].part;
+    r.triangle_count = po[i].triangle_count;
+    r.vertex_offset = po[i].out_base;
+    f->part_first_tri[i] = po[i].triangle_start;
+    f->pair_tbase[i] = po[i].tbase;
+    // what the raster sink's draw of this flush reads (camera.go:1838-1878, 1914-1917)
+    const ModelGPU& M = f->stage_models[po[i].mrec];
+    const MaterialGPU* mat = nullptr;
+    if ((M.flags & MF_MATERIALS) && (f->stage_mats[M.mat + po[i].part].bits & MB_PRESENT))
+      mat = &f->stage_mats[M.mat + po[i].part];
+    RasterFlush rf{};
+    rf.tex = nullptr;
+    rf.tw = rf.th = 1;
+    uint64_t tex = 0;
+    if (mat && (mat->bits & MB_TEXTURE) && mat->texture) {
+      tex = mat->texture;
+      auto it = c->textures.find(tex);
+      if (it == c->textures.end()) rf.flags |= RF_NO_TEXTURE;
+      else {
+        rf.tex = reinterpret_cast<const uint8_t*>(it->second->buf.ptr);
+        rf.tw = it->second->w;
+        rf.th = it->second->h;
+      }
+    }
+    if (mat && (mat->bits & MB_LINEAR)) rf.flags |= RF_LINEAR;
+    if (mat && mat->transparency_mode == T3D_TRANSPARENCY_ALPHA_CLIP) rf.flags |= RF_CLIP;
+    if (mat && (mat->bits & MB_FOGLESS)) rf.flags |= RF_FOGLESS;
+    bool transparent;
+    if (mat) {
+      const bool matT = mat->transparency_mode == T3D_TRANSPARENCY_TRANSPARENT || !(mat->bits & MB_BLEND_SOURCE_OVER) ||
+                        (mat->transparency_mode == T3D_TRANSPARENCY_AUTO && mat->color[3] < 0.999f);
+      transparent = matT || (mat->transparency_mode != T3D_TRANSPARENCY_OPAQUE && M.color[3] < 0.999f);
+    } else {
+      transparent = M.color[3] < 0.999f;
+    }
+    if (!transparent) rf.flags |= RF_COMPOSITE;
+    if (mat && !(mat->bits & MB_BLEND_SOURCE_OVER)) rf.flags |= RF_BAD_BLEND;
+    f->rflush[i] = rf;
+    f->rflush_tex[i] = mat && (mat->bits & MB_TEXTURE) ? mat->texture : 0;
+    (void)tex;
+  }
+  f->job_pair_begin.assign(f->stage_jobs + 1, 0);
+  if (f->max_pairs)
+    for (uint32_t j = 0; j <= f->stage_jobs; j++) f->job_pair_begin[j] = jp[j];
+  if (profile && c->profiling && f->max_pairs) {
+    auto w64 = [&](uint32_t lo) { return (uint64_t)cnt[lo] | ((uint64_t)cnt[lo + 1] << 32); };
+    c->prof["k_vertex"].bytes += w64(FC_BYTES_VERTEX_LO);
+    c->prof["k_cull"].bytes += w64(FC_BYTES_CULL_LO);
+    c->prof["k_assemble"].bytes += w64(FC_BYTES_UV_LO);
+  }
+}
+
+}  // namespace
+
 void frame_collect(t3d_ctx* c, t3d_frame* f, bool profile) {
+  if (f->device_stage) collect_stage(c, f, profile);
   uint64_t total = 0;
   for (uint32_t i = 0; i < f->nparts; i++) {
     f->pairs[i].survivor_count = f->stats_host[4 * i + 2];
@@ -1982,7 +2596,7 @@ t3d_status frame_download_range(t3d_ctx* c, const t3d_frame* f, uint32_t p0, uin
     for (uint32_t i = p0; i < p1; i++) {
       const auto& p = f->pairs[i];
       if (!p.survivor_count) continue;
-      DL_TRY(hipMemcpyAsync(recs.data() + toff, f->args.tri_vo + f->parts[i].tbase, 16ull * p.survivor_count,
+      DL_TRY(hipMemcpyAsync(recs.data() + toff, f->args.tri_vo + f->pair_tbase[i], 16ull * p.survivor_count,
                             hipMemcpyDeviceToHost, stream));
       toff += p.survivor_count;
     }
@@ -2043,9 +2657,12 @@ void frame_release(t3d_ctx* c, t3d_frame* f) {
   f->image_color = f->image_depth = nullptr;
   if (f->pinned) (void)hipHostFree(f->pinned);
   if (f->pinned_stats) (void)hipHostFree(f->pinned_stats);
+  if (f->pinned_rb) (void)hipHostFree(f->pinned_rb);
   f->pinned = nullptr;
   f->pinned_stats = nullptr;
-  f->pinned_cap = f->pinned_stats_cap = 0;
+  f->pinned_rb = nullptr;
+  f->pinned_cap = f->pinned_stats_cap = f->pinned_rb_cap = 0;
+  f->device_stage = false;
   if (f->graph_exec) (void)hipGraphExecDestroy(f->graph_exec);
   f->graph_exec = nullptr;
   if (f->done) (void)hipEventDestroy(f->done);
@@ -2073,12 +2690,13 @@ t3d_status render_submit(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc) 
   const auto wall0 = std::chrono::steady_clock::now();
   t3d_status st = frame_finish(c, f);
   if (st) return st;
-  st = frame_build(c, f, &desc, 1);
+  st = frame_build(c, f, &desc, 1, nullptr, desc->exec_mode == T3D_EXEC_BATCHED);
   if (st) {   // nothing of a refused frame is published
     f->nparts = 0;
     f->pairs.clear();
     f->flush.clear();
     f->grouped = false;
+    f->device_stage = false;
     return st;
   }
   st = frame_launch(c, f, f->stream ? f->stream : c->stream, c->copy_stream, desc->exec_mode, true, true);
@@ -2259,6 +2877,7 @@ t3d_status t3d_set_tuning(t3d_ctx* c, const t3d_tuning* t) {
   if (t->vertex_split > 1) return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: vertex_split is 0 or 1");
   if (t->cull_block != 256 && t->cull_block != 512 && t->cull_block != 1024)
     return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: cull_block is 256, 512 or 1024");
+  if (t->model_stage > 1) return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: model_stage is 0 or 1");
   c->tuning = *t;
   return T3D_OK;
 }
@@ -2304,6 +2923,7 @@ t3d_status t3d_autotune(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc, u
       {&t3d_tuning::xcd_swizzle, {0, 1}},
       {&t3d_tuning::vertex_split, {0, 1}},
       {&t3d_tuning::cull_block, {256, 512, 1024}},
+      {&t3d_tuning::model_stage, {0, 1}},
   };
   t3d_tuning best = base;
   if (!valid(best)) best.assemble_split = 1;

@@ -144,6 +144,143 @@ struct PaletteJob {
   uint32_t nchannels, pad;
 };
 
+// ---- the model-level stage on the device (camera.go:1343-1457, model.go:142-182) ----
+//
+// With it, the host writes one ModelGPU per drawable model (and the materials
+// of its parts) instead of one PartGPU per render pair: k_models tests every
+// model's sphere against its camera's frustum and marks each (model, part)
+// candidate; k_plist orders the candidates into render pairs (solids in
+// model order, then the transparents by render order and distance, stable),
+// sizes every pair's slices of the frame arrays and writes the prefix arrays
+// the chain's kernels index by; k_uniforms then builds each pair's PartGPU.
+// Every later launch is sized for all candidates being drawn; blocks past the
+// actual counts (FrameCounts) return at once.
+
+// Resident with each mesh (after its arrays): what the stage reads of it.
+struct alignas(16) MeshPartGPU {
+  uint32_t triangle_start, triangle_count, vmin, vcount;
+};
+struct alignas(16) MeshGPU {
+  const float* pos;
+  const float* nrm;
+  const float* col;
+  const float* uv;
+  const uint16_t* bones;
+  const float* weights;
+  const uint32_t* idx;
+  const uint16_t* idx16;
+  const float* sk_pos;
+  const float* sk_nrm;
+  const MeshPartGPU* parts;
+  uint32_t V, K, nparts, pad;
+  float dmin[4], dmax[4];
+  float max_span;
+  uint32_t pad2[3];
+};
+
+// Material state of one MeshPart as the stage reads it (t3d_material_desc).
+enum : uint32_t {
+  MB_PRESENT = 1u << 0, MB_VISIBLE = 1u << 1, MB_BACKFACE = 1u << 2, MB_SHADELESS = 1u << 3,
+  MB_BLEND_SOURCE_OVER = 1u << 4, MB_TEXTURE = 1u << 5, MB_BILLBOARD = 1u << 6,
+  MB_LOCK_X = 1u << 7, MB_LOCK_Y = 1u << 8, MB_LOCK_Z = 1u << 9, MB_UP_CAMERA_Y = 1u << 10,
+  MB_UNBILLBOARDED = 1u << 11,
+  MB_FOGLESS = 1u << 12, MB_LINEAR = 1u << 13,   // read by the host only (raster records)
+};
+struct alignas(16) MaterialGPU {
+  float color[4];
+  uint32_t bits;            // MB_*
+  int32_t transparency_mode;
+  int32_t render_order;
+  uint32_t modes;           // sort mode | lighting mode << 2 | LightVolume shading mode << 4
+  float texture_size[2];
+  uint64_t texture;         // Material.Texture handle (with use_texture), for the host's raster records
+};
+
+enum : uint32_t {
+  MF_FRUSTUM = 1u << 0,     // Model.FrustumCulling
+  MF_SKINNED = 1u << 1,
+  MF_SHADELESS = 1u << 2,
+  MF_DIRECTIONAL = 1u << 3, // a directional light is among the model's lights
+  MF_MATERIALS = 1u << 4,   // the model has a material array (MaterialGPU from ModelGPU::mat)
+};
+
+struct alignas(16) ModelGPU {
+  M4 transform;             // Node.Transform()
+  M4 world_rotation;        // Node.WorldRotation()
+  float color[4];           // Model.Color
+  const MeshGPU* mesh;
+  uint32_t mat;             // first MaterialGPU of its parts (MF_MATERIALS)
+  uint32_t flags;           // MF_*
+  uint32_t light0, light_count;  // the lights its lit pairs see: the job's scene lights or its LightGroup's
+  uint32_t cam;             // job of the frame
+  uint32_t cand0;           // its first candidate; candidate cand0 + p is its MeshPart p
+  uint32_t palette;         // float offset of its bone palette in the table block (skinned)
+  uint32_t skw;             // float offset of its shape-key weights in the table block
+  uint32_t bone_count;
+  uint32_t index;           // its index in its Camera.Render's model list
+};
+
+// Per job: the camera's frustum (camera.go:933-983) and what the pairs read of the world.
+struct alignas(16) JobGPU {
+  float pos[4], fwd[4], up[4], right[4];
+  float near_, far_, sf_tang, sf_x;
+  float sf_y, aspect, ortho_scale;
+  int32_t perspective;
+  int32_t lighting_on;      // World != nil && World.LightingOn
+  uint32_t cand0, cand1;    // its candidates
+  uint32_t pad;
+};
+
+// Actual sizes of a frame built on the device (read back with the stats).
+enum : uint32_t {
+  FC_PAIRS = 0, FC_VCHUNKS, FC_TILES, FC_SBLOCKS, FC_PSCAN, FC_TRANSPARENTS, FC_SUMV, FC_SUMT,
+  FC_BYTES_VERTEX_LO, FC_BYTES_VERTEX_HI, FC_BYTES_CULL_LO, FC_BYTES_CULL_HI, FC_BYTES_UV_LO, FC_BYTES_UV_HI,
+  FC_WORDS = 16
+};
+
+// Per render pair of a device-built frame, read back after the frame: what
+// t3d_frame_pairs and the downloads report.
+struct alignas(16) PairOut {
+  uint32_t model;           // index in its Camera.Render's model list
+  uint32_t part;
+  uint32_t triangle_count;  // MeshPart.TriangleCount()
+  uint32_t tbase;           // first slot in the triangle scratch
+  uint64_t out_base;        // first output vertex
+  uint32_t triangle_start;  // the part's first mesh triangle
+  uint32_t mrec;            // its ModelGPU record
+};
+
+struct ModelStage {
+  const ModelGPU* models;   // [nmodels], job by job in model order
+  const MaterialGPU* mats;
+  const JobGPU* jobs;       // [njobs]
+  uint32_t nmodels, njobs, ncand, nlights;
+  uint32_t* cand;           // [ncand] bit 0: drawn, bit 1: transparent, bits 2-31: its job
+  float* cand_dist;         // [ncand] distance² of the candidate's model to its camera
+  uint32_t* cand_model;     // [ncand] its ModelGPU record
+  int32_t* cand_order;      // [ncand] its Material.RenderOrder
+  uint4* cand_cnt;          // [2 * ncand] what its pair would need: vertices, triangles, vertex blocks, tiles;
+                            // scan blocks, one-block scan, histogram entries, per-light values
+  uint4* cand_bytes;        // [ncand] its algorithmic bytes: vertex pass, cull pass, UVs
+  uint4* cand_info;         // [ncand] model index, part, MeshPart triangle count, first mesh triangle
+  uint32_t* job_s0;         // [njobs] solid and transparent drawn candidates before each job's first
+  uint32_t* job_t0;
+  uint32_t* tlist;          // [ncand] drawn transparents, candidate order (jobs contiguous)
+  uint32_t* pair_cand;      // [ncand] candidate of each render pair
+  uint32_t* job_pairs;      // [njobs + 1] first render pair of each job
+  PairOut* pair_out;        // [ncand]
+  uint32_t* pc;             // [8][ncand] per-pair counts, then their exclusive scans
+  uint32_t* vchunk_prefix;  // [max_pairs + 1] the frame's prefix arrays (FrameArgs), written here
+  uint32_t* tile_prefix;
+  uint32_t* sblock_prefix;
+  uint32_t* pscan_prefix;
+  const float* pal_base;    // palettes and shape-key weights of ModelGPU::palette / skw
+  const float* skw_base;
+  uint32_t vchunk, tile;
+  uint32_t max_pairs;       // = ncand
+  uint32_t pad;
+};
+
 struct FrameArgs {
   const PartGPU* parts;
   uint32_t nparts;
@@ -183,6 +320,10 @@ struct FrameArgs {
   uint32_t npal_jobs;
   uint32_t ngroups;             // dynamic-batch flushes of more than one render pair
   const uint2* groups;          // [ngroups] {first render pair, render pairs} of each, pairs consecutive
+  // a frame whose render pairs the device builds (the model-level stage): its actual sizes (FC_*), and the
+  // stage's inputs and scratch; null for a frame built on the host, whose launches are sized exactly
+  uint32_t* counts;
+  ModelStage ms;
 };
 
 // Packing of a range of render pairs' survivors for download: pair j of the
@@ -254,6 +395,8 @@ hipError_t launch_assemble(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uin
 hipError_t launch_pack(const PackArgs& a, uint32_t nblocks, hipStream_t s);
 hipError_t launch_normals(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s);
 hipError_t launch_groups(const FrameArgs& a, uint32_t part0, uint32_t part1, hipStream_t s);
+// The model-level stage of a device-built frame (t3d_models.hip): k_models, then k_plist.
+hipError_t launch_model_stage(const FrameArgs& a, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 hipError_t launch_fill(void* dst, uint64_t bytes, uint32_t word, hipStream_t s);
 hipError_t launch_fp_selftest(const float* a, const float* b, float* out, uint32_t n, hipStream_t s);

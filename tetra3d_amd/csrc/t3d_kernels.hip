@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include "t3d_internal.h"
+#include "t3d_stage.h"
 #include "t3d_anim.h"
 
 namespace t3d {
@@ -50,11 +51,31 @@ __device__ __forceinline__ uint32_t find_part(const uint32_t* __restrict__ prefi
 // render pair (which gather the same vertex records) share an L2.  The map
 // is a bijection of [0, n).
 constexpr uint32_t kXcds = 8;
-__device__ __forceinline__ uint32_t block_index(uint32_t swizzle) {
-  const uint32_t i = blockIdx.x, n = gridDim.x;
+__device__ __forceinline__ uint32_t block_index_n(uint32_t swizzle, uint32_t n) {
+  const uint32_t i = blockIdx.x;
   if (!swizzle) return i;
   const uint32_t per = n / kXcds, rem = n % kXcds, x = i % kXcds;
   return x * per + min(x, rem) + i / kXcds;
+}
+__device__ __forceinline__ uint32_t block_index(uint32_t swizzle) { return block_index_n(swizzle, gridDim.x); }
+
+// The logical block of this workgroup.  A frame built on the device is
+// launched for every candidate pair being drawn; its actual block count is
+// FrameCounts word `which` (times `mult`), and a workgroup past it gets
+// kNoBlock (block-uniform: the whole workgroup returns).  The swizzle then
+// deals the actual blocks, not the launch's, over the XCDs.
+constexpr uint32_t kNoBlock = 0xFFFFFFFFu;
+__device__ __forceinline__ uint32_t frame_block(const FrameArgs& a, uint32_t which, uint32_t mult) {
+  uint32_t n = gridDim.x;
+  if (a.counts) {
+    n = a.counts[which] * mult;
+    if (blockIdx.x >= n) return kNoBlock;
+  }
+  return block_index_n(a.xcd_swizzle, n);
+}
+// The same for the launches that never swizzle (scan blocks, pairs).
+__device__ __forceinline__ bool past_count(const FrameArgs& a, uint32_t which) {
+  return a.counts && blockIdx.x >= a.counts[which];
 }
 
 __device__ __forceinline__ float clampf(float v, float lo, float hi) {   // math32.go:38-45
@@ -173,7 +194,9 @@ __device__ __forceinline__ void light_vertex(const LightGPU& l, const PartLight&
 // ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
-  const uint32_t b = blk0 + block_index(a.xcd_swizzle);
+  const uint32_t lb = frame_block(a, FC_VCHUNKS, 1);
+  if (lb == kNoBlock) return;
+  const uint32_t b = blk0 + lb;
   const uint32_t p = a.vchunk_part[b];
   const PartGPU P = a.parts[p];        // by value: read once, before any store
   const uint32_t flags = P.flags;
@@ -434,7 +457,18 @@ __global__ __launch_bounds__(kPoseThreads) void k_palettes(FrameArgs a) {
 // and survivor count, so the latency is one light's, not the sum over lights;
 // all lanes first write the pair's entries of the block->pair maps.
 __global__ __launch_bounds__(64) void k_uniforms(FrameArgs a, uint32_t part0, uint32_t nparts) {
+  if (past_count(a, FC_PAIRS)) return;
   PartGPU& P = const_cast<PartGPU*>(a.parts)[part0 + blockIdx.x];
+  if (a.counts) {   // a device-built frame: the pair's record is built here from the model, mesh and material
+    __shared__ PartGPU s_part;
+    if (threadIdx.x == 0) build_pair(a, part0 + blockIdx.x, s_part);
+    __syncthreads();
+    static_assert(sizeof(PartGPU) % 16 == 0, "whole float4 rows");
+    const float4* src = reinterpret_cast<const float4*>(&s_part);
+    float4* dst = reinterpret_cast<float4*>(&P);
+    for (uint32_t i = threadIdx.x; i < sizeof(PartGPU) / 16; i += 64) dst[i] = src[i];
+    __syncthreads();
+  }
   const uint32_t k = threadIdx.x;
   {  // block -> pair maps of the later kernels: one load where a search over the prefixes took nine
     const uint32_t p = part0 + blockIdx.x;
@@ -521,7 +555,9 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 template <uint32_t kThreads>
 __global__ __launch_bounds__(kThreads) void k_cull(FrameArgs a, uint32_t blk0) {
   constexpr uint32_t kCullWaves = kThreads / 64;
-  const uint32_t b = blk0 + block_index(a.xcd_swizzle);
+  const uint32_t lb = frame_block(a, FC_TILES, 1);
+  if (lb == kNoBlock) return;
+  const uint32_t b = blk0 + lb;
   const uint32_t p = a.tile_part[b];
   const PartGPU P = a.parts[p];        // by value: read once, before any store
   const uint32_t flags = P.flags;
@@ -701,7 +737,9 @@ __device__ __forceinline__ void part_range(const uint32_t* stats, uint32_t p, fl
 }
 
 __global__ __launch_bounds__(64) void k_hist(FrameArgs a, uint32_t blk0) {
-  const uint32_t b = blk0 + block_index(a.xcd_swizzle);
+  const uint32_t lb = frame_block(a, FC_TILES, 1);
+  if (lb == kNoBlock) return;
+  const uint32_t b = blk0 + lb;
   const uint32_t p = a.tile_part[b];
   const PartGPU P = a.parts[p];        // by value: read once, before any store
   if (P.ntiles <= 1) return;
@@ -750,6 +788,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_
 }
 
 __global__ __launch_bounds__(256) void k_scan_reduce(FrameArgs a, uint32_t sblk0) {
+  if (past_count(a, FC_SBLOCKS)) return;
   const uint32_t b = sblk0 + blockIdx.x;
   const uint32_t p = find_part(a.sblock_prefix, a.nparts, b);
   const PartGPU P = a.parts[p];        // by value: read once, before any store
@@ -772,6 +811,7 @@ __global__ __launch_bounds__(256) void k_scan_reduce(FrameArgs a, uint32_t sblk0
 // part, 256 sums per step with the running total carried across steps (a
 // serial walk would pay one dependent memory round trip per scan block).
 __global__ __launch_bounds__(256) void k_scan_bsums(FrameArgs a, uint32_t part0) {
+  if (past_count(a, FC_PAIRS)) return;
   const uint32_t n = a.parts[part0 + blockIdx.x].nsblocks;
   if (n == 0) return;
   uint32_t* __restrict__ bs = a.bsum + a.parts[part0 + blockIdx.x].sblock0;
@@ -788,6 +828,7 @@ __global__ __launch_bounds__(256) void k_scan_bsums(FrameArgs a, uint32_t part0)
 }
 
 __global__ __launch_bounds__(256) void k_scan_down(FrameArgs a, uint32_t sblk0) {
+  if (past_count(a, FC_SBLOCKS)) return;
   const uint32_t b = sblk0 + blockIdx.x;
   const uint32_t p = find_part(a.sblock_prefix, a.nparts, b);
   const PartGPU P = a.parts[p];        // by value: read once, before any store
@@ -823,6 +864,7 @@ __global__ __launch_bounds__(256) void k_scan_down(FrameArgs a, uint32_t sblk0) 
 __global__ __launch_bounds__(1024) void k_scan_part(FrameArgs a, uint32_t blk0) {
   constexpr uint32_t kPer = kScanPartMax / 4 / 1024;
   static_assert(kScanPartMax % 4096 == 0, "whole uint4 segments");
+  if (past_count(a, FC_PSCAN)) return;
   const uint32_t b = blk0 + blockIdx.x;
   const uint32_t p = find_part(a.pscan_prefix, a.nparts, b);
   const uint32_t n4 = kBins * a.parts[p].ntiles / 4;           // the matrix holds a multiple of 512 entries
@@ -867,7 +909,9 @@ __global__ __launch_bounds__(1024) void k_scan_part(FrameArgs a, uint32_t blk0) 
 // visited 511->0 (BackToFront) or 0->511, insertion order kept within a bin.
 // The survivor's record {vertex-record offsets, triangle} moves to its slot.
 __global__ __launch_bounds__(64) void k_scatter(FrameArgs a, uint32_t blk0) {
-  const uint32_t b = blk0 + block_index(a.xcd_swizzle);
+  const uint32_t lb = frame_block(a, FC_TILES, 1);
+  if (lb == kNoBlock) return;
+  const uint32_t b = blk0 + lb;
   const uint32_t p = a.tile_part[b];
   const PartGPU P = a.parts[p];        // by value: read once, before any store
   if (a.stats[4 * p + 2] == 0) return;
@@ -943,7 +987,9 @@ __global__ __launch_bounds__(64) void k_scatter(FrameArgs a, uint32_t blk0) {
 // independently and the result is the single-wave order exactly.
 constexpr uint32_t kScatterWaves = 4;
 __global__ __launch_bounds__(256) void k_scatter4(FrameArgs a, uint32_t blk0) {
-  const uint32_t b = blk0 + block_index(a.xcd_swizzle);
+  const uint32_t lb = frame_block(a, FC_TILES, 1);
+  if (lb == kNoBlock) return;
+  const uint32_t b = blk0 + lb;
   const uint32_t p = a.tile_part[b];
   const PartGPU P = a.parts[p];        // by value: read once, before any store
   if (a.stats[4 * p + 2] == 0) return;
@@ -1064,7 +1110,8 @@ __device__ __forceinline__ void store1(uint32_t* p, uint32_t v) {
 // tile), so the grid follows the survivors more finely than the sort's tiles.
 template <int kStage, bool kNT, uint32_t kWaves>
 __global__ __launch_bounds__(64 * kWaves) void k_assemble(FrameArgs a, uint32_t blk0, uint32_t split) {
-  const uint32_t li = block_index(a.xcd_swizzle);
+  const uint32_t li = frame_block(a, FC_TILES, split);
+  if (li == kNoBlock) return;
   const uint32_t b = blk0 + li / split;
   const uint32_t p = a.tile_part[b];
   const PartGPU P = a.parts[p];        // by value: read once, before any store
@@ -1210,7 +1257,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_assemble(FrameArgs a, uint32_t 
 // current chunk's stores stay in flight instead of draining them.
 template <uint32_t kRows, bool kNT>
 __global__ __launch_bounds__(64) void k_assemble_pipe(FrameArgs a, uint32_t blk0, uint32_t split) {
-  const uint32_t li = block_index(a.xcd_swizzle);
+  const uint32_t li = frame_block(a, FC_TILES, split);
+  if (li == kNoBlock) return;
   const uint32_t b = blk0 + li / split;
   const uint32_t p = a.tile_part[b];
   const PartGPU P = a.parts[p];
@@ -1425,7 +1473,9 @@ __global__ __launch_bounds__(256) void k_groups(FrameArgs a, uint32_t part0, uin
 // (k_vertex's vnrm); alpha and the custom fields stay 0.  One 256-thread
 // block per sort tile, one thread per (triangle, corner).
 __global__ __launch_bounds__(256) void k_normals(FrameArgs a, uint32_t blk0) {
-  const uint32_t b = blk0 + block_index(a.xcd_swizzle);
+  const uint32_t lb = frame_block(a, FC_TILES, 1);
+  if (lb == kNoBlock) return;
+  const uint32_t b = blk0 + lb;
   const uint32_t p = a.tile_part[b];
   const PartGPU P = a.parts[p];
   if (!a.fc[P.cam].render_normals) return;

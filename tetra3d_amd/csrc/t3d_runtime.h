@@ -43,6 +43,11 @@ struct Mesh {
   uint32_t max_bone_index = 0;
   bool has_skin = false;
   int refs = 0;                    // submitted jobs referencing the mesh
+  const MeshGPU* gpu = nullptr;    // its record for the device model stage (in buf, after the arrays)
+  // what all its parts would need if every one were drawn, for launch shape (vchunk, tile) (device model stage)
+  uint32_t wc_vchunk = 0, wc_tile = 0;
+  uint32_t wc_vchunks = 0, wc_tiles = 0, wc_sblocks = 0, wc_pscan = 0;
+  uint64_t wc_hist = 0, wc_V = 0, wc_T = 0;
 };
 
 struct Volume {
@@ -117,6 +122,18 @@ struct t3d_frame {
   uint8_t* image_depth = nullptr;
   int32_t image_w = 0, image_h = 0;
   uint32_t job_out_begin(uint32_t k) const { return grouped ? job_flush_begin[k] : job_pair_begin[k]; }
+  std::vector<uint32_t> pair_tbase;      // first triangle-scratch slot of each render pair
+  // the model-level stage ran on the device: the chain is launched for max_pairs candidate pairs, and the
+  // render pairs, their offsets and the jobs' first pairs are read back with the stats (rb: counts, job
+  // pairs, PairOut per pair, stats per pair, at rb_* byte offsets)
+  bool device_stage = false;
+  uint32_t max_pairs = 0, stage_jobs = 0;
+  uint64_t rb_bytes = 0, rb_jobs = 0, rb_pairs = 0, rb_stats = 0;
+  uint8_t* rb_dev = nullptr;
+  uint8_t* pinned_rb = nullptr;
+  uint64_t pinned_rb_cap = 0;
+  const t3d::ModelGPU* stage_models = nullptr;     // host copies in the table staging, for the raster records
+  const t3d::MaterialGPU* stage_mats = nullptr;
   uint64_t table_bytes = 0;
   uint64_t total_triangles = 0;
   uint32_t nparts = 0;
@@ -148,7 +165,7 @@ struct t3d_ctx {
   std::map<uint64_t, std::unique_ptr<t3d::Animation>> animations;
   std::map<uint64_t, std::unique_ptr<t3d::Texture>> textures;
   std::string err;
-  t3d_tuning tuning{256, 1, 2048, 4, 1, 2, 8, 1, 0, 256};
+  t3d_tuning tuning{256, 1, 2048, 4, 1, 2, 8, 1, 0, 256, 1};
   // Guards resources, the budget and err between callers and the serving
   // thread; frame builds hold it, device work runs outside it.
   std::mutex mu;
@@ -202,8 +219,10 @@ struct Placement {
 // holds ctx->mu), launch (enqueue on `stream`), collect (after completion).
 // With `place`, nothing is allocated: the frame's device memory is the
 // placement's, its table staging pageable host memory.
+// With device_stage, a frame that allows it (and whose tuning asks for it) has its model-level stage run on the
+// device: frustum tests, render pairs and their slices are then known only after the frame (frame_collect).
 t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* descs, uint32_t ndesc,
-                       const Placement* place = nullptr);
+                       const Placement* place = nullptr, bool device_stage = false);
 // Runs a frame built with a placement on `stream` and waits for it: table
 // upload from pageable memory, the chain (per pair for the plain form, one
 // chain for the batched form; never a graph), stats readback.  Calls no

@@ -454,7 +454,7 @@ void worker_main(t3d_server* s) {
             frame_release(s->ctx, f);
             delete f;
           }
-          st = frame_build(s->ctx, b->frame, descs.data(), (uint32_t)descs.size());
+          st = frame_build(s->ctx, b->frame, descs.data(), (uint32_t)descs.size(), nullptr, true);
           if (st == T3D_OK) st = frame_launch(s->ctx, b->frame, sl.stream, sl.stream, T3D_EXEC_BATCHED, false, false);
           if (st != T3D_OK) {
             err = s->ctx->err;
