@@ -269,7 +269,9 @@ struct ModelStage {
   uint32_t* pair_cand;      // [ncand] candidate of each render pair
   uint32_t* job_pairs;      // [njobs + 1] first render pair of each job
   PairOut* pair_out;        // [ncand]
-  uint32_t* pc;             // [8][ncand] per-pair counts, then their exclusive scans
+  uint32_t* pc;             // [8][ncand] per pair the exclusive scans of its vertex records (row 0), triangles
+                            // (1), histogram entries (6) and per-light values (7); the other rows are unused
+                            // (the prefix arrays hold those scans)
   uint32_t* vchunk_prefix;  // [max_pairs + 1] the frame's prefix arrays (FrameArgs), written here
   uint32_t* tile_prefix;
   uint32_t* sblock_prefix;

@@ -168,8 +168,9 @@ __device__ __forceinline__ bool t_before(int32_t oa, float da, uint32_t ia, int3
   return ia < ib;
 }
 
-// Each thread takes kItems consecutive candidates (then pairs) per round, so
-// a round covers kListThreads * kItems of them with one block scan.
+// In the first phase each thread takes kItems consecutive candidates per
+// round (loaded coalesced through LDS), so a round of kListThreads * kItems
+// candidates needs one block scan.
 constexpr uint32_t kItems = 4;
 constexpr uint32_t kRound = kListThreads * kItems;
 
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(kListThreads, 1) void k_plist(FrameArgs a) {
   __shared__ uint32_t s_w2[kListWaves + 1][2];
   __shared__ uint32_t s_w8[kListWaves + 1][8];
   __shared__ uint32_t s_S[kListThreads], s_T[kListThreads];   // each thread's first prefix in the round
-  __shared__ uint32_t s_stage[kRound];                          // a round's candidate flags, then pair candidates
+  __shared__ uint32_t s_stage[kRound];                          // a round's candidate flags
   __shared__ unsigned long long s_bytes[kListWaves][3];
   uint32_t* job_S0 = ms.job_s0;   // [njobs] solid and transparent drawn candidates before each job's first
   uint32_t* job_T0 = ms.job_t0;
@@ -267,63 +268,49 @@ __global__ __launch_bounds__(kListThreads, 1) void k_plist(FrameArgs a) {
   }
   __syncthreads();
 
-  // Phase 3: slices and prefix arrays over the pairs
+  // Phase 3: slices and prefix arrays over the pairs, one pair per thread
+  // and round, so every load and store of a wavefront is contiguous
   uint32_t carry[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   unsigned long long bv = 0, bc = 0, bu = 0;
-  for (uint32_t base = 0; base < P; base += kRound) {
-    const uint32_t p0 = base + tid * kItems;
-    uint32_t cc[kItems];
+  const size_t mp = ms.max_pairs;
+  for (uint32_t base = 0; base < P; base += kListThreads) {
+    const uint32_t p = base + tid;
     uint32_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (uint32_t i = 0; i < kItems; i++) {
-      const uint32_t p = base + i * kListThreads + tid;
-      s_stage[i * kListThreads + tid] = p < P ? ms.pair_cand[p] : 0u;
-    }
-    __syncthreads();
-#pragma unroll
-    for (uint32_t i = 0; i < kItems; i++) cc[i] = s_stage[tid * kItems + i];
-#pragma unroll
-    for (uint32_t i = 0; i < kItems; i++) {
-      if (p0 + i >= P) continue;
-      const uint4 q0 = ms.cand_cnt[2 * cc[i]], q1 = ms.cand_cnt[2 * cc[i] + 1];
-      const uint4 b = ms.cand_bytes[cc[i]];
+    uint32_t c = 0;
+    if (p < P) {
+      c = ms.pair_cand[p];
+      const uint4 q0 = ms.cand_cnt[2 * c], q1 = ms.cand_cnt[2 * c + 1];
+      v[0] = q0.x; v[1] = q0.y; v[2] = q0.z; v[3] = q0.w;
+      v[4] = q1.x; v[5] = q1.y; v[6] = q1.z; v[7] = q1.w;
+      const uint4 b = ms.cand_bytes[c];
       bv += b.x;
       bc += b.y;
       bu += b.z;
-      v[0] += q0.x; v[1] += q0.y; v[2] += q0.z; v[3] += q0.w;
-      v[4] += q1.x; v[5] += q1.y; v[6] += q1.z; v[7] += q1.w;
     }
     uint32_t tot[8];
     block_scan<8>(v, tot, s_w8);
-    // v: this thread's first prefixes; walk its pairs again (their counts are cached by now)
-#pragma unroll
-    for (int k = 0; k < 8; k++) v[k] += carry[k];
-#pragma unroll
-    for (uint32_t i = 0; i < kItems; i++) {
-      const uint32_t p = p0 + i;
-      if (p >= P) continue;
-#pragma unroll
-      for (int k = 0; k < 8; k++) ms.pc[(size_t)k * ms.max_pairs + p] = v[k];
-      ms.vchunk_prefix[p] = v[2];
-      ms.tile_prefix[p] = v[3];
-      ms.sblock_prefix[p] = v[4];
-      ms.pscan_prefix[p] = v[5];
-      const uint4 info = ms.cand_info[cc[i]];
+    if (p < P) {
+      ms.pc[0 * mp + p] = carry[0] + v[0];   // first vertex record
+      ms.pc[1 * mp + p] = carry[1] + v[1];   // first triangle slot
+      ms.pc[6 * mp + p] = carry[6] + v[6];   // first histogram entry
+      ms.pc[7 * mp + p] = carry[7] + v[7];   // first per-light value
+      ms.vchunk_prefix[p] = carry[2] + v[2];
+      ms.tile_prefix[p] = carry[3] + v[3];
+      ms.sblock_prefix[p] = carry[4] + v[4];
+      ms.pscan_prefix[p] = carry[5] + v[5];
+      const uint4 info = ms.cand_info[c];
       PairOut o;
       o.model = info.x;
       o.part = info.y;
       o.triangle_count = info.z;
-      o.tbase = v[1];
-      o.out_base = 3ull * v[1];
+      o.tbase = carry[1] + v[1];
+      o.out_base = 3ull * o.tbase;
       o.triangle_start = info.w;
-      o.mrec = ms.cand_model[cc[i]];
+      o.mrec = ms.cand_model[c];
       ms.pair_out[p] = o;
-      const uint4 q0 = ms.cand_cnt[2 * cc[i]], q1 = ms.cand_cnt[2 * cc[i] + 1];
-      v[0] += q0.x; v[1] += q0.y; v[2] += q0.z; v[3] += q0.w;
-      v[4] += q1.x; v[5] += q1.y; v[6] += q1.z; v[7] += q1.w;
     }
+#pragma unroll
     for (int k = 0; k < 8; k++) carry[k] += tot[k];
-    __syncthreads();   // s_stage is rewritten by the next round
   }
   // past the last pair the prefix arrays hold the totals, so a search over
   // all max_pairs + 1 entries finds only actual pairs

@@ -133,7 +133,7 @@ __device__ __forceinline__ void build_pair(const FrameArgs& a, uint32_t p, PartG
   P.ntiles = z.ntiles;
   if (z.ntiles > 1) {
     P.hbase = ms.pc[6 * mp + p];
-    P.sblock0 = ms.pc[4 * mp + p];
+    P.sblock0 = ms.sblock_prefix[p];
     P.nsblocks = z.nsblocks;
   }
 }
