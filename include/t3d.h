@@ -507,6 +507,45 @@ t3d_status t3d_set_tuning(t3d_ctx* ctx, const t3d_tuning* tuning);
 t3d_status t3d_autotune(t3d_ctx* ctx, t3d_frame* frame, const t3d_render_desc* desc, uint32_t reps,
                         t3d_tuning* out);
 
+/* ---- glTF -> resident mesh pool ------------------------------------------
+ * LoadGLTFData's mesh and skin reading (gltf.go:560-840, 1449-1505): a glTF 2.0
+ * document (GLB, or JSON with base64 data URIs) decoded on the host into the
+ * t3d_mesh_desc arrays the reference's Mesh holds after loading (one MeshPart
+ * per primitive, vertices appended, indices offset by the part's first vertex,
+ * UV V flipped, COLOR_0 as 8-bit sRGB-converted colours that are the active
+ * channel when the mesh extras name its colour channels, influences of weight
+ * <= 0 dropped, morph targets as shape keys, dimensions over the positions),
+ * and the skins' joints and inverse bind matrices.  t3d_gltf_upload puts every
+ * mesh in the context's resident pool.  Refused (T3D_ERR_INVALID_ARGUMENT,
+ * message in err): sparse accessors, external buffer files, primitives without
+ * indices or with an index count that is not a multiple of three, morph
+ * targets on a mesh of several primitives, auto-subdivided meshes. */
+typedef struct t3d_gltf t3d_gltf;
+typedef struct {
+  char name[64];                     /* glTF mesh name (truncated) */
+  uint32_t vertex_count, triangle_count, part_count, shape_key_count;
+  int32_t has_colors;                /* VertexActiveColorChannel is 0 (else -1: desc.colors is NULL) */
+  int32_t skinned;                   /* JOINTS_0 / WEIGHTS_0 present */
+} t3d_gltf_mesh_info;
+/* Needs no context and no GPU.  err (may be NULL) receives the reason of a refusal. */
+t3d_status t3d_gltf_parse(const void* data, uint64_t size, t3d_gltf** out, char* err, uint32_t err_len);
+void t3d_gltf_free(t3d_gltf* gltf);
+uint32_t t3d_gltf_mesh_count(const t3d_gltf* gltf);
+/* Mesh i: its summary, its arrays as t3d_mesh_upload takes them (pointers into
+ * the document, valid until t3d_gltf_free), and per part the glTF material
+ * index of its primitive (-1: none).  Any output may be NULL. */
+t3d_status t3d_gltf_mesh(const t3d_gltf* gltf, uint32_t i, t3d_gltf_mesh_info* info, t3d_mesh_desc* desc,
+                         const int32_t** part_materials);
+uint32_t t3d_gltf_skin_count(const t3d_gltf* gltf);
+/* Skin i: its joint node indices (bone b of a skinned mesh's JOINTS_0 is joint
+ * b) and their inverse bind matrices as Matrix4 (column c = the c-th four
+ * floats of the accessor element, gltf.go:1487-1497). */
+t3d_status t3d_gltf_skin(const t3d_gltf* gltf, uint32_t i, uint32_t* joint_count, const int32_t** joints,
+                         const t3d_mat4** inverse_bind);
+/* Every mesh of the document into the resident pool: handles[i] for mesh i
+ * (handles holds t3d_gltf_mesh_count entries).  On failure none stays resident. */
+t3d_status t3d_gltf_upload(t3d_ctx* ctx, const t3d_gltf* gltf, uint64_t* handles);
+
 /* ---- serving ----------------------------------------------------------- */
 
 /* A render-request loop over a context's resident meshes.  Jobs (one

@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libt3d.so")
-SOURCES = ["t3d_kernels.hip", "t3d_models.hip", "t3d_host.cpp", "t3d_serve.cpp", "t3d_noalloc.hip", "t3d_raster.hip"]
+SOURCES = ["t3d_kernels.hip", "t3d_models.hip", "t3d_host.cpp", "t3d_serve.cpp", "t3d_noalloc.hip", "t3d_raster.hip", "t3d_gltf.cpp"]
 HEADERS = ["t3d_math.h", "t3d_anim.h", "t3d_internal.h", "t3d_stage.h", os.path.join("..", "..", "include", "t3d.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
