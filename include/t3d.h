@@ -305,6 +305,13 @@ typedef struct {
   uint32_t model_count;
   const t3d_model_desc* models;      /* the `models` NodeIterator, in order */
   int32_t exec_mode;                 /* T3D_EXEC_* */
+  /* 1: every render pair runs ProcessVertices with processOnlyVisible false
+   * (model.go:600-960), as BakeLighting and the debug views do (model.go:1233,
+   * camera.go:2434, 2528, 2581): no alpha-0 early out, and every triangle is
+   * kept (no near/far, screen or backface test, NaN and infinite depth keys
+   * kept; a NaN key, which the reference's bucket index int(NaN) panics on,
+   * sorts into bin 0).  0: Camera.Render's processing. */
+  int32_t process_all_triangles;
 } t3d_render_desc;
 
 /* One render pair = one ProcessVertices + flush (camera.go:2057-2109).  For a

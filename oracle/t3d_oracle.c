@@ -618,6 +618,7 @@ typedef struct {
   v3 forward, right, up;  /* cached at Clear (camera.go:1085-1088) */
   float sf_x, sf_y, sf_tang;
   float aspect;
+  int only_visible;       /* ProcessVertices' processOnlyVisible for this call's pairs (model.go:600) */
 } ocam;
 
 /* ViewMatrix: camera.go:638-648. */
@@ -1333,7 +1334,8 @@ static int render_pair(t3do_ctx* c, const ocam* cam, const t3d_world_desc* world
   const uint32_t vli0 = 3 * out_pair->survivor_count;              /* vertexListIndex when this render starts */
 
   /* ProcessVertices early out: model.go:604-606 */
-  if ((model->color[3] == 0 && is_transparent(model, mat)) || !model->visible) return T3D_OK;
+  const int only_visible = cam->only_visible;
+  if (only_visible && ((model->color[3] == 0 && is_transparent(model, mat)) || !model->visible)) return T3D_OK;
   if (mat && mat->custom_depth_function)
     return set_err(c, "custom depth function"), T3D_ERR_UNSUPPORTED;
 
@@ -1408,13 +1410,13 @@ static int render_pair(t3do_ctx* c, const ocam* cam, const t3d_world_desc* world
       tvp[i].y = s->clip[v].y / w;
       if (s->clip[v].z + 1 >= camNear && s->clip[v].z < camFar) outOfBounds = 0;
     }
-    if (outOfBounds) continue;
-    if ((tvp[0].x < -0.5f && tvp[1].x < -0.5f && tvp[2].x < -0.5f) ||          /* model.go:890-898 */
-        (tvp[0].x > 0.5f && tvp[1].x > 0.5f && tvp[2].x > 0.5f) ||
-        (tvp[0].y < -0.5f && tvp[1].y < -0.5f && tvp[2].y < -0.5f) ||
-        (tvp[0].y > 0.5f && tvp[1].y > 0.5f && tvp[2].y > 0.5f))
+    if (only_visible && outOfBounds) continue;                              /* model.go:883-886 */
+    if (only_visible && ((tvp[0].x < -0.5f && tvp[1].x < -0.5f && tvp[2].x < -0.5f) ||   /* model.go:890-898 */
+                         (tvp[0].x > 0.5f && tvp[1].x > 0.5f && tvp[2].x > 0.5f) ||
+                         (tvp[0].y < -0.5f && tvp[1].y < -0.5f && tvp[2].y < -0.5f) ||
+                         (tvp[0].y > 0.5f && tvp[1].y > 0.5f && tvp[2].y > 0.5f)))
       continue;
-    if (backface) {                                                          /* model.go:902-925 */
+    if (only_visible && backface) {                                          /* model.go:902-925 */
       float n0x = tvp[0].x - tvp[1].x;
       float n0y = tvp[0].y - tvp[1].y;
       float n1x = tvp[1].x - tvp[2].x;
@@ -1435,7 +1437,7 @@ static int render_pair(t3do_ctx* c, const ocam* cam, const t3d_world_desc* world
       float dz = icp.z - ctr.z;
       depth = dx * dx + dy * dy + dz * dz;
     }
-    if (isnan32(depth) || isinf32(depth)) continue;                          /* model.go:957-960 */
+    if (only_visible && (isnan32(depth) || isinf32(depth))) continue;        /* model.go:957-960 */
     if (depth < minDepth) minDepth = depth;
     if (depth > maxDepth) maxDepth = depth;
     bucket_tri[nb] = t;                                                      /* AddTriangle, sortingTriangles.go:32-36 */
@@ -1742,6 +1744,7 @@ int t3do_render(t3do_ctx* c, const t3d_render_desc* desc, t3do_result* res) {
 
   ocam cam;
   cam_setup(&cam, cd);
+  cam.only_visible = !desc->process_all_triangles;
 
   /* a LightVolume must name registered cells (as the product refuses an unknown volume handle) */
   for (uint32_t i = 0; i < desc->light_count; i++)

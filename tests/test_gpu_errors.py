@@ -67,10 +67,6 @@ def test_invalid_render_inputs_are_refused(gpu_ctx, oracle):
         return f
     for owner, part in (len(sk.models) + 5, 0), (1, 0), (0, 99):
         _refused(gpu_ctx, sk, batched(owner, part), A.T3D_ERR_INVALID_ARGUMENT)
-    # more lights on one model than the vertex pass stages (64)
-    many = [W._point("p%d" % i, (0.1 * i, 2, 2), 10) for i in range(70)]
-    lights_wl = W.Workload("many_lights", wl.camera, wl.scene, many, wl.models)
-    _refused(gpu_ctx, lights_wl, lambda r: None, A.T3D_ERR_UNSUPPORTED)
     # the context still renders both scenes exactly
     compare_frames(gpu_ctx.render_workload(wl), oracle.render_workload(wl))
     compare_frames(gpu_ctx.render_workload(sk), oracle.render_workload(sk))

@@ -912,3 +912,68 @@ def test_dynamic_batch_ties_keep_slice_order(oracle):
     owner.dynamic_batch_add(0, a)
     fr = render(oracle, [owner, a, b])
     assert _drawn_x(fr, 0, 2) == [160 * 1 + 320, 160 * -1 + 320]
+
+
+# ---- ProcessVertices with processOnlyVisible false (model.go:600-960) -------
+
+def _unculled_scene():
+    """Five triangles under an orthographic camera looking down -Z (near 1,
+    far 9): 0 visible and front-facing at z -2, 1 the same but wound the
+    other way (back-facing), 2 wholly right of the screen (x > 0.5 in NDC),
+    3 out of the clip-Z range at z -40, 4 visible at z -6.  Backface culling on."""
+    tris = [[(-1, -1, -2), (1, -1, -2), (0, 1, -2)],
+            [(-1, -1, -3), (0, 1, -3), (1, -1, -3)],
+            [(9, 0, -4), (11, 0, -4), (10, 1, -4)],
+            [(-1, -1, -40), (1, -1, -40), (0, 1, -40)],
+            [(-1, -1, -6), (1, -1, -6), (0, 1, -6)]]
+    mesh = tri_mesh(np.array(tris, dtype=f32).reshape(-1, 3))
+    mesh.mesh_parts[0].material = Material("m")
+    return model(mesh, backface=True)
+
+
+def test_unculled_keeps_every_triangle_in_bucket_order():
+    """processOnlyVisible false: no near/far, screen or backface test, so all
+    five triangles are drawn, farthest first (BackToFront over the depth
+    keys |icp - centre|² of a camera at the origin); Camera.Render's
+    processing keeps 0 and 4 only."""
+    from oracle.oracle import Oracle
+    o = Oracle()
+    try:
+        cam = ortho_camera()
+        m = _unculled_scene()
+        fr = render(o, [m], cam=cam)
+        assert list(fr.triangle_ids) == [4, 0]
+        cam.process_only_visible = False
+        fr = render(o, [m], cam=cam)
+        assert fr.pairs[0][3] == 5                       # every triangle survives
+        # keys |centre|²: 1600.1 (3), 116.1 (2, centred at x 10, z -4), 36.1 (4), 9.1 (1), 4.1 (0)
+        assert list(fr.triangle_ids) == [3, 2, 4, 1, 0]
+    finally:
+        o.close()
+
+
+def test_unculled_draws_alpha_zero_models_and_nan_keys():
+    """No alpha-0 early out (model.go:604-606), and a NaN depth key is kept
+    (model.go:957-960 skipped): it does not move the depth range and its bin
+    index is int(NaN) (the reference's slice index panics on it; here it is
+    bin 0, drawn last back to front)."""
+    from oracle.oracle import Oracle
+    o = Oracle()
+    try:
+        cam = ortho_camera()
+        a = _unculled_scene()
+        a.color = np.array([1, 1, 1, 0], dtype=f32)     # alpha 0: transparent, early out when culling
+        assert render(o, [a], cam=cam).survivors == 0
+        cam.process_only_visible = False
+        assert render(o, [a], cam=cam).survivors == 5
+        # a NaN vertex makes triangle 4's key NaN; the others keep their order
+        b = _unculled_scene()
+        pos = b.mesh.positions.copy()
+        pos[12, 2] = np.nan
+        nm = tri_mesh(pos)
+        nm.mesh_parts[0].material = Material("m")
+        fr = render(o, [model(nm, backface=True)], cam=cam)
+        assert fr.pairs[0][3] == 5
+        assert list(fr.triangle_ids) == [3, 2, 1, 0, 4]
+    finally:
+        o.close()

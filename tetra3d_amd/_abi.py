@@ -238,6 +238,7 @@ class RenderDesc(C.Structure):
         ("model_count", C.c_uint32),
         ("models", C.POINTER(ModelDesc)),
         ("exec_mode", C.c_int32),
+        ("process_all_triangles", C.c_int32),
     ]
 
 

@@ -1051,14 +1051,10 @@ t3d_status validate_desc(t3d_ctx* c, const t3d_render_desc* desc) {
       return fail(c, T3D_ERR_NOT_FOUND, "render: %s %u references unknown volume", what, i);
     return T3D_OK;
   };
-  // scene lights a lit pair sees: the world ambient light and the visible lights, cut to MaxLightCount
-  uint32_t n_scene = (world && world->lighting_on && world->ambient_visible) ? 1u : 0u;
   for (uint32_t i = 0; i < desc->light_count; i++) {
     t3d_status st = check_light(desc->lights[i], "light", i);
     if (st) return st;
-    if ((!world || world->lighting_on) && desc->lights[i].visible) n_scene++;
   }
-  if (cd->max_light_count > 0) n_scene = std::min<uint32_t>(n_scene, (uint32_t)cd->max_light_count);
   for (uint32_t mi = 0; mi < desc->model_count; mi++) {
     const t3d_model_desc& md = desc->models[mi];
     if (md.dynamic_batched) {   // DynamicBatchAdd (model.go:217-247): a part of another model of this call
@@ -1119,10 +1115,7 @@ t3d_status validate_desc(t3d_ctx* c, const t3d_render_desc* desc) {
         return fail(c, T3D_ERR_UNSUPPORTED, "render: skinned model %u part %u with an unbillboarded depth mode", mi, p);
       if (mat && (mat->triangle_sort_mode < T3D_SORT_BACK_TO_FRONT || mat->triangle_sort_mode > T3D_SORT_NONE))
         return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: triangle sort mode %d", mat->triangle_sort_mode);
-      const bool lighting = world && world->lighting_on && !md.shadeless && !(mat && mat->shadeless);   // camera.go:1521-1528
-      const uint32_t lcount = md.light_group_active ? md.light_group_count : n_scene;
-      if (lighting && lcount > kMaxLights)
-        return fail(c, T3D_ERR_UNSUPPORTED, "render: %u lights on model %u (max %u)", lcount, mi, kMaxLights);
+
     }
   }
   return T3D_OK;
@@ -1140,7 +1133,7 @@ bool stage_applies(const t3d_ctx* c, const t3d_render_desc* const* descs, uint32
   if (!c->tuning.model_stage) return false;
   for (uint32_t j = 0; j < ndesc; j++) {
     const t3d_render_desc* d = descs[j];
-    if (!d || !d->camera || !d->camera->render_depth) return false;
+    if (!d || !d->camera || !d->camera->render_depth || d->process_all_triangles) return false;
     if (d->model_count && !d->models) return false;
     const V3 cp = row3(d->camera->transform);
     for (uint32_t mi = 0; mi < d->model_count; mi++) {
@@ -1845,7 +1838,8 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
         const t3d_part_range& pr = mesh.parts[ppart];
         bool lighting = false;                                           // camera.go:1521-1528
         if (world) lighting = mat ? (world->lighting_on && !mat->shadeless && !md.shadeless) : (world->lighting_on && !md.shadeless);
-        const bool early_out = (md.color[3] == 0 && is_transparent(md, mat)) || !md.visible;   // model.go:604-606
+        const bool only_visible = !desc->process_all_triangles;
+        const bool early_out = only_visible && ((md.color[3] == 0 && is_transparent(md, mat)) || !md.visible);   // model.go:604-606
 
         // MVP, invertedCamPos, the directional working rotation and the
         // per-light values are computed on the device (k_uniforms)
@@ -1877,6 +1871,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
         if (mesh.uv) flags |= PF_HAS_UV;
         if (mesh.K) flags |= PF_SHAPE_KEYS;
         if (mat && mat->depth_mode == T3D_DEPTH_MODE_UNBILLBOARDED) flags |= PF_UNBB;
+        if (!only_visible) flags |= PF_ALL;
         const uint32_t sort_mode = mat ? (uint32_t)mat->triangle_sort_mode : T3D_SORT_BACK_TO_FRONT;
         flags |= sort_mode << PF_SORT_SHIFT;
         flags |= ((uint32_t)(mat ? mat->lighting_mode : 0) & 3u) << PF_LMODE_SHIFT;
@@ -1898,8 +1893,6 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
             lbase = g->second;
             lcount = md.light_group_count;
           }
-          if (lcount > kMaxLights)   // validate_desc refuses the descriptor first
-            return fail(c, T3D_ERR_UNSUPPORTED, "render: %u lights on one model (max %u)", lcount, kMaxLights);
           P.light_base = (uint32_t)n_plights;
           P.light_count = lcount;
           P.light0 = lbase;

@@ -25,7 +25,7 @@
 namespace t3d {
 
 constexpr uint32_t kBins = 512;        // sortingTriangles.go:166-168
-constexpr uint32_t kMaxLights = 64;    // lights one part may see on the GPU path
+constexpr uint32_t kMaxLights = 64;    // lights of one pair the vertex pass stages in LDS (more are read from HBM)
 constexpr uint32_t kMaxLdsBones = 64;  // bone palettes staged in LDS up to this size
 constexpr uint32_t kScanBlock = 2048;  // elements per scan block (256 threads x 8)
 constexpr uint32_t kMaxTile = 2048;    // triangles per tile (keys of a tile are staged in LDS)
@@ -44,6 +44,7 @@ enum : uint32_t {
   PF_SORT_SHIFT = 8,     // 2 bits: T3D_SORT_*
   PF_LMODE_SHIFT = 10,   // 2 bits: T3D_LIGHTING_*
   PF_LVMODE_SHIFT = 12,  // 2 bits: T3D_LV_*
+  PF_ALL = 1u << 14,     // ProcessVertices with processOnlyVisible false: every triangle kept (model.go:883-960)
 };
 
 // Per render pair: uniforms (camera.go:1507-1601, model.go:600-695) and the

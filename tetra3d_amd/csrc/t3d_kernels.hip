@@ -210,10 +210,12 @@ __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
   __shared__ float4 s_bone_rows[kMaxLdsBones * kBoneRows];
 
   const uint32_t nl = (flags & PF_LIGHTING) ? P.light_count : 0u;
-  for (uint32_t i = threadIdx.x; i < nl; i += blockDim.x) {
-    s_pl[i] = a.plights[P.light_base + i];
-    s_light[i] = a.lights[s_pl[i].light];
-  }
+  const bool lds_lights = nl <= kMaxLights;   // more lights are read from device memory as the loop needs them
+  if (lds_lights)
+    for (uint32_t i = threadIdx.x; i < nl; i += blockDim.x) {
+      s_pl[i] = a.plights[P.light_base + i];
+      s_light[i] = a.lights[s_pl[i].light];
+    }
   const bool skinned = (flags & PF_SKINNED) != 0;
   const bool lds_bones = skinned && P.bone_count <= kMaxLdsBones;
   if (lds_bones) {
@@ -315,9 +317,17 @@ __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
     if (flags & PF_LIGHTING) {
       // vertexLights reset to (0,0,0,1) then every light in order (camera.go:1613-1641)
       float lr = 0, lg = 0, lb = 0;
-      for (uint32_t k = 0; k < nl; k++) {
-        if (!s_pl[k].enabled) continue;
-        light_vertex(s_light[k], s_pl[k], P, flags, vp, vn, lr, lg, lb);
+      if (lds_lights) {
+        for (uint32_t k = 0; k < nl; k++) {
+          if (!s_pl[k].enabled) continue;
+          light_vertex(s_light[k], s_pl[k], P, flags, vp, vn, lr, lg, lb);
+        }
+      } else {
+        for (uint32_t k = 0; k < nl; k++) {
+          const PartLight pl = a.plights[P.light_base + k];
+          if (!pl.enabled) continue;
+          light_vertex(a.lights[pl.light], pl, P, flags, vp, vn, lr, lg, lb);
+        }
       }
       cr *= lr;
       cg *= lg;
@@ -452,8 +462,7 @@ __global__ __launch_bounds__(kPoseThreads) void k_palettes(FrameArgs a) {
 // point-light skip (camera.go:1592, 1625-1629) and the per-object LightVolume
 // cell (light.go:610-628).  The same float32 functions the host would run, so
 // the values are the ones the reference computes on the CPU.  One wavefront
-// per render pair: lane k handles the pair's light k (at most kMaxLights =
-// 64), lane 0 also the pair's own uniforms and the reset of its depth range
+// per render pair: lane k handles the pair's lights k, k + 64, ..., lane 0 also the pair's own uniforms and the reset of its depth range
 // and survivor count, so the latency is one light's, not the sum over lights;
 // all lanes first write the pair's entries of the block->pair maps.
 __global__ __launch_bounds__(64) void k_uniforms(FrameArgs a, uint32_t part0, uint32_t nparts) {
@@ -484,12 +493,12 @@ __global__ __launch_bounds__(64) void k_uniforms(FrameArgs a, uint32_t part0, ui
   M4 dr;
   decompose(inverted(M), dp, ds, dr);
   const V3 is = v3(fdiv(1.0f, ds.x), fdiv(1.0f, ds.y), fdiv(1.0f, ds.z));
-  if (k < nl) {
+  for (uint32_t kk = k; kk < nl; kk += 64) {
     const bool skinned = (flags & PF_SKINNED) != 0;
     const uint32_t lvmode = (flags >> PF_LVMODE_SHIFT) & 3u;
     const V3 modelPos = row3(M);
     PartLight pl{};
-    pl.light = P.light0 + k;
+    pl.light = P.light0 + kk;
     pl.enabled = 1;
     const LightGPU& L = a.lights[pl.light];
     if (L.type == T3D_LIGHT_POINT) {
@@ -511,7 +520,7 @@ __global__ __launch_bounds__(64) void k_uniforms(FrameArgs a, uint32_t part0, ui
         pl.v[2] = cell[2];
       }
     }
-    a.plights[P.light_base + k] = pl;
+    a.plights[P.light_base + kk] = pl;
   }
   if (k == 0) {
     uint32_t* st = a.stats + 4 * (part0 + blockIdx.x);
@@ -566,6 +575,7 @@ __global__ __launch_bounds__(kThreads) void k_cull(FrameArgs a, uint32_t blk0) {
   const uint32_t t1 = min(t0 + a.tile, P.tcount);
   const bool skinned = (flags & PF_SKINNED) != 0;
   const bool backface = (flags & PF_BACKFACE) != 0;
+  const bool all = (flags & PF_ALL) != 0;     // processOnlyVisible false: no test removes a triangle
   const V3 camPos = v3(fc.campos[0], fc.campos[1], fc.campos[2]);
   const V3 icp = v3(P.icp[0], P.icp[1], P.icp[2]);
 
@@ -621,13 +631,13 @@ __global__ __launch_bounds__(kThreads) void k_cull(FrameArgs a, uint32_t blk0) {
         ny[k] = fdiv(c.y, w);
         if (c.z + 1.0f >= fc.near_ && c.z < fc.far_) outOfBounds = false;
       }
-      keep = !outOfBounds;
-      if (keep && ((nx[0] < -0.5f && nx[1] < -0.5f && nx[2] < -0.5f) ||      // model.go:890-898
+      keep = all || !outOfBounds;
+      if (keep && !all && ((nx[0] < -0.5f && nx[1] < -0.5f && nx[2] < -0.5f) ||      // model.go:890-898
                    (nx[0] > 0.5f && nx[1] > 0.5f && nx[2] > 0.5f) ||
                    (ny[0] < -0.5f && ny[1] < -0.5f && ny[2] < -0.5f) ||
                    (ny[0] > 0.5f && ny[1] > 0.5f && ny[2] > 0.5f)))
         keep = false;
-      if (keep && backface) {                                          // model.go:902-925
+      if (keep && !all && backface) {                                  // model.go:902-925
         const float n0x = nx[0] - nx[1];
         const float n0y = ny[0] - ny[1];
         const float n1x = nx[1] - nx[2];
@@ -652,7 +662,7 @@ __global__ __launch_bounds__(kThreads) void k_cull(FrameArgs a, uint32_t blk0) {
           const float dz = icp.z - cz;
           depth = dx * dx + dy * dy + dz * dz;
         }
-        if (isnan(depth) || isinf(depth)) {
+        if (!all && (isnan(depth) || isinf(depth))) {
           keep = false;
           depth = -1.0f;
         }
@@ -756,10 +766,11 @@ __global__ __launch_bounds__(64) void k_hist(FrameArgs a, uint32_t blk0) {
     float d[8];
     for (int j = 0; j < 8; j++) {
       const uint32_t i = ib + j * 64 + threadIdx.x;
-      d[j] = i < n ? keys[i] : -1.0f;
+      d[j] = i < n ? keys[i] : 0.0f;
     }
+    // every survivor counts, a NaN key (kept when processOnlyVisible is false) included
     for (int j = 0; j < 8; j++)
-      if (d[j] >= 0) atomicAdd(&h[visit_bin(d[j], mn, rd, mode)], 1u);
+      if (ib + j * 64 + threadIdx.x < n) atomicAdd(&h[visit_bin(d[j], mn, rd, mode)], 1u);
   }
   __syncthreads();
   uint32_t* col = a.hist + P.hbase + tl;
@@ -1015,18 +1026,19 @@ __global__ __launch_bounds__(256) void k_scatter4(FrameArgs a, uint32_t blk0) {
   float d[kChunks];
   uint4 rec[kChunks];
   uint32_t bin[kChunks];
+  bool in[kChunks];
 #pragma unroll
   for (uint32_t j = 0; j < kChunks; j++) {
     const uint32_t i = (c0 + j) * 64 + lane;
-    const bool in = c0 + j < c1 && i < n;
-    d[j] = in ? a.key[tbase + t0 + i] : -1.0f;
-    rec[j] = in ? rec_in[t0 + i] : make_uint4(0, 0, 0, 0);
+    in[j] = c0 + j < c1 && i < n;
+    d[j] = in[j] ? a.key[tbase + t0 + i] : 0.0f;
+    rec[j] = in[j] ? rec_in[t0 + i] : make_uint4(0, 0, 0, 0);
   }
   __syncthreads();
 #pragma unroll
   for (uint32_t j = 0; j < kChunks; j++) {
     bin[j] = 0xFFFFu;                             // no survivor in this lane
-    if (d[j] >= 0) {
+    if (in[j]) {                                  // a NaN key (processOnlyVisible false) is a survivor too
       bin[j] = visit_bin(d[j], mn, rd, mode);
       atomicAdd(&s_off[wv][bin[j]], 1u);
     }

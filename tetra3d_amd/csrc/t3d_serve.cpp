@@ -210,6 +210,7 @@ t3d_status copy_desc(t3d_ctx* c, const t3d_render_desc* in, Job& j) {
   o.desc.model_count = in->model_count;
   o.desc.models = o.models.data();
   o.desc.exec_mode = T3D_EXEC_BATCHED;
+  o.desc.process_all_triangles = in->process_all_triangles;
   return T3D_OK;
 }
 

@@ -545,6 +545,9 @@ class Camera(Node):
         self.vertex_snapping = f32(0)
         self.max_light_count = 0
         self.max_light_count_focus = None
+        # not Camera state: ProcessVertices' processOnlyVisible for every render pair of the call (true in
+        # Camera.Render, false for BakeLighting and the debug views)
+        self.process_only_visible = True
 
     def desc(self):
         d = A.CameraDesc()
@@ -626,3 +629,4 @@ class RenderInputs:
         self.render.model_count = len(models)
         self.render.models = self.models
         self.render.exec_mode = exec_mode
+        self.render.process_all_triangles = 0 if getattr(camera, "process_only_visible", True) else 1
