@@ -493,8 +493,10 @@ typedef struct {
   uint32_t store_mode;               /* assembly stores: bit 0 = direct per-lane stores instead of
                                         LDS-staged rows, bit 1 = non-temporal (streaming) stores,
                                         bit 2 = stage 32 rows at a time instead of 64, bit 3 =
-                                        software-pipelined gathers, bit 4 = four wavefronts per block
-                                        (0-4, 6, 8, 10, 12, 14, 16, 18, 20, 22) */
+                                        software-pipelined gathers, bit 4 = four wavefronts per block,
+                                        bit 5 = one output stream per block (colour vertices and indices,
+                                        or depth vertices; twice the blocks)
+                                        (0-4, 6, 8, 10, 12, 14, 16, 18, 20, 22, 34, 38, 50, 54) */
   uint32_t assemble_split;           /* assembly blocks per tile: a power of two, tile/split >= 64 */
   uint32_t xcd_swizzle;              /* 1: each XCD takes a contiguous range of a launch's blocks */
   uint32_t vertex_split;             /* 1: clip positions and colours in two arrays instead of one

@@ -16,9 +16,10 @@ SCENES = {
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("store_mode", [0, 1, 2, 3, 4, 6, 8, 10, 12, 14, 16, 18, 20, 22],
+@pytest.mark.parametrize("store_mode", [0, 1, 2, 3, 4, 6, 8, 10, 12, 14, 16, 18, 20, 22, 34, 38, 50, 54],
                          ids=["staged", "direct", "staged_nt", "direct_nt", "half", "half_nt", "pipe", "pipe_nt",
-                              "pipe_half", "pipe_half_nt", "w4", "w4_nt", "w4_half", "w4_half_nt"])
+                              "pipe_half", "pipe_half_nt", "w4", "w4_nt", "w4_half", "w4_half_nt", "one_nt",
+                              "one_half_nt", "w4_one_nt", "w4_one_half_nt"])
 @pytest.mark.parametrize("mode", [A.EXEC_BATCHED, A.EXEC_PLAIN], ids=["batched", "plain"])
 @pytest.mark.parametrize("scene", list(SCENES))
 def test_scene_parity(gpu_ctx, oracle, scene, mode, store_mode):

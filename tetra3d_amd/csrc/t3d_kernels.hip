@@ -1120,10 +1120,18 @@ __device__ __forceinline__ void store1(uint32_t* p, uint32_t v) {
 // lane stores its own 144-byte runs); kNT: non-temporal (streaming) stores.
 // `split` blocks share one tile's slot range (a power of two dividing the
 // tile), so the grid follows the survivors more finely than the sort's tiles.
-template <int kStage, bool kNT, uint32_t kWaves>
+// kOne: each slot range goes to two blocks, adjacent in launch order (so on
+// one XCD, sharing the gathered records in its L2): the first writes the
+// colour vertices and the indices, the second the depth vertices, so every
+// block writes one contiguous output stream.
+template <int kStage, bool kNT, uint32_t kWaves, bool kOne>
 __global__ __launch_bounds__(64 * kWaves) void k_assemble(FrameArgs a, uint32_t blk0, uint32_t split) {
-  const uint32_t li = frame_block(a, FC_TILES, split);
-  if (li == kNoBlock) return;
+  constexpr uint32_t kHalves = kOne ? 2 : 1;
+  const uint32_t lf = frame_block(a, FC_TILES, split * kHalves);
+  if (lf == kNoBlock) return;
+  const bool w_color = !kOne || (lf % kHalves) == 0;   // block-uniform
+  const bool w_depth = !kOne || (lf % kHalves) == 1;
+  const uint32_t li = lf / kHalves;
   const uint32_t b = blk0 + li / split;
   const uint32_t p = a.tile_part[b];
   const PartGPU P = a.parts[p];        // by value: read once, before any store
@@ -1183,7 +1191,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_assemble(FrameArgs a, uint32_t 
       float2 uv[3];
       for (int vi = 0; vi < 3; vi++) {
         c[vi] = vclip[(size_t)ov[vi] * vstride];
-        col[vi] = vcol[(size_t)ov[vi] * vstride];
+        col[vi] = w_color ? vcol[(size_t)ov[vi] * vstride] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         uv[vi] = has_uv ? reinterpret_cast<const float2*>(uvp)[ov[vi] - vbase + vmin] : make_float2(0.0f, 0.0f);
       }
       for (int vi = 0; vi < 3; vi++) {
@@ -1203,7 +1211,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_assemble(FrameArgs a, uint32_t 
           uvV = (1.0f - V) * src_h;
         }
         float depth = 0.0f;
-        if (rdepth) {                                             // camera.go:1765-1784
+        if (rdepth && w_depth) {                                  // camera.go:1765-1784
           const float zd = unbb ? vud[ov[vi]] : c[vi].z;          // DepthModeUnbillboarded: the unaltered Z
           depth = fdiv(zd + dmp, spread);
           if (depth < 0) depth = 0;
@@ -1221,42 +1229,56 @@ __global__ __launch_bounds__(64 * kWaves) void k_assemble(FrameArgs a, uint32_t 
         const uint32_t rows = wcount > r0 ? min(kRows, wcount - r0) : 0u;
         const bool mine = active && lane >= r0 && lane < r0 + kRows;
         const uint32_t row = lane - r0;
-        if (mine)
-          for (int q = 0; q < 9; q++) stage[row * 9 + q] = cq[q];
-        __syncthreads();
-        // a fixed number of (predicated) stores per lane, so the compiler
-        // can count them and wait only for the loads it needs
+        if (w_color) {
+          if (mine)
+            for (int q = 0; q < 9; q++) stage[row * 9 + q] = cq[q];
+          __syncthreads();
+          // a fixed number of (predicated) stores per lane, so the compiler
+          // can count them and wait only for the loads it needs
 #pragma unroll
-        for (uint32_t q = 0; q < (kRows * 9 + 63) / 64; q++) {
-          const uint32_t f = q * 64 + lane;
-          if (f < rows * 9) store4<kNT>(&out_c[(size_t)(wk0 + r0) * 9 + f], stage[f]);
+          for (uint32_t q = 0; q < (kRows * 9 + 63) / 64; q++) {
+            const uint32_t f = q * 64 + lane;
+            if (f < rows * 9) store4<kNT>(&out_c[(size_t)(wk0 + r0) * 9 + f], stage[f]);
+          }
+          __syncthreads();
         }
-        __syncthreads();
-        if (mine)
-          for (int vi = 0; vi < 3; vi++) stage[row * 9 + 3 * vi + 1] = make_float4(dd[vi], dd[vi], dd[vi], da);
-        __syncthreads();
+        if (w_depth) {
+          if (mine)
+            for (int vi = 0; vi < 3; vi++) {
+              if (!w_color) {                                     // rows not staged for the colour list
+                stage[row * 9 + 3 * vi + 0] = cq[3 * vi + 0];
+                stage[row * 9 + 3 * vi + 2] = cq[3 * vi + 2];
+              }
+              stage[row * 9 + 3 * vi + 1] = make_float4(dd[vi], dd[vi], dd[vi], da);
+            }
+          __syncthreads();
 #pragma unroll
-        for (uint32_t q = 0; q < (kRows * 9 + 63) / 64; q++) {
-          const uint32_t f = q * 64 + lane;
-          if (f < rows * 9) store4<kNT>(&out_d[(size_t)(wk0 + r0) * 9 + f], stage[f]);
+          for (uint32_t q = 0; q < (kRows * 9 + 63) / 64; q++) {
+            const uint32_t f = q * 64 + lane;
+            if (f < rows * 9) store4<kNT>(&out_d[(size_t)(wk0 + r0) * 9 + f], stage[f]);
+          }
+          __syncthreads();
         }
-        __syncthreads();
       }
     } else if (active) {
       float4* oc = out_c + (size_t)k * 9;
       float4* od = out_d + (size_t)k * 9;
-      for (int q = 0; q < 9; q++) store4<kNT>(&oc[q], cq[q]);
-      for (int vi = 0; vi < 3; vi++) {
-        store4<kNT>(&od[3 * vi + 0], cq[3 * vi + 0]);
-        store4<kNT>(&od[3 * vi + 1], make_float4(dd[vi], dd[vi], dd[vi], da));
-        store4<kNT>(&od[3 * vi + 2], cq[3 * vi + 2]);
-      }
+      if (w_color)
+        for (int q = 0; q < 9; q++) store4<kNT>(&oc[q], cq[q]);
+      if (w_depth)
+        for (int vi = 0; vi < 3; vi++) {
+          store4<kNT>(&od[3 * vi + 0], cq[3 * vi + 0]);
+          store4<kNT>(&od[3 * vi + 1], make_float4(dd[vi], dd[vi], dd[vi], da));
+          store4<kNT>(&od[3 * vi + 2], cq[3 * vi + 2]);
+        }
     }
     // indices: flush-local iota, as the reference's indexList (camera.go:1816)
+    if (w_color) {
 #pragma unroll
-    for (uint32_t q = 0; q < 3; q++) {
-      const uint32_t f = q * 64 + lane;
-      if (f < wcount * 3) store1<kNT>(&out_i[(size_t)wk0 * 3 + f], idx_base + wk0 * 3 + f);
+      for (uint32_t q = 0; q < 3; q++) {
+        const uint32_t f = q * 64 + lane;
+        if (f < wcount * 3) store1<kNT>(&out_i[(size_t)wk0 * 3 + f], idx_base + wk0 * 3 + f);
+      }
     }
   }
 }
@@ -1581,23 +1603,28 @@ hipError_t launch_assemble(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uin
                            hipStream_t s) {
   if (!nblk) return hipSuccess;
   if (split == 0 || (split & (split - 1)) || a.tile % split) return hipErrorInvalidValue;
-  const dim3 g(nblk * split);
+  const dim3 g(nblk * split), g2(2 * nblk * split);
   switch (mode) {
     case 8: hipLaunchKernelGGL((k_assemble_pipe<64, false>), g, dim3(64), 0, s, a, blk0, split); break;
     case 10: hipLaunchKernelGGL((k_assemble_pipe<64, true>), g, dim3(64), 0, s, a, blk0, split); break;
     case 12: hipLaunchKernelGGL((k_assemble_pipe<32, false>), g, dim3(64), 0, s, a, blk0, split); break;
     case 14: hipLaunchKernelGGL((k_assemble_pipe<32, true>), g, dim3(64), 0, s, a, blk0, split); break;
-    case 0: hipLaunchKernelGGL((k_assemble<1, false, 1>), g, dim3(64), 0, s, a, blk0, split); break;
-    case 1: hipLaunchKernelGGL((k_assemble<0, false, 1>), g, dim3(64), 0, s, a, blk0, split); break;
-    case 2: hipLaunchKernelGGL((k_assemble<1, true, 1>), g, dim3(64), 0, s, a, blk0, split); break;
-    case 3: hipLaunchKernelGGL((k_assemble<0, true, 1>), g, dim3(64), 0, s, a, blk0, split); break;
-    case 4: hipLaunchKernelGGL((k_assemble<2, false, 1>), g, dim3(64), 0, s, a, blk0, split); break;
-    case 6: hipLaunchKernelGGL((k_assemble<2, true, 1>), g, dim3(64), 0, s, a, blk0, split); break;
+    case 0: hipLaunchKernelGGL((k_assemble<1, false, 1, false>), g, dim3(64), 0, s, a, blk0, split); break;
+    case 1: hipLaunchKernelGGL((k_assemble<0, false, 1, false>), g, dim3(64), 0, s, a, blk0, split); break;
+    case 2: hipLaunchKernelGGL((k_assemble<1, true, 1, false>), g, dim3(64), 0, s, a, blk0, split); break;
+    case 3: hipLaunchKernelGGL((k_assemble<0, true, 1, false>), g, dim3(64), 0, s, a, blk0, split); break;
+    case 4: hipLaunchKernelGGL((k_assemble<2, false, 1, false>), g, dim3(64), 0, s, a, blk0, split); break;
+    case 6: hipLaunchKernelGGL((k_assemble<2, true, 1, false>), g, dim3(64), 0, s, a, blk0, split); break;
     // bit 4: four wavefronts per block, each staging its own rows
-    case 16: hipLaunchKernelGGL((k_assemble<1, false, 4>), g, dim3(256), 0, s, a, blk0, split); break;
-    case 18: hipLaunchKernelGGL((k_assemble<1, true, 4>), g, dim3(256), 0, s, a, blk0, split); break;
-    case 20: hipLaunchKernelGGL((k_assemble<2, false, 4>), g, dim3(256), 0, s, a, blk0, split); break;
-    case 22: hipLaunchKernelGGL((k_assemble<2, true, 4>), g, dim3(256), 0, s, a, blk0, split); break;
+    case 16: hipLaunchKernelGGL((k_assemble<1, false, 4, false>), g, dim3(256), 0, s, a, blk0, split); break;
+    case 18: hipLaunchKernelGGL((k_assemble<1, true, 4, false>), g, dim3(256), 0, s, a, blk0, split); break;
+    case 20: hipLaunchKernelGGL((k_assemble<2, false, 4, false>), g, dim3(256), 0, s, a, blk0, split); break;
+    case 22: hipLaunchKernelGGL((k_assemble<2, true, 4, false>), g, dim3(256), 0, s, a, blk0, split); break;
+    // bit 5: one output stream per block (colour vertices and indices, or depth vertices), twice the blocks
+    case 34: hipLaunchKernelGGL((k_assemble<1, true, 1, true>), g2, dim3(64), 0, s, a, blk0, split); break;
+    case 38: hipLaunchKernelGGL((k_assemble<2, true, 1, true>), g2, dim3(64), 0, s, a, blk0, split); break;
+    case 50: hipLaunchKernelGGL((k_assemble<1, true, 4, true>), g2, dim3(256), 0, s, a, blk0, split); break;
+    case 54: hipLaunchKernelGGL((k_assemble<2, true, 4, true>), g2, dim3(256), 0, s, a, blk0, split); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
