@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define T3D_ABI_VERSION 1
+#define T3D_ABI_VERSION 2
 
 typedef int32_t t3d_status;
 enum {
@@ -155,7 +155,7 @@ typedef struct {
 } t3d_material_desc;
 
 /* A light as seen by Camera.Render (light.go:12-30 and the concrete types). */
-typedef struct {
+typedef struct t3d_light_desc {
   int32_t type;                      /* T3D_LIGHT_* */
   int32_t visible;                   /* Node IsVisible() */
   float color[3];                    /* LightBase.color RGB */
@@ -171,6 +171,15 @@ typedef struct {
   int32_t cell_count[3];
   float offset[3];                   /* LightVolumeOffset */
   float light_bias[3];               /* LightVolumeLightBias */
+  /* LightVolume.useLightsRenderSet (light.go:392, 566-580, 1508-1513): the
+   * lights its cells' useLights name, as t3d_volume_set_cell_lights indexes
+   * them.  In the per-vertex shading modes each of a cell's lights runs its
+   * VertexLight before the volume adds its own colour (light.go:650-654),
+   * whatever its visibility and range.  Ambient, point and directional
+   * lights; read only when the volume has cell light lists. */
+  const struct t3d_light_desc* use_lights;
+  uint32_t use_light_count;
+  int32_t pad_use;
 } t3d_light_desc;
 
 /* One bone of an armature as Node state (node.go:572-608): local position,
@@ -353,6 +362,15 @@ t3d_status t3d_mesh_free(t3d_ctx* ctx, uint64_t handle);
 t3d_status t3d_volume_upload(t3d_ctx* ctx, const float* cells_rgba, uint32_t cell_count, uint64_t* out_handle);
 /* T3D_ERR_BUSY while a submitted job still references the volume. */
 t3d_status t3d_volume_free(t3d_ctx* ctx, uint64_t handle);
+/* Every cell's useLights list at once (LightVolumeCell.SetUseLights and
+ * ClearUseLights, light.go:1508-1518): cell i names the lights
+ * light_index[cell_first[i] .. cell_first[i+1]), in order and repeats kept,
+ * each an index into the volume light's t3d_light_desc.use_lights.
+ * cell_first holds cell_count + 1 non-decreasing entries from 0; a null
+ * cell_first clears every list.  T3D_ERR_BUSY while a submitted job still
+ * references the volume. */
+t3d_status t3d_volume_set_cell_lights(t3d_ctx* ctx, uint64_t handle, const uint32_t* cell_first,
+                                      const uint32_t* light_index);
 
 /* An Animation's channels (animation.go:188-260), resident on the device;
  * replaces the per-frame AnimationTrack.ValueAsVector / ValueAsQuaternion

@@ -53,6 +53,7 @@ def lib():
         L.t3do_destroy.argtypes = [C.c_void_p]
         L.t3do_mesh_register.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(A.MeshDesc)]
         L.t3do_volume_register.argtypes = [C.c_void_p, C.c_uint64, FP, C.c_uint32]
+        L.t3do_volume_set_cell_lights.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.t3do_render.argtypes = [C.c_void_p, C.POINTER(A.RenderDesc), C.POINTER(OracleResult)]
         L.t3do_result_free.argtypes = [C.POINTER(OracleResult)]
         L.t3do_last_error.restype = C.c_char_p
@@ -172,6 +173,10 @@ class Oracle:
     def register_volume(self, vol, handle):
         cells = np.ascontiguousarray(vol.cells, dtype=np.float32)
         self.L.t3do_volume_register(self.ctx, handle, cells.ctypes.data_as(C.POINTER(C.c_float)), len(cells))
+        first, index = vol.cell_light_lists()
+        if first is not None:
+            U = C.POINTER(C.c_uint32)
+            self.L.t3do_volume_set_cell_lights(self.ctx, handle, first.ctypes.data_as(U), index.ctypes.data_as(U))
 
     def register_animation(self, anim, handle):
         arr, keep = anim.channels_desc()

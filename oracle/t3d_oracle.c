@@ -443,6 +443,9 @@ typedef struct {
   uint64_t handle;
   float* cells;
   uint32_t n;
+  uint32_t* first;        /* cell light lists (LightVolumeCell.useLights): [n + 1] offsets, NULL for none */
+  uint32_t* index;
+  uint32_t need;          /* 1 + the largest light index the lists name */
 } ovolume;
 
 typedef struct {
@@ -488,7 +491,11 @@ static void omesh_free(omesh* m) {
 void t3do_destroy(t3do_ctx* c) {
   if (!c) return;
   for (int i = 0; i < c->nmeshes; i++) omesh_free(&c->meshes[i]);
-  for (int i = 0; i < c->nvols; i++) free(c->vols[i].cells);
+  for (int i = 0; i < c->nvols; i++) {
+    free(c->vols[i].cells);
+    free(c->vols[i].first);
+    free(c->vols[i].index);
+  }
   for (int i = 0; i < c->nanims; i++) {
     free(c->anims[i].ch);
     free(c->anims[i].data);
@@ -547,14 +554,47 @@ int t3do_mesh_register(t3do_ctx* c, uint64_t handle, const t3d_mesh_desc* d) {
 }
 
 int t3do_volume_register(t3do_ctx* c, uint64_t handle, const float* cells, uint32_t n) {
-  if (c->nvols == c->cap_vols) {
-    c->cap_vols = c->cap_vols ? 2 * c->cap_vols : 8;
-    c->vols = (ovolume*)realloc(c->vols, sizeof(ovolume) * c->cap_vols);
+  ovolume* v = NULL;
+  for (int i = 0; i < c->nvols && !v; i++)   /* registering a handle again replaces its cells and lists */
+    if (c->vols[i].handle == handle) {
+      v = &c->vols[i];
+      free(v->cells);
+      free(v->first);
+      free(v->index);
+    }
+  if (!v) {
+    if (c->nvols == c->cap_vols) {
+      c->cap_vols = c->cap_vols ? 2 * c->cap_vols : 8;
+      c->vols = (ovolume*)realloc(c->vols, sizeof(ovolume) * c->cap_vols);
+    }
+    v = &c->vols[c->nvols++];
   }
-  ovolume* v = &c->vols[c->nvols++];
   v->handle = handle;
   v->n = n;
   v->cells = (float*)dup_mem(cells, (size_t)n * 16);
+  v->first = NULL;
+  v->index = NULL;
+  v->need = 0;
+  return T3D_OK;
+}
+
+static ovolume* find_vol(t3do_ctx* c, uint64_t h);
+
+/* LightVolumeCell.SetUseLights / ClearUseLights of every cell (light.go:1508-1518). */
+int t3do_volume_set_cell_lights(t3do_ctx* c, uint64_t handle, const uint32_t* first, const uint32_t* index) {
+  ovolume* v = find_vol(c, handle);
+  if (!v) return set_err(c, "volume_set_cell_lights: unknown handle"), T3D_ERR_NOT_FOUND;
+  free(v->first);
+  free(v->index);
+  v->first = NULL;
+  v->index = NULL;
+  v->need = 0;
+  if (!first) return T3D_OK;
+  uint32_t total = first[v->n];
+  v->first = (uint32_t*)dup_mem(first, sizeof(uint32_t) * ((size_t)v->n + 1));
+  v->index = (uint32_t*)dup_mem(index, sizeof(uint32_t) * (total ? total : 1));
+  for (uint32_t j = 0; j < total; j++)
+    if (index[j] + 1 > v->need) v->need = index[j] + 1;
   return T3D_OK;
 }
 
@@ -716,7 +756,7 @@ static v3 inverted_pos(const m4* model_transform, v3 world) {
 /* lights (light.go)                                                         */
 /* ------------------------------------------------------------------------ */
 
-typedef struct {
+typedef struct olight_s {
   int type;
   int visible;
   float color[3];
@@ -733,6 +773,11 @@ typedef struct {
   uint32_t ncells;
   v3 vmin, vmax, cell, offset, bias;
   int cx, cy, cz;
+  const uint32_t* cfirst;          /* cell light lists of the volume (NULL: none) */
+  const uint32_t* cindex;
+  struct olight_s* use;            /* useLightsRenderSet, light.go:392 (owned) */
+  const t3d_light_desc* use_descs;
+  int nuse;
 } olight;
 
 static void light_from_desc(t3do_ctx* c, olight* l, const t3d_light_desc* d) {
@@ -758,7 +803,21 @@ static void light_from_desc(t3do_ctx* c, olight* l, const t3d_light_desc* d) {
     l->cx = d->cell_count[0];
     l->cy = d->cell_count[1];
     l->cz = d->cell_count[2];
+    if (v && v->first) {
+      l->cfirst = v->first;
+      l->cindex = v->index;
+      l->nuse = (int)d->use_light_count;
+      l->use_descs = d->use_lights;
+      l->use = (olight*)calloc(l->nuse + 1, sizeof(olight));
+      for (int i = 0; i < l->nuse; i++) light_from_desc(c, &l->use[i], &d->use_lights[i]);
+    }
   }
+}
+
+static void light_release(olight* l) {
+  for (int i = 0; l->use && i < l->nuse; i++) light_release(&l->use[i]);
+  free(l->use);
+  l->use = NULL;
 }
 
 /* beginRender: light.go:110-112 (ambient), 322-324 (directional). */
@@ -769,6 +828,8 @@ static void light_begin_render(olight* l, const t3d_light_desc* d) {
     l->amb[2] = l->color[2] * l->energy;
   } else if (l->type == T3D_LIGHT_DIRECTIONAL) {
     l->forward = m4_forward(&d->world_rotation);
+  } else if (l->type == T3D_LIGHT_VOLUME) {                          /* light.go:566-571 */
+    for (int i = 0; i < l->nuse; i++) light_begin_render(&l->use[i], &l->use_descs[i]);
   }
 }
 
@@ -782,6 +843,8 @@ static void light_begin_model(olight* l, const t3d_model_desc* model) {
       m4 inv = m4_inverted(&model->world_rotation);
       l->working_rot = m4_transposed(&inv);
     }
+  } else if (l->type == T3D_LIGHT_VOLUME) {                          /* light.go:574-580 */
+    for (int i = 0; i < l->nuse; i++) light_begin_model(&l->use[i], model);
   }
 }
 
@@ -795,14 +858,19 @@ typedef struct {
 } oscratch;
 
 /* getLightCell with convertToXYZIndices: light.go:717-724, 1267-1272. */
-static const float* volume_cell(const olight* l, v3 p) {
+static int64_t volume_cell_index(const olight* l, v3 p) {
   int64_t x = go_int(round32((p.x - l->offset.x - l->vmin.x - (l->cell.x / 2)) / l->cell.x));
   int64_t y = go_int(round32((p.y - l->offset.y - l->vmin.y - (l->cell.y / 2)) / l->cell.y));
   int64_t z = go_int(round32((p.z - l->offset.z - l->vmin.z - (l->cell.z / 2)) / l->cell.z));
-  if (x < 0 || y < 0 || z < 0 || x >= l->cx || y >= l->cy || z >= l->cz) return NULL;
+  if (x < 0 || y < 0 || z < 0 || x >= l->cx || y >= l->cy || z >= l->cz) return -1;
   int64_t i = x + (y * l->cx) + (z * l->cx * l->cy);
-  if (!l->cells || i >= (int64_t)l->ncells) return NULL;
-  return l->cells + 4 * i;
+  if (!l->cells || i >= (int64_t)l->ncells) return -1;
+  return i;
+}
+
+static const float* volume_cell(const olight* l, v3 p) {
+  int64_t i = volume_cell_index(l, p);
+  return i < 0 ? NULL : l->cells + 4 * i;
 }
 
 static int dims_contains(const olight* l, v3 p) {                  /* mesh.go:80-82 */
@@ -877,12 +945,15 @@ static void vertex_light(const olight* l, uint32_t v, const t3d_model_desc* mode
       }
       if (!dims_contains(l, vpos)) return;
       cc[0] = cc[1] = cc[2] = 0;
-      const float* cell = volume_cell(l, mode == T3D_LV_PER_VERTEX_WITH_NORMAL ? vpn : vpos);
-      if (cell) {
+      int64_t ci = volume_cell_index(l, mode == T3D_LV_PER_VERTEX_WITH_NORMAL ? vpn : vpos);
+      if (ci >= 0) {
+        const float* cell = l->cells + 4 * ci;
         cc[0] = cell[0];
         cc[1] = cell[1];
         cc[2] = cell[2];
         cc[3] = cell[3];
+        for (uint32_t j = l->cfirst ? l->cfirst[ci] : 0; l->cfirst && j < l->cfirst[ci + 1]; j++)  /* light.go:650-653 */
+          vertex_light(&l->use[l->cindex[j]], v, model, mat, s, out);
       }
     }
     out[0] += l->color[0] * cc[0] * l->energy;
@@ -1678,6 +1749,7 @@ static int render_model_part(t3do_ctx* c, const t3d_render_desc* desc, const oca
   int rc = render_pair(c, cam, world, model, mi, part, m, lights, nl, glights, model->light_group, ng, s, ob, res,
                        pair_cap, flush);
   free(pal);
+  for (int g = 0; g < ng; g++) light_release(&glights[g]);
   free(glights);
   return rc;
 }
@@ -1735,6 +1807,20 @@ static int render_batch(t3do_ctx* c, const t3d_render_desc* desc, const ocam* ca
   return rc;
 }
 
+/* A volume light's cells and cell lights, a few levels down (a volume may name volumes). */
+static int check_volume_light(t3do_ctx* c, const t3d_light_desc* l, int depth) {
+  if (l->type != T3D_LIGHT_VOLUME) return T3D_OK;
+  ovolume* v = find_vol(c, l->volume);
+  if (!v) return T3D_ERR_NOT_FOUND;
+  if (!v->first) return T3D_OK;
+  if (depth >= 8 || l->use_light_count < v->need || (v->need && !l->use_lights)) return T3D_ERR_INVALID_ARGUMENT;
+  for (uint32_t j = 0; j < l->use_light_count; j++) {
+    int r = check_volume_light(c, &l->use_lights[j], depth + 1);
+    if (r) return r;
+  }
+  return T3D_OK;
+}
+
 int t3do_render(t3do_ctx* c, const t3d_render_desc* desc, t3do_result* res) {
   memset(res, 0, sizeof *res);
   if (!desc || !desc->camera) return set_err(c, "render: missing camera");
@@ -1746,15 +1832,18 @@ int t3do_render(t3do_ctx* c, const t3d_render_desc* desc, t3do_result* res) {
   cam_setup(&cam, cd);
   cam.only_visible = !desc->process_all_triangles;
 
-  /* a LightVolume must name registered cells (as the product refuses an unknown volume handle) */
-  for (uint32_t i = 0; i < desc->light_count; i++)
-    if (desc->lights[i].type == T3D_LIGHT_VOLUME && !find_vol(c, desc->lights[i].volume))
-      return set_err(c, "render: light %u names an unregistered volume", i), T3D_ERR_NOT_FOUND;
+  /* a LightVolume must name registered cells (as the product refuses an unknown volume handle), and the lights
+   * its cells name must be in its use_lights */
+  for (uint32_t i = 0; i < desc->light_count; i++) {
+    int vr = check_volume_light(c, &desc->lights[i], 0);
+    if (vr) return set_err(c, "render: light %u: %s", i, vr == T3D_ERR_NOT_FOUND ? "unregistered volume" : "cell lights"), vr;
+  }
   for (uint32_t mi = 0; mi < desc->model_count; mi++) {
     const t3d_model_desc* md = &desc->models[mi];
-    for (uint32_t k = 0; md->light_group_active && md->light_group && k < md->light_group_count; k++)
-      if (md->light_group[k].type == T3D_LIGHT_VOLUME && !find_vol(c, md->light_group[k].volume))
-        return set_err(c, "render: model %u light group names an unregistered volume", mi), T3D_ERR_NOT_FOUND;
+    for (uint32_t k = 0; md->light_group_active && md->light_group && k < md->light_group_count; k++) {
+      int vr = check_volume_light(c, &md->light_group[k], 0);
+      if (vr) return set_err(c, "render: model %u light group: %s", mi, vr == T3D_ERR_NOT_FOUND ? "unregistered volume" : "cell lights"), vr;
+    }
   }
 
   /* scene light list: camera.go:1275-1303 */
@@ -1780,6 +1869,7 @@ int t3do_render(t3do_ctx* c, const t3d_render_desc* desc, t3do_result* res) {
       nl++;
     }
   }
+  const int nl_built = nl;
 
   /* render pairs: camera.go:1343-1457 */
   opair* pairs = (opair*)malloc(sizeof(opair) * 16);
@@ -1941,6 +2031,7 @@ done:
   free(pairs);
   free(solids);
   free(transp);
+  for (int i = 0; i < nl_built; i++) light_release(&lights[i]);
   free(lights);
   if (rc != T3D_OK) t3do_result_free(res);
   return rc;

@@ -40,6 +40,9 @@ void t3do_destroy(t3do_ctx* ctx);
 /* Register a mesh / light volume under a caller-chosen handle (copies data). */
 int t3do_mesh_register(t3do_ctx* ctx, uint64_t handle, const t3d_mesh_desc* desc);
 int t3do_volume_register(t3do_ctx* ctx, uint64_t handle, const float* cells_rgba, uint32_t cell_count);
+/* Every cell's useLights (LightVolumeCell.SetUseLights, light.go:1508-1513) as t3d_volume_set_cell_lights
+ * takes them; a NULL cell_first clears them. */
+int t3do_volume_set_cell_lights(t3do_ctx* ctx, uint64_t handle, const uint32_t* cell_first, const uint32_t* light_index);
 int t3do_animation_register(t3do_ctx* ctx, uint64_t handle, const t3d_anim_channel* channels, uint32_t channel_count);
 /* Camera.Render restated.  Fills *out (free with t3do_result_free). */
 int t3do_render(t3do_ctx* ctx, const t3d_render_desc* desc, t3do_result* out);

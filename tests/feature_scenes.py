@@ -193,7 +193,73 @@ def feature_scene(seed):
         others = [m for m in models if m is not owner]
         for i in extra.choice(len(others), size=int(extra.integers(1, min(len(others), 6) + 1)), replace=False):
             owner.dynamic_batch_add(int(extra.integers(0, len(owner.mesh.mesh_parts))), others[int(i)])
+    cells = np.random.default_rng(20_000 + seed)                     # volume cell lights (light.go:650-653)
+    for l in lights:
+        if isinstance(l, LightVolume) and cells.random() < 0.6:
+            add_cell_lights(cells, l, lo, hi)
     return W.Workload("features_%d" % seed, cam, Scene("features", world), lights, models)
+
+
+def add_cell_lights(rng, vol, lo, hi, n_lights=None, frac=0.3):
+    """Point, directional and ambient lights (some hidden, which cell lights
+    ignore) named by about `frac` of the volume's cells, one to three each,
+    repeats allowed (LightVolumeCell.SetUseLights, light.go:1508-1513)."""
+    nested = []
+    for k in range(n_lights or int(rng.integers(1, 5))):
+        t = rng.choice(["point", "point", "dir", "amb"])
+        if t == "point":
+            l = PointLight("%s.p%d" % (vol.name, k), *rng.uniform(0.2, 1, 3), float(rng.uniform(0.5, 2)))
+            l.set_local_position(*rng.uniform(lo - 2, hi + 2))
+            l.range = f32(rng.choice([0.0, rng.uniform(1, 6)]))
+        elif t == "dir":
+            l = DirectionalLight("%s.d%d" % (vol.name, k), *rng.uniform(0.2, 1, 3), float(rng.uniform(0.5, 1.5)))
+            l.set_local_rotation(_rotation(rng))
+        else:
+            l = AmbientLight("%s.a%d" % (vol.name, k), *rng.uniform(0.2, 1, 3), float(rng.uniform(0.1, 0.5)))
+        l.visible = bool(rng.random() < 0.7)
+        nested.append(l)
+    for c in np.flatnonzero(rng.random(len(vol.cells)) < frac):
+        vol.set_use_lights(int(c), *[nested[int(i)] for i in rng.integers(0, len(nested), int(rng.integers(1, 4)))])
+    return nested
+
+
+def cell_lights_scene(seed, many=False):
+    """Models inside a LightVolume whose cells name lights, in every shading
+    mode, rigid and skinned, the volume in the scene lights and in a
+    LightGroup; `many` gives the volume 80 cell lights, so a pair's light list
+    passes what the vertex pass stages in LDS."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.array([-6, -4, -14], dtype=f32), np.array([6, 4, -3], dtype=f32)
+    cam = Camera("cam", 640, 360)
+    cam.far = f32(60)
+    world = World()
+    world.ambient_light.energy = f32(0.2)
+    vol = LightVolume("vol", lo - 1, hi + 1, (2.0, 2.0, 2.0))
+    vol.cells[:, :3] = rng.random((len(vol.cells), 3)).astype(f32)
+    add_cell_lights(rng, vol, lo, hi, n_lights=80 if many else 6, frac=0.8)
+    sun = DirectionalLight("sun", 0.9, 0.8, 0.7, 1.0)
+    sun.set_local_rotation(_rotation(rng))
+    lights = [sun, vol]
+    models = []
+    for i in range(8):
+        kind = ["sphere", "grid", "cube", "tri"][i % 4]
+        mesh = _mesh(rng, kind, "m%d" % i)
+        for p in mesh.mesh_parts:
+            p.material = p.material or Material("mat%d" % i)
+            _material(rng, p.material)
+            p.material.light_volume_shading_mode = [A.LV_PER_VERTEX_WITHOUT_NORMAL, A.LV_PER_VERTEX_WITH_NORMAL,
+                                                    A.LV_PER_OBJECT, A.LV_OFF][i % 4 if i < 4 else int(rng.integers(0, 4))]
+            p.material.blend_source_over = True
+        m = Model("model%d" % i, mesh)
+        m.set_local_position(*rng.uniform(lo, hi))
+        m.set_local_rotation(_rotation(rng))
+        m.frustum_culling = False
+        if kind in ("sphere", "grid") and i >= 4:
+            _skin(rng, m)
+        if i % 3 == 2:
+            m.light_group = [vol, sun] if i % 2 else [vol]
+        models.append(m)
+    return W.Workload("cell_lights_%d" % seed, cam, Scene("cells", world), lights, models)
 
 
 def pair_order_scene(seed, n_models=24):

@@ -73,6 +73,50 @@ def test_invalid_render_inputs_are_refused(gpu_ctx, oracle):
 
 
 @pytest.mark.gpu
+def test_invalid_cell_lights_are_refused(gpu_ctx, oracle):
+    """Cell light lists that do not fit the volume, a volume light with fewer
+    use_lights than its cells name, and a LightVolume among a volume's cell
+    lights are refused; the scene renders afterwards."""
+    from tests.feature_scenes import cell_lights_scene
+    from tetra3d_amd.scene import LightVolume
+    wl = cell_lights_scene(1)
+    gpu_ctx.prepare(wl)
+    vol = wl.lights[1]
+    print("invalid cell lights", flush=True)
+    U = C.POINTER(C.c_uint32)
+    n = len(vol.cells)
+    bad_first = np.zeros(n + 1, dtype=np.uint32)
+    bad_first[0] = 1
+    idx = np.zeros(4, dtype=np.uint32)
+    assert gpu_ctx.L.t3d_volume_set_cell_lights(gpu_ctx.ctx, vol.handle, bad_first.ctypes.data_as(U),
+                                                idx.ctypes.data_as(U)) == A.T3D_ERR_INVALID_ARGUMENT
+    dec = np.zeros(n + 1, dtype=np.uint32)
+    dec[1] = 3
+    assert gpu_ctx.L.t3d_volume_set_cell_lights(gpu_ctx.ctx, vol.handle, dec.ctypes.data_as(U),
+                                                idx.ctypes.data_as(U)) == A.T3D_ERR_INVALID_ARGUMENT
+    assert gpu_ctx.L.t3d_volume_set_cell_lights(gpu_ctx.ctx, 0x7FFFFFFF, None, None) == A.T3D_ERR_NOT_FOUND
+
+    def fewer(r):
+        r.lights[1].use_light_count = 1
+    _refused(gpu_ctx, wl, fewer, A.T3D_ERR_INVALID_ARGUMENT)
+
+    inner = LightVolume("inner", (-1, -1, -1), (1, 1, 1), (1, 1, 1))
+    gpu_ctx.upload_volume(inner)
+    keep = []
+
+    def nested_volume(r):
+        arr = (A.LightDesc * r.lights[1].use_light_count)()
+        for k in range(r.lights[1].use_light_count):
+            arr[k] = r.lights[1].use_lights[k]
+        arr[0] = inner.desc()
+        keep.append(arr)
+        r.lights[1].use_lights = arr
+    _refused(gpu_ctx, wl, nested_volume, A.T3D_ERR_UNSUPPORTED)
+    gpu_ctx.sync_cell_lights(vol)
+    compare_frames(gpu_ctx.render_workload(wl), oracle.render_workload(wl))
+
+
+@pytest.mark.gpu
 def test_invalid_tuning_is_refused(gpu_ctx, oracle):
     print("invalid tuning", flush=True)
     before = gpu_ctx.tuning()

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 from tetra3d_amd import _abi as A
-from tests.feature_scenes import feature_scene, many_bones_scene, pair_order_scene
+from tests.feature_scenes import cell_lights_scene, feature_scene, many_bones_scene, pair_order_scene
 from tests.parity import compare_frames
 
 SEEDS = list(range(40))
@@ -128,3 +128,36 @@ def test_many_lights(gpu_ctx, oracle, mode, n):
     print("lights", n, "mode", mode, flush=True)
     s = compare_frames(gpu_ctx.render_workload(wl, exec_mode=mode), oracle.render_workload(wl))
     assert s["survivors"] > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [A.EXEC_BATCHED, A.EXEC_PLAIN], ids=["batched", "plain"])
+@pytest.mark.parametrize("many", [False, True], ids=["few", "many"])
+def test_cell_lights(gpu_ctx, oracle, mode, many):
+    """A LightVolume whose cells name lights (light.go:646-660): each of a
+    cell's lights lights the vertex before the volume's colour, unskipped and
+    whatever its visibility, with beginModel for the model at hand; rigid and
+    skinned models in every shading mode, the volume in the scene lights and
+    in LightGroups; with 80 cell lights a pair's list passes the 64 lights
+    the vertex pass stages in LDS."""
+    for seed in range(4):
+        wl = cell_lights_scene(seed, many)
+        print("cell-lights scene", seed, "many", many, "mode", mode, flush=True)
+        s = compare_frames(gpu_ctx.render_workload(wl, exec_mode=mode), oracle.render_workload(wl))
+        assert s["survivors"] > 0
+
+
+@pytest.mark.gpu
+def test_cell_lights_follow_list_changes(gpu_ctx, oracle):
+    """Cell lists changed between renders (SetUseLights, ClearUseLights)
+    reach the device before the next render."""
+    wl = cell_lights_scene(7)
+    vol = wl.lights[1]
+    for step in range(3):
+        if step == 1:
+            for c in range(0, len(vol.cells), 2):
+                vol.clear_use_lights(c)
+        elif step == 2:
+            vol.set_use_lights(0, *vol.use_lights_render_set[:2])
+        print("cell-lights change", step, flush=True)
+        compare_frames(gpu_ctx.render_workload(wl), oracle.render_workload(wl))

@@ -178,6 +178,23 @@ def test_invalid_job_refused_at_submit_mix_unaffected(gpu_ctx, oracle, pool):
     compare_frames(alone[1], oracle.render(jobs[1].camera, scene, jobs[1].lights, job_models(models, jobs[1])))
 
 
+def test_cell_lights_through_the_server(gpu_ctx, oracle):
+    """A submitted job's LightVolume cell lights are copied with the job
+    (use_lights deep-copied at submit) and render as Camera.Render does."""
+    from tests.feature_scenes import cell_lights_scene
+    from tetra3d_amd.serving import Server
+    wl = cell_lights_scene(3)
+    gpu_ctx.prepare(wl)
+    ref = oracle.render_workload(wl)
+    inp = gpu_ctx.inputs(wl.camera, wl.scene, wl.lights, wl.models)
+    with Server(gpu_ctx, slots=1, max_batch_jobs=1) as srv:
+        i = srv.submit_inputs(inp)
+        del inp                     # the job holds its own copy of every descriptor
+        srv.wait(i)
+        compare_frames(srv.download(i), ref)
+        srv.release(i)
+
+
 def test_animation_and_volume_references(gpu_ctx):
     """A submitted job holds its animations and light volumes as it holds
     its meshes: freeing one returns BUSY until the job is released."""

@@ -19,6 +19,7 @@ import numpy as np
 import pytest
 
 from tetra3d_amd import _abi as A
+from tetra3d_amd import workloads as W
 from tetra3d_amd.scene import (AmbientLight, Camera, DirectionalLight, LightVolume, Material, Mesh, MeshPart, Model,
                                PointLight, Scene, World, mat_translate, rotation_from_forward, with_batched)
 
@@ -416,6 +417,139 @@ def test_light_volume_per_object(oracle):
     # a model position outside the grid has no cell: no light
     vol, fr = _lv(oracle, A.LV_PER_OBJECT, pos - np.array([0, 0, 30], dtype=f32), mdl_pos=(3, 3, 30))
     assert np.all(fr.color[:, 4:7] == 0)
+
+
+# ---------------------------------------------------------------------------
+# LightVolume cell lights (light.go:646-660, 1508-1518)
+# ---------------------------------------------------------------------------
+
+FAR_TRI = [(-1, -1, -30), (1, -1, -30), (0, 1, -30)]
+
+
+def _cells_render(oracle, vol, lights_before=(), mode=A.LV_PER_VERTEX_WITHOUT_NORMAL, tris=LV_TRI, w=None,
+                  place=None):
+    mesh = tri_mesh(tris)
+    mesh.mesh_parts[0].material.light_volume_shading_mode = mode
+    m = model(mesh)
+    if place is not None:
+        place(m)
+    return render(oracle, [m], list(lights_before) + [vol], w=w, cam=persp_camera(far=50))
+
+
+def test_cell_lights_run_before_the_volume_colour(oracle):
+    """A cell's lights add into the vertex light before the volume's own
+    colour (light.go:650-660), even when hidden.  With the world ambient
+    0.5 - 2^-25 first, a cell light of 2^-25 and a cell colour of 1.5 * 2^-25,
+    that order sums to exactly 0.5 + 2^-24 in float32; the volume's colour
+    first, or the cell light left out, rounds to 0.5."""
+    vol = _indexed_volume()
+    a = _cell(vol, 4, 4, 2)                      # corner A (-1, -1, -5)
+    vol.cells[a] = (1.5 * 2.0 ** -25, 0.5, 0.25, 1)
+    inner = AmbientLight("cell light", 1, 1, 1, 2.0 ** -25)
+    inner.visible = False
+    vol.set_use_lights((4, 4, 2), inner)
+    amb = f32(0.5 - 2.0 ** -25)
+    fr = _cells_render(oracle, vol, w=world(ambient=amb))
+    c, _ = corner(fr, 0, 0)
+    assert c[4] == f32(0.5 + 2.0 ** -24)
+    assert f32(f32(amb + f32(1.5 * 2.0 ** -25)) + f32(2.0 ** -25)) == f32(0.5)   # the other order
+    # corners B (1, -1, -5) and C (0, 1, -5), whose cells name no light: ambient + cell colour
+    got = sorted(float(corner(fr, 0, k)[0][4]) for k in (1, 2))
+    assert got == sorted(float(f32(amb + vol.cells[_cell(vol, *cell), 0])) for cell in ((5, 4, 2), (5, 5, 2)))
+
+
+def test_cell_lights_not_in_per_object_mode(oracle):
+    """PerObject shading takes the model-position cell's colour and runs no
+    cell lights (light.go:597-605)."""
+    vol = _indexed_volume()
+    vol.set_use_lights((5, 5, 5), AmbientLight("cell light", 1, 1, 1, 0.5))   # the origin's cell
+    fr = _cells_render(oracle, vol, mode=A.LV_PER_OBJECT)
+    assert np.all(fr.color[:, 4] == vol.cells[_cell(vol, 5, 5, 5), 0])
+
+
+def test_cell_lights_skip_no_range_test_and_repeat(oracle):
+    """A cell's point light runs VertexLight without the per-model range skip
+    of the camera's light loop (camera.go:1623-1629), and a light named twice
+    runs twice: the same sum as the same lights first in the scene list,
+    where a mesh of larger span keeps the skip away."""
+    def setup():
+        vol = _indexed_volume()
+        p = PointLight("p", 0.75, 0.5, 0.25, 1.0)
+        p.set_local_position(-1, -1, -4)
+        p.range = f32(1.2)
+        return vol, p
+    # as a scene light over the one-triangle mesh (span 2.83): skipped, since |(-1,-1,-4)|^2 = 18 > (2.83 + 1.2)^2
+    vol, p = setup()
+    skipped = _cells_render(oracle, vol, lights_before=[p])
+    assert np.all(skipped.color[:, 4:7] == _cells_render(oracle, _indexed_volume()).color[:, 4:7])
+    # as a scene light over a mesh with a far triangle too (span 25): lit
+    vol, p = setup()
+    lit = _cells_render(oracle, vol, lights_before=[p, p], tris=LV_TRI + FAR_TRI)
+    # as cell light of every cell, hidden, over the one-triangle mesh
+    vol, p = setup()
+    p.visible = False
+    for i in range(len(vol.cells)):
+        vol.set_use_lights(i, p, p)
+    nested = _cells_render(oracle, vol, tris=LV_TRI + FAR_TRI)
+    i_near = 3 * list(lit.triangle_ids).index(0)
+    assert np.array_equal(nested.color[i_near:i_near + 3], lit.color[i_near:i_near + 3])
+    a_lit, _ = corner(lit, 0, 0)
+    a_skip, _ = corner(skipped, 0, 0)
+    assert a_lit[4] > a_skip[4]
+
+
+def test_cell_lights_begin_model(oracle):
+    """Cell lights get beginModel for the model being lit (light.go:574-580):
+    a directional light's inverted model rotation and a point light's
+    inverted-transform position, for a moved, turned and scaled model, give
+    the same colours as the same lights first in the scene list."""
+    def place(m):
+        m.set_local_position(0.5, -0.25, -1)
+        m.set_local_rotation(W.axis_angle_matrix((0.3, 1, 0.2), 0.7))
+        m.set_local_scale(1.5, 0.75, 1.25)
+    def lights():
+        d = DirectionalLight("d", 0.5, 0.75, 1.0, 0.9)
+        d.set_local_rotation(W.axis_angle_matrix((1, 0.2, 0), -0.9))
+        p = PointLight("p", 1.0, 0.5, 0.25, 2.0)
+        p.set_local_position(1, 2, -3)
+        return d, p
+    d, p = lights()
+    ref = _cells_render(oracle, _indexed_volume(), lights_before=[d, p], place=place)
+    d, p = lights()
+    vol = _indexed_volume()
+    for i in range(len(vol.cells)):
+        vol.set_use_lights(i, d, p)
+    got = _cells_render(oracle, vol, place=place)
+    assert np.array_equal(got.color, ref.color)
+    assert np.any(ref.color[:, 4:7] != _cells_render(oracle, _indexed_volume(), place=place).color[:, 4:7])
+
+
+def test_cell_lights_cleared(oracle):
+    """ClearUseLights empties a cell's list; the volume keeps its render set,
+    which changes nothing drawn (light.go:1516-1518)."""
+    vol = _indexed_volume()
+    vol.set_use_lights((4, 4, 2), AmbientLight("cell light", 1, 1, 1, 0.5))
+    vol.clear_use_lights((4, 4, 2))
+    assert vol.use_lights_render_set and not vol.use_lights((4, 4, 2))
+    assert np.array_equal(_cells_render(oracle, vol).color, _cells_render(oracle, _indexed_volume()).color)
+
+
+def test_cell_lights_must_be_in_use_lights(oracle):
+    """Cell lists naming a light past the volume's use_lights are refused."""
+    import ctypes as C
+    vol = _indexed_volume()
+    vol.set_use_lights(0, AmbientLight("cell light", 1, 1, 1, 0.5))
+    vol.handle = 900
+    oracle.register_volume(vol, vol.handle)
+    first = np.zeros(len(vol.cells) + 1, dtype=np.uint32)
+    first[1:] = 1
+    index = np.array([3], dtype=np.uint32)
+    U = C.POINTER(C.c_uint32)
+    oracle.L.t3do_volume_set_cell_lights(oracle.ctx, vol.handle, first.ctypes.data_as(U), index.ctypes.data_as(U))
+    mesh = tri_mesh(LV_TRI)
+    oracle.register_mesh(mesh, 901)
+    with pytest.raises(RuntimeError, match="cell lights"):
+        oracle.render(persp_camera(far=50), Scene("s", world()), [vol], [model(mesh)])
 
 
 # ---------------------------------------------------------------------------

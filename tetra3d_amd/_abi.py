@@ -5,7 +5,7 @@ sizes against the library's own sizeof values.
 """
 import ctypes as C
 
-T3D_ABI_VERSION = 1
+T3D_ABI_VERSION = 2
 
 # status codes
 T3D_OK = 0
@@ -113,22 +113,28 @@ class MaterialDesc(C.Structure):
 
 
 class LightDesc(C.Structure):
-    _fields_ = [
-        ("type", C.c_int32),
-        ("visible", C.c_int32),
-        ("color", F3),
-        ("energy", C.c_float),
-        ("range", C.c_float),
-        ("transform", Mat4),
-        ("world_rotation", Mat4),
-        ("volume", C.c_uint64),
-        ("volume_min", F3),
-        ("volume_max", F3),
-        ("cell_size", F3),
-        ("cell_count", I3),
-        ("offset", F3),
-        ("light_bias", F3),
-    ]
+    pass
+
+
+LightDesc._fields_ = [
+    ("type", C.c_int32),
+    ("visible", C.c_int32),
+    ("color", F3),
+    ("energy", C.c_float),
+    ("range", C.c_float),
+    ("transform", Mat4),
+    ("world_rotation", Mat4),
+    ("volume", C.c_uint64),
+    ("volume_min", F3),
+    ("volume_max", F3),
+    ("cell_size", F3),
+    ("cell_count", I3),
+    ("offset", F3),
+    ("light_bias", F3),
+    ("use_lights", C.POINTER(LightDesc)),
+    ("use_light_count", C.c_uint32),
+    ("pad_use", C.c_int32),
+]
 
 
 class BonePose(C.Structure):

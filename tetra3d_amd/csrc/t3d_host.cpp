@@ -409,6 +409,7 @@ LightGPU make_light(t3d_ctx* c, const t3d_light_desc& d) {
     if (it != c->volumes.end()) {
       l.cells = reinterpret_cast<const float*>(it->second->buf.ptr);
       l.ncells = it->second->n;
+      if (it->second->lists_need) l.cell_lights = reinterpret_cast<const uint32_t*>(it->second->lists.ptr);
     }
     l.cx = d.cell_count[0];
     l.cy = d.cell_count[1];
@@ -422,6 +423,48 @@ LightGPU make_light(t3d_ctx* c, const t3d_light_desc& d) {
     }
   }
   return l;
+}
+
+// Lights a volume's cells run (LightVolume.useLightsRenderSet, light.go:566-580):
+// its use_lights when its cells name any, else none.
+uint32_t nested_count(t3d_ctx* c, const t3d_light_desc& d) {
+  if (d.type != T3D_LIGHT_VOLUME) return 0;
+  auto it = c->volumes.find(d.volume);
+  return (it != c->volumes.end() && it->second->lists_need) ? d.use_light_count : 0u;
+}
+
+// One light list of the frame (a camera's scene lights, or a model's
+// LightGroup) into out: the listed lights, then each volume's cell lights
+// after them (beginRender of the volume's set, light.go:566-571), marked
+// nested; a volume's nest_off counts from out[0].  Returns the entries
+// written; *directional tells whether one of them is a DirectionalLight.
+uint32_t emit_light_list(t3d_ctx* c, const t3d_light_desc* const* list, uint32_t n, LightGPU* out, bool* directional) {
+  bool dir = false;
+  for (uint32_t i = 0; i < n; i++) {
+    out[i] = make_light(c, *list[i]);
+    dir = dir || list[i]->type == T3D_LIGHT_DIRECTIONAL;
+  }
+  uint32_t at = n;
+  for (uint32_t i = 0; i < n; i++) {
+    const uint32_t k = nested_count(c, *list[i]);
+    if (!k) continue;
+    out[i].nest_off = at;
+    for (uint32_t j = 0; j < k; j++) {
+      const t3d_light_desc& u = list[i]->use_lights[j];
+      out[at] = make_light(c, u);
+      out[at].nested = 1;
+      dir = dir || u.type == T3D_LIGHT_DIRECTIONAL;
+      at++;
+    }
+  }
+  if (directional) *directional = dir;
+  return at;
+}
+
+uint32_t light_list_size(t3d_ctx* c, const t3d_light_desc* const* list, uint32_t n) {
+  uint32_t at = n;
+  for (uint32_t i = 0; i < n; i++) at += nested_count(c, *list[i]);
+  return at;
 }
 
 // A scene light entry: the descriptor (nullptr for the world ambient light).
@@ -645,7 +688,10 @@ t3d_status t3d_ctx_destroy(t3d_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   (void)hipStreamSynchronize(c->copy_stream);
   for (auto& kv : c->meshes) dev_free(c, kv.second->buf);
-  for (auto& kv : c->volumes) dev_free(c, kv.second->buf);
+  for (auto& kv : c->volumes) {
+    dev_free(c, kv.second->buf);
+    dev_free(c, kv.second->lists);
+  }
   (void)hipStreamDestroy(c->stream);
   (void)hipStreamDestroy(c->copy_stream);
   delete c;
@@ -816,7 +862,45 @@ t3d_status t3d_volume_free(t3d_ctx* c, uint64_t handle) {
   if (it->second->refs > 0) return fail(c, T3D_ERR_BUSY, "volume_free: volume %llu is referenced by in-flight work", (unsigned long long)handle);
   (void)hipDeviceSynchronize();
   dev_free(c, it->second->buf);
+  dev_free(c, it->second->lists);
   c->volumes.erase(it);
+  return T3D_OK;
+}
+
+t3d_status t3d_volume_set_cell_lights(t3d_ctx* c, uint64_t handle, const uint32_t* first, const uint32_t* index) {
+  if (!c) return T3D_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(c->mu);
+  auto it = c->volumes.find(handle);
+  if (it == c->volumes.end()) return fail(c, T3D_ERR_NOT_FOUND, "volume_set_cell_lights: unknown handle");
+  Volume& v = *it->second;
+  if (v.refs > 0)
+    return fail(c, T3D_ERR_BUSY, "volume_set_cell_lights: volume %llu is referenced by in-flight work", (unsigned long long)handle);
+  uint32_t need = 0;
+  uint64_t total = 0;
+  if (first) {
+    if (first[0] != 0) return fail(c, T3D_ERR_INVALID_ARGUMENT, "volume_set_cell_lights: cell_first[0] is %u, not 0", first[0]);
+    for (uint32_t i = 0; i < v.n; i++)
+      if (first[i + 1] < first[i])
+        return fail(c, T3D_ERR_INVALID_ARGUMENT, "volume_set_cell_lights: cell_first decreases at cell %u", i);
+    total = first[v.n];
+    if (total && !index) return fail(c, T3D_ERR_INVALID_ARGUMENT, "volume_set_cell_lights: light_index missing");
+    for (uint64_t j = 0; j < total; j++) {
+      if (index[j] >= 0x7FFFFFFFu)
+        return fail(c, T3D_ERR_INVALID_ARGUMENT, "volume_set_cell_lights: light index %u out of range", index[j]);
+      need = std::max(need, index[j] + 1);
+    }
+  }
+  (void)hipDeviceSynchronize();   // frames of any stream may still read the old lists
+  dev_free(c, v.lists);
+  v.lists_need = 0;
+  if (!need) return T3D_OK;        // no cell names a light: the volume lights as before
+  std::vector<uint32_t> host((size_t)v.n + 1 + total);
+  std::copy(first, first + v.n + 1, host.begin());
+  std::copy(index, index + total, host.begin() + v.n + 1);
+  t3d_status st = dev_alloc(c, v.lists, host.size() * sizeof(uint32_t));
+  if (st) return st;
+  HIP_TRY(c, hipMemcpy(v.lists.ptr, host.data(), host.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  v.lists_need = need;
   return T3D_OK;
 }
 
@@ -1047,8 +1131,21 @@ t3d_status validate_desc(t3d_ctx* c, const t3d_render_desc* desc) {
   auto check_light = [&](const t3d_light_desc& l, const char* what, uint32_t i) -> t3d_status {
     if (l.type < T3D_LIGHT_AMBIENT || l.type > T3D_LIGHT_VOLUME)
       return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: %s %u has unknown type %d", what, i, l.type);
-    if (l.type == T3D_LIGHT_VOLUME && !c->volumes.count(l.volume))
-      return fail(c, T3D_ERR_NOT_FOUND, "render: %s %u references unknown volume", what, i);
+    if (l.type == T3D_LIGHT_VOLUME) {
+      auto vit = c->volumes.find(l.volume);
+      if (vit == c->volumes.end()) return fail(c, T3D_ERR_NOT_FOUND, "render: %s %u references unknown volume", what, i);
+      const uint32_t need = vit->second->lists_need;
+      if (need && (l.use_light_count < need || !l.use_lights))
+        return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: %s %u: the volume's cells name light %u of %u use_lights", what,
+                    i, need - 1, l.use_light_count);
+      for (uint32_t j = 0; need && j < l.use_light_count; j++) {
+        const int32_t t = l.use_lights[j].type;
+        if (t == T3D_LIGHT_VOLUME)
+          return fail(c, T3D_ERR_UNSUPPORTED, "render: %s %u: use_lights %u is a LightVolume (cells of cells)", what, i, j);
+        if (t < T3D_LIGHT_AMBIENT || t > T3D_LIGHT_VOLUME)
+          return fail(c, T3D_ERR_INVALID_ARGUMENT, "render: %s %u: use_lights %u has unknown type %d", what, i, j, t);
+      }
+    }
     return T3D_OK;
   };
   for (uint32_t i = 0; i < desc->light_count; i++) {
@@ -1234,9 +1331,14 @@ t3d_status frame_build_stage(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* co
     if (vst) return vst;
     const t3d_world_desc* world = (desc->world && desc->world->present) ? desc->world : nullptr;
     uint32_t n_scene = (world && world->lighting_on && world->ambient_visible) ? 1u : 0u;
+    uint32_t n_nested = 0;   // bound: the cell lights of every visible volume, whichever MaxLightCount keeps
     for (uint32_t i = 0; i < desc->light_count; i++)
-      if ((!world || world->lighting_on) && desc->lights[i].visible) n_scene++;
+      if ((!world || world->lighting_on) && desc->lights[i].visible) {
+        n_scene++;
+        n_nested += nested_count(c, desc->lights[i]);
+      }
     if (desc->camera->max_light_count > 0) n_scene = std::min<uint32_t>(n_scene, (uint32_t)desc->camera->max_light_count);
+    n_scene += n_nested;
     nlights += n_scene;
     any_normals = any_normals || desc->camera->render_normals;
     for (uint32_t mi = 0; mi < desc->model_count; mi++) {
@@ -1248,8 +1350,12 @@ t3d_status frame_build_stage(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* co
       nm++;
       ncand += np;
       if (md.materials) nmat += np;
-      const uint32_t lc = md.light_group_active ? md.light_group_count : n_scene;
-      if (md.light_group_active) nlights += md.light_group_count;
+      uint32_t lc = n_scene;
+      if (md.light_group_active) {
+        lc = md.light_group_count;
+        for (uint32_t k = 0; k < md.light_group_count; k++) lc += nested_count(c, md.light_group[k]);
+        nlights += lc;
+      }
       wpl += (uint64_t)np * lc;
       if (md.skinned) {
         pal_floats += 16ull * md.bone_count;
@@ -1393,9 +1499,11 @@ t3d_status frame_build_stage(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* co
     // the scene lights (camera.go:1275-1303, 1474-1497)
     const uint32_t lbase_scene = il;
     bool scene_dir = false;
-    for (const SceneLight& sl : scene_light_list(desc, world, cam)) {
-      h_lights[il++] = make_light(c, sl.get());
-      scene_dir = scene_dir || sl.get().type == T3D_LIGHT_DIRECTIONAL;
+    {
+      const std::vector<SceneLight> sls = scene_light_list(desc, world, cam);
+      std::vector<const t3d_light_desc*> lst;
+      for (const SceneLight& sl : sls) lst.push_back(&sl.get());
+      il += emit_light_list(c, lst.data(), (uint32_t)lst.size(), h_lights + il, &scene_dir);
     }
     const uint32_t n_scene = il - lbase_scene;
     for (uint32_t mi = 0; mi < desc->model_count; mi++) {
@@ -1420,13 +1528,11 @@ t3d_status frame_build_stage(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* co
       }
       bool dir = scene_dir;
       if (md.light_group_active) {   // its LightGroup replaces the scene lights (camera.go:1576-1581)
+        std::vector<const t3d_light_desc*> lst(md.light_group_count);
+        for (uint32_t k = 0; k < md.light_group_count; k++) lst[k] = &md.light_group[k];
         M.light0 = il;
-        M.light_count = md.light_group_count;
-        dir = false;
-        for (uint32_t k = 0; k < md.light_group_count; k++) {
-          h_lights[il++] = make_light(c, md.light_group[k]);
-          dir = dir || md.light_group[k].type == T3D_LIGHT_DIRECTIONAL;
-        }
+        M.light_count = emit_light_list(c, lst.data(), md.light_group_count, h_lights + il, &dir);
+        il += M.light_count;
       } else {
         M.light0 = lbase_scene;
         M.light_count = n_scene;
@@ -1823,9 +1929,14 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
 
     // this job's lights: scene lights first, then light-group lights as needed
     const uint32_t lbase_scene = (uint32_t)lights.size();
-    for (auto& sl : scene_lights) lights.push_back(make_light(c, sl.get()));
-    const uint32_t n_scene = (uint32_t)scene_lights.size();
-    std::map<uint32_t, uint32_t> group_base;  // model -> first frame light of its group
+    uint32_t n_scene;
+    {
+      std::vector<const t3d_light_desc*> lst;
+      for (const SceneLight& sl : scene_lights) lst.push_back(&sl.get());
+      lights.resize(lbase_scene + light_list_size(c, lst.data(), (uint32_t)lst.size()));
+      n_scene = emit_light_list(c, lst.data(), (uint32_t)lst.size(), lights.data() + lbase_scene, nullptr);
+    }
+    std::map<uint32_t, std::pair<uint32_t, uint32_t>> group_base;  // model -> first frame light of its group, entries
     std::map<uint32_t, uint32_t> model_pose_job;  // posed model -> its palette job
 
     any_normals = any_normals || cd->render_normals;
@@ -1886,12 +1997,16 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
           if (md.light_group_active) {
             auto g = group_base.find(pmi);
             if (g == group_base.end()) {
-              uint32_t gb = (uint32_t)lights.size();
-              for (uint32_t k = 0; k < md.light_group_count; k++) lights.push_back(make_light(c, md.light_group[k]));
-              g = group_base.emplace(pmi, gb).first;
+              std::vector<const t3d_light_desc*> lst(md.light_group_count);
+              for (uint32_t k = 0; k < md.light_group_count; k++) lst[k] = &md.light_group[k];
+              const uint32_t gb = (uint32_t)lights.size();
+              const uint32_t gn = light_list_size(c, lst.data(), md.light_group_count);
+              lights.resize(gb + gn);
+              emit_light_list(c, lst.data(), md.light_group_count, lights.data() + gb, nullptr);
+              g = group_base.emplace(pmi, std::make_pair(gb, gn)).first;
             }
-            lbase = g->second;
-            lcount = md.light_group_count;
+            lbase = g->second.first;
+            lcount = g->second.second;
           }
           P.light_base = (uint32_t)n_plights;
           P.light_count = lcount;

@@ -102,7 +102,11 @@ struct alignas(16) LightGPU {
   int32_t cx, cy, cz;
   float vmin[3], vmax[3], cell[3], offset[3], bias[3];
   float pos[3];         // world position (row 3 of the light's transform), for beginModel
-  float pad[2];
+  // LightVolume cell lights (light.go:650-653): [ncells + 1] offsets, then the
+  // light indices; index j is entry nest_off + j of the pair's light list
+  const uint32_t* cell_lights;
+  uint32_t nest_off;
+  uint32_t nested;      // 1: one of a volume's cell lights, run only from its cells (never by the list loop)
 };
 
 // Per (render pair, light) values from beginModel and the per-model skip,

@@ -98,20 +98,27 @@ __device__ __forceinline__ V3 ld3(const float* __restrict__ p, uint32_t i) {
 // lighting
 // ---------------------------------------------------------------------------
 
-// LightVolume cell lookup: convertToXYZIndices + getLightCell (light.go:717-724, 1267-1272).
-__device__ __forceinline__ const float* volume_cell(const LightGPU& l, V3 p) {
+// LightVolume cell lookup: convertToXYZIndices + getLightCell (light.go:717-724, 1267-1272);
+// the cell's index, or -1 where there is none.
+__device__ __forceinline__ int64_t volume_cell_index(const LightGPU& l, V3 p) {
   int64_t x = go_int(roundf(fdiv(p.x - l.offset[0] - l.vmin[0] - fdiv(l.cell[0], 2.0f), l.cell[0])));
   int64_t y = go_int(roundf(fdiv(p.y - l.offset[1] - l.vmin[1] - fdiv(l.cell[1], 2.0f), l.cell[1])));
   int64_t z = go_int(roundf(fdiv(p.z - l.offset[2] - l.vmin[2] - fdiv(l.cell[2], 2.0f), l.cell[2])));
-  if (x < 0 || y < 0 || z < 0 || x >= l.cx || y >= l.cy || z >= l.cz) return nullptr;
+  if (x < 0 || y < 0 || z < 0 || x >= l.cx || y >= l.cy || z >= l.cz) return -1;
   int64_t i = x + (y * l.cx) + (z * l.cx * l.cy);
-  if (!l.cells || i >= (int64_t)l.ncells) return nullptr;
-  return l.cells + 4 * i;
+  if (!l.cells || i >= (int64_t)l.ncells) return -1;
+  return i;
 }
 
-// One light's VertexLight on one vertex (light.go:116-120, 191-258, 332-360,
-// 583-663), accumulating into rgb in the reference's order.
-__device__ __forceinline__ void light_vertex(const LightGPU& l, const PartLight& pl, const PartGPU& P, uint32_t flags,
+__device__ __forceinline__ const float* volume_cell(const LightGPU& l, V3 p) {
+  const int64_t i = volume_cell_index(l, p);
+  return i < 0 ? nullptr : l.cells + 4 * i;
+}
+
+// VertexLight of an ambient, point or directional light on one vertex
+// (light.go:116-120, 191-258, 332-360), accumulating into rgb in the
+// reference's order.
+__device__ __forceinline__ void light_direct(const LightGPU& l, const PartLight& pl, const PartGPU& P, uint32_t flags,
                                              V3 vp, V3 vn, float& r, float& g, float& b) {
   const uint32_t lmode = (flags >> PF_LMODE_SHIFT) & 3u;
   if (l.type == T3D_LIGHT_AMBIENT) {
@@ -152,7 +159,19 @@ __device__ __forceinline__ void light_vertex(const LightGPU& l, const PartLight&
     r += l.color[0] * df * l.energy;
     g += l.color[1] * df * l.energy;
     b += l.color[2] * df * l.energy;
-  } else if (l.type == T3D_LIGHT_VOLUME) {
+  }
+}
+
+// One light of a pair's list on one vertex: light_direct, or a LightVolume
+// (light.go:583-663), whose cell first runs the lights its useLights names
+// (light.go:650-653; entries ls/pls[nest_off + j] of the same list) and then
+// adds its own colour.
+__device__ __forceinline__ void light_vertex(const LightGPU& l, const PartLight& pl, const LightGPU* ls,
+                                             const PartLight* pls, const PartGPU& P, uint32_t flags, V3 vp, V3 vn,
+                                             float& r, float& g, float& b) {
+  if (l.type != T3D_LIGHT_VOLUME) {
+    light_direct(l, pl, P, flags, vp, vn, r, g, b);
+  } else {
     const uint32_t mode = (flags >> PF_LVMODE_SHIFT) & 3u;
     if (mode == T3D_LV_OFF) return;                                // LightVolume.Light, light.go:671-673
     float c0 = 0, c1 = 0, c2 = 0;
@@ -176,11 +195,20 @@ __device__ __forceinline__ void light_vertex(const LightGPU& l, const PartLight&
       if (!(vpos.x >= l.vmin[0] && vpos.y >= l.vmin[1] && vpos.z >= l.vmin[2] && vpos.x <= l.vmax[0] &&
             vpos.y <= l.vmax[1] && vpos.z <= l.vmax[2]))
         return;                                                      // Dimensions.Contains, mesh.go:80-82
-      const float* cell = volume_cell(l, mode == T3D_LV_PER_VERTEX_WITH_NORMAL ? vpn : vpos);
-      if (cell) {
+      const int64_t ci = volume_cell_index(l, mode == T3D_LV_PER_VERTEX_WITH_NORMAL ? vpn : vpos);
+      if (ci >= 0) {
+        const float* cell = l.cells + 4 * ci;
         c0 = cell[0];
         c1 = cell[1];
         c2 = cell[2];
+        if (l.cell_lights) {
+          const uint32_t j1 = l.cell_lights[ci + 1];
+          const uint32_t* ix = l.cell_lights + l.ncells + 1;
+          for (uint32_t j = l.cell_lights[ci]; j < j1; j++) {
+            const uint32_t li = l.nest_off + ix[j];
+            light_direct(ls[li], pls[li], P, flags, vp, vn, r, g, b);
+          }
+        }
       }
     }
     r += l.color[0] * c0 * l.energy;
@@ -320,13 +348,15 @@ __global__ __launch_bounds__(256) void k_vertex(FrameArgs a, uint32_t blk0) {
       if (lds_lights) {
         for (uint32_t k = 0; k < nl; k++) {
           if (!s_pl[k].enabled) continue;
-          light_vertex(s_light[k], s_pl[k], P, flags, vp, vn, lr, lg, lb);
+          light_vertex(s_light[k], s_pl[k], s_light, s_pl, P, flags, vp, vn, lr, lg, lb);
         }
       } else {
+        const LightGPU* ls = a.lights + P.light0;
+        const PartLight* pls = a.plights + P.light_base;
         for (uint32_t k = 0; k < nl; k++) {
-          const PartLight pl = a.plights[P.light_base + k];
+          const PartLight pl = pls[k];
           if (!pl.enabled) continue;
-          light_vertex(a.lights[pl.light], pl, P, flags, vp, vn, lr, lg, lb);
+          light_vertex(ls[k], pl, ls, pls, P, flags, vp, vn, lr, lg, lb);
         }
       }
       cr *= lr;
@@ -499,15 +529,15 @@ __global__ __launch_bounds__(64) void k_uniforms(FrameArgs a, uint32_t part0, ui
     const V3 modelPos = row3(M);
     PartLight pl{};
     pl.light = P.light0 + kk;
-    pl.enabled = 1;
     const LightGPU& L = a.lights[pl.light];
+    pl.enabled = L.nested ? 0u : 1u;   // a volume's cell light runs from its cells only, unskipped
     if (L.type == T3D_LIGHT_POINT) {
       const V3 wp = v3(L.pos[0], L.pos[1], L.pos[2]);
       const V3 w = skinned ? wp : add(mult_vec(dr, wp), hmul(dp, is));
       pl.v[0] = w.x;
       pl.v[1] = w.y;
       pl.v[2] = w.z;
-      if (L.range > 0) {
+      if (L.range > 0 && !L.nested) {
         const float dist = P.max_span + L.range;
         if (dist_sq(modelPos, wp) > dist * dist) pl.enabled = 0;
       }

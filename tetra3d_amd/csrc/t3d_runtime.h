@@ -56,6 +56,8 @@ struct Volume {
   uint32_t n = 0;
   std::vector<float> host;         // for per-object lookups on the host
   int refs = 0;                    // submitted jobs referencing the volume
+  DevBuf lists;                    // cell light lists: [n + 1] offsets then the light indices (empty: none)
+  uint32_t lists_need = 0;         // 1 + the largest light index the lists name (0: no lists)
 };
 
 struct Animation {

@@ -46,6 +46,7 @@ struct OwnedDesc {
   std::vector<t3d_mat4> roots;     // armature parents of posed models
   std::vector<float> skw;
   std::vector<t3d_light_desc> group;
+  std::vector<t3d_light_desc> nested;   // the volumes' use_lights
   t3d_render_desc desc{};
 };
 
@@ -137,6 +138,28 @@ t3d_status copy_desc(t3d_ctx* c, const t3d_render_desc* in, Job& j) {
   if (in->world) o.world = *in->world;
   o.lights.assign(in->lights, in->lights + in->light_count);
   o.models.assign(in->models, in->models + in->model_count);
+  // a volume's use_lights into o.nested (sized first: pointers are taken as it fills)
+  auto nested_of = [](const t3d_light_desc& l) -> uint32_t {
+    return (l.type == T3D_LIGHT_VOLUME && l.use_lights) ? l.use_light_count : 0u;
+  };
+  size_t nn = 0;
+  for (const auto& l : o.lights) nn += nested_of(l);
+  for (const auto& m : o.models)
+    if (m.light_group_active && m.light_group)
+      for (uint32_t k = 0; k < m.light_group_count; k++) nn += nested_of(m.light_group[k]);
+  o.nested.reserve(nn);
+  auto own_nested = [&](t3d_light_desc& l) {
+    const uint32_t n = nested_of(l);
+    if (!n) return;
+    const size_t at = o.nested.size();
+    o.nested.insert(o.nested.end(), l.use_lights, l.use_lights + n);
+    for (size_t k = at; k < o.nested.size(); k++) {   // cells of cells are refused at build; keep no foreign pointer
+      o.nested[k].use_lights = nullptr;
+      o.nested[k].use_light_count = 0;
+    }
+    l.use_lights = o.nested.data() + at;
+  };
+  for (auto& l : o.lights) own_nested(l);
   // sizes first so the vectors never reallocate while pointers are taken
   size_t nm = 0, nb = 0, np = 0, nr = 0, nk = 0, ng = 0;
   for (const auto& m : o.models) {
@@ -193,6 +216,7 @@ t3d_status copy_desc(t3d_ctx* c, const t3d_render_desc* in, Job& j) {
       size_t at = o.group.size();
       o.group.insert(o.group.end(), m.light_group, m.light_group + m.light_group_count);
       m.light_group = o.group.data() + at;
+      for (size_t k = at; k < o.group.size(); k++) own_nested(o.group[k]);
     }
     j.meshes.push_back(m.mesh);
     for (const auto& pr : mesh.parts) j.triangles += pr.triangle_count;

@@ -56,6 +56,7 @@ def lib():
     L.t3d_mesh_free.argtypes = [vp, C.c_uint64]
     L.t3d_volume_upload.argtypes = [vp, P(C.c_float), C.c_uint32, P(C.c_uint64)]
     L.t3d_volume_free.argtypes = [vp, C.c_uint64]
+    L.t3d_volume_set_cell_lights.argtypes = [vp, C.c_uint64, P(C.c_uint32), P(C.c_uint32)]
     L.t3d_animation_upload.argtypes = [vp, P(A.AnimChannel), C.c_uint32, P(C.c_uint64)]
     L.t3d_animation_free.argtypes = [vp, C.c_uint64]
     L.t3d_frame_create.argtypes = [vp, P(vp)]
@@ -88,7 +89,7 @@ def lib():
     L.t3d_debug_model_spheres.argtypes = [P(A.Mat4), P(C.c_float), P(C.c_uint8), C.c_uint32, P(C.c_float),
                                           P(C.c_float)]
     for name in ("t3d_ctx_create", "t3d_ctx_destroy", "t3d_mesh_upload", "t3d_mesh_free", "t3d_volume_upload",
-                 "t3d_volume_free", "t3d_frame_create", "t3d_frame_destroy", "t3d_render", "t3d_frame_pairs",
+                 "t3d_volume_free", "t3d_volume_set_cell_lights", "t3d_frame_create", "t3d_frame_destroy", "t3d_render", "t3d_frame_pairs",
                  "t3d_frame_device_outputs", "t3d_frame_download", "t3d_frame_download_triangles",
                  "t3d_profile_enable", "t3d_profile_read", "t3d_get_tuning", "t3d_set_tuning", "t3d_autotune",
                  "t3d_selftest_fp", "t3d_selftest_copy", "t3d_debug_model_spheres", "t3d_animation_upload",
@@ -199,7 +200,21 @@ class Context:
         self._check(self.L.t3d_volume_upload(self.ctx, cells.ctypes.data_as(C.POINTER(C.c_float)), len(cells),
                                              C.byref(out)))
         vol.handle = out.value
+        vol._lists_synced = None
+        self.sync_cell_lights(vol)
         return out.value
+
+    def sync_cell_lights(self, vol):
+        """The volume's cell light lists to the device when they changed since
+        the last upload (t3d_volume_set_cell_lights)."""
+        if getattr(vol, "_lists_synced", None) == vol.lists_version:
+            return
+        first, index = vol.cell_light_lists()
+        U = C.POINTER(C.c_uint32)
+        self._check(self.L.t3d_volume_set_cell_lights(
+            self.ctx, vol.handle, first.ctypes.data_as(U) if first is not None else None,
+            index.ctypes.data_as(U) if index is not None else None))
+        vol._lists_synced = vol.lists_version
 
     def upload_animation(self, anim):
         arr, keep = anim.channels_desc()
@@ -222,6 +237,8 @@ class Context:
         for v in workload.volumes:
             if not v.handle:
                 self.upload_volume(v)
+            else:
+                self.sync_cell_lights(v)
         for an in workload.animations:
             if not an.handle:
                 self.upload_animation(an)
@@ -363,8 +380,11 @@ class Context:
                 self.upload_mesh(m.mesh)
         self.upload_textures(models)
         for l in lights:
-            if getattr(l, "TYPE", None) == A.LIGHT_VOLUME and not l.handle:
-                self.upload_volume(l)
+            if getattr(l, "TYPE", None) == A.LIGHT_VOLUME:
+                if not l.handle:
+                    self.upload_volume(l)
+                else:
+                    self.sync_cell_lights(l)
         inp = self.inputs(camera, scene, lights, models, exec_mode)
         self.render_inputs(inp)
         return self.download() if download else None

@@ -458,6 +458,53 @@ class LightVolume(_Light):
         self.offset = np.zeros(3, dtype=f32)
         self.light_bias = np.zeros(3, dtype=f32)
         self.handle = 0
+        self.cell_lights = {}        # cell index -> LightVolumeCell.useLights (light.go:1381)
+        self.use_lights_render_set = []   # useLightsRenderSet (light.go:392), first-added order
+        self.lists_version = 0       # bumped by every change of the cell lists
+
+    def cell_index(self, x, y, z):
+        """getLightCell's index of cell (x, y, z) (light.go:1267-1272)."""
+        cx, cy, _ = self.cell_count
+        return x + y * cx + z * cx * cy
+
+    def _cell(self, cell):
+        i = self.cell_index(*cell) if isinstance(cell, (tuple, list)) else int(cell)
+        if not 0 <= i < len(self.cells):
+            raise IndexError("cell %r outside the volume's %d cells" % (cell, len(self.cells)))
+        return i
+
+    def set_use_lights(self, cell, *lights):
+        """LightVolumeCell.SetUseLights (light.go:1508-1513): append to the
+        cell's lights (repeats kept); each joins the volume's render set."""
+        i = self._cell(cell)
+        for l in lights:
+            if not any(l is x for x in self.use_lights_render_set):
+                self.use_lights_render_set.append(l)
+        self.cell_lights.setdefault(i, []).extend(lights)
+        self.lists_version += 1
+
+    def clear_use_lights(self, cell):
+        """LightVolumeCell.ClearUseLights (light.go:1516-1518); the render set keeps its lights."""
+        self.cell_lights.pop(self._cell(cell), None)
+        self.lists_version += 1
+
+    def use_lights(self, cell):
+        return list(self.cell_lights.get(self._cell(cell), []))
+
+    def cell_light_lists(self):
+        """(cell_first[n + 1], light_index) as t3d_volume_set_cell_lights takes
+        them, indices into use_lights_render_set; (None, None) when no cell
+        names a light."""
+        if not any(self.cell_lights.values()):
+            return None, None
+        pos = {id(l): k for k, l in enumerate(self.use_lights_render_set)}
+        n = len(self.cells)
+        first = np.zeros(n + 1, dtype=np.uint32)
+        index = []
+        for i in range(n):
+            index += [pos[id(l)] for l in self.cell_lights.get(i, [])]
+            first[i + 1] = len(index)
+        return first, np.array(index if index else [0], dtype=np.uint32)
 
     def _count_cells(self):
         """Cell counts as LightVolumeResize's float32 loops produce them (light.go:467-494)."""
@@ -483,6 +530,13 @@ class LightVolume(_Light):
         d.cell_count = A.I3(*self.cell_count)
         d.offset = A.F3(*[float(x) for x in self.offset])
         d.light_bias = A.F3(*[float(x) for x in self.light_bias])
+        if self.use_lights_render_set:
+            arr = (A.LightDesc * len(self.use_lights_render_set))()
+            for k, l in enumerate(self.use_lights_render_set):
+                arr[k] = l.desc()
+            self._use_keep = arr
+            d.use_lights = arr
+            d.use_light_count = len(self.use_lights_render_set)
         return d
 
 
