@@ -751,7 +751,7 @@ t3d_status t3d_mesh_upload(t3d_ctx* c, const t3d_mesh_desc* d, uint64_t* out_han
   uint64_t bytes = carve_size<float>(3 * V) * 2 + (d->colors ? carve_size<float>(4 * V) : 0) +
                    (d->uvs ? carve_size<float>(2 * V) : 0) + (d->bone_indices ? carve_size<uint16_t>(4 * V) + carve_size<float>(4 * V) : 0) +
                    (K ? carve_size<float>(3 * K * V) * 2 : 0) +
-                   (V <= 65536 ? carve_size<uint16_t>(3 * T) : carve_size<uint32_t>(3 * T)) +
+                   (V <= 65536 ? carve_size<uint16_t>(3 * T) : carve_size<uint32_t>(3 * T)) + carve_size<float>(3 * T) +
                    carve_size<MeshGPU>(1) + carve_size<MeshPartGPU>(std::max<size_t>(1, m->parts.size()));
   (void)hipSetDevice(c->device);
   t3d_status st = dev_alloc(c, m->buf, bytes);
@@ -786,6 +786,16 @@ t3d_status t3d_mesh_upload(t3d_ctx* c, const t3d_mesh_desc* d, uint64_t* out_han
   } else {
     m->idx = put(d->indices, 3 * T, (uint32_t*)nullptr);
   }
+  // Triangle.Center = (A + B + C) / 3, as RecalculateCenter leaves it per triangle (mesh.go:1980-1982,
+  // 2327-2328): the cull pass of a static pair reads it instead of gathering three positions
+  std::vector<float> ctr(3 * T);
+  for (uint64_t t = 0; t < T; t++) {
+    const float* pa = d->positions + 3 * (size_t)d->indices[3 * t + 0];
+    const float* pb = d->positions + 3 * (size_t)d->indices[3 * t + 1];
+    const float* pc = d->positions + 3 * (size_t)d->indices[3 * t + 2];
+    for (int k = 0; k < 3; k++) ctr[3 * t + k] = (pa[k] + pb[k] + pc[k]) / 3.0f;
+  }
+  m->ctr = put(ctr.data(), 3 * T, (float*)nullptr);
   // the mesh's record for the device model stage: its arrays and its parts' ranges
   MeshGPU g{};
   g.pos = m->pos;
@@ -796,6 +806,7 @@ t3d_status t3d_mesh_upload(t3d_ctx* c, const t3d_mesh_desc* d, uint64_t* out_han
   g.weights = m->weights;
   g.idx = m->idx;
   g.idx16 = m->idx16;
+  g.ctr = m->ctr;
   g.sk_pos = m->sk_pos;
   g.sk_nrm = m->sk_nrm;
   g.V = m->V;
@@ -2030,6 +2041,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
         P.weights = mesh.weights;
         P.idx = mesh.idx ? mesh.idx + 3 * (size_t)pr.triangle_start : nullptr;
         P.idx16 = mesh.idx16 ? mesh.idx16 + 3 * (size_t)pr.triangle_start : nullptr;
+        P.ctr = mesh.ctr + 3 * (size_t)pr.triangle_start;
         P.sk_pos = mesh.sk_pos;
         P.sk_nrm = mesh.sk_nrm;
         P.shape_keys = mesh.K;
@@ -2083,7 +2095,7 @@ t3d_status frame_build(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* const* d
         // algorithmic bytes: mesh streams read once, triangle indices once, outputs
         const uint64_t vb = 24 + (mesh.col ? 16 : 0) + (md.skinned ? 24 : 0);
         f->bytes_vertex.push_back((uint64_t)P.vcount * vb + (md.skinned ? 64ull * md.bone_count : 0) + sizeof(PartGPU));
-        f->bytes_cull.push_back((mesh.idx16 ? 6ull : 12ull) * P.tcount);
+        f->bytes_cull.push_back(((mesh.idx16 ? 6ull : 12ull) + (md.skinned ? 0ull : 12ull)) * P.tcount);
         f->bytes_uv.push_back(mesh.uv ? 8ull * P.vcount : 0);   // UVs are read by the assembly
         f->part_first_tri.push_back(pr.triangle_start);
         f->pair_tbase.push_back(P.tbase);

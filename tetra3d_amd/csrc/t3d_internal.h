@@ -69,6 +69,7 @@ struct alignas(16) PartGPU {
   const float* palette;  // bone_count x 16 floats
   const uint32_t* idx;   // this part's first triangle in the mesh index array (32-bit indices)
   const uint16_t* idx16; // the same for a mesh stored with 16-bit indices (idx is then null)
+  const float* ctr;      // Triangle.Center of this part's first triangle, xyz per triangle (mesh.go:1980-1982)
   const float* sk_pos;
   const float* sk_nrm;
   const float* sk_w;     // per-frame shape key weights
@@ -174,6 +175,7 @@ struct alignas(16) MeshGPU {
   const float* weights;
   const uint32_t* idx;
   const uint16_t* idx16;
+  const float* ctr;
   const float* sk_pos;
   const float* sk_nrm;
   const MeshPartGPU* parts;

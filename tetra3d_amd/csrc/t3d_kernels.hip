@@ -637,16 +637,22 @@ __global__ __launch_bounds__(kThreads) void k_cull(FrameArgs a, uint32_t blk0) {
       bool outOfBounds = true;
       V3 stc = v3(0, 0, 0);
       // every gather of the triangle first, so their latencies overlap: clip
-      // positions, and the positions its depth key needs
-      // (the skinned position, or the mesh position for a static model: one
-      // branch-free load either way)
+      // positions, and what its depth key needs (a static triangle's stored
+      // Triangle.Center, or the three skinned positions)
       float4 cc[3];
       V3 q[3];
       for (int k = 0; k < 3; k++) {
         ov[k] = P.vbase + (vi[k] - P.vmin);
         cc[k] = a.vclip[(size_t)ov[k] * a.vstride];
-        const float* qs = skinned ? reinterpret_cast<const float*>(a.vpos + ov[k]) : P.pos + 3 * (size_t)vi[k];
-        q[k] = v3(qs[0], qs[1], qs[2]);
+      }
+      if (skinned) {
+        for (int k = 0; k < 3; k++) {
+          const float4 s = a.vpos[ov[k]];
+          q[k] = v3(s.x, s.y, s.z);
+        }
+      } else {
+        const float* cp = P.ctr + 3 * (size_t)t;
+        q[0] = v3(cp[0], cp[1], cp[2]);
       }
       for (int k = 0; k < 3; k++) {                                   // model.go:855-886
         const float4 c = cc[k];
@@ -682,14 +688,9 @@ __global__ __launch_bounds__(kThreads) void k_cull(FrameArgs a, uint32_t blk0) {
           const float dz = camPos.z - fdiv(stc.z, 3.0f);
           depth = dx * dx + dy * dy + dz * dz;
         } else {
-          // Triangle.Center from the base positions (mesh.go:1980-1982)
-          const V3 pa = q[0], pb = q[1], pc = q[2];
-          const float cx = fdiv(pa.x + pb.x + pc.x, 3.0f);
-          const float cy = fdiv(pa.y + pb.y + pc.y, 3.0f);
-          const float cz = fdiv(pa.z + pb.z + pc.z, 3.0f);
-          const float dx = icp.x - cx;
-          const float dy = icp.y - cy;
-          const float dz = icp.z - cz;
+          const float dx = icp.x - q[0].x;    // Triangle.Center, stored at upload (mesh.go:1980-1982)
+          const float dy = icp.y - q[0].y;
+          const float dz = icp.z - q[0].z;
           depth = dx * dx + dy * dy + dz * dz;
         }
         if (!all && (isnan(depth) || isinf(depth))) {

@@ -35,6 +35,7 @@ struct Mesh {
   const float* sk_nrm = nullptr;
   const uint32_t* idx = nullptr;   // triangle indices, uint32 ...
   const uint16_t* idx16 = nullptr; // ... or uint16 when the mesh has at most 65,536 vertices
+  const float* ctr = nullptr;      // Triangle.Center per triangle, xyz (mesh.go:1980-1982, kept as AddTriangles does)
   uint32_t V = 0, T = 0, K = 0;
   std::vector<t3d_part_range> parts;
   std::vector<uint32_t> part_vmin, part_vcount;

@@ -52,7 +52,7 @@ __device__ __forceinline__ PairSizes pair_sizes(const ModelStage& ms, const Mode
   z.lcount = z.lighting ? M.light_count : 0u;
   const uint32_t vb = 24u + (mesh.col ? 16u : 0u) + (skinned ? 24u : 0u);
   z.bytes_vertex = z.vcount * vb + (skinned ? 64u * M.bone_count : 0u) + (uint32_t)sizeof(PartGPU);
-  z.bytes_cull = (mesh.idx16 ? 6u : 12u) * z.tcount;
+  z.bytes_cull = ((mesh.idx16 ? 6u : 12u) + (skinned ? 0u : 12u)) * z.tcount;
   z.bytes_uv = mesh.uv ? 8u * z.vcount : 0u;
   return z;
 }
@@ -117,6 +117,7 @@ __device__ __forceinline__ void build_pair(const FrameArgs& a, uint32_t p, PartG
   P.weights = mesh.weights;
   P.idx = mesh.idx ? mesh.idx + 3 * (size_t)pr.triangle_start : nullptr;
   P.idx16 = mesh.idx16 ? mesh.idx16 + 3 * (size_t)pr.triangle_start : nullptr;
+  P.ctr = mesh.ctr + 3 * (size_t)pr.triangle_start;
   P.sk_pos = mesh.sk_pos;
   P.sk_nrm = mesh.sk_nrm;
   P.shape_keys = mesh.K;
