@@ -527,13 +527,15 @@ typedef struct {
 } t3d_tuning;
 t3d_status t3d_get_tuning(const t3d_ctx* ctx, t3d_tuning* out);
 t3d_status t3d_set_tuning(t3d_ctx* ctx, const t3d_tuning* tuning);
-/* Time candidate launch shapes on `desc` (coordinate descent, two sweeps;
- * each shape timed as the batched form runs, with two frames in flight: the
- * given frame and a second one the call creates and destroys, or
- * synchronous frames when the second does not fit; median of `reps`
- * samples; a change is kept only when it is at least 1 % faster) and keep
- * the fastest; writes the winner to *out (may be NULL).  The given frame
- * holds the last candidate's results afterwards. */
+/* Time candidate launch shapes on `desc` (coordinate descent, two sweeps).
+ * Each shape is timed as the batched form runs, with two frames in flight
+ * (the given frame and a second one the call creates and destroys), and by
+ * its synchronous frame; medians of `reps` samples; synchronous frames only
+ * when the second frame does not fit.  A change is kept when it is at least
+ * 1 % better on one time and no worse on the other (pipelined within 0.5 %,
+ * or 1 % for a synchronous frame 10 % shorter; synchronous within 2 %).
+ * Writes the winner to *out (may be NULL).  The given frame holds the last
+ * candidate's results afterwards. */
 t3d_status t3d_autotune(t3d_ctx* ctx, t3d_frame* frame, const t3d_render_desc* desc, uint32_t reps,
                         t3d_tuning* out);
 
