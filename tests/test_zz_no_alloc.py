@@ -88,6 +88,30 @@ def test_noalloc_matches_render_twice_shrinking(mode):
 
 
 @pytest.mark.gpu
+def test_noalloc_cell_lights():
+    """A LightVolume whose cells name 80 lights (past the 64 the vertex pass
+    stages in LDS), skinned and rigid models: the no-alloc render matches
+    t3d_render bit for bit."""
+    from tests.feature_scenes import cell_lights_scene
+    wl = cell_lights_scene(2, many=True)
+    hip = _hip()
+    with Context(0) as ctx:
+        frame = ctx.new_frame()
+        arrays = DeviceArrays(hip, _max_vertices(wl))
+        try:
+            print("noalloc cell lights", flush=True)
+            ref = ctx.render_workload(wl)
+            pairs, c, d, i = _noalloc_frame(ctx, wl, arrays, frame, A.EXEC_BATCHED)
+            assert [p[:4] for p in pairs] == [p[:4] for p in ref.pairs]
+            assert ref.survivors > 0
+            np.testing.assert_array_equal(c.view(np.uint32), ref.color.view(np.uint32))
+            np.testing.assert_array_equal(d.view(np.uint32), ref.depth.view(np.uint32))
+            np.testing.assert_array_equal(i, ref.indices)
+        finally:
+            arrays.free()
+
+
+@pytest.mark.gpu
 def test_noalloc_refusals():
     hip = _hip()
     wl = W.mixed_scene(count=6)
