@@ -777,7 +777,8 @@ __device__ __forceinline__ void part_range(const uint32_t* stats, uint32_t p, fl
   if (rd == 0) rd += 0.001f;                                      // sortingTriangles.go:43-46
 }
 
-__global__ __launch_bounds__(64) void k_hist(FrameArgs a, uint32_t blk0) {
+constexpr uint32_t kHistThreads = 256;   // four wavefronts: a 2,048-triangle tile's keys in one round of loads
+__global__ __launch_bounds__(kHistThreads) void k_hist(FrameArgs a, uint32_t blk0) {
   const uint32_t lb = frame_block(a, FC_TILES, 1);
   if (lb == kNoBlock) return;
   const uint32_t b = blk0 + lb;
@@ -793,15 +794,15 @@ __global__ __launch_bounds__(64) void k_hist(FrameArgs a, uint32_t blk0) {
   float mn, rd;
   part_range(a.stats, p, mn, rd);
   const uint32_t mode = (P.flags >> PF_SORT_SHIFT) & 3u;
-  for (uint32_t ib = 0; ib < n; ib += 8 * 64) {     // eight independent loads in flight per lane
+  for (uint32_t ib = 0; ib < n; ib += 8 * kHistThreads) {     // eight independent loads in flight per lane
     float d[8];
     for (int j = 0; j < 8; j++) {
-      const uint32_t i = ib + j * 64 + threadIdx.x;
+      const uint32_t i = ib + j * kHistThreads + threadIdx.x;
       d[j] = i < n ? keys[i] : 0.0f;
     }
     // every survivor counts, a NaN key (kept when processOnlyVisible is false) included
     for (int j = 0; j < 8; j++)
-      if (ib + j * 64 + threadIdx.x < n) atomicAdd(&h[visit_bin(d[j], mn, rd, mode)], 1u);
+      if (ib + j * kHistThreads + threadIdx.x < n) atomicAdd(&h[visit_bin(d[j], mn, rd, mode)], 1u);
   }
   __syncthreads();
   uint32_t* col = a.hist + P.hbase + tl;
@@ -1608,7 +1609,7 @@ hipError_t launch_cull(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uint32_
 }
 hipError_t launch_hist(const FrameArgs& a, uint32_t blk0, uint32_t nblk, hipStream_t s) {
   if (!nblk) return hipSuccess;
-  hipLaunchKernelGGL(k_hist, dim3(nblk), dim3(64), 0, s, a, blk0);
+  hipLaunchKernelGGL(k_hist, dim3(nblk), dim3(kHistThreads), 0, s, a, blk0);
   return hipGetLastError();
 }
 hipError_t launch_scan(const FrameArgs& a, uint32_t sblk0, uint32_t nsblk, uint32_t part0, uint32_t nparts,
