@@ -1050,6 +1050,14 @@ __global__ __launch_bounds__(256) void k_scatter4(FrameArgs a, uint32_t blk0) {
   float mn, rd;
   part_range(a.stats, p, mn, rd);
   for (uint32_t i = tid; i < kScatterWaves * kBins; i += blockDim.x) (&s_off[0][0])[i] = 0;
+  // a multi-tile pair's scanned column (each thread's two bins) is loaded now, beside the keys
+  static_assert(kBins == 2 * 256, "two bins per thread");
+  uint32_t col_run[2] = {0, 0};
+  if (P.ntiles > 1) {
+    const uint32_t* col = a.hist + P.hbase + tl;
+    col_run[0] = col[(size_t)tid * P.ntiles];
+    col_run[1] = col[(size_t)(tid + 256) * P.ntiles];
+  }
   const uint32_t nchunks = (n + 63) / 64;
   const uint32_t c0 = (wv * nchunks) / kScatterWaves, c1 = ((wv + 1) * nchunks) / kScatterWaves;
   // every key and record of this wave's chunks (at most kChunks of 64) is
@@ -1078,9 +1086,9 @@ __global__ __launch_bounds__(256) void k_scatter4(FrameArgs a, uint32_t blk0) {
   __syncthreads();
   // per bin: base of each wave (two bins per thread)
   if (P.ntiles > 1) {
-    const uint32_t* col = a.hist + P.hbase + tl;
-    for (uint32_t k = tid; k < kBins; k += blockDim.x) {
-      uint32_t run = col[(size_t)k * P.ntiles];
+    for (uint32_t j = 0; j < 2; j++) {
+      const uint32_t k = tid + 256 * j;
+      uint32_t run = col_run[j];
       for (uint32_t w = 0; w < kScatterWaves; w++) {
         const uint32_t c = s_off[w][k];
         s_off[w][k] = run;
