@@ -909,7 +909,18 @@ __global__ __launch_bounds__(1024) void k_scan_part(FrameArgs a, uint32_t blk0) 
   static_assert(kScanPartMax % 4096 == 0, "whole uint4 segments");
   if (past_count(a, FC_PSCAN)) return;
   const uint32_t b = blk0 + blockIdx.x;
-  const uint32_t p = find_part(a.pscan_prefix, a.nparts, b);
+  // the pair: a binary search of the prefixes, staged in LDS first when they fit (one memory round trip
+  // instead of one per step)
+  constexpr uint32_t kPrefixLds = 4096;
+  __shared__ uint32_t s_pre[kPrefixLds];
+  uint32_t p;
+  if (a.nparts < kPrefixLds) {
+    for (uint32_t i = threadIdx.x; i <= a.nparts; i += blockDim.x) s_pre[i] = a.pscan_prefix[i];
+    __syncthreads();
+    p = find_part(s_pre, a.nparts, b);
+  } else {
+    p = find_part(a.pscan_prefix, a.nparts, b);
+  }
   const uint32_t n4 = kBins * a.parts[p].ntiles / 4;           // the matrix holds a multiple of 512 entries
   uint4* __restrict__ h = reinterpret_cast<uint4*>(a.hist + a.parts[p].hbase);   // 2 KB aligned
   const uint32_t tid = threadIdx.x, lane = tid % 64u, wv = tid / 64u;
