@@ -513,8 +513,10 @@ typedef struct {
                                         bit 2 = stage 32 rows at a time instead of 64, bit 3 =
                                         software-pipelined gathers, bit 4 = four wavefronts per block,
                                         bit 5 = one output stream per block (colour vertices and indices,
-                                        or depth vertices; twice the blocks)
-                                        (0-4, 6, 8, 10, 12, 14, 16, 18, 20, 22, 34, 38, 50, 54) */
+                                        or depth vertices; twice the blocks), bit 6 = the assembly's
+                                        blocks in launch order whatever xcd_swizzle says for the other
+                                        passes (0-4, 6, 8, 10, 12, 14, 16, 18, 20, 22, 34, 38, 50, 54,
+                                        and 80, 82, 114 for 16, 18, 50 with bit 6) */
   uint32_t assemble_split;           /* assembly blocks per tile: a power of two, tile/split >= 64 */
   uint32_t xcd_swizzle;              /* 1: each XCD takes a contiguous range of a launch's blocks */
   uint32_t vertex_split;             /* 1: clip positions and colours in two arrays instead of one

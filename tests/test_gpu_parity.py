@@ -16,10 +16,11 @@ SCENES = {
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("store_mode", [0, 1, 2, 3, 4, 6, 8, 10, 12, 14, 16, 18, 20, 22, 34, 38, 50, 54],
+@pytest.mark.parametrize("store_mode", [0, 1, 2, 3, 4, 6, 8, 10, 12, 14, 16, 18, 20, 22, 34, 38, 50, 54, 80, 82, 114],
                          ids=["staged", "direct", "staged_nt", "direct_nt", "half", "half_nt", "pipe", "pipe_nt",
                               "pipe_half", "pipe_half_nt", "w4", "w4_nt", "w4_half", "w4_half_nt", "one_nt",
-                              "one_half_nt", "w4_one_nt", "w4_one_half_nt"])
+                              "one_half_nt", "w4_one_nt", "w4_one_half_nt", "w4_noswz", "w4_nt_noswz",
+                              "w4_one_nt_noswz"])
 @pytest.mark.parametrize("mode", [A.EXEC_BATCHED, A.EXEC_PLAIN], ids=["batched", "plain"])
 @pytest.mark.parametrize("scene", list(SCENES))
 def test_scene_parity(gpu_ctx, oracle, scene, mode, store_mode):

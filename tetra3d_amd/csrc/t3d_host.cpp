@@ -2987,9 +2987,10 @@ t3d_status t3d_set_tuning(t3d_ctx* c, const t3d_tuning* t) {
     return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: triangle_tile must be a multiple of 64 in [64, %u]", kMaxTile);
   if (!(t->store_mode <= 4 || t->store_mode == 6 || t->store_mode == 8 || t->store_mode == 10 || t->store_mode == 12 ||
         t->store_mode == 14 || t->store_mode == 16 || t->store_mode == 18 || t->store_mode == 20 || t->store_mode == 22 ||
-        t->store_mode == 34 || t->store_mode == 38 || t->store_mode == 50 || t->store_mode == 54))
+        t->store_mode == 34 || t->store_mode == 38 || t->store_mode == 50 || t->store_mode == 54 ||
+        t->store_mode == 80 || t->store_mode == 82 || t->store_mode == 114))
     return fail(c, T3D_ERR_INVALID_ARGUMENT,
-                "tuning: store_mode is 0-4, 6, 8, 10, 12, 14, 16, 18, 20, 22, 34, 38, 50 or 54");
+                "tuning: store_mode is 0-4, 6, 8, 10, 12, 14, 16, 18, 20, 22, 34, 38, 50, 54, 80, 82 or 114");
   if (t->scatter_waves != 1 && t->scatter_waves != 4)
     return fail(c, T3D_ERR_INVALID_ARGUMENT, "tuning: scatter_waves must be 1 or 4");
   if (t->assemble_split == 0 || (t->assemble_split & (t->assemble_split - 1)) ||
@@ -3069,7 +3070,7 @@ t3d_status t3d_autotune(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc, u
       {&t3d_tuning::vertices_per_thread, {1, 2, 4}},
       {&t3d_tuning::scatter_waves, {1, 4}},
       {&t3d_tuning::assemble_split, {1, 2, 4, 8, 16, 32}},
-      {&t3d_tuning::store_mode, {0, 2, 4, 6, 8, 10, 12, 14, 16, 18, 20, 22, 34, 38, 50, 54}},   // staged rows (64 or
+      {&t3d_tuning::store_mode, {0, 2, 4, 6, 8, 10, 12, 14, 16, 18, 20, 22, 34, 38, 50, 54, 80, 82, 114}},   // staged rows (64 or
                                       // 32), streaming or not, pipelined or not, 1 or 4 waves, one stream per block
       {&t3d_tuning::xcd_swizzle, {0, 1}},
       {&t3d_tuning::vertex_split, {0, 1}},
@@ -3108,7 +3109,7 @@ t3d_status t3d_autotune(t3d_ctx* c, t3d_frame* f, const t3d_render_desc* desc, u
     }
     // four-wave assembly blocks want larger slot ranges per block than one-wave blocks: store mode and
     // assembly split together
-    for (uint32_t mode : {16u, 18u, 20u, 22u, 50u, 54u})
+    for (uint32_t mode : {16u, 18u, 20u, 22u, 50u, 54u, 80u, 82u})
       for (uint32_t split : {1u, 2u, 4u, 8u}) {
         if (st) break;
         t3d_tuning t = best;

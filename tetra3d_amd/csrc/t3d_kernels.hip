@@ -1654,6 +1654,11 @@ hipError_t launch_assemble(const FrameArgs& a, uint32_t blk0, uint32_t nblk, uin
                            hipStream_t s) {
   if (!nblk) return hipSuccess;
   if (split == 0 || (split & (split - 1)) || a.tile % split) return hipErrorInvalidValue;
+  if (mode & 64u) {   // bit 6: the assembly's blocks in launch order, whatever the other passes use
+    FrameArgs b = a;
+    b.xcd_swizzle = 0;
+    return launch_assemble(b, blk0, nblk, mode & ~64u, split, s);
+  }
   const dim3 g(nblk * split), g2(2 * nblk * split);
   switch (mode) {
     case 8: hipLaunchKernelGGL((k_assemble_pipe<64, false>), g, dim3(64), 0, s, a, blk0, split); break;
